@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the full-batch t-SNE gradient pass on one MI355X.
+
+A step is one iteration of the t-SNE schedule (attraction over the CSR P, FFT-interpolated repulsion,
+gains / momentum update) over the whole synthetic data set. Default workload: the MNIST-shaped
+precision run -- 70,000 x 784 float32 (10 prototypes with ~19% nonzero pixels, + N(0, 0.1^2) noise,
+clipped, quantised to /255), perplexity-30 affinities (90-NN), 2-D, 750 iterations (250 early
+exaggeration) -- with --steps 750 timing the whole schedule from iteration 0. Inputs (P, Y0) are resident
+in HBM before the timed region. value = iterations/s of the FP32 path (FP32 bulk, FP64 Z and
+reductions); the plain FP64 path is timed on the same workload and reported beside it.
+
+The per-stage roofline comes from a second, profiled pass of the same iterations (HIP events on the
+library's stream, algorithmic bytes from the engine). cpu_baseline times the reference's own compiled
+module (oracle/_ref) on this host's cores on a bounded sample of iterations taken along the GPU
+trajectory.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lib-magnet_amd"))
+
+METRIC = "t-SNE iters/s at 70k & 1M pts (FP32 vs plain FP64 GPU); transform p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "mnist70k": dict(n=70_000, d=784, n_iter=750, early=250, desc="MNIST-shaped 70,000 x 784, perplexity 30, 750 iterations"),
+    "gmm1m": dict(n=1_000_000, d=50, n_iter=1000, early=250, desc="1,000,000 x 50 Gaussian mixture (100 clusters, Dirichlet sizes), perplexity 30, 1,000 iterations"),
+    "gmm10k": dict(n=10_000, d=50, n_iter=500, early=166, desc="10,000 x 50 Gaussian mixture (10 clusters), perplexity 30, 500 iterations"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_data(name, seed=0):
+    rng = np.random.default_rng(seed)
+    if name == "mnist70k":
+        n, d = 70_000, 784
+        protos = (rng.random((10, d)) < 0.19) * rng.uniform(0.3, 1.0, (10, d))
+        lab = rng.integers(0, 10, n)
+        X = protos[lab] + rng.normal(0, 0.1, (n, d))
+        X = np.round(np.clip(X, 0, 1) * 255) / 255
+        return X.astype(np.float32), lab
+    if name == "gmm1m":
+        n, d, k = 1_000_000, 50, 100
+        sizes = rng.multinomial(n, rng.dirichlet(np.ones(k)))
+        lab = np.repeat(np.arange(k), sizes)
+        X = rng.normal(0, 4, (k, d))[lab] + rng.normal(0, 1, (n, d))
+        return X.astype(np.float32), lab
+    n, d, k = 10_000, 50, 10
+    lab = np.repeat(np.arange(k), n // k)
+    X = rng.normal(0, 5, (k, d))[lab] + rng.normal(0, 1, (n, d))
+    return X.astype(np.float32), lab
+
+
+def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False):
+    import torch
+    from magnet import capi
+    dt = np.float32 if prec == "fp32" else np.float64
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    ip, ix, data = dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32)), dev(P.data.astype(dt))
+    kw = dict(n_iter=n_iter, early_iter=early)
+    if warmup > 0:
+        Yw, Uw, Gw = dev(Y0.astype(dt)), dev(np.zeros_like(Y0, dt)), dev(np.ones_like(Y0, dt))
+        capi.optimize(ip, ix, data, Yw, Uw, Gw, 0, min(warmup, n_iter), **kw)
+    Y, U, G = dev(Y0.astype(dt)), dev(np.zeros_like(Y0, dt)), dev(np.ones_like(Y0, dt))
+    capi.profile(profile)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    capi.optimize(ip, ix, data, Y, U, G, 0, steps, **kw)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    stages = capi.profile_read() if profile else None
+    capi.profile(False)
+    return sec, stages, (Y, U, G, ip, ix, data)
+
+
+def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads):
+    """The reference's compiled module on this host: per-iteration time at six points of the schedule
+    (GPU FP32 states), 2 warm-up + 5 timed reference iterations at each; it/s from the mean."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import scipy.fft as sfft
+    R = oracle.ref_module()
+    if R is None:
+        return None
+    n = P.shape[0]
+    ip, ix, d = P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data.astype(np.float32)
+    lr = max(200.0, n / 12.0)
+    cache = {}
+
+    def spectra(nb, width):   # magnet/tsne.py:48-71, cached per grid like the reference
+        key = (nb, round(width, 6))
+        if key not in cache:
+            N = nb * 3
+            M = sfft.next_fast_len(2 * N, real=True)
+            off = np.arange(M)
+            off = np.where(off <= M // 2, off, off - M).astype(np.float64) * (width / 3)
+            r2 = off[:, None] ** 2 + off[None, :] ** 2
+            K1 = (1.0 / (1.0 + r2)).astype(np.float32); K2 = (1.0 / (1.0 + r2) ** 2).astype(np.float32)
+            conv = R.FFTConv2D(int(M), 3)
+            conv.set_kernels(np.ascontiguousarray(sfft.rfft2(K1, workers=threads).real, dtype=np.float32),
+                             np.ascontiguousarray(sfft.rfft2(K2, workers=threads).real, dtype=np.float32))
+            cache[key] = conv
+        return cache[key]
+
+    from magnet.tsne import grid_params
+    per_it = []
+    points = [int(round(f * (n_iter - 1))) for f in (0.0, 0.2, 0.4, 0.6, 0.8, 1.0)]
+    for t in points:
+        Y, U, G = gpu_state_fn(t)
+        times = []
+        for r in range(7):
+            it = min(t + r, n_iter - 1)
+            rho, mom = (12.0, 0.5) if it < early else (1.0, 0.8)
+            t0 = time.perf_counter()
+            attr = np.asarray(R.tsne_attraction(ip, ix, d, Y, 1.0, threads))
+            xmin, ymin, width, nb = grid_params(R.bbox_2d(Y, threads))
+            sumQ, F = spectra(nb, width).compute(Y, xmin, ymin, width, nb, threads)
+            R.tsne_step(Y, U, G, attr, np.asarray(F), float(rho), float(1.0 / max(sumQ, 1e-12)), float(lr), float(mom), threads)
+            if r >= 2:
+                times.append(time.perf_counter() - t0)
+        per_it.append(float(np.median(times)))
+    mean = float(np.mean(per_it))
+    return {"value": round(1.0 / mean, 3), "unit": "iters/s", "cores": threads, "kind": "reference",
+            "sample": f"reference _magnet (oracle/_ref, n_threads={threads}) on the same P: median of 5 iterations "
+                      f"(after 2 warm-up) at iterations {points} of the schedule, started from the GPU FP32 states; "
+                      f"it/s = 1 / mean of the six medians ({', '.join(f'{1e3 * v:.1f}' for v in per_it)} ms)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="iterations timed (default: the whole schedule)")
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="mnist70k", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the t-SNE pass of one embedding runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from magnet import capi
+    from magnet.affinity import perplexity_affinities
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    capi.lib()
+    capi.use_torch_stream()
+    W = WORKLOADS[args.workload]
+    steps = args.steps or W["n_iter"]
+    steps = min(steps, W["n_iter"])
+    t0 = time.time()
+    X, lab = make_data(args.workload)
+    P = perplexity_affinities(X, 30.0)
+    del X
+    Y0 = np.random.default_rng(1).normal(0, 1e-4, (P.shape[0], 2))
+    log(f"bench: {W['desc']}: n={P.shape[0]} nnz={P.nnz} (P built in {time.time() - t0:.1f}s)")
+
+    sec32, _, _ = run_gpu(P, Y0, "fp32", steps, args.warmup, W["n_iter"], W["early"])
+    log(f"bench: fp32 {steps} iterations in {sec32:.3f}s -> {steps / sec32:.1f} it/s")
+    _, stages, state = run_gpu(P, Y0, "fp32", steps, 0, W["n_iter"], W["early"], profile=True)
+    stage_rows = {k: dict(ms_total=round(v[0], 3), launches=v[1], avg_us=round(1e3 * v[0] / max(v[1], 1), 2),
+                          GBs=round(v[2] / max(v[0], 1e-9) / 1e6, 1)) for k, v in stages.items() if v[1] > 0}
+    dom = max(stages.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcnt, dbytes) = dom
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(1e3 * dms / dcnt, 2), "algo_bytes_per_launch": round(dbytes / dcnt)}
+    tot_ms = sum(v[0] for v in stages.values())
+    tot_bytes = sum(v[2] for v in stages.values())
+    log(f"bench: stages {json.dumps(stage_rows)}")
+    extra = {"stages": stage_rows, "pass_GBs": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+             "algo_bytes_per_iter": round(tot_bytes / steps)}
+    if not args.no_fp64:
+        sec64, _, _ = run_gpu(P, Y0, "fp64", steps, min(args.warmup, 10), W["n_iter"], W["early"])
+        log(f"bench: fp64 {steps} iterations in {sec64:.3f}s -> {steps / sec64:.1f} it/s")
+        extra["fp64"] = {"value": round(steps / sec64, 2), "unit": "iters/s", "ms_per_step": round(1e3 * sec64 / steps, 4)}
+        extra["fp32_over_fp64"] = round(sec64 / sec32, 3)
+    cpu = None
+    if not args.no_cpu:
+        Yd, Ud, Gd, ip, ix, data = state
+        threads = min(16, os.cpu_count() or 1)
+
+        def gpu_state(t):   # FP32 GPU state at iteration t of the schedule (fresh run to t)
+            Y = torch.from_numpy(Y0.astype(np.float32)).cuda()
+            U, G = torch.zeros_like(Y), torch.ones_like(Y)
+            if t > 0:
+                capi.optimize(ip, ix, data, Y, U, G, 0, t, n_iter=W["n_iter"], early_iter=W["early"])
+            torch.cuda.synchronize()
+            return (np.ascontiguousarray(Y.cpu().numpy()), np.ascontiguousarray(U.cpu().numpy()),
+                    np.ascontiguousarray(G.cpu().numpy()))
+        tc = time.time()
+        cpu = cpu_baseline(P, Y0, W["n_iter"], W["early"], gpu_state, threads)
+        log(f"bench: cpu baseline {cpu and cpu['value']} it/s in {time.time() - tc:.1f}s")
+    line = {
+        "metric": METRIC, "value": round(steps / sec32, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * sec32 / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": W["desc"], "n": int(P.shape[0]), "nnz": int(P.nnz), "n_iter": W["n_iter"],
+                   "early_iter": W["early"], "precision": "fp32 (FP64 Z and reductions)", "n_interp": 3,
+                   "box_width": 0.7},
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/* C-ABI of the MI355X (gfx950) implementation of MAGNET's full-batch loss="tsne" gradient pass.
+ *
+ * Every entry point is a drop-in for one function of the reference's native module `_magnet`
+ * (kerighan/lib-magnet v3.0.2, src/bindings.cpp:8-83); the reference interface each one replaces is
+ * cited beside it. Conventions shared by all entry points:
+ *   - Return value: MG_OK (0) or an MG_ERR_* code; mg_last_error() gives the message of the calling
+ *     thread's last failure (the reference's `require` -> std::runtime_error, src/layout.hpp:34).
+ *   - `prec` selects the arithmetic: MG_F64 (double arrays) or MG_F32 (float arrays). Index arrays are
+ *     int32 in both. Point arrays are (n, dim) row-major, 2-D paths use dim == 2.
+ *   - Pointers named d_* are device (HBM) pointers; h_* are host pointers. Work is issued on the
+ *     library stream (mg_set_stream, default the null stream) and is ordered on it.
+ *   - Entry points that return a host scalar (sumQ, bbox) synchronise the stream before returning.
+ */
+#ifndef MAGNET_HIP_H
+#define MAGNET_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MG_OK 0
+#define MG_ERR_ARG 1     /* invalid argument or shape (reference: require() -> RuntimeError) */
+#define MG_ERR_HIP 2     /* HIP runtime error */
+#define MG_ERR_NOMEM 3   /* device allocation failed */
+#define MG_ERR_UNSUP 4   /* valid in the reference, not supported on this path (message says why) */
+
+#define MG_F32 0
+#define MG_F64 1
+
+/* ---- runtime ------------------------------------------------------------------------------------ */
+const char* mg_last_error(void);
+int mg_version(void);                                   /* 100 * major + minor */
+int mg_set_stream(void* hip_stream);                    /* hipStream_t; NULL = null stream */
+int mg_synchronize(void);
+int mg_malloc(void** d_ptr, size_t bytes);
+int mg_free(void* d_ptr);
+int mg_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int mg_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int mg_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+/* HBM bytes free / total on the current device (serving admission). */
+int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* Per-stage timing of mg_tsne_optimize with HIP events on the working stream (the reference prints
+ * per-stage laps of FFTConv2D::compute under MAGNET_DEBUG, src/fftconv.cpp:69-70; setting MAGNET_DEBUG
+ * here enables this and prints every stage of every iteration to stderr). Stages: attraction,
+ * box_sort, spread, rows_forward, columns, rows_inverse, gather, update, spectra. algo_bytes sums the
+ * algorithmic HBM bytes of the timed launches (each input read once, each output written once). */
+int mg_profile_enable(int on);
+int mg_profile_reset(void);
+int mg_profile_num_stages(void);
+int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* count, double* algo_bytes);
+
+/* ---- per-call kernels (device pointers) ----------------------------------------------------------- */
+
+/* bbox_2d (src/fftrep.cpp:415-434): h_out = {xmin, ymin, xmax, ymax} of d_Y (n, 2), in double. */
+int mg_bbox_2d(int prec, const void* d_Y, int64_t n, double* h_out);
+
+/* tsne_attraction (src/fftrep.cpp:129-177): out_i = sum_j p_ij (1 + |y_i - y_j|^2 / dof)^-1 (y_i - y_j).
+ * CSR (n + 1, nnz, nnz) with data in `prec`; dim in [1, 64]. The CSR is validated on the device
+ * (validate_csr, src/layout.hpp:21-32). The order of every sum is fixed, so the result is deterministic. */
+int mg_tsne_attraction(int prec, const int32_t* d_indptr, const int32_t* d_indices, const void* d_data,
+                       int64_t nnz, const void* d_Y, int64_t n, int dim, double dof, void* d_out);
+
+/* tsne_step (src/fftrep.cpp:225-266): grad = rho * attr - F * inv_sumQ, delta-bar-delta gains,
+ * momentum, Y += U, Y -= mean(Y); Y, U, G updated in place. */
+int mg_tsne_step(int prec, void* d_Y, void* d_U, void* d_G, const void* d_attr, const void* d_F, int64_t n,
+                 int dim, double rho, double inv_sumQ, double lr, double momentum);
+
+/* tsne_update (src/fftrep.cpp:181-220): the same update from a given gradient. */
+int mg_tsne_update(int prec, void* d_Y, void* d_U, void* d_G, const void* d_grad, int64_t n, int dim,
+                   double lr, double momentum);
+
+/* mid_spring (src/fftrep.cpp:270-287): grad_i += w (Y_i - mean_k Y[pairs[i, k]]), pairs (n, m). */
+int mg_mid_spring(int prec, const void* d_Y, const int32_t* d_pairs, void* d_grad, int64_t n, int dim, int m,
+                  double w);
+
+/* ---- FFT repulsion object: FFTConv2D (src/fftconv.cpp:43-237) ------------------------------------- */
+typedef struct mg_fftconv mg_fftconv;
+
+/* FFTConv2D(M, n_interp) (src/fftconv.cpp:45-47). M is the padded size the caller's spectra use; the
+ * device transforms run on the smallest power of two >= max(M, 2N) (any M >= 2N - 1 gives the same
+ * convolution). n_interp in [1, 8]. */
+int mg_fftconv_create(mg_fftconv** out, int prec, int M, int n_interp);
+int mg_fftconv_destroy(mg_fftconv* conv);
+
+/* set_kernels (src/fftconv.cpp:51-55) in kernel-value form: h_K1, h_K2 are (M_in, M_in) kernel tables
+ * on the padded torus, sampled at offsets (m or m - M_in) * delta (magnet/tsne.py:57-67), i.e. the
+ * inverse transforms of the spectra FFTConv2D.set_kernels takes (the binding converts one into the
+ * other). Offsets up to min(M_in, M_dev) / 2 are used; the device re-transforms at its own size. */
+int mg_fftconv_set_kernel_values(mg_fftconv* conv, int M_in, const double* h_K1, const double* h_K2);
+
+/* The t-SNE kernels of FFTRepulsion2D._kernels (magnet/tsne.py:48-71): K1 = (1 + r^2/dof)^-dof,
+ * K2 = (1 + r^2/dof)^(-dof-1) at node spacing width / n_interp. Spectra are computed on the device
+ * (FP32 objects round the kernel values to float first, like the reference) for each padded size a
+ * compute call needs, and cached. */
+int mg_fftconv_set_kernels_tsne(mg_fftconv* conv, double width, double dof);
+
+/* compute (src/fftconv.cpp:58-230): F (n, 2) and h_sumQ = sum_i pot0_i - n for the grid
+ * (xmin, ymin, width, n_boxes). */
+int mg_fftconv_compute(mg_fftconv* conv, const void* d_Y, int64_t n, double xmin, double ymin, double width,
+                       int n_boxes, void* d_F, double* h_sumQ);
+
+/* Intermediate grids of the last compute, for tests: W (3, N, N) charges, phi (N, N, 4) potentials
+ * (interleaved, after the 1/M^2 scale). Either pointer may be NULL. */
+int mg_fftconv_grids(mg_fftconv* conv, void* d_W, void* d_phi);
+
+/* ---- whole loop: optimize_tsne (magnet/tsne.py:132-214) ------------------------------------------ */
+typedef struct {
+    int32_t n_iter;                /* schedule length (tsne.py:132 n_iter) */
+    int32_t early_iter;            /* iterations with early exaggeration */
+    double early_exaggeration;     /* rho during the early phase (12) */
+    double exaggeration;           /* rho afterwards (1) */
+    double learning_rate;          /* <= 0: max(200, n / early_exaggeration) (tsne.py:152-153) */
+    double momentum0, momentum1;   /* (0.5, 0.8) */
+    int32_t n_interp;              /* Lagrange nodes per box (3) */
+    int32_t min_boxes, max_boxes;  /* (20, 1000) (tsne.py:18) */
+    double box_width;              /* 0.7 */
+    double dof;                    /* Student-t degrees of freedom (1) */
+    double mid_weight, mid_phase;  /* mid-range spring (tsne.py:194-198); pairs may be NULL */
+} mg_tsne_params;
+
+/* Called after iteration `it` when it % 50 == 0 or it == n_iter - 1 (tsne.py:180-181) with the device
+ * layout; return non-zero to stop. */
+typedef int (*mg_tsne_callback)(int it, const void* d_Y, int64_t n, void* user);
+
+/* Runs iterations [it0, it1) of the n_iter schedule on the device state (Y, U, G), all (n, 2) in `prec`
+ * (U = 0 and G = 1 at it0 = 0 for a fresh run; keeping them makes a resumed run identical to an
+ * uninterrupted one). The CSR data must be normalised to sum 1 (tsne.py:149-151). d_pairs (n, m_pairs)
+ * may be NULL. h_sumQ, when non-NULL, receives sumQ of each iteration (it1 - it0 doubles). */
+int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_indptr, const int32_t* d_indices,
+                     const void* d_data, int64_t nnz, int64_t n, void* d_Y, void* d_U, void* d_G, const int32_t* d_pairs,
+                     int m_pairs, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAGNET_HIP_H */

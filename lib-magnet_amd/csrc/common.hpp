@@ -1,0 +1,131 @@
+// Shared device helpers and the internal launcher interface of libmagnet_hip (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <stdexcept>
+
+namespace mg {
+
+typedef int64_t i64;
+
+// ---- errors: every launcher throws mg::Error; the C-ABI turns it into a status code + message
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void hip_check(hipError_t e, const char* what);
+#define MG_HIP(call) ::mg::hip_check((call), #call)
+inline void require(bool ok, const std::string& msg, int code = 1) { if (!ok) throw Error(code, msg); }
+
+constexpr int WAVE = 64;
+constexpr int MAX_M = 8192;          // largest device FFT size (n_interp * n_boxes <= 4096)
+constexpr int LOG_MAX_M = 13;
+
+template <class T> struct cplx { T x, y; };
+template <class T> __host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <class T> __host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T> __host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T> __host__ __device__ inline cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
+
+template <class T> struct vec2;
+template <> struct vec2<float> { typedef float2 type; };
+template <> struct vec2<double> { typedef double2 type; };
+
+__device__ inline int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Lagrange weights of n_interp nodes at (k + 1/2) / n_interp, local coordinate u in [0, 1]
+// (closed form for three nodes as in src/fftconv.cpp:24-29, product form otherwise).
+template <class T, int NI> __device__ inline void lagrange(T u, T* w) {
+    if constexpr (NI == 3) {
+        const T a = u - T(1) / T(6), b = u - T(0.5), c = u - T(5) / T(6);
+        w[0] = b * c * T(4.5); w[1] = -a * c * T(9); w[2] = a * b * T(4.5);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const T uk = (k + T(0.5)) / NI;
+            T num = 1, den = 1;
+#pragma unroll
+            for (int m = 0; m < NI; ++m) {
+                if (m == k) continue;
+                const T um = (m + T(0.5)) / NI;
+                num *= (u - um); den *= (uk - um);
+            }
+            w[k] = num / den;
+        }
+    }
+}
+
+// Grid of one repulsion call, in the arithmetic type T (float for the FP32 path, as the reference
+// passes float xmin / ymin / width to FFTConv2D::compute).
+template <class T> struct GridT {
+    T xmin, ymin, width, inv_w;
+    int nb, N;
+};
+
+// Box of a point: x-box by multiplication with 1/width (the counting-sort key of src/fftconv.cpp:88),
+// y-box and both local coordinates by division (:106-110). Key = bx * nb + by.
+template <class T> __device__ inline int box_key(T x, T y, const GridT<T>& g) {
+    const int bx = min(g.nb - 1, max(0, (int)((x - g.xmin) * g.inv_w)));
+    const T uy = (y - g.ymin) / g.width;
+    const int by = min(g.nb - 1, max(0, (int)uy));
+    return bx * g.nb + by;
+}
+
+// ---- scratch of one repulsion engine (device pointers), sized by the host
+template <class T> struct RepBuffers {
+    int *keysA, *valsA, *keysB, *valsB;   // radix-sort ping-pong (box key, point index)
+    int* hist;                            // 256 * nblocks digit histograms
+    int* item_key;                        // 2 per spread chunk
+    int* item_valid;
+    T* item_val;                          // 2 per chunk x NI x 3NI
+    T* W;                                 // (3, N, N) charges
+    cplx<T>* Wh;                          // (3, Mh, N) row spectra, transposed
+    cplx<T>* Ph;                          // (4, Mh, N) products after the column inverse
+    cplx<T>* colstash;                    // (Mh, M) column spectra when LDS holds one column only
+    T* phi;                               // (N, N, 4) potentials, interleaved
+    double* zpart;                        // per-block Z partials
+    unsigned* counter;                    // last-block counters (zeroed at creation, restored by kernels)
+    double* zout;                         // [0] sumQ, [1] 1/max(sumQ, 1e-12)
+    T* inv_out;                           // 1/max(sumQ, 1e-12) rounded to T
+};
+
+// ---- launchers (hipcc-compiled translation units), all on `st`
+template <class T> void launch_bbox(const T* Y, i64 n, T* part, unsigned* counter, T* out, hipStream_t st);
+template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, const T* P, const T* Y, i64 n,
+                                          int dim, T inv_dof, T* out, i64 nnz, hipStream_t st);
+// mode 0: grad = rho * attr - F * inv (inv from d_inv when non-null, else inv_h); mode 1: grad given in A.
+// pairs != null adds the mid-range spring w * (Y_i - mean partners) to the gradient.
+template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h,
+                                      T rho, T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim,
+                                      double* part, unsigned* counter, T* mean_bbox, hipStream_t st);
+template <class T> void launch_mid_spring(const T* Y, const int32_t* pairs, T* grad, i64 n, int dim, int m, T w,
+                                          hipStream_t st);
+// gradient of the non-fused path: combine ? rho * A - F * inv : grad, plus the spring when pairs != null
+template <class T> void launch_grad(const T* A, const T* F, const T* d_inv, T inv_h, T rho, const T* Y, const int32_t* pairs,
+                                    int m, T w, i64 n, int dim, int combine, T* grad, hipStream_t st);
+template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st);
+
+// box sort: returns which buffer holds the result (0 = A, 1 = B)
+template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st);
+int sort_hist_blocks(i64 n);
+int spread_chunks(i64 n);
+template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
+                                      RepBuffers<T>& b, hipStream_t st);
+template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
+                                      const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
+int gather_blocks(i64 n);
+
+// FFT stages (M a power of two, N <= M / 2)
+void fft_init_twiddles();   // uploads the twiddle tables once per device
+template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);
+template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* K1, const T* K2, cplx<T>* Ph,
+                                       cplx<T>* stash, hipStream_t st);
+template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* phi, hipStream_t st);
+// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, M x M);
+// output (Mh, M) real, ky-major, rounded to T (round_kernel: values rounded to float first).
+template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel,
+                                       double* tmp, T* K1, T* K2, hipStream_t st);
+
+}  // namespace mg

@@ -1,0 +1,603 @@
+// Host side of libmagnet_hip: the FFT repulsion object (FFTConv2D, src/fftconv.cpp:43-237), the
+// whole-loop optimiser (optimize_tsne, magnet/tsne.py:132-214) and the C-ABI of include/magnet_hip.h.
+// Every kernel is issued on one stream; the only host round trip per iteration is the 48-byte
+// read-back of the layout's bounding box, which the grid decision of the next iteration needs.
+#include "common.hpp"
+#include "../../include/magnet_hip.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace mg {
+
+int update_blocks(i64 n);
+int fft_lds_max();
+
+static thread_local std::string g_err;
+static hipStream_t g_stream = nullptr;
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    const int code = (e == hipErrorOutOfMemory) ? MG_ERR_NOMEM : MG_ERR_HIP;
+    throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// grow-only device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void* ensure(size_t b) {
+        if (b <= bytes && p) return p;
+        if (p) { MG_HIP(hipFree(p)); p = nullptr; bytes = 0; }
+        const size_t want = b < 256 ? 256 : b;
+        MG_HIP(hipMalloc(&p, want));
+        bytes = want;
+        return p;
+    }
+};
+template <class U> inline U* dptr(const DevBuf& b) { return reinterpret_cast<U*>(b.p); }
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    void* ensure(size_t b) {
+        if (b <= bytes && p) return p;
+        if (p) { MG_HIP(hipHostFree(p)); p = nullptr; }
+        MG_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+        return p;
+    }
+};
+
+inline int pow2_at_least(i64 v) { int m = 1; while (m < v) m <<= 1; return m; }
+
+// ---- CSR check on the device (validate_csr, src/layout.hpp:21-32): bit 1 indptr[0] != 0,
+// 2 decreasing indptr, 4 indptr[n] != nnz, 8 column index out of range
+__global__ void k_validate_csr(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix, i64 n, i64 nnz, int* flag) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < (n > nnz ? n : nnz); k += stride) {
+        if (k < n && ip[k + 1] < ip[k]) atomicOr(flag, 2);
+        if (k < nnz && (ix[k] < 0 || ix[k] >= n)) atomicOr(flag, 8);
+        if (k == 0) { if (ip[0] != 0) atomicOr(flag, 1); if ((i64)ip[n] != nnz) atomicOr(flag, 4); }
+    }
+}
+
+void validate_csr(const int32_t* ip, const int32_t* ix, i64 n, i64 nnz, hipStream_t st) {
+    require(ip != nullptr && (nnz == 0 || ix != nullptr), "indptr / indices must be device arrays");
+    require(nnz >= 0 && nnz < (i64)INT32_MAX, "nnz out of range");
+    DevBuf flag;
+    int* f = reinterpret_cast<int*>(flag.ensure(sizeof(int)));
+    MG_HIP(hipMemsetAsync(f, 0, sizeof(int), st));
+    const i64 work = n > nnz ? n : nnz;
+    const unsigned blocks = (unsigned)std::min<i64>(4096, std::max<i64>(1, (work + 255) / 256));
+    hipLaunchKernelGGL(k_validate_csr, dim3(blocks), dim3(256), 0, st, ip, ix, n, nnz, f);
+    MG_HIP(hipGetLastError());
+    int h = 0;
+    MG_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    require(!(h & 1), "indptr[0] must be 0");
+    require(!(h & 2), "indptr must be non-decreasing");
+    require(!(h & 4), "indices must have indptr[n] entries");
+    require(!(h & 8), "column index out of range");
+}
+
+// ---- grid decision of FFTRepulsion2D._grid (magnet/tsne.py:30-46) on the bbox (in double)
+struct GridDecision { double xmin, ymin, width; int nb; };
+GridDecision decide_grid(const double* bb, double bw, int minb, int maxb) {
+    const double lox = bb[0] - 1e-3, loy = bb[1] - 1e-3, hix = bb[2] + 1e-3, hiy = bb[3] + 1e-3;
+    const double span = std::max(hix - lox, hiy - loy);
+    double nbf = std::ceil(span / bw);
+    nbf = std::min((double)maxb, std::max((double)minb, nbf));
+    int nb = (int)nbf;
+    nb = std::min(maxb, ((nb + 3) / 4) * 4);
+    const double width = (nb * bw >= span) ? bw : std::max(span / nb, 1e-12);
+    return {lox, loy, width, nb};
+}
+
+template <class T> GridT<T> make_grid(double xmin, double ymin, double width, int nb, int ni) {
+    GridT<T> g;
+    g.xmin = (T)xmin; g.ymin = (T)ymin; g.width = (T)width;
+    g.inv_w = T(1) / g.width;
+    g.nb = nb; g.N = nb * ni;
+    return g;
+}
+
+// ---- per-stage timing with HIP events on the working stream (the reference's MAGNET_DEBUG laps,
+// src/fftconv.cpp:69-70): enabled by mg_profile_enable or the MAGNET_DEBUG environment variable
+enum Stage { ST_ATTR = 0, ST_SORT, ST_SPREAD, ST_ROWS_FWD, ST_COLUMNS, ST_ROWS_INV, ST_GATHER, ST_UPDATE, ST_SPECTRA, NSTAGE };
+static const char* kStageNames[NSTAGE] = {"attraction", "box_sort", "spread", "rows_forward", "columns", "rows_inverse",
+                                          "gather", "update", "spectra"};
+struct Prof {
+    bool on = false, debug = false;
+    hipEvent_t ev[2 * NSTAGE] = {};
+    bool pending[NSTAGE] = {};
+    double total[NSTAGE] = {};
+    double bytes[NSTAGE] = {};     // algorithmic HBM bytes of the launches timed (inputs read once, outputs written once)
+    double pend_bytes[NSTAGE] = {};
+    long long count[NSTAGE] = {};
+    void begin(int s, hipStream_t st) {
+        if (!on) return;
+        if (!ev[2 * s]) { MG_HIP(hipEventCreate(&ev[2 * s])); MG_HIP(hipEventCreate(&ev[2 * s + 1])); }
+        MG_HIP(hipEventRecord(ev[2 * s], st));
+    }
+    void end(int s, hipStream_t st, double algo_bytes = 0) {
+        if (!on) return;
+        MG_HIP(hipEventRecord(ev[2 * s + 1], st));
+        pending[s] = true;
+        pend_bytes[s] = algo_bytes;
+    }
+    // after the stream has passed every recorded event
+    void collect(int it) {
+        if (!on) return;
+        for (int s = 0; s < NSTAGE; ++s) {
+            if (!pending[s]) continue;
+            float ms = 0.f;
+            MG_HIP(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+            total[s] += ms; ++count[s]; pending[s] = false; bytes[s] += pend_bytes[s];
+            if (debug) fprintf(stderr, "  [it %d] %s %.3f ms\n", it, kStageNames[s], ms);
+        }
+    }
+    void reset() { for (int s = 0; s < NSTAGE; ++s) { total[s] = 0; bytes[s] = 0; count[s] = 0; pending[s] = false; } }
+};
+static Prof g_prof;
+
+// ------------------------------------------------------------------------------ repulsion engine
+template <class T> struct Repulsion {
+    int ni;
+    int M_user;
+    // kernel definition: t-SNE (delta from width, dof) or a value table on an M_in torus
+    bool have_kernel = false, tsne_kernel = true;
+    double kwidth = 0.7, kdof = 1.0;
+    int M_in = 0;
+    std::vector<double> kvals;   // (2, M_in, M_in)
+    struct Spec { int M; double delta, dof; DevBuf K1, K2; unsigned long use; };
+    std::list<Spec> cache;
+    unsigned long tick = 0;
+    DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, stash, phi;
+    DevBuf zpart, counter, zout, inv_out, spec_tmp, table;
+    int last_N = 0, last_M = 0, sorted_buf = 0;
+    i64 last_n = 0;
+
+    Repulsion(int M, int ni_) : ni(ni_), M_user(M) {
+        require(ni >= 1 && ni <= 8, "n_interp must be in [1, 8]");
+        fft_init_twiddles();
+        unsigned* c = reinterpret_cast<unsigned*>(counter.ensure(64 * sizeof(unsigned)));
+        MG_HIP(hipMemset(c, 0, 64 * sizeof(unsigned)));
+        zout.ensure(4 * sizeof(double));
+        inv_out.ensure(4 * sizeof(T));
+    }
+
+    // spectra for the device size M (cached; the reference keeps up to 9, magnet/tsne.py:52-54)
+    Spec& spectra(int M, hipStream_t st) {
+        require(have_kernel, "kernels not set");
+        const double delta = tsne_kernel ? kwidth / ni : -1.0;
+        for (auto& s : cache)
+            if (s.M == M && s.delta == delta && s.dof == kdof) { s.use = ++tick; return s; }
+        if (cache.size() >= 9) {
+            auto lru = cache.begin();
+            for (auto it = cache.begin(); it != cache.end(); ++it) if (it->use < lru->use) lru = it;
+            cache.erase(lru);
+        }
+        cache.emplace_back();
+        Spec& s = cache.back();
+        s.M = M; s.delta = delta; s.dof = kdof; s.use = ++tick;
+        const int Mh = M / 2 + 1;
+        s.K1.ensure((size_t)Mh * M * sizeof(T));
+        s.K2.ensure((size_t)Mh * M * sizeof(T));
+        double* tmp = reinterpret_cast<double*>(spec_tmp.ensure((size_t)2 * M * Mh * sizeof(double)));
+        const double* tab = nullptr;
+        if (!tsne_kernel) {
+            // resample the caller's table (offsets up to M_in / 2) onto the M x M torus
+            std::vector<double> h((size_t)2 * M * M, 0.0);
+            const int lim = std::min(M_in / 2, M / 2);
+            for (int w = 0; w < 2; ++w)
+                for (int a = 0; a < M; ++a) {
+                    const int oa = a <= M / 2 ? a : a - M;
+                    if (std::abs(oa) > lim) continue;
+                    const int ia = (oa + M_in) % M_in;
+                    for (int b = 0; b < M; ++b) {
+                        const int ob = b <= M / 2 ? b : b - M;
+                        if (std::abs(ob) > lim) continue;
+                        h[((size_t)w * M + a) * M + b] = kvals[((size_t)w * M_in + ia) * M_in + (ob + M_in) % M_in];
+                    }
+                }
+            double* d = reinterpret_cast<double*>(table.ensure(h.size() * sizeof(double)));
+            MG_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            MG_HIP(hipStreamSynchronize(st));
+            tab = d;
+        }
+        launch_spectra<T>(M, delta, kdof, tab, tsne_kernel && sizeof(T) == 4, tmp, dptr<T>(s.K1), dptr<T>(s.K2), st);
+        return s;
+    }
+
+    void ensure(i64 n, int N, int M) {
+        const size_t nk = (size_t)std::max<i64>(n, 1);
+        keysA.ensure(nk * sizeof(int)); valsA.ensure(nk * sizeof(int));
+        keysB.ensure(nk * sizeof(int)); valsB.ensure(nk * sizeof(int));
+        hist.ensure((size_t)256 * std::max(1, sort_hist_blocks(n)) * sizeof(int));
+        const int nch = std::max(1, spread_chunks(n));
+        item_key.ensure((size_t)2 * nch * sizeof(int));
+        item_valid.ensure((size_t)2 * nch * sizeof(int));
+        item_val.ensure((size_t)2 * nch * ni * 3 * ni * sizeof(T));
+        const int Mh = M / 2 + 1;
+        W.ensure((size_t)3 * N * N * sizeof(T));
+        Wh.ensure((size_t)3 * Mh * N * sizeof(cplx<T>));
+        Ph.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
+        if (2 * M * (int)sizeof(cplx<T>) > fft_lds_max()) stash.ensure((size_t)Mh * M * sizeof(cplx<T>));
+        phi.ensure((size_t)4 * N * N * sizeof(T));
+        zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
+    }
+
+    RepBuffers<T> bufs() {
+        RepBuffers<T> b;
+        b.keysA = dptr<int>(keysA); b.valsA = dptr<int>(valsA); b.keysB = dptr<int>(keysB); b.valsB = dptr<int>(valsB);
+        b.hist = dptr<int>(hist); b.item_key = dptr<int>(item_key); b.item_valid = dptr<int>(item_valid);
+        b.item_val = dptr<T>(item_val); b.W = dptr<T>(W); b.Wh = dptr<cplx<T>>(Wh); b.Ph = dptr<cplx<T>>(Ph);
+        b.colstash = dptr<cplx<T>>(stash); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
+        b.counter = dptr<unsigned>(counter); b.zout = dptr<double>(zout); b.inv_out = dptr<T>(inv_out);
+        return b;
+    }
+
+    int device_M(int N) const {
+        int M = pow2_at_least(2 * (i64)N);
+        if (M < 4) M = 4;
+        return M;
+    }
+
+    // enqueue the whole repulsion; sumQ and 1/sumQ stay on the device (zout, inv_out)
+    void compute(const T* Y, i64 n, const GridT<T>& g, T* F, double* sumq_hist, hipStream_t st) {
+        require(n >= 1, "Y must be (n, 2) with n >= 1");
+        require(g.N >= 1 && g.width > T(0), "grid larger than the buffer");
+        require(n < (i64)INT32_MAX, "too many points");
+        if (!tsne_kernel) require(M_in >= 2 * g.N - 1, "padded size below 2N - 1 (aliased convolution) is not supported", MG_ERR_UNSUP);
+        const int M = device_M(g.N);
+        require(M <= MAX_M, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+        ensure(n, g.N, M);
+        Prof& P = g_prof;
+        const double ts = sizeof(T), NN = (double)g.N * g.N, cs = 2 * ts, Mh = M / 2 + 1, dn = (double)n;
+        P.begin(ST_SPECTRA, st);
+        Spec& s = spectra(M, st);
+        P.end(ST_SPECTRA, st);
+        RepBuffers<T> b = bufs();
+        P.begin(ST_SORT, st);
+        sorted_buf = launch_box_sort<T>(Y, n, g, b, st);
+        P.end(ST_SORT, st, 2 * dn * ts + 8 * dn);
+        const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
+        const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
+        P.begin(ST_SPREAD, st);
+        launch_spread<T>(keys, vals, Y, n, g, ni, b, st);
+        P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
+        P.begin(ST_ROWS_FWD, st);
+        launch_rows_forward<T>(b.W, g.N, M, b.Wh, st);
+        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);
+        P.begin(ST_COLUMNS, st);
+        launch_columns<T>(b.Wh, g.N, M, dptr<T>(s.K1), dptr<T>(s.K2), b.Ph, b.colstash, st);
+        P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + 2 * Mh * M * ts + 4 * Mh * g.N * cs);
+        P.begin(ST_ROWS_INV, st);
+        launch_rows_inverse<T>(b.Ph, g.N, M, b.phi, st);
+        P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts);
+        P.begin(ST_GATHER, st);
+        launch_gather<T>(keys, vals, Y, n, g, ni, b.phi, F, b, sumq_hist, st);
+        P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
+        last_N = g.N; last_M = M; last_n = n;
+    }
+};
+
+// ------------------------------------------------------------------------------ whole loop
+template <class T> struct Optimizer {
+    Repulsion<T> rep;
+    DevBuf attr, F, grad, part, upd_counter, mb, bb_part, bb_counter, bb_out, sumq;
+    PinnedBuf host_mb;
+    Optimizer(int ni) : rep(0, ni) {}
+
+    int run(const mg_tsne_params* p, const int32_t* ip, const int32_t* ix, const T* data, i64 nnz, i64 n, T* Y, T* U,
+            T* G, const int32_t* pairs, int m, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user,
+            hipStream_t st) {
+        require(p != nullptr, "params must not be null");
+        require(n >= 1 && Y && U && G, "Y, U, G must be (n, 2) device arrays");
+        require(p->n_iter >= 0 && it0 >= 0 && it1 >= it0 && it1 <= p->n_iter, "iteration range outside the schedule");
+        require(p->dof > 0, "dof must be positive");
+        require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
+        require(!pairs || m >= 1, "pairs must be (n, m) with m >= 1");
+        validate_csr(ip, ix, n, nnz, st);
+        if (getenv("MAGNET_DEBUG")) { g_prof.on = true; g_prof.debug = true; }
+        rep.have_kernel = true; rep.tsne_kernel = true; rep.kdof = p->dof; rep.kwidth = p->box_width;
+        const double lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
+        const bool fused = !(pairs && m > 0 && p->mid_weight > 0);
+        attr.ensure((size_t)2 * n * sizeof(T));
+        F.ensure((size_t)2 * n * sizeof(T));
+        if (!fused) grad.ensure((size_t)2 * n * sizeof(T));
+        part.ensure((size_t)6 * 1024 * sizeof(double));
+        unsigned* uc = reinterpret_cast<unsigned*>(upd_counter.ensure(64));
+        unsigned* bc = reinterpret_cast<unsigned*>(bb_counter.ensure(64));
+        if (upd_counter.bytes && !counters_zeroed) {
+            MG_HIP(hipMemsetAsync(uc, 0, 64, st)); MG_HIP(hipMemsetAsync(bc, 0, 64, st)); counters_zeroed = true;
+        }
+        mb.ensure(8 * sizeof(T));
+        bb_part.ensure((size_t)4 * 1024 * sizeof(T));
+        bb_out.ensure(4 * sizeof(T));
+        double* dsum = reinterpret_cast<double*>(sumq.ensure((size_t)std::max(1, it1 - it0) * sizeof(double)));
+        T* hmb = reinterpret_cast<T*>(host_mb.ensure(8 * sizeof(T)));
+        // bbox of the starting layout
+        launch_bbox<T>(Y, n, dptr<T>(bb_part), bc, dptr<T>(bb_out), st);
+        MG_HIP(hipMemcpyAsync(hmb + 2, dptr<T>(bb_out), 4 * sizeof(T), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipStreamSynchronize(st));
+        const T inv_dof = T(1) / (T)p->dof;
+        int stopped = 0;
+        for (int it = it0; it < it1; ++it) {
+            const double bb[4] = {(double)hmb[2], (double)hmb[3], (double)hmb[4], (double)hmb[5]};
+            require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
+                    "layout is not finite");
+            const GridDecision gd = decide_grid(bb, p->box_width, p->min_boxes, p->max_boxes);
+            const GridT<T> g = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, p->n_interp);
+            rep.kwidth = gd.width;
+            const bool early = it < p->early_iter;
+            const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
+            const T mom = (T)(early ? p->momentum0 : p->momentum1);
+            g_prof.begin(ST_ATTR, st);
+            launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
+            g_prof.end(ST_ATTR, st, 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T));
+            rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st);
+            const T* dinv = dptr<T>(rep.inv_out);
+            g_prof.begin(ST_UPDATE, st);
+            if (fused) {
+                launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
+                                 dptr<double>(part), uc, dptr<T>(mb), st);
+            } else {
+                const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
+                launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
+                               dptr<T>(grad), st);
+                launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
+                                 dptr<double>(part), uc, dptr<T>(mb), st);
+            }
+            g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
+            MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
+            MG_HIP(hipStreamSynchronize(st));
+            g_prof.collect(it);
+            if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
+                if (cb(it, Y, n, user) != 0) { stopped = 1; it1 = it + 1; break; }
+            }
+        }
+        if (h_sumQ && it1 > it0) {
+            MG_HIP(hipMemcpyAsync(h_sumQ, dsum, (size_t)(it1 - it0) * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        MG_HIP(hipStreamSynchronize(st));
+        return stopped;
+    }
+    bool counters_zeroed = false;
+};
+
+}  // namespace mg
+
+// ================================================================================== C-ABI
+using namespace mg;
+
+struct mg_fftconv {
+    int prec;
+    std::unique_ptr<Repulsion<float>> f32;
+    std::unique_ptr<Repulsion<double>> f64;
+};
+
+#define MG_API_BEGIN try {
+#define MG_API_END                                                           \
+    }                                                                        \
+    catch (const mg::Error& e) { g_err = e.what(); return e.code; }          \
+    catch (const std::bad_alloc&) { g_err = "host allocation failed"; return MG_ERR_NOMEM; } \
+    catch (const std::exception& e) { g_err = e.what(); return MG_ERR_HIP; } \
+    return MG_OK;
+
+static void check_prec(int prec) { require(prec == MG_F32 || prec == MG_F64, "prec must be MG_F32 or MG_F64"); }
+
+template <class T> static void step_impl(int mode, void* Y, void* U, void* G, const void* A, const void* F, int64_t n, int dim,
+                                         double rho, double inv, double lr, double mom) {
+    require(n >= 1 && dim >= 1 && dim <= 64, "arrays must match Y");
+    require(Y && U && G && A && (mode == 1 || F), "null array");
+    DevBuf part, cnt, mb;
+    unsigned* c = reinterpret_cast<unsigned*>(cnt.ensure(64));
+    MG_HIP(hipMemsetAsync(c, 0, 64, g_stream));
+    launch_update<T>(mode, (T*)Y, (T*)U, (T*)G, (const T*)A, (const T*)F, nullptr, (T)inv, (T)rho, (T)lr, (T)mom, nullptr, 0,
+                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), c, (T*)mb.ensure(std::max(8, dim) * sizeof(T)), g_stream);
+    MG_HIP(hipStreamSynchronize(g_stream));
+}
+
+template <class T> static void set_values(Repulsion<T>& r, int M_in, const double* K1, const double* K2) {
+    require(M_in >= 1 && K1 && K2, "K1 / K2 must be (M, M) tables");
+    r.kvals.assign((size_t)2 * M_in * M_in, 0.0);
+    std::memcpy(r.kvals.data(), K1, (size_t)M_in * M_in * sizeof(double));
+    std::memcpy(r.kvals.data() + (size_t)M_in * M_in, K2, (size_t)M_in * M_in * sizeof(double));
+    r.M_in = M_in; r.tsne_kernel = false; r.have_kernel = true; r.cache.clear();
+}
+
+extern "C" {
+
+const char* mg_last_error(void) { return g_err.c_str(); }
+int mg_version(void) { return 100; }
+
+int mg_set_stream(void* s) { MG_API_BEGIN g_stream = reinterpret_cast<hipStream_t>(s); MG_API_END }
+int mg_synchronize(void) { MG_API_BEGIN MG_HIP(hipStreamSynchronize(g_stream)); MG_API_END }
+int mg_malloc(void** p, size_t bytes) {
+    MG_API_BEGIN
+    require(p != nullptr, "null output pointer");
+    *p = nullptr;
+    MG_HIP(hipMalloc(p, bytes ? bytes : 1));
+    MG_API_END
+}
+int mg_free(void* p) { MG_API_BEGIN if (p) MG_HIP(hipFree(p)); MG_API_END }
+int mg_memcpy_htod(void* d, const void* h, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, g_stream)); MG_HIP(hipStreamSynchronize(g_stream)); } MG_API_END }
+int mg_memcpy_dtoh(void* h, const void* d, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, g_stream)); MG_HIP(hipStreamSynchronize(g_stream)); } MG_API_END }
+int mg_memcpy_dtod(void* d, const void* s, size_t b) { MG_API_BEGIN if (b) MG_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, g_stream)); MG_API_END }
+int mg_mem_info(size_t* f, size_t* t) { MG_API_BEGIN require(f && t, "null output pointer"); MG_HIP(hipMemGetInfo(f, t)); MG_API_END }
+
+int mg_profile_enable(int on) {
+    MG_API_BEGIN
+    g_prof.on = on != 0;
+    MG_API_END
+}
+int mg_profile_reset(void) { MG_API_BEGIN g_prof.reset(); MG_API_END }
+int mg_profile_num_stages(void) { return NSTAGE; }
+int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* count, double* algo_bytes) {
+    MG_API_BEGIN
+    require(stage >= 0 && stage < NSTAGE, "stage out of range");
+    if (name) *name = kStageNames[stage];
+    if (total_ms) *total_ms = g_prof.total[stage];
+    if (algo_bytes) *algo_bytes = g_prof.bytes[stage];
+    if (count) *count = g_prof.count[stage];
+    MG_API_END
+}
+
+int mg_bbox_2d(int prec, const void* Y, int64_t n, double* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(Y && out && n >= 1, "Y must be (n, 2) with n >= 1");
+    DevBuf part, cnt, res;
+    unsigned* c = reinterpret_cast<unsigned*>(cnt.ensure(64));
+    MG_HIP(hipMemsetAsync(c, 0, 64, g_stream));
+    if (prec == MG_F64) {
+        launch_bbox<double>((const double*)Y, n, (double*)part.ensure(4 * 1024 * 8), c, (double*)res.ensure(32), g_stream);
+        MG_HIP(hipMemcpyAsync(out, res.p, 32, hipMemcpyDeviceToHost, g_stream));
+        MG_HIP(hipStreamSynchronize(g_stream));
+    } else {
+        float h[4];
+        launch_bbox<float>((const float*)Y, n, (float*)part.ensure(4 * 1024 * 4), c, (float*)res.ensure(16), g_stream);
+        MG_HIP(hipMemcpyAsync(h, res.p, 16, hipMemcpyDeviceToHost, g_stream));
+        MG_HIP(hipStreamSynchronize(g_stream));
+        for (int k = 0; k < 4; ++k) out[k] = h[k];
+    }
+    MG_API_END
+}
+
+int mg_tsne_attraction(int prec, const int32_t* ip, const int32_t* ix, const void* data, int64_t nnz, const void* Y,
+                       int64_t n, int dim, double dof, void* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(dof > 0, "dof must be positive");
+    require(n >= 0 && dim >= 1 && dim <= 64, "dim out of range");
+    require(n == 0 || (Y && out && (nnz == 0 || data)), "null array");
+    validate_csr(ip, ix, n, nnz, g_stream);
+    if (prec == MG_F64) launch_attraction<double>(ip, ix, (const double*)data, (const double*)Y, n, dim, 1.0 / dof, (double*)out, nnz, g_stream);
+    else launch_attraction<float>(ip, ix, (const float*)data, (const float*)Y, n, dim, 1.f / (float)dof, (float*)out, nnz, g_stream);
+    MG_API_END
+}
+
+int mg_tsne_step(int prec, void* Y, void* U, void* G, const void* A, const void* F, int64_t n, int dim, double rho,
+                 double inv, double lr, double mom) {
+    MG_API_BEGIN
+    check_prec(prec);
+    if (prec == MG_F64) step_impl<double>(0, Y, U, G, A, F, n, dim, rho, inv, lr, mom);
+    else step_impl<float>(0, Y, U, G, A, F, n, dim, rho, inv, lr, mom);
+    MG_API_END
+}
+
+int mg_tsne_update(int prec, void* Y, void* U, void* G, const void* grad, int64_t n, int dim, double lr, double mom) {
+    MG_API_BEGIN
+    check_prec(prec);
+    if (prec == MG_F64) step_impl<double>(1, Y, U, G, grad, nullptr, n, dim, 0, 0, lr, mom);
+    else step_impl<float>(1, Y, U, G, grad, nullptr, n, dim, 0, 0, lr, mom);
+    MG_API_END
+}
+
+int mg_mid_spring(int prec, const void* Y, const int32_t* pairs, void* grad, int64_t n, int dim, int m, double w) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(n >= 0 && dim >= 1 && dim <= 64 && m >= 1, "bad shapes");
+    require(n == 0 || (Y && pairs && grad), "null array");
+    if (prec == MG_F64) launch_mid_spring<double>((const double*)Y, pairs, (double*)grad, n, dim, m, w, g_stream);
+    else launch_mid_spring<float>((const float*)Y, pairs, (float*)grad, n, dim, m, (float)w, g_stream);
+    MG_API_END
+}
+
+int mg_fftconv_create(mg_fftconv** out, int prec, int M, int n_interp) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(out != nullptr, "null output pointer");
+    require(M >= 1 && n_interp >= 1 && n_interp <= 8, "bad grid parameters");
+    std::unique_ptr<mg_fftconv> c(new mg_fftconv());
+    c->prec = prec;
+    if (prec == MG_F64) c->f64.reset(new Repulsion<double>(M, n_interp));
+    else c->f32.reset(new Repulsion<float>(M, n_interp));
+    *out = c.release();
+    MG_API_END
+}
+
+int mg_fftconv_destroy(mg_fftconv* c) { MG_API_BEGIN delete c; MG_API_END }
+
+int mg_fftconv_set_kernel_values(mg_fftconv* c, int M_in, const double* K1, const double* K2) {
+    MG_API_BEGIN
+    require(c != nullptr, "null object");
+    if (c->prec == MG_F64) set_values(*c->f64, M_in, K1, K2); else set_values(*c->f32, M_in, K1, K2);
+    MG_API_END
+}
+
+int mg_fftconv_set_kernels_tsne(mg_fftconv* c, double width, double dof) {
+    MG_API_BEGIN
+    require(c != nullptr, "null object");
+    require(width > 0 && dof > 0, "width and dof must be positive");
+    auto set = [&](auto& r) { r.tsne_kernel = true; r.have_kernel = true; r.kwidth = width; r.kdof = dof; };
+    if (c->prec == MG_F64) set(*c->f64); else set(*c->f32);
+    MG_API_END
+}
+
+int mg_fftconv_compute(mg_fftconv* c, const void* Y, int64_t n, double xmin, double ymin, double width, int n_boxes,
+                       void* F, double* h_sumQ) {
+    MG_API_BEGIN
+    require(c != nullptr, "null object");
+    require(Y && F, "null array");
+    require(n_boxes >= 1 && width > 0, "grid larger than the buffer");
+    auto run = [&](auto& r, auto tag) {
+        typedef decltype(tag) T;
+        require(n_boxes * r.ni <= r.M_user, "grid larger than the buffer");
+        const GridT<T> g = make_grid<T>(xmin, ymin, width, n_boxes, r.ni);
+        r.compute((const T*)Y, n, g, (T*)F, nullptr, g_stream);
+        double z[2];
+        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, g_stream));
+        MG_HIP(hipStreamSynchronize(g_stream));
+        if (h_sumQ) *h_sumQ = z[0];
+    };
+    if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));
+    MG_API_END
+}
+
+int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
+    MG_API_BEGIN
+    require(c != nullptr, "null object");
+    auto copy = [&](auto& r, size_t ts) {
+        require(r.last_N > 0, "no compute yet");
+        const size_t NN = (size_t)r.last_N * r.last_N;
+        if (d_W) MG_HIP(hipMemcpyAsync(d_W, r.W.p, 3 * NN * ts, hipMemcpyDeviceToDevice, g_stream));
+        if (d_phi) MG_HIP(hipMemcpyAsync(d_phi, r.phi.p, 4 * NN * ts, hipMemcpyDeviceToDevice, g_stream));
+        MG_HIP(hipStreamSynchronize(g_stream));
+    };
+    if (c->prec == MG_F64) copy(*c->f64, sizeof(double)); else copy(*c->f32, sizeof(float));
+    MG_API_END
+}
+
+int mg_tsne_optimize(int prec, const mg_tsne_params* p, const int32_t* ip, const int32_t* ix, const void* data, int64_t nnz,
+                     int64_t n, void* Y, void* U, void* G, const int32_t* pairs, int m, int it0, int it1, double* h_sumQ,
+                     mg_tsne_callback cb, void* user) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(p != nullptr, "params must not be null");
+    require(nnz == 0 || data != nullptr, "data must be a device array");
+    if (prec == MG_F64) {
+        Optimizer<double> o(p->n_interp);
+        o.run(p, ip, ix, (const double*)data, nnz, n, (double*)Y, (double*)U, (double*)G, pairs, m, it0, it1, h_sumQ, cb, user, g_stream);
+    } else {
+        Optimizer<float> o(p->n_interp);
+        o.run(p, ip, ix, (const float*)data, nnz, n, (float*)Y, (float*)U, (float*)G, pairs, m, it0, it1, h_sumQ, cb, user, g_stream);
+    }
+    MG_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,462 @@
+// Point <-> grid kernels of the FFT repulsion (src/fftconv.cpp:83-123 and :200-229):
+//   1. box sort: a stable LSD radix sort of (box key, point index), 8-bit digits, ranks by wave ballots
+//      (no float atomics anywhere, so the order and every sum below are the same on every run);
+//   2. spread: a segmented reduction of the Lagrange-weighted charges (1, x, y) over the sorted points.
+//      A box's n_interp^2 nodes belong to that box only, so each box is written exactly once: by the
+//      chunk that holds all its points, or by the span pass for boxes whose points cross chunk ends;
+//   3. gather: potentials at the points in box order, forces, and Z by a fixed-order reduction.
+#include "common.hpp"
+
+namespace mg {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = TPB * RS_ITEMS;
+constexpr int SP_PT = 4;                 // points per thread in the spread
+constexpr int SP_CHUNK = TPB * SP_PT;    // points per spread chunk (one workgroup)
+
+// local coordinates and weights of a point whose box is known from its key (src/fftconv.cpp:105-111)
+template <class T, int NI> __device__ inline void weights(T x, T y, int key, const GridT<T>& g, int& bx, int& by,
+                                                          T* wx, T* wy) {
+    bx = key / g.nb; by = key - bx * g.nb;
+    const T ux = (x - g.xmin) / g.width, uy = (y - g.ymin) / g.width;
+    T fx = ux - (T)bx, fy = uy - (T)by;
+    fx = fx < T(0) ? T(0) : (fx > T(1) ? T(1) : fx);
+    fy = fy < T(0) ? T(0) : (fy > T(1) ? T(1) : fy);
+    lagrange<T, NI>(fx, wx);
+    lagrange<T, NI>(fy, wy);
+}
+
+// ---------------------------------------------------------------------------------------------- sort
+template <class T, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
+                                                 i64 n, int shift, int* __restrict__ hist, int nblocks) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    typedef typename vec2<T>::type V;
+    const i64 base = (i64)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const i64 i = base + r * TPB + threadIdx.x;
+        if (i < n) {
+            int key;
+            if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); }
+            else key = keys_in[i];
+            atomicAdd(&h[(key >> shift) & 255], 1);
+        }
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of the digit-major histogram (one workgroup)
+__global__ __launch_bounds__(1024) void k_rs_scan(int* __restrict__ hist, int total) {
+    __shared__ int sh[1024];
+    const int t = threadIdx.x;
+    const int per = (total + 1023) / 1024;
+    const int s0 = min(total, t * per), s1 = min(total, s0 + per);
+    int s = 0;
+    for (int k = s0; k < s1; ++k) s += hist[k];
+    sh[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int off = sh[t] - s;
+    for (int k = s0; k < s1; ++k) { const int v = hist[k]; hist[k] = off; off += v; }
+}
+
+// stable scatter: an item's rank is (items of its digit in earlier blocks) + (in earlier rounds of this
+// block) + (in earlier waves of this round) + (in earlier lanes of its wave, from 8 ballots)
+template <class T, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
+                                                    const int* __restrict__ vals_in, i64 n, int shift,
+                                                    const int* __restrict__ hist, int nblocks, int* __restrict__ keys_out,
+                                                    int* __restrict__ vals_out) {
+    __shared__ int off[256];
+    __shared__ int wcnt[TPB / WAVE][256];
+    const int t = threadIdx.x, wave = t / WAVE, lane = t % WAVE;
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    off[t] = hist[t * nblocks + blockIdx.x];
+    typedef typename vec2<T>::type V;
+    const i64 base = (i64)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const i64 i = base + r * TPB + t;
+        const bool valid = i < n;
+        int key = 0, val = 0;
+        if (valid) {
+            if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); val = (int)i; }
+            else { key = keys_in[i]; val = vals_in[i]; }
+        }
+        const int d = (key >> shift) & 255;
+#pragma unroll
+        for (int w = 0; w < TPB / WAVE; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = off[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            keys_out[pos] = key; vals_out[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < TPB / WAVE; ++w) add += wcnt[w][t];
+        off[t] += add;
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------------------------------- spread
+// combine of two segmented-scan elements, `a` before `b`
+template <class T, int V> __device__ inline void seg_combine(bool fa, const T* va, bool& fb, T* vb) {
+    if (!fb) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) vb[v] = va[v] + vb[v];
+    }
+    fb = fa || fb;
+}
+
+template <class T, int NI>
+__global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, const int* __restrict__ vals,
+                                                const T* __restrict__ Y, i64 n, GridT<T> g, T* __restrict__ W,
+                                                int* __restrict__ item_key, int* __restrict__ item_valid,
+                                                T* __restrict__ item_val) {
+    constexpr int V = 3 * NI;
+    __shared__ T sA[V][TPB];
+    __shared__ int sF[TPB];
+    typedef typename vec2<T>::type V2;
+    const int t = threadIdx.x;
+    const i64 c0 = (i64)blockIdx.x * SP_CHUNK;
+    const i64 cend = min(c0 + (i64)SP_CHUNK, n);
+    const int key_first = keys[c0], key_last = keys[cend - 1];
+    const bool starts_new = c0 == 0 || keys[c0 - 1] != key_first;
+    const bool ends_closed = cend == n || keys[cend] != key_last;
+    if (t == 0) {
+        item_key[2 * blockIdx.x] = key_first;
+        item_valid[2 * blockIdx.x] = !starts_new;
+        item_key[2 * blockIdx.x + 1] = key_last;
+        item_valid[2 * blockIdx.x + 1] = !ends_closed && (key_first != key_last || starts_new);
+    }
+    const i64 p0 = c0 + (i64)t * SP_PT;
+    int key[SP_PT];
+    bool valid[SP_PT], head[SP_PT], endseg[SP_PT];
+    T x[SP_PT], y[SP_PT], wx[SP_PT][NI], wy[SP_PT][NI];
+    int first_head = SP_PT;
+#pragma unroll
+    for (int j = 0; j < SP_PT; ++j) {
+        const i64 p = p0 + j;
+        valid[j] = p < cend;
+        key[j] = valid[j] ? keys[p] : -1;
+        head[j] = false; endseg[j] = false;
+        x[j] = y[j] = T(0);
+#pragma unroll
+        for (int a = 0; a < NI; ++a) wx[j][a] = wy[j][a] = T(0);
+        if (!valid[j]) continue;
+        const V2 pt = reinterpret_cast<const V2*>(Y)[vals[p]];
+        x[j] = pt.x; y[j] = pt.y;
+        int bx, by;
+        weights<T, NI>(x[j], y[j], key[j], g, bx, by, wx[j], wy[j]);
+        head[j] = p == c0 || keys[p - 1] != key[j];
+        endseg[j] = p == cend - 1 || keys[p + 1] != key[j];
+        if (first_head == SP_PT && head[j]) first_head = j;
+    }
+
+    for (int a = 0; a < NI; ++a) {
+        // local inclusive segmented scan of this thread's points (reset at its first point and at heads)
+        T s[SP_PT][V];
+#pragma unroll
+        for (int j = 0; j < SP_PT; ++j) {
+#pragma unroll
+            for (int b = 0; b < NI; ++b) {
+                const T w = wx[j][a] * wy[j][b];   // zero weights for positions past the end
+                const T v0 = w, v1 = w * x[j], v2 = w * y[j];
+                if (j == 0 || head[j]) {
+                    s[j][3 * b + 0] = v0; s[j][3 * b + 1] = v1; s[j][3 * b + 2] = v2;
+                } else {
+                    s[j][3 * b + 0] = s[j - 1][3 * b + 0] + v0;
+                    s[j][3 * b + 1] = s[j - 1][3 * b + 1] + v1;
+                    s[j][3 * b + 2] = s[j - 1][3 * b + 2] + v2;
+                }
+            }
+        }
+        // block exclusive segmented scan of the per-thread aggregates (Hillis-Steele, fixed order)
+        bool f = first_head < SP_PT;
+        T agg[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) agg[v] = s[SP_PT - 1][v];
+        sF[t] = f;
+#pragma unroll
+        for (int v = 0; v < V; ++v) sA[v][t] = agg[v];
+        __syncthreads();
+        for (int d = 1; d < TPB; d <<= 1) {
+            bool fo = false;
+            T vo[V];
+            if (t >= d) {
+                fo = sF[t - d];
+#pragma unroll
+                for (int v = 0; v < V; ++v) vo[v] = sA[v][t - d];
+            }
+            __syncthreads();
+            if (t >= d) {
+                seg_combine<T, V>(fo, vo, f, agg);
+                sF[t] = f;
+#pragma unroll
+                for (int v = 0; v < V; ++v) sA[v][t] = agg[v];
+            }
+            __syncthreads();
+        }
+        T carry[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) carry[v] = t > 0 ? sA[v][t - 1] : T(0);
+        __syncthreads();
+        // segment ends: the whole segment is in this chunk -> grid; else -> the chunk's item slot
+#pragma unroll
+        for (int j = 0; j < SP_PT; ++j) {
+            if (!endseg[j]) continue;
+            T tot[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) tot[v] = j < first_head ? carry[v] + s[j][v] : s[j][v];
+            const bool inc_start = key[j] == key_first, inc_end = p0 + j == cend - 1;
+            const bool complete = (!inc_start || starts_new) && (!inc_end || ends_closed);
+            if (complete) {
+                const int bx = key[j] / g.nb, by = key[j] - bx * g.nb;
+                const int gx = bx * NI + a;
+#pragma unroll
+                for (int b = 0; b < NI; ++b) {
+                    const int gy = by * NI + b;
+                    W[((size_t)0 * g.N + gx) * g.N + gy] = tot[3 * b + 0];
+                    W[((size_t)1 * g.N + gx) * g.N + gy] = tot[3 * b + 1];
+                    W[((size_t)2 * g.N + gx) * g.N + gy] = tot[3 * b + 2];
+                }
+            } else {
+                const int slot = (inc_start && !starts_new) ? 2 * blockIdx.x : 2 * blockIdx.x + 1;
+                T* dst = item_val + ((size_t)slot * NI + a) * V;
+#pragma unroll
+                for (int v = 0; v < V; ++v) dst[v] = tot[v];
+            }
+        }
+    }
+}
+
+// boxes whose points cross chunk ends: the chunk where such a box starts sums its open last item with
+// the first items of the following chunks holding the same key (found by binary search on the chunk
+// first keys), in a fixed tree order
+template <class T, int NI>
+__global__ __launch_bounds__(TPB) void k_spread_span(const int* __restrict__ keys, i64 n, int nchunks, GridT<T> g,
+                                                     T* __restrict__ W, const int* __restrict__ item_key,
+                                                     const int* __restrict__ item_valid, const T* __restrict__ item_val) {
+    constexpr int V = 3 * NI;
+    __shared__ T sA[V][TPB];
+    __shared__ int s_c1;
+    const int c = blockIdx.x, t = threadIdx.x;
+    if (!item_valid[2 * c + 1]) return;
+    const int skey = item_key[2 * c + 1];
+    if (t == 0) {
+        int lo = c + 1, hi = nchunks - 1;   // keys[lo * CHUNK] == skey holds; find the last such chunk
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (keys[(i64)mid * SP_CHUNK] == skey) lo = mid; else hi = mid - 1;
+        }
+        s_c1 = lo;
+    }
+    __syncthreads();
+    const int count = 1 + (s_c1 - c);   // item 0: slot 2c + 1; item k >= 1: slot 2(c + k)
+    const int bx = skey / g.nb, by = skey - bx * g.nb;
+    for (int a = 0; a < NI; ++a) {
+        T acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = T(0);
+        for (int k = t; k < count; k += TPB) {
+            const int slot = k == 0 ? 2 * c + 1 : 2 * (c + k);
+            const T* src = item_val + ((size_t)slot * NI + a) * V;
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] += src[v];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) sA[v][t] = acc[v];
+        __syncthreads();
+        for (int s = TPB / 2; s > 0; s >>= 1) {
+            if (t < s) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) sA[v][t] += sA[v][t + s];
+            }
+            __syncthreads();
+        }
+        if (t < V) {
+            const int b = t / 3, ch = t % 3;
+            W[((size_t)ch * g.N + bx * NI + a) * g.N + by * NI + b] = sA[t][0];
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------------------------------- gather
+template <class T> struct vec4;
+template <> struct vec4<float> { typedef float4 type; };
+template <> struct vec4<double> { typedef double4 type; };
+
+__device__ inline bool gather_last_block(unsigned* counter) {
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <class T, int NI>
+__global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
+                                                const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
+                                                T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
+                                                double* __restrict__ zout, T* __restrict__ inv_out,
+                                                double* __restrict__ sumq_hist) {
+    __shared__ double shd[TPB];
+    typedef typename vec2<T>::type V2;
+    typedef typename vec4<T>::type V4;
+    const V4* phi4 = reinterpret_cast<const V4*>(phi);
+    double z = 0;
+    for (i64 p = (i64)blockIdx.x * TPB + threadIdx.x; p < n; p += (i64)gridDim.x * TPB) {
+        const int key = keys[p], idx = vals[p];
+        const V2 pt = reinterpret_cast<const V2*>(Y)[idx];
+        int bx, by;
+        T wx[NI], wy[NI];
+        weights<T, NI>(pt.x, pt.y, key, g, bx, by, wx, wy);
+        T p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+#pragma unroll
+        for (int a = 0; a < NI; ++a) {
+            const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+#pragma unroll
+            for (int b = 0; b < NI; ++b) {
+                const T w = wx[a] * wy[b];
+                const V4 c = phi4[row + b];
+                p0 += w * c.x; p1 += w * c.y; p2 += w * c.z; p3 += w * c.w;
+            }
+        }
+        z += (double)p0;
+        V2 f; f.x = p1 * pt.x - p2; f.y = p1 * pt.y - p3;
+        reinterpret_cast<V2*>(F)[idx] = f;
+    }
+    z = block_sum(z, shd);
+    if (threadIdx.x == 0) zpart[blockIdx.x] = z;
+    if (!gather_last_block(counter)) return;
+    double s = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) s += zpart[b];
+    s = block_sum(s, shd);
+    if (threadIdx.x == 0) {
+        const double sumq = s - (double)n;
+        const double inv = 1.0 / (sumq > 1e-12 ? sumq : 1e-12);
+        zout[0] = sumq; zout[1] = inv;
+        *inv_out = (T)inv;
+        if (sumq_hist) *sumq_hist = sumq;
+        *counter = 0u;
+    }
+}
+
+}  // namespace
+
+int sort_hist_blocks(i64 n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
+int spread_chunks(i64 n) { return (int)((n + SP_CHUNK - 1) / SP_CHUNK); }
+int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024); }
+
+template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
+    const int nblocks = sort_hist_blocks(n);
+    const long long nkeys = (long long)g.nb * g.nb;
+    int bits = 0;
+    while ((1ll << bits) < nkeys) ++bits;
+    const int passes = bits <= 8 ? 1 : (bits + 7) / 8;
+    int* kin = nullptr; int* vin = nullptr;
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        int* kout = (p % 2 == 0) ? b.keysA : b.keysB;
+        int* vout = (p % 2 == 0) ? b.valsA : b.valsB;
+        if (p == 0) hipLaunchKernelGGL((k_rs_hist<T, true>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
+        else hipLaunchKernelGGL((k_rs_hist<T, false>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
+        MG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, b.hist, 256 * nblocks);
+        MG_HIP(hipGetLastError());
+        if (p == 0) hipLaunchKernelGGL((k_rs_scatter<T, true>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, nblocks, kout, vout);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, nblocks, kout, vout);
+        MG_HIP(hipGetLastError());
+        kin = kout; vin = vout; cur = p % 2;
+    }
+    return cur;
+}
+
+template <class T, int NI> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
+                                                 RepBuffers<T>& b, hipStream_t st) {
+    const int nch = spread_chunks(n);
+    hipLaunchKernelGGL((k_spread<T, NI>), dim3(nch), dim3(TPB), 0, st, keys, vals, Y, n, g, b.W, b.item_key, b.item_valid, b.item_val);
+    MG_HIP(hipGetLastError());
+    if (nch > 1) {
+        hipLaunchKernelGGL((k_spread_span<T, NI>), dim3(nch), dim3(TPB), 0, st, keys, n, nch, g, b.W, b.item_key, b.item_valid, b.item_val);
+        MG_HIP(hipGetLastError());
+    }
+}
+
+template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
+                                      RepBuffers<T>& b, hipStream_t st) {
+    launch_fill<T>(b.W, T(0), (i64)3 * g.N * g.N, st);
+    if (n == 0) return;
+    switch (ni) {
+        case 1: spread_ni<T, 1>(keys, vals, Y, n, g, b, st); break;
+        case 2: spread_ni<T, 2>(keys, vals, Y, n, g, b, st); break;
+        case 3: spread_ni<T, 3>(keys, vals, Y, n, g, b, st); break;
+        case 4: spread_ni<T, 4>(keys, vals, Y, n, g, b, st); break;
+        case 5: spread_ni<T, 5>(keys, vals, Y, n, g, b, st); break;
+        case 6: spread_ni<T, 6>(keys, vals, Y, n, g, b, st); break;
+        case 7: spread_ni<T, 7>(keys, vals, Y, n, g, b, st); break;
+        case 8: spread_ni<T, 8>(keys, vals, Y, n, g, b, st); break;
+        default: require(false, "n_interp must be in [1, 8]");
+    }
+}
+
+template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
+                                      const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st) {
+    const unsigned blocks = (unsigned)gather_blocks(n);
+#define MG_G(NI) hipLaunchKernelGGL((k_gather<T, NI>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist)
+    switch (ni) {
+        case 1: MG_G(1); break; case 2: MG_G(2); break; case 3: MG_G(3); break; case 4: MG_G(4); break;
+        case 5: MG_G(5); break; case 6: MG_G(6); break; case 7: MG_G(7); break; case 8: MG_G(8); break;
+        default: require(false, "n_interp must be in [1, 8]");
+    }
+#undef MG_G
+    MG_HIP(hipGetLastError());
+}
+
+#define MG_INST(T) \
+    template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t); \
+    template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, RepBuffers<T>&, hipStream_t); \
+    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, const T*, T*, RepBuffers<T>&, double*, hipStream_t);
+MG_INST(float)
+MG_INST(double)
+
+}  // namespace mg

@@ -1,0 +1,335 @@
+// Per-point kernels of the t-SNE pass: bounding box, CSR attraction, gradient combination and the
+// delta-bar-delta / momentum update with recentring. All reductions run in a fixed order (block tree,
+// then the last block over the block partials in index order), so every result is deterministic.
+#include "common.hpp"
+#include <cmath>
+#include <limits>
+
+namespace mg {
+
+namespace {
+
+constexpr int TPB = 256;
+
+// The last block of a grid: every block publishes its partial, then bumps `counter`; the block that
+// sees gridDim.x - 1 reads all partials (in block order) and resets the counter for the next launch.
+__device__ inline bool last_block(unsigned* counter) {
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last;
+}
+
+template <class T> struct MinOp { __device__ T operator()(T a, T b) const { return a < b ? a : b; } };
+template <class T> struct MaxOp { __device__ T operator()(T a, T b) const { return a > b ? a : b; } };
+template <class T> struct AddOp { __device__ T operator()(T a, T b) const { return a + b; } };
+
+// Tree reduction of one value per thread over a TPB-thread block; result valid in thread 0.
+template <class T, class Op> __device__ T block_reduce(T v, T* sh, Op op) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = op(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <class T> __device__ inline T pinf() { return std::numeric_limits<T>::infinity(); }
+
+// ---- bbox_2d (src/fftrep.cpp:415-434)
+template <class T>
+__global__ __launch_bounds__(TPB) void k_bbox(const T* __restrict__ Y, i64 n, T* __restrict__ part,
+                                              unsigned* counter, T* __restrict__ out) {
+    __shared__ T sh[TPB];
+    T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
+    typedef typename vec2<T>::type V;
+    const V* Y2 = reinterpret_cast<const V*>(Y);
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        const V p = Y2[i];
+        x0 = p.x < x0 ? p.x : x0; x1 = p.x > x1 ? p.x : x1;
+        y0 = p.y < y0 ? p.y : y0; y1 = p.y > y1 ? p.y : y1;
+    }
+    x0 = block_reduce(x0, sh, MinOp<T>()); y0 = block_reduce(y0, sh, MinOp<T>());
+    x1 = block_reduce(x1, sh, MaxOp<T>()); y1 = block_reduce(y1, sh, MaxOp<T>());
+    if (threadIdx.x == 0) { T* p = part + 4 * blockIdx.x; p[0] = x0; p[1] = y0; p[2] = x1; p[3] = y1; }
+    if (!last_block(counter)) return;
+    x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) {
+        const T* p = part + 4 * b;
+        x0 = p[0] < x0 ? p[0] : x0; y0 = p[1] < y0 ? p[1] : y0;
+        x1 = p[2] > x1 ? p[2] : x1; y1 = p[3] > y1 ? p[3] : y1;
+    }
+    x0 = block_reduce(x0, sh, MinOp<T>()); y0 = block_reduce(y0, sh, MinOp<T>());
+    x1 = block_reduce(x1, sh, MaxOp<T>()); y1 = block_reduce(y1, sh, MaxOp<T>());
+    if (threadIdx.x == 0) { out[0] = x0; out[1] = y0; out[2] = x1; out[3] = y1; *counter = 0u; }
+}
+
+// ---- tsne_attraction, 2-D path (src/fftrep.cpp:142-158): G lanes sweep one CSR row with stride G,
+// then a butterfly over the G lanes (the same order on every run).
+template <class T, int G>
+__global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                               const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
+                                               T* __restrict__ out) {
+    typedef typename vec2<T>::type V;
+    const V* Y2 = reinterpret_cast<const V*>(Y);
+    const i64 row = (i64)blockIdx.x * (TPB / G) + threadIdx.x / G;
+    const int sub = threadIdx.x % G;
+    T ax = 0, ay = 0;
+    if (row < n) {
+        const V yi = Y2[row];
+        const int e1 = ip[row + 1];
+        for (int e = ip[row] + sub; e < e1; e += G) {
+            const V yj = Y2[ix[e]];
+            const T dx = yi.x - yj.x, dy = yi.y - yj.y;
+            const T w = P[e] / (T(1) + (dx * dx + dy * dy) * inv_dof);
+            ax += w * dx; ay += w * dy;
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) { ax += __shfl_xor(ax, o, G); ay += __shfl_xor(ay, o, G); }
+    if (row < n && sub == 0) { V r; r.x = ax; r.y = ay; reinterpret_cast<V*>(out)[row] = r; }
+}
+
+// any dim in [1, 64] (src/fftrep.cpp:160-176): one thread per row
+template <class T>
+__global__ __launch_bounds__(TPB) void k_attr_any(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                                  const T* __restrict__ P, const T* __restrict__ Y, i64 n, int dim,
+                                                  T inv_dof, T* __restrict__ out) {
+    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    T acc[64];
+    for (int d = 0; d < dim; ++d) acc[d] = 0;
+    const T* yi = Y + i * dim;
+    for (int e = ip[i]; e < ip[i + 1]; ++e) {
+        const T* yj = Y + (i64)ix[e] * dim;
+        T d2 = 0;
+        for (int d = 0; d < dim; ++d) { const T df = yi[d] - yj[d]; d2 += df * df; }
+        const T w = P[e] / (T(1) + d2 * inv_dof);
+        for (int d = 0; d < dim; ++d) acc[d] += w * (yi[d] - yj[d]);
+    }
+    for (int d = 0; d < dim; ++d) out[i * dim + d] = acc[d];
+}
+
+// delta-bar-delta gain, momentum and position of one coordinate (src/fftrep.cpp:245-255)
+template <class T> __device__ inline T dbd(T g_, T& u, T& gain, T lr, T mom) {
+    const bool same = (g_ > T(0)) == (u > T(0));
+    T g = same ? gain * T(0.8) : gain + T(0.2);
+    if (g < T(0.01)) g = T(0.01);
+    gain = g;
+    u = mom * u - lr * g * g_;
+    return u;
+}
+
+// ---- fused step / update for 2-D layouts, then the mean (double block sums) and the bbox of the
+// un-centred layout; the last block writes mb = {mean_x, mean_y, xmin, ymin, xmax, ymax} of the
+// centred layout (rounding is monotone, so min(y_i - m) = min(y_i) - m exactly).
+template <class T, int MODE>
+__global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                                 const T* __restrict__ A, const T* __restrict__ F,
+                                                 const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
+                                                 double* __restrict__ part, unsigned* counter, T* __restrict__ mb) {
+    __shared__ double shd[TPB];
+    __shared__ T sht[TPB];
+    typedef typename vec2<T>::type V;
+    const T inv = d_inv ? *d_inv : inv_h;
+    double sx = 0, sy = 0;
+    T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
+    V* Y2 = reinterpret_cast<V*>(Y); V* U2 = reinterpret_cast<V*>(U); V* G2 = reinterpret_cast<V*>(Gn);
+    const V* A2 = reinterpret_cast<const V*>(A); const V* F2 = reinterpret_cast<const V*>(F);
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        const V a = A2[i];
+        V gx;
+        if (MODE == 0) { const V f = F2[i]; gx.x = rho * a.x - f.x * inv; gx.y = rho * a.y - f.y * inv; }
+        else gx = a;
+        V u = U2[i], g = G2[i], y = Y2[i];
+        y.x = y.x + dbd(gx.x, u.x, g.x, lr, mom);
+        y.y = y.y + dbd(gx.y, u.y, g.y, lr, mom);
+        U2[i] = u; G2[i] = g; Y2[i] = y;
+        sx += (double)y.x; sy += (double)y.y;
+        x0 = y.x < x0 ? y.x : x0; x1 = y.x > x1 ? y.x : x1;
+        y0 = y.y < y0 ? y.y : y0; y1 = y.y > y1 ? y.y : y1;
+    }
+    sx = block_reduce(sx, shd, AddOp<double>()); sy = block_reduce(sy, shd, AddOp<double>());
+    x0 = block_reduce(x0, sht, MinOp<T>()); y0 = block_reduce(y0, sht, MinOp<T>());
+    x1 = block_reduce(x1, sht, MaxOp<T>()); y1 = block_reduce(y1, sht, MaxOp<T>());
+    if (threadIdx.x == 0) {
+        double* p = part + 6 * blockIdx.x;
+        p[0] = sx; p[1] = sy; p[2] = (double)x0; p[3] = (double)y0; p[4] = (double)x1; p[5] = (double)y1;
+    }
+    if (!last_block(counter)) return;
+    sx = 0; sy = 0; x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) {
+        const double* p = part + 6 * b;
+        sx += p[0]; sy += p[1];
+        x0 = (T)p[2] < x0 ? (T)p[2] : x0; y0 = (T)p[3] < y0 ? (T)p[3] : y0;
+        x1 = (T)p[4] > x1 ? (T)p[4] : x1; y1 = (T)p[5] > y1 ? (T)p[5] : y1;
+    }
+    sx = block_reduce(sx, shd, AddOp<double>()); sy = block_reduce(sy, shd, AddOp<double>());
+    x0 = block_reduce(x0, sht, MinOp<T>()); y0 = block_reduce(y0, sht, MinOp<T>());
+    x1 = block_reduce(x1, sht, MaxOp<T>()); y1 = block_reduce(y1, sht, MaxOp<T>());
+    if (threadIdx.x == 0) {
+        const T mx = (T)(sx / (double)n), my = (T)(sy / (double)n);
+        mb[0] = mx; mb[1] = my;
+        mb[2] = x0 - mx; mb[3] = y0 - my; mb[4] = x1 - mx; mb[5] = y1 - my;
+        *counter = 0u;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const T* __restrict__ mb) {
+    typedef typename vec2<T>::type V;
+    const T mx = mb[0], my = mb[1];
+    V* Y2 = reinterpret_cast<V*>(Y);
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        V y = Y2[i]; y.x -= mx; y.y -= my; Y2[i] = y;
+    }
+}
+
+// ---- any dim: elementwise update, then one block per dimension for the mean, then centring
+template <class T, int MODE>
+__global__ __launch_bounds__(TPB) void k_update_any(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                                    const T* __restrict__ A, const T* __restrict__ F, T inv, T rho,
+                                                    T lr, T mom, i64 total) {
+    for (i64 k = (i64)blockIdx.x * TPB + threadIdx.x; k < total; k += (i64)gridDim.x * TPB) {
+        const T g_ = MODE == 0 ? rho * A[k] - F[k] * inv : A[k];
+        T u = U[k], g = Gn[k];
+        Y[k] = Y[k] + dbd(g_, u, g, lr, mom);
+        U[k] = u; Gn[k] = g;
+    }
+}
+template <class T>
+__global__ __launch_bounds__(TPB) void k_colmean(const T* __restrict__ Y, i64 n, int dim, T* __restrict__ mean) {
+    __shared__ double shd[TPB];
+    const int d = blockIdx.x;
+    double s = 0;
+    for (i64 i = threadIdx.x; i < n; i += TPB) s += (double)Y[i * dim + d];
+    s = block_reduce(s, shd, AddOp<double>());
+    if (threadIdx.x == 0) mean[d] = (T)(s / (double)n);
+}
+template <class T>
+__global__ __launch_bounds__(TPB) void k_center_any(T* __restrict__ Y, i64 total, int dim, const T* __restrict__ mean) {
+    for (i64 k = (i64)blockIdx.x * TPB + threadIdx.x; k < total; k += (i64)gridDim.x * TPB) Y[k] -= mean[k % dim];
+}
+
+// ---- gradient of the non-fused path (magnet/tsne.py:185-198): rho * attr - F * inv, plus the
+// mid-range spring w (Y_i - mean of the partners) of src/fftrep.cpp:278-285 when pairs != null.
+template <class T>
+__global__ __launch_bounds__(TPB) void k_grad(const T* __restrict__ A, const T* __restrict__ F, const T* __restrict__ d_inv,
+                                              T inv_h, T rho, const T* __restrict__ Y, const int32_t* __restrict__ pairs,
+                                              int m, T w, i64 n, int dim, int combine, T* __restrict__ grad) {
+    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const T inv = d_inv ? *d_inv : inv_h;
+    for (int d = 0; d < dim; ++d) {
+        const i64 k = i * dim + d;
+        T g = combine ? rho * A[k] - F[k] * inv : grad[k];
+        if (pairs) {
+            T acc = 0;
+            for (int q = 0; q < m; ++q) acc += Y[(i64)pairs[i * m + q] * dim + d];
+            g += w * (Y[k] - acc / (T)m);
+        }
+        grad[k] = g;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(TPB) void k_fill(T* __restrict__ p, T v, i64 count) {
+    for (i64 k = (i64)blockIdx.x * TPB + threadIdx.x; k < count; k += (i64)gridDim.x * TPB) p[k] = v;
+}
+
+inline unsigned grid_for(i64 work, unsigned cap = 4096) {
+    i64 b = (work + TPB - 1) / TPB;
+    if (b < 1) b = 1;
+    return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace
+
+int update_blocks(i64 n) { return (int)grid_for(n, 1024); }
+
+template <class T> void launch_bbox(const T* Y, i64 n, T* part, unsigned* counter, T* out, hipStream_t st) {
+    require(n >= 1, "Y must be (n, 2) with n >= 1");
+    hipLaunchKernelGGL(k_bbox<T>, dim3(grid_for(n, 1024)), dim3(TPB), 0, st, Y, n, part, counter, out);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, const T* P, const T* Y, i64 n, int dim,
+                                          T inv_dof, T* out, i64 nnz, hipStream_t st) {
+    if (n == 0) return;
+    if (dim == 2) {
+        const double avg = (double)nnz / (double)n;
+        int G = 1;
+        while (G < 64 && G * 4 < avg) G <<= 1;   // about four edges per lane
+        const unsigned blocks = (unsigned)((n + (TPB / G) - 1) / (TPB / G));
+        switch (G) {
+#define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
+            MG_ATTR_CASE(1) MG_ATTR_CASE(2) MG_ATTR_CASE(4) MG_ATTR_CASE(8) MG_ATTR_CASE(16) MG_ATTR_CASE(32) MG_ATTR_CASE(64)
+#undef MG_ATTR_CASE
+        }
+    } else {
+        hipLaunchKernelGGL(k_attr_any<T>, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, ip, ix, P, Y, n, dim, inv_dof, out);
+    }
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h, T rho,
+                                      T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim, double* part,
+                                      unsigned* counter, T* mean_bbox, hipStream_t st) {
+    (void)pairs; (void)m; (void)w_mid;
+    if (n == 0) return;
+    if (dim == 2) {
+        const unsigned blocks = grid_for(n, 1024);
+        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mean_bbox);
+        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mean_bbox);
+        MG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, mean_bbox);
+    } else {
+        require(d_inv == nullptr, "device-side inv_sumQ needs a 2-D layout");
+        const i64 total = n * dim;
+        const unsigned blocks = grid_for(total, 4096);
+        if (mode == 0) hipLaunchKernelGGL((k_update_any<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, inv_h, rho, lr, mom, total);
+        else hipLaunchKernelGGL((k_update_any<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, inv_h, rho, lr, mom, total);
+        MG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_colmean<T>, dim3(dim), dim3(TPB), 0, st, Y, n, dim, mean_bbox);
+        MG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_center_any<T>, dim3(blocks), dim3(TPB), 0, st, Y, total, dim, mean_bbox);
+    }
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_grad(const T* A, const T* F, const T* d_inv, T inv_h, T rho, const T* Y, const int32_t* pairs,
+                                    int m, T w, i64 n, int dim, int combine, T* grad, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_grad<T>, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, A, F, d_inv, inv_h, rho, Y, pairs, m, w, n, dim, combine, grad);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_mid_spring(const T* Y, const int32_t* pairs, T* grad, i64 n, int dim, int m, T w, hipStream_t st) {
+    launch_grad<T>(nullptr, nullptr, nullptr, T(0), T(0), Y, pairs, m, w, n, dim, 0, grad, st);
+}
+
+template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(count, 4096)), dim3(TPB), 0, st, p, v, count);
+    MG_HIP(hipGetLastError());
+}
+
+#define MG_INST(T) \
+    template void launch_bbox<T>(const T*, i64, T*, unsigned*, T*, hipStream_t); \
+    template void launch_attraction<T>(const int32_t*, const int32_t*, const T*, const T*, i64, int, T, T*, i64, hipStream_t); \
+    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, unsigned*, T*, hipStream_t); \
+    template void launch_grad<T>(const T*, const T*, const T*, T, T, const T*, const int32_t*, int, T, i64, int, int, T*, hipStream_t); \
+    template void launch_mid_spring<T>(const T*, const int32_t*, T*, i64, int, int, T, hipStream_t); \
+    template void launch_fill<T>(T*, T, i64, hipStream_t);
+MG_INST(float)
+MG_INST(double)
+template void launch_fill<int>(int*, int, i64, hipStream_t);
+template void launch_fill<unsigned>(unsigned*, unsigned, i64, hipStream_t);
+
+}  // namespace mg

@@ -1,0 +1,185 @@
+"""ctypes binding of libmagnet_hip.so, the C-ABI of include/magnet_hip.h, for device-resident use.
+
+The pybind module `_magnet` copies host arrays in and out on every call, like the reference's module;
+callers whose data already sits in HBM (the benchmark, serving, the parity tests) call the C-ABI
+directly through this binding. Device memory comes from torch tensors (allocation and streams only):
+every compute call is a C-ABI call, and the library runs on torch's current stream.
+"""
+import ctypes as C
+import os
+import re
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libmagnet_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "magnet_hip.h")
+
+MG_F32, MG_F64 = 0, 1
+_lib = None
+
+
+class Params(C.Structure):
+    _fields_ = [("n_iter", C.c_int32), ("early_iter", C.c_int32), ("early_exaggeration", C.c_double),
+                ("exaggeration", C.c_double), ("learning_rate", C.c_double), ("momentum0", C.c_double),
+                ("momentum1", C.c_double), ("n_interp", C.c_int32), ("min_boxes", C.c_int32), ("max_boxes", C.c_int32),
+                ("box_width", C.c_double), ("dof", C.c_double), ("mid_weight", C.c_double), ("mid_phase", C.c_double)]
+
+
+CALLBACK = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p)
+
+
+def header_symbols():
+    """Names of the functions include/magnet_hip.h declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mg_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmagnet_hip.so is not built (make -C lib-magnet_amd)")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int, C.c_double
+        sig = {
+            "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32),
+            "mg_synchronize": ([], i32), "mg_mem_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
+            "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
+            "mg_profile_enable": ([i32], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
+            "mg_profile_get": ([i32, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_double)], i32),
+            "mg_tsne_attraction": ([i32, vp, vp, vp, i64, vp, i64, i32, d, vp], i32),
+            "mg_tsne_step": ([i32, vp, vp, vp, vp, vp, i64, i32, d, d, d, d], i32),
+            "mg_tsne_update": ([i32, vp, vp, vp, vp, i64, i32, d, d], i32),
+            "mg_mid_spring": ([i32, vp, vp, vp, i64, i32, i32, d], i32),
+            "mg_fftconv_create": ([C.POINTER(vp), i32, i32, i32], i32), "mg_fftconv_destroy": ([vp], i32),
+            "mg_fftconv_set_kernel_values": ([vp, i32, vp, vp], i32),
+            "mg_fftconv_set_kernels_tsne": ([vp, d, d], i32),
+            "mg_fftconv_compute": ([vp, vp, i64, d, d, d, i32, vp, C.POINTER(C.c_double)], i32),
+            "mg_fftconv_grids": ([vp, vp, vp], i32),
+            "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
+                                  C.POINTER(C.c_double), CALLBACK, vp], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+class MgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != 0:
+        raise MgError(rc, lib().mg_last_error().decode())
+
+
+def use_torch_stream():
+    import torch
+    check(lib().mg_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def prec_of(t):
+    import torch
+    return MG_F64 if t.dtype == torch.float64 else MG_F32
+
+
+# ---- thin wrappers on torch device tensors
+def bbox(Y):
+    out = (C.c_double * 4)()
+    check(lib().mg_bbox_2d(prec_of(Y), ptr(Y), Y.shape[0], out))
+    return list(out)
+
+
+def attraction(indptr, indices, data, Y, dof=1.0):
+    import torch
+    out = torch.empty_like(Y)
+    check(lib().mg_tsne_attraction(prec_of(Y), ptr(indptr), ptr(indices), ptr(data), indices.numel(), ptr(Y),
+                                   Y.shape[0], Y.shape[1], dof, ptr(out)))
+    return out
+
+
+def step(Y, U, G, A, F, rho, inv, lr, mom):
+    check(lib().mg_tsne_step(prec_of(Y), ptr(Y), ptr(U), ptr(G), ptr(A), ptr(F), Y.shape[0], Y.shape[1], rho, inv, lr, mom))
+
+
+def update(Y, U, G, grad, lr, mom):
+    check(lib().mg_tsne_update(prec_of(Y), ptr(Y), ptr(U), ptr(G), ptr(grad), Y.shape[0], Y.shape[1], lr, mom))
+
+
+def mid_spring(Y, pairs, grad, w):
+    check(lib().mg_mid_spring(prec_of(Y), ptr(Y), ptr(pairs), ptr(grad), Y.shape[0], Y.shape[1], pairs.shape[1], w))
+
+
+class FFTConv:
+    def __init__(self, prec, M, n_interp=3):
+        self.h = C.c_void_p()
+        self.prec = prec
+        check(lib().mg_fftconv_create(C.byref(self.h), prec, M, n_interp))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mg_fftconv_destroy(self.h)
+            self.h = None
+
+    def set_tsne(self, width, dof=1.0):
+        check(lib().mg_fftconv_set_kernels_tsne(self.h, width, dof))
+
+    def set_values(self, K1, K2):
+        import numpy as np
+        K1 = np.ascontiguousarray(K1, np.float64); K2 = np.ascontiguousarray(K2, np.float64)
+        check(lib().mg_fftconv_set_kernel_values(self.h, K1.shape[0], K1.ctypes.data_as(C.c_void_p),
+                                                  K2.ctypes.data_as(C.c_void_p)))
+
+    def compute(self, Y, xmin, ymin, width, n_boxes):
+        import torch
+        F = torch.empty_like(Y)
+        s = C.c_double()
+        check(lib().mg_fftconv_compute(self.h, ptr(Y), Y.shape[0], xmin, ymin, width, n_boxes, ptr(F), C.byref(s)))
+        return s.value, F
+
+    def grids(self, N, dtype):
+        import torch
+        W = torch.empty((3, N, N), dtype=dtype, device="cuda")
+        phi = torch.empty((N, N, 4), dtype=dtype, device="cuda")
+        check(lib().mg_fftconv_grids(self.h, ptr(W), ptr(phi)))
+        return W, phi
+
+
+def optimize(indptr, indices, data, Y, U, G, it0=0, it1=None, n_iter=750, early_iter=250, early_exaggeration=12.0,
+             exaggeration=1.0, learning_rate=0.0, momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, pairs=None,
+             mid_weight=0.0, mid_phase=0.5, min_boxes=20, max_boxes=1000, callback=None):
+    """Runs mg_tsne_optimize on device tensors; returns per-iteration sumQ (numpy)."""
+    import numpy as np
+    it1 = n_iter if it1 is None else it1
+    p = Params(n_iter, early_iter, early_exaggeration, exaggeration, learning_rate, momentum[0], momentum[1], n_interp,
+               min_boxes, max_boxes, box_width, dof, mid_weight, mid_phase)
+    sq = np.zeros(max(1, it1 - it0))
+    cb = CALLBACK(callback) if callback else CALLBACK(0)
+    check(lib().mg_tsne_optimize(prec_of(Y), C.byref(p), ptr(indptr), ptr(indices), ptr(data), indices.numel(), Y.shape[0],
+                                 ptr(Y), ptr(U), ptr(G), ptr(pairs), pairs.shape[1] if pairs is not None else 0, it0, it1,
+                                 sq.ctypes.data_as(C.POINTER(C.c_double)), cb, None))
+    return sq[: max(0, it1 - it0)]
+
+
+def profile(on=True):
+    check(lib().mg_profile_enable(1 if on else 0))
+    check(lib().mg_profile_reset())
+
+
+def profile_read():
+    """{stage: (total_ms, launches, algorithmic_bytes)} accumulated by mg_tsne_optimize since profile()."""
+    out = {}
+    for s in range(lib().mg_profile_num_stages()):
+        name, tot, cnt, by = C.c_char_p(), C.c_double(), C.c_int64(), C.c_double()
+        check(lib().mg_profile_get(s, C.byref(name), C.byref(tot), C.byref(cnt), C.byref(by)))
+        out[name.value.decode()] = (tot.value, cnt.value, by.value)
+    return out

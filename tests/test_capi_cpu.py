@@ -1,0 +1,62 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every symbol the header
+declares, the pybind module keeps the reference's names, and the host logic (grid decision, affinities)
+follows the reference."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from magnet import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = capi.header_symbols()
+    assert len(syms) >= 20
+    L = ctypes.CDLL(capi.LIB_PATH)
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_pybind_module_keeps_reference_names():
+    import _magnet
+    for name in ("FFTConv2D", "tsne_attraction", "tsne_step", "tsne_update", "mid_spring", "bbox_2d", "tsne_optimize"):
+        assert hasattr(_magnet, name), name
+    assert hasattr(_magnet.FFTConv2D, "set_kernels") and hasattr(_magnet.FFTConv2D, "compute")
+
+
+def test_shape_errors_raise_before_any_device_call():
+    import _magnet
+    with pytest.raises(RuntimeError, match="indptr must have n \\+ 1 entries"):
+        _magnet.tsne_attraction(np.zeros(3, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), np.zeros((4, 2), np.float32))
+    with pytest.raises(RuntimeError, match="dof must be positive"):
+        _magnet.tsne_attraction(np.zeros(5, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), np.zeros((4, 2), np.float32), 0.0)
+    with pytest.raises(RuntimeError, match="arrays must match Y"):
+        _magnet.tsne_update(np.zeros((4, 2), np.float32), np.zeros((4, 2), np.float32), np.zeros((3, 2), np.float32),
+                            np.zeros((4, 2), np.float32), 1.0, 0.5)
+    with pytest.raises(RuntimeError, match="Y must be \\(n, 2\\)"):
+        _magnet.bbox_2d(np.zeros((0, 2), np.float32))
+
+
+def test_grid_decision_matches_oracle():
+    import oracle as O
+    from magnet.tsne import grid_params
+    rng = np.random.default_rng(0)
+    for scale in (1e-4, 1.0, 30.0, 400.0):
+        Y = (rng.normal(0, scale, (500, 2))).astype(np.float32)
+        bb = (Y[:, 0].min(), Y[:, 1].min(), Y[:, 0].max(), Y[:, 1].max())
+        assert grid_params(bb) == O.grid(Y)
+
+
+def test_perplexity_calibration_hits_the_entropy():
+    from magnet.affinity import calibrate_perplexity, perplexity_affinities
+    rng = np.random.default_rng(1)
+    d2 = np.sort(rng.gamma(2.0, 3.0, (200, 90)), axis=1)
+    Pc = calibrate_perplexity(d2, 30.0)
+    H = -(Pc * np.log(np.maximum(Pc, 1e-300))).sum(1)
+    assert np.allclose(H, np.log(30.0), atol=1e-4) and np.allclose(Pc.sum(1), 1)
+    X = rng.normal(0, 1, (400, 5)).astype(np.float32)
+    P = perplexity_affinities(X, 10.0, device="cpu")
+    assert abs(P.sum() - 1) < 1e-5 and abs(P - P.T).max() < 1e-8 and P.indices.dtype == np.int32

@@ -1,0 +1,185 @@
+"""GPU parity of the whole optimiser loop (mg_tsne_optimize) against the oracle.
+
+FP64: per iteration, from the same state, the GPU step equals the oracle step to relative L2 <= 1e-10
+(the stated bar of the plain FP64 path). FP32: the gradient of the FP32 path against the FP64 one on
+the same layouts (relative L2 <= 1e-4), determinism, resume, callbacks and the reference's quality test."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from magnet import capi
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    capi.lib()
+    capi.use_torch_stream()
+    return torch
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def mixture(n, d, k, seed, sep=5.0):
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, sep, (k, d))
+    lab = rng.integers(0, k, n)
+    return (c[lab] + rng.normal(0, 1, (n, d))).astype(np.float32), lab
+
+
+_cache = {}
+
+
+def problem(n=10_000, d=50, k=10, seed=0, perplexity=30.0):
+    key = (n, d, k, seed, perplexity)
+    if key not in _cache:
+        from magnet.affinity import perplexity_affinities
+        X, lab = mixture(n, d, k, seed)
+        P = perplexity_affinities(X, perplexity)
+        Y0 = np.random.default_rng(seed + 1).normal(0, 1e-4, (n, 2))
+        _cache[key] = (P, Y0, lab)
+    return _cache[key]
+
+
+class State:
+    def __init__(self, P, Y0, dtype):
+        self.ip, self.ix = dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32))
+        self.data = dev(P.data.astype(dtype))
+        self.Y, self.U, self.G = dev(Y0.astype(dtype)), dev(np.zeros_like(Y0, dtype)), dev(np.ones_like(Y0, dtype))
+
+    def run(self, it0, it1, **kw):
+        return capi.optimize(self.ip, self.ix, self.data, self.Y, self.U, self.G, it0, it1, **kw)
+
+    def snapshot(self):
+        return host(self.Y).copy(), host(self.U).copy(), host(self.G).copy()
+
+
+SCHED = dict(n_iter=500, early_iter=166)
+
+
+def test_fp64_per_iteration_parity():
+    """config C1's schedule (10k x 50, 10 clusters, perplexity 30, 500 iterations, early 166), FP64:
+    at checkpoints across both phases the GPU iteration from state t equals the oracle's (1e-10)."""
+    P, Y0, _ = problem()
+    data = P.data.astype(np.float64)
+    st = State(P, Y0, np.float64)
+    it = 0
+    worst = 0.0
+    for t in (0, 1, 5, 60, 165, 166, 300, 499):
+        if t > it:
+            st.run(it, t, **SCHED)
+        Y, U, G = st.snapshot()
+        sq = st.run(t, t + 1, **SCHED)
+        it = t + 1
+        Yo, Uo, Go = Y.copy(), U.copy(), G.copy()
+        sqo = O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, t, t + 1, **SCHED)
+        Yg, Ug, Gg = st.snapshot()
+        e = max(rel(Yg - Y, Yo - Y), rel(Ug, Uo), abs(sq[0] - sqo[0]) / sqo[0])
+        worst = max(worst, e)
+        assert e < 1e-10, (t, e)
+        assert np.mean(Gg == Go) > 0.9999, t
+    print("worst per-iteration relative error", worst)
+
+
+def test_fp64_short_trajectory_matches_oracle():
+    P, Y0, _ = problem()
+    data = P.data.astype(np.float64)
+    st = State(P, Y0, np.float64)
+    sq = st.run(0, 25, **SCHED)
+    Yo, Uo, Go = Y0.copy(), np.zeros_like(Y0), np.ones_like(Y0)
+    sqo = O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, 0, 25, **SCHED)
+    assert rel(host(st.Y), Yo) < 1e-9 and np.abs(sq / sqo - 1).max() < 1e-10
+
+
+def test_fp64_nonfused_path_with_mid_pairs():
+    P, Y0, _ = problem(4000, 20, 5, 3)
+    n = P.shape[0]
+    pairs = np.random.default_rng(4).integers(0, n, (n, 5)).astype(np.int32)
+    data = P.data.astype(np.float64)
+    st = State(P, Y0, np.float64)
+    kw = dict(n_iter=200, early_iter=66, pairs=dev(pairs), mid_weight=0.3, mid_phase=0.5)
+    st.run(0, 40, **kw)
+    Y, U, G = st.snapshot()
+    st.run(40, 41, **kw)
+    Yo, Uo, Go = Y.copy(), U.copy(), G.copy()
+    O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, 40, 41, n_iter=200, early_iter=66, pairs=pairs, mid_weight=0.3,
+               mid_phase=0.5)
+    assert rel(host(st.Y) - Y, Yo - Y) < 1e-10
+
+
+def test_fp32_gradient_against_fp64():
+    """The FP32 path's gradient on FP64-trajectory layouts (early and late phase) against FP64."""
+    P, Y0, _ = problem()
+    st = State(P, Y0, np.float64)
+    it = 0
+    for t, rho in ((100, 12.0), (400, 1.0)):
+        st.run(it, t, **SCHED); it = t
+        Y = host(st.Y)
+        grads = {}
+        for dt in (np.float64, np.float32):
+            Yd = dev(Y.astype(dt))
+            A = host(capi.attraction(st.ip, st.ix, dev(P.data.astype(dt)), Yd, 1.0))
+            xmin, ymin, width, nb = O.grid(Y.astype(dt))
+            conv = capi.FFTConv(capi.MG_F64 if dt == np.float64 else capi.MG_F32, nb * 6, 3)
+            conv.set_tsne(width)
+            s, F = conv.compute(Yd, xmin, ymin, width, nb)
+            grads[dt] = rho * A.astype(np.float64) - host(F).astype(np.float64) / s
+        e = rel(grads[np.float32], grads[np.float64])
+        print(f"iteration {t}: FP32 gradient relative L2 error {e:.2e}")
+        assert e < 1e-4
+
+
+def test_resume_and_determinism_fp32():
+    P, Y0, _ = problem()
+    a = State(P, Y0, np.float32)
+    a.run(0, 40, **SCHED)
+    b = State(P, Y0, np.float32)
+    b.run(0, 17, **SCHED)
+    b.run(17, 40, **SCHED)
+    assert np.array_equal(host(a.Y), host(b.Y)) and np.array_equal(host(a.G), host(b.G))
+
+
+def test_callback_cadence_and_stop():
+    P, Y0, _ = problem(3000, 10, 5, 5)
+    st = State(P, Y0, np.float32)
+    seen = []
+
+    def cb(it, d_Y, n, user):
+        seen.append(it)
+        return 1 if it >= 100 else 0
+
+    st.run(0, 300, n_iter=300, early_iter=100, callback=cb)
+    assert seen == [0, 50, 100]
+
+
+def test_separates_blobs_like_the_reference():
+    """tests/test_magnet.py:234-240 of the reference: 10-NN label purity > 0.95 on 5 blobs, d=10."""
+    X, lab = mixture(2000, 10, 5, 5, sep=4.0)
+    from magnet.affinity import perplexity_affinities
+    P = perplexity_affinities(X, 30.0)
+    st = State(P, np.random.default_rng(0).normal(0, 1e-4, (2000, 2)), np.float32)
+    st.run(0, 300, n_iter=300, early_iter=100)
+    Y = host(st.Y)
+    assert np.isfinite(Y).all()
+    from sklearn.neighbors import NearestNeighbors
+    nn = NearestNeighbors(n_neighbors=11).fit(Y).kneighbors(Y, return_distance=False)[:, 1:]
+    assert (lab[nn] == lab[:, None]).mean() > 0.95

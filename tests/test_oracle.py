@@ -1,0 +1,251 @@
+"""The CPU oracle, pinned: against vectors the reference's own kernels wrote (tests/golden), against the
+compiled reference itself when oracle/_ref is built, and against what the mathematics fixes (closed
+forms, conservation laws, brute force, library FFTs). Every check here would fail for a plausible
+slip in the restatement (a dropped term, a wrong sign or index, a transposed operand)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / max(np.linalg.norm(np.asarray(b, np.float64)), 1e-300))
+
+
+# ------------------------------------------------------------------------------ golden vectors
+def test_attraction_bit_exact_to_reference(golden):
+    g = golden
+    out = O.attraction(g["csr_indptr"], g["csr_indices"], g["csr_data"], g["att_Y"], np.float32(1.0))
+    assert np.array_equal(out, g["att_out"])
+    out3 = O.attraction(g["csr_indptr"], g["csr_indices"], g["csr_data"], g["att_Y3"], np.float32(0.7))
+    assert np.array_equal(out3, g["att_out3_dof07"])
+
+
+def test_step_update_spring_bit_exact_to_reference(golden):
+    g = golden
+    Y, U, G = g["att_Y"].copy(), g["st_U"].copy(), g["st_G"].copy()
+    O.step(Y, U, G, g["att_out"], g["st_F"], 12.0, np.float32(1 / 3.7), 250.0, 0.5)
+    assert np.array_equal(Y, g["step_Y"]) and np.array_equal(U, g["step_U"]) and np.array_equal(G, g["step_G"])
+    Y, U, G = g["att_Y"].copy(), g["st_U"].copy(), g["st_G"].copy()
+    O.update(Y, U, G, g["upd_grad"], 250.0, 0.8)
+    assert np.array_equal(Y, g["upd_Y"]) and np.array_equal(U, g["upd_U"]) and np.array_equal(G, g["upd_G"])
+    gr = g["upd_grad"].copy()
+    O.mid_spring(g["att_Y"], g["spr_pairs"], gr, np.float32(0.01))
+    assert np.array_equal(gr, g["spr_grad"])
+
+
+@pytest.mark.parametrize("case,ni", [("rep3", 3), ("rep5", 5), ("repc", 3)])
+def test_repulsion_matches_reference_float32(golden, case, ni):
+    Y = golden[case + "_Y"]
+    gx = golden[case + "_grid"]
+    # the same grid decision as the reference's (float32 bbox, double arithmetic)
+    xmin, ymin, width, nb = O.grid(Y)
+    assert (xmin, ymin, width, nb) == (gx[0], gx[1], gx[2], int(gx[3]))
+    s, F = O.repulsion(Y, np.float32(xmin), np.float32(ymin), np.float32(width), nb, ni)
+    # the reference transforms in float32 (pocketfft) on a 5-smooth padding; the oracle in double on
+    # a power of two: the two agree to float32 rounding of the FFT
+    assert abs(s - float(golden[case + "_sumQ"])) / float(golden[case + "_sumQ"]) < 2e-6
+    assert rel(F, golden[case + "_F"]) < 5e-5
+
+
+def test_trajectory_matches_reference_float32(golden):
+    g = golden
+    n = len(g["traj_Y0"])
+    Y = g["traj_Y0"].copy()
+    U, G = np.zeros_like(Y), np.ones_like(Y)
+    sq = O.optimize(g["csr_indptr"], g["csr_indices"], g["csr_data"], Y, U, G, 0, 1, n_iter=40, early_iter=13,
+                    learning_rate=max(200.0, n / 12.0))
+    assert rel(Y, g["traj_Y1"]) < 1e-5
+    O.optimize(g["csr_indptr"], g["csr_indices"], g["csr_data"], Y, U, G, 1, 10, n_iter=40, early_iter=13,
+               learning_rate=max(200.0, n / 12.0))
+    assert rel(Y, g["traj_Y10"]) < 1e-4
+    sq2 = O.optimize(g["csr_indptr"], g["csr_indices"], g["csr_data"], Y, U, G, 10, 40, n_iter=40, early_iter=13,
+                     learning_rate=max(200.0, n / 12.0))
+    assert rel(Y, g["traj_Y40"]) < 1e-3
+    assert abs(sq[0] - g["traj_sumQ"][0]) / g["traj_sumQ"][0] < 1e-5
+    assert np.abs(sq2 / g["traj_sumQ"][10:] - 1).max() < 1e-3
+
+
+_REF_CHECK = r"""
+import sys, numpy as np, scipy.sparse as sp
+sys.path.insert(0, sys.argv[1])
+import oracle as O
+R = O.ref_module()
+rng = np.random.default_rng(5)
+Y = np.ascontiguousarray(rng.normal(0, 5, (3000, 2)).astype(np.float32))
+assert np.allclose(R.bbox_2d(Y, 1), (Y[:, 0].min(), Y[:, 1].min(), Y[:, 0].max(), Y[:, 1].max()))
+A = sp.random(3000, 3000, density=0.004, random_state=2, format="csr", dtype=np.float32)
+A = (A + A.T).tocsr(); A.sort_indices(); A = sp.csr_matrix(A / A.sum(), dtype=np.float32)
+ip, ix, d = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float32)
+assert np.array_equal(np.asarray(R.tsne_attraction(ip, ix, d, Y, 1.0, 1)), O.attraction(ip, ix, d, Y, np.float32(1.0)))
+F = rng.normal(0, 1, (3000, 2)).astype(np.float32); U = rng.normal(0, .1, (3000, 2)).astype(np.float32)
+G = rng.uniform(.01, 2, (3000, 2)).astype(np.float32); A2 = rng.normal(0, 1, (3000, 2)).astype(np.float32)
+Y1, U1, G1 = Y.copy(), U.copy(), G.copy(); R.tsne_step(Y1, U1, G1, A2, F, 4.0, 0.01, 300.0, 0.8, 1)
+Y2, U2, G2 = Y.copy(), U.copy(), G.copy(); O.step(Y2, U2, G2, A2, F, 4.0, np.float32(0.01), 300.0, 0.8)
+assert np.array_equal(Y1, Y2) and np.array_equal(U1, U2) and np.array_equal(G1, G2)
+print("ok")
+"""
+
+
+def test_against_compiled_reference_when_built():
+    import glob
+    import os
+    import subprocess
+    import sys
+    if not glob.glob(os.path.join(os.path.dirname(O.__file__), "_ref", "_magnet*.so")):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    r = subprocess.run([sys.executable, "-c", _REF_CHECK, os.path.dirname(O.__file__)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
+
+
+# ------------------------------------------------------------------------------ mathematics
+@pytest.mark.parametrize("ni", [1, 2, 3, 4, 5, 8])
+def test_lagrange_partition_reproduction_and_nodes(ni):
+    nodes = (np.arange(ni) + 0.5) / ni
+    for u in np.linspace(0, 1, 17):
+        w = np.zeros(8)
+        O.lib().or_lagrange_f64(u, ni, w)
+        w = w[:ni]
+        for p in range(ni):   # reproduces polynomials up to degree ni - 1 (p = 0: partition of unity)
+            assert abs((w * nodes ** p).sum() - u ** p) < 1e-12
+    for j, u in enumerate(nodes):
+        w = np.zeros(8)
+        O.lib().or_lagrange_f64(u, ni, w)
+        assert np.allclose(w[:ni], np.eye(ni)[j], atol=1e-12)
+
+
+def _layout(n=800, seed=0, scale=4.0):
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, scale, (5, 2))
+    return np.ascontiguousarray(c[rng.integers(0, 5, n)] + rng.normal(0, 1.0, (n, 2)))
+
+
+def test_spread_conserves_charges_and_gather_is_its_adjoint():
+    Y = _layout()
+    xmin, ymin, width, nb = O.grid(Y)
+    s, F, W, phi = O.repulsion(Y, xmin, ymin, width, nb, 3, grids=True)
+    assert abs(W[0].sum() - len(Y)) < 1e-9
+    assert abs(W[1].sum() - Y[:, 0].sum()) < 1e-9 * np.abs(Y).sum()
+    assert abs(W[2].sum() - Y[:, 1].sum()) < 1e-9 * np.abs(Y).sum()
+    # Z = sum_i sum_g w_i(g) phi0(g) = <W0, phi0>
+    assert abs((W[0] * phi[0]).sum() - (s + len(Y))) < 1e-9 * (s + len(Y))
+
+
+def test_point_on_a_node_charges_one_cell():
+    nb, ni, width = 20, 3, 0.7
+    xmin = ymin = 0.0
+    Y = np.array([[xmin + width * (7 + 0.5 / 3), ymin + width * (11 + 2.5 / 3)]])
+    _, _, W, _ = O.repulsion(Y, xmin, ymin, width, nb, ni, grids=True)
+    nz = np.argwhere(np.abs(W[0]) > 1e-12)
+    assert nz.tolist() == [[7 * 3 + 0, 11 * 3 + 2]]
+    assert abs(W[0][21, 35] - 1) < 1e-12 and abs(W[1][21, 35] - Y[0, 0]) < 1e-12
+
+
+def test_fft_convolution_equals_direct_sum_and_ignores_padding():
+    Y = _layout(300, 1, 1.5)
+    xmin, ymin, width, nb = O.grid(Y)
+    assert nb == 20
+    N = nb * 3
+    s_d, F_d = O.repulsion(Y, xmin, ymin, width, nb, 3, direct=True)
+    for M in (2 * N, 128, 256, 2 * N + 7):   # 5-smooth, powers of two, odd size: the same convolution
+        s, F = O.repulsion(Y, xmin, ymin, width, nb, 3, M=M)
+        assert abs(s - s_d) / s_d < 1e-12 and rel(F, F_d) < 1e-11
+
+
+def test_kernel_spectra_match_numpy_and_are_real():
+    M, delta = 64, 0.7 / 3
+    K1, K2 = O.kernel_spectra(M, delta, 1.0)
+    off = np.arange(M); off = np.where(off <= M // 2, off, off - M) * delta
+    r2 = off[:, None] ** 2 + off[None, :] ** 2
+    for Kh, K in ((K1, 1 / (1 + r2)), (K2, 1 / (1 + r2) ** 2)):
+        full = np.fft.rfft2(K)
+        assert np.abs(full.imag).max() < 1e-12 * np.abs(full.real).max()
+        assert np.allclose(Kh, full.real, rtol=0, atol=1e-12 * np.abs(full.real).max())
+    K1s, _ = O.kernel_spectra(M, delta, 0.6)   # heavier tail, dof != 1
+    full = np.fft.rfft2((1 + r2 / 0.6) ** -0.6)
+    assert np.allclose(K1s, full.real, atol=1e-11 * np.abs(full.real).max())
+
+
+@pytest.mark.parametrize("ni,tz,tf", [(3, 1e-3, 2e-2), (5, 1e-4, 2e-3)])
+def test_repulsion_matches_exact_sum(ni, tz, tf):
+    # the reference's own acceptance test (tests/test_magnet.py:221-231: n_interp=5, 1e-3 / 2e-2)
+    Y = np.random.default_rng(0).normal(0, 8, (1500, 2))
+    Z, Fe = O.exact_repulsion(Y)
+    s, F = O.repulsion(Y, *O.grid(Y), n_interp=ni)
+    assert abs(s - Z) / Z < tz and rel(F, Fe) < tf
+
+
+def test_repulsion_symmetries():
+    Y = _layout(600, 3)
+    s, F = O.repulsion(Y, *O.grid(Y))
+    assert np.abs(F.sum(0)).max() < 1e-3 * np.abs(F).sum()   # Newton's third law, to the interpolation error
+    Ys = np.ascontiguousarray(Y[:, ::-1])
+    s2, F2 = O.repulsion(Ys, *O.grid(Ys))
+    assert abs(s2 - s) / s < 1e-12 and rel(F2[:, ::-1], F) < 1e-11
+    Yt = Y + np.array([0.5, -0.25])   # exact shift: the grid moves with the points
+    s3, F3 = O.repulsion(Yt, *O.grid(Yt))
+    assert abs(s3 - s) / s < 1e-9 and rel(F3, F) < 1e-9
+
+
+def test_dof_kernels_match_exact_sum():
+    Y = _layout(500, 4, 2.0)
+    for dof in (0.5, 1.0, 2.0):
+        Z, Fe = O.exact_repulsion(Y, dof)
+        s, F = O.repulsion(Y, *O.grid(Y), n_interp=5, dof=dof)
+        assert abs(s - Z) / Z < 1e-3 and rel(F, Fe) < 5e-3
+
+
+def _dense_P(n, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=0.02, random_state=seed, format="csr")
+    A = (A + A.T).tocsr(); A.setdiag(0); A.eliminate_zeros(); A.sort_indices()
+    A = A / A.sum()
+    return sp.csr_matrix(A), rng
+
+
+def test_attraction_brute_force_and_zero_sum():
+    P, rng = _dense_P(400, 7)
+    Y = rng.normal(0, 2, (400, 2))
+    out = O.attraction(P.indptr, P.indices, P.data, Y, 1.0)
+    D = Y[:, None, :] - Y[None, :, :]
+    Q = 1.0 / (1.0 + (D ** 2).sum(-1))
+    ref = ((P.toarray() * Q)[:, :, None] * D).sum(1)
+    assert rel(out, ref) < 1e-13
+    assert np.abs(out.sum(0)).max() < 1e-12 * np.abs(out).sum()
+
+
+def test_step_closed_form_and_recentring():
+    rng = np.random.default_rng(3)
+    n = 3000
+    Y, U = rng.normal(0, 1, (n, 2)), rng.normal(0, 0.1, (n, 2))
+    G, A, F = rng.uniform(0.01, 2, (n, 2)), rng.normal(0, 1, (n, 2)), rng.normal(0, 1, (n, 2))
+    rho, inv, lr, mom = 4.0, 0.01, 200.0, 0.8
+    g = rho * A - F * inv
+    Gn = np.maximum(np.where((g > 0) == (U > 0), G * 0.8, G + 0.2), 0.01)
+    Un = mom * U - lr * Gn * g
+    Yn = Y + Un
+    Yn -= Yn.mean(0)
+    Y2, U2, G2 = Y.copy(), U.copy(), G.copy()
+    O.step(Y2, U2, G2, A, F, rho, inv, lr, mom)
+    assert np.allclose(G2, Gn, rtol=0, atol=1e-15) and np.allclose(U2, Un, rtol=1e-14) and np.allclose(Y2, Yn, atol=1e-12)
+    assert np.abs(Y2.mean(0)).max() < 1e-13
+
+
+def test_one_iteration_follows_the_exact_gradient():
+    """Whole-loop restatement: one early-exaggeration step from a layout equals the closed-form update
+    with the exact t-SNE gradient, to the interpolation error of the repulsion."""
+    P, rng = _dense_P(500, 11)
+    Y = rng.normal(0, 3, (500, 2))
+    Y1 = Y.copy(); U = np.zeros_like(Y); G = np.ones_like(Y)
+    O.optimize(P.indptr, P.indices, P.data, Y1, U, G, 0, 1, n_iter=10, early_iter=5, n_interp=5)
+    D = Y[:, None, :] - Y[None, :, :]
+    Q = 1.0 / (1.0 + (D ** 2).sum(-1)); np.fill_diagonal(Q, 0)
+    Z = Q.sum()
+    grad = 12.0 * ((P.toarray() * Q)[:, :, None] * D).sum(1) - ((Q ** 2)[:, :, None] * D).sum(1) / Z
+    lr = max(200.0, 500 / 12.0)
+    Gn = np.where(grad > 0, 1.2, 0.8)   # U = 0: (g > 0) == (U > 0) holds where g <= 0 -> 0.8, else 1 + 0.2
+    Yn = Y - lr * Gn * grad
+    Yn -= Yn.mean(0)
+    assert rel(Y1 - Y, Yn - Y) < 1e-3
