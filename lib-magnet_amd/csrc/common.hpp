@@ -61,6 +61,7 @@ template <class T, int NI> __device__ inline void lagrange(T u, T* w) {
 // passes float xmin / ymin / width to FFTConv2D::compute).
 template <class T> struct GridT {
     T xmin, ymin, width, inv_w;
+    double xmin_d, ymin_d, width_d;   // the same values in double (local coordinates of the FP32 path)
     int nb, N;
 };
 
@@ -83,7 +84,7 @@ template <class T> struct RepBuffers {
     T* W;                                 // (3, N, N) charges
     cplx<T>* Wh;                          // (3, Mh, N) row spectra, transposed
     cplx<T>* Ph;                          // (4, Mh, N) products after the column inverse
-    cplx<T>* colstash;                    // (Mh, M) column spectra when LDS holds one column only
+    cplx<T>* colstash;                    // (Mh, M) channel-0 column spectra when LDS is short
     T* phi;                               // (N, N, 4) potentials, interleaved
     double* zpart;                        // per-block Z partials
     unsigned* counter;                    // last-block counters (zeroed at creation, restored by kernels)
@@ -111,21 +112,25 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st);
 template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st);
 int sort_hist_blocks(i64 n);
 int spread_chunks(i64 n);
+// centred: charges (1, x - x_node, y - y_node) and forces from the centred potentials (FP32 path)
 template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      RepBuffers<T>& b, hipStream_t st);
+                                      bool centred, RepBuffers<T>& b, hipStream_t st);
 template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
+                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
 int gather_blocks(i64 n);
 
 // FFT stages (M a power of two, N <= M / 2)
 void fft_init_twiddles();   // uploads the twiddle tables once per device
 template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);
-template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* K1, const T* K2, cplx<T>* Ph,
+// K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]
+template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
                                        cplx<T>* stash, hipStream_t st);
+int column_buffers(int M, int elem, bool centred);   // LDS column buffers that fit (0: none)
 template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* phi, hipStream_t st);
-// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, M x M);
-// output (Mh, M) real, ky-major, rounded to T (round_kernel: values rounded to float first).
-template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel,
-                                       double* tmp, T* K1, T* K2, hipStream_t st);
+// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M));
+// ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx], rounded to T
+// (round_kernel: kernel values rounded to float before the transform). tmp: 4 M Mh doubles.
+template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
+                                       double* tmp, T* const* K, hipStream_t st);
 
 }  // namespace mg

@@ -110,6 +110,7 @@ template <class T> GridT<T> make_grid(double xmin, double ymin, double width, in
     GridT<T> g;
     g.xmin = (T)xmin; g.ymin = (T)ymin; g.width = (T)width;
     g.inv_w = T(1) / g.width;
+    g.xmin_d = (double)g.xmin; g.ymin_d = (double)g.ymin; g.width_d = (double)g.width;
     g.nb = nb; g.N = nb * ni;
     return g;
 }
@@ -162,7 +163,10 @@ template <class T> struct Repulsion {
     double kwidth = 0.7, kdof = 1.0;
     int M_in = 0;
     std::vector<double> kvals;   // (2, M_in, M_in)
-    struct Spec { int M; double delta, dof; DevBuf K1, K2; unsigned long use; };
+    struct Spec { int M; double delta, dof; DevBuf K[4]; unsigned long use; };
+    // FP32: the centred product set (node-relative charges); FP64: the reference's
+    static constexpr bool centred = sizeof(T) == 4;
+    static constexpr int ntab = centred ? 4 : 2;
     std::list<Spec> cache;
     unsigned long tick = 0;
     DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, stash, phi;
@@ -194,9 +198,9 @@ template <class T> struct Repulsion {
         Spec& s = cache.back();
         s.M = M; s.delta = delta; s.dof = kdof; s.use = ++tick;
         const int Mh = M / 2 + 1;
-        s.K1.ensure((size_t)Mh * M * sizeof(T));
-        s.K2.ensure((size_t)Mh * M * sizeof(T));
-        double* tmp = reinterpret_cast<double*>(spec_tmp.ensure((size_t)2 * M * Mh * sizeof(double)));
+        T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure((size_t)Mh * Mh * sizeof(T)));
+        double* tmp = reinterpret_cast<double*>(spec_tmp.ensure((size_t)4 * M * Mh * sizeof(double)));
         const double* tab = nullptr;
         if (!tsne_kernel) {
             // resample the caller's table (offsets up to M_in / 2) onto the M x M torus
@@ -218,7 +222,7 @@ template <class T> struct Repulsion {
             MG_HIP(hipStreamSynchronize(st));
             tab = d;
         }
-        launch_spectra<T>(M, delta, kdof, tab, tsne_kernel && sizeof(T) == 4, tmp, dptr<T>(s.K1), dptr<T>(s.K2), st);
+        launch_spectra<T>(M, delta, kdof, tab, tsne_kernel && sizeof(T) == 4, ntab, tmp, Kp, st);
         return s;
     }
 
@@ -235,7 +239,8 @@ template <class T> struct Repulsion {
         W.ensure((size_t)3 * N * N * sizeof(T));
         Wh.ensure((size_t)3 * Mh * N * sizeof(cplx<T>));
         Ph.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
-        if (2 * M * (int)sizeof(cplx<T>) > fft_lds_max()) stash.ensure((size_t)Mh * M * sizeof(cplx<T>));
+        const int nbuf = column_buffers(M, (int)sizeof(cplx<T>), centred);
+        if (nbuf < (centred ? 3 : 2)) stash.ensure((size_t)Mh * M * sizeof(cplx<T>));
         phi.ensure((size_t)4 * N * N * sizeof(T));
         zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
     }
@@ -277,19 +282,20 @@ template <class T> struct Repulsion {
         const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
         const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
         P.begin(ST_SPREAD, st);
-        launch_spread<T>(keys, vals, Y, n, g, ni, b, st);
+        launch_spread<T>(keys, vals, Y, n, g, ni, centred, b, st);
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
         P.begin(ST_ROWS_FWD, st);
         launch_rows_forward<T>(b.W, g.N, M, b.Wh, st);
         P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);
         P.begin(ST_COLUMNS, st);
-        launch_columns<T>(b.Wh, g.N, M, dptr<T>(s.K1), dptr<T>(s.K2), b.Ph, b.colstash, st);
-        P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + 2 * Mh * M * ts + 4 * Mh * g.N * cs);
+        const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
+        launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, b.colstash, st);
+        P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs);
         P.begin(ST_ROWS_INV, st);
         launch_rows_inverse<T>(b.Ph, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts);
         P.begin(ST_GATHER, st);
-        launch_gather<T>(keys, vals, Y, n, g, ni, b.phi, F, b, sumq_hist, st);
+        launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
         P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
         last_N = g.N; last_M = M; last_n = n;
     }

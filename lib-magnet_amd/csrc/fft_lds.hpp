@@ -1,0 +1,114 @@
+// Workgroup FFT in LDS for gfx950: batched power-of-two complex transforms, Stockham auto-sort with
+// register radix-16/8/4/2 passes. Each pass: every work item reads its R inputs (stride L/R) into
+// registers, applies the twiddles exp(sign 2 pi i r k / (Ns R)), runs a radix-R DFT in registers and
+// writes back in the expanded order; a barrier separates the reads from the writes, so one LDS buffer
+// per transform suffices. Natural order in and out. A transform of 2^13 points takes 4 passes.
+#pragma once
+#include "common.hpp"
+
+namespace mg {
+
+// cos / sin of 2 pi k / 16, k = 0..15 (exact decimal expansions to 20 digits)
+__device__ constexpr double kC16[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                        0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                        -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                                        0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+__device__ constexpr double kS16[16] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                                        1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                        0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                        -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+// in-register DFT of size R (natural order), radix-2 decimation in time with constant twiddles
+template <class T, int R, int SIGN> struct RegDFT {
+    __device__ static inline void run(cplx<T>* v) {
+        cplx<T> e[R / 2], o[R / 2];
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+        RegDFT<T, R / 2, SIGN>::run(e);
+        RegDFT<T, R / 2, SIGN>::run(o);
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k) {
+            cplx<T> t;
+            const int q = k * (16 / R);   // exp(SIGN 2 pi i k / R) = entry q of the 16-point table
+            if (q == 0) t = o[k];
+            else if (q == 4) t = SIGN < 0 ? cplx<T>{o[k].y, -o[k].x} : cplx<T>{-o[k].y, o[k].x};
+            else t = cmul(o[k], cplx<T>{(T)kC16[q], (T)(SIGN * kS16[q])});
+            v[k] = cadd(e[k], t);
+            v[k + R / 2] = csub(e[k], t);
+        }
+    }
+};
+template <class T, int SIGN> struct RegDFT<T, 1, SIGN> { __device__ static inline void run(cplx<T>*) {} };
+
+// Elements each thread holds in one pass, at most: ntr * L <= FFT_MAXE * blockDim.x.
+constexpr int FFT_MAXE = 32;
+
+// One Stockham pass of radix R = 2^logR over `ntr` transforms of length 2^logL stored back to back at buf.
+// Ns = 2^logNs is the product of the radices of the earlier passes. tw: exp(-2 pi i m / MAX_M), m < MAX_M.
+template <class T, int logR, int SIGN>
+__device__ inline void stockham_pass(cplx<T>* buf, int ntr, int logL, int logNs, const cplx<T>* __restrict__ tw) {
+    constexpr int R = 1 << logR;
+    constexpr int MAXJ = FFT_MAXE / R;
+    const int L = 1 << logL, stride = L >> logR;
+    const int items = ntr * stride;
+    cplx<T> v[MAXJ][R];
+    int out[MAXJ];
+#pragma unroll
+    for (int q = 0; q < MAXJ; ++q) {
+        const int j = threadIdx.x + q * blockDim.x;
+        out[q] = -1;
+        if (j >= items) continue;
+        const int tr = j >> (logL - logR), jj = j & (stride - 1);
+        const int k = jj & ((1 << logNs) - 1);
+        const cplx<T>* src = buf + (tr << logL) + jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][r] = src[r * stride];
+        if (logNs > 0) {
+            const int step = k << (LOG_MAX_M - logNs - logR);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cplx<T> w = tw[(r * step) & (MAX_M - 1)];
+                if (SIGN > 0) w.y = -w.y;
+                v[q][r] = cmul(v[q][r], w);
+            }
+        }
+        RegDFT<T, R, SIGN>::run(v[q]);
+        out[q] = (tr << logL) + ((jj >> logNs) << (logNs + logR)) + k;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXJ; ++q) {
+        if (out[q] < 0) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[out[q] + (r << logNs)] = v[q][r];
+    }
+    __syncthreads();
+}
+
+// Full transform(s): radix-16 passes while at least 8 bits remain, then 8 / 4 / 2. Requires
+// ntr * 2^logL <= FFT_MAXE * blockDim.x. Unnormalised; SIGN -1 forward, +1 inverse.
+template <class T, int SIGN>
+__device__ inline void lds_fft(cplx<T>* buf, int ntr, int logL, const cplx<T>* __restrict__ tw) {
+    int logNs = 0, rem = logL;
+    while (rem > 0) {
+        const int lr = rem >= 8 ? 4 : (rem == 7 ? 4 : (rem >= 5 ? 3 : rem));
+        switch (lr) {
+            case 4: stockham_pass<T, 4, SIGN>(buf, ntr, logL, logNs, tw); break;
+            case 3: stockham_pass<T, 3, SIGN>(buf, ntr, logL, logNs, tw); break;
+            case 2: stockham_pass<T, 2, SIGN>(buf, ntr, logL, logNs, tw); break;
+            default: stockham_pass<T, 1, SIGN>(buf, ntr, logL, logNs, tw); break;
+        }
+        logNs += lr;
+        rem -= lr;
+    }
+}
+
+// Batched: any number of transforms, in groups that fit the register budget of one pass.
+template <class T, int SIGN>
+__device__ inline void lds_fft_batch(cplx<T>* buf, int ntr, int logL, const cplx<T>* __restrict__ tw) {
+    const int per = (FFT_MAXE * (int)blockDim.x) >> logL;
+    const int g = per < 1 ? 1 : per;
+    for (int t0 = 0; t0 < ntr; t0 += g) lds_fft<T, SIGN>(buf + ((size_t)t0 << logL), ntr - t0 < g ? ntr - t0 : g, logL, tw);
+}
+
+}  // namespace mg

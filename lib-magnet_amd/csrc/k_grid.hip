@@ -17,16 +17,32 @@ constexpr int RS_TILE = TPB * RS_ITEMS;
 constexpr int SP_PT = 4;                 // points per thread in the spread
 constexpr int SP_CHUNK = TPB * SP_PT;    // points per spread chunk (one workgroup)
 
-// local coordinates and weights of a point whose box is known from its key (src/fftconv.cpp:105-111)
-template <class T, int NI> __device__ inline void weights(T x, T y, int key, const GridT<T>& g, int& bx, int& by,
-                                                          T* wx, T* wy) {
+// local coordinates and weights of a point whose box is known from its key (src/fftconv.cpp:105-111).
+// CEN (FP32 path): the local coordinates come from double arithmetic on the float inputs, and fxu / fyu
+// return them unclamped, for the node-relative charges and forces.
+template <class T, int NI, bool CEN> __device__ inline void weights(T x, T y, int key, const GridT<T>& g, int& bx, int& by,
+                                                                   T* wx, T* wy, double& fxu, double& fyu) {
     bx = key / g.nb; by = key - bx * g.nb;
-    const T ux = (x - g.xmin) / g.width, uy = (y - g.ymin) / g.width;
-    T fx = ux - (T)bx, fy = uy - (T)by;
-    fx = fx < T(0) ? T(0) : (fx > T(1) ? T(1) : fx);
-    fy = fy < T(0) ? T(0) : (fy > T(1) ? T(1) : fy);
+    T fx, fy;
+    if (CEN) {
+        fxu = ((double)x - g.xmin_d) / g.width_d - (double)bx;
+        fyu = ((double)y - g.ymin_d) / g.width_d - (double)by;
+        fx = (T)(fxu < 0.0 ? 0.0 : (fxu > 1.0 ? 1.0 : fxu));
+        fy = (T)(fyu < 0.0 ? 0.0 : (fyu > 1.0 ? 1.0 : fyu));
+    } else {
+        const T ux = (x - g.xmin) / g.width, uy = (y - g.ymin) / g.width;
+        fx = ux - (T)bx; fy = uy - (T)by;
+        fx = fx < T(0) ? T(0) : (fx > T(1) ? T(1) : fx);
+        fy = fy < T(0) ? T(0) : (fy > T(1) ? T(1) : fy);
+        fxu = fyu = 0.0;
+    }
     lagrange<T, NI>(fx, wx);
     lagrange<T, NI>(fy, wy);
+}
+
+// offset of a point from node k of its box, in coordinate units: width (f - (k + 1/2) / NI)
+template <int NI> __device__ inline double node_offset(double f, double width, int k) {
+    return width * (f - (k + 0.5) / NI);
 }
 
 // ---------------------------------------------------------------------------------------------- sort
@@ -131,7 +147,7 @@ template <class T, int V> __device__ inline void seg_combine(bool fa, const T* v
     fb = fa || fb;
 }
 
-template <class T, int NI>
+template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, const int* __restrict__ vals,
                                                 const T* __restrict__ Y, i64 n, GridT<T> g, T* __restrict__ W,
                                                 int* __restrict__ item_key, int* __restrict__ item_valid,
@@ -156,6 +172,7 @@ __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, co
     int key[SP_PT];
     bool valid[SP_PT], head[SP_PT], endseg[SP_PT];
     T x[SP_PT], y[SP_PT], wx[SP_PT][NI], wy[SP_PT][NI];
+    double fxu[SP_PT], fyu[SP_PT];
     int first_head = SP_PT;
 #pragma unroll
     for (int j = 0; j < SP_PT; ++j) {
@@ -164,13 +181,14 @@ __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, co
         key[j] = valid[j] ? keys[p] : -1;
         head[j] = false; endseg[j] = false;
         x[j] = y[j] = T(0);
+        fxu[j] = fyu[j] = 0.0;
 #pragma unroll
         for (int a = 0; a < NI; ++a) wx[j][a] = wy[j][a] = T(0);
         if (!valid[j]) continue;
         const V2 pt = reinterpret_cast<const V2*>(Y)[vals[p]];
         x[j] = pt.x; y[j] = pt.y;
         int bx, by;
-        weights<T, NI>(x[j], y[j], key[j], g, bx, by, wx[j], wy[j]);
+        weights<T, NI, CEN>(x[j], y[j], key[j], g, bx, by, wx[j], wy[j], fxu[j], fyu[j]);
         head[j] = p == c0 || keys[p - 1] != key[j];
         endseg[j] = p == cend - 1 || keys[p + 1] != key[j];
         if (first_head == SP_PT && head[j]) first_head = j;
@@ -181,10 +199,12 @@ __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, co
         T s[SP_PT][V];
 #pragma unroll
         for (int j = 0; j < SP_PT; ++j) {
+            const T qa = CEN ? (T)node_offset<NI>(fxu[j], g.width_d, a) : x[j];
 #pragma unroll
             for (int b = 0; b < NI; ++b) {
                 const T w = wx[j][a] * wy[j][b];   // zero weights for positions past the end
-                const T v0 = w, v1 = w * x[j], v2 = w * y[j];
+                const T qb = CEN ? (T)node_offset<NI>(fyu[j], g.width_d, b) : y[j];
+                const T v0 = w, v1 = w * qa, v2 = w * qb;
                 if (j == 0 || head[j]) {
                     s[j][3 * b + 0] = v0; s[j][3 * b + 1] = v1; s[j][3 * b + 2] = v2;
                 } else {
@@ -332,7 +352,7 @@ __device__ inline double block_sum(double v, double* sh) {
     return r;
 }
 
-template <class T, int NI>
+template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
                                                 const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
                                                 T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
@@ -348,20 +368,43 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
         const V2 pt = reinterpret_cast<const V2*>(Y)[idx];
         int bx, by;
         T wx[NI], wy[NI];
-        weights<T, NI>(pt.x, pt.y, key, g, bx, by, wx, wy);
-        T p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+        double fxu, fyu;
+        weights<T, NI, CEN>(pt.x, pt.y, key, g, bx, by, wx, wy, fxu, fyu);
+        V2 f;
+        if (CEN) {
+            // F = sum_g w(g) (phi1(g) (y_i - y_g) + psi(g)), offsets and sums in double
+            double p0 = 0, fx = 0, fy = 0;
 #pragma unroll
-        for (int a = 0; a < NI; ++a) {
-            const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+            for (int a = 0; a < NI; ++a) {
+                const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+                const double ex = node_offset<NI>(fxu, g.width_d, a);
 #pragma unroll
-            for (int b = 0; b < NI; ++b) {
-                const T w = wx[a] * wy[b];
-                const V4 c = phi4[row + b];
-                p0 += w * c.x; p1 += w * c.y; p2 += w * c.z; p3 += w * c.w;
+                for (int b = 0; b < NI; ++b) {
+                    const double w = (double)(wx[a] * wy[b]);
+                    const V4 c = phi4[row + b];
+                    const double ey = node_offset<NI>(fyu, g.width_d, b);
+                    p0 += w * (double)c.x;
+                    fx += w * ((double)c.y * ex + (double)c.z);
+                    fy += w * ((double)c.y * ey + (double)c.w);
+                }
             }
+            z += p0;
+            f.x = (T)fx; f.y = (T)fy;
+        } else {
+            T p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+#pragma unroll
+            for (int a = 0; a < NI; ++a) {
+                const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+#pragma unroll
+                for (int b = 0; b < NI; ++b) {
+                    const T w = wx[a] * wy[b];
+                    const V4 c = phi4[row + b];
+                    p0 += w * c.x; p1 += w * c.y; p2 += w * c.z; p3 += w * c.w;
+                }
+            }
+            z += (double)p0;
+            f.x = p1 * pt.x - p2; f.y = p1 * pt.y - p3;
         }
-        z += (double)p0;
-        V2 f; f.x = p1 * pt.x - p2; f.y = p1 * pt.y - p3;
         reinterpret_cast<V2*>(F)[idx] = f;
     }
     z = block_sum(z, shd);
@@ -411,10 +454,10 @@ template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, Rep
     return cur;
 }
 
-template <class T, int NI> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
+template <class T, int NI, bool CEN> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
                                                  RepBuffers<T>& b, hipStream_t st) {
     const int nch = spread_chunks(n);
-    hipLaunchKernelGGL((k_spread<T, NI>), dim3(nch), dim3(TPB), 0, st, keys, vals, Y, n, g, b.W, b.item_key, b.item_valid, b.item_val);
+    hipLaunchKernelGGL((k_spread<T, NI, CEN>), dim3(nch), dim3(TPB), 0, st, keys, vals, Y, n, g, b.W, b.item_key, b.item_valid, b.item_val);
     MG_HIP(hipGetLastError());
     if (nch > 1) {
         hipLaunchKernelGGL((k_spread_span<T, NI>), dim3(nch), dim3(TPB), 0, st, keys, n, nch, g, b.W, b.item_key, b.item_valid, b.item_val);
@@ -423,26 +466,25 @@ template <class T, int NI> static void spread_ni(const int* keys, const int* val
 }
 
 template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      RepBuffers<T>& b, hipStream_t st) {
+                                      bool centred, RepBuffers<T>& b, hipStream_t st) {
     launch_fill<T>(b.W, T(0), (i64)3 * g.N * g.N, st);
     if (n == 0) return;
+#define MG_S(NI) (centred ? spread_ni<T, NI, true>(keys, vals, Y, n, g, b, st) : spread_ni<T, NI, false>(keys, vals, Y, n, g, b, st))
     switch (ni) {
-        case 1: spread_ni<T, 1>(keys, vals, Y, n, g, b, st); break;
-        case 2: spread_ni<T, 2>(keys, vals, Y, n, g, b, st); break;
-        case 3: spread_ni<T, 3>(keys, vals, Y, n, g, b, st); break;
-        case 4: spread_ni<T, 4>(keys, vals, Y, n, g, b, st); break;
-        case 5: spread_ni<T, 5>(keys, vals, Y, n, g, b, st); break;
-        case 6: spread_ni<T, 6>(keys, vals, Y, n, g, b, st); break;
-        case 7: spread_ni<T, 7>(keys, vals, Y, n, g, b, st); break;
-        case 8: spread_ni<T, 8>(keys, vals, Y, n, g, b, st); break;
+        case 1: MG_S(1); break; case 2: MG_S(2); break; case 3: MG_S(3); break; case 4: MG_S(4); break;
+        case 5: MG_S(5); break; case 6: MG_S(6); break; case 7: MG_S(7); break; case 8: MG_S(8); break;
         default: require(false, "n_interp must be in [1, 8]");
     }
+#undef MG_S
 }
 
 template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st) {
+                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st) {
     const unsigned blocks = (unsigned)gather_blocks(n);
-#define MG_G(NI) hipLaunchKernelGGL((k_gather<T, NI>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist)
+#define MG_G(NI) do { \
+        if (centred) hipLaunchKernelGGL((k_gather<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist); \
+        else hipLaunchKernelGGL((k_gather<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist); \
+    } while (0)
     switch (ni) {
         case 1: MG_G(1); break; case 2: MG_G(2); break; case 3: MG_G(3); break; case 4: MG_G(4); break;
         case 5: MG_G(5); break; case 6: MG_G(6); break; case 7: MG_G(7); break; case 8: MG_G(8); break;
@@ -454,8 +496,8 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
 
 #define MG_INST(T) \
     template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t); \
-    template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, RepBuffers<T>&, hipStream_t); \
-    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, const T*, T*, RepBuffers<T>&, double*, hipStream_t);
+    template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
+    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t);
 MG_INST(float)
 MG_INST(double)
 

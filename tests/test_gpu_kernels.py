@@ -188,6 +188,16 @@ def test_repulsion_fp32_against_reference(golden):
     assert rel(F, golden["rep5_F"]) < 2e-4
 
 
+@pytest.mark.parametrize("n,scale", [(20_000, 25.0), (70_000, 60.0)])
+def test_repulsion_fp32_against_fp64_oracle(n, scale):
+    """FP32 path (node-relative charges, FP64 Z) against the FP64 oracle on the same float32 layout:
+    sumQ to 2e-6 and F to 2e-6 relative (the reference's float32 path is at ~1e-5 on such layouts)."""
+    Y = layout(n, 11, scale, k=12, spread=0.6).astype(np.float32)
+    s, F = rep_gpu(Y)
+    so, Fo = O.repulsion(Y.astype(np.float64), *O.grid(Y.astype(np.float64)), n_interp=3)
+    assert abs(s - so) / so < 2e-6 and rel(F, Fo) < 2e-6, (abs(s - so) / so, rel(F, Fo))
+
+
 def test_repulsion_matches_exact_sum_reference_tolerance():
     # tests/test_magnet.py:221-231 of the reference, on the GPU: n_interp=5, sumQ 1e-3, F 2e-2
     Y = np.random.default_rng(0).normal(0, 8, (1500, 2))

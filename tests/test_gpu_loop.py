@@ -126,26 +126,33 @@ def test_fp64_nonfused_path_with_mid_pairs():
     assert rel(host(st.Y) - Y, Yo - Y) < 1e-10
 
 
+def fp32_vs_fp64_gradient(st, P, Y, rho):
+    """relative L2 of the FP32 path's gradient against the FP64 path's on the same (float32) layout"""
+    Y = Y.astype(np.float32)
+    grads = {}
+    for dt in (np.float64, np.float32):
+        Yd = dev(Y.astype(dt))
+        A = host(capi.attraction(st.ip, st.ix, dev(P.data.astype(dt)), Yd, 1.0))
+        xmin, ymin, width, nb = O.grid(Y)
+        conv = capi.FFTConv(capi.MG_F64 if dt == np.float64 else capi.MG_F32, nb * 6, 3)
+        conv.set_tsne(width)
+        s, F = conv.compute(Yd, xmin, ymin, width, nb)
+        grads[dt] = rho * A.astype(np.float64) - host(F).astype(np.float64) / s
+    return rel(grads[np.float32], grads[np.float64])
+
+
 def test_fp32_gradient_against_fp64():
-    """The FP32 path's gradient on FP64-trajectory layouts (early and late phase) against FP64."""
+    """FP32 path (bulk FP32, FP64 Z) against FP64 on identical inputs, on the layouts of an FP64 run
+    through both phases: relative L2 <= 1e-4 (late in the run the gradient is a small difference of
+    attraction and repulsion, which magnifies every rounding)."""
     P, Y0, _ = problem()
     st = State(P, Y0, np.float64)
     it = 0
-    for t, rho in ((100, 12.0), (400, 1.0)):
+    for t in (50, 100, 165, 200, 300, 400, 499):
         st.run(it, t, **SCHED); it = t
-        Y = host(st.Y)
-        grads = {}
-        for dt in (np.float64, np.float32):
-            Yd = dev(Y.astype(dt))
-            A = host(capi.attraction(st.ip, st.ix, dev(P.data.astype(dt)), Yd, 1.0))
-            xmin, ymin, width, nb = O.grid(Y.astype(dt))
-            conv = capi.FFTConv(capi.MG_F64 if dt == np.float64 else capi.MG_F32, nb * 6, 3)
-            conv.set_tsne(width)
-            s, F = conv.compute(Yd, xmin, ymin, width, nb)
-            grads[dt] = rho * A.astype(np.float64) - host(F).astype(np.float64) / s
-        e = rel(grads[np.float32], grads[np.float64])
+        e = fp32_vs_fp64_gradient(st, P, host(st.Y), 12.0 if t < 166 else 1.0)
         print(f"iteration {t}: FP32 gradient relative L2 error {e:.2e}")
-        assert e < 1e-4
+        assert e < 1e-4, (t, e)
 
 
 def test_resume_and_determinism_fp32():
