@@ -78,6 +78,8 @@ def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False):
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     stages = capi.profile_read() if profile else None
+    if profile:
+        stages["_grid"] = capi.profile_grid()
     capi.profile(False)
     return sec, stages, (Y, U, G, ip, ix, data)
 
@@ -168,6 +170,7 @@ def main():
     sec32, _, _ = run_gpu(P, Y0, "fp32", steps, args.warmup, W["n_iter"], W["early"])
     log(f"bench: fp32 {steps} iterations in {sec32:.3f}s -> {steps / sec32:.1f} it/s")
     _, stages, state = run_gpu(P, Y0, "fp32", steps, 0, W["n_iter"], W["early"], profile=True)
+    gridstat = stages.pop("_grid")
     stage_rows = {k: dict(ms_total=round(v[0], 3), launches=v[1], avg_us=round(1e3 * v[0] / max(v[1], 1), 2),
                           GBs=round(v[2] / max(v[0], 1e-9) / 1e6, 1)) for k, v in stages.items() if v[1] > 0}
     dom = max(stages.items(), key=lambda kv: kv[1][0])
@@ -180,6 +183,7 @@ def main():
     tot_bytes = sum(v[2] for v in stages.values())
     log(f"bench: stages {json.dumps(stage_rows)}")
     extra = {"stages": stage_rows, "pass_GBs": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+             "grid": {"mean_N": round(gridstat[0], 1), "max_N": gridstat[1], "max_M": gridstat[2]},
              "algo_bytes_per_iter": round(tot_bytes / steps)}
     if not args.no_fp64:
         sec64, _, _ = run_gpu(P, Y0, "fp64", steps, min(args.warmup, 10), W["n_iter"], W["early"])

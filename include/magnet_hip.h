@@ -51,6 +51,12 @@ int mg_profile_enable(int on);
 int mg_profile_reset(void);
 int mg_profile_num_stages(void);
 int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* count, double* algo_bytes);
+/* Grid sizes of the profiled iterations: mean and largest live side N = n_interp * n_boxes, largest M. */
+int mg_profile_grid(double* mean_N, int* max_N, int* max_M);
+
+/* Caps the LDS bytes per workgroup the FFT stages plan for (0 = the device limit). Smaller caps select
+ * the variants that park column spectra in HBM; for tests of those variants at small grid sizes. */
+int mg_set_lds_limit(int bytes);
 
 /* ---- per-call kernels (device pointers) ----------------------------------------------------------- */
 

@@ -121,6 +121,8 @@ int gather_blocks(i64 n);
 
 // FFT stages (M a power of two, N <= M / 2)
 void fft_init_twiddles();   // uploads the twiddle tables once per device
+int fft_lds_max();
+void fft_set_lds_cap(int bytes);
 template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);
 // K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,

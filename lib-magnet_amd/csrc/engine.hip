@@ -17,7 +17,6 @@
 namespace mg {
 
 int update_blocks(i64 n);
-int fft_lds_max();
 
 static thread_local std::string g_err;
 static hipStream_t g_stream = nullptr;
@@ -128,6 +127,8 @@ struct Prof {
     double bytes[NSTAGE] = {};     // algorithmic HBM bytes of the launches timed (inputs read once, outputs written once)
     double pend_bytes[NSTAGE] = {};
     long long count[NSTAGE] = {};
+    double n_sum = 0; int n_max = 0, m_max = 0; long long iters = 0;   // live grid side N, padded M
+    void grid(int N, int M) { if (!on) return; n_sum += N; n_max = N > n_max ? N : n_max; m_max = M > m_max ? M : m_max; ++iters; }
     void begin(int s, hipStream_t st) {
         if (!on) return;
         if (!ev[2 * s]) { MG_HIP(hipEventCreate(&ev[2 * s])); MG_HIP(hipEventCreate(&ev[2 * s + 1])); }
@@ -150,7 +151,10 @@ struct Prof {
             if (debug) fprintf(stderr, "  [it %d] %s %.3f ms\n", it, kStageNames[s], ms);
         }
     }
-    void reset() { for (int s = 0; s < NSTAGE; ++s) { total[s] = 0; bytes[s] = 0; count[s] = 0; pending[s] = false; } }
+    void reset() {
+        for (int s = 0; s < NSTAGE; ++s) { total[s] = 0; bytes[s] = 0; count[s] = 0; pending[s] = false; }
+        n_sum = 0; n_max = 0; m_max = 0; iters = 0;
+    }
 };
 static Prof g_prof;
 
@@ -298,6 +302,7 @@ template <class T> struct Repulsion {
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
         P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
         last_N = g.N; last_M = M; last_n = n;
+        P.grid(g.N, M);
     }
 };
 
@@ -453,6 +458,19 @@ int mg_profile_enable(int on) {
 }
 int mg_profile_reset(void) { MG_API_BEGIN g_prof.reset(); MG_API_END }
 int mg_profile_num_stages(void) { return NSTAGE; }
+int mg_set_lds_limit(int bytes) {
+    MG_API_BEGIN
+    require(bytes >= 0, "bytes must be >= 0");
+    fft_set_lds_cap(bytes);
+    MG_API_END
+}
+int mg_profile_grid(double* mean_N, int* max_N, int* max_M) {
+    MG_API_BEGIN
+    if (mean_N) *mean_N = g_prof.iters ? g_prof.n_sum / (double)g_prof.iters : 0.0;
+    if (max_N) *max_N = g_prof.n_max;
+    if (max_M) *max_M = g_prof.m_max;
+    MG_API_END
+}
 int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* count, double* algo_bytes) {
     MG_API_BEGIN
     require(stage >= 0 && stage < NSTAGE, "stage out of range");

@@ -385,7 +385,12 @@ void fft_init_twiddles() {
     g_tw[dev].lds_max = lds;
 }
 
-int fft_lds_max() { return g_tw[current_device()].lds_max; }
+static int g_lds_cap = 0;   // test knob: LDS bytes the FFT planner may assume (0 = the device's)
+int fft_lds_max() {
+    const int dev = g_tw[current_device()].lds_max;
+    return g_lds_cap > 0 && g_lds_cap < dev ? g_lds_cap : dev;
+}
+void fft_set_lds_cap(int bytes) { g_lds_cap = bytes; }
 
 // LDS per workgroup for one stage, and whether the column pass needs the global stash
 int column_buffers(int M, int elem, bool centred) {

@@ -44,7 +44,8 @@ def lib():
             "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32),
             "mg_synchronize": ([], i32), "mg_mem_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
             "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
-            "mg_profile_enable": ([i32], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
+            "mg_profile_enable": ([i32], i32), "mg_set_lds_limit": ([i32], i32),
+            "mg_profile_grid": ([C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
             "mg_profile_get": ([i32, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_double)], i32),
             "mg_tsne_attraction": ([i32, vp, vp, vp, i64, vp, i64, i32, d, vp], i32),
@@ -183,3 +184,15 @@ def profile_read():
         check(lib().mg_profile_get(s, C.byref(name), C.byref(tot), C.byref(cnt), C.byref(by)))
         out[name.value.decode()] = (tot.value, cnt.value, by.value)
     return out
+
+
+def profile_grid():
+    """(mean N, max N, max M) of the profiled iterations."""
+    a, b, c = C.c_double(), C.c_int(), C.c_int()
+    check(lib().mg_profile_grid(C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+def set_lds_limit(nbytes):
+    """Caps the LDS the FFT stages plan for (0 = device limit): selects the HBM-stash variants."""
+    check(lib().mg_set_lds_limit(int(nbytes)))

@@ -194,7 +194,10 @@ def test_repulsion_fp32_against_fp64_oracle(n, scale):
     sumQ to 2e-6 and F to 2e-6 relative (the reference's float32 path is at ~1e-5 on such layouts)."""
     Y = layout(n, 11, scale, k=12, spread=0.6).astype(np.float32)
     s, F = rep_gpu(Y)
-    so, Fo = O.repulsion(Y.astype(np.float64), *O.grid(Y.astype(np.float64)), n_interp=3)
+    # the FP32 path works on the float-rounded grid (origin, box width): give the oracle the same grid
+    xmin, ymin, width, nb = O.grid(Y)
+    g32 = [float(np.float32(v)) for v in (xmin, ymin, width)]
+    so, Fo = O.repulsion(Y.astype(np.float64), *g32, nb, n_interp=3)
     assert abs(s - so) / so < 2e-6 and rel(F, Fo) < 2e-6, (abs(s - so) / so, rel(F, Fo))
 
 
@@ -205,6 +208,60 @@ def test_repulsion_matches_exact_sum_reference_tolerance():
     for dt in (np.float64, np.float32):
         s, F = rep_gpu(Y.astype(dt), 5)
         assert abs(s - Z) / Z < 1e-3 and rel(F, Fe) < 2e-2
+
+
+def test_repulsion_fp64_clipped_grid_three_sort_passes():
+    """a layout wider than 1000 boxes of 0.7: n_boxes clips at 1000 and the width grows to span / 1000
+    (magnet/tsne.py:40-45); 10^6 box keys take three radix passes. n_interp = 1 keeps N = 1000."""
+    rng = np.random.default_rng(12)
+    c = rng.uniform(-450, 450, (40, 2))
+    Y = c[rng.integers(0, 40, 50_000)] + rng.normal(0, 2.0, (50_000, 2))
+    xmin, ymin, width, nb = O.grid(Y)
+    assert nb == 1000 and width > 0.7
+    s, F = rep_gpu(Y, 1)
+    so, Fo = O.repulsion(Y, xmin, ymin, width, nb, 1)
+    assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-10
+
+
+@pytest.mark.parametrize("cap", [0, 65536, 40000])
+def test_repulsion_column_variants(cap):
+    """every column-pass variant (spectra kept in LDS, or channel 0 parked in HBM) and both row-inverse
+    variants against the oracle: the LDS cap forces the parked variants at a grid the oracle can check
+    (N = 540, M = 2048; cap 40000: FP64 one LDS column, FP32 two, rows inverse one channel at a time)."""
+    Y = layout(30_000, 13, 25.0, k=12)
+    xmin, ymin, width, nb = O.grid(Y)
+    assert O.fft_size(nb * 3) == 2048
+    try:
+        capi.set_lds_limit(cap)
+        s, F = rep_gpu(Y)
+        Yf = Y.astype(np.float32)
+        s32, F32 = rep_gpu(Yf)
+    finally:
+        capi.set_lds_limit(0)
+    so, Fo = O.repulsion(Y, xmin, ymin, width, nb, 3)
+    assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-11
+    g32 = [float(np.float32(v)) for v in O.grid(Yf)[:3]]
+    so32, Fo32 = O.repulsion(Yf.astype(np.float64), *g32, O.grid(Yf)[3], 3)
+    assert abs(s32 - so32) / so32 < 2e-6 and rel(F32, Fo32) < 2e-6
+
+
+def test_repulsion_largest_grid_fp32_against_fp64():
+    """N = 3000 (n_boxes clipped at 1000, n_interp 3), M = 8192: FP32 (centred, two LDS columns and an
+    HBM stash) against FP64 (reference products, one LDS column and an HBM stash) on one float32 layout."""
+    rng = np.random.default_rng(14)
+    c = rng.uniform(-500, 500, (60, 2))
+    Yf = (c[rng.integers(0, 60, 70_000)] + rng.normal(0, 3.0, (70_000, 2))).astype(np.float32)
+    xmin, ymin, width, nb = O.grid(Yf)
+    assert nb == 1000
+    g = [float(np.float32(v)) for v in (xmin, ymin, width)]
+    out = {}
+    for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
+        conv = capi.FFTConv(prec, 6000, 3)
+        conv.set_tsne(g[2])
+        out[dt] = conv.compute(dev(Yf.astype(dt)), g[0], g[1], g[2], nb)
+    s64, F64 = out[np.float64]
+    s32, F32 = out[np.float32]
+    assert abs(s32 - s64) / s64 < 2e-6 and rel(host(F32), host(F64)) < 5e-6
 
 
 def test_repulsion_boxes_spanning_many_chunks():
