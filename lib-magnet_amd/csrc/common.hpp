@@ -61,15 +61,18 @@ template <class T, int NI> __device__ inline void lagrange(T u, T* w) {
 // passes float xmin / ymin / width to FFTConv2D::compute).
 template <class T> struct GridT {
     T xmin, ymin, width, inv_w;
-    double xmin_d, ymin_d, width_d;   // the same values in double (local coordinates of the FP32 path)
+    double xmin_d, ymin_d, width_d, inv_w_d;   // the same grid in double (T's values, exactly)
     int nb, N;
 };
 
 // Box of a point: x-box by multiplication with 1/width (the counting-sort key of src/fftconv.cpp:88),
-// y-box and both local coordinates by division (:106-110). Key = bx * nb + by.
+// y-box by division (:106-110). Key = bx * nb + by. Evaluated in double for both paths: the FP32 path
+// then puts every point in the box the FP64 path would on the same grid (float evaluation moves
+// points that lie within a rounding of a box edge, and each such point carries the interpolation
+// error of the neighbouring box).
 template <class T> __device__ inline int box_key(T x, T y, const GridT<T>& g) {
-    const int bx = min(g.nb - 1, max(0, (int)((x - g.xmin) * g.inv_w)));
-    const T uy = (y - g.ymin) / g.width;
+    const int bx = min(g.nb - 1, max(0, (int)(((double)x - g.xmin_d) * g.inv_w_d)));
+    const double uy = ((double)y - g.ymin_d) / g.width_d;
     const int by = min(g.nb - 1, max(0, (int)uy));
     return bx * g.nb + by;
 }

@@ -3,10 +3,29 @@
 // registers, applies the twiddles exp(sign 2 pi i r k / (Ns R)), runs a radix-R DFT in registers and
 // writes back in the expanded order; a barrier separates the reads from the writes, so one LDS buffer
 // per transform suffices. Natural order in and out. A transform of 2^13 points takes 4 passes.
+// All functions address one LDS region `lds`; a transform batch starts at element offset `base`.
+//
+// LDS layout: element i of a buffer lives at swz(i) = i ^ ((i >> S) & (2^S - 1)), with 2^S elements
+// per 128 bytes (S = 4 for float complex, 3 for double complex). The XOR keeps every aligned block of
+// 2^S elements in place as a whole and permutes inside it, so both the strided reads and the expanded
+// writes of a pass hit distinct banks across 2^S consecutive lanes. Every LDS access of a transform
+// buffer goes through swz.
+//
+// Twiddles: exp(-2 pi i m / MAX_M) = hi[m >> 6] * lo[m & 63], two small tables staged in LDS by
+// fft_stage_twiddles (MAX_M / 64 + 64 entries); the product of two correctly rounded entries.
 #pragma once
 #include "common.hpp"
 
 namespace mg {
+
+template <class T> struct Swz { static constexpr int S = sizeof(cplx<T>) == 8 ? 4 : 3; };
+template <class T> __device__ __forceinline__ int swz(int i) {
+    constexpr int S = Swz<T>::S;
+    return i ^ ((i >> S) & ((1 << S) - 1));
+}
+
+constexpr int TW_HI = MAX_M / 64;   // 128 entries: exp(-2 pi i 64 a / MAX_M)
+constexpr int TW_LO = 64;           // 64 entries:  exp(-2 pi i b / MAX_M)
 
 // cos / sin of 2 pi k / 16, k = 0..15 (exact decimal expansions to 20 digits)
 __device__ constexpr double kC16[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
@@ -40,13 +59,27 @@ template <class T, int R, int SIGN> struct RegDFT {
 };
 template <class T, int SIGN> struct RegDFT<T, 1, SIGN> { __device__ static inline void run(cplx<T>*) {} };
 
+// LDS twiddle tables of one workgroup
+template <class T> struct TwLDS { cplx<T> hi[TW_HI]; cplx<T> lo[TW_LO]; };
+
+template <class T> __device__ inline void fft_stage_twiddles(TwLDS<T>& s, const cplx<T>* __restrict__ hi,
+                                                             const cplx<T>* __restrict__ lo) {
+    for (int i = threadIdx.x; i < TW_HI; i += blockDim.x) s.hi[i] = hi[i];
+    for (int i = threadIdx.x; i < TW_LO; i += blockDim.x) s.lo[i] = lo[i];
+}
+template <class T, int SIGN> __device__ __forceinline__ cplx<T> twiddle(const TwLDS<T>& s, int m) {
+    cplx<T> w = cmul(s.hi[(m >> 6) & (TW_HI - 1)], s.lo[m & 63]);
+    if (SIGN > 0) w.y = -w.y;
+    return w;
+}
+
 // Elements each thread holds in one pass, at most: ntr * L <= FFT_MAXE * blockDim.x.
 constexpr int FFT_MAXE = 32;
 
-// One Stockham pass of radix R = 2^logR over `ntr` transforms of length 2^logL stored back to back at buf.
-// Ns = 2^logNs is the product of the radices of the earlier passes. tw: exp(-2 pi i m / MAX_M), m < MAX_M.
+// One Stockham pass of radix R = 2^logR over `ntr` transforms of length 2^logL stored back to back from
+// element `base` of the swizzled region `lds`. Ns = 2^logNs is the product of the earlier radices.
 template <class T, int logR, int SIGN>
-__device__ inline void stockham_pass(cplx<T>* buf, int ntr, int logL, int logNs, const cplx<T>* __restrict__ tw) {
+__device__ inline void stockham_pass(cplx<T>* lds, int base, int ntr, int logL, int logNs, const TwLDS<T>& tw) {
     constexpr int R = 1 << logR;
     constexpr int MAXJ = FFT_MAXE / R;
     const int L = 1 << logL, stride = L >> logR;
@@ -60,27 +93,23 @@ __device__ inline void stockham_pass(cplx<T>* buf, int ntr, int logL, int logNs,
         if (j >= items) continue;
         const int tr = j >> (logL - logR), jj = j & (stride - 1);
         const int k = jj & ((1 << logNs) - 1);
-        const cplx<T>* src = buf + (tr << logL) + jj;
+        const int src = base + (tr << logL) + jj;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q][r] = src[r * stride];
+        for (int r = 0; r < R; ++r) v[q][r] = lds[swz<T>(src + r * stride)];
         if (logNs > 0) {
             const int step = k << (LOG_MAX_M - logNs - logR);
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cplx<T> w = tw[(r * step) & (MAX_M - 1)];
-                if (SIGN > 0) w.y = -w.y;
-                v[q][r] = cmul(v[q][r], w);
-            }
+            for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle<T, SIGN>(tw, r * step));
         }
         RegDFT<T, R, SIGN>::run(v[q]);
-        out[q] = (tr << logL) + ((jj >> logNs) << (logNs + logR)) + k;
+        out[q] = base + (tr << logL) + ((jj >> logNs) << (logNs + logR)) + k;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXJ; ++q) {
         if (out[q] < 0) continue;
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[out[q] + (r << logNs)] = v[q][r];
+        for (int r = 0; r < R; ++r) lds[swz<T>(out[q] + (r << logNs))] = v[q][r];
     }
     __syncthreads();
 }
@@ -88,27 +117,28 @@ __device__ inline void stockham_pass(cplx<T>* buf, int ntr, int logL, int logNs,
 // Full transform(s): radix-16 passes while at least 8 bits remain, then 8 / 4 / 2. Requires
 // ntr * 2^logL <= FFT_MAXE * blockDim.x. Unnormalised; SIGN -1 forward, +1 inverse.
 template <class T, int SIGN>
-__device__ inline void lds_fft(cplx<T>* buf, int ntr, int logL, const cplx<T>* __restrict__ tw) {
+__device__ inline void lds_fft(cplx<T>* lds, int base, int ntr, int logL, const TwLDS<T>& tw) {
     int logNs = 0, rem = logL;
     while (rem > 0) {
         const int lr = rem >= 8 ? 4 : (rem == 7 ? 4 : (rem >= 5 ? 3 : rem));
         switch (lr) {
-            case 4: stockham_pass<T, 4, SIGN>(buf, ntr, logL, logNs, tw); break;
-            case 3: stockham_pass<T, 3, SIGN>(buf, ntr, logL, logNs, tw); break;
-            case 2: stockham_pass<T, 2, SIGN>(buf, ntr, logL, logNs, tw); break;
-            default: stockham_pass<T, 1, SIGN>(buf, ntr, logL, logNs, tw); break;
+            case 4: stockham_pass<T, 4, SIGN>(lds, base, ntr, logL, logNs, tw); break;
+            case 3: stockham_pass<T, 3, SIGN>(lds, base, ntr, logL, logNs, tw); break;
+            case 2: stockham_pass<T, 2, SIGN>(lds, base, ntr, logL, logNs, tw); break;
+            default: stockham_pass<T, 1, SIGN>(lds, base, ntr, logL, logNs, tw); break;
         }
         logNs += lr;
         rem -= lr;
     }
 }
 
-// Batched: any number of transforms, in groups that fit the register budget of one pass.
+// Batched: any number of transforms (contiguous in the swizzled buffer), in groups that fit the register
+// budget of one pass.
 template <class T, int SIGN>
-__device__ inline void lds_fft_batch(cplx<T>* buf, int ntr, int logL, const cplx<T>* __restrict__ tw) {
+__device__ inline void lds_fft_batch(cplx<T>* lds, int base, int ntr, int logL, const TwLDS<T>& tw) {
     const int per = (FFT_MAXE * (int)blockDim.x) >> logL;
     const int g = per < 1 ? 1 : per;
-    for (int t0 = 0; t0 < ntr; t0 += g) lds_fft<T, SIGN>(buf + ((size_t)t0 << logL), ntr - t0 < g ? ntr - t0 : g, logL, tw);
+    for (int t0 = 0; t0 < ntr; t0 += g) lds_fft<T, SIGN>(lds, base + (t0 << logL), ntr - t0 < g ? ntr - t0 : g, logL, tw);
 }
 
 }  // namespace mg

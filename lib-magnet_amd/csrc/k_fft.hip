@@ -25,38 +25,46 @@ namespace mg {
 
 namespace {
 
-constexpr int TPB = 256;
-
-struct Twiddles { cplx<float>* f32 = nullptr; cplx<double>* f64 = nullptr; int lds_max = 65536; };
+struct Twiddles {
+    cplx<float>* hi32 = nullptr; cplx<float>* lo32 = nullptr;
+    cplx<double>* hi64 = nullptr; cplx<double>* lo64 = nullptr;
+    int lds_max = 65536;
+};
 Twiddles g_tw[64];
 std::mutex g_tw_mu;
 
-template <class T> const cplx<T>* twiddles(int dev);
-template <> const cplx<float>* twiddles<float>(int dev) { return g_tw[dev].f32; }
-template <> const cplx<double>* twiddles<double>(int dev) { return g_tw[dev].f64; }
+template <class T> struct TwPtr { const cplx<T>* hi; const cplx<T>* lo; };
+template <class T> TwPtr<T> twiddles(int dev);
+template <> TwPtr<float> twiddles<float>(int dev) { return {g_tw[dev].hi32, g_tw[dev].lo32}; }
+template <> TwPtr<double> twiddles<double>(int dev) { return {g_tw[dev].hi64, g_tw[dev].lo64}; }
+
+constexpr int FTPB = 512;   // threads of the FFT workgroups
 
 int current_device() { int d = 0; MG_HIP(hipGetDevice(&d)); return d; }
 
 // X[k] (k = 0..L) of a real sequence from Z = DFT_L(even + i odd): E = (Z_k + conj Z_{L-k}) / 2,
 // O = -i (Z_k - conj Z_{L-k}) / 2, X_k = E + exp(-2 pi i k / M) O
-template <class T> __device__ inline cplx<T> r2c_post(const cplx<T>* z, int k, int L, int logM, const cplx<T>* tw) {
-    const cplx<T> zk = z[k & (L - 1)], zc = cconj(z[(L - k) & (L - 1)]);
+template <class T> __device__ inline cplx<T> r2c_post(const cplx<T>* lds, int base, int k, int L, int logM,
+                                                     const TwLDS<T>& tw) {
+    const cplx<T> zk = lds[swz<T>(base + (k & (L - 1)))], zc = cconj(lds[swz<T>(base + ((L - k) & (L - 1)))]);
     const cplx<T> e = {(zk.x + zc.x) * T(0.5), (zk.y + zc.y) * T(0.5)};
     const cplx<T> d = csub(zk, zc);
     const cplx<T> o = {d.y * T(0.5), -d.x * T(0.5)};
-    return cadd(e, cmul(tw[(k << (LOG_MAX_M - logM)) & (MAX_M - 1)], o));
+    return cadd(e, cmul(twiddle<T, -1>(tw, (k << (LOG_MAX_M - logM)) & (MAX_M - 1)), o));
 }
 
 // ---- rows forward: blockIdx.y = channel, R rows per workgroup
 template <class T>
-__global__ __launch_bounds__(TPB) void k_rows_fwd(const T* __restrict__ W, int N, int logM, int R,
-                                                  cplx<T>* __restrict__ Wh, const cplx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(FTPB) void k_rows_fwd(const T* __restrict__ W, int N, int logM, int R,
+                                                   cplx<T>* __restrict__ Wh, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int L = 1 << (logM - 1), logL = logM - 1, Mh = L + 1;
     const int c = blockIdx.y, gx0 = blockIdx.x * R;
     const int rows = min(R, N - gx0);
-    for (int e = threadIdx.x; e < R * L; e += TPB) {
+    for (int e = threadIdx.x; e < R * L; e += FTPB) {
         const int r = e >> logL, j = e & (L - 1);
         cplx<T> z = {T(0), T(0)};
         if (r < rows) {
@@ -64,13 +72,13 @@ __global__ __launch_bounds__(TPB) void k_rows_fwd(const T* __restrict__ W, int N
             if (2 * j < N) z.x = src[2 * j];
             if (2 * j + 1 < N) z.y = src[2 * j + 1];
         }
-        buf[e] = z;
+        lds[swz<T>(e)] = z;
     }
     __syncthreads();
-    lds_fft_batch<T, -1>(buf, R, logL, tw);
-    for (int e = threadIdx.x; e < R * Mh; e += TPB) {
+    lds_fft_batch<T, -1>(lds, 0, R, logL, tw);
+    for (int e = threadIdx.x; e < R * Mh; e += FTPB) {
         const int k = e / R, r = e - k * R;
-        if (r < rows) Wh[((size_t)c * Mh + k) * N + gx0 + r] = r2c_post(buf + r * L, k, L, logM, tw);
+        if (r < rows) Wh[((size_t)c * Mh + k) * N + gx0 + r] = r2c_post(lds, r * L, k, L, logM, tw);
     }
 }
 
@@ -82,162 +90,166 @@ template <class T> __device__ inline T ktab(const T* row, int kx, int M, bool od
     return odd ? -v : v;
 }
 
-template <class T> __device__ inline void load_column(cplx<T>* A, const cplx<T>* __restrict__ src, int N, int M) {
-    for (int i = threadIdx.x; i < M; i += TPB) A[i] = i < N ? src[i] : cplx<T>{T(0), T(0)};
+template <class T> __device__ inline void load_column(cplx<T>* lds, int base, const cplx<T>* __restrict__ src, int N, int M) {
+    for (int i = threadIdx.x; i < M; i += FTPB) lds[swz<T>(base + i)] = i < N ? src[i] : cplx<T>{T(0), T(0)};
 }
-template <class T> __device__ inline void store_column(cplx<T>* __restrict__ dst, const cplx<T>* A, int N) {
-    for (int g = threadIdx.x; g < N; g += TPB) dst[g] = A[g];
+template <class T> __device__ inline void store_column(cplx<T>* __restrict__ dst, const cplx<T>* lds, int base, int N) {
+    for (int g = threadIdx.x; g < N; g += FTPB) dst[g] = lds[swz<T>(base + g)];
 }
 
-// ---- columns, reference product set. NBUF 2: both channel-0 products held in LDS; 1: spectrum of
-// channel 0 parked in `stash` (global) between its two products.
+// ---- columns, reference product set. NBUF 2: both channel-0 products held in LDS; 1: one product of
+// channel 0 parked in `stash` (HBM) while the other is transformed back.
 template <class T, int NBUF>
-__global__ __launch_bounds__(TPB) void k_columns(const cplx<T>* __restrict__ Wh, int N, int logM,
-                                                 const T* __restrict__ K1, const T* __restrict__ K2,
-                                                 cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash,
-                                                 const cplx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(FTPB) void k_columns(const cplx<T>* __restrict__ Wh, int N, int logM,
+                                                  const T* __restrict__ K1, const T* __restrict__ K2,
+                                                  cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int M = 1 << logM, Mh = M / 2 + 1, ky = blockIdx.x;
-    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
-    cplx<T>* B = A + M;
+    const int A = 0, B = M;   // element offsets of the two buffers
     const T* k1 = K1 + (size_t)ky * Mh;
     const T* k2 = K2 + (size_t)ky * Mh;
     auto col = [&](int ch) { return Wh + ((size_t)ch * Mh + ky) * N; };
     auto out = [&](int o) { return Ph + ((size_t)o * Mh + ky) * N; };
+    auto at = [&](int buf, int k) -> cplx<T>& { return lds[swz<T>(buf + k)]; };
     // channel 0: phi0 = K1 W0, phi1 = K2 W0
-    load_column(A, col(0), N, M);
+    load_column(lds, A, col(0), N, M);
     __syncthreads();
-    lds_fft<T, -1>(A, 1, logM, tw);
-    for (int k = threadIdx.x; k < M; k += TPB) {
-        const cplx<T> a = A[k];
+    lds_fft<T, -1>(lds, A, 1, logM, tw);
+    for (int k = threadIdx.x; k < M; k += FTPB) {
+        const cplx<T> a = at(A, k);
         const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-        if (NBUF == 2) B[k] = {a.x * c2, a.y * c2};
+        if (NBUF == 2) at(B, k) = {a.x * c2, a.y * c2};
         else stash[(size_t)ky * M + k] = {a.x * c2, a.y * c2};
-        A[k] = {a.x * c1, a.y * c1};
+        at(A, k) = {a.x * c1, a.y * c1};
     }
     __syncthreads();
     if (NBUF == 2) {
-        lds_fft_batch<T, +1>(A, 2, logM, tw);
-        store_column(out(0), A, N);
-        store_column(out(1), B, N);
+        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
+        store_column(out(0), lds, A, N);
+        store_column(out(1), lds, B, N);
         __syncthreads();
         // channels 1 and 2 together: phi2 = K2 W1, phi3 = K2 W2
-        load_column(A, col(1), N, M);
-        load_column(B, col(2), N, M);
+        load_column(lds, A, col(1), N, M);
+        load_column(lds, B, col(2), N, M);
         __syncthreads();
-        lds_fft_batch<T, -1>(A, 2, logM, tw);
-        for (int k = threadIdx.x; k < M; k += TPB) {
+        lds_fft_batch<T, -1>(lds, A, 2, logM, tw);
+        for (int k = threadIdx.x; k < M; k += FTPB) {
             const T c2 = ktab(k2, k, M, false);
-            const cplx<T> a = A[k], b = B[k];
-            A[k] = {a.x * c2, a.y * c2};
-            B[k] = {b.x * c2, b.y * c2};
+            const cplx<T> a = at(A, k), b = at(B, k);
+            at(A, k) = {a.x * c2, a.y * c2};
+            at(B, k) = {b.x * c2, b.y * c2};
         }
         __syncthreads();
-        lds_fft_batch<T, +1>(A, 2, logM, tw);
-        store_column(out(2), A, N);
-        store_column(out(3), B, N);
+        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
+        store_column(out(2), lds, A, N);
+        store_column(out(3), lds, B, N);
     } else {
-        lds_fft<T, +1>(A, 1, logM, tw);
-        store_column(out(0), A, N);
+        lds_fft<T, +1>(lds, A, 1, logM, tw);
+        store_column(out(0), lds, A, N);
         __syncthreads();
-        for (int k = threadIdx.x; k < M; k += TPB) A[k] = stash[(size_t)ky * M + k];
+        for (int k = threadIdx.x; k < M; k += FTPB) at(A, k) = stash[(size_t)ky * M + k];
         __syncthreads();
-        lds_fft<T, +1>(A, 1, logM, tw);
-        store_column(out(1), A, N);
+        lds_fft<T, +1>(lds, A, 1, logM, tw);
+        store_column(out(1), lds, A, N);
         for (int ch = 1; ch < 3; ++ch) {
             __syncthreads();
-            load_column(A, col(ch), N, M);
+            load_column(lds, A, col(ch), N, M);
             __syncthreads();
-            lds_fft<T, -1>(A, 1, logM, tw);
-            for (int k = threadIdx.x; k < M; k += TPB) {
+            lds_fft<T, -1>(lds, A, 1, logM, tw);
+            for (int k = threadIdx.x; k < M; k += FTPB) {
                 const T c2 = ktab(k2, k, M, false);
-                const cplx<T> a = A[k];
-                A[k] = {a.x * c2, a.y * c2};
+                const cplx<T> a = at(A, k);
+                at(A, k) = {a.x * c2, a.y * c2};
             }
             __syncthreads();
-            lds_fft<T, +1>(A, 1, logM, tw);
-            store_column(out(ch + 1), A, N);
+            lds_fft<T, +1>(lds, A, 1, logM, tw);
+            store_column(out(ch + 1), lds, A, N);
         }
     }
 }
 
 // ---- columns, centred product set. NBUF 3: the three spectra in LDS; 2: the channel-0 spectrum parked
-// in `stash` while channels 1 and 2 are transformed one at a time.
+// in `stash` (HBM) while channels 1 and 2 are transformed one at a time.
 template <class T, int NBUF>
-__global__ __launch_bounds__(TPB) void k_columns_centred(const cplx<T>* __restrict__ Wh, int N, int logM,
-                                                         const T* __restrict__ K1, const T* __restrict__ K2,
-                                                         const T* __restrict__ TX, const T* __restrict__ TY,
-                                                         cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash,
-                                                         const cplx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(FTPB) void k_columns_centred(const cplx<T>* __restrict__ Wh, int N, int logM,
+                                                          const T* __restrict__ K1, const T* __restrict__ K2,
+                                                          const T* __restrict__ TX, const T* __restrict__ TY,
+                                                          cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash,
+                                                          TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int M = 1 << logM, Mh = M / 2 + 1, ky = blockIdx.x;
-    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
-    cplx<T>* B = A + M;
-    cplx<T>* C = B + M;
+    const int A = 0, B = M, C = 2 * M;
     const T* k1 = K1 + (size_t)ky * Mh;
     const T* k2 = K2 + (size_t)ky * Mh;
     const T* tx = TX + (size_t)ky * Mh;
     const T* ty = TY + (size_t)ky * Mh;
     auto col = [&](int ch) { return Wh + ((size_t)ch * Mh + ky) * N; };
     auto out = [&](int o) { return Ph + ((size_t)o * Mh + ky) * N; };
+    auto at = [&](int buf, int k) -> cplx<T>& { return lds[swz<T>(buf + k)]; };
     // odd-kernel product i t s0 - k2 s1
     auto oddprod = [](cplx<T> s0, cplx<T> s1, T t, T c2) { return cplx<T>{-t * s0.y - c2 * s1.x, t * s0.x - c2 * s1.y}; };
     if (NBUF == 3) {
-        load_column(A, col(0), N, M);
-        load_column(B, col(1), N, M);
-        load_column(C, col(2), N, M);
+        load_column(lds, A, col(0), N, M);
+        load_column(lds, B, col(1), N, M);
+        load_column(lds, C, col(2), N, M);
         __syncthreads();
-        lds_fft_batch<T, -1>(A, 3, logM, tw);
-        for (int k = threadIdx.x; k < M; k += TPB) {
+        lds_fft_batch<T, -1>(lds, A, 3, logM, tw);
+        for (int k = threadIdx.x; k < M; k += FTPB) {
             const T c2 = ktab(k2, k, M, false), tX = ktab(tx, k, M, true), tY = ktab(ty, k, M, false);
-            const cplx<T> s0 = A[k];
-            B[k] = oddprod(s0, B[k], tX, c2);
-            C[k] = oddprod(s0, C[k], tY, c2);
+            const cplx<T> s0 = at(A, k);
+            at(B, k) = oddprod(s0, at(B, k), tX, c2);
+            at(C, k) = oddprod(s0, at(C, k), tY, c2);
         }
         __syncthreads();
-        lds_fft_batch<T, +1>(B, 2, logM, tw);
-        store_column(out(2), B, N);
-        store_column(out(3), C, N);
+        lds_fft_batch<T, +1>(lds, B, 2, logM, tw);
+        store_column(out(2), lds, B, N);
+        store_column(out(3), lds, C, N);
         __syncthreads();
-        for (int k = threadIdx.x; k < M; k += TPB) {
+        for (int k = threadIdx.x; k < M; k += FTPB) {
             const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-            const cplx<T> s0 = A[k];
-            B[k] = {s0.x * c2, s0.y * c2};
-            A[k] = {s0.x * c1, s0.y * c1};
+            const cplx<T> s0 = at(A, k);
+            at(B, k) = {s0.x * c2, s0.y * c2};
+            at(A, k) = {s0.x * c1, s0.y * c1};
         }
         __syncthreads();
-        lds_fft_batch<T, +1>(A, 2, logM, tw);
-        store_column(out(0), A, N);
-        store_column(out(1), B, N);
+        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
+        store_column(out(0), lds, A, N);
+        store_column(out(1), lds, B, N);
     } else {
         cplx<T>* S0 = stash + (size_t)ky * M;
-        load_column(A, col(0), N, M);
+        load_column(lds, A, col(0), N, M);
         __syncthreads();
-        lds_fft<T, -1>(A, 1, logM, tw);
-        for (int k = threadIdx.x; k < M; k += TPB) {
+        lds_fft<T, -1>(lds, A, 1, logM, tw);
+        for (int k = threadIdx.x; k < M; k += FTPB) {
             const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-            const cplx<T> s0 = A[k];
+            const cplx<T> s0 = at(A, k);
             S0[k] = s0;
-            B[k] = {s0.x * c2, s0.y * c2};
-            A[k] = {s0.x * c1, s0.y * c1};
+            at(B, k) = {s0.x * c2, s0.y * c2};
+            at(A, k) = {s0.x * c1, s0.y * c1};
         }
         __syncthreads();
-        lds_fft_batch<T, +1>(A, 2, logM, tw);
-        store_column(out(0), A, N);
-        store_column(out(1), B, N);
+        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
+        store_column(out(0), lds, A, N);
+        store_column(out(1), lds, B, N);
         for (int ch = 1; ch < 3; ++ch) {
             __syncthreads();
-            load_column(B, col(ch), N, M);
+            load_column(lds, B, col(ch), N, M);
             __syncthreads();
-            lds_fft<T, -1>(B, 1, logM, tw);
+            lds_fft<T, -1>(lds, B, 1, logM, tw);
             const T* tt = ch == 1 ? tx : ty;
-            for (int k = threadIdx.x; k < M; k += TPB) {
+            for (int k = threadIdx.x; k < M; k += FTPB) {
                 const T c2 = ktab(k2, k, M, false), t = ktab(tt, k, M, ch == 1);
-                B[k] = oddprod(S0[k], B[k], t, c2);
+                at(B, k) = oddprod(S0[k], at(B, k), t, c2);
             }
             __syncthreads();
-            lds_fft<T, +1>(B, 1, logM, tw);
-            store_column(out(ch + 1), B, N);
+            lds_fft<T, +1>(lds, B, 1, logM, tw);
+            store_column(out(ch + 1), lds, B, N);
         }
     }
 }
@@ -248,15 +260,18 @@ template <> struct vec4<float> { typedef float4 type; };
 template <> struct vec4<double> { typedef double4 type; };
 
 template <class T, int CH>
-__global__ __launch_bounds__(TPB) void k_rows_inv(const cplx<T>* __restrict__ Ph, int N, int logM, int R,
-                                                  T* __restrict__ phi, T scale, const cplx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(FTPB) void k_rows_inv(const cplx<T>* __restrict__ Ph, int N, int logM, int R,
+                                                   T* __restrict__ phi, T scale, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem);
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
+    __syncthreads();
     const int L = 1 << (logM - 1), logL = logM - 1, Mh = L + 1;
     const int gx0 = blockIdx.x * R, o0 = CH == 4 ? 0 : blockIdx.y;
     const int rows = min(R, N - gx0);
     // Z_k = E + i O with E = X_k + conj X_{L-k}, O = (X_k - conj X_{L-k}) exp(+2 pi i k / M)
-    for (int e = threadIdx.x; e < CH * R * L; e += TPB) {
+    for (int e = threadIdx.x; e < CH * R * L; e += FTPB) {
         const int oc = e / (R * L), rem = e - oc * (R * L);
         const int k = rem / R, r = rem - k * R;
         cplx<T> z = {T(0), T(0)};
@@ -264,94 +279,111 @@ __global__ __launch_bounds__(TPB) void k_rows_inv(const cplx<T>* __restrict__ Ph
             const size_t base = (size_t)(o0 + oc) * Mh * N + gx0 + r;
             const cplx<T> xk = Ph[base + (size_t)k * N], xc = cconj(Ph[base + (size_t)(L - k) * N]);
             const cplx<T> ev = cadd(xk, xc);
-            const cplx<T> od = cmul(csub(xk, xc), cconj(tw[(k << (LOG_MAX_M - logM)) & (MAX_M - 1)]));
+            const cplx<T> od = cmul(csub(xk, xc), twiddle<T, +1>(tw, (k << (LOG_MAX_M - logM)) & (MAX_M - 1)));
             z = {ev.x - od.y, ev.y + od.x};
         }
-        buf[(oc * R + r) * L + k] = z;
+        lds[swz<T>((oc * R + r) * L + k)] = z;
     }
     __syncthreads();
-    lds_fft_batch<T, +1>(buf, CH * R, logL, tw);
-    for (int e = threadIdx.x; e < R * N; e += TPB) {
+    lds_fft_batch<T, +1>(lds, 0, CH * R, logL, tw);
+    for (int e = threadIdx.x; e < R * N; e += FTPB) {
         const int r = e / N, gy = e - r * N;
         if (r >= rows) continue;
         const int j = gy >> 1;
         const bool odd = gy & 1;
         if (CH == 4) {
             typename vec4<T>::type v;
-            const cplx<T> a = buf[(0 * R + r) * L + j], b = buf[(1 * R + r) * L + j];
-            const cplx<T> c = buf[(2 * R + r) * L + j], d = buf[(3 * R + r) * L + j];
+            const cplx<T> a = lds[swz<T>((0 * R + r) * L + j)], b = lds[swz<T>((1 * R + r) * L + j)];
+            const cplx<T> c = lds[swz<T>((2 * R + r) * L + j)], d = lds[swz<T>((3 * R + r) * L + j)];
             v.x = (odd ? a.y : a.x) * scale; v.y = (odd ? b.y : b.x) * scale;
             v.z = (odd ? c.y : c.x) * scale; v.w = (odd ? d.y : d.x) * scale;
             reinterpret_cast<typename vec4<T>::type*>(phi)[(size_t)(gx0 + r) * N + gy] = v;
         } else {
-            const cplx<T> a = buf[r * L + j];
+            const cplx<T> a = lds[swz<T>(r * L + j)];
             phi[((size_t)(gx0 + r) * N + gy) * 4 + o0] = (odd ? a.y : a.x) * scale;
         }
     }
 }
 
-// ---- kernel spectra (double). Tables: 0 K1, 1 K2, 2 x K2, 3 y K2 (offsets in grid units * delta).
-// Pass 1: per row a, the real transform along b (keep Re, or Im for the table odd in b: y K2).
-// Pass 2: per column kb, the real transform along a (keep Re, or Im for the table odd in a: x K2);
-// output (Mh, Mh) per table: [ky][kx], kx <= M/2.
-__device__ inline double kernel_value(int a, int b, int M, double delta, double dof, const double* table, int which,
-                                      bool round_kernel) {
+// ---- kernel spectra, computed in TC (double for the FP64 path, float for the FP32 path, whose
+// reference counterpart rounds the kernel values to float32 and transforms them in single precision).
+// Tables: 0 K1, 1 K2, 2 x K2, 3 y K2 (offsets in grid units * delta).
+// Pass 1: per row a, the real transform along b (keep Re, or Im for the table odd in b: y K2); the row
+// transform of x K2 is off_a times that of K2. Pass 2: per column kb, the real transform along a (keep
+// Re, or Im for the table odd in a: x K2). Output (Mh, Mh) per table: [ky][kx], kx <= M/2.
+__device__ inline void kernel_values(int a, int b, int M, double delta, double dof, const double* table, bool round_kernel,
+                                     double& k1, double& k2, double& ky2) {
     const double ox = (a <= M / 2 ? a : a - M) * delta, oy = (b <= M / 2 ? b : b - M) * delta;
-    double k2;
     if (table) {
-        if (which == 0) return table[(size_t)a * M + b];
+        k1 = table[(size_t)a * M + b];
         k2 = table[(size_t)M * M + (size_t)a * M + b];
+    } else if (dof == 1.0) {
+        const double inv = 1.0 / (1.0 + ox * ox + oy * oy);
+        k1 = inv; k2 = inv * inv;
     } else {
         const double base = 1.0 + (ox * ox + oy * oy) / dof;
-        if (which == 0) { double v = pow(base, -dof); return round_kernel ? (double)(float)v : v; }
-        k2 = pow(base, -dof - 1.0);
-        if (round_kernel) k2 = (double)(float)k2;
+        k1 = pow(base, -dof); k2 = pow(base, -dof - 1.0);
     }
+    if (round_kernel) { k1 = (double)(float)k1; k2 = (double)(float)k2; }
     // the odd tables drop the Nyquist offset M/2 (no negative twin on the torus), which keeps them exactly
     // odd -- so their spectra are imaginary -- and is never used: the live offsets stay below N <= M/2
-    if (which == 2) return a == M / 2 ? 0.0 : ox * k2;
-    if (which == 3) return b == M / 2 ? 0.0 : oy * k2;
-    return k2;
+    ky2 = b == M / 2 ? 0.0 : oy * k2;
 }
 
-__global__ __launch_bounds__(TPB) void k_spec_rows(int logM, int ntab, double delta, double dof,
-                                                   const double* __restrict__ table, int round_kernel,
-                                                   double* __restrict__ tmp, const cplx<double>* __restrict__ tw) {
+template <class TC>
+__global__ __launch_bounds__(FTPB) void k_spec_rows(int logM, int ntab, double delta, double dof,
+                                                    const double* __restrict__ table, int round_kernel,
+                                                    TC* __restrict__ tmp, TwPtr<TC> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double>* buf = reinterpret_cast<cplx<double>*>(smem);
+    __shared__ TwLDS<TC> tw;
+    cplx<TC>* lds = reinterpret_cast<cplx<TC>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int M = 1 << logM, L = M >> 1, logL = logM - 1, Mh = L + 1, a = blockIdx.x;
-    for (int which = 0; which < ntab; ++which) {
-        for (int j = threadIdx.x; j < L; j += TPB)
-            buf[j] = {kernel_value(a, 2 * j, M, delta, dof, table, which, round_kernel),
-                      kernel_value(a, 2 * j + 1, M, delta, dof, table, which, round_kernel)};
-        __syncthreads();
-        lds_fft<double, -1>(buf, 1, logL, tw);
-        for (int k = threadIdx.x; k < Mh; k += TPB) {
-            const cplx<double> X = r2c_post(buf, k, L, logM, tw);
-            tmp[((size_t)which * M + a) * Mh + k] = which == 3 ? X.y : X.x;
+    const int nfft = ntab == 2 ? 2 : 3;   // K1, K2 (, y K2)
+    for (int j = threadIdx.x; j < L; j += FTPB) {
+        double e1, e2, ey, o1, o2, oy;
+        kernel_values(a, 2 * j, M, delta, dof, table, round_kernel, e1, e2, ey);
+        kernel_values(a, 2 * j + 1, M, delta, dof, table, round_kernel, o1, o2, oy);
+        lds[swz<TC>(j)] = {(TC)e1, (TC)o1};
+        lds[swz<TC>(L + j)] = {(TC)e2, (TC)o2};
+        if (nfft == 3) lds[swz<TC>(2 * L + j)] = {(TC)ey, (TC)oy};
+    }
+    __syncthreads();
+    lds_fft_batch<TC, -1>(lds, 0, nfft, logL, tw);
+    const double ox = (a <= M / 2 ? a : a - M) * delta;
+    const TC xa = a == M / 2 ? TC(0) : (TC)ox;
+    for (int k = threadIdx.x; k < Mh; k += FTPB) {
+        const TC r1 = r2c_post(lds, 0, k, L, logM, tw).x, r2 = r2c_post(lds, L, k, L, logM, tw).x;
+        tmp[((size_t)0 * M + a) * Mh + k] = r1;
+        tmp[((size_t)1 * M + a) * Mh + k] = r2;
+        if (ntab == 4) {
+            tmp[((size_t)2 * M + a) * Mh + k] = xa * r2;
+            tmp[((size_t)3 * M + a) * Mh + k] = r2c_post(lds, 2 * L, k, L, logM, tw).y;
         }
-        __syncthreads();
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(TPB) void k_spec_cols(int logM, int ntab, const double* __restrict__ tmp, T* __restrict__ K1,
-                                                   T* __restrict__ K2, T* __restrict__ TX, T* __restrict__ TY,
-                                                   const cplx<double>* __restrict__ tw) {
+template <class TC, class T>
+__global__ __launch_bounds__(FTPB) void k_spec_cols(int logM, int ntab, const TC* __restrict__ tmp, T* __restrict__ K1,
+                                                    T* __restrict__ K2, T* __restrict__ TX, T* __restrict__ TY,
+                                                    TwPtr<TC> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double>* buf = reinterpret_cast<cplx<double>*>(smem);
+    __shared__ TwLDS<TC> tw;
+    cplx<TC>* lds = reinterpret_cast<cplx<TC>*>(smem);
+    fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int M = 1 << logM, L = M >> 1, logL = logM - 1, Mh = L + 1, kb = blockIdx.x;
-    for (int which = 0; which < ntab; ++which) {
-        const double* src = tmp + (size_t)which * M * Mh + kb;
-        for (int j = threadIdx.x; j < L; j += TPB) buf[j] = {src[(size_t)(2 * j) * Mh], src[(size_t)(2 * j + 1) * Mh]};
-        __syncthreads();
-        lds_fft<double, -1>(buf, 1, logL, tw);
-        T* out = (which == 0 ? K1 : which == 1 ? K2 : which == 2 ? TX : TY) + (size_t)kb * Mh;
-        for (int k = threadIdx.x; k < Mh; k += TPB) {
-            const cplx<double> X = r2c_post(buf, k, L, logM, tw);
-            out[k] = (T)(which == 2 ? X.y : X.x);
+    for (int w = 0; w < ntab; ++w) {
+        const TC* src = tmp + (size_t)w * M * Mh + kb;
+        for (int j = threadIdx.x; j < L; j += FTPB) lds[swz<TC>(w * L + j)] = {src[(size_t)(2 * j) * Mh], src[(size_t)(2 * j + 1) * Mh]};
+    }
+    __syncthreads();
+    lds_fft_batch<TC, -1>(lds, 0, ntab, logL, tw);
+    for (int w = 0; w < ntab; ++w) {
+        T* out = (w == 0 ? K1 : w == 1 ? K2 : w == 2 ? TX : TY) + (size_t)kb * Mh;
+        for (int k = threadIdx.x; k < Mh; k += FTPB) {
+            const cplx<TC> X = r2c_post(lds, w * L, k, L, logM, tw);
+            out[k] = (T)(w == 2 ? X.y : X.x);
         }
-        __syncthreads();
     }
 }
 
@@ -367,27 +399,32 @@ void fft_init_twiddles() {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     const int dev = current_device();
     require(dev >= 0 && dev < 64, "device index out of range");
-    if (g_tw[dev].f64) return;
-    std::vector<cplx<double>> t64(MAX_M);
-    std::vector<cplx<float>> t32(MAX_M);
+    if (g_tw[dev].hi64) return;
     const long double pi = 3.141592653589793238462643383279502884L;
-    for (int k = 0; k < MAX_M; ++k) {
-        const long double ang = -2.0L * pi * (long double)k / (long double)MAX_M;
-        t64[k] = {(double)cosl(ang), (double)sinl(ang)};
-        t32[k] = {(float)t64[k].x, (float)t64[k].y};
-    }
-    MG_HIP(hipMalloc(&g_tw[dev].f64, MAX_M * sizeof(cplx<double>)));
-    MG_HIP(hipMalloc(&g_tw[dev].f32, MAX_M * sizeof(cplx<float>)));
-    MG_HIP(hipMemcpy(g_tw[dev].f64, t64.data(), MAX_M * sizeof(cplx<double>), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(g_tw[dev].f32, t32.data(), MAX_M * sizeof(cplx<float>), hipMemcpyHostToDevice));
+    auto entry = [&](int m) {
+        const long double ang = -2.0L * pi * (long double)m / (long double)MAX_M;
+        return cplx<double>{(double)cosl(ang), (double)sinl(ang)};
+    };
+    std::vector<cplx<double>> hi64(TW_HI), lo64(TW_LO);
+    std::vector<cplx<float>> hi32(TW_HI), lo32(TW_LO);
+    for (int a = 0; a < TW_HI; ++a) { hi64[a] = entry(64 * a); hi32[a] = {(float)hi64[a].x, (float)hi64[a].y}; }
+    for (int b = 0; b < TW_LO; ++b) { lo64[b] = entry(b); lo32[b] = {(float)lo64[b].x, (float)lo64[b].y}; }
+    MG_HIP(hipMalloc(&g_tw[dev].hi64, TW_HI * sizeof(cplx<double>)));
+    MG_HIP(hipMalloc(&g_tw[dev].lo64, TW_LO * sizeof(cplx<double>)));
+    MG_HIP(hipMalloc(&g_tw[dev].hi32, TW_HI * sizeof(cplx<float>)));
+    MG_HIP(hipMalloc(&g_tw[dev].lo32, TW_LO * sizeof(cplx<float>)));
+    MG_HIP(hipMemcpy(g_tw[dev].hi64, hi64.data(), TW_HI * sizeof(cplx<double>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(g_tw[dev].lo64, lo64.data(), TW_LO * sizeof(cplx<double>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(g_tw[dev].hi32, hi32.data(), TW_HI * sizeof(cplx<float>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(g_tw[dev].lo32, lo32.data(), TW_LO * sizeof(cplx<float>), hipMemcpyHostToDevice));
     int lds = 65536;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
-    g_tw[dev].lds_max = lds;
+    g_tw[dev].lds_max = lds - (int)sizeof(TwLDS<double>);   // room for the staged twiddle tables
 }
 
 static int g_lds_cap = 0;   // test knob: LDS bytes the FFT planner may assume (0 = the device's)
 int fft_lds_max() {
-    const int dev = g_tw[current_device()].lds_max;
+    const int dev = g_tw[current_device()].lds_max;   // dynamic LDS a workgroup may use
     return g_lds_cap > 0 && g_lds_cap < dev ? g_lds_cap : dev;
 }
 void fft_set_lds_cap(int bytes) { g_lds_cap = bytes; }
@@ -408,7 +445,7 @@ template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* W
     int R = budget / per; R = R < 1 ? 1 : (R > 8 ? 8 : R);
     const int bytes = R * per;
     set_lds(k_rows_fwd<T>, bytes);
-    hipLaunchKernelGGL(k_rows_fwd<T>, dim3((N + R - 1) / R, 3), dim3(TPB), bytes, st, W, N, logM, R, Wh, twiddles<T>(current_device()));
+    hipLaunchKernelGGL(k_rows_fwd<T>, dim3((N + R - 1) / R, 3), dim3(FTPB), bytes, st, W, N, logM, R, Wh, twiddles<T>(current_device()));
     MG_HIP(hipGetLastError());
 }
 
@@ -418,24 +455,24 @@ template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T*
     const int one = M * (int)sizeof(cplx<T>);
     const int nb = column_buffers(M, (int)sizeof(cplx<T>), centred);
     require(nb > 0, "FFT size too large for LDS", 4);
-    const cplx<T>* tw = twiddles<T>(current_device());
+    const TwPtr<T> tw = twiddles<T>(current_device());
     if (centred) {
         if (nb == 3) {
             set_lds(k_columns_centred<T, 3>, 3 * one);
-            hipLaunchKernelGGL((k_columns_centred<T, 3>), dim3(Mh), dim3(TPB), 3 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
+            hipLaunchKernelGGL((k_columns_centred<T, 3>), dim3(Mh), dim3(FTPB), 3 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
         } else {
             require(stash != nullptr, "column stash missing");
             set_lds(k_columns_centred<T, 2>, 2 * one);
-            hipLaunchKernelGGL((k_columns_centred<T, 2>), dim3(Mh), dim3(TPB), 2 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
+            hipLaunchKernelGGL((k_columns_centred<T, 2>), dim3(Mh), dim3(FTPB), 2 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
         }
     } else {
         if (nb == 2) {
             set_lds(k_columns<T, 2>, 2 * one);
-            hipLaunchKernelGGL((k_columns<T, 2>), dim3(Mh), dim3(TPB), 2 * one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
+            hipLaunchKernelGGL((k_columns<T, 2>), dim3(Mh), dim3(FTPB), 2 * one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
         } else {
             require(stash != nullptr, "column stash missing");
             set_lds(k_columns<T, 1>, one);
-            hipLaunchKernelGGL((k_columns<T, 1>), dim3(Mh), dim3(TPB), one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
+            hipLaunchKernelGGL((k_columns<T, 1>), dim3(Mh), dim3(FTPB), one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
         }
     }
     MG_HIP(hipGetLastError());
@@ -451,28 +488,32 @@ template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* 
         int R = budget / (4 * per); R = R < 1 ? 1 : (R > 8 ? 8 : R);
         const int bytes = 4 * R * per;
         set_lds(k_rows_inv<T, 4>, bytes);
-        hipLaunchKernelGGL((k_rows_inv<T, 4>), dim3((N + R - 1) / R), dim3(TPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
+        hipLaunchKernelGGL((k_rows_inv<T, 4>), dim3((N + R - 1) / R), dim3(FTPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
     } else {
         require(per <= lds_max, "FFT size too large for LDS", 4);
         int R = budget / per; R = R < 1 ? 1 : (R > 8 ? 8 : R);
         const int bytes = R * per;
         set_lds(k_rows_inv<T, 1>, bytes);
-        hipLaunchKernelGGL((k_rows_inv<T, 1>), dim3((N + R - 1) / R, 4), dim3(TPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
+        hipLaunchKernelGGL((k_rows_inv<T, 1>), dim3((N + R - 1) / R, 4), dim3(FTPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
     }
     MG_HIP(hipGetLastError());
 }
 
 template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
                                        double* tmp, T* const* K, hipStream_t st) {
+    typedef T TC;   // the path's own precision
     const int logM = log2i(M), Mh = M / 2 + 1;
-    const int bytes = (M / 2) * (int)sizeof(cplx<double>);
-    require(bytes <= fft_lds_max(), "FFT size too large for LDS", 4);
-    const cplx<double>* tw = twiddles<double>(current_device());
-    set_lds(k_spec_rows, bytes);
-    hipLaunchKernelGGL(k_spec_rows, dim3(M), dim3(TPB), bytes, st, logM, ntab, delta, dof, table, (int)round_kernel, tmp, tw);
+    const int nfft = ntab == 2 ? 2 : 3;
+    const int bytes_rows = nfft * (M / 2) * (int)sizeof(cplx<TC>);
+    const int bytes_cols = ntab * (M / 2) * (int)sizeof(cplx<TC>);
+    require(bytes_rows <= fft_lds_max() && bytes_cols <= fft_lds_max(), "FFT size too large for LDS", 4);
+    const TwPtr<TC> tw = twiddles<TC>(current_device());
+    TC* t = reinterpret_cast<TC*>(tmp);
+    set_lds(k_spec_rows<TC>, bytes_rows);
+    hipLaunchKernelGGL(k_spec_rows<TC>, dim3(M), dim3(FTPB), bytes_rows, st, logM, ntab, delta, dof, table, (int)round_kernel, t, tw);
     MG_HIP(hipGetLastError());
-    set_lds(k_spec_cols<T>, bytes);
-    hipLaunchKernelGGL(k_spec_cols<T>, dim3(Mh), dim3(TPB), bytes, st, logM, ntab, tmp, K[0], K[1], K[2], K[3], tw);
+    set_lds(k_spec_cols<TC, T>, bytes_cols);
+    hipLaunchKernelGGL((k_spec_cols<TC, T>), dim3(Mh), dim3(FTPB), bytes_cols, st, logM, ntab, t, K[0], K[1], K[2], K[3], tw);
     MG_HIP(hipGetLastError());
 }
 
