@@ -244,8 +244,7 @@ template <class T> struct Repulsion {
         W.ensure((size_t)3 * N * N * sizeof(T));
         Wh.ensure((size_t)3 * Mh * N * sizeof(cplx<T>));
         Ph.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
-        const int nbuf = column_buffers(M, (int)sizeof(cplx<T>), centred);
-        if (nbuf < (centred ? 3 : 2)) stash.ensure((size_t)Mh * M * sizeof(cplx<T>));
+
         phi.ensure((size_t)4 * N * N * sizeof(T));
         zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
     }

@@ -12,7 +12,8 @@
 // buffer goes through swz.
 //
 // Twiddles: exp(-2 pi i m / MAX_M) = hi[m >> 6] * lo[m & 63], two small tables staged in LDS by
-// fft_stage_twiddles (MAX_M / 64 + 64 entries); the product of two correctly rounded entries.
+// fft_stage_twiddles (MAX_M / 64 + 64 entries); the product of two correctly rounded entries. A pass
+// looks up w^1, w^2, w^4, w^8 and forms the other powers with at most three complex products.
 #pragma once
 #include "common.hpp"
 
@@ -97,9 +98,16 @@ __device__ inline void stockham_pass(cplx<T>* lds, int base, int ntr, int logL, 
 #pragma unroll
         for (int r = 0; r < R; ++r) v[q][r] = lds[swz<T>(src + r * stride)];
         if (logNs > 0) {
+            // w^r for r < R from exact table entries at r = 1, 2, 4, 8 and at most three products
             const int step = k << (LOG_MAX_M - logNs - logR);
+            cplx<T> w[R];
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle<T, SIGN>(tw, r * step));
+            for (int r = 1; r < R; ++r) {
+                if ((r & (r - 1)) == 0) w[r] = twiddle<T, SIGN>(tw, r * step);
+                else { const int hi = 1 << (31 - __builtin_clz(r)); w[r] = cmul(w[hi], w[r - hi]); }
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[r]);
         }
         RegDFT<T, R, SIGN>::run(v[q]);
         out[q] = base + (tr << logL) + ((jj >> logNs) << (logNs + logR)) + k;

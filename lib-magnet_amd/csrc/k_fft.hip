@@ -97,159 +97,97 @@ template <class T> __device__ inline void store_column(cplx<T>* __restrict__ dst
     for (int g = threadIdx.x; g < N; g += FTPB) dst[g] = lds[swz<T>(base + g)];
 }
 
-// ---- columns, reference product set. NBUF 2: both channel-0 products held in LDS; 1: one product of
-// channel 0 parked in `stash` (HBM) while the other is transformed back.
-template <class T, int NBUF>
+// ---- columns. One workgroup per column ky: the three channel columns are transformed (rows >= N are
+// zero), multiplied by the kernel spectra and transformed back; only the N live rows are stored. The
+// channel-0 spectrum, used by every product, stays in registers (element k = tid + FTPB e of thread tid).
+// NB = LDS buffers of M elements: 2 pairs the transforms (channels 1 and 2, and the two channel-0
+// products), 1 runs them one at a time in half the LDS (two workgroups per CU at the largest M).
+// CEN selects the product set: false = the reference's, true = the centred one (odd-kernel spectra
+// i Tx, i Ty).
+template <class T, int NB, bool CEN>
 __global__ __launch_bounds__(FTPB) void k_columns(const cplx<T>* __restrict__ Wh, int N, int logM,
                                                   const T* __restrict__ K1, const T* __restrict__ K2,
-                                                  cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash, TwPtr<T> twp) {
+                                                  const T* __restrict__ TX, const T* __restrict__ TY,
+                                                  cplx<T>* __restrict__ Ph, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TwLDS<T> tw;
+    constexpr int EMAX = MAX_M / FTPB;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     fft_stage_twiddles(tw, twp.hi, twp.lo);
     const int M = 1 << logM, Mh = M / 2 + 1, ky = blockIdx.x;
-    const int A = 0, B = M;   // element offsets of the two buffers
+    const int A = 0, B = NB == 2 ? M : 0;
+    const int E = (M + FTPB - 1) / FTPB;
     const T* k1 = K1 + (size_t)ky * Mh;
     const T* k2 = K2 + (size_t)ky * Mh;
+    const T* tx = CEN ? TX + (size_t)ky * Mh : nullptr;
+    const T* ty = CEN ? TY + (size_t)ky * Mh : nullptr;
     auto col = [&](int ch) { return Wh + ((size_t)ch * Mh + ky) * N; };
     auto out = [&](int o) { return Ph + ((size_t)o * Mh + ky) * N; };
     auto at = [&](int buf, int k) -> cplx<T>& { return lds[swz<T>(buf + k)]; };
-    // channel 0: phi0 = K1 W0, phi1 = K2 W0
+    // product of channel ch (1, 2) at frequency k with spectrum s: reference K2 s; centred i t s0 - K2 s
+    auto prod = [&](int ch, int k, cplx<T> s0, cplx<T> s) {
+        const T c2 = ktab(k2, k, M, false);
+        if (!CEN) return cplx<T>{s.x * c2, s.y * c2};
+        const T t = ch == 1 ? ktab(tx, k, M, true) : ktab(ty, k, M, false);
+        return cplx<T>{-t * s0.y - c2 * s.x, t * s0.x - c2 * s.y};
+    };
+    cplx<T> s0[EMAX];
+    // channel 0 forward, kept in registers
     load_column(lds, A, col(0), N, M);
     __syncthreads();
     lds_fft<T, -1>(lds, A, 1, logM, tw);
-    for (int k = threadIdx.x; k < M; k += FTPB) {
-        const cplx<T> a = at(A, k);
-        const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-        if (NBUF == 2) at(B, k) = {a.x * c2, a.y * c2};
-        else stash[(size_t)ky * M + k] = {a.x * c2, a.y * c2};
-        at(A, k) = {a.x * c1, a.y * c1};
-    }
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) if (e < E) { const int k = threadIdx.x + e * FTPB; if (k < M) s0[e] = at(A, k); }
     __syncthreads();
-    if (NBUF == 2) {
+    if (NB == 2) {
+        // phi0 = K1 s0, phi1 = K2 s0, back together
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            const int k = threadIdx.x + e * FTPB;
+            if (e >= E || k >= M) continue;
+            const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
+            at(A, k) = {s0[e].x * c1, s0[e].y * c1};
+            at(B, k) = {s0[e].x * c2, s0[e].y * c2};
+        }
+        __syncthreads();
         lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
         store_column(out(0), lds, A, N);
         store_column(out(1), lds, B, N);
         __syncthreads();
-        // channels 1 and 2 together: phi2 = K2 W1, phi3 = K2 W2
+        // channels 1 and 2 together
         load_column(lds, A, col(1), N, M);
         load_column(lds, B, col(2), N, M);
         __syncthreads();
         lds_fft_batch<T, -1>(lds, A, 2, logM, tw);
-        for (int k = threadIdx.x; k < M; k += FTPB) {
-            const T c2 = ktab(k2, k, M, false);
-            const cplx<T> a = at(A, k), b = at(B, k);
-            at(A, k) = {a.x * c2, a.y * c2};
-            at(B, k) = {b.x * c2, b.y * c2};
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            const int k = threadIdx.x + e * FTPB;
+            if (e >= E || k >= M) continue;
+            at(A, k) = prod(1, k, s0[e], at(A, k));
+            at(B, k) = prod(2, k, s0[e], at(B, k));
         }
         __syncthreads();
         lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
         store_column(out(2), lds, A, N);
         store_column(out(3), lds, B, N);
     } else {
-        lds_fft<T, +1>(lds, A, 1, logM, tw);
-        store_column(out(0), lds, A, N);
-        __syncthreads();
-        for (int k = threadIdx.x; k < M; k += FTPB) at(A, k) = stash[(size_t)ky * M + k];
-        __syncthreads();
-        lds_fft<T, +1>(lds, A, 1, logM, tw);
-        store_column(out(1), lds, A, N);
-        for (int ch = 1; ch < 3; ++ch) {
-            __syncthreads();
-            load_column(lds, A, col(ch), N, M);
-            __syncthreads();
-            lds_fft<T, -1>(lds, A, 1, logM, tw);
-            for (int k = threadIdx.x; k < M; k += FTPB) {
-                const T c2 = ktab(k2, k, M, false);
-                const cplx<T> a = at(A, k);
-                at(A, k) = {a.x * c2, a.y * c2};
+        for (int o = 0; o < 4; ++o) {
+            if (o >= 2) {   // channel o - 1 forward
+                load_column(lds, A, col(o - 1), N, M);
+                __syncthreads();
+                lds_fft<T, -1>(lds, A, 1, logM, tw);
+            }
+#pragma unroll
+            for (int e = 0; e < EMAX; ++e) {
+                const int k = threadIdx.x + e * FTPB;
+                if (e >= E || k >= M) continue;
+                if (o < 2) { const T c = ktab(o == 0 ? k1 : k2, k, M, false); at(A, k) = {s0[e].x * c, s0[e].y * c}; }
+                else at(A, k) = prod(o - 1, k, s0[e], at(A, k));
             }
             __syncthreads();
             lds_fft<T, +1>(lds, A, 1, logM, tw);
-            store_column(out(ch + 1), lds, A, N);
-        }
-    }
-}
-
-// ---- columns, centred product set. NBUF 3: the three spectra in LDS; 2: the channel-0 spectrum parked
-// in `stash` (HBM) while channels 1 and 2 are transformed one at a time.
-template <class T, int NBUF>
-__global__ __launch_bounds__(FTPB) void k_columns_centred(const cplx<T>* __restrict__ Wh, int N, int logM,
-                                                          const T* __restrict__ K1, const T* __restrict__ K2,
-                                                          const T* __restrict__ TX, const T* __restrict__ TY,
-                                                          cplx<T>* __restrict__ Ph, cplx<T>* __restrict__ stash,
-                                                          TwPtr<T> twp) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
-    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp.hi, twp.lo);
-    const int M = 1 << logM, Mh = M / 2 + 1, ky = blockIdx.x;
-    const int A = 0, B = M, C = 2 * M;
-    const T* k1 = K1 + (size_t)ky * Mh;
-    const T* k2 = K2 + (size_t)ky * Mh;
-    const T* tx = TX + (size_t)ky * Mh;
-    const T* ty = TY + (size_t)ky * Mh;
-    auto col = [&](int ch) { return Wh + ((size_t)ch * Mh + ky) * N; };
-    auto out = [&](int o) { return Ph + ((size_t)o * Mh + ky) * N; };
-    auto at = [&](int buf, int k) -> cplx<T>& { return lds[swz<T>(buf + k)]; };
-    // odd-kernel product i t s0 - k2 s1
-    auto oddprod = [](cplx<T> s0, cplx<T> s1, T t, T c2) { return cplx<T>{-t * s0.y - c2 * s1.x, t * s0.x - c2 * s1.y}; };
-    if (NBUF == 3) {
-        load_column(lds, A, col(0), N, M);
-        load_column(lds, B, col(1), N, M);
-        load_column(lds, C, col(2), N, M);
-        __syncthreads();
-        lds_fft_batch<T, -1>(lds, A, 3, logM, tw);
-        for (int k = threadIdx.x; k < M; k += FTPB) {
-            const T c2 = ktab(k2, k, M, false), tX = ktab(tx, k, M, true), tY = ktab(ty, k, M, false);
-            const cplx<T> s0 = at(A, k);
-            at(B, k) = oddprod(s0, at(B, k), tX, c2);
-            at(C, k) = oddprod(s0, at(C, k), tY, c2);
-        }
-        __syncthreads();
-        lds_fft_batch<T, +1>(lds, B, 2, logM, tw);
-        store_column(out(2), lds, B, N);
-        store_column(out(3), lds, C, N);
-        __syncthreads();
-        for (int k = threadIdx.x; k < M; k += FTPB) {
-            const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-            const cplx<T> s0 = at(A, k);
-            at(B, k) = {s0.x * c2, s0.y * c2};
-            at(A, k) = {s0.x * c1, s0.y * c1};
-        }
-        __syncthreads();
-        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
-        store_column(out(0), lds, A, N);
-        store_column(out(1), lds, B, N);
-    } else {
-        cplx<T>* S0 = stash + (size_t)ky * M;
-        load_column(lds, A, col(0), N, M);
-        __syncthreads();
-        lds_fft<T, -1>(lds, A, 1, logM, tw);
-        for (int k = threadIdx.x; k < M; k += FTPB) {
-            const T c1 = ktab(k1, k, M, false), c2 = ktab(k2, k, M, false);
-            const cplx<T> s0 = at(A, k);
-            S0[k] = s0;
-            at(B, k) = {s0.x * c2, s0.y * c2};
-            at(A, k) = {s0.x * c1, s0.y * c1};
-        }
-        __syncthreads();
-        lds_fft_batch<T, +1>(lds, A, 2, logM, tw);
-        store_column(out(0), lds, A, N);
-        store_column(out(1), lds, B, N);
-        for (int ch = 1; ch < 3; ++ch) {
+            store_column(out(o), lds, A, N);
             __syncthreads();
-            load_column(lds, B, col(ch), N, M);
-            __syncthreads();
-            lds_fft<T, -1>(lds, B, 1, logM, tw);
-            const T* tt = ch == 1 ? tx : ty;
-            for (int k = threadIdx.x; k < M; k += FTPB) {
-                const T c2 = ktab(k2, k, M, false), t = ktab(tt, k, M, ch == 1);
-                at(B, k) = oddprod(S0[k], at(B, k), t, c2);
-            }
-            __syncthreads();
-            lds_fft<T, +1>(lds, B, 1, logM, tw);
-            store_column(out(ch + 1), lds, B, N);
         }
     }
 }
@@ -429,11 +367,14 @@ int fft_lds_max() {
 }
 void fft_set_lds_cap(int bytes) { g_lds_cap = bytes; }
 
-// LDS per workgroup for one stage, and whether the column pass needs the global stash
+// LDS buffers of M elements the column pass uses: 2 (paired transforms) when two fit in 64 KB, so that
+// at least two workgroups share a CU, else 1 (0: the column does not fit at all)
 int column_buffers(int M, int elem, bool centred) {
+    (void)centred;
     const int lds = fft_lds_max(), one = M * elem;
-    if (centred) return 3 * one <= lds ? 3 : (2 * one <= lds ? 2 : 0);
-    return 2 * one <= lds ? 2 : (one <= lds ? 1 : 0);
+    const int budget = lds < 65536 ? lds : 65536;
+    if (2 * one <= budget) return 2;
+    return one <= lds ? 1 : 0;
 }
 
 template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st) {
@@ -451,30 +392,19 @@ template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* W
 
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
                                        cplx<T>* stash, hipStream_t st) {
+    (void)stash;
     const int logM = log2i(M), Mh = M / 2 + 1;
     const int one = M * (int)sizeof(cplx<T>);
     const int nb = column_buffers(M, (int)sizeof(cplx<T>), centred);
     require(nb > 0, "FFT size too large for LDS", 4);
     const TwPtr<T> tw = twiddles<T>(current_device());
-    if (centred) {
-        if (nb == 3) {
-            set_lds(k_columns_centred<T, 3>, 3 * one);
-            hipLaunchKernelGGL((k_columns_centred<T, 3>), dim3(Mh), dim3(FTPB), 3 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
-        } else {
-            require(stash != nullptr, "column stash missing");
-            set_lds(k_columns_centred<T, 2>, 2 * one);
-            hipLaunchKernelGGL((k_columns_centred<T, 2>), dim3(Mh), dim3(FTPB), 2 * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, stash, tw);
-        }
-    } else {
-        if (nb == 2) {
-            set_lds(k_columns<T, 2>, 2 * one);
-            hipLaunchKernelGGL((k_columns<T, 2>), dim3(Mh), dim3(FTPB), 2 * one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
-        } else {
-            require(stash != nullptr, "column stash missing");
-            set_lds(k_columns<T, 1>, one);
-            hipLaunchKernelGGL((k_columns<T, 1>), dim3(Mh), dim3(FTPB), one, st, Wh, N, logM, K[0], K[1], Ph, stash, tw);
-        }
-    }
+#define MG_COL(NB, CEN) do { \
+        set_lds(k_columns<T, NB, CEN>, NB * one); \
+        hipLaunchKernelGGL((k_columns<T, NB, CEN>), dim3(Mh), dim3(FTPB), NB * one, st, Wh, N, logM, K[0], K[1], K[2], K[3], Ph, tw); \
+    } while (0)
+    if (nb == 2) { if (centred) MG_COL(2, true); else MG_COL(2, false); }
+    else { if (centred) MG_COL(1, true); else MG_COL(1, false); }
+#undef MG_COL
     MG_HIP(hipGetLastError());
 }
 

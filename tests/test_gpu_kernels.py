@@ -225,9 +225,9 @@ def test_repulsion_fp64_clipped_grid_three_sort_passes():
 
 @pytest.mark.parametrize("cap", [0, 65536, 40000])
 def test_repulsion_column_variants(cap):
-    """every column-pass variant (spectra kept in LDS, or channel 0 parked in HBM) and both row-inverse
-    variants against the oracle: the LDS cap forces the parked variants at a grid the oracle can check
-    (N = 540, M = 2048; cap 40000: FP64 one LDS column, FP32 two, rows inverse one channel at a time)."""
+    """every column-pass variant (paired transforms in two LDS buffers, or one at a time in one) and both
+    row-inverse variants against the oracle: the LDS cap forces the one-buffer variants at a grid the
+    oracle can check (N = 540, M = 2048; cap 40000: FP64 one buffer, rows inverse one channel at a time)."""
     Y = layout(30_000, 13, 25.0, k=12)
     xmin, ymin, width, nb = O.grid(Y)
     assert O.fft_size(nb * 3) == 2048
