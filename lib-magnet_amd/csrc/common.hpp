@@ -85,8 +85,9 @@ template <class T> struct RepBuffers {
     int* item_valid;
     T* item_val;                          // 2 per chunk x NI x 3NI
     T* W;                                 // (3, N, N) charges
-    cplx<T>* Wh;                          // (3, Mh, N) row spectra, transposed
+    cplx<T>* Wh;                          // (3, Mh, N) row spectra, column-major for the column pass
     cplx<T>* Ph;                          // (4, Mh, N) products after the column inverse
+    cplx<T>* Tr;                          // (4, N, Mh) row-major staging of the transposes
     cplx<T>* colstash;                    // (Mh, M) channel-0 column spectra when LDS is short
     T* phi;                               // (N, N, 4) potentials, interleaved
     double* zpart;                        // per-block Z partials
@@ -126,12 +127,13 @@ int gather_blocks(i64 n);
 void fft_init_twiddles();   // uploads the twiddle tables once per device
 int fft_lds_max();
 void fft_set_lds_cap(int bytes);
-template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);
+template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);   // Wh (3, N, Mh)
+template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int batch, int R, int C, hipStream_t st);
 // K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
                                        cplx<T>* stash, hipStream_t st);
 int column_buffers(int M, int elem, bool centred);   // LDS column buffers that fit (0: none)
-template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* phi, hipStream_t st);
+template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* phi, hipStream_t st);   // Ph (4, N, Mh)
 // kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M));
 // ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx], rounded to T
 // (round_kernel: kernel values rounded to float before the transform). tmp: 4 M Mh doubles.

@@ -174,7 +174,7 @@ template <class T> struct Repulsion {
     static constexpr int ntab = centred ? 4 : 2;
     std::list<Spec> cache;
     unsigned long tick = 0;
-    DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, stash, phi;
+    DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, Tr, phi;
     DevBuf zpart, counter, zout, inv_out, spec_tmp, table;
     int last_N = 0, last_M = 0, sorted_buf = 0;
     i64 last_n = 0;
@@ -244,6 +244,7 @@ template <class T> struct Repulsion {
         W.ensure((size_t)3 * N * N * sizeof(T));
         Wh.ensure((size_t)3 * Mh * N * sizeof(cplx<T>));
         Ph.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
+        Tr.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
 
         phi.ensure((size_t)4 * N * N * sizeof(T));
         zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
@@ -254,7 +255,7 @@ template <class T> struct Repulsion {
         b.keysA = dptr<int>(keysA); b.valsA = dptr<int>(valsA); b.keysB = dptr<int>(keysB); b.valsB = dptr<int>(valsB);
         b.hist = dptr<int>(hist); b.item_key = dptr<int>(item_key); b.item_valid = dptr<int>(item_valid);
         b.item_val = dptr<T>(item_val); b.W = dptr<T>(W); b.Wh = dptr<cplx<T>>(Wh); b.Ph = dptr<cplx<T>>(Ph);
-        b.colstash = dptr<cplx<T>>(stash); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
+        b.colstash = nullptr; b.Tr = dptr<cplx<T>>(Tr); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
         b.counter = dptr<unsigned>(counter); b.zout = dptr<double>(zout); b.inv_out = dptr<T>(inv_out);
         return b;
     }
@@ -289,14 +290,16 @@ template <class T> struct Repulsion {
         launch_spread<T>(keys, vals, Y, n, g, ni, centred, b, st);
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
         P.begin(ST_ROWS_FWD, st);
-        launch_rows_forward<T>(b.W, g.N, M, b.Wh, st);
-        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);
+        launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
+        launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
+        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);   // the transpose is not algorithmic traffic
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, b.colstash, st);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs);
         P.begin(ST_ROWS_INV, st);
-        launch_rows_inverse<T>(b.Ph, g.N, M, b.phi, st);
+        launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
+        launch_rows_inverse<T>(b.Tr, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts);
         P.begin(ST_GATHER, st);
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);

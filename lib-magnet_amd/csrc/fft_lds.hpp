@@ -38,27 +38,34 @@ __device__ constexpr double kS16[16] = {0.0, 0.38268343236508977173, 0.707106781
                                         0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
                                         -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
 
-// in-register DFT of size R (natural order), radix-2 decimation in time with constant twiddles
-template <class T, int R, int SIGN> struct RegDFT {
-    __device__ static inline void run(cplx<T>* v) {
-        cplx<T> e[R / 2], o[R / 2];
+// in-register DFT of size R: input in bit-reversed order, output natural, in place; log2(R) radix-2
+// decimation-in-time stages with constant twiddles exp(SIGN 2 pi i k / len) from the 16-point table
+template <class T, int R, int SIGN> __device__ __forceinline__ void reg_dft(cplx<T>* v) {
 #pragma unroll
-        for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
-        RegDFT<T, R / 2, SIGN>::run(e);
-        RegDFT<T, R / 2, SIGN>::run(o);
+    for (int len = 2; len <= R; len <<= 1) {
 #pragma unroll
-        for (int k = 0; k < R / 2; ++k) {
-            cplx<T> t;
-            const int q = k * (16 / R);   // exp(SIGN 2 pi i k / R) = entry q of the 16-point table
-            if (q == 0) t = o[k];
-            else if (q == 4) t = SIGN < 0 ? cplx<T>{o[k].y, -o[k].x} : cplx<T>{-o[k].y, o[k].x};
-            else t = cmul(o[k], cplx<T>{(T)kC16[q], (T)(SIGN * kS16[q])});
-            v[k] = cadd(e[k], t);
-            v[k + R / 2] = csub(e[k], t);
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int k = 0; k < len / 2; ++k) {
+                const int q = k * (16 / len);
+                const cplx<T> o = v[i + k + len / 2];
+                cplx<T> t;
+                if (q == 0) t = o;
+                else if (q == 4) t = SIGN < 0 ? cplx<T>{o.y, -o.x} : cplx<T>{-o.y, o.x};
+                else t = cmul(o, cplx<T>{(T)kC16[q], (T)(SIGN * kS16[q])});
+                const cplx<T> e = v[i + k];
+                v[i + k] = cadd(e, t);
+                v[i + k + len / 2] = csub(e, t);
+            }
         }
     }
-};
-template <class T, int SIGN> struct RegDFT<T, 1, SIGN> { __device__ static inline void run(cplx<T>*) {} };
+}
+
+template <int logR> __device__ constexpr int brev_c(int r) {
+    int o = 0;
+    for (int b = 0; b < logR; ++b) o |= ((r >> b) & 1) << (logR - 1 - b);
+    return o;
+}
 
 // LDS twiddle tables of one workgroup
 template <class T> struct TwLDS { cplx<T> hi[TW_HI]; cplx<T> lo[TW_LO]; };
@@ -74,8 +81,9 @@ template <class T, int SIGN> __device__ __forceinline__ cplx<T> twiddle(const Tw
     return w;
 }
 
-// Elements each thread holds in one pass, at most: ntr * L <= FFT_MAXE * blockDim.x.
-constexpr int FFT_MAXE = 32;
+// Elements each thread holds in one pass, at most: ntr * L <= FFT_MAXE * blockDim.x (16 complex values:
+// 32 VGPRs in FP32, 64 in FP64, so a column kernel also holding a spectrum stays clear of spills).
+constexpr int FFT_MAXE = 16;
 
 // One Stockham pass of radix R = 2^logR over `ntr` transforms of length 2^logL stored back to back from
 // element `base` of the swizzled region `lds`. Ns = 2^logNs is the product of the earlier radices.
@@ -85,7 +93,7 @@ __device__ inline void stockham_pass(cplx<T>* lds, int base, int ntr, int logL, 
     constexpr int MAXJ = FFT_MAXE / R;
     const int L = 1 << logL, stride = L >> logR;
     const int items = ntr * stride;
-    cplx<T> v[MAXJ][R];
+    cplx<T> v[MAXJ][R];   // v[q][brev(r)] holds input r: the in-register DFT takes bit-reversed input
     int out[MAXJ];
 #pragma unroll
     for (int q = 0; q < MAXJ; ++q) {
@@ -96,20 +104,24 @@ __device__ inline void stockham_pass(cplx<T>* lds, int base, int ntr, int logL, 
         const int k = jj & ((1 << logNs) - 1);
         const int src = base + (tr << logL) + jj;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q][r] = lds[swz<T>(src + r * stride)];
+        for (int r = 0; r < R; ++r) v[q][brev_c<logR>(r)] = lds[swz<T>(src + r * stride)];
         if (logNs > 0) {
             // w^r for r < R from exact table entries at r = 1, 2, 4, 8 and at most three products
             const int step = k << (LOG_MAX_M - logNs - logR);
-            cplx<T> w[R];
+            cplx<T> p2[4];
+#pragma unroll
+            for (int b = 0; b < logR; ++b) p2[b] = twiddle<T, SIGN>(tw, (1 << b) * step);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                if ((r & (r - 1)) == 0) w[r] = twiddle<T, SIGN>(tw, r * step);
-                else { const int hi = 1 << (31 - __builtin_clz(r)); w[r] = cmul(w[hi], w[r - hi]); }
-            }
+                cplx<T> w = {T(1), T(0)};
+                bool first = true;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[r]);
+                for (int b = 0; b < logR; ++b)
+                    if ((r >> b) & 1) { w = first ? p2[b] : cmul(w, p2[b]); first = false; }
+                v[q][brev_c<logR>(r)] = cmul(v[q][brev_c<logR>(r)], w);
+            }
         }
-        RegDFT<T, R, SIGN>::run(v[q]);
+        reg_dft<T, R, SIGN>(v[q]);
         out[q] = base + (tr << logL) + ((jj >> logNs) << (logNs + logR)) + k;
     }
     __syncthreads();

@@ -2,13 +2,13 @@
 // on gfx950 and written for this path (no rocFFT). Transforms are the Stockham radix-16/8/4/2 passes of
 // fft_lds.hpp.
 //   rows forward : real rows of length M (N live values, zero padding) -> M/2 + 1 complex, through one
-//                  complex transform of length M/2 (even / odd samples as real / imaginary parts); the
-//                  output is stored transposed, (channel, ky, gx), so the column pass reads contiguously;
+//                  complex transform of length M/2 (even / odd samples as real / imaginary parts), stored
+//                  by rows; a tiled transpose then lays them out (channel, ky, gx) for the column pass;
 //   columns      : per ky, the three columns are transformed (rows >= N are zero), multiplied by the
 //                  kernel spectra and transformed back; only the N live rows are stored. Forward,
 //                  products and inverse stay in one workgroup;
-//   rows inverse : Hermitian half rows -> real rows of length M through one complex transform of length
-//                  M/2, scaled by 1/M^2, stored interleaved (gx, gy, 4) for the gather.
+//   rows inverse : (after a transpose back to rows) Hermitian half rows -> real rows of length M through
+//                  one complex transform of length M/2, scaled by 1/M^2, stored interleaved (gx, gy, 4).
 // Two product sets. The reference's, used by the FP64 path: phi0 = K1*W0, phi1..3 = K2*W0..2 with the
 // charges (1, x, y). The centred one, used by the FP32 path, where the charges are (1, x - x_node,
 // y - y_node): phi0 = K1*W0, phi1 = K2*W0, phi2 = (x K2)*W0 - K2*W1', phi3 = (y K2)*W0 - K2*W2'. The
@@ -38,7 +38,8 @@ template <class T> TwPtr<T> twiddles(int dev);
 template <> TwPtr<float> twiddles<float>(int dev) { return {g_tw[dev].hi32, g_tw[dev].lo32}; }
 template <> TwPtr<double> twiddles<double>(int dev) { return {g_tw[dev].hi64, g_tw[dev].lo64}; }
 
-constexpr int FTPB = 512;   // threads of the FFT workgroups
+// threads of the FFT workgroups; __launch_bounds__(FTPB, 1) lets the compiler use up to 256 VGPRs
+constexpr int FTPB = 512;
 
 int current_device() { int d = 0; MG_HIP(hipGetDevice(&d)); return d; }
 
@@ -55,7 +56,7 @@ template <class T> __device__ inline cplx<T> r2c_post(const cplx<T>* lds, int ba
 
 // ---- rows forward: blockIdx.y = channel, R rows per workgroup
 template <class T>
-__global__ __launch_bounds__(FTPB) void k_rows_fwd(const T* __restrict__ W, int N, int logM, int R,
+__global__ __launch_bounds__(FTPB, 1) void k_rows_fwd(const T* __restrict__ W, int N, int logM, int R,
                                                    cplx<T>* __restrict__ Wh, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TwLDS<T> tw;
@@ -76,11 +77,30 @@ __global__ __launch_bounds__(FTPB) void k_rows_fwd(const T* __restrict__ W, int 
     }
     __syncthreads();
     lds_fft_batch<T, -1>(lds, 0, R, logL, tw);
-    for (int e = threadIdx.x; e < R * Mh; e += FTPB) {
-        const int k = e / R, r = e - k * R;
-        if (r < rows) Wh[((size_t)c * Mh + k) * N + gx0 + r] = r2c_post(lds, r * L, k, L, logM, tw);
+    for (int e = threadIdx.x; e < R * Mh; e += FTPB) {   // row-major (channel, gx, ky): contiguous rows
+        const int r = e / Mh, k = e - r * Mh;
+        if (r < rows) Wh[((size_t)c * N + gx0 + r) * Mh + k] = r2c_post(lds, r * L, k, L, logM, tw);
     }
 }
+
+// ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C) {
+    __shared__ cplx<T> tile[32][33];
+    const size_t off = (size_t)blockIdx.z * R * C;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < R && c < C) tile[i][tx] = src[off + (size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (r < R && c < C) dst[off + (size_t)c * R + r] = tile[tx][i];
+    }
+}
+
 
 // kernel table value at column frequency kx (stored for kx <= M/2; `odd` flips the sign above M/2)
 template <class T> __device__ inline T ktab(const T* row, int kx, int M, bool odd) {
@@ -105,7 +125,7 @@ template <class T> __device__ inline void store_column(cplx<T>* __restrict__ dst
 // CEN selects the product set: false = the reference's, true = the centred one (odd-kernel spectra
 // i Tx, i Ty).
 template <class T, int NB, bool CEN>
-__global__ __launch_bounds__(FTPB) void k_columns(const cplx<T>* __restrict__ Wh, int N, int logM,
+__global__ __launch_bounds__(FTPB, 1) void k_columns(const cplx<T>* __restrict__ Wh, int N, int logM,
                                                   const T* __restrict__ K1, const T* __restrict__ K2,
                                                   const T* __restrict__ TX, const T* __restrict__ TY,
                                                   cplx<T>* __restrict__ Ph, TwPtr<T> twp) {
@@ -198,7 +218,7 @@ template <> struct vec4<float> { typedef float4 type; };
 template <> struct vec4<double> { typedef double4 type; };
 
 template <class T, int CH>
-__global__ __launch_bounds__(FTPB) void k_rows_inv(const cplx<T>* __restrict__ Ph, int N, int logM, int R,
+__global__ __launch_bounds__(FTPB, 1) void k_rows_inv(const cplx<T>* __restrict__ Ph, int N, int logM, int R,
                                                    T* __restrict__ phi, T scale, TwPtr<T> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TwLDS<T> tw;
@@ -211,11 +231,11 @@ __global__ __launch_bounds__(FTPB) void k_rows_inv(const cplx<T>* __restrict__ P
     // Z_k = E + i O with E = X_k + conj X_{L-k}, O = (X_k - conj X_{L-k}) exp(+2 pi i k / M)
     for (int e = threadIdx.x; e < CH * R * L; e += FTPB) {
         const int oc = e / (R * L), rem = e - oc * (R * L);
-        const int k = rem / R, r = rem - k * R;
+        const int r = rem / L, k = rem - r * L;
         cplx<T> z = {T(0), T(0)};
-        if (r < rows) {
-            const size_t base = (size_t)(o0 + oc) * Mh * N + gx0 + r;
-            const cplx<T> xk = Ph[base + (size_t)k * N], xc = cconj(Ph[base + (size_t)(L - k) * N]);
+        if (r < rows) {   // row-major input (channel, gx, ky)
+            const size_t base = ((size_t)(o0 + oc) * N + gx0 + r) * Mh;
+            const cplx<T> xk = Ph[base + k], xc = cconj(Ph[base + (L - k)]);
             const cplx<T> ev = cadd(xk, xc);
             const cplx<T> od = cmul(csub(xk, xc), twiddle<T, +1>(tw, (k << (LOG_MAX_M - logM)) & (MAX_M - 1)));
             z = {ev.x - od.y, ev.y + od.x};
@@ -269,7 +289,7 @@ __device__ inline void kernel_values(int a, int b, int M, double delta, double d
 }
 
 template <class TC>
-__global__ __launch_bounds__(FTPB) void k_spec_rows(int logM, int ntab, double delta, double dof,
+__global__ __launch_bounds__(FTPB, 1) void k_spec_rows(int logM, int ntab, double delta, double dof,
                                                     const double* __restrict__ table, int round_kernel,
                                                     TC* __restrict__ tmp, TwPtr<TC> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -302,7 +322,7 @@ __global__ __launch_bounds__(FTPB) void k_spec_rows(int logM, int ntab, double d
 }
 
 template <class TC, class T>
-__global__ __launch_bounds__(FTPB) void k_spec_cols(int logM, int ntab, const TC* __restrict__ tmp, T* __restrict__ K1,
+__global__ __launch_bounds__(FTPB, 1) void k_spec_cols(int logM, int ntab, const TC* __restrict__ tmp, T* __restrict__ K1,
                                                     T* __restrict__ K2, T* __restrict__ TX, T* __restrict__ TY,
                                                     TwPtr<TC> twp) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -383,10 +403,15 @@ template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* W
     const int per = L * (int)sizeof(cplx<T>);
     require(per <= lds_max, "FFT size too large for LDS", 4);
     const int budget = lds_max < 65536 ? lds_max : 65536;
-    int R = budget / per; R = R < 1 ? 1 : (R > 8 ? 8 : R);
+    int R = budget / per; R = R < 1 ? 1 : (R > 4 ? 4 : R);
     const int bytes = R * per;
     set_lds(k_rows_fwd<T>, bytes);
     hipLaunchKernelGGL(k_rows_fwd<T>, dim3((N + R - 1) / R, 3), dim3(FTPB), bytes, st, W, N, logM, R, Wh, twiddles<T>(current_device()));
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int batch, int R, int C, hipStream_t st) {
+    hipLaunchKernelGGL(k_transpose<T>, dim3((C + 31) / 32, (R + 31) / 32, batch), dim3(256), 0, st, src, dst, R, C);
     MG_HIP(hipGetLastError());
 }
 
@@ -414,14 +439,14 @@ template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* 
     const int per = L * (int)sizeof(cplx<T>);
     const T scale = (T)(1.0 / ((double)M * (double)M));
     const int budget = lds_max < 65536 ? lds_max : 65536;
-    if (4 * per <= lds_max) {
-        int R = budget / (4 * per); R = R < 1 ? 1 : (R > 8 ? 8 : R);
+    if (4 * per <= lds_max) {   // all four channels of a row: interleaved (gx, gy, 4) stores
+        int R = budget / (4 * per); R = R < 1 ? 1 : (R > 4 ? 4 : R);
         const int bytes = 4 * R * per;
         set_lds(k_rows_inv<T, 4>, bytes);
         hipLaunchKernelGGL((k_rows_inv<T, 4>), dim3((N + R - 1) / R), dim3(FTPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
     } else {
         require(per <= lds_max, "FFT size too large for LDS", 4);
-        int R = budget / per; R = R < 1 ? 1 : (R > 8 ? 8 : R);
+        int R = budget / per; R = R < 1 ? 1 : (R > 4 ? 4 : R);
         const int bytes = R * per;
         set_lds(k_rows_inv<T, 1>, bytes);
         hipLaunchKernelGGL((k_rows_inv<T, 1>), dim3((N + R - 1) / R, 4), dim3(FTPB), bytes, st, Ph, N, logM, R, phi, scale, twiddles<T>(current_device()));
@@ -449,6 +474,7 @@ template <class T> void launch_spectra(int M, double delta, double dof, const do
 
 #define MG_INST(T) \
     template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
+    template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
     template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, cplx<T>*, hipStream_t); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
     template void launch_spectra<T>(int, double, double, const double*, bool, int, double*, T* const*, hipStream_t);
