@@ -54,9 +54,11 @@ int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* coun
 /* Grid sizes of the profiled iterations: mean and largest live side N = n_interp * n_boxes, largest M. */
 int mg_profile_grid(double* mean_N, int* max_N, int* max_M);
 
-/* Caps the LDS bytes per workgroup the FFT stages plan for (0 = the device limit). Smaller caps select
- * the variants that park column spectra in HBM; for tests of those variants at small grid sizes. */
-int mg_set_lds_limit(int bytes);
+/* Padded FFT size the device uses for a live grid side N = n_interp * n_boxes: the smallest of 2^s and
+ * 3 * 2^s (128 <= M <= 8192) that is >= 2N, or 0 when N > 4096. The reference pads to
+ * scipy.fft.next_fast_len(2N) (magnet/tsne.py:50); the results do not depend on the padding beyond
+ * rounding, since any M >= 2N - 1 holds the linear convolution. */
+int mg_fft_size(int N);
 
 /* ---- per-call kernels (device pointers) ----------------------------------------------------------- */
 

@@ -18,10 +18,10 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char* what);
 #define MG_HIP(call) ::mg::hip_check((call), #call)
 inline void require(bool ok, const std::string& msg, int code = 1) { if (!ok) throw Error(code, msg); }
+constexpr int ERR_UNSUP = 4;   // the C-ABI's MG_ERR_UNSUP
 
 constexpr int WAVE = 64;
 constexpr int MAX_M = 8192;          // largest device FFT size (n_interp * n_boxes <= 4096)
-constexpr int LOG_MAX_M = 13;
 
 template <class T> struct cplx { T x, y; };
 template <class T> __host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
@@ -88,8 +88,7 @@ template <class T> struct RepBuffers {
     cplx<T>* Wh;                          // (3, Mh, N) row spectra, column-major for the column pass
     cplx<T>* Ph;                          // (4, Mh, N) products after the column inverse
     cplx<T>* Tr;                          // (4, N, Mh) row-major staging of the transposes
-    cplx<T>* colstash;                    // (Mh, M) channel-0 column spectra when LDS is short
-    T* phi;                               // (N, N, 4) potentials, interleaved
+    T* phi;                               // (4, N, N) potentials, planar
     double* zpart;                        // per-block Z partials
     unsigned* counter;                    // last-block counters (zeroed at creation, restored by kernels)
     double* zout;                         // [0] sumQ, [1] 1/max(sumQ, 1e-12)
@@ -123,21 +122,21 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
                                       bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
 int gather_blocks(i64 n);
 
-// FFT stages (M a power of two, N <= M / 2)
+// FFT stages. M = 2^s or 3 * 2^s, 128 <= M <= MAX_M, N <= M / 2.
 void fft_init_twiddles();   // uploads the twiddle tables once per device
-int fft_lds_max();
-void fft_set_lds_cap(int bytes);
-template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* Wh, hipStream_t st);   // Wh (3, N, Mh)
+int fft_size(int N);        // smallest supported M >= 2N (0 when none)
+template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* out, hipStream_t st);   // out (3, N, Mh)
 template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int batch, int R, int C, hipStream_t st);
-// K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]
+// K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]. Wh (3, Mh, N),
+// Ph (4, Mh, N). centred must be true for float and false for double.
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
-                                       cplx<T>* stash, hipStream_t st);
-int column_buffers(int M, int elem, bool centred);   // LDS column buffers that fit (0: none)
-template <class T> void launch_rows_inverse(const cplx<T>* Ph, int N, int M, T* phi, hipStream_t st);   // Ph (4, N, Mh)
-// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M));
-// ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx], rounded to T
-// (round_kernel: kernel values rounded to float before the transform). tmp: 4 M Mh doubles.
+                                       hipStream_t st);
+template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st);   // X (4, N, Mh)
+// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M), even
+// in each offset); ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx]
+// (round_kernel: kernel values rounded to float before the transform). tmp: 4 Mh^2 complex values.
 template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
-                                       double* tmp, T* const* K, hipStream_t st);
+                                       cplx<T>* tmp, T* const* K, hipStream_t st);
+template <class T> void launch_phi_interleave(const T* planar, T* out, i64 NN, hipStream_t st);   // (4,N,N) -> (N,N,4)
 
 }  // namespace mg

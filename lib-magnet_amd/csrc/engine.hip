@@ -60,7 +60,6 @@ struct PinnedBuf {
     }
 };
 
-inline int pow2_at_least(i64 v) { int m = 1; while (m < v) m <<= 1; return m; }
 
 // ---- CSR check on the device (validate_csr, src/layout.hpp:21-32): bit 1 indptr[0] != 0,
 // 2 decreasing indptr, 4 indptr[n] != nnz, 8 column index out of range
@@ -205,7 +204,7 @@ template <class T> struct Repulsion {
         const int Mh = M / 2 + 1;
         T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure((size_t)Mh * Mh * sizeof(T)));
-        double* tmp = reinterpret_cast<double*>(spec_tmp.ensure((size_t)4 * M * Mh * sizeof(double)));
+        cplx<T>* tmp = reinterpret_cast<cplx<T>*>(spec_tmp.ensure((size_t)4 * Mh * Mh * sizeof(cplx<T>)));
         const double* tab = nullptr;
         if (!tsne_kernel) {
             // resample the caller's table (offsets up to M_in / 2) onto the M x M torus
@@ -255,16 +254,12 @@ template <class T> struct Repulsion {
         b.keysA = dptr<int>(keysA); b.valsA = dptr<int>(valsA); b.keysB = dptr<int>(keysB); b.valsB = dptr<int>(valsB);
         b.hist = dptr<int>(hist); b.item_key = dptr<int>(item_key); b.item_valid = dptr<int>(item_valid);
         b.item_val = dptr<T>(item_val); b.W = dptr<T>(W); b.Wh = dptr<cplx<T>>(Wh); b.Ph = dptr<cplx<T>>(Ph);
-        b.colstash = nullptr; b.Tr = dptr<cplx<T>>(Tr); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
+        b.Tr = dptr<cplx<T>>(Tr); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
         b.counter = dptr<unsigned>(counter); b.zout = dptr<double>(zout); b.inv_out = dptr<T>(inv_out);
         return b;
     }
 
-    int device_M(int N) const {
-        int M = pow2_at_least(2 * (i64)N);
-        if (M < 4) M = 4;
-        return M;
-    }
+    int device_M(int N) const { return fft_size(N); }
 
     // enqueue the whole repulsion; sumQ and 1/sumQ stay on the device (zout, inv_out)
     void compute(const T* Y, i64 n, const GridT<T>& g, T* F, double* sumq_hist, hipStream_t st) {
@@ -273,7 +268,7 @@ template <class T> struct Repulsion {
         require(n < (i64)INT32_MAX, "too many points");
         if (!tsne_kernel) require(M_in >= 2 * g.N - 1, "padded size below 2N - 1 (aliased convolution) is not supported", MG_ERR_UNSUP);
         const int M = device_M(g.N);
-        require(M <= MAX_M, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+        require(M > 0, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
         ensure(n, g.N, M);
         Prof& P = g_prof;
         const double ts = sizeof(T), NN = (double)g.N * g.N, cs = 2 * ts, Mh = M / 2 + 1, dn = (double)n;
@@ -295,7 +290,7 @@ template <class T> struct Repulsion {
         P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);   // the transpose is not algorithmic traffic
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
-        launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, b.colstash, st);
+        launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, st);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs);
         P.begin(ST_ROWS_INV, st);
         launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
@@ -461,12 +456,7 @@ int mg_profile_enable(int on) {
 }
 int mg_profile_reset(void) { MG_API_BEGIN g_prof.reset(); MG_API_END }
 int mg_profile_num_stages(void) { return NSTAGE; }
-int mg_set_lds_limit(int bytes) {
-    MG_API_BEGIN
-    require(bytes >= 0, "bytes must be >= 0");
-    fft_set_lds_cap(bytes);
-    MG_API_END
-}
+int mg_fft_size(int N) { return N >= 1 ? fft_size(N) : 0; }
 int mg_profile_grid(double* mean_N, int* max_N, int* max_M) {
     MG_API_BEGIN
     if (mean_N) *mean_N = g_prof.iters ? g_prof.n_sum / (double)g_prof.iters : 0.0;
@@ -599,14 +589,15 @@ int mg_fftconv_compute(mg_fftconv* c, const void* Y, int64_t n, double xmin, dou
 int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
     MG_API_BEGIN
     require(c != nullptr, "null object");
-    auto copy = [&](auto& r, size_t ts) {
+    auto copy = [&](auto& r, auto zero) {
+        typedef decltype(zero) T;
         require(r.last_N > 0, "no compute yet");
         const size_t NN = (size_t)r.last_N * r.last_N;
-        if (d_W) MG_HIP(hipMemcpyAsync(d_W, r.W.p, 3 * NN * ts, hipMemcpyDeviceToDevice, g_stream));
-        if (d_phi) MG_HIP(hipMemcpyAsync(d_phi, r.phi.p, 4 * NN * ts, hipMemcpyDeviceToDevice, g_stream));
+        if (d_W) MG_HIP(hipMemcpyAsync(d_W, r.W.p, 3 * NN * sizeof(T), hipMemcpyDeviceToDevice, g_stream));
+        if (d_phi) launch_phi_interleave<T>(dptr<T>(r.phi), reinterpret_cast<T*>(d_phi), (i64)NN, g_stream);
         MG_HIP(hipStreamSynchronize(g_stream));
     };
-    if (c->prec == MG_F64) copy(*c->f64, sizeof(double)); else copy(*c->f32, sizeof(float));
+    if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
     MG_API_END
 }
 

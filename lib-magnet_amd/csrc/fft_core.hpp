@@ -1,0 +1,250 @@
+// Workgroup FFTs for gfx950 with plans fixed at compile time. Lengths L = 2^s or 3 * 2^s (64 <= L <= 8192),
+// each transform held by TH = L / 16 threads. A plan is a list of radices (3 .. 32); every pass is a Stockham
+// auto-sort step computed in registers: item j (< L / R) reads elements j + r L / R (r < R), multiplies element
+// r by w^(r k) with w = exp(sign 2 pi i / (Ns R)), Ns the product of the earlier radices and k = j mod Ns, runs
+// an R-point DFT and writes element r to (j - k) R + k + r Ns. Natural order in and out.
+//
+// Only the passes in between exchange data through LDS: the first pass takes its inputs from a caller's
+// functor (global memory, registers) and the last pass hands its outputs to one. The last pass leaves element
+// j + r L / R_last (j = t + q TH) in slot (q, r) of thread t, which is also where the first pass of a plan
+// starting with R_last takes it from; so a forward transform, a pointwise product and an inverse transform
+// whose plan is the reverse of the forward one chain through registers.
+//
+// LDS layout: element i of a buffer lives at swz(i) = i ^ ((i >> S) & (2^S - 1)), 2^S elements per 128
+// bytes (S = 4 for float complex, 3 for double complex); the XOR keeps the strided reads and the expanded
+// writes of a pass on distinct banks across 2^S consecutive lanes.
+//
+// Twiddles: exp(-2 pi i m / TW_PERIOD) = hi[m >> 6] * lo[m & 63], TW_PERIOD = 3 * 8192 so that every
+// supported length divides it; two small tables staged in LDS per workgroup. A pass looks up w^(2^b) and
+// forms the other powers with products (at most three deep).
+#pragma once
+#include "common.hpp"
+#include <utility>
+
+namespace mg {
+
+template <class T> struct Swz { static constexpr int S = sizeof(cplx<T>) == 8 ? 4 : 3; };
+template <class T> __device__ __forceinline__ int swz(int i) {
+    constexpr int S = Swz<T>::S;
+    return i ^ ((i >> S) & ((1 << S) - 1));
+}
+
+constexpr int TW_PERIOD = 3 * MAX_M;       // 24576
+constexpr int TW_LO = 64;                  // exp(-2 pi i b / TW_PERIOD), b < 64
+constexpr int TW_HI = TW_PERIOD / TW_LO;   // exp(-2 pi i 64 a / TW_PERIOD), a < 384
+
+template <class T> struct TwLDS { cplx<T> hi[TW_HI]; cplx<T> lo[TW_LO]; };
+template <class T> struct TwPtr { const cplx<T>* hi; const cplx<T>* lo; };
+template <class T> TwPtr<T> fft_twiddles();   // this device's tables (after fft_init_twiddles)
+
+template <class T> __device__ inline void fft_stage_twiddles(TwLDS<T>& s, TwPtr<T> p) {
+    for (int i = threadIdx.x; i < TW_HI; i += blockDim.x) s.hi[i] = p.hi[i];
+    for (int i = threadIdx.x; i < TW_LO; i += blockDim.x) s.lo[i] = p.lo[i];
+}
+// exp(SIGN 2 pi i m / TW_PERIOD), 0 <= m < TW_PERIOD
+template <class T, int SIGN> __device__ __forceinline__ cplx<T> twiddle(const TwLDS<T>& s, int m) {
+    cplx<T> w = cmul(s.hi[m >> 6], s.lo[m & 63]);
+    if (SIGN > 0) w.y = -w.y;
+    return w;
+}
+
+// cos and sin of 2 pi m / 96 (every radix used divides 96), built from the first octant by symmetry
+__device__ constexpr double kC96[96] = {
+    1.0, 0.9978589232386035, 0.9914448613738104, 0.9807852804032304, 0.9659258262890683, 0.9469301294951057,
+    0.9238795325112867, 0.8968727415326883, 0.8660254037844386, 0.8314696123025452, 0.7933533402912352, 0.7518398074789774,
+    0.7071067811865476, 0.6593458151000688, 0.6087614290087207, 0.5555702330196022, 0.5, 0.4422886902190013,
+    0.3826834323650898, 0.3214394653031616, 0.25881904510252074, 0.19509032201612828, 0.1305261922200516, 0.06540312923014306,
+    0.0, -0.06540312923014306, -0.1305261922200516, -0.19509032201612828, -0.25881904510252074, -0.3214394653031616,
+    -0.3826834323650898, -0.4422886902190013, -0.5, -0.5555702330196022, -0.6087614290087207, -0.6593458151000688,
+    -0.7071067811865476, -0.7518398074789774, -0.7933533402912352, -0.8314696123025452, -0.8660254037844386, -0.8968727415326883,
+    -0.9238795325112867, -0.9469301294951057, -0.9659258262890683, -0.9807852804032304, -0.9914448613738104, -0.9978589232386035,
+    -1.0, -0.9978589232386035, -0.9914448613738104, -0.9807852804032304, -0.9659258262890683, -0.9469301294951057,
+    -0.9238795325112867, -0.8968727415326883, -0.8660254037844386, -0.8314696123025452, -0.7933533402912352, -0.7518398074789774,
+    -0.7071067811865476, -0.6593458151000688, -0.6087614290087207, -0.5555702330196022, -0.5, -0.4422886902190013,
+    -0.3826834323650898, -0.3214394653031616, -0.25881904510252074, -0.19509032201612828, -0.1305261922200516, -0.06540312923014306,
+    0.0, 0.06540312923014306, 0.1305261922200516, 0.19509032201612828, 0.25881904510252074, 0.3214394653031616,
+    0.3826834323650898, 0.4422886902190013, 0.5, 0.5555702330196022, 0.6087614290087207, 0.6593458151000688,
+    0.7071067811865476, 0.7518398074789774, 0.7933533402912352, 0.8314696123025452, 0.8660254037844386, 0.8968727415326883,
+    0.9238795325112867, 0.9469301294951057, 0.9659258262890683, 0.9807852804032304, 0.9914448613738104, 0.9978589232386035};
+__device__ constexpr double kS96[96] = {
+    0.0, 0.06540312923014306, 0.1305261922200516, 0.19509032201612828, 0.25881904510252074, 0.3214394653031616,
+    0.3826834323650898, 0.4422886902190013, 0.5, 0.5555702330196022, 0.6087614290087207, 0.6593458151000688,
+    0.7071067811865476, 0.7518398074789774, 0.7933533402912352, 0.8314696123025452, 0.8660254037844386, 0.8968727415326883,
+    0.9238795325112867, 0.9469301294951057, 0.9659258262890683, 0.9807852804032304, 0.9914448613738104, 0.9978589232386035,
+    1.0, 0.9978589232386035, 0.9914448613738104, 0.9807852804032304, 0.9659258262890683, 0.9469301294951057,
+    0.9238795325112867, 0.8968727415326883, 0.8660254037844386, 0.8314696123025452, 0.7933533402912352, 0.7518398074789774,
+    0.7071067811865476, 0.6593458151000688, 0.6087614290087207, 0.5555702330196022, 0.5, 0.4422886902190013,
+    0.3826834323650898, 0.3214394653031616, 0.25881904510252074, 0.19509032201612828, 0.1305261922200516, 0.06540312923014306,
+    0.0, -0.06540312923014306, -0.1305261922200516, -0.19509032201612828, -0.25881904510252074, -0.3214394653031616,
+    -0.3826834323650898, -0.4422886902190013, -0.5, -0.5555702330196022, -0.6087614290087207, -0.6593458151000688,
+    -0.7071067811865476, -0.7518398074789774, -0.7933533402912352, -0.8314696123025452, -0.8660254037844386, -0.8968727415326883,
+    -0.9238795325112867, -0.9469301294951057, -0.9659258262890683, -0.9807852804032304, -0.9914448613738104, -0.9978589232386035,
+    -1.0, -0.9978589232386035, -0.9914448613738104, -0.9807852804032304, -0.9659258262890683, -0.9469301294951057,
+    -0.9238795325112867, -0.8968727415326883, -0.8660254037844386, -0.8314696123025452, -0.7933533402912352, -0.7518398074789774,
+    -0.7071067811865476, -0.6593458151000688, -0.6087614290087207, -0.5555702330196022, -0.5, -0.4422886902190013,
+    -0.3826834323650898, -0.3214394653031616, -0.25881904510252074, -0.19509032201612828, -0.1305261922200516, -0.06540312923014306};
+
+// x * exp(SIGN 2 pi i m / R); m and R are constants once the callers' loops are unrolled, quarter turns
+// cost no multiplication
+template <class T, int SIGN> __device__ __forceinline__ cplx<T> rot(cplx<T> x, int m, int R) {
+    const int q = ((m % R) * (96 / R)) % 96;
+    if (q == 0) return x;
+    if (q == 48) return {-x.x, -x.y};
+    if (q == 24) return SIGN > 0 ? cplx<T>{-x.y, x.x} : cplx<T>{x.y, -x.x};
+    if (q == 72) return SIGN > 0 ? cplx<T>{x.y, -x.x} : cplx<T>{-x.y, x.x};
+    return cmul(x, cplx<T>{(T)kC96[q], (T)(SIGN * kS96[q])});
+}
+
+// R-point DFT in registers, natural order in and out: X_k = sum_n x_n exp(SIGN 2 pi i n k / R).
+// R = R1 R2 splits into R2 DFTs of size R1 over x[n2 + R2 n1], twiddles exp(SIGN 2 pi i n2 k1 / R), and R1
+// DFTs of size R2 (Cooley-Tukey); R1 = 4 when 4 divides R, else 2 or 3.
+template <class T, int R, int SIGN> __device__ __forceinline__ void dft(cplx<T>* x) {
+    if constexpr (R == 2) {
+        const cplx<T> a = x[0], b = x[1];
+        x[0] = cadd(a, b); x[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        constexpr double h = SIGN * 0.86602540378443864676;   // SIGN sqrt(3) / 2
+        const cplx<T> a = x[0], s = cadd(x[1], x[2]), d = csub(x[1], x[2]);
+        const cplx<T> m = {a.x - T(0.5) * s.x, a.y - T(0.5) * s.y};
+        const cplx<T> t = {(T)(-h) * d.y, (T)h * d.x};
+        x[0] = cadd(a, s); x[1] = cadd(m, t); x[2] = csub(m, t);
+    } else if constexpr (R == 4) {
+        const cplx<T> t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]), t2 = cadd(x[1], x[3]);
+        const cplx<T> d = csub(x[1], x[3]);
+        const cplx<T> t3 = SIGN > 0 ? cplx<T>{-d.y, d.x} : cplx<T>{d.y, -d.x};
+        x[0] = cadd(t0, t2); x[2] = csub(t0, t2); x[1] = cadd(t1, t3); x[3] = csub(t1, t3);
+    } else {
+        constexpr int R1 = R % 4 == 0 ? 4 : (R % 2 == 0 ? 2 : 3), R2 = R / R1;
+        cplx<T> t[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            cplx<T> a[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) a[n1] = x[n2 + R2 * n1];
+            dft<T, R1, SIGN>(a);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) t[n2 + R2 * k1] = rot<T, SIGN>(a[k1], n2 * k1, R);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            cplx<T> b[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) b[n2] = t[n2 + R2 * k1];
+            dft<T, R2, SIGN>(b);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) x[k1 + R1 * k2] = b[k2];
+        }
+    }
+}
+
+constexpr int hibit(int r) { int h = 1; while (2 * h <= r) h *= 2; return h; }
+constexpr int log2c(int r) { int l = 0; while ((1 << l) < r) ++l; return l; }
+
+// a[r] *= w^r (r < R), w = exp(SIGN 2 pi i m / TW_PERIOD), m (R - 1) < TW_PERIOD
+template <class T, int R, int SIGN> __device__ __forceinline__ void twiddle_apply(cplx<T>* a, int m, const TwLDS<T>& tw) {
+    constexpr int NB = log2c(R);   // lookups at 2^b, b < NB (2^b <= R - 1)
+    cplx<T> w[R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) w[1 << b] = twiddle<T, SIGN>(tw, m << b);
+#pragma unroll
+    for (int r = 3; r < R; ++r)
+        if (r != hibit(r)) w[r] = cmul(w[r - hibit(r)], w[hibit(r)]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+}
+
+// ---- plans
+template <int... Rs> struct Plan {
+    static constexpr int P = sizeof...(Rs);
+    static constexpr int R[P] = {Rs...};
+    static constexpr int L = (1 * ... * Rs);
+};
+template <class Pl> constexpr int plan_ns(int I) { int n = 1; for (int i = 0; i < I; ++i) n *= Pl::R[i]; return n; }
+template <class Pl, class S> struct RevPlan;
+template <class Pl, int... I> struct RevPlan<Pl, std::integer_sequence<int, I...>> { using type = Plan<Pl::R[Pl::P - 1 - I]...>; };
+template <class Pl> using Rev = typename RevPlan<Pl, std::make_integer_sequence<int, Pl::P>>::type;
+
+// forward plans: a first radix that takes up the odd factor, then radix-16 passes (the last one is the
+// register link: 16 elements per thread, L / 16 threads per transform)
+template <class T, int L> struct FwdPlan;
+#define MG_PLAN(L_, ...) \
+    template <> struct FwdPlan<float, L_> { using type = Plan<__VA_ARGS__>; }; \
+    template <> struct FwdPlan<double, L_> { using type = Plan<__VA_ARGS__>; };
+MG_PLAN(64, 4, 16)
+MG_PLAN(96, 6, 16)
+MG_PLAN(128, 8, 16)
+MG_PLAN(192, 12, 16)
+MG_PLAN(256, 16, 16)
+MG_PLAN(384, 24, 16)
+MG_PLAN(512, 32, 16)
+MG_PLAN(768, 3, 16, 16)
+MG_PLAN(1024, 4, 16, 16)
+MG_PLAN(1536, 6, 16, 16)
+MG_PLAN(2048, 8, 16, 16)
+MG_PLAN(3072, 12, 16, 16)
+MG_PLAN(4096, 16, 16, 16)
+MG_PLAN(6144, 24, 16, 16)
+MG_PLAN(8192, 32, 16, 16)
+#undef MG_PLAN
+
+template <int L> constexpr int fft_th() { return L / 16; }                                       // threads per transform
+template <int L> constexpr int fft_batch() { return L / 16 >= 256 ? 1 : 256 / (L / 16); }         // transforms per workgroup
+template <int L> constexpr int fft_block() { return fft_th<L>() * fft_batch<L>(); }
+
+// One Stockham pass (see the top of the file). ld(e, q, r) gives input element e for slot (q, r); mid() runs
+// between the loads and the stores (a barrier when both touch the same LDS); st(e, v, q, r) takes output e.
+template <class T, int L, int R, int Ns, int TH, int SIGN, class Ld, class Mid, class St>
+__device__ __forceinline__ void stockham(int t, const TwLDS<T>& tw, Ld&& ld, Mid&& mid, St&& st) {
+    constexpr int NI = L / R, Q = (NI + TH - 1) / TH;
+    constexpr bool exact = NI % TH == 0;
+    cplx<T> v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + q * TH;
+        if (!exact && j >= NI) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * NI, q, r);
+        if constexpr (Ns > 1) twiddle_apply<T, R, SIGN>(v[q], (j % Ns) * (TW_PERIOD / (Ns * R)), tw);
+        dft<T, R, SIGN>(v[q]);
+    }
+    mid();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + q * TH;
+        if (!exact && j >= NI) continue;
+        const int k = Ns > 1 ? j % Ns : 0;
+        const int o = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) st(o + r * Ns, v[q][r], q, r);
+    }
+}
+
+// Whole transform of plan Pl by the TH threads of one transform (t = index among them), LDS buffer at element
+// offset `base` of `lds`. Passes that write LDS are bracketed by barriers, so consecutive transforms on one
+// buffer need no extra synchronisation; LAST_BAR puts a barrier before the last pass's stores too (for a
+// functor that writes the same LDS). Unnormalised; SIGN -1 forward, +1 inverse.
+template <class T, class Pl, int SIGN, int TH, bool LAST_BAR, int I = 0, class Ld, class St>
+__device__ __forceinline__ void fft_run(int t, cplx<T>* lds, int base, const TwLDS<T>& tw, Ld&& ld, St&& st) {
+    constexpr int P = Pl::P, L = Pl::L, R = Pl::R[I], Ns = plan_ns<Pl>(I);
+    auto lds_ld = [&](int e, int, int) { return lds[swz<T>(base + e)]; };
+    auto lds_st = [&](int e, cplx<T> v, int, int) { lds[swz<T>(base + e)] = v; };
+    auto bar = [] { __syncthreads(); };
+    auto nop = [] {};
+    if constexpr (P == 1) {
+        if constexpr (LAST_BAR) stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, bar, st);
+        else stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, nop, st);
+    } else if constexpr (I == 0) {
+        stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, bar, lds_st);
+        __syncthreads();
+        fft_run<T, Pl, SIGN, TH, LAST_BAR, 1>(t, lds, base, tw, ld, st);
+    } else if constexpr (I + 1 < P) {
+        stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, bar, lds_st);
+        __syncthreads();
+        fft_run<T, Pl, SIGN, TH, LAST_BAR, I + 1>(t, lds, base, tw, ld, st);
+    } else {
+        if constexpr (LAST_BAR) stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, bar, st);
+        else stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, nop, st);
+    }
+}
+
+}  // namespace mg

@@ -1,0 +1,442 @@
+// FFT convolution of the charge grids with the Cauchy kernels (src/fftconv.cpp:124-198), and the kernel
+// spectra (magnet/tsne.py:48-71), on gfx950 with the compile-time plans of fft_core.hpp (no rocFFT).
+// Included by one translation unit per precision (k_fft32.hip, k_fft64.hip).
+//   rows forward : real rows of length M (N live values, zero padding) -> M/2 + 1 complex values, through
+//                  one complex transform of length L = M/2 (even / odd samples as real / imaginary parts);
+//                  stored by rows, (channel, gx, ky); a tiled transpose then lays them out (channel, ky, gx);
+//   columns      : per ky, the three channel columns are transformed (rows >= N are zero), multiplied by
+//                  the kernel spectra in registers and transformed back; only the N live rows are stored;
+//   rows inverse : (after a transpose back to rows) Hermitian half rows -> real rows of length M through one
+//                  complex transform of length M/2, scaled by 1/M^2; planar potentials (4, N, N).
+// Two product sets. The reference's, used by the FP64 path: phi0 = K1*W0, phi1..3 = K2*W0..2 with the
+// charges (1, x, y). The centred one, used by the FP32 path, where the charges are (1, x - x_node,
+// y - y_node): phi0 = K1*W0, phi1 = K2*W0, phi2 = (x K2)*W0 - K2*W1', phi3 = (y K2)*W0 - K2*W2'. The odd
+// kernels x K2, y K2 have imaginary spectra (i Tx, i Ty). Both sets give the same forces in exact arithmetic;
+// the centred one avoids the cancellation of pot1 * x - pot2 at large |x|.
+// Kernel spectra are stored (Mh, Mh), [ky][kx], kx <= M/2 (they are even in kx, or odd for Tx).
+#pragma once
+#include "common.hpp"
+#include "fft_core.hpp"
+#include <algorithm>
+
+namespace mg {
+namespace {
+
+template <class T> __device__ __forceinline__ cplx<T> czero() { return {T(0), T(0)}; }
+
+// X_k (0 <= k <= L) of a real sequence of length 2L from Z = DFT_L(even + i odd) held in LDS:
+// E = (Z_k + conj Z_{L-k}) / 2, O = -i (Z_k - conj Z_{L-k}) / 2, X_k = E + exp(-2 pi i k / 2L) O
+template <class T, int L> __device__ inline cplx<T> r2c_post(const cplx<T>* lds, int base, int k, const TwLDS<T>& tw) {
+    const cplx<T> zk = lds[swz<T>(base + (k == L ? 0 : k))];
+    const cplx<T> zc = cconj(lds[swz<T>(base + (k == 0 ? 0 : L - k))]);
+    const cplx<T> e = {(zk.x + zc.x) * T(0.5), (zk.y + zc.y) * T(0.5)};
+    const cplx<T> d = csub(zk, zc);
+    const cplx<T> o = {d.y * T(0.5), -d.x * T(0.5)};
+    return cadd(e, cmul(twiddle<T, -1>(tw, k * (TW_PERIOD / (2 * L))), o));
+}
+
+// ---- rows forward: blockIdx.y = channel, fft_batch rows per workgroup
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict__ W, int N, cplx<T>* __restrict__ out,
+                                                             TwPtr<T> twp) {
+    using Pl = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, R0 = Pl::R[0];
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp);
+    __syncthreads();
+    const int t = threadIdx.x % TH, b = threadIdx.x / TH;
+    const int c = blockIdx.y, gx = blockIdx.x * B + b;
+    const bool live = gx < N, even = (N & 1) == 0;
+    const T* src = W + ((size_t)c * N + (live ? gx : 0)) * N;
+    auto ld = [&](int e, int, int r) -> cplx<T> {
+        cplx<T> z = czero<T>();
+        if (2 * r >= R0) return z;   // e >= L/2: samples 2e >= L >= N are padding
+        const int n0 = 2 * e;
+        if (live && n0 < N) {
+            if (even) { const V2 p = reinterpret_cast<const V2*>(src)[e]; z.x = p.x; z.y = p.y; }
+            else { z.x = src[n0]; if (n0 + 1 < N) z.y = src[n0 + 1]; }
+        }
+        return z;
+    };
+    auto st = [&](int e, cplx<T> v, int, int) { lds[swz<T>(b * L + e)] = v; };
+    fft_run<T, Pl, -1, TH, true>(t, lds, b * L, tw, ld, st);
+    __syncthreads();
+    if (!live) return;
+    cplx<T>* dst = out + ((size_t)c * N + gx) * Mh;
+    for (int k = t; k < Mh; k += TH) dst[k] = r2c_post<T, L>(lds, b * L, k, tw);
+}
+
+// ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C) {
+    __shared__ cplx<T> tile[32][33];
+    const size_t off = (size_t)blockIdx.z * R * C;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < R && c < C) tile[i][tx] = src[off + (size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (r < R && c < C) dst[off + (size_t)c * R + r] = tile[tx][i];
+    }
+}
+
+// kernel table value at column frequency kx (stored for kx <= M/2; `odd` flips the sign above M/2)
+template <class T, int M> __device__ __forceinline__ T ktab(const T* __restrict__ row, int kx, bool odd) {
+    if (kx <= M / 2) return row[kx];
+    const T v = row[M - kx];
+    return odd ? -v : v;
+}
+
+// ---- columns: one column ky per fft_th threads. Channel 0 is transformed once and its spectrum kept in
+// registers (slot (q, r) of thread t holds frequency t + q TH + r M / 16); each output is a product formed
+// in those registers and transformed back by the reversed plan, with no LDS round trip in between.
+template <class T, int L, bool CEN>
+__global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
+                                                            const T* __restrict__ K1, const T* __restrict__ K2,
+                                                            const T* __restrict__ TX, const T* __restrict__ TY,
+                                                            cplx<T>* __restrict__ Ph, TwPtr<T> twp) {
+    using F = typename FwdPlan<T, L>::type;
+    using I = Rev<F>;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L / 2 + 1, R0 = F::R[0], RL = F::R[F::P - 1];
+    constexpr int QL = L / RL / TH;
+    static_assert((L / RL) % TH == 0, "the register link needs the last pass to cover every thread evenly");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp);
+    __syncthreads();
+    const int t = threadIdx.x % TH, b = threadIdx.x / TH;
+    const int ky = blockIdx.x * B + b;
+    const bool live = ky < Mh;
+    const int kyc = live ? ky : 0;
+    cplx<T> s0[QL][RL], v[QL][RL];
+    auto fwd = [&](int ch, cplx<T> (&dst)[QL][RL]) {
+        const cplx<T>* col = Wh + ((size_t)ch * Mh + kyc) * N;
+        auto ld = [&](int e, int, int r) -> cplx<T> {
+            if (2 * r >= R0 || e >= N) return czero<T>();   // rows >= N (all of r >= R0/2) are padding
+            return col[e];
+        };
+        auto st = [&](int, cplx<T> x, int q, int r) { dst[q][r] = x; };
+        fft_run<T, F, -1, TH, false>(t, lds, b * L, tw, ld, st);
+    };
+    auto inv = [&](int o, cplx<T> (&src)[QL][RL]) {
+        cplx<T>* dst = Ph + ((size_t)o * Mh + kyc) * N;
+        auto ld = [&](int, int q, int r) { return src[q][r]; };
+        auto st = [&](int e, cplx<T> x, int, int r) {
+            if (2 * r < R0 && e < N && live) dst[e] = x;   // only the N live rows
+        };
+        fft_run<T, I, +1, TH, false>(t, lds, b * L, tw, ld, st);
+    };
+    const T* k1 = K1 + (size_t)kyc * Mh;
+    const T* k2 = K2 + (size_t)kyc * Mh;
+    auto freq = [&](int q, int r) { return t + q * TH + r * (L / RL); };
+    fwd(0, s0);
+#pragma unroll
+    for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) { const T c = ktab<T, L>(k1, freq(q, r), false); v[q][r] = {s0[q][r].x * c, s0[q][r].y * c}; }
+    inv(0, v);
+#pragma unroll
+    for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) { const T c = ktab<T, L>(k2, freq(q, r), false); v[q][r] = {s0[q][r].x * c, s0[q][r].y * c}; }
+    inv(1, v);
+    for (int ch = 1; ch <= 2; ++ch) {
+        fwd(ch, v);
+        const T* tt = CEN ? (ch == 1 ? TX : TY) + (size_t)kyc * Mh : nullptr;
+#pragma unroll
+        for (int q = 0; q < QL; ++q)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const int k = freq(q, r);
+                const T c2 = ktab<T, L>(k2, k, false);
+                const cplx<T> s = v[q][r];
+                if (CEN) {   // i t s0 - K2 s, t odd in kx for x K2
+                    const T tk = ktab<T, L>(tt, k, ch == 1);
+                    v[q][r] = {-tk * s0[q][r].y - c2 * s.x, tk * s0[q][r].x - c2 * s.y};
+                } else {
+                    v[q][r] = {s.x * c2, s.y * c2};
+                }
+            }
+        inv(ch + 1, v);
+    }
+}
+
+// ---- rows inverse: blockIdx.y = channel, fft_batch rows per workgroup; planar output (4, N, N)
+// Z_e = E + i O with E = X_e + conj X_{L-e}, O = (X_e - conj X_{L-e}) exp(+2 pi i e / 2L)
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __restrict__ X, int N, T* __restrict__ phi,
+                                                             T scale, TwPtr<T> twp) {
+    using Pl = Rev<typename FwdPlan<T, L>::type>;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1];
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp);
+    __syncthreads();
+    const int t = threadIdx.x % TH, b = threadIdx.x / TH;
+    const int c = blockIdx.y, gx = blockIdx.x * B + b;
+    const bool live = gx < N, even = (N & 1) == 0;
+    const cplx<T>* src = X + ((size_t)c * N + (live ? gx : 0)) * Mh;
+    auto ld = [&](int e, int, int) -> cplx<T> {
+        const cplx<T> xk = src[e], xc = cconj(src[L - e]);
+        const cplx<T> ev = cadd(xk, xc);
+        const cplx<T> od = cmul(csub(xk, xc), twiddle<T, +1>(tw, e * (TW_PERIOD / (2 * L))));
+        return {ev.x - od.y, ev.y + od.x};
+    };
+    T* dst = phi + ((size_t)c * N + (live ? gx : 0)) * N;
+    auto st = [&](int e, cplx<T> v, int, int r) {
+        if (2 * r >= RL) return;   // e >= L/2: samples 2e >= L >= N are padding
+        const int n0 = 2 * e;
+        if (!live || n0 >= N) return;
+        if (even) { V2 p; p.x = v.x * scale; p.y = v.y * scale; reinterpret_cast<V2*>(dst)[e] = p; }
+        else { dst[n0] = v.x * scale; if (n0 + 1 < N) dst[n0 + 1] = v.y * scale; }
+    };
+    fft_run<T, Pl, +1, TH, false>(t, lds, b * L, tw, ld, st);
+}
+
+// ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
+// in each offset except x K2 (odd in a) and y K2 (odd in b). Values in double, rounded to float first when
+// round_kernel (the reference builds its tables in float32), transformed in T.
+__device__ inline void kernel_values(int a, int b, int M, double delta, double dof, const double* table, bool round_kernel,
+                                     double& k1, double& k2, double& ky2) {
+    const double ox = (a <= M / 2 ? a : a - M) * delta, oy = (b <= M / 2 ? b : b - M) * delta;
+    if (table) {
+        k1 = table[(size_t)a * M + b];
+        k2 = table[(size_t)M * M + (size_t)a * M + b];
+    } else if (dof == 1.0) {
+        const double inv = 1.0 / (1.0 + ox * ox + oy * oy);
+        k1 = inv; k2 = inv * inv;
+    } else {
+        const double base = 1.0 + (ox * ox + oy * oy) / dof;
+        k1 = pow(base, -dof); k2 = pow(base, -dof - 1.0);
+    }
+    if (round_kernel) { k1 = (double)(float)k1; k2 = (double)(float)k2; }
+    // the odd tables drop the Nyquist offset M/2 (no negative twin on the torus), which keeps them exactly
+    // odd -- so their spectra are imaginary -- and is never used: the live offsets stay below N <= M/2
+    ky2 = b == M / 2 ? 0.0 : oy * k2;
+}
+
+// Stage 1, per row a <= M/2, along b. FFT(K1_a + i K2_a): both rows are real and even, so the real and
+// imaginary parts of the result are their (real) spectra T1, T2. FFT(yK2_a) = i S (odd row). Output
+// P0[a][ky] = (T1, T2), P1[a][ky] = (S, x_a T2) -- the row transform of x K2 is x_a times that of K2.
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double dof, const double* __restrict__ table,
+                                                          int round_kernel, cplx<T>* __restrict__ P0, cplx<T>* __restrict__ P1,
+                                                          TwPtr<T> twp) {
+    using F = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
+    constexpr int QL = L / RL / TH;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp);
+    __syncthreads();
+    const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
+    const int a = blockIdx.x * B + bb;
+    const bool live = a < Mh;
+    const int ac = live ? a : 0;
+    cplx<T> za[QL][RL], zb[QL][RL];
+    auto put = [](cplx<T> (&d)[QL][RL]) { return [&d](int, cplx<T> x, int q, int r) { d[q][r] = x; }; };
+    {
+        auto ld = [&](int e, int, int) -> cplx<T> {
+            double k1, k2, ky;
+            kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
+            return {(T)k1, (T)k2};
+        };
+        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, put(za));
+    }
+    if constexpr (NT == 4) {
+        auto ld = [&](int e, int, int) -> cplx<T> {
+            double k1, k2, ky;
+            kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
+            return {(T)ky, T(0)};
+        };
+        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, put(zb));
+    }
+    if (!live) return;
+    const T xa = a == M / 2 ? T(0) : (T)(a * delta);
+#pragma unroll
+    for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const int k = t + q * TH + r * (L / RL);
+            if (k >= Mh) continue;
+            P0[(size_t)a * Mh + k] = za[q][r];
+            if constexpr (NT == 4) P1[(size_t)a * Mh + k] = {zb[q][r].y, xa * za[q][r].y};
+        }
+}
+
+// Stage 2, per ky, along a (inputs transposed to [ky][a], a <= M/2, mirrored: even, and odd for x K2).
+// FFT(T1 + i T2) gives K1 and K2 as real and imaginary parts; FFT(S + i X) = S^ - U with S^ even and U odd
+// in kx (FFT(X) = i U), split through Z(kx) and Z(M - kx): Ty = S^, Tx = U.
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restrict__ Q0, const cplx<T>* __restrict__ Q1,
+                                                          T* __restrict__ K1, T* __restrict__ K2, T* __restrict__ TX,
+                                                          T* __restrict__ TY, TwPtr<T> twp) {
+    using F = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
+    constexpr int QL = L / RL / TH;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TwLDS<T> tw;
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    fft_stage_twiddles(tw, twp);
+    __syncthreads();
+    const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
+    const int ky = blockIdx.x * B + bb;
+    const bool live = ky < Mh;
+    const size_t row = (size_t)(live ? ky : 0) * Mh;
+    cplx<T> z[QL][RL];
+    {
+        auto ld = [&](int e, int, int) -> cplx<T> { return Q0[row + (e <= M / 2 ? e : M - e)]; };
+        auto st = [&](int, cplx<T> x, int q, int r) { z[q][r] = x; };
+        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, st);
+    }
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < QL; ++q)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const int k = t + q * TH + r * (L / RL);
+                if (k >= Mh) continue;
+                K1[row + k] = z[q][r].x;
+                K2[row + k] = z[q][r].y;
+            }
+    }
+    if constexpr (NT == 4) {
+        auto ld = [&](int e, int, int) -> cplx<T> {
+            if (e <= M / 2) return Q1[row + e];
+            const cplx<T> p = Q1[row + (M - e)];
+            return {p.x, -p.y};
+        };
+        auto st = [&](int e, cplx<T> x, int, int) { lds[swz<T>(bb * L + e)] = x; };
+        fft_run<T, F, -1, TH, true>(t, lds, bb * L, tw, ld, st);
+        __syncthreads();
+        if (!live) return;
+        for (int k = t; k < Mh; k += TH) {
+            const T zk = lds[swz<T>(bb * L + k)].x, zc = lds[swz<T>(bb * L + (k == 0 ? 0 : M - k))].x;
+            TY[row + k] = T(0.5) * (zk + zc);
+            TX[row + k] = T(-0.5) * (zk - zc);
+        }
+    }
+}
+
+// potentials planar (4, N, N) -> interleaved (N, N, 4)
+template <class T>
+__global__ void k_interleave4(const T* __restrict__ src, T* __restrict__ dst, i64 NN) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < NN; i += (i64)gridDim.x * blockDim.x)
+        for (int c = 0; c < 4; ++c) dst[4 * i + c] = src[c * NN + i];
+}
+
+template <class F> void set_lds(F* f, int bytes) {
+    if (bytes > 65536) MG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
+template <class T, int L> int lds_bytes() { return fft_batch<L>() * L * (int)sizeof(cplx<T>); }
+
+}  // namespace
+
+#define MG_ROW_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
+#define MG_COL_SIZES(X) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096) X(6144) X(8192)
+
+template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* out, hipStream_t st) {
+    require(N <= M / 2, "FFT size below 2N");
+    const TwPtr<T> tw = fft_twiddles<T>();
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_rows_fwd<T, L_>, by); \
+        hipLaunchKernelGGL((k_rows_fwd<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 3), dim3(fft_block<L_>()), by, st, W, N, out, tw); \
+        break; }
+    switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int batch, int R, int C, hipStream_t st) {
+    hipLaunchKernelGGL(k_transpose<T>, dim3((C + 31) / 32, (R + 31) / 32, batch), dim3(256), 0, st, src, dst, R, C);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
+                                       hipStream_t st) {
+    require(N <= M / 2, "FFT size below 2N");
+    require(centred == (sizeof(T) == 4), "the FP32 path uses the centred product set, the FP64 path the reference's");
+    constexpr bool CEN = sizeof(T) == 4;
+    const TwPtr<T> tw = fft_twiddles<T>();
+    const int Mh = M / 2 + 1;
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN>, by); \
+        hipLaunchKernelGGL((k_columns<T, L_, CEN>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
+                           Wh, N, K[0], K[1], K[2], K[3], Ph, tw); \
+        break; }
+    switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st) {
+    require(N <= M / 2, "FFT size below 2N");
+    const TwPtr<T> tw = fft_twiddles<T>();
+    const T scale = (T)(1.0 / ((double)M * (double)M));
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_rows_inv<T, L_>, by); \
+        hipLaunchKernelGGL((k_rows_inv<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 4), dim3(fft_block<L_>()), by, st, X, N, phi, scale, tw); \
+        break; }
+    switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
+                                       cplx<T>* tmp, T* const* K, hipStream_t st) {
+    require(ntab == 2 || ntab == 4, "ntab must be 2 or 4");
+    const TwPtr<T> tw = fft_twiddles<T>();
+    const int Mh = M / 2 + 1;
+    const size_t plane = (size_t)Mh * Mh;
+    cplx<T>* P0 = tmp;              // stage 1 output [a][ky]
+    cplx<T>* P1 = tmp + plane;
+    cplx<T>* Q0 = tmp + 2 * plane;  // transposed [ky][a]
+    cplx<T>* Q1 = tmp + 3 * plane;
+#define MG_S1(L_, NT) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_spec1<T, L_, NT>, by); \
+        hipLaunchKernelGGL((k_spec1<T, L_, NT>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
+                           delta, dof, table, (int)round_kernel, P0, P1, tw); \
+    } while (0)
+#define MG_S2(L_, NT) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_spec2<T, L_, NT>, by); \
+        hipLaunchKernelGGL((k_spec2<T, L_, NT>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
+                           Q0, Q1, K[0], K[1], K[2], K[3], tw); \
+    } while (0)
+#define MG_C(L_) case L_: \
+        if (ntab == 4) { MG_S1(L_, 4); launch_transpose<T>(P0, Q0, 2, Mh, Mh, st); MG_S2(L_, 4); } \
+        else { MG_S1(L_, 2); launch_transpose<T>(P0, Q0, 1, Mh, Mh, st); MG_S2(L_, 2); } \
+        break;
+    switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+#undef MG_S1
+#undef MG_S2
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_phi_interleave(const T* planar, T* out, i64 NN, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<i64>(4096, std::max<i64>(1, (NN + 255) / 256));
+    hipLaunchKernelGGL(k_interleave4<T>, dim3(blocks), dim3(256), 0, st, planar, out, NN);
+    MG_HIP(hipGetLastError());
+}
+
+#define MG_FFT_INST(T) \
+    template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
+    template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
+    template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t); \
+    template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
+    template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t); \
+    template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t);
+
+}  // namespace mg

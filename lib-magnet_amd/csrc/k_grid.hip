@@ -360,8 +360,12 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
                                                 double* __restrict__ sumq_hist) {
     __shared__ double shd[TPB];
     typedef typename vec2<T>::type V2;
-    typedef typename vec4<T>::type V4;
-    const V4* phi4 = reinterpret_cast<const V4*>(phi);
+    struct V4 { T x, y, z, w; };
+    const size_t NN = (size_t)g.N * g.N;   // planar potentials (4, N, N)
+    const T* __restrict__ ph0 = phi;
+    const T* __restrict__ ph1 = phi + NN;
+    const T* __restrict__ ph2 = phi + 2 * NN;
+    const T* __restrict__ ph3 = phi + 3 * NN;
     double z = 0;
     for (i64 p = (i64)blockIdx.x * TPB + threadIdx.x; p < n; p += (i64)gridDim.x * TPB) {
         const int key = keys[p], idx = vals[p];
@@ -381,7 +385,8 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
 #pragma unroll
                 for (int b = 0; b < NI; ++b) {
                     const double w = (double)(wx[a] * wy[b]);
-                    const V4 c = phi4[row + b];
+                    const size_t o = row + b;
+                    const V4 c = {ph0[o], ph1[o], ph2[o], ph3[o]};
                     const double ey = node_offset<NI>(fyu, g.width_d, b);
                     p0 += w * (double)c.x;
                     fx += w * ((double)c.y * ex + (double)c.z);
@@ -398,7 +403,8 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
 #pragma unroll
                 for (int b = 0; b < NI; ++b) {
                     const T w = wx[a] * wy[b];
-                    const V4 c = phi4[row + b];
+                    const size_t o = row + b;
+                    const V4 c = {ph0[o], ph1[o], ph2[o], ph3[o]};
                     p0 += w * c.x; p1 += w * c.y; p2 += w * c.z; p3 += w * c.w;
                 }
             }

@@ -44,7 +44,7 @@ def lib():
             "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32),
             "mg_synchronize": ([], i32), "mg_mem_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
             "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
-            "mg_profile_enable": ([i32], i32), "mg_set_lds_limit": ([i32], i32),
+            "mg_profile_enable": ([i32], i32), "mg_fft_size": ([i32], i32),
             "mg_profile_grid": ([C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
             "mg_profile_get": ([i32, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_double)], i32),
@@ -186,13 +186,14 @@ def profile_read():
     return out
 
 
+def fft_size(N):
+    """Padded FFT size the device uses for a live grid side N (0: unsupported)."""
+    return int(lib().mg_fft_size(int(N)))
+
+
 def profile_grid():
     """(mean N, max N, max M) of the profiled iterations."""
     a, b, c = C.c_double(), C.c_int(), C.c_int()
     check(lib().mg_profile_grid(C.byref(a), C.byref(b), C.byref(c)))
     return a.value, b.value, c.value
 
-
-def set_lds_limit(nbytes):
-    """Caps the LDS the FFT stages plan for (0 = device limit): selects the HBM-stash variants."""
-    check(lib().mg_set_lds_limit(int(nbytes)))

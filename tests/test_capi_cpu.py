@@ -60,3 +60,13 @@ def test_perplexity_calibration_hits_the_entropy():
     X = rng.normal(0, 1, (400, 5)).astype(np.float32)
     P = perplexity_affinities(X, 10.0, device="cpu")
     assert abs(P.sum() - 1) < 1e-5 and abs(P - P.T).max() < 1e-8 and P.indices.dtype == np.int32
+
+
+def test_fft_size_choice():
+    """smallest 2^s or 3 * 2^s >= 2N, from 128 up to 8192 (no device needed)"""
+    from magnet import capi
+    sizes = [128 * 2 ** k for k in range(7)] + [192 * 2 ** k for k in range(6)]
+    for N in [1, 60, 64, 65, 96, 97, 500, 1020, 1536, 2049, 3000, 3072, 3073, 4096]:
+        want = min(m for m in sizes if m >= 2 * N)
+        assert capi.fft_size(N) == want, N
+    assert capi.fft_size(4097) == 0

@@ -223,31 +223,61 @@ def test_repulsion_fp64_clipped_grid_three_sort_passes():
     assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-10
 
 
-@pytest.mark.parametrize("cap", [0, 65536, 40000])
-def test_repulsion_column_variants(cap):
-    """every column-pass variant (paired transforms in two LDS buffers, or one at a time in one) and both
-    row-inverse variants against the oracle: the LDS cap forces the one-buffer variants at a grid the
-    oracle can check (N = 540, M = 2048; cap 40000: FP64 one buffer, rows inverse one channel at a time)."""
-    Y = layout(30_000, 13, 25.0, k=12)
-    xmin, ymin, width, nb = O.grid(Y)
-    assert O.fft_size(nb * 3) == 2048
-    try:
-        capi.set_lds_limit(cap)
-        s, F = rep_gpu(Y)
-        Yf = Y.astype(np.float32)
-        s32, F32 = rep_gpu(Yf)
-    finally:
-        capi.set_lds_limit(0)
-    so, Fo = O.repulsion(Y, xmin, ymin, width, nb, 3)
-    assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-11
-    g32 = [float(np.float32(v)) for v in O.grid(Yf)[:3]]
-    so32, Fo32 = O.repulsion(Yf.astype(np.float64), *g32, O.grid(Yf)[3], 3)
-    assert abs(s32 - so32) / so32 < 2e-6 and rel(F32, Fo32) < 2e-6
+# one grid per compiled transform size M (2^s and 3 * 2^s, the smallest >= 2N): (n_interp, n_boxes)
+FFT_SIZE_GRIDS = [(128, 3, 20), (192, 3, 32), (256, 3, 40), (384, 3, 64), (512, 3, 84), (768, 3, 128),
+                  (1024, 3, 168), (1536, 3, 256), (2048, 3, 340), (3072, 3, 512), (4096, 3, 680),
+                  (6144, 3, 1000)]
+
+
+@pytest.mark.parametrize("M,ni,nb", FFT_SIZE_GRIDS)
+def test_repulsion_every_fft_size(M, ni, nb):
+    """each plan (first radix 3 .. 32, then radix 16) in all five FFT kernels, FP64 and FP32, against the
+    oracle on a grid of that size: points spread over the whole grid, width 0.7."""
+    N = ni * nb
+    assert capi.fft_size(N) == M
+    rng = np.random.default_rng(M)
+    span = nb * 0.7
+    Y = rng.uniform(0.05, span - 0.05, (4000, 2))
+    Y[:2000] = rng.normal(span / 2, span / 10, (2000, 2)).clip(0.05, span - 0.05)
+    xmin = ymin = 0.0
+    out = {}
+    for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
+        conv = capi.FFTConv(prec, 2 * N, ni)
+        conv.set_tsne(0.7)
+        s, F = conv.compute(dev(Y.astype(dt)), xmin, ymin, 0.7, nb)
+        out[dt] = (s, host(F))
+    so, Fo = O.repulsion(Y, xmin, ymin, 0.7, nb, ni)
+    s, F = out[np.float64]
+    assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-11, (abs(s - so) / so, rel(F, Fo))
+    s, F = out[np.float32]
+    so32, Fo32 = O.repulsion(Y.astype(np.float32).astype(np.float64), xmin, ymin, float(np.float32(0.7)), nb, ni)
+    assert abs(s - so32) / so32 < 2e-6 and rel(F, Fo32) < 2e-6, (abs(s - so32) / so32, rel(F, Fo32))
+
+
+def test_repulsion_largest_fft_size():
+    """M = 8192 (n_interp 4, 1000 boxes): FP64 against the exact O(n^2) sum at the interpolation error level
+    of tests/test_magnet.py:221-231, and FP32 against FP64 on the same float32 layout."""
+    rng = np.random.default_rng(15)
+    nb, ni = 1000, 4
+    assert capi.fft_size(nb * ni) == 8192
+    span = nb * 0.7
+    Yf = rng.uniform(1.0, span - 1.0, (3000, 2)).astype(np.float32)
+    out = {}
+    for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
+        conv = capi.FFTConv(prec, 2 * nb * ni, ni)
+        conv.set_tsne(0.7)
+        s, F = conv.compute(dev(Yf.astype(dt)), 0.0, 0.0, 0.7, nb)
+        out[dt] = (s, host(F))
+    Z, Fe = O.exact_repulsion(Yf.astype(np.float64))
+    s64, F64 = out[np.float64]
+    assert abs(s64 - Z) / Z < 1e-3 and rel(F64, Fe) < 2e-2
+    s32, F32 = out[np.float32]
+    assert abs(s32 - s64) / s64 < 2e-6 and rel(F32, F64) < 5e-6
 
 
 def test_repulsion_largest_grid_fp32_against_fp64():
-    """N = 3000 (n_boxes clipped at 1000, n_interp 3), M = 8192: FP32 (centred, two LDS columns and an
-    HBM stash) against FP64 (reference products, one LDS column and an HBM stash) on one float32 layout."""
+    """N = 3000 (n_boxes clipped at 1000, n_interp 3), M = 6144: FP32 (centred products) against FP64
+    (reference products) on one float32 layout."""
     rng = np.random.default_rng(14)
     c = rng.uniform(-500, 500, (60, 2))
     Yf = (c[rng.integers(0, 60, 70_000)] + rng.normal(0, 3.0, (70_000, 2))).astype(np.float32)
