@@ -60,7 +60,10 @@ def make_data(name, seed=0):
     return X.astype(np.float32), lab
 
 
-def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False):
+def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False, checkpoints=()):
+    """Whole-loop run of `steps` iterations from Y0 on resident inputs. With profile=True the per-stage HIP
+    event timers run; checkpoints: iterations at which (Y, U, G) device copies are kept (the run is then split
+    at those iterations, which the resumable loop reproduces exactly)."""
     import torch
     from magnet import capi
     dt = np.float32 if prec == "fp32" else np.float64
@@ -72,19 +75,55 @@ def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False):
         capi.optimize(ip, ix, data, Yw, Uw, Gw, 0, min(warmup, n_iter), **kw)
     Y, U, G = dev(Y0.astype(dt)), dev(np.zeros_like(Y0, dt)), dev(np.ones_like(Y0, dt))
     capi.profile(profile)
+    snaps = {}
+    cuts = sorted(t for t in set(checkpoints) if 0 <= t <= steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    capi.optimize(ip, ix, data, Y, U, G, 0, steps, **kw)
+    it = 0
+    for c in cuts + [steps]:
+        if c > it:
+            capi.optimize(ip, ix, data, Y, U, G, it, c, **kw)
+            it = c
+        if c in cuts:
+            snaps[c] = (Y.clone(), U.clone(), G.clone())
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     stages = capi.profile_read() if profile else None
     if profile:
         stages["_grid"] = capi.profile_grid()
     capi.profile(False)
-    return sec, stages, (Y, U, G, ip, ix, data)
+    return sec, stages, (Y, U, G, ip, ix, data), snaps
 
 
-def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads):
+def matched_precision(P, snaps, n_iter, early, k):
+    """FP32 and FP64 from the same states: k iterations of each from every checkpoint state (FP32 states,
+    widened to float64 for the FP64 run); returns per-checkpoint (t, ms32, ms64)."""
+    import torch
+    from magnet import capi
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    csr = {np.float32: None, np.float64: None}
+    for dt in csr:
+        csr[dt] = (dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32)), dev(P.data.astype(dt)))
+    rows = []
+    for t, (Y, U, G) in sorted(snaps.items()):
+        t1 = min(t + k, n_iter)
+        if t1 <= t:
+            continue
+        ms = {}
+        for dt, tdt in ((np.float32, torch.float32), (np.float64, torch.float64)):
+            Yc, Uc, Gc = Y.to(tdt).clone(), U.to(tdt).clone(), G.to(tdt).clone()
+            ip, ix, data = csr[dt]
+            capi.optimize(ip, ix, data, Yc.clone(), Uc.clone(), Gc.clone(), t, min(t + 2, t1), n_iter=n_iter, early_iter=early)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            capi.optimize(ip, ix, data, Yc, Uc, Gc, t, t1, n_iter=n_iter, early_iter=early)
+            torch.cuda.synchronize()
+            ms[dt] = 1e3 * (time.perf_counter() - a) / (t1 - t)
+        rows.append((t, ms[np.float32], ms[np.float64]))
+    return rows
+
+
+def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads, points):
     """The reference's compiled module on this host: per-iteration time at six points of the schedule
     (GPU FP32 states), 2 warm-up + 5 timed reference iterations at each; it/s from the mean."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -115,7 +154,6 @@ def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads):
 
     from magnet.tsne import grid_params
     per_it = []
-    points = [int(round(f * (n_iter - 1))) for f in (0.0, 0.2, 0.4, 0.6, 0.8, 1.0)]
     for t in points:
         Y, U, G = gpu_state_fn(t)
         times = []
@@ -137,6 +175,38 @@ def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads):
                       f"it/s = 1 / mean of the six medians ({', '.join(f'{1e3 * v:.1f}' for v in per_it)} ms)"}
 
 
+# single-kernel stages: the roofline's dominant kernel is the one of these with the largest total time
+SINGLE_KERNEL = {"columns": "k_columns", "attraction": "k_attr2", "rows_forward": "k_rows_fwd",
+                 "rows_inverse": "k_rows_inv", "gather": "k_gather"}
+
+
+def traffic_of(workload, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the reading of a plain copy of known size), from profiles/traffic_<workload>.json; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(kernel)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of `mib` MiB (the PMC scale reference): prints its size and time."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes": n * 4, "seconds": dt, "GBs": round(2 * n * 4 / dt / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +215,7 @@ def main():
     ap.add_argument("--workload", default="mnist70k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB", help="only run a plain copy of MIB MiB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the t-SNE pass of one embedding runs on a single MI355X "
@@ -155,6 +226,9 @@ def main():
     from magnet.affinity import perplexity_affinities
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     capi.lib()
     capi.use_torch_stream()
     W = WORKLOADS[args.workload]
@@ -167,17 +241,22 @@ def main():
     Y0 = np.random.default_rng(1).normal(0, 1e-4, (P.shape[0], 2))
     log(f"bench: {W['desc']}: n={P.shape[0]} nnz={P.nnz} (P built in {time.time() - t0:.1f}s)")
 
-    sec32, _, _ = run_gpu(P, Y0, "fp32", steps, args.warmup, W["n_iter"], W["early"])
+    sec32, _, _, _ = run_gpu(P, Y0, "fp32", steps, args.warmup, W["n_iter"], W["early"])
     log(f"bench: fp32 {steps} iterations in {sec32:.3f}s -> {steps / sec32:.1f} it/s")
-    _, stages, state = run_gpu(P, Y0, "fp32", steps, 0, W["n_iter"], W["early"], profile=True)
+    points = [int(round(f * (steps - 1))) for f in (0.0, 0.2, 0.4, 0.6, 0.8, 1.0)]
+    K_MATCH = 25
+    mpoints = [t for t in points if t + K_MATCH <= steps] + [max(0, steps - K_MATCH)]
+    _, stages, state, snaps = run_gpu(P, Y0, "fp32", steps, 0, W["n_iter"], W["early"], profile=True,
+                                      checkpoints=points + mpoints)
     gridstat = stages.pop("_grid")
     stage_rows = {k: dict(ms_total=round(v[0], 3), launches=v[1], avg_us=round(1e3 * v[0] / max(v[1], 1), 2),
                           GBs=round(v[2] / max(v[0], 1e-9) / 1e6, 1)) for k, v in stages.items() if v[1] > 0}
-    dom = max(stages.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dcnt, dbytes) = dom
+    dname, (dms, dcnt, dbytes) = max(((k, v) for k, v in stages.items() if k in SINGLE_KERNEL and v[1] > 0),
+                                     key=lambda kv: kv[1][0])
     achieved = dbytes / (dms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "hbm", "kernel": SINGLE_KERNEL[dname], "stage": dname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_of(args.workload, SINGLE_KERNEL[dname]),
                 "avg_launch_us": round(1e3 * dms / dcnt, 2), "algo_bytes_per_launch": round(dbytes / dcnt)}
     tot_ms = sum(v[0] for v in stages.values())
     tot_bytes = sum(v[2] for v in stages.values())
@@ -186,25 +265,26 @@ def main():
              "grid": {"mean_N": round(gridstat[0], 1), "max_N": gridstat[1], "max_M": gridstat[2]},
              "algo_bytes_per_iter": round(tot_bytes / steps)}
     if not args.no_fp64:
-        sec64, _, _ = run_gpu(P, Y0, "fp64", steps, min(args.warmup, 10), W["n_iter"], W["early"])
-        log(f"bench: fp64 {steps} iterations in {sec64:.3f}s -> {steps / sec64:.1f} it/s")
-        extra["fp64"] = {"value": round(steps / sec64, 2), "unit": "iters/s", "ms_per_step": round(1e3 * sec64 / steps, 4)}
-        extra["fp32_over_fp64"] = round(sec64 / sec32, 3)
+        sec64, _, _, _ = run_gpu(P, Y0, "fp64", steps, min(args.warmup, 10), W["n_iter"], W["early"])
+        log(f"bench: fp64 {steps} iterations in {sec64:.3f}s -> {steps / sec64:.1f} it/s (its own trajectory)")
+        extra["fp64"] = {"value": round(steps / sec64, 2), "unit": "iters/s", "ms_per_step": round(1e3 * sec64 / steps, 4),
+                         "note": "the FP64 schedule from the same Y0; the trajectory is chaotic, so its grids differ"}
+        rows = matched_precision(P, {t: snaps[t] for t in mpoints}, W["n_iter"], W["early"], K_MATCH)
+        m32, m64 = sum(r[1] for r in rows), sum(r[2] for r in rows)
+        extra["fp32_over_fp64_same_states"] = round(m64 / m32, 3)
+        extra["matched_ms_per_iter"] = [[t, round(a, 4), round(b, 4)] for t, a, b in rows]
+        extra["fp32_over_fp64_own_trajectories"] = round(sec64 / sec32, 3)
+        log(f"bench: same states FP32 vs FP64 (t, ms32, ms64): {extra['matched_ms_per_iter']} -> {m64 / m32:.2f}x")
     cpu = None
     if not args.no_cpu:
-        Yd, Ud, Gd, ip, ix, data = state
         threads = min(16, os.cpu_count() or 1)
 
-        def gpu_state(t):   # FP32 GPU state at iteration t of the schedule (fresh run to t)
-            Y = torch.from_numpy(Y0.astype(np.float32)).cuda()
-            U, G = torch.zeros_like(Y), torch.ones_like(Y)
-            if t > 0:
-                capi.optimize(ip, ix, data, Y, U, G, 0, t, n_iter=W["n_iter"], early_iter=W["early"])
+        def gpu_state(t):   # FP32 GPU state at checkpoint t of the profiled pass
+            Y, U, G = snaps[t]
             torch.cuda.synchronize()
-            return (np.ascontiguousarray(Y.cpu().numpy()), np.ascontiguousarray(U.cpu().numpy()),
-                    np.ascontiguousarray(G.cpu().numpy()))
+            return tuple(np.ascontiguousarray(a.cpu().numpy()) for a in (Y, U, G))
         tc = time.time()
-        cpu = cpu_baseline(P, Y0, W["n_iter"], W["early"], gpu_state, threads)
+        cpu = cpu_baseline(P, Y0, W["n_iter"], W["early"], gpu_state, threads, points)
         log(f"bench: cpu baseline {cpu and cpu['value']} it/s in {time.time() - tc:.1f}s")
     line = {
         "metric": METRIC, "value": round(steps / sec32, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps,

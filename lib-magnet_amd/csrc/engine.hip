@@ -116,9 +116,10 @@ template <class T> GridT<T> make_grid(double xmin, double ymin, double width, in
 
 // ---- per-stage timing with HIP events on the working stream (the reference's MAGNET_DEBUG laps,
 // src/fftconv.cpp:69-70): enabled by mg_profile_enable or the MAGNET_DEBUG environment variable
-enum Stage { ST_ATTR = 0, ST_SORT, ST_SPREAD, ST_ROWS_FWD, ST_COLUMNS, ST_ROWS_INV, ST_GATHER, ST_UPDATE, ST_SPECTRA, NSTAGE };
-static const char* kStageNames[NSTAGE] = {"attraction", "box_sort", "spread", "rows_forward", "columns", "rows_inverse",
-                                          "gather", "update", "spectra"};
+enum Stage { ST_ATTR = 0, ST_SORT, ST_SPREAD, ST_ROWS_FWD, ST_TR_FWD, ST_COLUMNS, ST_TR_INV, ST_ROWS_INV, ST_GATHER,
+             ST_UPDATE, ST_SPECTRA, NSTAGE };
+static const char* kStageNames[NSTAGE] = {"attraction", "box_sort", "spread", "rows_forward", "transpose_forward", "columns",
+                                          "transpose_inverse", "rows_inverse", "gather", "update", "spectra"};
 struct Prof {
     bool on = false, debug = false;
     hipEvent_t ev[2 * NSTAGE] = {};
@@ -188,8 +189,9 @@ template <class T> struct Repulsion {
     }
 
     // spectra for the device size M (cached; the reference keeps up to 9, magnet/tsne.py:52-54)
-    Spec& spectra(int M, hipStream_t st) {
+    Spec& spectra(int M, hipStream_t st, bool* fresh = nullptr) {
         require(have_kernel, "kernels not set");
+        if (fresh) *fresh = false;
         const double delta = tsne_kernel ? kwidth / ni : -1.0;
         for (auto& s : cache)
             if (s.M == M && s.delta == delta && s.dof == kdof) { s.use = ++tick; return s; }
@@ -201,6 +203,7 @@ template <class T> struct Repulsion {
         cache.emplace_back();
         Spec& s = cache.back();
         s.M = M; s.delta = delta; s.dof = kdof; s.use = ++tick;
+        if (fresh) *fresh = true;
         const int Mh = M / 2 + 1;
         T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure((size_t)Mh * Mh * sizeof(T)));
@@ -273,8 +276,9 @@ template <class T> struct Repulsion {
         Prof& P = g_prof;
         const double ts = sizeof(T), NN = (double)g.N * g.N, cs = 2 * ts, Mh = M / 2 + 1, dn = (double)n;
         P.begin(ST_SPECTRA, st);
-        Spec& s = spectra(M, st);
-        P.end(ST_SPECTRA, st);
+        bool fresh = false;
+        Spec& s = spectra(M, st, &fresh);
+        P.end(ST_SPECTRA, st, fresh ? ntab * Mh * Mh * ts : 0.0);   // tables written when the grid changed
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
         sorted_buf = launch_box_sort<T>(Y, n, g, b, st);
@@ -286,14 +290,18 @@ template <class T> struct Repulsion {
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
         P.begin(ST_ROWS_FWD, st);
         launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
+        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);
+        P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
         launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
-        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);   // the transpose is not algorithmic traffic
+        P.end(ST_TR_FWD, st, 0);
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, st);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs);
-        P.begin(ST_ROWS_INV, st);
+        P.begin(ST_TR_INV, st);
         launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
+        P.end(ST_TR_INV, st, 0);
+        P.begin(ST_ROWS_INV, st);
         launch_rows_inverse<T>(b.Tr, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts);
         P.begin(ST_GATHER, st);
@@ -344,7 +352,17 @@ template <class T> struct Optimizer {
         MG_HIP(hipMemcpyAsync(hmb + 2, dptr<T>(bb_out), 4 * sizeof(T), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
         const T inv_dof = T(1) / (T)p->dof;
+        const double attr_bytes = 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T);
+        auto attraction = [&]() {
+            g_prof.begin(ST_ATTR, st);
+            launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
+            g_prof.end(ST_ATTR, st, attr_bytes);
+        };
+        if (!mb_ready) MG_HIP(hipEventCreateWithFlags(&mb_ready, hipEventDisableTiming));
         int stopped = 0;
+        // The attraction needs no grid: iteration it + 1's is queued right behind iteration it's update, so the
+        // device runs it while the host waits for the layout's bounding box and decides the next grid.
+        if (it0 < it1) attraction();
         for (int it = it0; it < it1; ++it) {
             const double bb[4] = {(double)hmb[2], (double)hmb[3], (double)hmb[4], (double)hmb[5]};
             require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
@@ -355,9 +373,6 @@ template <class T> struct Optimizer {
             const bool early = it < p->early_iter;
             const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
             const T mom = (T)(early ? p->momentum0 : p->momentum1);
-            g_prof.begin(ST_ATTR, st);
-            launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
-            g_prof.end(ST_ATTR, st, 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T));
             rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st);
             const T* dinv = dptr<T>(rep.inv_out);
             g_prof.begin(ST_UPDATE, st);
@@ -373,8 +388,11 @@ template <class T> struct Optimizer {
             }
             g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
             MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
-            MG_HIP(hipStreamSynchronize(st));
-            g_prof.collect(it);
+            MG_HIP(hipEventRecord(mb_ready, st));
+            const bool more = it + 1 < it1;
+            if (more) attraction();
+            MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
+            if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); g_prof.collect(it); }
             if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
                 if (cb(it, Y, n, user) != 0) { stopped = 1; it1 = it + 1; break; }
             }
@@ -386,6 +404,8 @@ template <class T> struct Optimizer {
         return stopped;
     }
     bool counters_zeroed = false;
+    hipEvent_t mb_ready = nullptr;
+    ~Optimizer() { if (mb_ready) (void)hipEventDestroy(mb_ready); }
 };
 
 }  // namespace mg

@@ -164,8 +164,10 @@ template <class Pl, class S> struct RevPlan;
 template <class Pl, int... I> struct RevPlan<Pl, std::integer_sequence<int, I...>> { using type = Plan<Pl::R[Pl::P - 1 - I]...>; };
 template <class Pl> using Rev = typename RevPlan<Pl, std::make_integer_sequence<int, Pl::P>>::type;
 
-// forward plans: a first radix that takes up the odd factor, then radix-16 passes (the last one is the
-// register link: 16 elements per thread, L / 16 threads per transform)
+// forward plans: radix 16 at both ends (the first pass of a column transform then reads only the N <= L/2
+// live rows, and the last one is the register link: 16 elements per thread, L / 16 threads per transform),
+// the remaining factor in between; lengths below 512 take that factor first. The radix-16 ends also keep
+// the column pass clear of register spills at the largest sizes.
 template <class T, int L> struct FwdPlan;
 #define MG_PLAN(L_, ...) \
     template <> struct FwdPlan<float, L_> { using type = Plan<__VA_ARGS__>; }; \
@@ -176,15 +178,15 @@ MG_PLAN(128, 8, 16)
 MG_PLAN(192, 12, 16)
 MG_PLAN(256, 16, 16)
 MG_PLAN(384, 24, 16)
-MG_PLAN(512, 32, 16)
-MG_PLAN(768, 3, 16, 16)
-MG_PLAN(1024, 4, 16, 16)
-MG_PLAN(1536, 6, 16, 16)
-MG_PLAN(2048, 8, 16, 16)
-MG_PLAN(3072, 12, 16, 16)
+MG_PLAN(512, 16, 2, 16)
+MG_PLAN(768, 16, 3, 16)
+MG_PLAN(1024, 16, 4, 16)
+MG_PLAN(1536, 16, 6, 16)
+MG_PLAN(2048, 16, 8, 16)
+MG_PLAN(3072, 16, 12, 16)
 MG_PLAN(4096, 16, 16, 16)
-MG_PLAN(6144, 24, 16, 16)
-MG_PLAN(8192, 32, 16, 16)
+MG_PLAN(6144, 16, 24, 16)
+MG_PLAN(8192, 16, 32, 16)
 #undef MG_PLAN
 
 template <int L> constexpr int fft_th() { return L / 16; }                                       // threads per transform
