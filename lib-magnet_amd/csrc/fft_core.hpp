@@ -10,43 +10,27 @@
 // starting with R_last takes it from; so a forward transform, a pointwise product and an inverse transform
 // whose plan is the reverse of the forward one chain through registers.
 //
-// LDS layout: element i of a buffer lives at swz(i) = i ^ ((i >> S) & (2^S - 1)), 2^S elements per 128
-// bytes (S = 4 for float complex, 3 for double complex); the XOR keeps the strided reads and the expanded
-// writes of a pass on distinct banks across 2^S consecutive lanes.
+// LDS layout: element i of a buffer lives at pad(i) = i + i / 16 (one spare element after every 16). The
+// strided reads of a pass (stride a multiple of 16 elements) and its expanded writes then fall on distinct
+// banks across a wave's lanes for 8- and 16-byte elements alike, and pad(i + 16 k) = pad(i) + 17 k, so the
+// R accesses of one item are one address plus compile-time offsets.
 //
-// Twiddles: exp(-2 pi i m / TW_PERIOD) = hi[m >> 6] * lo[m & 63], TW_PERIOD = 3 * 8192 so that every
-// supported length divides it; two small tables staged in LDS per workgroup. A pass looks up w^(2^b) and
-// forms the other powers with products (at most three deep).
+// Twiddles: per plan and pass, a table of exp(-2 pi i k 2^b / (Ns R)) for k < Ns and 2^b < R in global memory
+// (consecutive k on consecutive lanes: coalesced, L1-resident), correctly rounded from long double on the
+// host. A pass reads w^(2^b) and forms the other powers with products (at most three deep).
 #pragma once
 #include "common.hpp"
 #include <utility>
 
 namespace mg {
 
-template <class T> struct Swz { static constexpr int S = sizeof(cplx<T>) == 8 ? 4 : 3; };
-template <class T> __device__ __forceinline__ int swz(int i) {
-    constexpr int S = Swz<T>::S;
-    return i ^ ((i >> S) & ((1 << S) - 1));
-}
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+constexpr int padded(int n) { return n + n / 16; }   // LDS elements of a buffer of n (n a multiple of 16)
 
-constexpr int TW_PERIOD = 3 * MAX_M;       // 24576
-constexpr int TW_LO = 64;                  // exp(-2 pi i b / TW_PERIOD), b < 64
-constexpr int TW_HI = TW_PERIOD / TW_LO;   // exp(-2 pi i 64 a / TW_PERIOD), a < 384
-
-template <class T> struct TwLDS { cplx<T> hi[TW_HI]; cplx<T> lo[TW_LO]; };
-template <class T> struct TwPtr { const cplx<T>* hi; const cplx<T>* lo; };
-template <class T> TwPtr<T> fft_twiddles();   // this device's tables (after fft_init_twiddles)
-
-template <class T> __device__ inline void fft_stage_twiddles(TwLDS<T>& s, TwPtr<T> p) {
-    for (int i = threadIdx.x; i < TW_HI; i += blockDim.x) s.hi[i] = p.hi[i];
-    for (int i = threadIdx.x; i < TW_LO; i += blockDim.x) s.lo[i] = p.lo[i];
-}
-// exp(SIGN 2 pi i m / TW_PERIOD), 0 <= m < TW_PERIOD
-template <class T, int SIGN> __device__ __forceinline__ cplx<T> twiddle(const TwLDS<T>& s, int m) {
-    cplx<T> w = cmul(s.hi[m >> 6], s.lo[m & 63]);
-    if (SIGN > 0) w.y = -w.y;
-    return w;
-}
+// twiddle tables of one length L: the forward plan's, the reversed plan's, and exp(-2 pi i k / 2L), k <= L
+// (the real-transform post / pre step of the row passes)
+template <class T> struct TwSet { const cplx<T>* fwd; const cplx<T>* rev; const cplx<T>* half; };
+template <class T> TwSet<T> fft_twiddle_set(int L);   // this device's tables (after fft_init_twiddles)
 
 // cos and sin of 2 pi m / 96 (every radix used divides 96), built from the first octant by symmetry
 __device__ constexpr double kC96[96] = {
@@ -140,12 +124,17 @@ template <class T, int R, int SIGN> __device__ __forceinline__ void dft(cplx<T>*
 constexpr int hibit(int r) { int h = 1; while (2 * h <= r) h *= 2; return h; }
 constexpr int log2c(int r) { int l = 0; while ((1 << l) < r) ++l; return l; }
 
-// a[r] *= w^r (r < R), w = exp(SIGN 2 pi i m / TW_PERIOD), m (R - 1) < TW_PERIOD
-template <class T, int R, int SIGN> __device__ __forceinline__ void twiddle_apply(cplx<T>* a, int m, const TwLDS<T>& tw) {
+// a[r] *= w^r (r < R), w = exp(SIGN 2 pi i k / (Ns R)); tab[b Ns + k] = exp(-2 pi i k 2^b / (Ns R))
+template <class T, int R, int Ns, int SIGN>
+__device__ __forceinline__ void twiddle_apply(cplx<T>* a, const cplx<T>* __restrict__ tab, int k) {
     constexpr int NB = log2c(R);   // lookups at 2^b, b < NB (2^b <= R - 1)
     cplx<T> w[R];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) w[1 << b] = twiddle<T, SIGN>(tw, m << b);
+    for (int b = 0; b < NB; ++b) {
+        cplx<T> x = tab[b * Ns + k];
+        if (SIGN > 0) x.y = -x.y;
+        w[1 << b] = x;
+    }
 #pragma unroll
     for (int r = 3; r < R; ++r)
         if (r != hibit(r)) w[r] = cmul(w[r - hibit(r)], w[hibit(r)]);
@@ -160,6 +149,13 @@ template <int... Rs> struct Plan {
     static constexpr int L = (1 * ... * Rs);
 };
 template <class Pl> constexpr int plan_ns(int I) { int n = 1; for (int i = 0; i < I; ++i) n *= Pl::R[i]; return n; }
+// offset of pass I's twiddle table in the plan's (passes 1 .. P-1: log2c(R) Ns entries each)
+template <class Pl> constexpr int plan_tw_offset(int I) {
+    int o = 0;
+    for (int i = 1; i < I; ++i) o += log2c(Pl::R[i]) * plan_ns<Pl>(i);
+    return o;
+}
+template <class Pl> constexpr int plan_tw_size() { return plan_tw_offset<Pl>(Pl::P); }
 template <class Pl, class S> struct RevPlan;
 template <class Pl, int... I> struct RevPlan<Pl, std::integer_sequence<int, I...>> { using type = Plan<Pl::R[Pl::P - 1 - I]...>; };
 template <class Pl> using Rev = typename RevPlan<Pl, std::make_integer_sequence<int, Pl::P>>::type;
@@ -193,10 +189,11 @@ template <int L> constexpr int fft_th() { return L / 16; }                      
 template <int L> constexpr int fft_batch() { return L / 16 >= 256 ? 1 : 256 / (L / 16); }         // transforms per workgroup
 template <int L> constexpr int fft_block() { return fft_th<L>() * fft_batch<L>(); }
 
-// One Stockham pass (see the top of the file). ld(e, q, r) gives input element e for slot (q, r); mid() runs
-// between the loads and the stores (a barrier when both touch the same LDS); st(e, v, q, r) takes output e.
-template <class T, int L, int R, int Ns, int TH, int SIGN, class Ld, class Mid, class St>
-__device__ __forceinline__ void stockham(int t, const TwLDS<T>& tw, Ld&& ld, Mid&& mid, St&& st) {
+// One Stockham pass (see the top of the file). Inputs come from LDS (SRC = 1, transform at element offset
+// `base`) or from ld(e, q, r) for slot (q, r); outputs go to LDS (DST = 1) or to st(e, v, q, r). BAR puts a
+// barrier between the loads and the stores (both on the same LDS buffer). tab: this pass's twiddles.
+template <class T, int L, int R, int Ns, int TH, int SIGN, bool SRC, bool DST, bool BAR, class Ld, class St>
+__device__ __forceinline__ void stockham(int t, cplx<T>* lds, int base, const cplx<T>* __restrict__ tab, Ld&& ld, St&& st) {
     constexpr int NI = L / R, Q = (NI + TH - 1) / TH;
     constexpr bool exact = NI % TH == 0;
     cplx<T> v[Q][R];
@@ -204,48 +201,72 @@ __device__ __forceinline__ void stockham(int t, const TwLDS<T>& tw, Ld&& ld, Mid
     for (int q = 0; q < Q; ++q) {
         const int j = t + q * TH;
         if (!exact && j >= NI) continue;
+        if constexpr (SRC) {
+            if constexpr (NI % 16 == 0) {
+                const int p0 = pad(base + j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * NI, q, r);
-        if constexpr (Ns > 1) twiddle_apply<T, R, SIGN>(v[q], (j % Ns) * (TW_PERIOD / (Ns * R)), tw);
+                for (int r = 0; r < R; ++r) v[q][r] = lds[p0 + r * padded(NI)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[q][r] = lds[pad(base + j + r * NI)];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * NI, q, r);
+        }
+        if constexpr (Ns > 1) twiddle_apply<T, R, Ns, SIGN>(v[q], tab, j % Ns);
         dft<T, R, SIGN>(v[q]);
     }
-    mid();
+    if constexpr (BAR) __syncthreads();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int j = t + q * TH;
         if (!exact && j >= NI) continue;
         const int k = Ns > 1 ? j % Ns : 0;
         const int o = (j - k) * R + k;
+        if constexpr (DST) {
+            if constexpr (Ns % 16 == 0) {
+                const int p0 = pad(base + o);
 #pragma unroll
-        for (int r = 0; r < R; ++r) st(o + r * Ns, v[q][r], q, r);
+                for (int r = 0; r < R; ++r) lds[p0 + r * padded(Ns)] = v[q][r];
+            } else if constexpr (Ns == 1 && R % 16 == 0) {   // base + o is a multiple of 16
+                const int p0 = pad(base + o);
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[p0 + r + (r >> 4)] = v[q][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[pad(base + o + r * Ns)] = v[q][r];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(o + r * Ns, v[q][r], q, r);
+        }
     }
 }
 
 // Whole transform of plan Pl by the TH threads of one transform (t = index among them), LDS buffer at element
-// offset `base` of `lds`. Passes that write LDS are bracketed by barriers, so consecutive transforms on one
-// buffer need no extra synchronisation; LAST_BAR puts a barrier before the last pass's stores too (for a
-// functor that writes the same LDS). Unnormalised; SIGN -1 forward, +1 inverse.
-template <class T, class Pl, int SIGN, int TH, bool LAST_BAR, int I = 0, class Ld, class St>
-__device__ __forceinline__ void fft_run(int t, cplx<T>* lds, int base, const TwLDS<T>& tw, Ld&& ld, St&& st) {
+// offset `base` (a multiple of 16) of `lds`, twiddle tables of the plan at tw. Passes that write LDS are bracketed by barriers, so consecutive
+// transforms on one buffer need no extra synchronisation. LAST: where the last pass puts its outputs --
+// FN: st(e, v, q, r); FN_BAR: the same after a barrier (st writes the transform's own LDS); TO_LDS: the
+// transform's LDS buffer in natural order, after a barrier. Unnormalised; SIGN -1 forward, +1 inverse.
+enum FftLast { FN = 0, FN_BAR = 1, TO_LDS = 2 };
+template <class T, class Pl, int SIGN, int TH, int LAST, int I = 0, class Ld, class St>
+__device__ __forceinline__ void fft_run(int t, cplx<T>* lds, int base, const cplx<T>* __restrict__ tw, Ld&& ld, St&& st) {
     constexpr int P = Pl::P, L = Pl::L, R = Pl::R[I], Ns = plan_ns<Pl>(I);
-    auto lds_ld = [&](int e, int, int) { return lds[swz<T>(base + e)]; };
-    auto lds_st = [&](int e, cplx<T> v, int, int) { lds[swz<T>(base + e)] = v; };
-    auto bar = [] { __syncthreads(); };
-    auto nop = [] {};
+    const cplx<T>* tab = tw + plan_tw_offset<Pl>(I);
+    constexpr bool last_lds = LAST == TO_LDS, last_bar = LAST != FN;
     if constexpr (P == 1) {
-        if constexpr (LAST_BAR) stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, bar, st);
-        else stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, nop, st);
+        stockham<T, L, R, Ns, TH, SIGN, false, last_lds, last_bar>(t, lds, base, tab, ld, st);
     } else if constexpr (I == 0) {
-        stockham<T, L, R, Ns, TH, SIGN>(t, tw, ld, bar, lds_st);
+        stockham<T, L, R, Ns, TH, SIGN, false, true, true>(t, lds, base, tab, ld, st);
         __syncthreads();
-        fft_run<T, Pl, SIGN, TH, LAST_BAR, 1>(t, lds, base, tw, ld, st);
+        fft_run<T, Pl, SIGN, TH, LAST, 1>(t, lds, base, tw, ld, st);
     } else if constexpr (I + 1 < P) {
-        stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, bar, lds_st);
+        stockham<T, L, R, Ns, TH, SIGN, true, true, true>(t, lds, base, tab, ld, st);
         __syncthreads();
-        fft_run<T, Pl, SIGN, TH, LAST_BAR, I + 1>(t, lds, base, tw, ld, st);
+        fft_run<T, Pl, SIGN, TH, LAST, I + 1>(t, lds, base, tw, ld, st);
     } else {
-        if constexpr (LAST_BAR) stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, bar, st);
-        else stockham<T, L, R, Ns, TH, SIGN>(t, tw, lds_ld, nop, st);
+        stockham<T, L, R, Ns, TH, SIGN, true, last_lds, last_bar>(t, lds, base, tab, ld, st);
     }
 }
 

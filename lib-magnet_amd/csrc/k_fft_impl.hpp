@@ -26,27 +26,24 @@ template <class T> __device__ __forceinline__ cplx<T> czero() { return {T(0), T(
 
 // X_k (0 <= k <= L) of a real sequence of length 2L from Z = DFT_L(even + i odd) held in LDS:
 // E = (Z_k + conj Z_{L-k}) / 2, O = -i (Z_k - conj Z_{L-k}) / 2, X_k = E + exp(-2 pi i k / 2L) O
-template <class T, int L> __device__ inline cplx<T> r2c_post(const cplx<T>* lds, int base, int k, const TwLDS<T>& tw) {
-    const cplx<T> zk = lds[swz<T>(base + (k == L ? 0 : k))];
-    const cplx<T> zc = cconj(lds[swz<T>(base + (k == 0 ? 0 : L - k))]);
+template <class T, int L> __device__ inline cplx<T> r2c_post(const cplx<T>* lds, int base, int k, const cplx<T>* __restrict__ half) {
+    const cplx<T> zk = lds[pad(base + (k == L ? 0 : k))];
+    const cplx<T> zc = cconj(lds[pad(base + (k == 0 ? 0 : L - k))]);
     const cplx<T> e = {(zk.x + zc.x) * T(0.5), (zk.y + zc.y) * T(0.5)};
     const cplx<T> d = csub(zk, zc);
     const cplx<T> o = {d.y * T(0.5), -d.x * T(0.5)};
-    return cadd(e, cmul(twiddle<T, -1>(tw, k * (TW_PERIOD / (2 * L))), o));
+    return cadd(e, cmul(half[k], o));
 }
 
 // ---- rows forward: blockIdx.y = channel, fft_batch rows per workgroup
 template <class T, int L>
 __global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict__ W, int N, cplx<T>* __restrict__ out,
-                                                             TwPtr<T> twp) {
+                                                             TwSet<T> tws) {
     using Pl = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, R0 = Pl::R[0];
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp);
-    __syncthreads();
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
     const int c = blockIdx.y, gx = blockIdx.x * B + b;
     const bool live = gx < N, even = (N & 1) == 0;
@@ -61,12 +58,12 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict
         }
         return z;
     };
-    auto st = [&](int e, cplx<T> v, int, int) { lds[swz<T>(b * L + e)] = v; };
-    fft_run<T, Pl, -1, TH, true>(t, lds, b * L, tw, ld, st);
+    auto none = [](int, cplx<T>, int, int) {};
+    fft_run<T, Pl, -1, TH, TO_LDS>(t, lds, b * L, tws.fwd, ld, none);
     __syncthreads();
     if (!live) return;
     cplx<T>* dst = out + ((size_t)c * N + gx) * Mh;
-    for (int k = t; k < Mh; k += TH) dst[k] = r2c_post<T, L>(lds, b * L, k, tw);
+    for (int k = t; k < Mh; k += TH) dst[k] = r2c_post<T, L>(lds, b * L, k, tws.half);
 }
 
 // ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
@@ -101,17 +98,14 @@ template <class T, int L, bool CEN>
 __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
                                                             const T* __restrict__ K1, const T* __restrict__ K2,
                                                             const T* __restrict__ TX, const T* __restrict__ TY,
-                                                            cplx<T>* __restrict__ Ph, TwPtr<T> twp) {
+                                                            cplx<T>* __restrict__ Ph, TwSet<T> tws) {
     using F = typename FwdPlan<T, L>::type;
     using I = Rev<F>;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L / 2 + 1, R0 = F::R[0], RL = F::R[F::P - 1];
     constexpr int QL = L / RL / TH;
     static_assert((L / RL) % TH == 0, "the register link needs the last pass to cover every thread evenly");
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp);
-    __syncthreads();
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
     const int ky = blockIdx.x * B + b;
     const bool live = ky < Mh;
@@ -124,7 +118,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
             return col[e];
         };
         auto st = [&](int, cplx<T> x, int q, int r) { dst[q][r] = x; };
-        fft_run<T, F, -1, TH, false>(t, lds, b * L, tw, ld, st);
+        fft_run<T, F, -1, TH, FN>(t, lds, b * L, tws.fwd, ld, st);
     };
     auto inv = [&](int o, cplx<T> (&src)[QL][RL]) {
         cplx<T>* dst = Ph + ((size_t)o * Mh + kyc) * N;
@@ -132,7 +126,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
         auto st = [&](int e, cplx<T> x, int, int r) {
             if (2 * r < R0 && e < N && live) dst[e] = x;   // only the N live rows
         };
-        fft_run<T, I, +1, TH, false>(t, lds, b * L, tw, ld, st);
+        fft_run<T, I, +1, TH, FN>(t, lds, b * L, tws.rev, ld, st);
     };
     const T* k1 = K1 + (size_t)kyc * Mh;
     const T* k2 = K2 + (size_t)kyc * Mh;
@@ -173,15 +167,12 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
 // Z_e = E + i O with E = X_e + conj X_{L-e}, O = (X_e - conj X_{L-e}) exp(+2 pi i e / 2L)
 template <class T, int L>
 __global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __restrict__ X, int N, T* __restrict__ phi,
-                                                             T scale, TwPtr<T> twp) {
+                                                             T scale, TwSet<T> tws) {
     using Pl = Rev<typename FwdPlan<T, L>::type>;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1];
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp);
-    __syncthreads();
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
     const int c = blockIdx.y, gx = blockIdx.x * B + b;
     const bool live = gx < N, even = (N & 1) == 0;
@@ -189,7 +180,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __re
     auto ld = [&](int e, int, int) -> cplx<T> {
         const cplx<T> xk = src[e], xc = cconj(src[L - e]);
         const cplx<T> ev = cadd(xk, xc);
-        const cplx<T> od = cmul(csub(xk, xc), twiddle<T, +1>(tw, e * (TW_PERIOD / (2 * L))));
+        const cplx<T> od = cmul(csub(xk, xc), cconj(tws.half[e]));
         return {ev.x - od.y, ev.y + od.x};
     };
     T* dst = phi + ((size_t)c * N + (live ? gx : 0)) * N;
@@ -200,7 +191,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __re
         if (even) { V2 p; p.x = v.x * scale; p.y = v.y * scale; reinterpret_cast<V2*>(dst)[e] = p; }
         else { dst[n0] = v.x * scale; if (n0 + 1 < N) dst[n0 + 1] = v.y * scale; }
     };
-    fft_run<T, Pl, +1, TH, false>(t, lds, b * L, tw, ld, st);
+    fft_run<T, Pl, +1, TH, FN>(t, lds, b * L, tws.rev, ld, st);
 }
 
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
@@ -231,15 +222,12 @@ __device__ inline void kernel_values(int a, int b, int M, double delta, double d
 template <class T, int L, int NT>
 __global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double dof, const double* __restrict__ table,
                                                           int round_kernel, cplx<T>* __restrict__ P0, cplx<T>* __restrict__ P1,
-                                                          TwPtr<T> twp) {
+                                                          TwSet<T> tws) {
     using F = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
     constexpr int QL = L / RL / TH;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp);
-    __syncthreads();
     const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
     const int a = blockIdx.x * B + bb;
     const bool live = a < Mh;
@@ -252,7 +240,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double d
             kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
             return {(T)k1, (T)k2};
         };
-        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, put(za));
+        fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, put(za));
     }
     if constexpr (NT == 4) {
         auto ld = [&](int e, int, int) -> cplx<T> {
@@ -260,7 +248,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double d
             kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
             return {(T)ky, T(0)};
         };
-        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, put(zb));
+        fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, put(zb));
     }
     if (!live) return;
     const T xa = a == M / 2 ? T(0) : (T)(a * delta);
@@ -281,15 +269,12 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double d
 template <class T, int L, int NT>
 __global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restrict__ Q0, const cplx<T>* __restrict__ Q1,
                                                           T* __restrict__ K1, T* __restrict__ K2, T* __restrict__ TX,
-                                                          T* __restrict__ TY, TwPtr<T> twp) {
+                                                          T* __restrict__ TY, TwSet<T> tws) {
     using F = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
     constexpr int QL = L / RL / TH;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TwLDS<T> tw;
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    fft_stage_twiddles(tw, twp);
-    __syncthreads();
     const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
     const int ky = blockIdx.x * B + bb;
     const bool live = ky < Mh;
@@ -298,7 +283,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restr
     {
         auto ld = [&](int e, int, int) -> cplx<T> { return Q0[row + (e <= M / 2 ? e : M - e)]; };
         auto st = [&](int, cplx<T> x, int q, int r) { z[q][r] = x; };
-        fft_run<T, F, -1, TH, false>(t, lds, bb * L, tw, ld, st);
+        fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, st);
     }
     if (live) {
 #pragma unroll
@@ -317,12 +302,12 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restr
             const cplx<T> p = Q1[row + (M - e)];
             return {p.x, -p.y};
         };
-        auto st = [&](int e, cplx<T> x, int, int) { lds[swz<T>(bb * L + e)] = x; };
-        fft_run<T, F, -1, TH, true>(t, lds, bb * L, tw, ld, st);
+        auto none = [](int, cplx<T>, int, int) {};
+        fft_run<T, F, -1, TH, TO_LDS>(t, lds, bb * L, tws.fwd, ld, none);
         __syncthreads();
         if (!live) return;
         for (int k = t; k < Mh; k += TH) {
-            const T zk = lds[swz<T>(bb * L + k)].x, zc = lds[swz<T>(bb * L + (k == 0 ? 0 : M - k))].x;
+            const T zk = lds[pad(bb * L + k)].x, zc = lds[pad(bb * L + (k == 0 ? 0 : M - k))].x;
             TY[row + k] = T(0.5) * (zk + zc);
             TX[row + k] = T(-0.5) * (zk - zc);
         }
@@ -340,7 +325,7 @@ template <class F> void set_lds(F* f, int bytes) {
     if (bytes > 65536) MG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
 }
 
-template <class T, int L> int lds_bytes() { return fft_batch<L>() * L * (int)sizeof(cplx<T>); }
+template <class T, int L> int lds_bytes() { return fft_batch<L>() * padded(L) * (int)sizeof(cplx<T>); }
 
 }  // namespace
 
@@ -349,10 +334,9 @@ template <class T, int L> int lds_bytes() { return fft_batch<L>() * L * (int)siz
 
 template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* out, hipStream_t st) {
     require(N <= M / 2, "FFT size below 2N");
-    const TwPtr<T> tw = fft_twiddles<T>();
 #define MG_C(L_) case L_: { \
         const int by = lds_bytes<T, L_>(); set_lds(k_rows_fwd<T, L_>, by); \
-        hipLaunchKernelGGL((k_rows_fwd<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 3), dim3(fft_block<L_>()), by, st, W, N, out, tw); \
+        hipLaunchKernelGGL((k_rows_fwd<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 3), dim3(fft_block<L_>()), by, st, W, N, out, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
@@ -369,12 +353,11 @@ template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T*
     require(N <= M / 2, "FFT size below 2N");
     require(centred == (sizeof(T) == 4), "the FP32 path uses the centred product set, the FP64 path the reference's");
     constexpr bool CEN = sizeof(T) == 4;
-    const TwPtr<T> tw = fft_twiddles<T>();
     const int Mh = M / 2 + 1;
 #define MG_C(L_) case L_: { \
         const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN>, by); \
         hipLaunchKernelGGL((k_columns<T, L_, CEN>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
-                           Wh, N, K[0], K[1], K[2], K[3], Ph, tw); \
+                           Wh, N, K[0], K[1], K[2], K[3], Ph, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
@@ -383,11 +366,10 @@ template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T*
 
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st) {
     require(N <= M / 2, "FFT size below 2N");
-    const TwPtr<T> tw = fft_twiddles<T>();
     const T scale = (T)(1.0 / ((double)M * (double)M));
 #define MG_C(L_) case L_: { \
         const int by = lds_bytes<T, L_>(); set_lds(k_rows_inv<T, L_>, by); \
-        hipLaunchKernelGGL((k_rows_inv<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 4), dim3(fft_block<L_>()), by, st, X, N, phi, scale, tw); \
+        hipLaunchKernelGGL((k_rows_inv<T, L_>), dim3((N + fft_batch<L_>() - 1) / fft_batch<L_>(), 4), dim3(fft_block<L_>()), by, st, X, N, phi, scale, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
@@ -397,7 +379,6 @@ template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* p
 template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
                                        cplx<T>* tmp, T* const* K, hipStream_t st) {
     require(ntab == 2 || ntab == 4, "ntab must be 2 or 4");
-    const TwPtr<T> tw = fft_twiddles<T>();
     const int Mh = M / 2 + 1;
     const size_t plane = (size_t)Mh * Mh;
     cplx<T>* P0 = tmp;              // stage 1 output [a][ky]
@@ -407,12 +388,12 @@ template <class T> void launch_spectra(int M, double delta, double dof, const do
 #define MG_S1(L_, NT) do { \
         const int by = lds_bytes<T, L_>(); set_lds(k_spec1<T, L_, NT>, by); \
         hipLaunchKernelGGL((k_spec1<T, L_, NT>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
-                           delta, dof, table, (int)round_kernel, P0, P1, tw); \
+                           delta, dof, table, (int)round_kernel, P0, P1, fft_twiddle_set<T>(L_)); \
     } while (0)
 #define MG_S2(L_, NT) do { \
         const int by = lds_bytes<T, L_>(); set_lds(k_spec2<T, L_, NT>, by); \
         hipLaunchKernelGGL((k_spec2<T, L_, NT>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
-                           Q0, Q1, K[0], K[1], K[2], K[3], tw); \
+                           Q0, Q1, K[0], K[1], K[2], K[3], fft_twiddle_set<T>(L_)); \
     } while (0)
 #define MG_C(L_) case L_: \
         if (ntab == 4) { MG_S1(L_, 4); launch_transpose<T>(P0, Q0, 2, Mh, Mh, st); MG_S2(L_, 4); } \

@@ -70,9 +70,12 @@ __global__ __launch_bounds__(TPB) void k_bbox(const T* __restrict__ Y, i64 n, T*
     if (threadIdx.x == 0) { out[0] = x0; out[1] = y0; out[2] = x1; out[3] = y1; *counter = 0u; }
 }
 
-// ---- tsne_attraction, 2-D path (src/fftrep.cpp:142-158): G lanes sweep one CSR row with stride G,
-// then a butterfly over the G lanes (the same order on every run).
-template <class T, int G>
+// ---- tsne_attraction, 2-D path (src/fftrep.cpp:142-158): G lanes sweep one CSR row with stride G, then a
+// butterfly over the G lanes (the same order on every run). Each lane takes U edges per round: their indices
+// and weights are loaded together (streamed past the caches), then the U neighbour positions (L2-resident),
+// so a round costs two memory latencies whatever the row length. Per lane the edges are summed in increasing
+// order, as a one-edge-per-round sweep would.
+template <class T, int G, int U>
 __global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
                                                const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
                                                T* __restrict__ out) {
@@ -83,12 +86,24 @@ __global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, c
     T ax = 0, ay = 0;
     if (row < n) {
         const V yi = Y2[row];
-        const int e1 = ip[row + 1];
-        for (int e = ip[row] + sub; e < e1; e += G) {
-            const V yj = Y2[ix[e]];
-            const T dx = yi.x - yj.x, dy = yi.y - yj.y;
-            const T w = P[e] / (T(1) + (dx * dx + dy * dy) * inv_dof);
-            ax += w * dx; ay += w * dy;
+        const int e0 = ip[row], e1 = ip[row + 1];
+        for (int base = e0 + sub; base < e1; base += G * U) {
+            int j[U];
+            T p[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = base + u * G;
+                j[u] = e < e1 ? __builtin_nontemporal_load(ix + e) : -1;
+                p[u] = e < e1 ? __builtin_nontemporal_load(P + e) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (j[u] < 0) break;
+                const V yj = Y2[j[u]];
+                const T dx = yi.x - yj.x, dy = yi.y - yj.y;
+                const T w = p[u] / (T(1) + (dx * dx + dy * dy) * inv_dof);
+                ax += w * dx; ay += w * dy;
+            }
         }
     }
 #pragma unroll
@@ -263,12 +278,14 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
                                           T inv_dof, T* out, i64 nnz, hipStream_t st) {
     if (n == 0) return;
     if (dim == 2) {
+        // lanes per row: enough for about eight edges each in one round; a whole wave for the long rows of
+        // perplexity affinities (random neighbour reads from L2 bound the sweep: more rows per wave measured slower)
         const double avg = (double)nnz / (double)n;
         int G = 1;
-        while (G < 64 && G * 4 < avg) G <<= 1;   // about four edges per lane
+        while (G < 64 && G * 8 < avg) G <<= 1;
         const unsigned blocks = (unsigned)((n + (TPB / G) - 1) / (TPB / G));
         switch (G) {
-#define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
+#define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g, 8>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
             MG_ATTR_CASE(1) MG_ATTR_CASE(2) MG_ATTR_CASE(4) MG_ATTR_CASE(8) MG_ATTR_CASE(16) MG_ATTR_CASE(32) MG_ATTR_CASE(64)
 #undef MG_ATTR_CASE
         }
