@@ -278,11 +278,11 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
                                           T inv_dof, T* out, i64 nnz, hipStream_t st) {
     if (n == 0) return;
     if (dim == 2) {
-        // lanes per row: enough for about eight edges each in one round; a whole wave for the long rows of
-        // perplexity affinities (random neighbour reads from L2 bound the sweep: more rows per wave measured slower)
+        // lanes per row: up to a whole wave, about four edges per lane; the long rows of perplexity affinities
+        // get a wave each (random neighbour reads from L2 bound the sweep: more rows per wave measured slower)
         const double avg = (double)nnz / (double)n;
         int G = 1;
-        while (G < 64 && G * 8 < avg) G <<= 1;
+        while (G < 64 && G * 4 < avg) G <<= 1;
         const unsigned blocks = (unsigned)((n + (TPB / G) - 1) / (TPB / G));
         switch (G) {
 #define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g, 8>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
