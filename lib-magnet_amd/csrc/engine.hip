@@ -441,8 +441,21 @@ template <class T> static void step_impl(int mode, void* Y, void* U, void* G, co
     MG_HIP(hipStreamSynchronize(g_stream));
 }
 
+// The device spectra use the symmetry of the kernel in each offset separately (K(a, b) = K(-a, b) = K(a, -b)),
+// which every radial kernel has; a table without it is refused rather than convolved wrongly.
 template <class T> static void set_values(Repulsion<T>& r, int M_in, const double* K1, const double* K2) {
     require(M_in >= 1 && K1 && K2, "K1 / K2 must be (M, M) tables");
+    for (const double* K : {K1, K2}) {
+        double scale = 0, dev = 0;
+        for (int a = 0; a < M_in; ++a)
+            for (int b = 0; b < M_in; ++b) {
+                const double v = K[(size_t)a * M_in + b];
+                scale = std::max(scale, std::fabs(v));
+                dev = std::max(dev, std::fabs(v - K[(size_t)((M_in - a) % M_in) * M_in + b]));
+                dev = std::max(dev, std::fabs(v - K[(size_t)a * M_in + (M_in - b) % M_in]));
+            }
+        require(dev <= 1e-5 * scale, "kernel tables must be even in each offset (radial kernels)", MG_ERR_UNSUP);
+    }
     r.kvals.assign((size_t)2 * M_in * M_in, 0.0);
     std::memcpy(r.kvals.data(), K1, (size_t)M_in * M_in * sizeof(double));
     std::memcpy(r.kvals.data() + (size_t)M_in * M_in, K2, (size_t)M_in * M_in * sizeof(double));
