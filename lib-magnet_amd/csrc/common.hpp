@@ -80,7 +80,7 @@ template <class T> __device__ inline int box_key(T x, T y, const GridT<T>& g) {
 // ---- scratch of one repulsion engine (device pointers), sized by the host
 template <class T> struct RepBuffers {
     int *keysA, *valsA, *keysB, *valsB;   // radix-sort ping-pong (box key, point index)
-    int* hist;                            // 256 * nblocks digit histograms
+    int* hist;                            // digit rows (up to 1024 x nblocks) + digit totals
     int* item_key;                        // 2 per spread chunk
     int* item_valid;
     T* item_val;                          // 2 per chunk x NI x 3NI

@@ -237,7 +237,7 @@ template <class T> struct Repulsion {
         const size_t nk = (size_t)std::max<i64>(n, 1);
         keysA.ensure(nk * sizeof(int)); valsA.ensure(nk * sizeof(int));
         keysB.ensure(nk * sizeof(int)); valsB.ensure(nk * sizeof(int));
-        hist.ensure((size_t)256 * std::max(1, sort_hist_blocks(n)) * sizeof(int));
+        hist.ensure(((size_t)1024 * std::max(1, sort_hist_blocks(n)) + 1024) * sizeof(int));   // digit rows + totals
         const int nch = std::max(1, spread_chunks(n));
         item_key.ensure((size_t)2 * nch * sizeof(int));
         item_valid.ensure((size_t)2 * nch * sizeof(int));

@@ -46,11 +46,17 @@ template <int NI> __device__ inline double node_offset(double f, double width, i
 }
 
 // ---------------------------------------------------------------------------------------------- sort
-template <class T, bool FIRST>
+// Stable LSD radix sort of the box keys (src/fftconv.cpp:83-97 sorts by x-box with a counting sort; the full
+// key groups each box's points for the deterministic spread). Digits of DB bits (8 or 10: at most two passes
+// for up to 1024 boxes per side). Per pass: per-block digit counts, hist[d][block]; an exclusive scan of each
+// digit's row (one workgroup per digit); the stable scatter, whose blocks turn the digit totals into digit
+// bases themselves.
+template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
                                                  i64 n, int shift, int* __restrict__ hist, int nblocks) {
-    __shared__ int h[256];
-    h[threadIdx.x] = 0;
+    constexpr int NBK = 1 << DB;
+    __shared__ int h[NBK];
+    for (int d = threadIdx.x; d < NBK; d += TPB) h[d] = 0;
     __syncthreads();
     typedef typename vec2<T>::type V;
     const i64 base = (i64)blockIdx.x * RS_TILE;
@@ -60,45 +66,68 @@ __global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<
             int key;
             if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); }
             else key = keys_in[i];
-            atomicAdd(&h[(key >> shift) & 255], 1);
+            atomicAdd(&h[(key >> shift) & (NBK - 1)], 1);
         }
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NBK; d += TPB) hist[d * nblocks + blockIdx.x] = h[d];
 }
 
-// exclusive scan of the digit-major histogram (one workgroup)
-__global__ __launch_bounds__(1024) void k_rs_scan(int* __restrict__ hist, int total) {
-    __shared__ int sh[1024];
+// exclusive block-wide scan of one value per thread; returns the thread's prefix, *total the sum
+__device__ inline int block_exscan(int v, int* sh, int* total) {
     const int t = threadIdx.x;
-    const int per = (total + 1023) / 1024;
-    const int s0 = min(total, t * per), s1 = min(total, s0 + per);
-    int s = 0;
-    for (int k = s0; k < s1; ++k) s += hist[k];
-    sh[t] = s;
+    sh[t] = v;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? sh[t - d] : 0;
+    for (int d = 1; d < TPB; d <<= 1) {
+        const int u = t >= d ? sh[t - d] : 0;
         __syncthreads();
-        sh[t] += v;
+        sh[t] += u;
         __syncthreads();
     }
-    int off = sh[t] - s;
-    for (int k = s0; k < s1; ++k) { const int v = hist[k]; hist[k] = off; off += v; }
+    const int incl = sh[t];
+    *total = sh[TPB - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// one workgroup per digit: exclusive scan of hist[d][0 .. nblocks) in place, digit total to tot[d]
+__global__ __launch_bounds__(TPB) void k_rs_rowscan(int* __restrict__ hist, int nblocks, int* __restrict__ tot) {
+    __shared__ int sh[TPB];
+    int* row = hist + (size_t)blockIdx.x * nblocks;
+    int carry = 0;
+    for (int c0 = 0; c0 < nblocks; c0 += TPB) {
+        const int k = c0 + threadIdx.x;
+        const int v = k < nblocks ? row[k] : 0;
+        int sum;
+        const int ex = block_exscan(v, sh, &sum);
+        if (k < nblocks) row[k] = carry + ex;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
 // stable scatter: an item's rank is (items of its digit in earlier blocks) + (in earlier rounds of this
-// block) + (in earlier waves of this round) + (in earlier lanes of its wave, from 8 ballots)
-template <class T, bool FIRST>
+// block) + (in earlier waves of this round) + (in earlier lanes of its wave, from DB ballots)
+template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
                                                     const int* __restrict__ vals_in, i64 n, int shift,
-                                                    const int* __restrict__ hist, int nblocks, int* __restrict__ keys_out,
-                                                    int* __restrict__ vals_out) {
-    __shared__ int off[256];
-    __shared__ int wcnt[TPB / WAVE][256];
+                                                    const int* __restrict__ hist, const int* __restrict__ tot, int nblocks,
+                                                    int* __restrict__ keys_out, int* __restrict__ vals_out) {
+    constexpr int NBK = 1 << DB, PER = NBK / TPB > 0 ? NBK / TPB : 1;
+    __shared__ int off[NBK];
+    __shared__ int wcnt[TPB / WAVE][NBK];
+    __shared__ int sh[TPB];
     const int t = threadIdx.x, wave = t / WAVE, lane = t % WAVE;
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    off[t] = hist[t * nblocks + blockIdx.x];
+    {   // digit bases: exclusive scan of the digit totals (thread t owns digits t*PER .. t*PER + PER - 1)
+        int loc[PER], s = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) { const int d = t * PER + u; loc[u] = d < NBK ? tot[d] : 0; s += loc[u]; }
+        int all;
+        int ex = block_exscan(s, sh, &all);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) { const int d = t * PER + u; if (d < NBK) off[d] = ex + hist[(size_t)d * nblocks + blockIdx.x]; ex += loc[u]; }
+    }
     typedef typename vec2<T>::type V;
     const i64 base = (i64)blockIdx.x * RS_TILE;
     for (int r = 0; r < RS_ITEMS; ++r) {
@@ -109,13 +138,12 @@ __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, Gri
             if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); val = (int)i; }
             else { key = keys_in[i]; val = vals_in[i]; }
         }
-        const int d = (key >> shift) & 255;
-#pragma unroll
-        for (int w = 0; w < TPB / WAVE; ++w) wcnt[w][t] = 0;
+        const int d = (key >> shift) & (NBK - 1);
+        for (int k = t; k < (TPB / WAVE) * NBK; k += TPB) (&wcnt[0][0])[k] = 0;
         __syncthreads();
         unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < DB; ++b) {
             const bool bit = (d >> b) & 1;
             const unsigned long long m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -129,10 +157,12 @@ __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, Gri
             keys_out[pos] = key; vals_out[pos] = val;
         }
         __syncthreads();
-        int add = 0;
+        for (int k = t; k < NBK; k += TPB) {
+            int add = 0;
 #pragma unroll
-        for (int w = 0; w < TPB / WAVE; ++w) add += wcnt[w][t];
-        off[t] += add;
+            for (int w = 0; w < TPB / WAVE; ++w) add += wcnt[w][k];
+            off[k] += add;
+        }
         __syncthreads();
     }
 }
@@ -435,29 +465,36 @@ int sort_hist_blocks(i64 n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
 int spread_chunks(i64 n) { return (int)((n + SP_CHUNK - 1) / SP_CHUNK); }
 int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024); }
 
-template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
+template <class T, int DB> static int box_sort_db(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, int passes,
+                                                 hipStream_t st) {
     const int nblocks = sort_hist_blocks(n);
-    const long long nkeys = (long long)g.nb * g.nb;
-    int bits = 0;
-    while ((1ll << bits) < nkeys) ++bits;
-    const int passes = bits <= 8 ? 1 : (bits + 7) / 8;
-    int* kin = nullptr; int* vin = nullptr;
+    int* tot = b.hist + (size_t)(1 << DB) * nblocks;
+    const int* kin = nullptr; const int* vin = nullptr;
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
+        const int shift = DB * p;
         int* kout = (p % 2 == 0) ? b.keysA : b.keysB;
         int* vout = (p % 2 == 0) ? b.valsA : b.valsB;
-        if (p == 0) hipLaunchKernelGGL((k_rs_hist<T, true>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
-        else hipLaunchKernelGGL((k_rs_hist<T, false>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
+        if (p == 0) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
+        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
         MG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, b.hist, 256 * nblocks);
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, b.hist, nblocks, tot);
         MG_HIP(hipGetLastError());
-        if (p == 0) hipLaunchKernelGGL((k_rs_scatter<T, true>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, nblocks, kout, vout);
-        else hipLaunchKernelGGL((k_rs_scatter<T, false>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, nblocks, kout, vout);
+        if (p == 0) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, tot, nblocks, kout, vout);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, tot, nblocks, kout, vout);
         MG_HIP(hipGetLastError());
         kin = kout; vin = vout; cur = p % 2;
     }
     return cur;
+}
+
+template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
+    const long long nkeys = (long long)g.nb * g.nb;
+    int bits = 0;
+    while ((1ll << bits) < nkeys) ++bits;
+    if (bits <= 8) return box_sort_db<T, 8>(Y, n, g, b, 1, st);
+    if (bits <= 20) return box_sort_db<T, 10>(Y, n, g, b, (bits + 9) / 10, st);
+    return box_sort_db<T, 8>(Y, n, g, b, (bits + 7) / 8, st);
 }
 
 template <class T, int NI, bool CEN> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,

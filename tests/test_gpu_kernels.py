@@ -210,9 +210,9 @@ def test_repulsion_matches_exact_sum_reference_tolerance():
         assert abs(s - Z) / Z < 1e-3 and rel(F, Fe) < 2e-2
 
 
-def test_repulsion_fp64_clipped_grid_three_sort_passes():
+def test_repulsion_fp64_clipped_grid_two_sort_passes():
     """a layout wider than 1000 boxes of 0.7: n_boxes clips at 1000 and the width grows to span / 1000
-    (magnet/tsne.py:40-45); 10^6 box keys take three radix passes. n_interp = 1 keeps N = 1000."""
+    (magnet/tsne.py:40-45); 10^6 box keys take two 10-bit radix passes. n_interp = 1 keeps N = 1000."""
     rng = np.random.default_rng(12)
     c = rng.uniform(-450, 450, (40, 2))
     Y = c[rng.integers(0, 40, 50_000)] + rng.normal(0, 2.0, (50_000, 2))
@@ -221,6 +221,19 @@ def test_repulsion_fp64_clipped_grid_three_sort_passes():
     s, F = rep_gpu(Y, 1)
     so, Fo = O.repulsion(Y, xmin, ymin, width, nb, 1)
     assert abs(s - so) / so < 1e-12 and rel(F, Fo) < 1e-10
+
+
+def test_repulsion_fp64_more_than_1024_boxes():
+    """2048 boxes per side through the FFTConv2D call (n_interp 1): 2^22 box keys take the 8-bit digit sort
+    (three passes)"""
+    rng = np.random.default_rng(21)
+    nb = 2048
+    Y = rng.uniform(0.5, nb * 0.7 - 0.5, (20_000, 2))
+    conv = capi.FFTConv(capi.MG_F64, 2 * nb, 1)
+    conv.set_tsne(0.7)
+    s, F = conv.compute(dev(Y), 0.0, 0.0, 0.7, nb)
+    so, Fo = O.repulsion(Y, 0.0, 0.0, 0.7, nb, 1)
+    assert abs(s - so) / so < 1e-12 and rel(host(F), Fo) < 1e-11
 
 
 # one grid per compiled transform size M (2^s and 3 * 2^s, the smallest >= 2N): (n_interp, n_boxes)
