@@ -114,6 +114,26 @@ int mg_fftconv_compute(mg_fftconv* conv, const void* d_Y, int64_t n, double xmin
  * (interleaved, after the 1/M^2 scale). Either pointer may be NULL. */
 int mg_fftconv_grids(mg_fftconv* conv, void* d_W, void* d_phi);
 
+/* ---- fallback kernels (src/fftrep.cpp:51-125, 294-411; magnet/tsne.py:92-129 calls them when FFTConv2D is not
+ * used). Box of a point: u = (x - xmin) / width clamped to [0, n_boxes - 1], product-form Lagrange weights;
+ * N = n_boxes * n_interp. Sums run in the reference's single-thread order without fused multiply-adds, so the
+ * float32 results are the reference's. Device pointers; complex arrays are interleaved (re, im) pairs. */
+/* interp_scatter (src/fftrep.cpp:51-92): Y (n, 2), charges (n, C) -> out (C, N, N) */
+int mg_interp_scatter(int prec, const void* d_Y, int64_t n, const void* d_charges, int C, double xmin, double ymin,
+                      double width, int n_boxes, int n_interp, void* d_out);
+/* interp_gather (src/fftrep.cpp:94-125): grid (C, N, N) -> out (n, C) */
+int mg_interp_gather(int prec, const void* d_Y, int64_t n, const void* d_grid, int C, double xmin, double ymin,
+                     double width, int n_boxes, int n_interp, void* d_out);
+/* scatter_padded_2d (src/fftrep.cpp:294-343): charges (1, x, y) on the zero-padded grid, out (3, M, M), M >= N */
+int mg_scatter_padded_2d(int prec, const void* d_Y, int64_t n, double xmin, double ymin, double width, int n_boxes,
+                         int n_interp, int M, void* d_out);
+/* spectral_products (src/fftrep.cpp:346-371): K1h, K2h (M, Mh) complex, Wh (3, M, Mh) complex ->
+ * out (4, M, Mh) complex: K1h Wh[0], K2h Wh[0..2] */
+int mg_spectral_products(int prec, const void* d_K1h, const void* d_K2h, const void* d_Wh, int64_t M, int64_t Mh, void* d_out);
+/* gather_forces_2d (src/fftrep.cpp:373-411): phi (4, M, M) -> F (n, 2), *h_sumQ = sum pot0 - n */
+int mg_gather_forces_2d(int prec, const void* d_Y, int64_t n, const void* d_phi, int M, double xmin, double ymin,
+                        double width, int n_boxes, int n_interp, void* d_F, double* h_sumQ);
+
 /* ---- whole loop: optimize_tsne (magnet/tsne.py:132-214) ------------------------------------------ */
 typedef struct {
     int32_t n_iter;                /* schedule length (tsne.py:132 n_iter) */

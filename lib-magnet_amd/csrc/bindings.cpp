@@ -127,6 +127,83 @@ std::tuple<float, float, float, float> bbox_2d(arr<float> Y, int) {
     return std::make_tuple((float)out[0], (float)out[1], (float)out[2], (float)out[3]);
 }
 
+// --------------------------------------------------------------------------- fallback kernels
+// (src/fftrep.cpp:51-125, 294-411), float32 like the reference's, run on the device
+arr<float> interp_scatter(arr<float> Y, arr<float> charges, float xmin, float ymin, float width, int n_boxes, int n_interp, int) {
+    require(Y.ndim() == 2 && Y.shape(1) == 2, "Y must be (n, 2)");
+    const int64_t n = Y.shape(0);
+    require(charges.ndim() == 2 && charges.shape(0) == n, "charges must be (n, T)");
+    require(n_boxes >= 1 && n_interp >= 1 && n_interp <= 8 && width > 0, "bad grid parameters");
+    const int C = (int)charges.shape(1);
+    const int64_t N = (int64_t)n_boxes * n_interp;
+    arr<float> out({(int64_t)C, N, N});
+    Dev dy = up(Y), dq = up(charges), dout((size_t)C * N * N * sizeof(float));
+    py::gil_scoped_release rel;
+    check(mg_interp_scatter(MG_F32, dy.p, n, dq.p, C, xmin, ymin, width, n_boxes, n_interp, dout.p));
+    dout.to_host(out.mutable_data());
+    return out;
+}
+
+arr<float> interp_gather(arr<float> Y, arr<float> grid, float xmin, float ymin, float width, int n_boxes, int n_interp, int) {
+    require(Y.ndim() == 2 && Y.shape(1) == 2, "Y must be (n, 2)");
+    const int64_t n = Y.shape(0);
+    require(grid.ndim() == 3 && grid.shape(1) == (int64_t)n_boxes * n_interp && grid.shape(2) == (int64_t)n_boxes * n_interp,
+            "grid must be (T, N, N)");
+    const int C = (int)grid.shape(0);
+    arr<float> out({n, (int64_t)C});
+    if (n == 0) return out;
+    Dev dy = up(Y), dg = up(grid), dout((size_t)n * C * sizeof(float));
+    py::gil_scoped_release rel;
+    check(mg_interp_gather(MG_F32, dy.p, n, dg.p, C, xmin, ymin, width, n_boxes, n_interp, dout.p));
+    dout.to_host(out.mutable_data());
+    return out;
+}
+
+arr<float> scatter_padded_2d(arr<float> Y, float xmin, float ymin, float width, int n_boxes, int n_interp, int M, int,
+                             arr<float> out) {
+    require(Y.ndim() == 2 && Y.shape(1) == 2, "Y must be (n, 2)");
+    const int64_t n = Y.shape(0);
+    require(n_interp >= 1 && n_interp <= 8 && M >= (int64_t)n_boxes * n_interp && width > 0, "bad grid parameters");
+    if (out.size() == 0) out = arr<float>({(int64_t)3, (int64_t)M, (int64_t)M});
+    require(out.ndim() == 3 && out.shape(0) == 3 && out.shape(1) == M && out.shape(2) == M, "out must be (3, M, M)");
+    Dev dy = up(Y), dout((size_t)3 * M * M * sizeof(float));
+    {
+        py::gil_scoped_release rel;
+        check(mg_scatter_padded_2d(MG_F32, dy.p, n, xmin, ymin, width, n_boxes, n_interp, M, dout.p));
+        dout.to_host(out.mutable_data());
+    }
+    return out;
+}
+
+arr<float> spectral_products(arr<float> K1h, arr<float> K2h, arr<float> Wh, int) {
+    require(K1h.ndim() == 3 && K1h.shape(2) == 2 && Wh.ndim() == 4 && Wh.shape(0) == 3 && Wh.shape(3) == 2, "bad shapes");
+    const int64_t M = K1h.shape(0), Mh = K1h.shape(1);
+    require(K2h.ndim() == 3 && K2h.shape(0) == M && K2h.shape(1) == Mh && K2h.shape(2) == 2 && Wh.shape(1) == M &&
+            Wh.shape(2) == Mh, "bad shapes");
+    arr<float> out({(int64_t)4, M, Mh, (int64_t)2});
+    Dev d1 = up(K1h), d2 = up(K2h), dw = up(Wh), dout((size_t)8 * M * Mh * sizeof(float));
+    py::gil_scoped_release rel;
+    check(mg_spectral_products(MG_F32, d1.p, d2.p, dw.p, M, Mh, dout.p));
+    dout.to_host(out.mutable_data());
+    return out;
+}
+
+std::tuple<float, arr<float>> gather_forces_2d(arr<float> Y, arr<float> phi, float xmin, float ymin, float width, int n_boxes,
+                                               int n_interp, int) {
+    require(Y.ndim() == 2 && Y.shape(1) == 2 && phi.ndim() == 3 && phi.shape(0) == 4, "bad shapes");
+    const int64_t n = Y.shape(0), M = phi.shape(1);
+    require(phi.shape(2) == M && M >= (int64_t)n_boxes * n_interp, "phi must be (4, M, M) with M >= N");
+    arr<float> F({n, (int64_t)2});
+    double sq = -(double)n;
+    if (n > 0) {
+        Dev dy = up(Y), dp = up(phi), df((size_t)n * 2 * sizeof(float));
+        py::gil_scoped_release rel;
+        check(mg_gather_forces_2d(MG_F32, dy.p, n, dp.p, (int)M, xmin, ymin, width, n_boxes, n_interp, df.p, &sq));
+        df.to_host(F.mutable_data());
+    }
+    return std::make_tuple((float)sq, F);
+}
+
 // ------------------------------------------------------------------------------------- FFTConv2D
 class FFTConv2D {
 public:
@@ -266,6 +343,19 @@ PYBIND11_MODULE(_magnet, m) {
     m.def("mid_spring", &mid_spring, "grad += w (Y_i - mean of partner positions)", py::arg("Y"), py::arg("pairs"),
           py::arg("grad"), py::arg("w"), py::arg("n_threads") = -1);
     m.def("bbox_2d", &bbox_2d, "Bounding box of a (n, 2) array", py::arg("Y"), py::arg("n_threads") = -1);
+    m.def("interp_scatter", &interp_scatter, "Spread point charges on an interpolation grid (2-D)", py::arg("Y"),
+          py::arg("charges"), py::arg("xmin"), py::arg("ymin"), py::arg("width"), py::arg("n_boxes"), py::arg("n_interp"),
+          py::arg("n_threads") = -1);
+    m.def("interp_gather", &interp_gather, "Read grid potentials back at the points (2-D)", py::arg("Y"), py::arg("grid"),
+          py::arg("xmin"), py::arg("ymin"), py::arg("width"), py::arg("n_boxes"), py::arg("n_interp"), py::arg("n_threads") = -1);
+    m.def("scatter_padded_2d", &scatter_padded_2d, "Charges (1, x, y) on a zero-padded M x M grid", py::arg("Y"),
+          py::arg("xmin"), py::arg("ymin"), py::arg("width"), py::arg("n_boxes"), py::arg("n_interp"), py::arg("M"),
+          py::arg("n_threads") = -1, py::arg("out") = arr<float>(0));
+    m.def("spectral_products", &spectral_products, "Kernel-spectrum products of the FFT repulsion", py::arg("K1h"),
+          py::arg("K2h"), py::arg("Wh"), py::arg("n_threads") = -1);
+    m.def("gather_forces_2d", &gather_forces_2d, "Potentials at the points and repulsive forces: (sumQ, F)", py::arg("Y"),
+          py::arg("phi"), py::arg("xmin"), py::arg("ymin"), py::arg("width"), py::arg("n_boxes"), py::arg("n_interp"),
+          py::arg("n_threads") = -1);
     m.def("tsne_optimize", &tsne_optimize,
           "Whole t-SNE loop on the device (float32 or float64 state, in place); returns sumQ per iteration",
           py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("Y"), py::arg("update"), py::arg("gains"),

@@ -17,6 +17,12 @@
 namespace mg {
 
 int update_blocks(i64 n);
+size_t shim_scratch_ints(i64 n, int nb);
+template <class T> void launch_interp_scatter(const T*, i64, const T*, int, double, double, double, int, int, T*, int*, hipStream_t);
+template <class T> void launch_scatter_padded(const T*, i64, double, double, double, int, int, int, T*, int*, hipStream_t);
+template <class T> void launch_interp_gather(const T*, i64, const T*, int, double, double, double, int, int, T*, hipStream_t);
+template <class T> int launch_gather_forces(const T*, i64, const T*, int, double, double, double, int, int, T*, double*, hipStream_t);
+template <class T> void launch_spectral_products(const T*, const T*, const T*, i64, i64, T*, hipStream_t);
 
 static thread_local std::string g_err;
 static hipStream_t g_stream = nullptr;
@@ -631,6 +637,74 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
         MG_HIP(hipStreamSynchronize(g_stream));
     };
     if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
+    MG_API_END
+}
+
+// ---- fallback kernels
+int mg_interp_scatter(int prec, const void* Y, int64_t n, const void* q, int C, double xmin, double ymin, double width,
+                      int nb, int ni, void* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(n >= 0 && (n == 0 || (Y && q)) && out, "null array");
+    require(nb >= 1 && ni >= 1 && ni <= 8, "bad grid parameters");
+    DevBuf ws;
+    int* w = reinterpret_cast<int*>(ws.ensure(shim_scratch_ints(n, nb) * sizeof(int)));
+    if (prec == MG_F64) launch_interp_scatter<double>((const double*)Y, n, (const double*)q, C, xmin, ymin, width, nb, ni, (double*)out, w, g_stream);
+    else launch_interp_scatter<float>((const float*)Y, n, (const float*)q, C, xmin, ymin, width, nb, ni, (float*)out, w, g_stream);
+    MG_HIP(hipStreamSynchronize(g_stream));
+    MG_API_END
+}
+
+int mg_interp_gather(int prec, const void* Y, int64_t n, const void* G, int C, double xmin, double ymin, double width,
+                     int nb, int ni, void* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(n >= 0 && (n == 0 || (Y && G && out)), "null array");
+    if (prec == MG_F64) launch_interp_gather<double>((const double*)Y, n, (const double*)G, C, xmin, ymin, width, nb, ni, (double*)out, g_stream);
+    else launch_interp_gather<float>((const float*)Y, n, (const float*)G, C, xmin, ymin, width, nb, ni, (float*)out, g_stream);
+    MG_API_END
+}
+
+int mg_scatter_padded_2d(int prec, const void* Y, int64_t n, double xmin, double ymin, double width, int nb, int ni, int M,
+                         void* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(n >= 0 && (n == 0 || Y) && out, "null array");
+    require(nb >= 1 && ni >= 1 && ni <= 8, "bad grid parameters");
+    DevBuf ws;
+    int* w = reinterpret_cast<int*>(ws.ensure(shim_scratch_ints(n, nb) * sizeof(int)));
+    if (prec == MG_F64) launch_scatter_padded<double>((const double*)Y, n, xmin, ymin, width, nb, ni, M, (double*)out, w, g_stream);
+    else launch_scatter_padded<float>((const float*)Y, n, xmin, ymin, width, nb, ni, M, (float*)out, w, g_stream);
+    MG_HIP(hipStreamSynchronize(g_stream));
+    MG_API_END
+}
+
+int mg_spectral_products(int prec, const void* K1, const void* K2, const void* W, int64_t M, int64_t Mh, void* out) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(M >= 0 && Mh >= 0, "bad shapes");
+    require(M * Mh == 0 || (K1 && K2 && W && out), "null array");
+    if (prec == MG_F64) launch_spectral_products<double>((const double*)K1, (const double*)K2, (const double*)W, M, Mh, (double*)out, g_stream);
+    else launch_spectral_products<float>((const float*)K1, (const float*)K2, (const float*)W, M, Mh, (float*)out, g_stream);
+    MG_API_END
+}
+
+int mg_gather_forces_2d(int prec, const void* Y, int64_t n, const void* phi, int M, double xmin, double ymin, double width,
+                        int nb, int ni, void* F, double* h_sumQ) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(n >= 0 && (n == 0 || (Y && phi && F)), "null array");
+    DevBuf part;
+    double* p = reinterpret_cast<double*>(part.ensure((size_t)std::max<i64>(1, (n + 255) / 256) * sizeof(double)));
+    int nparts;
+    if (prec == MG_F64) nparts = launch_gather_forces<double>((const double*)Y, n, (const double*)phi, M, xmin, ymin, width, nb, ni, (double*)F, p, g_stream);
+    else nparts = launch_gather_forces<float>((const float*)Y, n, (const float*)phi, M, xmin, ymin, width, nb, ni, (float*)F, p, g_stream);
+    std::vector<double> h((size_t)std::max(1, nparts), 0.0);
+    if (nparts > 0) MG_HIP(hipMemcpyAsync(h.data(), p, (size_t)nparts * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    MG_HIP(hipStreamSynchronize(g_stream));
+    double s = 0.0;
+    for (int k = 0; k < nparts; ++k) s += h[k];
+    if (h_sumQ) *h_sumQ = s - (double)n;
     MG_API_END
 }
 

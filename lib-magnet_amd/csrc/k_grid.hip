@@ -465,36 +465,53 @@ int sort_hist_blocks(i64 n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
 int spread_chunks(i64 n) { return (int)((n + SP_CHUNK - 1) / SP_CHUNK); }
 int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024); }
 
-template <class T, int DB> static int box_sort_db(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, int passes,
-                                                 hipStream_t st) {
+// LSD passes of DB-bit digits. Box keys computed from Y in the first pass (keys0 == nullptr), or precomputed
+// keys0 / vals0 (read by the first pass). Pass p writes to buffer (p + first) % 2 (0 = A, 1 = B), first = 0 when
+// keys come from Y and 1 when they are precomputed in A; returns the buffer holding the result.
+template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int* keysA,
+                                              int* valsA, int* keysB, int* valsB, int* hist, int passes, hipStream_t st) {
     const int nblocks = sort_hist_blocks(n);
-    int* tot = b.hist + (size_t)(1 << DB) * nblocks;
-    const int* kin = nullptr; const int* vin = nullptr;
+    int* tot = hist + (size_t)(1 << DB) * nblocks;
+    const int first = keys0 ? 1 : 0;
+    const int* kin = keys0; const int* vin = vals0;
     int cur = 0;
+    const GridT<T> gg = g ? *g : GridT<T>{};
     for (int p = 0; p < passes; ++p) {
         const int shift = DB * p;
-        int* kout = (p % 2 == 0) ? b.keysA : b.keysB;
-        int* vout = (p % 2 == 0) ? b.valsA : b.valsB;
-        if (p == 0) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
-        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, n, shift, b.hist, nblocks);
+        cur = (p + first) % 2;
+        int* kout = cur == 0 ? keysA : keysB;
+        int* vout = cur == 0 ? valsA : valsB;
+        const bool fromY = p == 0 && !keys0;
+        if (fromY) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks);
+        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks);
         MG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, b.hist, nblocks, tot);
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, hist, nblocks, tot);
         MG_HIP(hipGetLastError());
-        if (p == 0) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, tot, nblocks, kout, vout);
-        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, g, kin, vin, n, shift, b.hist, tot, nblocks, kout, vout);
+        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout);
         MG_HIP(hipGetLastError());
-        kin = kout; vin = vout; cur = p % 2;
+        kin = kout; vin = vout;
     }
     return cur;
+}
+
+template <class T> static int sort_keys_any(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int bits,
+                                            int* kA, int* vA, int* kB, int* vB, int* hist, hipStream_t st) {
+    if (bits <= 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 1, st);
+    if (bits <= 20) return sort_db<T, 10>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 9) / 10, st);
+    return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 7) / 8, st);
 }
 
 template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
     const long long nkeys = (long long)g.nb * g.nb;
     int bits = 0;
     while ((1ll << bits) < nkeys) ++bits;
-    if (bits <= 8) return box_sort_db<T, 8>(Y, n, g, b, 1, st);
-    if (bits <= 20) return box_sort_db<T, 10>(Y, n, g, b, (bits + 9) / 10, st);
-    return box_sort_db<T, 8>(Y, n, g, b, (bits + 7) / 8, st);
+    return sort_keys_any<T>(Y, &g, nullptr, nullptr, n, bits, b.keysA, b.valsA, b.keysB, b.valsB, b.hist, st);
+}
+
+// stable sort of precomputed keys (< 2^bits) with their values, both in A; hist as for the box sort
+template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* valsB, int* hist, i64 n, int bits, hipStream_t st) {
+    return sort_keys_any<T>(nullptr, nullptr, keysA, valsA, n, bits, keysA, valsA, keysB, valsB, hist, st);
 }
 
 template <class T, int NI, bool CEN> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
@@ -539,6 +556,7 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
 
 #define MG_INST(T) \
     template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t); \
+    template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
     template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
     template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t);
 MG_INST(float)

@@ -57,6 +57,11 @@ def lib():
             "mg_fftconv_set_kernels_tsne": ([vp, d, d], i32),
             "mg_fftconv_compute": ([vp, vp, i64, d, d, d, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_fftconv_grids": ([vp, vp, vp], i32),
+            "mg_interp_scatter": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp], i32),
+            "mg_interp_gather": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp], i32),
+            "mg_scatter_padded_2d": ([i32, vp, i64, d, d, d, i32, i32, i32, vp], i32),
+            "mg_spectral_products": ([i32, vp, vp, vp, i64, i64, vp], i32),
+            "mg_gather_forces_2d": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
                                   C.POINTER(C.c_double), CALLBACK, vp], i32),
         }
@@ -197,3 +202,45 @@ def profile_grid():
     check(lib().mg_profile_grid(C.byref(a), C.byref(b), C.byref(c)))
     return a.value, b.value, c.value
 
+
+# ---- fallback kernels (src/fftrep.cpp:51-125, 294-411) on device tensors
+def interp_scatter(Y, charges, xmin, ymin, width, n_boxes, n_interp):
+    import torch
+    N = n_boxes * n_interp
+    out = torch.empty((charges.shape[1], N, N), dtype=Y.dtype, device=Y.device)
+    check(lib().mg_interp_scatter(prec_of(Y), ptr(Y), Y.shape[0], ptr(charges), charges.shape[1], xmin, ymin, width,
+                                  n_boxes, n_interp, ptr(out)))
+    return out
+
+
+def interp_gather(Y, grid, xmin, ymin, width, n_boxes, n_interp):
+    import torch
+    out = torch.empty((Y.shape[0], grid.shape[0]), dtype=Y.dtype, device=Y.device)
+    check(lib().mg_interp_gather(prec_of(Y), ptr(Y), Y.shape[0], ptr(grid), grid.shape[0], xmin, ymin, width, n_boxes,
+                                 n_interp, ptr(out)))
+    return out
+
+
+def scatter_padded_2d(Y, xmin, ymin, width, n_boxes, n_interp, M):
+    import torch
+    out = torch.empty((3, M, M), dtype=Y.dtype, device=Y.device)
+    check(lib().mg_scatter_padded_2d(prec_of(Y), ptr(Y), Y.shape[0], xmin, ymin, width, n_boxes, n_interp, M, ptr(out)))
+    return out
+
+
+def spectral_products(K1h, K2h, Wh):
+    """float pairs: K (M, Mh, 2), Wh (3, M, Mh, 2) -> (4, M, Mh, 2)"""
+    import torch
+    M, Mh = K1h.shape[0], K1h.shape[1]
+    out = torch.empty((4, M, Mh, 2), dtype=K1h.dtype, device=K1h.device)
+    check(lib().mg_spectral_products(prec_of(K1h), ptr(K1h), ptr(K2h), ptr(Wh), M, Mh, ptr(out)))
+    return out
+
+
+def gather_forces_2d(Y, phi, xmin, ymin, width, n_boxes, n_interp):
+    import torch
+    F = torch.empty_like(Y)
+    s = C.c_double()
+    check(lib().mg_gather_forces_2d(prec_of(Y), ptr(Y), Y.shape[0], ptr(phi), phi.shape[1], xmin, ymin, width, n_boxes,
+                                    n_interp, ptr(F), C.byref(s)))
+    return s.value, F

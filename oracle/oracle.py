@@ -76,6 +76,13 @@ def lib():
             getattr(L, "or_mid_spring_" + suf).argtypes = [fp, _i32p, fp, C.c_int64, C.c_int, C.c_int, T]
             getattr(L, "or_optimize_" + suf).argtypes = [C.POINTER(Params), _i32p, _i32p, fp, C.c_int64, fp, fp, fp,
                                                         C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        for suf, fp, T in (("f64", _f64p, C.c_double), ("f32", _f32p, C.c_float)):
+            getattr(L, "or_interp_scatter_" + suf).argtypes = [fp, C.c_int64, fp, C.c_int, T, T, T, C.c_int, C.c_int, fp]
+            getattr(L, "or_interp_gather_" + suf).argtypes = [fp, C.c_int64, fp, C.c_int, T, T, T, C.c_int, C.c_int, fp]
+            getattr(L, "or_scatter_padded_" + suf).argtypes = [fp, C.c_int64, T, T, T, C.c_int, C.c_int, C.c_int, fp]
+            getattr(L, "or_spectral_products_" + suf).argtypes = [fp, fp, fp, C.c_int64, C.c_int64, fp]
+            getattr(L, "or_gather_forces_" + suf).argtypes = [fp, C.c_int64, fp, C.c_int, T, T, T, C.c_int, C.c_int, fp]
+            getattr(L, "or_gather_forces_" + suf).restype = C.c_double
         L.or_exact_repulsion.argtypes = [_f64p, C.c_int64, C.c_double, _f64p]
         L.or_exact_repulsion.restype = C.c_double
         L.or_lagrange_f64.argtypes = [C.c_double, C.c_int, _f64p]
@@ -173,3 +180,44 @@ def optimize(indptr, indices, data, Y, U=None, G=None, it0=0, it1=None, n_iter=7
                                                  len(Y), Y, U, G, pr.ctypes.data_as(C.c_void_p) if pr is not None else None,
                                                  pr.shape[1] if pr is not None else 0, it0, it1, sq.ctypes.data_as(C.c_void_p))
     return sq
+
+
+# ---- fallback kernels (src/fftrep.cpp:24-125, 294-411); arrays in Y's dtype
+def interp_scatter(Y, charges, xmin, ymin, width, n_boxes, n_interp):
+    """(C, N, N) charge grids, points summed in index order."""
+    Y = np.ascontiguousarray(Y); q = np.ascontiguousarray(charges, Y.dtype)
+    N = n_boxes * n_interp
+    out = np.zeros((q.shape[1], N, N), Y.dtype)
+    getattr(lib(), "or_interp_scatter_" + _suffix(Y))(Y, len(Y), q, q.shape[1], xmin, ymin, width, n_boxes, n_interp, out)
+    return out
+
+
+def interp_gather(Y, grid_, xmin, ymin, width, n_boxes, n_interp):
+    Y = np.ascontiguousarray(Y); G = np.ascontiguousarray(grid_, Y.dtype)
+    out = np.zeros((len(Y), G.shape[0]), Y.dtype)
+    getattr(lib(), "or_interp_gather_" + _suffix(Y))(Y, len(Y), G, G.shape[0], xmin, ymin, width, n_boxes, n_interp, out)
+    return out
+
+
+def scatter_padded_2d(Y, xmin, ymin, width, n_boxes, n_interp, M):
+    Y = np.ascontiguousarray(Y)
+    out = np.zeros((3, M, M), Y.dtype)
+    getattr(lib(), "or_scatter_padded_" + _suffix(Y))(Y, len(Y), xmin, ymin, width, n_boxes, n_interp, M, out)
+    return out
+
+
+def spectral_products(K1h, K2h, Wh):
+    """complex arrays as float pairs: K (M, Mh, 2), Wh (3, M, Mh, 2) -> (4, M, Mh, 2)"""
+    K1h = np.ascontiguousarray(K1h); K2h = np.ascontiguousarray(K2h, K1h.dtype); Wh = np.ascontiguousarray(Wh, K1h.dtype)
+    M, Mh = K1h.shape[0], K1h.shape[1]
+    out = np.zeros((4, M, Mh, 2), K1h.dtype)
+    getattr(lib(), "or_spectral_products_" + _suffix(K1h))(K1h, K2h, Wh, M, Mh, out)
+    return out
+
+
+def gather_forces_2d(Y, phi, xmin, ymin, width, n_boxes, n_interp):
+    """(sumQ, F) from the padded potentials (4, M, M)."""
+    Y = np.ascontiguousarray(Y); phi = np.ascontiguousarray(phi, Y.dtype)
+    F = np.zeros_like(Y)
+    s = getattr(lib(), "or_gather_forces_" + _suffix(Y))(Y, len(Y), phi, phi.shape[1], xmin, ymin, width, n_boxes, n_interp, F)
+    return s, F

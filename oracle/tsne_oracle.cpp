@@ -413,3 +413,144 @@ void or_optimize_f32(const or_params* p, const int32_t* ip, const int32_t* ix, c
 }
 
 }  // extern "C"
+
+// ============================================================== fallback kernels (src/fftrep.cpp)
+namespace {
+
+// product-form Lagrange weights of n_interp nodes at (k + 1/2) / n_interp (src/fftrep.cpp:24-35)
+template <class T> void lagrange_prod(T u, int ni, T* w) {
+    for (int k = 0; k < ni; ++k) {
+        const T uk = (k + T(0.5)) / ni;
+        T num = 1, den = 1;
+        for (int m = 0; m < ni; ++m) {
+            if (m == k) continue;
+            const T um = (m + T(0.5)) / ni;
+            num *= (u - um); den *= (uk - um);
+        }
+        w[k] = num / den;
+    }
+}
+
+// Box::locate (src/fftrep.cpp:38-45)
+template <class T> void locate(T x, T y, T xmin, T ymin, T width, int nb, int ni, int& bx, int& by, T* wx, T* wy) {
+    const T ux = (x - xmin) / width, uy = (y - ymin) / width;
+    bx = std::min(nb - 1, std::max(0, (int)ux));
+    by = std::min(nb - 1, std::max(0, (int)uy));
+    lagrange_prod<T>(std::min(T(1), std::max(T(0), ux - bx)), ni, wx);
+    lagrange_prod<T>(std::min(T(1), std::max(T(0), uy - by)), ni, wy);
+}
+
+// interp_scatter (src/fftrep.cpp:51-92), one thread: points in index order
+template <class T> void interp_scatter(const T* Y, int64_t n, const T* Cq, int C, T xmin, T ymin, T width, int nb, int ni, T* O) {
+    const int N = nb * ni;
+    std::fill(O, O + (size_t)C * N * N, T(0));
+    T wx[8], wy[8]; int bx, by;
+    for (int64_t i = 0; i < n; ++i) {
+        locate<T>(Y[2 * i], Y[2 * i + 1], xmin, ymin, width, nb, ni, bx, by, wx, wy);
+        for (int a = 0; a < ni; ++a) {
+            const int gx = bx * ni + a;
+            for (int b = 0; b < ni; ++b) {
+                const int gy = by * ni + b;
+                const T w = wx[a] * wy[b];
+                for (int c = 0; c < C; ++c) O[((size_t)c * N + gx) * N + gy] += w * Cq[i * C + c];
+            }
+        }
+    }
+}
+
+// interp_gather (src/fftrep.cpp:94-125)
+template <class T> void interp_gather(const T* Y, int64_t n, const T* G, int C, T xmin, T ymin, T width, int nb, int ni, T* O) {
+    const int N = nb * ni;
+    T wx[8], wy[8]; int bx, by;
+    for (int64_t i = 0; i < n; ++i) {
+        locate<T>(Y[2 * i], Y[2 * i + 1], xmin, ymin, width, nb, ni, bx, by, wx, wy);
+        for (int c = 0; c < C; ++c) O[i * C + c] = T(0);
+        for (int a = 0; a < ni; ++a) {
+            const int gx = bx * ni + a;
+            for (int b = 0; b < ni; ++b) {
+                const int gy = by * ni + b;
+                const T w = wx[a] * wy[b];
+                for (int c = 0; c < C; ++c) O[i * C + c] += w * G[((size_t)c * N + gx) * N + gy];
+            }
+        }
+    }
+}
+
+// scatter_padded_2d (src/fftrep.cpp:294-343), one thread: points in x-box counting-sort order
+template <class T> void scatter_padded(const T* Y, int64_t n, T xmin, T ymin, T width, int nb, int ni, int M, T* O) {
+    std::fill(O, O + (size_t)3 * M * M, T(0));
+    std::vector<int> bxs(n);
+    std::vector<int64_t> cnt(nb + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const T ux = (Y[2 * i] - xmin) / width;
+        bxs[i] = std::min(nb - 1, std::max(0, (int)ux));
+        ++cnt[bxs[i] + 1];
+    }
+    for (int b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    std::vector<int64_t> order(n), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < n; ++i) order[fill[bxs[i]]++] = i;
+    T wx[8], wy[8]; int bx, by;
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = order[k];
+        locate<T>(Y[2 * i], Y[2 * i + 1], xmin, ymin, width, nb, ni, bx, by, wx, wy);
+        const T ch[3] = {T(1), Y[2 * i], Y[2 * i + 1]};
+        for (int a = 0; a < ni; ++a) {
+            const int gx = bx * ni + a;
+            for (int b = 0; b < ni; ++b) {
+                const int gy = by * ni + b;
+                const T w = wx[a] * wy[b];
+                for (int c = 0; c < 3; ++c) O[((size_t)c * M + gx) * M + gy] += w * ch[c];
+            }
+        }
+    }
+}
+
+// spectral_products (src/fftrep.cpp:346-371)
+template <class T> void spectral_products(const T* K1, const T* K2, const T* W, int64_t M, int64_t Mh, T* O) {
+    const int64_t L = M * Mh;
+    for (int64_t k = 0; k < L; ++k) {
+        const T ar = K1[2 * k], ai = K1[2 * k + 1], br = W[2 * k], bi = W[2 * k + 1];
+        O[2 * k] = ar * br - ai * bi; O[2 * k + 1] = ar * bi + ai * br;
+        const T cr = K2[2 * k], ci = K2[2 * k + 1];
+        for (int c = 0; c < 3; ++c) {
+            const T wr = W[2 * (c * L + k)], wi = W[2 * (c * L + k) + 1];
+            O[2 * ((c + 1) * L + k)] = cr * wr - ci * wi; O[2 * ((c + 1) * L + k) + 1] = cr * wi + ci * wr;
+        }
+    }
+}
+
+// gather_forces_2d (src/fftrep.cpp:373-411)
+template <class T> double gather_forces(const T* Y, int64_t n, const T* phi, int M, T xmin, T ymin, T width, int nb, int ni, T* F) {
+    T wx[8], wy[8]; int bx, by;
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        locate<T>(Y[2 * i], Y[2 * i + 1], xmin, ymin, width, nb, ni, bx, by, wx, wy);
+        T pot[4] = {0, 0, 0, 0};
+        for (int a = 0; a < ni; ++a) {
+            const int64_t gx = bx * ni + a;
+            for (int b = 0; b < ni; ++b) {
+                const int64_t gy = by * ni + b;
+                const T w = wx[a] * wy[b];
+                for (int c = 0; c < 4; ++c) pot[c] += w * phi[((size_t)c * M + gx) * M + gy];
+            }
+        }
+        s += pot[0];
+        F[2 * i] = pot[1] * Y[2 * i] - pot[2];
+        F[2 * i + 1] = pot[1] * Y[2 * i + 1] - pot[3];
+    }
+    return s - (double)n;
+}
+
+}  // namespace
+
+extern "C" {
+#define OR_SHIMS(T, suf) \
+    void or_interp_scatter_##suf(const T* Y, int64_t n, const T* q, int C, T x0, T y0, T w, int nb, int ni, T* O) { interp_scatter<T>(Y, n, q, C, x0, y0, w, nb, ni, O); } \
+    void or_interp_gather_##suf(const T* Y, int64_t n, const T* G, int C, T x0, T y0, T w, int nb, int ni, T* O) { interp_gather<T>(Y, n, G, C, x0, y0, w, nb, ni, O); } \
+    void or_scatter_padded_##suf(const T* Y, int64_t n, T x0, T y0, T w, int nb, int ni, int M, T* O) { scatter_padded<T>(Y, n, x0, y0, w, nb, ni, M, O); } \
+    void or_spectral_products_##suf(const T* K1, const T* K2, const T* W, int64_t M, int64_t Mh, T* O) { spectral_products<T>(K1, K2, W, M, Mh, O); } \
+    double or_gather_forces_##suf(const T* Y, int64_t n, const T* phi, int M, T x0, T y0, T w, int nb, int ni, T* F) { return gather_forces<T>(Y, n, phi, M, x0, y0, w, nb, ni, F); }
+OR_SHIMS(double, f64)
+OR_SHIMS(float, f32)
+#undef OR_SHIMS
+}  // extern "C"

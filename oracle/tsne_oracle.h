@@ -8,6 +8,8 @@
 //   update / step        src/fftrep.cpp:181-266
 //   mid-range spring     src/fftrep.cpp:270-287
 //   optimiser schedule   magnet/tsne.py:132-214
+//   fallback kernels     src/fftrep.cpp:24-125, 294-411 (interp_scatter / interp_gather, scatter_padded_2d,
+//                        spectral_products, gather_forces_2d)
 // Every function exists in a double (_f64) and a float (_f32) form. The f64 form is the reference the
 // FP64 GPU path is checked against; the f32 form follows the reference's float32 arithmetic order and
 // is checked against vectors written by the compiled reference.
@@ -81,6 +83,33 @@ void or_optimize_f32(const or_params* p, const int32_t* indptr, const int32_t* i
 
 // Lagrange weights of n_interp nodes at (k + 1/2) / n_interp for local coordinate u (src/fftconv.cpp:24-40).
 void or_lagrange_f64(double u, int n_interp, double* w);
+
+// ---- fallback kernels (src/fftrep.cpp), in the order of the reference at n_threads = 1. Box of a point:
+// u = (x - xmin) / width, clamped to [0, n_boxes - 1]; weights from the product-form Lagrange polynomials
+// at the clamped local coordinate (src/fftrep.cpp:24-45).
+// interp_scatter: Y (n, 2), charges (n, C) -> out (C, N, N), N = n_boxes * n_interp (zeroed here).
+void or_interp_scatter_f64(const double* Y, int64_t n, const double* charges, int C, double xmin, double ymin, double width,
+                           int n_boxes, int n_interp, double* out);
+void or_interp_scatter_f32(const float* Y, int64_t n, const float* charges, int C, float xmin, float ymin, float width,
+                           int n_boxes, int n_interp, float* out);
+// interp_gather: grid (C, N, N) -> out (n, C)
+void or_interp_gather_f64(const double* Y, int64_t n, const double* grid, int C, double xmin, double ymin, double width,
+                          int n_boxes, int n_interp, double* out);
+void or_interp_gather_f32(const float* Y, int64_t n, const float* grid, int C, float xmin, float ymin, float width,
+                          int n_boxes, int n_interp, float* out);
+// scatter_padded_2d: charges (1, x, y) on the zero-padded (3, M, M) grid (zeroed here)
+void or_scatter_padded_f64(const double* Y, int64_t n, double xmin, double ymin, double width, int n_boxes, int n_interp,
+                           int M, double* out);
+void or_scatter_padded_f32(const float* Y, int64_t n, float xmin, float ymin, float width, int n_boxes, int n_interp,
+                           int M, float* out);
+// spectral_products: K1h, K2h (M, Mh) complex, Wh (3, M, Mh) complex -> out (4, M, Mh) complex (interleaved)
+void or_spectral_products_f64(const double* K1h, const double* K2h, const double* Wh, int64_t M, int64_t Mh, double* out);
+void or_spectral_products_f32(const float* K1h, const float* K2h, const float* Wh, int64_t M, int64_t Mh, float* out);
+// gather_forces_2d: phi (4, M, M) -> F (n, 2); returns sumQ = sum pot0 - n (double accumulation)
+double or_gather_forces_f64(const double* Y, int64_t n, const double* phi, int M, double xmin, double ymin, double width,
+                            int n_boxes, int n_interp, double* F);
+double or_gather_forces_f32(const float* Y, int64_t n, const float* phi, int M, float xmin, float ymin, float width,
+                            int n_boxes, int n_interp, float* F);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,28 @@ def main():
         if it in (0, 9, 39):
             out[f"traj_Y{it + 1}"] = Yt.copy()
     out["traj_sumQ"] = np.array(sq)
+
+    # fallback kernels (src/fftrep.cpp:51-125, 294-411), one thread
+    rng = np.random.default_rng(31)
+    Ys = np.ascontiguousarray(rng.normal(0, 2.0, (1500, 2)).astype(np.float32))
+    xmin, ymin, width, nb = grid(R, Ys)
+    q = np.ascontiguousarray(np.column_stack([np.ones(1500), Ys[:, 0], rng.normal(0, 1, 1500)]).astype(np.float32))
+    out["shim_Y"], out["shim_q"], out["shim_grid"] = Ys, q, np.array([xmin, ymin, width, nb], np.float64)
+    out["shim_scatter"] = np.asarray(R.interp_scatter(Ys, q, xmin, ymin, width, nb, 3, 1))
+    Gg = np.ascontiguousarray(rng.normal(0, 1, out["shim_scatter"].shape).astype(np.float32))
+    out["shim_G"], out["shim_gather"] = Gg, np.asarray(R.interp_gather(Ys, Gg, xmin, ymin, width, nb, 3, 1))
+    Mp = 3 * nb + 4
+    out["shim_M"] = np.array(Mp)
+    out["shim_padded"] = np.asarray(R.scatter_padded_2d(Ys, xmin, ymin, width, nb, 3, Mp, 1))
+    phi = np.ascontiguousarray(rng.normal(0, 1, (4, Mp, Mp)).astype(np.float32))
+    sQ, Fs = R.gather_forces_2d(Ys, phi, xmin, ymin, width, nb, 3, 1)
+    out["shim_phi"], out["shim_sumQ"], out["shim_F"] = phi, np.float64(sQ), np.asarray(Fs)
+    Ms, Msh = 24, 13
+    K1 = np.ascontiguousarray(rng.normal(0, 1, (Ms, Msh, 2)).astype(np.float32))
+    K2 = np.ascontiguousarray(rng.normal(0, 1, (Ms, Msh, 2)).astype(np.float32))
+    Wh = np.ascontiguousarray(rng.normal(0, 1, (3, Ms, Msh, 2)).astype(np.float32))
+    out["shim_K1"], out["shim_K2"], out["shim_Wh"] = K1, K2, Wh
+    out["shim_products"] = np.asarray(R.spectral_products(K1, K2, Wh, 1))
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
 

@@ -22,7 +22,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_pybind_module_keeps_reference_names():
     import _magnet
-    for name in ("FFTConv2D", "tsne_attraction", "tsne_step", "tsne_update", "mid_spring", "bbox_2d", "tsne_optimize"):
+    for name in ("FFTConv2D", "tsne_attraction", "tsne_step", "tsne_update", "mid_spring", "bbox_2d", "tsne_optimize",
+                 "interp_scatter", "interp_gather", "scatter_padded_2d", "spectral_products", "gather_forces_2d"):
         assert hasattr(_magnet, name), name
     assert hasattr(_magnet.FFTConv2D, "set_kernels") and hasattr(_magnet.FFTConv2D, "compute")
 
@@ -38,6 +39,14 @@ def test_shape_errors_raise_before_any_device_call():
                             np.zeros((4, 2), np.float32), 1.0, 0.5)
     with pytest.raises(RuntimeError, match="Y must be \\(n, 2\\)"):
         _magnet.bbox_2d(np.zeros((0, 2), np.float32))
+    with pytest.raises(RuntimeError, match="charges must be \\(n, T\\)"):
+        _magnet.interp_scatter(np.zeros((4, 2), np.float32), np.zeros((3, 2), np.float32), 0.0, 0.0, 0.7, 20, 3)
+    with pytest.raises(RuntimeError, match="grid must be \\(T, N, N\\)"):
+        _magnet.interp_gather(np.zeros((4, 2), np.float32), np.zeros((2, 59, 60), np.float32), 0.0, 0.0, 0.7, 20, 3)
+    with pytest.raises(RuntimeError, match="bad grid parameters"):
+        _magnet.scatter_padded_2d(np.zeros((4, 2), np.float32), 0.0, 0.0, 0.7, 20, 3, 50)
+    with pytest.raises(RuntimeError, match="bad shapes"):
+        _magnet.spectral_products(np.zeros((8, 5, 2), np.float32), np.zeros((8, 4, 2), np.float32), np.zeros((3, 8, 5, 2), np.float32))
 
 
 def test_grid_decision_matches_oracle():

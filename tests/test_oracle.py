@@ -67,6 +67,41 @@ def test_trajectory_matches_reference_float32(golden):
     assert np.abs(sq2 / g["traj_sumQ"][10:] - 1).max() < 1e-3
 
 
+def test_fallback_kernels_bit_exact_to_reference(golden):
+    """interp_scatter / interp_gather / scatter_padded_2d / spectral_products / gather_forces_2d
+    (src/fftrep.cpp:51-125, 294-411) against the reference's own outputs at n_threads = 1"""
+    g = golden
+    Y, q = g["shim_Y"], g["shim_q"]
+    xmin, ymin, width, nb = [float(v) for v in g["shim_grid"][:3]] + [int(g["shim_grid"][3])]
+    x0, y0, w = np.float32(xmin), np.float32(ymin), np.float32(width)
+    assert np.array_equal(O.interp_scatter(Y, q, x0, y0, w, nb, 3), g["shim_scatter"])
+    assert np.array_equal(O.interp_gather(Y, g["shim_G"], x0, y0, w, nb, 3), g["shim_gather"])
+    assert np.array_equal(O.scatter_padded_2d(Y, x0, y0, w, nb, 3, int(g["shim_M"])), g["shim_padded"])
+    assert np.array_equal(O.spectral_products(g["shim_K1"], g["shim_K2"], g["shim_Wh"]), g["shim_products"])
+    s, F = O.gather_forces_2d(Y, g["shim_phi"], x0, y0, w, nb, 3)
+    assert np.array_equal(F, g["shim_F"]) and np.float32(s) == np.float32(g["shim_sumQ"])
+
+
+def test_fallback_kernels_math():
+    """scatter: conservation of each charge column; gather is its adjoint; the padded scatter is the
+    (1, x, y) scatter placed in the corner; products are complex products"""
+    rng = np.random.default_rng(3)
+    Y = rng.normal(0, 2, (700, 2))
+    q = rng.normal(0, 1, (700, 4))
+    args = (-7.0, -7.5, 0.7, 22, 3)
+    Wg = O.interp_scatter(Y, q, *args)
+    assert np.allclose(Wg.reshape(4, -1).sum(1), q.sum(0), rtol=1e-12, atol=1e-10)
+    G = rng.normal(0, 1, Wg.shape)
+    assert abs((Wg * G).sum() - (q * O.interp_gather(Y, G, *args)).sum()) < 1e-9 * np.abs(Wg * G).sum()
+    P = O.scatter_padded_2d(Y, *args, 70)
+    W3 = O.interp_scatter(Y, np.column_stack([np.ones(700), Y]), *args)
+    assert np.allclose(P[:, :66, :66], W3, rtol=1e-12, atol=1e-12) and not P[:, 66:, :].any() and not P[:, :, 66:].any()
+    K1, K2, Wh = rng.normal(0, 1, (10, 6, 2)), rng.normal(0, 1, (10, 6, 2)), rng.normal(0, 1, (3, 10, 6, 2))
+    c = lambda a: a[..., 0] + 1j * a[..., 1]  # noqa: E731
+    out = c(O.spectral_products(K1, K2, Wh))
+    assert np.allclose(out[0], c(K1) * c(Wh)[0]) and np.allclose(out[1:], c(K2)[None] * c(Wh))
+
+
 _REF_CHECK = r"""
 import sys, numpy as np, scipy.sparse as sp
 sys.path.insert(0, sys.argv[1])
