@@ -114,6 +114,18 @@ int mg_fftconv_compute(mg_fftconv* conv, const void* d_Y, int64_t n, double xmin
  * (interleaved, after the 1/M^2 scale). Either pointer may be NULL. */
 int mg_fftconv_grids(mg_fftconv* conv, void* d_W, void* d_phi);
 
+/* Out-of-sample placement (MAGNET.transform, magnet/__init__.py:876-930, in the full-batch t-SNE form): m new points
+ * Y_new (m, 2, initialised by the caller) move for n_iter iterations under the attraction of their k training
+ * neighbours (nbr (m, k) int32 indices into Y_train, weights w (m, k) with rows summing to one, scaled by 1/n_train
+ * like the rows of the training P) and the repulsion of the training layout, read from the potential field `conv`
+ * holds from its last mg_fftconv_compute (call it on Y_train first); gradient rho A - F/Z with Z the training
+ * layout's, gains and momentum as in the training update (momentum0 before iteration momentum_switch, then
+ * momentum1). The training layout is not moved, and every new point is independent of the others: a point's
+ * result does not depend on which points share the call. */
+int mg_tsne_transform(mg_fftconv* conv, const void* d_Ytrain, int64_t n_train, const int32_t* d_nbr, const void* d_w,
+                      int k, void* d_Ynew, int64_t m, int n_iter, double learning_rate, double momentum0, double momentum1,
+                      int momentum_switch, double exaggeration, double dof);
+
 /* ---- fallback kernels (src/fftrep.cpp:51-125, 294-411; magnet/tsne.py:92-129 calls them when FFTConv2D is not
  * used). Box of a point: u = (x - xmin) / width clamped to [0, n_boxes - 1], product-form Lagrange weights;
  * N = n_boxes * n_interp. Sums run in the reference's single-thread order without fused multiply-adds, so the

@@ -121,6 +121,11 @@ template <class T> void launch_spread(const int* keys, const int* vals, const T*
 template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
                                       bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
 int gather_blocks(i64 n);
+// out-of-sample placement of m new points against a fixed training layout and its potential field phi
+template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_t* nbr, const T* w, int k, T* Yn, i64 m,
+                                         const GridT<T>& g, int ni, bool centred, const T* phi, double inv_Z, int n_iter,
+                                         double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
+                                         hipStream_t st);
 
 // FFT stages. M = 2^s or 3 * 2^s, 128 <= M <= MAX_M, N <= M / 2.
 void fft_init_twiddles();   // uploads the twiddle tables once per device

@@ -183,6 +183,7 @@ template <class T> struct Repulsion {
     DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, Tr, phi;
     DevBuf zpart, counter, zout, inv_out, spec_tmp, table;
     int last_N = 0, last_M = 0, sorted_buf = 0;
+    GridT<T> last_g{};
     i64 last_n = 0;
 
     Repulsion(int M, int ni_) : ni(ni_), M_user(M) {
@@ -313,7 +314,7 @@ template <class T> struct Repulsion {
         P.begin(ST_GATHER, st);
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
         P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
-        last_N = g.N; last_M = M; last_n = n;
+        last_N = g.N; last_M = M; last_n = n; last_g = g;
         P.grid(g.N, M);
     }
 };
@@ -637,6 +638,27 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
         MG_HIP(hipStreamSynchronize(g_stream));
     };
     if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
+    MG_API_END
+}
+
+int mg_tsne_transform(mg_fftconv* c, const void* Ytr, int64_t n_train, const int32_t* nbr, const void* w, int k, void* Yn,
+                      int64_t m, int n_iter, double lr, double mom0, double mom1, int mom_switch, double exaggeration, double dof) {
+    MG_API_BEGIN
+    require(c != nullptr, "null object");
+    require(n_train >= 1 && k >= 1 && m >= 0 && n_iter >= 0, "bad sizes");
+    require(Ytr && (m == 0 || (nbr && w && Yn)), "null array");
+    require(lr > 0 && dof > 0, "learning_rate and dof must be positive");
+    auto run = [&](auto& r, auto zero) {
+        typedef decltype(zero) T;
+        require(r.last_N > 0, "the field has not been computed: call mg_fftconv_compute on the training layout first");
+        double z[2];
+        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, g_stream));
+        MG_HIP(hipStreamSynchronize(g_stream));
+        launch_transform<T>((const T*)Ytr, n_train, nbr, (const T*)w, k, (T*)Yn, m, r.last_g, r.ni, r.centred, dptr<T>(r.phi), z[1],
+                            n_iter, lr, mom0, mom1, mom_switch, exaggeration, dof, g_stream);
+        MG_HIP(hipStreamSynchronize(g_stream));
+    };
+    if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));
     MG_API_END
 }
 

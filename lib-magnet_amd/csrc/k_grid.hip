@@ -554,11 +554,111 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
     MG_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------- out-of-sample placement
+// One thread per new point, all iterations in one launch (the points do not interact): the gradient
+// rho * A_i - F_i / Z with A_i = sum_j (w_ij / n_train) q_ij (y_i - y_j) over the point's k training neighbours
+// (w rows sum to one: the rows of the training P sum to about 1 / n_train) and F_i the repulsion of the fixed
+// training layout, read from its potential field phi (4, N, N) exactly as the gather reads it (box clamped to the
+// grid, local coordinate clamped to [0, 1]); Z the training layout's. Gains, momentum and step as in the training
+// update (src/fftrep.cpp:245-255); no recentring (the training layout fixes the frame).
+template <class T, int NI, bool CEN>
+__global__ __launch_bounds__(TPB) void k_transform(const T* __restrict__ Ytr, T inv_ntr, const int32_t* __restrict__ nbr,
+                                                   const T* __restrict__ w, int k, T* __restrict__ Yn, i64 m, GridT<T> g,
+                                                   const T* __restrict__ phi, T inv_Z, int n_iter, T lr, T mom0, T mom1,
+                                                   int mom_switch, T rho, T inv_dof) {
+    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    if (i >= m) return;
+    typedef typename vec2<T>::type V2;
+    const V2* Y2 = reinterpret_cast<const V2*>(Ytr);
+    const size_t NN = (size_t)g.N * g.N;
+    V2 y = reinterpret_cast<V2*>(Yn)[i];
+    T ux = 0, uy = 0, gx_ = 1, gy_ = 1;
+    for (int it = 0; it < n_iter; ++it) {
+        T ax = 0, ay = 0;
+        for (int j = 0; j < k; ++j) {
+            const int t = nbr[i * k + j];
+            const V2 yj = Y2[t];
+            const T dx = y.x - yj.x, dy = y.y - yj.y;
+            const T q = (w[i * k + j] * inv_ntr) / (T(1) + (dx * dx + dy * dy) * inv_dof);
+            ax += q * dx; ay += q * dy;
+        }
+        int bx, by;
+        T wx[NI], wy[NI];
+        double fxu, fyu;
+        weights<T, NI, CEN>(y.x, y.y, box_key<T>(y.x, y.y, g), g, bx, by, wx, wy, fxu, fyu);
+        T fx, fy;
+        if (CEN) {
+            double sx = 0, sy = 0;
+#pragma unroll
+            for (int a = 0; a < NI; ++a) {
+                const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+                const double ex = node_offset<NI>(fxu, g.width_d, a);
+#pragma unroll
+                for (int b = 0; b < NI; ++b) {
+                    const double wv = (double)(wx[a] * wy[b]);
+                    const size_t o = row + b;
+                    const double ey = node_offset<NI>(fyu, g.width_d, b);
+                    sx += wv * ((double)phi[NN + o] * ex + (double)phi[2 * NN + o]);
+                    sy += wv * ((double)phi[NN + o] * ey + (double)phi[3 * NN + o]);
+                }
+            }
+            fx = (T)sx; fy = (T)sy;
+        } else {
+            T p1 = 0, p2 = 0, p3 = 0;
+#pragma unroll
+            for (int a = 0; a < NI; ++a) {
+                const size_t row = (size_t)(bx * NI + a) * g.N + by * NI;
+#pragma unroll
+                for (int b = 0; b < NI; ++b) {
+                    const T wv = wx[a] * wy[b];
+                    const size_t o = row + b;
+                    p1 += wv * phi[NN + o]; p2 += wv * phi[2 * NN + o]; p3 += wv * phi[3 * NN + o];
+                }
+            }
+            fx = p1 * y.x - p2; fy = p1 * y.y - p3;
+        }
+        const T mom = it < mom_switch ? mom0 : mom1;
+        const T g0 = rho * ax - fx * inv_Z, g1 = rho * ay - fy * inv_Z;
+        // delta-bar-delta gains and momentum (src/fftrep.cpp:245-255)
+        T gg = ((g0 > T(0)) == (ux > T(0))) ? gx_ * T(0.8) : gx_ + T(0.2);
+        gx_ = gg < T(0.01) ? T(0.01) : gg;
+        ux = mom * ux - lr * gx_ * g0;
+        gg = ((g1 > T(0)) == (uy > T(0))) ? gy_ * T(0.8) : gy_ + T(0.2);
+        gy_ = gg < T(0.01) ? T(0.01) : gg;
+        uy = mom * uy - lr * gy_ * g1;
+        y.x += ux; y.y += uy;
+    }
+    reinterpret_cast<V2*>(Yn)[i] = y;
+}
+
+template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_t* nbr, const T* w, int k, T* Yn, i64 m,
+                                         const GridT<T>& g, int ni, bool centred, const T* phi, double inv_Z, int n_iter,
+                                         double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
+                                         hipStream_t st) {
+    if (m == 0) return;
+    const unsigned blocks = (unsigned)((m + TPB - 1) / TPB);
+    const T inv_ntr = (T)(1.0 / (double)n_train);
+#define MG_T(NI) do { \
+        if (centred) hipLaunchKernelGGL((k_transform<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, Ytr, inv_ntr, nbr, w, k, Yn, m, g, phi, \
+                                        (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof)); \
+        else hipLaunchKernelGGL((k_transform<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, Ytr, inv_ntr, nbr, w, k, Yn, m, g, phi, \
+                                (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof)); \
+    } while (0)
+    switch (ni) {
+        case 1: MG_T(1); break; case 2: MG_T(2); break; case 3: MG_T(3); break; case 4: MG_T(4); break;
+        case 5: MG_T(5); break; case 6: MG_T(6); break; case 7: MG_T(7); break; case 8: MG_T(8); break;
+        default: require(false, "n_interp must be in [1, 8]");
+    }
+#undef MG_T
+    MG_HIP(hipGetLastError());
+}
+
 #define MG_INST(T) \
     template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t); \
     template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
     template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
-    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t);
+    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t); \
+    template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, hipStream_t);
 MG_INST(float)
 MG_INST(double)
 

@@ -57,6 +57,7 @@ def lib():
             "mg_fftconv_set_kernels_tsne": ([vp, d, d], i32),
             "mg_fftconv_compute": ([vp, vp, i64, d, d, d, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_fftconv_grids": ([vp, vp, vp], i32),
+            "mg_tsne_transform": ([vp, vp, i64, vp, vp, i32, vp, i64, i32, d, d, d, i32, d, d], i32),
             "mg_interp_scatter": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp], i32),
             "mg_interp_gather": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp], i32),
             "mg_scatter_padded_2d": ([i32, vp, i64, d, d, d, i32, i32, i32, vp], i32),
@@ -151,6 +152,13 @@ class FFTConv:
         s = C.c_double()
         check(lib().mg_fftconv_compute(self.h, ptr(Y), Y.shape[0], xmin, ymin, width, n_boxes, ptr(F), C.byref(s)))
         return s.value, F
+
+    def transform(self, Y_train, nbr, w, Y_new, n_iter=100, learning_rate=200.0, momentum=(0.5, 0.8), momentum_switch=25,
+                  exaggeration=1.0, dof=1.0):
+        """In place on Y_new (m, 2): mg_tsne_transform against the field of the last compute (on Y_train)."""
+        check(lib().mg_tsne_transform(self.h, ptr(Y_train), Y_train.shape[0], ptr(nbr), ptr(w), nbr.shape[1], ptr(Y_new),
+                                      Y_new.shape[0], n_iter, learning_rate, momentum[0], momentum[1], momentum_switch,
+                                      exaggeration, dof))
 
     def grids(self, N, dtype):
         import torch

@@ -1,0 +1,57 @@
+"""Host side of MAGNET(loss="tsne"): parameter checks as the reference's (tests/test_magnet.py:425-428), and the
+graph inputs pinned by what they must satisfy (smooth k-NN row sums, fuzzy union, mid-near rule)."""
+import numpy as np
+import pytest
+
+from magnet import MAGNET
+from magnet.graph import smooth_knn, fuzzy_graph, mid_near_pairs, prepare_data
+
+
+def test_parameter_errors():
+    X = np.random.default_rng(0).normal(size=(50, 4)).astype(np.float32)
+    for bad in (dict(loss="foo"), dict(loss="tsne", n_components=3), dict(loss="tsne", metric="manhattan"),
+                dict(loss="tsne", n_components=0), dict(loss="tsne", precision="fp16")):
+        with pytest.raises(ValueError):
+            MAGNET(**bad).fit(X)
+    with pytest.raises(NotImplementedError):
+        MAGNET(loss="umap").fit(X)
+
+
+def test_smooth_knn_hits_the_target_sum():
+    rng = np.random.default_rng(1)
+    d = np.sort(rng.uniform(0.1, 3.0, (500, 14)).astype(np.float32), axis=1)
+    v = smooth_knn(d, 15)
+    assert v.shape == d.shape and (v > 0).all() and (v <= 1).all()
+    assert np.allclose(v[:, 0], 1.0)                         # the nearest neighbour (rho) has membership 1
+    assert np.abs(v.sum(1) - np.log2(15)).max() < 1e-3      # sum exp(-(d - rho)/sigma) = log2(k)
+
+
+def test_fuzzy_graph_is_the_fuzzy_union():
+    rng = np.random.default_rng(2)
+    n, k = 300, 10
+    ind = np.array([rng.choice(np.delete(np.arange(n), i), k, replace=False) for i in range(n)])
+    dist = np.sort(rng.uniform(0.5, 2.0, (n, k)).astype(np.float32), axis=1)
+    P = fuzzy_graph(ind, dist, k + 1).toarray()
+    v = smooth_knn(dist, k + 1)
+    A = np.zeros((n, n), np.float32)
+    A[np.repeat(np.arange(n), k), ind.ravel()] = v.ravel()
+    assert np.allclose(P, A + A.T - A * A.T, atol=1e-6) and np.allclose(P, P.T) and not np.diag(P).any()
+
+
+def test_mid_near_pairs_follow_the_rule():
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(400, 6)).astype(np.float32)
+    pairs = mid_near_pairs(X, 3, 6, random_state=7)
+    assert pairs.shape == (400, 3) and (pairs != np.arange(400)[:, None]).all()
+    # replay the candidates of the same generator: each pair is the second closest of its six candidates
+    cand = np.random.default_rng(7).integers(0, 400, (400, 3, 6))
+    cand = np.where(cand == np.arange(400)[:, None, None], (cand + 1) % 400, cand)
+    d = ((X[cand] - X[:, None, None, :]) ** 2).sum(-1)
+    second = np.take_along_axis(cand, np.argsort(d, axis=-1, kind="stable")[..., 1:2], -1)[..., 0]
+    assert np.array_equal(pairs, second)
+
+
+def test_prepare_data_cosine_rows():
+    X = np.array([[3.0, 4.0], [0.0, 0.0]], np.float32)
+    Q = prepare_data(X, "cosine")
+    assert np.allclose(Q, [[0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])

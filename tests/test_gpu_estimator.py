@@ -1,0 +1,98 @@
+"""MAGNET(loss="tsne") end to end on the GPU: the reference's own quality tests for this loss
+(tests/test_magnet.py:234-240 and :440-447), the transform's independence from the other points of a call, and
+the continuous-batching service (each request's result the same alone and in the mix)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from magnet import capi
+    capi.lib()
+    capi.use_torch_stream()
+    return torch
+
+
+def blobs(n=3000, d=20, centers=5, seed=0):
+    """the reference's test data (tests/test_magnet.py:15-19)"""
+    rng = np.random.default_rng(seed)
+    y = rng.integers(0, centers, n)
+    C = rng.normal(0, 10, (centers, d))
+    return (C[y] + rng.normal(0, 1, (n, d))).astype(np.float32), y
+
+
+def knn_purity(Y, y, k=10):
+    from sklearn.neighbors import NearestNeighbors
+    nn = NearestNeighbors(n_neighbors=k + 1).fit(Y).kneighbors(Y, return_distance=False)[:, 1:]
+    return (y[nn] == y[:, None]).mean()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_tsne_loss_separates_blobs(precision):
+    from magnet import MAGNET
+    X, y = blobs(2000, d=10, centers=5, seed=5)
+    m = MAGNET(loss="tsne", n_epochs=300, random_state=0, precision=precision).fit(X)
+    assert m.embedding_.shape == (2000, 2) and np.isfinite(m.embedding_).all()
+    assert knn_purity(m.embedding_, y) > 0.95
+    assert m.graph_.shape == (2000, 2000) and m.mid_pairs_.shape == (2000, 5)
+
+
+@pytest.fixture(scope="module")
+def fitted():
+    from magnet import MAGNET
+    X, y = blobs(2000, d=10, centers=5, seed=5)
+    m = MAGNET(loss="tsne", random_state=0).fit(X[:1700])
+    return m, X, y
+
+
+def test_transform_after_fit(fitted):
+    m, X, y = fitted
+    Yq = m.transform(X[1700:])
+    from sklearn.neighbors import KNeighborsClassifier
+    assert Yq.shape == (300, 2) and np.isfinite(Yq).all()
+    assert (KNeighborsClassifier(10).fit(m.embedding_, y[:1700]).predict(Yq) == y[1700:]).mean() > 0.95
+
+
+def test_transform_point_does_not_depend_on_its_batch(fitted):
+    m, X, y = fitted
+    Q = X[1700:]
+    full = m.transform(Q)
+    part = m.transform(Q[37:91])
+    one = m.transform(Q[5:6])
+    assert np.allclose(full[37:91], part, rtol=0, atol=1e-5 * np.abs(full).max())
+    assert np.allclose(full[5:6], one, rtol=0, atol=1e-5 * np.abs(full).max())
+
+
+def test_serving_same_alone_and_in_the_mix(fitted):
+    import threading
+    from magnet import TransformServer
+    m, X, y = fitted
+    Q = X[1700:]
+    rng = np.random.default_rng(0)
+    cuts = np.sort(rng.choice(np.arange(1, 300), 11, replace=False))
+    parts = np.split(np.arange(300), cuts)
+    alone = [m.transform(Q[p]) for p in parts]
+    srv = TransformServer(m, max_batch_points=120).start()
+    try:
+        reqs = [None] * len(parts)
+
+        def client(i):
+            reqs[i] = srv.submit(Q[parts[i]])
+        threads = [threading.Thread(target=client, args=(i,)) for i in range(len(parts))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        outs = [r.wait(120) for r in reqs]
+    finally:
+        srv.stop()
+    scale = np.abs(np.concatenate(alone)).max()
+    for a, o in zip(alone, outs):
+        assert o.shape == a.shape and np.allclose(a, o, rtol=0, atol=1e-5 * scale)
+    assert all(r.latency is not None and r.latency > 0 for r in reqs)
+    assert sum(b[1] for b in srv.batches) == 300 and max(b[1] for b in srv.batches) <= max(120, max(map(len, parts)))
