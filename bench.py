@@ -33,6 +33,9 @@ WORKLOADS = {
     "gmm1m": dict(n=1_000_000, d=50, n_iter=1000, early=250, desc="1,000,000 x 50 Gaussian mixture (100 clusters, Dirichlet sizes), perplexity 30, 1,000 iterations"),
     "gmm10k": dict(n=10_000, d=50, n_iter=500, early=166, desc="10,000 x 50 Gaussian mixture (10 clusters), perplexity 30, 500 iterations"),
 }
+SERVING = dict(n_train=200_000, d=50, clusters=50, pool=60_000, qps=200.0, duration=8.0, n_iter=100,
+               desc="MAGNET(loss='tsne') on 200,000 x 50 (Gaussian mixture, 50 clusters), resident; transform() requests "
+                    "of 1-4096 new rows (log-uniform sizes, same mixture) arriving by a Poisson process; 100 iterations each")
 
 
 def log(*a):
@@ -123,17 +126,87 @@ def matched_precision(P, snaps, n_iter, early, k):
     return rows
 
 
-def cpu_baseline(P, Y0, n_iter, early, gpu_state_fn, threads, points):
-    """The reference's compiled module on this host: per-iteration time at six points of the schedule
-    (GPU FP32 states), 2 warm-up + 5 timed reference iterations at each; it/s from the mean."""
+def serving_bench(qps=None, duration=None, seed=0):
+    """The serving configuration: fit, then Poisson transform() requests against the resident model through the
+    continuous-batching server; also the saturated rate (all requests queued at once)."""
+    import torch
+    from magnet import MAGNET, TransformServer
+    from magnet.serving import poisson_load, latency_report
+    S = SERVING
+    qps = qps or S["qps"]
+    duration = duration or S["duration"]
+    rng = np.random.default_rng(seed)
+    k, d = S["clusters"], S["d"]
+    ntot = S["n_train"] + S["pool"]
+    lab = rng.integers(0, k, ntot)
+    X = (rng.normal(0, 4, (k, d))[lab] + rng.normal(0, 1, (ntot, d))).astype(np.float32)
+    Xtr, pool = X[:S["n_train"]], X[S["n_train"]:]
+    t0 = time.perf_counter()
+    m = MAGNET(loss="tsne", random_state=0, transform_epochs=S["n_iter"]).fit(Xtr)
+    t_fit = time.perf_counter() - t0
+    srv = TransformServer(m).start()
+
+    def sampler(r):
+        size = int(round(np.exp(r.uniform(0, np.log(4096)))))
+        return pool[r.integers(0, len(pool), size)]
+    try:
+        srv.submit(pool[:64]).wait(120)   # warm-up: kernels, field, k-NN tiles
+        t1 = time.perf_counter()
+        reqs = poisson_load(srv, sampler, qps, duration, seed + 1)
+        wall = time.perf_counter() - t1
+        rep = latency_report(reqs, wall)
+        r2 = np.random.default_rng(seed + 2)
+        burst = [sampler(r2) for _ in range(200)]
+        t2 = time.perf_counter()
+        breqs = [srv.submit(b) for b in burst]
+        for r in breqs:
+            r.wait(600)
+        sat = sum(len(b) for b in burst) / (time.perf_counter() - t2)
+    finally:
+        srv.stop()
+    rep.update({"qps": qps, "duration_s": duration, "fit_s": round(t_fit, 2), "saturated_points_per_s": round(sat, 1),
+                "batches": len(srv.batches), "workload": S["desc"]})
+    return rep
+
+
+def cpu_baseline(P, n_iter, early, states, threads, points):
+    """The reference's compiled module on this host, in a process of its own (it registers the module name
+    `_magnet`, like the product): per-iteration time at six points of the schedule (GPU FP32 states), 2 warm-up
+    + 5 timed reference iterations at each; it/s from the mean."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "cpu_baseline.npz")
+        arrays = {"indptr": P.indptr.astype(np.int32), "indices": P.indices.astype(np.int32), "data": P.data.astype(np.float32),
+                  "meta": np.array([P.shape[0], n_iter, early, threads]), "points": np.array(points)}
+        for t in points:
+            for name, a in zip("YUG", states[t]):
+                arrays[f"{name}{t}"] = a
+        np.savez(path, **arrays)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path], capture_output=True, text=True)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        log(f"bench: cpu baseline failed: {r.stderr[-500:]}")
+        return None
+    return json.loads(lines[-1])
+
+
+def cpu_baseline_worker(path):
+    f = np.load(path)
+    n, n_iter, early, threads = [int(v) for v in f["meta"]]
+    points = [int(t) for t in f["points"]]
+    print(json.dumps(cpu_baseline_run(f, n, n_iter, early, threads, points)), flush=True)
+
+
+def cpu_baseline_run(f, n, n_iter, early, threads, points):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import scipy.fft as sfft
     R = oracle.ref_module()
     if R is None:
         return None
-    n = P.shape[0]
-    ip, ix, d = P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data.astype(np.float32)
+    ip, ix, d = f["indptr"], f["indices"], f["data"]
+    gpu_state_fn = lambda t: (f[f"Y{t}"].copy(), f[f"U{t}"].copy(), f[f"G{t}"].copy())  # noqa: E731
     lr = max(200.0, n / 12.0)
     cache = {}
 
@@ -208,6 +281,9 @@ def copy_probe(mib):
 
 
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-worker":
+        cpu_baseline_worker(sys.argv[2])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="iterations timed (default: the whole schedule)")
@@ -216,6 +292,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB", help="only run a plain copy of MIB MiB")
+    ap.add_argument("--no-serving", action="store_true", help="skip the transform() serving leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the t-SNE pass of one embedding runs on a single MI355X "
@@ -279,13 +356,16 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
 
-        def gpu_state(t):   # FP32 GPU state at checkpoint t of the profiled pass
-            Y, U, G = snaps[t]
-            torch.cuda.synchronize()
-            return tuple(np.ascontiguousarray(a.cpu().numpy()) for a in (Y, U, G))
+        torch.cuda.synchronize()
+        states = {t: tuple(np.ascontiguousarray(a.cpu().numpy()) for a in snaps[t]) for t in points}
         tc = time.time()
-        cpu = cpu_baseline(P, Y0, W["n_iter"], W["early"], gpu_state, threads, points)
+        cpu = cpu_baseline(P, W["n_iter"], W["early"], states, threads, points)
         log(f"bench: cpu baseline {cpu and cpu['value']} it/s in {time.time() - tc:.1f}s")
+    serving = None
+    if not args.no_serving and args.workload == "mnist70k":
+        ts = time.time()
+        serving = serving_bench()
+        log(f"bench: serving {json.dumps(serving)} ({time.time() - ts:.1f}s)")
     line = {
         "metric": METRIC, "value": round(steps / sec32, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * sec32 / steps, 4), "higher_is_better": True,
@@ -294,6 +374,7 @@ def main():
                    "early_iter": W["early"], "precision": "fp32 (FP64 Z and reductions)", "n_interp": 3,
                    "box_width": 0.7},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+        "transform_p99_ms": serving and serving["p99_ms"], "serving": serving,
     }
     print(json.dumps(line), flush=True)
 
