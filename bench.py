@@ -33,6 +33,9 @@ WORKLOADS = {
     "gmm1m": dict(n=1_000_000, d=50, n_iter=1000, early=250, desc="1,000,000 x 50 Gaussian mixture (100 clusters, Dirichlet sizes), perplexity 30, 1,000 iterations"),
     "gmm10k": dict(n=10_000, d=50, n_iter=500, early=166, desc="10,000 x 50 Gaussian mixture (10 clusters), perplexity 30, 500 iterations"),
 }
+BATCHED = dict(count=64, n_min=2000, n_max=50000, d=32, n_iter=500, early=166,
+               desc="64 independent embeddings, n_i log-uniform in [2000, 50000] (seed 1000 + i), 32-d Gaussian mixtures "
+                    "of 5-20 clusters, perplexity 30, 500 iterations (166 early), FP32")
 SERVING = dict(n_train=200_000, d=50, clusters=50, pool=60_000, qps=200.0, duration=8.0, n_iter=100,
                desc="MAGNET(loss='tsne') on 200,000 x 50 (Gaussian mixture, 50 clusters), resident; transform() requests "
                     "of 1-4096 new rows (log-uniform sizes, same mixture) arriving by a Poisson process; 100 iterations each")
@@ -169,6 +172,58 @@ def serving_bench(qps=None, duration=None, seed=0):
     return rep
 
 
+def batched_instances(seed0=1000):
+    """The batched configuration: 64 instances, n_i = round(exp(U(ln 2000, ln 50000))) with instance seed 1000 + i,
+    32-d Gaussian mixtures of 5-20 clusters (centres N(0, 25 I)), perplexity-30 affinities."""
+    from magnet.affinity import perplexity_affinities
+    B = BATCHED
+    Ps, Ys = [], []
+    for i in range(B["count"]):
+        rng = np.random.default_rng(seed0 + i)
+        n = int(round(np.exp(rng.uniform(np.log(B["n_min"]), np.log(B["n_max"])))))
+        k = int(rng.integers(5, 21))
+        X = (rng.normal(0, 5, (k, B["d"]))[rng.integers(0, k, n)] + rng.normal(0, 1, (n, B["d"]))).astype(np.float32)
+        Ps.append(perplexity_affinities(X, 30.0))
+        Ys.append(rng.normal(0, 1e-4, (n, 2)))
+    return Ps, Ys
+
+
+def batched_bench():
+    """All instances through mg_tsne_optimize_batch (stream count from the autotuner) against the same instances
+    one after another through mg_tsne_optimize; both from resident inputs, results compared bit for bit."""
+    import torch
+    from magnet import capi
+    from magnet.batch import device_instances, autotune_streams
+    B = BATCHED
+    t0 = time.perf_counter()
+    Ps, Ys = batched_instances()
+    t_build = time.perf_counter() - t0
+    kw = dict(n_iter=B["n_iter"], early_iter=B["early"])
+    seq = device_instances(Ps, Ys, "fp32")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for ip, ix, dv, Y, U, G in seq:
+        capi.optimize(ip, ix, dv, Y, U, G, 0, B["n_iter"], **kw)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t1
+    bat = device_instances(Ps, Ys, "fp32")
+    t2 = time.perf_counter()
+    k = autotune_streams(bat)
+    t_tune = time.perf_counter() - t2
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    capi.optimize_batch(bat, n_streams=k, **kw)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t3
+    same = all(torch.equal(a[3], b[3]) for a, b in zip(seq, bat))
+    npts = sum(P.shape[0] for P in Ps)
+    inst_it = B["count"] * B["n_iter"]
+    return {"workload": B["desc"], "instances": B["count"], "points": int(npts), "n_streams": k,
+            "instance_iters_per_s": round(inst_it / t_bat, 1), "sequential_instance_iters_per_s": round(inst_it / t_seq, 1),
+            "speedup_vs_sequential": round(t_seq / t_bat, 3), "batch_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
+            "autotune_s": round(t_tune, 2), "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
+
+
 def cpu_baseline(P, n_iter, early, states, threads, points):
     """The reference's compiled module on this host, in a process of its own (it registers the module name
     `_magnet`, like the product): per-iteration time at six points of the schedule (GPU FP32 states), 2 warm-up
@@ -293,6 +348,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB", help="only run a plain copy of MIB MiB")
     ap.add_argument("--no-serving", action="store_true", help="skip the transform() serving leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the t-SNE pass of one embedding runs on a single MI355X "
@@ -366,6 +422,11 @@ def main():
         ts = time.time()
         serving = serving_bench()
         log(f"bench: serving {json.dumps(serving)} ({time.time() - ts:.1f}s)")
+    batched = None
+    if not args.no_batched and args.workload == "mnist70k":
+        tb = time.time()
+        batched = batched_bench()
+        log(f"bench: batched {json.dumps(batched)} ({time.time() - tb:.1f}s)")
     line = {
         "metric": METRIC, "value": round(steps / sec32, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * sec32 / steps, 4), "higher_is_better": True,
@@ -374,7 +435,7 @@ def main():
                    "early_iter": W["early"], "precision": "fp32 (FP64 Z and reductions)", "n_interp": 3,
                    "box_width": 0.7},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
-        "transform_p99_ms": serving and serving["p99_ms"], "serving": serving,
+        "transform_p99_ms": serving and serving["p99_ms"], "serving": serving, "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -326,9 +326,36 @@ template <class T> struct Optimizer {
     PinnedBuf host_mb;
     Optimizer(int ni) : rep(0, ni) {}
 
-    int run(const mg_tsne_params* p, const int32_t* ip, const int32_t* ix, const T* data, i64 nnz, i64 n, T* Y, T* U,
-            T* G, const int32_t* pairs, int m, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user,
-            hipStream_t st) {
+    // one run: [it0, it1) of the schedule p on the state (Y, U, G), issued on stream st
+    const mg_tsne_params* p = nullptr;
+    const int32_t *ip = nullptr, *ix = nullptr, *pairs = nullptr;
+    const T* data = nullptr;
+    T *Y = nullptr, *U = nullptr, *G = nullptr;
+    i64 nnz = 0, n = 0;
+    int m = 0, it0 = 0, it1 = 0;
+    double* h_sumQ = nullptr;
+    mg_tsne_callback cb = nullptr;
+    void* user = nullptr;
+    hipStream_t st = nullptr;
+    double lr = 0, attr_bytes = 0;
+    bool fused = true;
+    T inv_dof = T(1);
+    double* dsum = nullptr;
+    T* hmb = nullptr;
+    unsigned* uc = nullptr;
+
+    void attraction() {
+        g_prof.begin(ST_ATTR, st);
+        launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
+        g_prof.end(ST_ATTR, st, attr_bytes);
+    }
+
+    // checks, buffers, the starting layout's bbox (read back), and the first iteration's attraction
+    void begin(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
+               T* G_, const int32_t* pairs_, int m_, int it0_, int it1_, double* h_sumQ_, mg_tsne_callback cb_, void* user_,
+               hipStream_t st_) {
+        p = p_; ip = ip_; ix = ix_; data = data_; nnz = nnz_; n = n_; Y = Y_; U = U_; G = G_; pairs = pairs_; m = m_;
+        it0 = it0_; it1 = it1_; h_sumQ = h_sumQ_; cb = cb_; user = user_; st = st_;
         require(p != nullptr, "params must not be null");
         require(n >= 1 && Y && U && G, "Y, U, G must be (n, 2) device arrays");
         require(p->n_iter >= 0 && it0 >= 0 && it1 >= it0 && it1 <= p->n_iter, "iteration range outside the schedule");
@@ -336,84 +363,126 @@ template <class T> struct Optimizer {
         require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
         require(!pairs || m >= 1, "pairs must be (n, m) with m >= 1");
         validate_csr(ip, ix, n, nnz, st);
-        if (getenv("MAGNET_DEBUG")) { g_prof.on = true; g_prof.debug = true; }
         rep.have_kernel = true; rep.tsne_kernel = true; rep.kdof = p->dof; rep.kwidth = p->box_width;
-        const double lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
-        const bool fused = !(pairs && m > 0 && p->mid_weight > 0);
+        lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
+        fused = !(pairs && m > 0 && p->mid_weight > 0);
         attr.ensure((size_t)2 * n * sizeof(T));
         F.ensure((size_t)2 * n * sizeof(T));
         if (!fused) grad.ensure((size_t)2 * n * sizeof(T));
         part.ensure((size_t)6 * 1024 * sizeof(double));
-        unsigned* uc = reinterpret_cast<unsigned*>(upd_counter.ensure(64));
+        uc = reinterpret_cast<unsigned*>(upd_counter.ensure(64));
         unsigned* bc = reinterpret_cast<unsigned*>(bb_counter.ensure(64));
-        if (upd_counter.bytes && !counters_zeroed) {
+        if (!counters_zeroed) {
             MG_HIP(hipMemsetAsync(uc, 0, 64, st)); MG_HIP(hipMemsetAsync(bc, 0, 64, st)); counters_zeroed = true;
         }
         mb.ensure(8 * sizeof(T));
         bb_part.ensure((size_t)4 * 1024 * sizeof(T));
         bb_out.ensure(4 * sizeof(T));
-        double* dsum = reinterpret_cast<double*>(sumq.ensure((size_t)std::max(1, it1 - it0) * sizeof(double)));
-        T* hmb = reinterpret_cast<T*>(host_mb.ensure(8 * sizeof(T)));
-        // bbox of the starting layout
+        dsum = reinterpret_cast<double*>(sumq.ensure((size_t)std::max(1, it1 - it0) * sizeof(double)));
+        hmb = reinterpret_cast<T*>(host_mb.ensure(8 * sizeof(T)));
         launch_bbox<T>(Y, n, dptr<T>(bb_part), bc, dptr<T>(bb_out), st);
         MG_HIP(hipMemcpyAsync(hmb + 2, dptr<T>(bb_out), 4 * sizeof(T), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
-        const T inv_dof = T(1) / (T)p->dof;
-        const double attr_bytes = 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T);
-        auto attraction = [&]() {
-            g_prof.begin(ST_ATTR, st);
-            launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
-            g_prof.end(ST_ATTR, st, attr_bytes);
-        };
+        inv_dof = T(1) / (T)p->dof;
+        attr_bytes = 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T);
         if (!mb_ready) MG_HIP(hipEventCreateWithFlags(&mb_ready, hipEventDisableTiming));
-        int stopped = 0;
-        // The attraction needs no grid: iteration it + 1's is queued right behind iteration it's update, so the
-        // device runs it while the host waits for the layout's bounding box and decides the next grid.
         if (it0 < it1) attraction();
-        for (int it = it0; it < it1; ++it) {
-            const double bb[4] = {(double)hmb[2], (double)hmb[3], (double)hmb[4], (double)hmb[5]};
-            require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
-                    "layout is not finite");
-            const GridDecision gd = decide_grid(bb, p->box_width, p->min_boxes, p->max_boxes);
-            const GridT<T> g = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, p->n_interp);
-            rep.kwidth = gd.width;
-            const bool early = it < p->early_iter;
-            const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
-            const T mom = (T)(early ? p->momentum0 : p->momentum1);
-            rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st);
-            const T* dinv = dptr<T>(rep.inv_out);
-            g_prof.begin(ST_UPDATE, st);
-            if (fused) {
-                launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
-                                 dptr<double>(part), uc, dptr<T>(mb), st);
-            } else {
-                const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
-                launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
-                               dptr<T>(grad), st);
-                launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
-                                 dptr<double>(part), uc, dptr<T>(mb), st);
-            }
-            g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
-            MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
-            MG_HIP(hipEventRecord(mb_ready, st));
-            const bool more = it + 1 < it1;
-            if (more) attraction();
-            MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
-            if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); g_prof.collect(it); }
-            if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
-                if (cb(it, Y, n, user) != 0) { stopped = 1; it1 = it + 1; break; }
-            }
+    }
+
+    // iteration it: grid from the last bbox, repulsion, update, bbox read-back; then the next iteration's
+    // attraction, which needs no grid, so the device runs it while the host waits for the bbox
+    void issue(int it) {
+        const double bb[4] = {(double)hmb[2], (double)hmb[3], (double)hmb[4], (double)hmb[5]};
+        require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
+                "layout is not finite");
+        const GridDecision gd = decide_grid(bb, p->box_width, p->min_boxes, p->max_boxes);
+        const GridT<T> g = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, p->n_interp);
+        rep.kwidth = gd.width;
+        const bool early = it < p->early_iter;
+        const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
+        const T mom = (T)(early ? p->momentum0 : p->momentum1);
+        rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st);
+        const T* dinv = dptr<T>(rep.inv_out);
+        g_prof.begin(ST_UPDATE, st);
+        if (fused) {
+            launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
+                             dptr<double>(part), uc, dptr<T>(mb), st);
+        } else {
+            const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
+            launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
+                           dptr<T>(grad), st);
+            launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
+                             dptr<double>(part), uc, dptr<T>(mb), st);
         }
-        if (h_sumQ && it1 > it0) {
-            MG_HIP(hipMemcpyAsync(h_sumQ, dsum, (size_t)(it1 - it0) * sizeof(double), hipMemcpyDeviceToHost, st));
-        }
+        g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
+        MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipEventRecord(mb_ready, st));
+        if (it + 1 < it1) attraction();
+    }
+
+    // waits for iteration it; returns true when the callback asks to stop
+    bool complete(int it) {
+        MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
+        if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); g_prof.collect(it); }
+        if (cb && (it % 50 == 0 || it == p->n_iter - 1)) return cb(it, Y, n, user) != 0;
+        return false;
+    }
+
+    void finish(int it_end) {
+        if (h_sumQ && it_end > it0)
+            MG_HIP(hipMemcpyAsync(h_sumQ, dsum, (size_t)(it_end - it0) * sizeof(double), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
+    }
+
+    int run(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
+            T* G_, const int32_t* pairs_, int m_, int it0_, int it1_, double* h_sumQ_, mg_tsne_callback cb_, void* user_,
+            hipStream_t st_) {
+        if (getenv("MAGNET_DEBUG")) { g_prof.on = true; g_prof.debug = true; }
+        begin(p_, ip_, ix_, data_, nnz_, n_, Y_, U_, G_, pairs_, m_, it0_, it1_, h_sumQ_, cb_, user_, st_);
+        int stopped = 0, it_end = it1;
+        for (int it = it0; it < it1; ++it) {
+            issue(it);
+            if (complete(it)) { stopped = 1; it_end = it + 1; break; }
+        }
+        finish(it_end);
         return stopped;
     }
     bool counters_zeroed = false;
     hipEvent_t mb_ready = nullptr;
     ~Optimizer() { if (mb_ready) (void)hipEventDestroy(mb_ready); }
 };
+
+// Many independent embeddings on one schedule: instance b is issued on stream b mod S, and all instances advance
+// together, so the kernels of different instances run concurrently on the device. Per instance the arithmetic is
+// exactly the single run's.
+template <class T> void optimize_batch(const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams) {
+    require(count >= 1 && inst != nullptr, "no instances");
+    require(n_streams >= 1 && n_streams <= 64, "n_streams must be in [1, 64]");
+    const bool prof = g_prof.on;
+    g_prof.on = false;   // stage timers are per process, not per instance
+    std::vector<hipStream_t> streams(std::min(n_streams, count), nullptr);
+    struct Guard {
+        std::vector<hipStream_t>& s;
+        bool prof;
+        ~Guard() { for (auto x : s) if (x) (void)hipStreamDestroy(x); g_prof.on = prof; }
+    } guard{streams, prof};
+    for (auto& s : streams) MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<std::unique_ptr<Optimizer<T>>> opt;
+    for (int b = 0; b < count; ++b) {
+        const mg_tsne_instance& I = inst[b];
+        require(I.nnz == 0 || I.data != nullptr, "data must be a device array");
+        opt.emplace_back(new Optimizer<T>(p->n_interp));
+        opt.back()->begin(p, I.indptr, I.indices, (const T*)I.data, I.nnz, I.n, (T*)I.Y, (T*)I.U, (T*)I.G, nullptr, 0, 0,
+                          p->n_iter, I.h_sumQ, nullptr, nullptr, streams[b % streams.size()]);
+    }
+    // instance b's iteration it is issued as soon as its iteration it - 1 is done; the others keep the device busy
+    // while the host waits for one of them
+    if (p->n_iter > 0) for (auto& o : opt) o->issue(0);
+    for (int it = 1; it < p->n_iter; ++it)
+        for (auto& o : opt) { o->complete(it - 1); o->issue(it); }
+    if (p->n_iter > 0) for (auto& o : opt) o->complete(p->n_iter - 1);
+    for (auto& o : opt) o->finish(p->n_iter);
+}
 
 }  // namespace mg
 
@@ -638,6 +707,15 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
         MG_HIP(hipStreamSynchronize(g_stream));
     };
     if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
+    MG_API_END
+}
+
+int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(p != nullptr, "params must not be null");
+    if (prec == MG_F64) optimize_batch<double>(p, count, inst, n_streams);
+    else optimize_batch<float>(p, count, inst, n_streams);
     MG_API_END
 }
 

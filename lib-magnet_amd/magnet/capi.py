@@ -24,6 +24,12 @@ class Params(C.Structure):
                 ("box_width", C.c_double), ("dof", C.c_double), ("mid_weight", C.c_double), ("mid_phase", C.c_double)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("indptr", C.c_void_p), ("indices", C.c_void_p), ("data", C.c_void_p), ("nnz", C.c_int64),
+                ("n", C.c_int64), ("Y", C.c_void_p), ("U", C.c_void_p), ("G", C.c_void_p),
+                ("h_sumQ", C.POINTER(C.c_double))]
+
+
 CALLBACK = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p)
 
 
@@ -65,6 +71,7 @@ def lib():
             "mg_gather_forces_2d": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
                                   C.POINTER(C.c_double), CALLBACK, vp], i32),
+            "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -182,6 +189,32 @@ def optimize(indptr, indices, data, Y, U, G, it0=0, it1=None, n_iter=750, early_
                                  ptr(Y), ptr(U), ptr(G), ptr(pairs), pairs.shape[1] if pairs is not None else 0, it0, it1,
                                  sq.ctypes.data_as(C.POINTER(C.c_double)), cb, None))
     return sq[: max(0, it1 - it0)]
+
+
+def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.0, exaggeration=1.0, learning_rate=0.0,
+                   momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, min_boxes=20, max_boxes=1000, n_streams=4,
+                   sumq=False):
+    """Runs mg_tsne_optimize_batch on a list of (indptr, indices, data, Y, U, G) device-tensor tuples, all of one
+    dtype; the states are updated in place. Returns the per-instance sumQ arrays when sumq is set."""
+    import numpy as np
+    if not instances:
+        raise ValueError("no instances")
+    prec = prec_of(instances[0][3])
+    if any(prec_of(t[3]) != prec for t in instances):
+        raise ValueError("all instances must have one dtype")
+    p = Params(n_iter, early_iter, early_exaggeration, exaggeration, learning_rate, momentum[0], momentum[1], n_interp,
+               min_boxes, max_boxes, box_width, dof, 0.0, 0.5)
+    arr = (Instance * len(instances))()
+    sq = [np.zeros(max(1, n_iter)) for _ in instances] if sumq else None
+    for b, (ip, ix, dv, Y, U, G) in enumerate(instances):
+        if Y.dim() != 2 or Y.shape[1] != 2 or U.shape != Y.shape or G.shape != Y.shape:
+            raise ValueError("Y, U, G must be (n, 2)")
+        if ip.numel() != Y.shape[0] + 1 or ix.numel() != dv.numel():
+            raise ValueError("CSR arrays do not match the layout")
+        arr[b] = Instance(ip.data_ptr(), ix.data_ptr(), dv.data_ptr(), ix.numel(), Y.shape[0], Y.data_ptr(),
+                          U.data_ptr(), G.data_ptr(), sq[b].ctypes.data_as(C.POINTER(C.c_double)) if sumq else None)
+    check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams)))
+    return [s[:n_iter] for s in sq] if sumq else None
 
 
 def profile(on=True):

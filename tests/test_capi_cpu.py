@@ -79,3 +79,14 @@ def test_fft_size_choice():
         want = min(m for m in sizes if m >= 2 * N)
         assert capi.fft_size(N) == want, N
     assert capi.fft_size(4097) == 0
+
+
+def test_batch_entry_rejects_empty_and_bad_stream_counts():
+    L = capi.lib()
+    p = capi.Params(10, 3, 12.0, 1.0, 0.0, 0.5, 0.8, 3, 20, 1000, 0.7, 1.0, 0.0, 0.5)
+    arr = (capi.Instance * 1)()
+    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 0, arr, 1) != 0
+    assert b"no instances" in L.mg_last_error()
+    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 0) != 0
+    assert b"n_streams" in L.mg_last_error()
+    assert L.mg_tsne_optimize_batch(7, ctypes.byref(p), 1, arr, 1) != 0

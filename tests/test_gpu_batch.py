@@ -1,0 +1,86 @@
+"""The batched form (mg_tsne_optimize_batch): instances of uneven sizes on several HIP streams give, each, exactly
+the result of the single-instance loop (same kernels, grids and order of operations), for any stream count."""
+import numpy as np
+import pytest
+
+from magnet import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    capi.lib()
+    capi.use_torch_stream()
+    return torch
+
+
+SIZES = (700, 2500, 1200, 4000, 900)
+SCHED = dict(n_iter=80, early_iter=30)
+
+
+def problems():
+    from magnet.affinity import perplexity_affinities
+    out = []
+    for i, n in enumerate(SIZES):
+        rng = np.random.default_rng(1000 + i)
+        k = int(rng.integers(5, 21))
+        c = rng.normal(0, 5, (k, 32))
+        X = (c[rng.integers(0, k, n)] + rng.normal(0, 1, (n, 32))).astype(np.float32)
+        out.append((perplexity_affinities(X, 30.0), rng.normal(0, 1e-4, (n, 2))))
+    return out
+
+
+_P = []
+
+
+def cached():
+    if not _P:
+        _P.extend(problems())
+    return _P
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("n_streams", [1, 3])
+def test_batch_equals_single_runs(precision, n_streams):
+    from magnet.batch import device_instances
+    probs = cached()
+    Ps, Ys = [p for p, _ in probs], [y for _, y in probs]
+    batch = device_instances(Ps, Ys, precision)
+    sq = capi.optimize_batch(batch, n_streams=n_streams, sumq=True, **SCHED)
+    for b, single in enumerate(device_instances(Ps, Ys, precision)):
+        ip, ix, dv, Y, U, G = single
+        s1 = capi.optimize(ip, ix, dv, Y, U, G, 0, SCHED["n_iter"], **SCHED)
+        for x, y in zip(batch[b][3:], (Y, U, G)):
+            assert np.array_equal(host(x), host(y)), (b, SIZES[b])
+        assert np.array_equal(sq[b], s1), b
+
+
+def test_optimize_tsne_batch_matches_optimize_tsne():
+    from magnet.batch import optimize_tsne_batch
+    from magnet.tsne import optimize_tsne
+    probs = cached()[:3]
+    out = optimize_tsne_batch([p for p, _ in probs], [y for _, y in probs], n_iter=60, early_iter=20, n_streams="auto")
+    for (P, Y0), Yb in zip(probs, out):
+        Ys = optimize_tsne(P, Y0.astype(np.float32), n_iter=60, early_iter=20)
+        assert Yb.dtype == np.float32 and np.array_equal(Yb, Ys)
+
+
+def test_batch_rejects_bad_instances():
+    from magnet.batch import device_instances
+    P, Y = cached()[0]
+    inst = device_instances([P], [Y])
+    ip, ix, dv, Yd, U, G = inst[0]
+    with pytest.raises(ValueError):
+        capi.optimize_batch([(ip, ix, dv, Yd, U[:-1], G)], **SCHED)
+    with pytest.raises(capi.MgError):
+        capi.optimize_batch(inst, n_streams=0, **SCHED)
