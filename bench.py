@@ -193,7 +193,7 @@ def batched_bench():
     one after another through mg_tsne_optimize; both from resident inputs, results compared bit for bit."""
     import torch
     from magnet import capi
-    from magnet.batch import device_instances, autotune_streams
+    from magnet.batch import device_instances, autotune_streams, LAST_SWEEP
     B = BATCHED
     t0 = time.perf_counter()
     Ps, Ys = batched_instances()
@@ -208,20 +208,20 @@ def batched_bench():
     t_seq = time.perf_counter() - t1
     bat = device_instances(Ps, Ys, "fp32")
     t2 = time.perf_counter()
-    k = autotune_streams(bat)
+    k, nt = autotune_streams(bat)
     t_tune = time.perf_counter() - t2
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    capi.optimize_batch(bat, n_streams=k, **kw)
+    capi.optimize_batch(bat, n_streams=k, n_threads=nt, **kw)
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t3
     same = all(torch.equal(a[3], b[3]) for a, b in zip(seq, bat))
     npts = sum(P.shape[0] for P in Ps)
     inst_it = B["count"] * B["n_iter"]
-    return {"workload": B["desc"], "instances": B["count"], "points": int(npts), "n_streams": k,
+    return {"workload": B["desc"], "instances": B["count"], "points": int(npts), "n_streams": k, "host_threads": nt,
             "instance_iters_per_s": round(inst_it / t_bat, 1), "sequential_instance_iters_per_s": round(inst_it / t_seq, 1),
             "speedup_vs_sequential": round(t_seq / t_bat, 3), "batch_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
-            "autotune_s": round(t_tune, 2), "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
+            "autotune_s": round(t_tune, 2), "autotune_sweep_12_iters": [list(x) for x in LAST_SWEEP], "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
 
 
 def cpu_baseline(P, n_iter, early, states, threads, points):

@@ -174,10 +174,11 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_in
                      int m_pairs, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user);
 
 /* Batched form: `count` independent embeddings (CSR P, state Y / U / G of their own sizes, device arrays) on one
- * schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs). Each instance
- * is issued on one of n_streams HIP streams and all advance in lockstep, so their kernels overlap on the device;
- * each instance's result is the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null.
- * No counterpart in the reference (it fits one embedding per call). */
+ * schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs). Instance b is
+ * issued on HIP stream b mod n_streams, stream s by host thread s mod n_threads (1 <= n_threads <= n_streams), and
+ * all advance together, so their kernels overlap on the device; each instance's result is the one
+ * mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null. The call returns when every
+ * instance is done. No counterpart in the reference (it fits one embedding per call). */
 typedef struct {
     const int32_t* indptr;
     const int32_t* indices;
@@ -186,7 +187,8 @@ typedef struct {
     void *Y, *U, *G;
     double* h_sumQ;
 } mg_tsne_instance;
-int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams);
+int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams,
+                           int n_threads);
 
 #ifdef __cplusplus
 }

@@ -3,15 +3,18 @@
 // Every kernel is issued on one stream; the only host round trip per iteration is the 48-byte
 // read-back of the layout's bounding box, which the grid decision of the next iteration needs.
 #include "common.hpp"
+#include "batch.hpp"
 #include "../../include/magnet_hip.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <list>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace mg {
@@ -452,36 +455,325 @@ template <class T> struct Optimizer {
     ~Optimizer() { if (mb_ready) (void)hipEventDestroy(mb_ready); }
 };
 
-// Many independent embeddings on one schedule: instance b is issued on stream b mod S, and all instances advance
-// together, so the kernels of different instances run concurrently on the device. Per instance the arithmetic is
-// exactly the single run's.
-template <class T> void optimize_batch(const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams) {
-    require(count >= 1 && inst != nullptr, "no instances");
-    require(n_streams >= 1 && n_streams <= 64, "n_streams must be in [1, 64]");
-    const bool prof = g_prof.on;
-    g_prof.on = false;   // stage timers are per process, not per instance
-    std::vector<hipStream_t> streams(std::min(n_streams, count), nullptr);
-    struct Guard {
-        std::vector<hipStream_t>& s;
-        bool prof;
-        ~Guard() { for (auto x : s) if (x) (void)hipStreamDestroy(x); g_prof.on = prof; }
-    } guard{streams, prof};
-    for (auto& s : streams) MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::vector<std::unique_ptr<Optimizer<T>>> opt;
-    for (int b = 0; b < count; ++b) {
-        const mg_tsne_instance& I = inst[b];
-        require(I.nnz == 0 || I.data != nullptr, "data must be a device array");
-        opt.emplace_back(new Optimizer<T>(p->n_interp));
-        opt.back()->begin(p, I.indptr, I.indices, (const T*)I.data, I.nnz, I.n, (T*)I.Y, (T*)I.U, (T*)I.G, nullptr, 0, 0,
-                          p->n_iter, I.h_sumQ, nullptr, nullptr, streams[b % streams.size()]);
+// ------------------------------------------------------------------------------ batched engine
+// Many embeddings on one schedule, each iteration's stages launched once for all of them (batched launches,
+// batch.hpp). Point arrays are concatenated (instance i owns rows [off_i, off_i + n_i)); the box sort is one
+// stable sort of (instance, box) keys over all points, which leaves every instance's points in the order its
+// own box sort gives; the spread, FFT, gather and update stages run each instance's blocks with its own
+// grid, buffers and block counts, so its result is the single-instance loop's result bit for bit. FFT stages
+// run one launch per FFT size in use.
+static int bits_for(long long v) { int b = 0; while ((1ll << b) < v) ++b; return b; }
+
+static void grow(DevBuf& b, size_t need) { if (need > b.bytes) b.ensure(need + need / 4); }
+
+template <class F> std::vector<BMap> make_maps(const std::vector<int>& which, F blocks) {
+    std::vector<BMap> out;
+    BMap m;
+    m.count = 0; m.start[0] = 0;
+    for (int i : which) {
+        const int b = blocks(i);
+        if (b <= 0) continue;
+        if (m.count == BMAX) { out.push_back(m); m.count = 0; m.start[0] = 0; }
+        m.inst[m.count] = i;
+        m.start[m.count + 1] = m.start[m.count] + b;
+        ++m.count;
     }
-    // instance b's iteration it is issued as soon as its iteration it - 1 is done; the others keep the device busy
-    // while the host waits for one of them
-    if (p->n_iter > 0) for (auto& o : opt) o->issue(0);
-    for (int it = 1; it < p->n_iter; ++it)
-        for (auto& o : opt) { o->complete(it - 1); o->issue(it); }
-    if (p->n_iter > 0) for (auto& o : opt) o->complete(p->n_iter - 1);
-    for (auto& o : opt) o->finish(p->n_iter);
+    if (m.count) out.push_back(m);
+    return out;
+}
+
+template <class T> struct BatchEngine {
+    static constexpr bool centred = sizeof(T) == 4;
+    static constexpr int ntab = centred ? 4 : 2;
+    const mg_tsne_params* p = nullptr;
+    std::vector<const mg_tsne_instance*> user;
+    int count = 0, ni = 3;
+    hipStream_t st = nullptr;
+    std::vector<i64> n, off;
+    std::vector<int> M, sM;
+    std::vector<double> sdelta;
+    std::vector<GridT<T>> g;
+    std::vector<bool> fresh;
+    i64 ntot = 0;
+    T inv_dof = T(1);
+    int sorted = 0;
+    DevBuf Yc, Uc, Gc, Ac, Fc, keysA, valsA, keysB, valsB, hist, table, mbd, sumq, counters, zout, inv, upart;
+    DevBuf aW, aWh, aPh, aTr, aphi, aik, aiv, aival, azp, aspec;
+    std::vector<std::unique_ptr<DevBuf[]>> K;   // per instance: ntab kernel spectra
+    PinnedBuf htab, hmb;
+    std::vector<BMap> attr_maps;
+    std::vector<int> attr_G;
+    hipEvent_t ready = nullptr;
+    ~BatchEngine() { if (ready) (void)hipEventDestroy(ready); }
+
+    BInst<T>* host_table() { return reinterpret_cast<BInst<T>*>(htab.p); }
+    const BInst<T>* dev_table() const { return reinterpret_cast<const BInst<T>*>(table.p); }
+    const int* sorted_keys() const { return dptr<int>(sorted == 0 ? keysA : keysB); }
+    const int* sorted_vals() const { return dptr<int>(sorted == 0 ? valsA : valsB); }
+
+    void begin(const mg_tsne_params* p_, const std::vector<const mg_tsne_instance*>& inst, hipStream_t st_) {
+        p = p_; user = inst; st = st_;
+        count = (int)inst.size();
+        ni = p->n_interp;
+        require(count >= 1 && count <= 1024, "a batch holds 1 to 1024 instances");
+        require(ni >= 1 && ni <= 8, "n_interp must be in [1, 8]");
+        require(p->dof > 0, "dof must be positive");
+        require(p->n_iter >= 0, "n_iter must be >= 0");
+        require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
+        require(p->max_boxes <= 1024, "max_boxes above 1024 is not supported in a batch", MG_ERR_UNSUP);
+        fft_init_twiddles();
+        n.resize(count); off.resize(count); M.assign(count, 0); sM.assign(count, 0); sdelta.assign(count, -1.0);
+        g.resize(count); fresh.assign(count, false);
+        ntot = 0;
+        for (int i = 0; i < count; ++i) {
+            const mg_tsne_instance& I = *inst[i];
+            require(I.n >= 1 && I.Y && I.U && I.G, "Y, U, G must be (n, 2) device arrays");
+            require(I.nnz == 0 || I.data != nullptr, "data must be a device array");
+            validate_csr(I.indptr, I.indices, I.n, I.nnz, st);
+            n[i] = I.n; off[i] = ntot; ntot += I.n;
+        }
+        require(ntot < (i64)INT32_MAX, "too many points in the batch");
+        const size_t pb = (size_t)2 * ntot * sizeof(T);
+        T* Y = reinterpret_cast<T*>(Yc.ensure(pb));
+        T* U = reinterpret_cast<T*>(Uc.ensure(pb));
+        T* G = reinterpret_cast<T*>(Gc.ensure(pb));
+        Ac.ensure(pb); Fc.ensure(pb);
+        for (int i = 0; i < count; ++i) {
+            const size_t b = (size_t)2 * n[i] * sizeof(T);
+            MG_HIP(hipMemcpyAsync(Y + 2 * off[i], inst[i]->Y, b, hipMemcpyDeviceToDevice, st));
+            MG_HIP(hipMemcpyAsync(U + 2 * off[i], inst[i]->U, b, hipMemcpyDeviceToDevice, st));
+            MG_HIP(hipMemcpyAsync(G + 2 * off[i], inst[i]->G, b, hipMemcpyDeviceToDevice, st));
+        }
+        const size_t nk = (size_t)ntot * sizeof(int);
+        keysA.ensure(nk); valsA.ensure(nk); keysB.ensure(nk); valsB.ensure(nk);
+        hist.ensure(((size_t)1024 * std::max(1, sort_hist_blocks(ntot)) + 1024) * sizeof(int));
+        unsigned* cnt = reinterpret_cast<unsigned*>(counters.ensure((size_t)(2 * count + 1) * sizeof(unsigned)));
+        MG_HIP(hipMemsetAsync(cnt, 0, (size_t)(2 * count + 1) * sizeof(unsigned), st));
+        zout.ensure((size_t)2 * count * sizeof(double));
+        inv.ensure((size_t)4 * count * sizeof(T));
+        upart.ensure((size_t)count * 6 * 1024 * sizeof(double));
+        T* mb = reinterpret_cast<T*>(mbd.ensure((size_t)8 * count * sizeof(T)));
+        sumq.ensure((size_t)count * std::max(1, p->n_iter) * sizeof(double));
+        table.ensure((size_t)count * sizeof(BInst<T>));
+        htab.ensure((size_t)count * sizeof(BInst<T>));
+        T* hm = reinterpret_cast<T*>(hmb.ensure((size_t)8 * count * sizeof(T)));
+        K.clear();
+        for (int i = 0; i < count; ++i) K.emplace_back(new DevBuf[4]);
+        // starting layouts' bboxes (the update of each iteration writes the next ones)
+        for (int i = 0; i < count; ++i)
+            launch_bbox<T>(Y + 2 * off[i], n[i], reinterpret_cast<T*>(upart.p), cnt + 2 * count, mb + 8 * i + 2, st);
+        MG_HIP(hipMemcpyAsync(hm, mb, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
+        inv_dof = T(1) / (T)p->dof;
+        // the fixed part of the table, and the attraction launches (rows grouped by lanes per row)
+        BInst<T>* h = host_table();
+        std::memset((void*)h, 0, (size_t)count * sizeof(BInst<T>));
+        std::vector<std::vector<int>> byG(7);
+        for (int i = 0; i < count; ++i) {
+            const mg_tsne_instance& I = *inst[i];
+            h[i].ip = I.indptr; h[i].ix = I.indices; h[i].P = reinterpret_cast<const T*>(I.data);
+            h[i].n = n[i]; h[i].off = off[i];
+            h[i].lr = (T)(p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n[i] / p->early_exaggeration));
+            const int Gl = attraction_lanes(n[i], I.nnz);
+            byG[bits_for(Gl)].push_back(i);
+        }
+        attr_maps.clear(); attr_G.clear();
+        for (int b = 0; b < 7; ++b) {
+            const int Gl = 1 << b;
+            for (const BMap& m : make_maps(byG[b], [&](int i) { return attraction_blocks(n[i], Gl); })) {
+                attr_maps.push_back(m); attr_G.push_back(Gl);
+            }
+        }
+        MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
+        MG_HIP(hipStreamSynchronize(st));
+        if (!ready) MG_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        if (p->n_iter > 0) attraction();
+    }
+
+    void attraction() {
+        for (size_t k = 0; k < attr_maps.size(); ++k)
+            launch_attraction_batch<T>(dev_table(), attr_maps[k], attr_G[k], dptr<T>(Yc), inv_dof, dptr<T>(Ac), st);
+    }
+
+    // iteration it for every instance: grids from the last bboxes, spectra where the grid changed, the
+    // repulsion stages, the update, the bbox read-back; then the next iteration's attraction
+    void issue(int it) {
+        const T* hm = reinterpret_cast<const T*>(hmb.p);
+        int box_bits = 1;
+        size_t oW = 0, oWh = 0, oPh = 0, oTr = 0, ophi = 0, oit = 0, oiv = 0, ozp = 0, osp = 0;
+        auto al = [](size_t x) { return (x + 63) / 64 * 64; };
+        std::vector<size_t> pW(count), pWh(count), pPh(count), pTr(count), pphi(count), pit(count), piv(count), pzp(count), psp(count);
+        for (int i = 0; i < count; ++i) {
+            const double bb[4] = {(double)hm[8 * i + 2], (double)hm[8 * i + 3], (double)hm[8 * i + 4], (double)hm[8 * i + 5]};
+            require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
+                    "layout is not finite");
+            const GridDecision gd = decide_grid(bb, p->box_width, p->min_boxes, p->max_boxes);
+            g[i] = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, ni);
+            M[i] = fft_size(g[i].N);
+            require(M[i] > 0, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+            box_bits = std::max(box_bits, bits_for((long long)gd.nb * gd.nb));
+            const double delta = gd.width / ni;
+            fresh[i] = !(sM[i] == M[i] && sdelta[i] == delta);
+            const size_t Mh = M[i] / 2 + 1, N = g[i].N, nch = spread_chunks(n[i]);
+            if (fresh[i]) {
+                sM[i] = M[i]; sdelta[i] = delta;
+                for (int t = 0; t < ntab; ++t) K[i][t].ensure(Mh * Mh * sizeof(T));
+                psp[i] = osp; osp += al(4 * Mh * Mh);
+            }
+            pW[i] = oW; oW += al(3 * N * N);
+            pWh[i] = oWh; oWh += al(3 * Mh * N);
+            pPh[i] = oPh; oPh += al(4 * Mh * N);
+            pTr[i] = oTr; oTr += al(4 * Mh * N);
+            pphi[i] = ophi; ophi += al(4 * N * N);
+            pit[i] = oit; oit += al(2 * nch);
+            piv[i] = oiv; oiv += al(2 * nch * ni * 3 * ni);
+            pzp[i] = ozp; ozp += al(gather_blocks(n[i]));
+        }
+        grow(aW, oW * sizeof(T)); grow(aWh, oWh * sizeof(cplx<T>)); grow(aPh, oPh * sizeof(cplx<T>));
+        grow(aTr, oTr * sizeof(cplx<T>)); grow(aphi, ophi * sizeof(T)); grow(aik, oit * sizeof(int));
+        grow(aiv, oit * sizeof(int)); grow(aival, oiv * sizeof(T)); grow(azp, ozp * sizeof(double));
+        if (osp) grow(aspec, osp * sizeof(cplx<T>));
+        const bool early = it < p->early_iter;
+        const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
+        const T mom = (T)(early ? p->momentum0 : p->momentum1);
+        BInst<T>* h = host_table();
+        unsigned* cnt = dptr<unsigned>(counters);
+        for (int i = 0; i < count; ++i) {
+            BInst<T>& b = h[i];
+            b.g = g[i]; b.M = M[i];
+            b.item_key = dptr<int>(aik) + pit[i]; b.item_valid = dptr<int>(aiv) + pit[i]; b.item_val = dptr<T>(aival) + piv[i];
+            b.W = dptr<T>(aW) + pW[i]; b.Wh = dptr<cplx<T>>(aWh) + pWh[i]; b.Ph = dptr<cplx<T>>(aPh) + pPh[i];
+            b.Tr = dptr<cplx<T>>(aTr) + pTr[i]; b.phi = dptr<T>(aphi) + pphi[i];
+            for (int t = 0; t < 4; ++t) { b.K[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; b.Kw[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; }
+            b.zpart = dptr<double>(azp) + pzp[i]; b.counter = cnt + 2 * i; b.zout = dptr<double>(zout) + 2 * i;
+            b.inv_out = dptr<T>(inv) + 4 * i; b.sumq = dptr<double>(sumq) + (size_t)i * std::max(1, p->n_iter) + it;
+            b.upart = dptr<double>(upart) + (size_t)i * 6 * 1024; b.ucounter = cnt + 2 * i + 1; b.mb = dptr<T>(mbd) + 8 * i;
+            b.delta = sdelta[i]; b.spec_tmp = fresh[i] ? dptr<cplx<T>>(aspec) + psp[i] : nullptr;
+        }
+        MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
+        const BInst<T>* I = dev_table();
+        std::vector<int> all(count);
+        for (int i = 0; i < count; ++i) all[i] = i;
+        std::vector<int> sizes;
+        for (int i = 0; i < count; ++i) if (std::find(sizes.begin(), sizes.end(), M[i]) == sizes.end()) sizes.push_back(M[i]);
+        std::sort(sizes.begin(), sizes.end());
+        auto of_size = [&](int Ms, bool only_fresh) {
+            std::vector<int> w;
+            for (int i = 0; i < count; ++i) if (M[i] == Ms && (!only_fresh || fresh[i])) w.push_back(i);
+            return w;
+        };
+        // kernel spectra of the changed grids
+        for (int Ms : sizes) {
+            const std::vector<int> w = of_size(Ms, true);
+            if (w.empty()) continue;
+            const int Mh = Ms / 2 + 1;
+            const auto m1 = make_maps(w, [&](int) { return fft_col_blocks(Ms); });
+            const auto mt = make_maps(w, [&](int) { return transpose_blocks(Mh, Mh, ntab == 4 ? 2 : 1); });
+            for (size_t k = 0; k < m1.size(); ++k) launch_spectra_batch<T>(I, m1[k], mt[k], Ms, p->dof, centred, ntab, st);
+        }
+        // box sort of all points by (instance, box)
+        for (const BMap& m : make_maps(all, [&](int i) { return key_blocks(n[i]); }))
+            launch_keys_batch<T>(I, m, dptr<T>(Yc), box_bits, dptr<int>(keysA), dptr<int>(valsA), st);
+        sorted = launch_key_sort_masked(dptr<int>(keysA), dptr<int>(valsA), dptr<int>(keysB), dptr<int>(valsB), dptr<int>(hist),
+                                        ntot, box_bits + bits_for(count), box_bits, st);
+        // spread
+        launch_fill<T>(dptr<T>(aW), T(0), (i64)oW, st);
+        for (const BMap& m : make_maps(all, [&](int i) { return spread_chunks(n[i]); }))
+            launch_spread_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, st);
+        // FFT convolution
+        for (int Ms : sizes)
+            for (const BMap& m : make_maps(of_size(Ms, false), [&](int i) { return fft_row_blocks(g[i].N, Ms, 3); }))
+                launch_rows_forward_batch<T>(I, m, Ms, st);
+        for (const BMap& m : make_maps(all, [&](int i) { return transpose_blocks(g[i].N, M[i] / 2 + 1, 3); }))
+            launch_transpose_batch<T>(I, m, 0, st);
+        for (int Ms : sizes)
+            for (const BMap& m : make_maps(of_size(Ms, false), [&](int) { return fft_col_blocks(Ms); }))
+                launch_columns_batch<T>(I, m, Ms, st);
+        for (const BMap& m : make_maps(all, [&](int i) { return transpose_blocks(M[i] / 2 + 1, g[i].N, 4); }))
+            launch_transpose_batch<T>(I, m, 1, st);
+        for (int Ms : sizes)
+            for (const BMap& m : make_maps(of_size(Ms, false), [&](int i) { return fft_row_blocks(g[i].N, Ms, 4); }))
+                launch_rows_inverse_batch<T>(I, m, Ms, st);
+        // gather, Z, and the update
+        for (const BMap& m : make_maps(all, [&](int i) { return gather_blocks(n[i]); }))
+            launch_gather_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, dptr<T>(Fc), st);
+        for (const BMap& m : make_maps(all, [&](int i) { return update_blocks(n[i]); }))
+            launch_update_batch<T>(I, m, dptr<T>(Yc), dptr<T>(Uc), dptr<T>(Gc), dptr<T>(Ac), dptr<T>(Fc), rho, mom, st);
+        MG_HIP(hipMemcpyAsync(hmb.p, mbd.p, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipEventRecord(ready, st));
+        if (it + 1 < p->n_iter) attraction();
+    }
+
+    void complete() { MG_HIP(hipEventSynchronize(ready)); }
+
+    void finish() {
+        const T* Y = dptr<T>(Yc); const T* U = dptr<T>(Uc); const T* G = dptr<T>(Gc);
+        for (int i = 0; i < count; ++i) {
+            const size_t b = (size_t)2 * n[i] * sizeof(T);
+            MG_HIP(hipMemcpyAsync(user[i]->Y, Y + 2 * off[i], b, hipMemcpyDeviceToDevice, st));
+            MG_HIP(hipMemcpyAsync(user[i]->U, U + 2 * off[i], b, hipMemcpyDeviceToDevice, st));
+            MG_HIP(hipMemcpyAsync(user[i]->G, G + 2 * off[i], b, hipMemcpyDeviceToDevice, st));
+            if (user[i]->h_sumQ && p->n_iter > 0)
+                MG_HIP(hipMemcpyAsync(user[i]->h_sumQ, dptr<double>(sumq) + (size_t)i * p->n_iter,
+                                      (size_t)p->n_iter * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        MG_HIP(hipStreamSynchronize(st));
+    }
+};
+
+// The batch in n_groups groups of about equal point counts (largest instances dealt first), each group one
+// batched engine on its own HIP stream; group s is driven by host thread s mod n_threads. While the host
+// decides the grids of one group, the device runs the others.
+template <class T> void optimize_batch(const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_groups,
+                                      int n_threads) {
+    require(p != nullptr, "params must not be null");
+    require(count >= 1 && inst != nullptr, "no instances");
+    require(n_groups >= 1 && n_groups <= 64, "n_streams must be in [1, 64]");
+    require(n_threads >= 1 && n_threads <= n_groups, "n_threads must be in [1, n_streams]");
+    const int S = std::min(n_groups, count), NT = std::min(n_threads, S);
+    std::vector<int> order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].n > inst[b].n; });
+    std::vector<std::vector<const mg_tsne_instance*>> groups(S);
+    std::vector<i64> load(S, 0);
+    for (int i : order) {
+        const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups[s].push_back(&inst[i]);
+        load[s] += inst[i].n;
+    }
+    std::vector<hipStream_t> streams(S, nullptr);
+    struct Guard { std::vector<hipStream_t>& s; ~Guard() { for (auto x : s) if (x) (void)hipStreamDestroy(x); } } guard{streams};
+    for (auto& s : streams) MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<std::unique_ptr<BatchEngine<T>>> eng;
+    for (int s = 0; s < S; ++s) {
+        eng.emplace_back(new BatchEngine<T>());
+        eng.back()->begin(p, groups[s], streams[s]);
+    }
+    int dev = 0;
+    MG_HIP(hipGetDevice(&dev));
+    std::atomic<bool> failed{false};
+    std::vector<std::string> errs(NT);
+    std::vector<int> codes(NT, 0);
+    auto work = [&](int t) {
+        try {
+            MG_HIP(hipSetDevice(dev));
+            std::vector<BatchEngine<T>*> mine;
+            for (int s = t; s < S; s += NT) mine.push_back(eng[s].get());
+            if (p->n_iter > 0) for (auto* e : mine) e->issue(0);
+            for (int it = 1; it < p->n_iter && !failed.load(std::memory_order_relaxed); ++it)
+                for (auto* e : mine) { e->complete(); e->issue(it); }
+            for (auto* e : mine) { e->complete(); e->finish(); }
+        } catch (const Error& e) {
+            codes[t] = e.code; errs[t] = e.what(); failed = true;
+        } catch (const std::exception& e) {
+            codes[t] = 1; errs[t] = e.what(); failed = true;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < NT; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (auto x : streams) (void)hipStreamSynchronize(x);
+    for (int t = 0; t < NT; ++t) if (codes[t]) throw Error(codes[t], errs[t]);
 }
 
 }  // namespace mg
@@ -710,12 +1002,13 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
     MG_API_END
 }
 
-int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams) {
+int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams,
+                           int n_threads) {
     MG_API_BEGIN
     check_prec(prec);
     require(p != nullptr, "params must not be null");
-    if (prec == MG_F64) optimize_batch<double>(p, count, inst, n_streams);
-    else optimize_batch<float>(p, count, inst, n_streams);
+    if (prec == MG_F64) optimize_batch<double>(p, count, inst, n_streams, n_threads);
+    else optimize_batch<float>(p, count, inst, n_streams, n_threads);
     MG_API_END
 }
 

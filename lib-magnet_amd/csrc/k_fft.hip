@@ -2,6 +2,7 @@
 // their launchers live in k_fft_impl.hpp, compiled once per precision (k_fft32.hip, k_fft64.hip).
 #include "common.hpp"
 #include "fft_core.hpp"
+#include "batch.hpp"
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -81,6 +82,11 @@ template <class T> static TwSet<T> set_of(const Tables& t, const cplx<T>* base, 
 }
 template <> TwSet<float> fft_twiddle_set<float>(int L) { const Tables& t = g_tw[current_device()]; return set_of<float>(t, t.f32, L); }
 template <> TwSet<double> fft_twiddle_set<double>(int L) { const Tables& t = g_tw[current_device()]; return set_of<double>(t, t.f64, L); }
+
+// blocks of the batched row and column passes (transforms per workgroup as in fft_batch)
+static int batch_of(int L) { (void)size_index(L); return L / 16 >= 256 ? 1 : 256 / (L / 16); }
+int fft_row_blocks(int N, int M, int channels) { const int B = batch_of(M / 2); return (N + B - 1) / B * channels; }
+int fft_col_blocks(int M) { const int B = batch_of(M); return (M / 2 + 1 + B - 1) / B; }
 
 // the transform sizes with compiled plans: 2^s and 3 * 2^s from 128 to 8192
 int fft_size(int N) {

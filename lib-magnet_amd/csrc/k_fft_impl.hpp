@@ -17,6 +17,7 @@
 #pragma once
 #include "common.hpp"
 #include "fft_core.hpp"
+#include "batch.hpp"
 #include <algorithm>
 
 namespace mg {
@@ -37,15 +38,15 @@ template <class T, int L> __device__ inline cplx<T> r2c_post(const cplx<T>* lds,
 
 // ---- rows forward: blockIdx.y = channel, fft_batch rows per workgroup
 template <class T, int L>
-__global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict__ W, int N, cplx<T>* __restrict__ out,
-                                                             TwSet<T> tws) {
+__device__ __forceinline__ void rows_fwd_block(const T* __restrict__ W, int N, cplx<T>* __restrict__ out, const TwSet<T>& tws,
+                                               int bx, int by) {
     using Pl = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, R0 = Pl::R[0];
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
-    const int c = blockIdx.y, gx = blockIdx.x * B + b;
+    const int c = by, gx = bx * B + b;
     const bool live = gx < N, even = (N & 1) == 0;
     const T* src = W + ((size_t)c * N + (live ? gx : 0)) * N;
     auto ld = [&](int e, int, int r) -> cplx<T> {
@@ -65,13 +66,27 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict
     cplx<T>* dst = out + ((size_t)c * N + gx) * Mh;
     for (int k = t; k < Mh; k += TH) dst[k] = r2c_post<T, L>(lds, b * L, k, tws.half);
 }
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict__ W, int N, cplx<T>* __restrict__ out,
+                                                             TwSet<T> tws) {
+    rows_fwd_block<T, L>(W, N, out, tws, blockIdx.x, blockIdx.y);
+}
+// batched: instance i's blocks cover (ceil(N / batch) row groups) x (3 channels), row groups fastest
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void kb_rows_fwd(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const int nbx = (b.g.N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
+    rows_fwd_block<T, L>(b.W, b.g.N, b.Tr, tws, lb % nbx, lb / nbx);
+}
 
 // ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
 template <class T>
-__global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C) {
+__device__ __forceinline__ void transpose_block(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C,
+                                                int bx, int by, int bz) {
     __shared__ cplx<T> tile[32][33];
-    const size_t off = (size_t)blockIdx.z * R * C;
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const size_t off = (size_t)bz * R * C;
+    const int c0 = bx * 32, r0 = by * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
     for (int i = ty; i < 32; i += 8) {
         const int r = r0 + i, c = c0 + tx;
@@ -82,6 +97,25 @@ __global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ s
         const int c = c0 + i, r = r0 + tx;
         if (r < R && c < C) dst[off + (size_t)c * R + r] = tile[tx][i];
     }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C) {
+    transpose_block<T>(src, dst, R, C, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+// batched transposes of each instance's buffers (which: see launch_transpose_batch)
+template <class T>
+__global__ __launch_bounds__(256) void kb_transpose(const BInst<T>* __restrict__ I, BMap m, int which) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const int Mh = b.M / 2 + 1;
+    const cplx<T>* src;
+    cplx<T>* dst;
+    int R, C;
+    if (which == 0) { src = b.Tr; dst = b.Wh; R = b.g.N; C = Mh; }
+    else if (which == 1) { src = b.Ph; dst = b.Tr; R = Mh; C = b.g.N; }
+    else { src = b.spec_tmp; dst = b.spec_tmp + 2 * (size_t)Mh * Mh; R = Mh; C = Mh; }
+    const int nx = (C + 31) / 32, ny = (R + 31) / 32, lb = blockIdx.x - m.start[k];
+    transpose_block<T>(src, dst, R, C, lb % nx, (lb / nx) % ny, lb / (nx * ny));
 }
 
 // kernel table value at column frequency kx (stored for kx <= M/2; `odd` flips the sign above M/2)
@@ -95,10 +129,10 @@ template <class T, int M> __device__ __forceinline__ T ktab(const T* __restrict_
 // registers (slot (q, r) of thread t holds frequency t + q TH + r M / 16); each output is a product formed
 // in those registers and transformed back by the reversed plan, with no LDS round trip in between.
 template <class T, int L, bool CEN>
-__global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
-                                                            const T* __restrict__ K1, const T* __restrict__ K2,
-                                                            const T* __restrict__ TX, const T* __restrict__ TY,
-                                                            cplx<T>* __restrict__ Ph, TwSet<T> tws) {
+__device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, int N,
+                                              const T* __restrict__ K1, const T* __restrict__ K2,
+                                              const T* __restrict__ TX, const T* __restrict__ TY,
+                                              cplx<T>* __restrict__ Ph, const TwSet<T>& tws, int bx) {
     using F = typename FwdPlan<T, L>::type;
     using I = Rev<F>;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L / 2 + 1, R0 = F::R[0], RL = F::R[F::P - 1];
@@ -107,7 +141,7 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
-    const int ky = blockIdx.x * B + b;
+    const int ky = bx * B + b;
     const bool live = ky < Mh;
     const int kyc = live ? ky : 0;
     cplx<T> s0[QL][RL], v[QL][RL];
@@ -162,19 +196,32 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
         inv(ch + 1, v);
     }
 }
+template <class T, int L, bool CEN>
+__global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
+                                                            const T* __restrict__ K1, const T* __restrict__ K2,
+                                                            const T* __restrict__ TX, const T* __restrict__ TY,
+                                                            cplx<T>* __restrict__ Ph, TwSet<T> tws) {
+    columns_block<T, L, CEN>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x);
+}
+template <class T, int L, bool CEN>
+__global__ __launch_bounds__(fft_block<L>()) void kb_columns(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    columns_block<T, L, CEN>(b.Wh, b.g.N, b.K[0], b.K[1], b.K[2], b.K[3], b.Ph, tws, blockIdx.x - m.start[k]);
+}
 
 // ---- rows inverse: blockIdx.y = channel, fft_batch rows per workgroup; planar output (4, N, N)
 // Z_e = E + i O with E = X_e + conj X_{L-e}, O = (X_e - conj X_{L-e}) exp(+2 pi i e / 2L)
 template <class T, int L>
-__global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __restrict__ X, int N, T* __restrict__ phi,
-                                                             T scale, TwSet<T> tws) {
+__device__ __forceinline__ void rows_inv_block(const cplx<T>* __restrict__ X, int N, T* __restrict__ phi, T scale,
+                                               const TwSet<T>& tws, int bx, int by) {
     using Pl = Rev<typename FwdPlan<T, L>::type>;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1];
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
-    const int c = blockIdx.y, gx = blockIdx.x * B + b;
+    const int c = by, gx = bx * B + b;
     const bool live = gx < N, even = (N & 1) == 0;
     const cplx<T>* src = X + ((size_t)c * N + (live ? gx : 0)) * Mh;
     auto ld = [&](int e, int, int) -> cplx<T> {
@@ -192,6 +239,18 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __re
         else { dst[n0] = v.x * scale; if (n0 + 1 < N) dst[n0 + 1] = v.y * scale; }
     };
     fft_run<T, Pl, +1, TH, FN>(t, lds, b * L, tws.rev, ld, st);
+}
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __restrict__ X, int N, T* __restrict__ phi,
+                                                             T scale, TwSet<T> tws) {
+    rows_inv_block<T, L>(X, N, phi, scale, tws, blockIdx.x, blockIdx.y);
+}
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void kb_rows_inv(const BInst<T>* __restrict__ I, BMap m, T scale, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const int nbx = (b.g.N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
+    rows_inv_block<T, L>(b.Tr, b.g.N, b.phi, scale, tws, lb % nbx, lb / nbx);
 }
 
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
@@ -220,16 +279,15 @@ __device__ inline void kernel_values(int a, int b, int M, double delta, double d
 // imaginary parts of the result are their (real) spectra T1, T2. FFT(yK2_a) = i S (odd row). Output
 // P0[a][ky] = (T1, T2), P1[a][ky] = (S, x_a T2) -- the row transform of x K2 is x_a times that of K2.
 template <class T, int L, int NT>
-__global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double dof, const double* __restrict__ table,
-                                                          int round_kernel, cplx<T>* __restrict__ P0, cplx<T>* __restrict__ P1,
-                                                          TwSet<T> tws) {
+__device__ __forceinline__ void spec1_block(double delta, double dof, const double* __restrict__ table, int round_kernel,
+                                            cplx<T>* __restrict__ P0, cplx<T>* __restrict__ P1, const TwSet<T>& tws, int bx) {
     using F = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
     constexpr int QL = L / RL / TH;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
-    const int a = blockIdx.x * B + bb;
+    const int a = bx * B + bb;
     const bool live = a < Mh;
     const int ac = live ? a : 0;
     cplx<T> za[QL][RL], zb[QL][RL];
@@ -262,21 +320,35 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double d
             if constexpr (NT == 4) P1[(size_t)a * Mh + k] = {zb[q][r].y, xa * za[q][r].y};
         }
 }
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void k_spec1(double delta, double dof, const double* __restrict__ table,
+                                                          int round_kernel, cplx<T>* __restrict__ P0, cplx<T>* __restrict__ P1,
+                                                          TwSet<T> tws) {
+    spec1_block<T, L, NT>(delta, dof, table, round_kernel, P0, P1, tws, blockIdx.x);
+}
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void kb_spec1(const BInst<T>* __restrict__ I, BMap m, double dof,
+                                                           const double* __restrict__ table, int round_kernel, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const size_t plane = (size_t)(L / 2 + 1) * (L / 2 + 1);
+    spec1_block<T, L, NT>(b.delta, dof, table, round_kernel, b.spec_tmp, b.spec_tmp + plane, tws, blockIdx.x - m.start[k]);
+}
 
 // Stage 2, per ky, along a (inputs transposed to [ky][a], a <= M/2, mirrored: even, and odd for x K2).
 // FFT(T1 + i T2) gives K1 and K2 as real and imaginary parts; FFT(S + i X) = S^ - U with S^ even and U odd
 // in kx (FFT(X) = i U), split through Z(kx) and Z(M - kx): Ty = S^, Tx = U.
 template <class T, int L, int NT>
-__global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restrict__ Q0, const cplx<T>* __restrict__ Q1,
-                                                          T* __restrict__ K1, T* __restrict__ K2, T* __restrict__ TX,
-                                                          T* __restrict__ TY, TwSet<T> tws) {
+__device__ __forceinline__ void spec2_block(const cplx<T>* __restrict__ Q0, const cplx<T>* __restrict__ Q1,
+                                            T* __restrict__ K1, T* __restrict__ K2, T* __restrict__ TX,
+                                            T* __restrict__ TY, const TwSet<T>& tws, int bx) {
     using F = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
     constexpr int QL = L / RL / TH;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
-    const int ky = blockIdx.x * B + bb;
+    const int ky = bx * B + bb;
     const bool live = ky < Mh;
     const size_t row = (size_t)(live ? ky : 0) * Mh;
     cplx<T> z[QL][RL];
@@ -312,6 +384,20 @@ __global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restr
             TX[row + k] = T(-0.5) * (zk - zc);
         }
     }
+}
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void k_spec2(const cplx<T>* __restrict__ Q0, const cplx<T>* __restrict__ Q1,
+                                                          T* __restrict__ K1, T* __restrict__ K2, T* __restrict__ TX,
+                                                          T* __restrict__ TY, TwSet<T> tws) {
+    spec2_block<T, L, NT>(Q0, Q1, K1, K2, TX, TY, tws, blockIdx.x);
+}
+template <class T, int L, int NT>
+__global__ __launch_bounds__(fft_block<L>()) void kb_spec2(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const size_t plane = (size_t)(L / 2 + 1) * (L / 2 + 1);
+    spec2_block<T, L, NT>(b.spec_tmp + 2 * plane, b.spec_tmp + 3 * plane, b.Kw[0], b.Kw[1], b.Kw[2], b.Kw[3], tws,
+                          blockIdx.x - m.start[k]);
 }
 
 // potentials planar (4, N, N) -> interleaved (N, N, 4)
@@ -412,12 +498,81 @@ template <class T> void launch_phi_interleave(const T* planar, T* out, i64 NN, h
     MG_HIP(hipGetLastError());
 }
 
+template <class T> void launch_rows_forward_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const dim3 grid((unsigned)m.start[m.count]);
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(kb_rows_fwd<T, L_>, by); \
+        hipLaunchKernelGGL((kb_rows_fwd<T, L_>), grid, dim3(fft_block<L_>()), by, st, I, m, fft_twiddle_set<T>(L_)); \
+        break; }
+    switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_transpose_batch(const BInst<T>* I, const BMap& m, int which, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    hipLaunchKernelGGL(kb_transpose<T>, dim3((unsigned)m.start[m.count]), dim3(256), 0, st, I, m, which);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_columns_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    constexpr bool CEN = sizeof(T) == 4;
+    const dim3 grid((unsigned)m.start[m.count]);
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(kb_columns<T, L_, CEN>, by); \
+        hipLaunchKernelGGL((kb_columns<T, L_, CEN>), grid, dim3(fft_block<L_>()), by, st, I, m, fft_twiddle_set<T>(L_)); \
+        break; }
+    switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_rows_inverse_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const T scale = (T)(1.0 / ((double)M * (double)M));
+    const dim3 grid((unsigned)m.start[m.count]);
+#define MG_C(L_) case L_: { \
+        const int by = lds_bytes<T, L_>(); set_lds(kb_rows_inv<T, L_>, by); \
+        hipLaunchKernelGGL((kb_rows_inv<T, L_>), grid, dim3(fft_block<L_>()), by, st, I, m, scale, fft_twiddle_set<T>(L_)); \
+        break; }
+    switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+    MG_HIP(hipGetLastError());
+}
+
+// spectra of the instances in m1 (all of FFT size M): stage 1, transpose (blocks mt), stage 2
+template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, int M, double dof,
+                                             bool round_kernel, int ntab, hipStream_t st) {
+    if (m1.count == 0 || m1.start[m1.count] == 0) return;
+    require(ntab == 2 || ntab == 4, "ntab must be 2 or 4");
+    const dim3 grid((unsigned)m1.start[m1.count]);
+#define MG_S(L_, NT) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(kb_spec1<T, L_, NT>, by); set_lds(kb_spec2<T, L_, NT>, by); \
+        hipLaunchKernelGGL((kb_spec1<T, L_, NT>), grid, dim3(fft_block<L_>()), by, st, I, m1, dof, (const double*)nullptr, (int)round_kernel, fft_twiddle_set<T>(L_)); \
+        MG_HIP(hipGetLastError()); \
+        launch_transpose_batch<T>(I, mt, NT == 4 ? 3 : 2, st); \
+        hipLaunchKernelGGL((kb_spec2<T, L_, NT>), grid, dim3(fft_block<L_>()), by, st, I, m1, fft_twiddle_set<T>(L_)); \
+    } while (0)
+#define MG_C(L_) case L_: if (ntab == 4) MG_S(L_, 4); else MG_S(L_, 2); break;
+    switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
+#undef MG_C
+#undef MG_S
+    MG_HIP(hipGetLastError());
+}
+
 #define MG_FFT_INST(T) \
     template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
     template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
     template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t); \
-    template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t);
+    template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t); \
+    template void launch_rows_forward_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
+    template void launch_transpose_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
+    template void launch_columns_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
+    template void launch_rows_inverse_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
+    template void launch_spectra_batch<T>(const BInst<T>*, const BMap&, const BMap&, int, double, bool, int, hipStream_t);
 
 }  // namespace mg

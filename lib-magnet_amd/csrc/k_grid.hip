@@ -6,6 +6,7 @@
 //      chunk that holds all its points, or by the span pass for boxes whose points cross chunk ends;
 //   3. gather: potentials at the points in box order, forces, and Z by a fixed-order reduction.
 #include "common.hpp"
+#include "batch.hpp"
 
 namespace mg {
 
@@ -112,7 +113,7 @@ template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
                                                     const int* __restrict__ vals_in, i64 n, int shift,
                                                     const int* __restrict__ hist, const int* __restrict__ tot, int nblocks,
-                                                    int* __restrict__ keys_out, int* __restrict__ vals_out) {
+                                                    int* __restrict__ keys_out, int* __restrict__ vals_out, int kmask) {
     constexpr int NBK = 1 << DB, PER = NBK / TPB > 0 ? NBK / TPB : 1;
     __shared__ int off[NBK];
     __shared__ int wcnt[TPB / WAVE][NBK];
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, Gri
         if (valid) {
             int pos = off[d] + rank;
             for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-            keys_out[pos] = key; vals_out[pos] = val;
+            keys_out[pos] = key & kmask; vals_out[pos] = val;
         }
         __syncthreads();
         for (int k = t; k < NBK; k += TPB) {
@@ -178,25 +179,25 @@ template <class T, int V> __device__ inline void seg_combine(bool fa, const T* v
 }
 
 template <class T, int NI, bool CEN>
-__global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, const int* __restrict__ vals,
-                                                const T* __restrict__ Y, i64 n, GridT<T> g, T* __restrict__ W,
-                                                int* __restrict__ item_key, int* __restrict__ item_valid,
-                                                T* __restrict__ item_val) {
+__device__ __forceinline__ void spread_block(const int* __restrict__ keys, const int* __restrict__ vals,
+                                             const T* __restrict__ Y, i64 n, const GridT<T>& g, T* __restrict__ W,
+                                             int* __restrict__ item_key, int* __restrict__ item_valid,
+                                             T* __restrict__ item_val, int bid) {
     constexpr int V = 3 * NI;
     __shared__ T sA[V][TPB];
     __shared__ int sF[TPB];
     typedef typename vec2<T>::type V2;
     const int t = threadIdx.x;
-    const i64 c0 = (i64)blockIdx.x * SP_CHUNK;
+    const i64 c0 = (i64)bid * SP_CHUNK;
     const i64 cend = min(c0 + (i64)SP_CHUNK, n);
     const int key_first = keys[c0], key_last = keys[cend - 1];
     const bool starts_new = c0 == 0 || keys[c0 - 1] != key_first;
     const bool ends_closed = cend == n || keys[cend] != key_last;
     if (t == 0) {
-        item_key[2 * blockIdx.x] = key_first;
-        item_valid[2 * blockIdx.x] = !starts_new;
-        item_key[2 * blockIdx.x + 1] = key_last;
-        item_valid[2 * blockIdx.x + 1] = !ends_closed && (key_first != key_last || starts_new);
+        item_key[2 * bid] = key_first;
+        item_valid[2 * bid] = !starts_new;
+        item_key[2 * bid + 1] = key_last;
+        item_valid[2 * bid + 1] = !ends_closed && (key_first != key_last || starts_new);
     }
     const i64 p0 = c0 + (i64)t * SP_PT;
     int key[SP_PT];
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, co
                     W[((size_t)2 * g.N + gx) * g.N + gy] = tot[3 * b + 2];
                 }
             } else {
-                const int slot = (inc_start && !starts_new) ? 2 * blockIdx.x : 2 * blockIdx.x + 1;
+                const int slot = (inc_start && !starts_new) ? 2 * bid : 2 * bid + 1;
                 T* dst = item_val + ((size_t)slot * NI + a) * V;
 #pragma unroll
                 for (int v = 0; v < V; ++v) dst[v] = tot[v];
@@ -303,17 +304,25 @@ __global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, co
     }
 }
 
+template <class T, int NI, bool CEN>
+__global__ __launch_bounds__(TPB) void k_spread(const int* __restrict__ keys, const int* __restrict__ vals,
+                                                const T* __restrict__ Y, i64 n, GridT<T> g, T* __restrict__ W,
+                                                int* __restrict__ item_key, int* __restrict__ item_valid,
+                                                T* __restrict__ item_val) {
+    spread_block<T, NI, CEN>(keys, vals, Y, n, g, W, item_key, item_valid, item_val, blockIdx.x);
+}
+
 // boxes whose points cross chunk ends: the chunk where such a box starts sums its open last item with
 // the first items of the following chunks holding the same key (found by binary search on the chunk
 // first keys), in a fixed tree order
 template <class T, int NI>
-__global__ __launch_bounds__(TPB) void k_spread_span(const int* __restrict__ keys, i64 n, int nchunks, GridT<T> g,
-                                                     T* __restrict__ W, const int* __restrict__ item_key,
-                                                     const int* __restrict__ item_valid, const T* __restrict__ item_val) {
+__device__ __forceinline__ void span_block(const int* __restrict__ keys, i64 n, int nchunks, const GridT<T>& g,
+                                           T* __restrict__ W, const int* __restrict__ item_key,
+                                           const int* __restrict__ item_valid, const T* __restrict__ item_val, int bid) {
     constexpr int V = 3 * NI;
     __shared__ T sA[V][TPB];
     __shared__ int s_c1;
-    const int c = blockIdx.x, t = threadIdx.x;
+    const int c = bid, t = threadIdx.x;
     if (!item_valid[2 * c + 1]) return;
     const int skey = item_key[2 * c + 1];
     if (t == 0) {
@@ -354,17 +363,57 @@ __global__ __launch_bounds__(TPB) void k_spread_span(const int* __restrict__ key
         __syncthreads();
     }
 }
+template <class T, int NI>
+__global__ __launch_bounds__(TPB) void k_spread_span(const int* __restrict__ keys, i64 n, int nchunks, GridT<T> g,
+                                                     T* __restrict__ W, const int* __restrict__ item_key,
+                                                     const int* __restrict__ item_valid, const T* __restrict__ item_val) {
+    span_block<T, NI>(keys, n, nchunks, g, W, item_key, item_valid, item_val, blockIdx.x);
+}
+
+// batched spread and span passes: instance i's sorted points sit at [off, off + n) of keys / vals (vals index
+// the concatenated Y), chunked from its own start
+template <class T, int NI, bool CEN>
+__global__ __launch_bounds__(TPB) void kb_spread(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys,
+                                                 const int* __restrict__ vals, const T* __restrict__ Y) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    spread_block<T, NI, CEN>(keys + b.off, vals + b.off, Y, b.n, b.g, b.W, b.item_key, b.item_valid, b.item_val,
+                             blockIdx.x - m.start[k]);
+}
+template <class T, int NI>
+__global__ __launch_bounds__(TPB) void kb_spread_span(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    span_block<T, NI>(keys + b.off, b.n, m.start[k + 1] - m.start[k], b.g, b.W, b.item_key, b.item_valid, b.item_val,
+                      blockIdx.x - m.start[k]);
+}
+
+// batched sort keys: (instance << box_bits) | box key, and the global point index
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_keys(const BInst<T>* __restrict__ I, BMap m, const T* __restrict__ Y, int box_bits,
+                                               int* __restrict__ keys, int* __restrict__ vals) {
+    const int k = bmap_find(m, blockIdx.x);
+    const int inst = m.inst[k];
+    const BInst<T>& b = I[inst];
+    const i64 j = (i64)(blockIdx.x - m.start[k]) * TPB + threadIdx.x;
+    if (j >= b.n) return;
+    const i64 p = b.off + j;
+    typedef typename vec2<T>::type V;
+    const V y = reinterpret_cast<const V*>(Y)[p];
+    keys[p] = (inst << box_bits) | box_key<T>(y.x, y.y, b.g);
+    vals[p] = (int)p;
+}
 
 // -------------------------------------------------------------------------------------------- gather
 template <class T> struct vec4;
 template <> struct vec4<float> { typedef float4 type; };
 template <> struct vec4<double> { typedef double4 type; };
 
-__device__ inline bool gather_last_block(unsigned* counter) {
+__device__ inline bool gather_last_block(unsigned* counter, unsigned nblk) {
     __shared__ bool am_last;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == nblk - 1;
     __syncthreads();
     if (am_last) __threadfence();
     return am_last;
@@ -383,11 +432,11 @@ __device__ inline double block_sum(double v, double* sh) {
 }
 
 template <class T, int NI, bool CEN>
-__global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
-                                                const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
-                                                T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
-                                                double* __restrict__ zout, T* __restrict__ inv_out,
-                                                double* __restrict__ sumq_hist) {
+__device__ __forceinline__ void gather_block(const int* __restrict__ keys, const int* __restrict__ vals,
+                                             const T* __restrict__ Y, i64 n, const GridT<T>& g, const T* __restrict__ phi,
+                                             T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
+                                             double* __restrict__ zout, T* __restrict__ inv_out,
+                                             double* __restrict__ sumq_hist, int bid, int nblk) {
     __shared__ double shd[TPB];
     typedef typename vec2<T>::type V2;
     struct V4 { T x, y, z, w; };
@@ -397,7 +446,7 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
     const T* __restrict__ ph2 = phi + 2 * NN;
     const T* __restrict__ ph3 = phi + 3 * NN;
     double z = 0;
-    for (i64 p = (i64)blockIdx.x * TPB + threadIdx.x; p < n; p += (i64)gridDim.x * TPB) {
+    for (i64 p = (i64)bid * TPB + threadIdx.x; p < n; p += (i64)nblk * TPB) {
         const int key = keys[p], idx = vals[p];
         const V2 pt = reinterpret_cast<const V2*>(Y)[idx];
         int bx, by;
@@ -444,10 +493,10 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
         reinterpret_cast<V2*>(F)[idx] = f;
     }
     z = block_sum(z, shd);
-    if (threadIdx.x == 0) zpart[blockIdx.x] = z;
-    if (!gather_last_block(counter)) return;
+    if (threadIdx.x == 0) zpart[bid] = z;
+    if (!gather_last_block(counter, nblk)) return;
     double s = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) s += zpart[b];
+    for (int b = threadIdx.x; b < nblk; b += TPB) s += zpart[b];
     s = block_sum(s, shd);
     if (threadIdx.x == 0) {
         const double sumq = s - (double)n;
@@ -457,6 +506,22 @@ __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, co
         if (sumq_hist) *sumq_hist = sumq;
         *counter = 0u;
     }
+}
+template <class T, int NI, bool CEN>
+__global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
+                                                const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
+                                                T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
+                                                double* __restrict__ zout, T* __restrict__ inv_out,
+                                                double* __restrict__ sumq_hist) {
+    gather_block<T, NI, CEN>(keys, vals, Y, n, g, phi, F, zpart, counter, zout, inv_out, sumq_hist, blockIdx.x, gridDim.x);
+}
+template <class T, int NI, bool CEN>
+__global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys,
+                                                 const int* __restrict__ vals, const T* __restrict__ Y, T* __restrict__ F) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    gather_block<T, NI, CEN>(keys + b.off, vals + b.off, Y, b.n, b.g, b.phi, F, b.zpart, b.counter, b.zout, b.inv_out,
+                             b.sumq, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 
 }  // namespace
@@ -469,7 +534,8 @@ int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? 
 // keys0 / vals0 (read by the first pass). Pass p writes to buffer (p + first) % 2 (0 = A, 1 = B), first = 0 when
 // keys come from Y and 1 when they are precomputed in A; returns the buffer holding the result.
 template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int* keysA,
-                                              int* valsA, int* keysB, int* valsB, int* hist, int passes, hipStream_t st) {
+                                              int* valsA, int* keysB, int* valsB, int* hist, int passes, hipStream_t st,
+                                              int kmask = -1) {
     const int nblocks = sort_hist_blocks(n);
     int* tot = hist + (size_t)(1 << DB) * nblocks;
     const int first = keys0 ? 1 : 0;
@@ -487,8 +553,9 @@ template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, con
         MG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, hist, nblocks, tot);
         MG_HIP(hipGetLastError());
-        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout);
-        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout);
+        const int mk = p == passes - 1 ? kmask : -1;
+        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk);
         MG_HIP(hipGetLastError());
         kin = kout; vin = vout;
     }
@@ -496,10 +563,10 @@ template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, con
 }
 
 template <class T> static int sort_keys_any(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int bits,
-                                            int* kA, int* vA, int* kB, int* vB, int* hist, hipStream_t st) {
-    if (bits <= 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 1, st);
-    if (bits <= 20) return sort_db<T, 10>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 9) / 10, st);
-    return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 7) / 8, st);
+                                            int* kA, int* vA, int* kB, int* vB, int* hist, hipStream_t st, int kmask = -1) {
+    if (bits <= 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 1, st, kmask);
+    if (bits <= 30) return sort_db<T, 10>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 9) / 10, st, kmask);
+    return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 7) / 8, st, kmask);
 }
 
 template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
@@ -512,6 +579,56 @@ template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, Rep
 // stable sort of precomputed keys (< 2^bits) with their values, both in A; hist as for the box sort
 template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* valsB, int* hist, i64 n, int bits, hipStream_t st) {
     return sort_keys_any<T>(nullptr, nullptr, keysA, valsA, n, bits, keysA, valsA, keysB, valsB, hist, st);
+}
+
+int launch_key_sort_masked(int* keysA, int* valsA, int* keysB, int* valsB, int* hist, i64 n, int bits, int keep_bits,
+                           hipStream_t st) {
+    require(bits >= 1 && bits <= 31 && keep_bits >= 1 && keep_bits <= bits, "bad key widths");
+    const int kmask = keep_bits >= 31 ? -1 : (int)((1u << keep_bits) - 1u);
+    return sort_keys_any<float>(nullptr, nullptr, keysA, valsA, n, bits, keysA, valsA, keysB, valsB, hist, st, kmask);
+}
+
+template <class T> void launch_keys_batch(const BInst<T>* I, const BMap& m, const T* Y, int box_bits, int* keys, int* vals,
+                                          hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    hipLaunchKernelGGL(kb_keys<T>, dim3((unsigned)m.start[m.count]), dim3(TPB), 0, st, I, m, Y, box_bits, keys, vals);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_spread_batch(const BInst<T>* I, const BMap& m, const int* keys, const int* vals, const T* Y,
+                                            int ni, bool centred, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const dim3 grid((unsigned)m.start[m.count]);
+#define MG_S(NI) do { \
+        if (centred) hipLaunchKernelGGL((kb_spread<T, NI, true>), grid, dim3(TPB), 0, st, I, m, keys, vals, Y); \
+        else hipLaunchKernelGGL((kb_spread<T, NI, false>), grid, dim3(TPB), 0, st, I, m, keys, vals, Y); \
+        MG_HIP(hipGetLastError()); \
+        hipLaunchKernelGGL((kb_spread_span<T, NI>), grid, dim3(TPB), 0, st, I, m, keys); \
+    } while (0)
+    switch (ni) {
+        case 1: MG_S(1); break; case 2: MG_S(2); break; case 3: MG_S(3); break; case 4: MG_S(4); break;
+        case 5: MG_S(5); break; case 6: MG_S(6); break; case 7: MG_S(7); break; case 8: MG_S(8); break;
+        default: require(false, "n_interp must be in [1, 8]");
+    }
+#undef MG_S
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_gather_batch(const BInst<T>* I, const BMap& m, const int* keys, const int* vals, const T* Y,
+                                            int ni, bool centred, T* F, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const dim3 grid((unsigned)m.start[m.count]);
+#define MG_G(NI) do { \
+        if (centred) hipLaunchKernelGGL((kb_gather<T, NI, true>), grid, dim3(TPB), 0, st, I, m, keys, vals, Y, F); \
+        else hipLaunchKernelGGL((kb_gather<T, NI, false>), grid, dim3(TPB), 0, st, I, m, keys, vals, Y, F); \
+    } while (0)
+    switch (ni) {
+        case 1: MG_G(1); break; case 2: MG_G(2); break; case 3: MG_G(3); break; case 4: MG_G(4); break;
+        case 5: MG_G(5); break; case 6: MG_G(6); break; case 7: MG_G(7); break; case 8: MG_G(8); break;
+        default: require(false, "n_interp must be in [1, 8]");
+    }
+#undef MG_G
+    MG_HIP(hipGetLastError());
 }
 
 template <class T, int NI, bool CEN> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
@@ -658,7 +775,10 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
     template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
     template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
     template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t); \
-    template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, hipStream_t);
+    template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, hipStream_t); \
+    template void launch_keys_batch<T>(const BInst<T>*, const BMap&, const T*, int, int*, int*, hipStream_t); \
+    template void launch_spread_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, hipStream_t); \
+    template void launch_gather_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, T*, hipStream_t);
 MG_INST(float)
 MG_INST(double)
 

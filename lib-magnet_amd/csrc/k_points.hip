@@ -2,6 +2,7 @@
 // delta-bar-delta / momentum update with recentring. All reductions run in a fixed order (block tree,
 // then the last block over the block partials in index order), so every result is deterministic.
 #include "common.hpp"
+#include "batch.hpp"
 #include <cmath>
 #include <limits>
 
@@ -11,13 +12,13 @@ namespace {
 
 constexpr int TPB = 256;
 
-// The last block of a grid: every block publishes its partial, then bumps `counter`; the block that
-// sees gridDim.x - 1 reads all partials (in block order) and resets the counter for the next launch.
-__device__ inline bool last_block(unsigned* counter) {
+// The last block of a grid of nblk blocks: every block publishes its partial, then bumps `counter`; the block
+// that sees nblk - 1 reads all partials (in block order) and resets the counter for the next launch.
+__device__ inline bool last_block(unsigned* counter, unsigned nblk) {
     __shared__ bool am_last;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == nblk - 1;
     __syncthreads();
     if (am_last) __threadfence();
     return am_last;
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(TPB) void k_bbox(const T* __restrict__ Y, i64 n, T*
     x0 = block_reduce(x0, sh, MinOp<T>()); y0 = block_reduce(y0, sh, MinOp<T>());
     x1 = block_reduce(x1, sh, MaxOp<T>()); y1 = block_reduce(y1, sh, MaxOp<T>());
     if (threadIdx.x == 0) { T* p = part + 4 * blockIdx.x; p[0] = x0; p[1] = y0; p[2] = x1; p[3] = y1; }
-    if (!last_block(counter)) return;
+    if (!last_block(counter, gridDim.x)) return;
     x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
     for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) {
         const T* p = part + 4 * b;
@@ -76,12 +77,12 @@ __global__ __launch_bounds__(TPB) void k_bbox(const T* __restrict__ Y, i64 n, T*
 // so a round costs two memory latencies whatever the row length. Per lane the edges are summed in increasing
 // order, as a one-edge-per-round sweep would.
 template <class T, int G, int U>
-__global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                                               const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
-                                               T* __restrict__ out) {
+__device__ __forceinline__ void attr2_block(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                            const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
+                                            T* __restrict__ out, int bid) {
     typedef typename vec2<T>::type V;
     const V* Y2 = reinterpret_cast<const V*>(Y);
-    const i64 row = (i64)blockIdx.x * (TPB / G) + threadIdx.x / G;
+    const i64 row = (i64)bid * (TPB / G) + threadIdx.x / G;
     const int sub = threadIdx.x % G;
     T ax = 0, ay = 0;
     if (row < n) {
@@ -109,6 +110,19 @@ __global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, c
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) { ax += __shfl_xor(ax, o, G); ay += __shfl_xor(ay, o, G); }
     if (row < n && sub == 0) { V r; r.x = ax; r.y = ay; reinterpret_cast<V*>(out)[row] = r; }
+}
+template <class T, int G, int U>
+__global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                               const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
+                                               T* __restrict__ out) {
+    attr2_block<T, G, U>(ip, ix, P, Y, n, inv_dof, out, blockIdx.x);
+}
+template <class T, int G, int U>
+__global__ __launch_bounds__(TPB) void kb_attr2(const BInst<T>* __restrict__ I, BMap m, const T* __restrict__ Y, T inv_dof,
+                                                T* __restrict__ out) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    attr2_block<T, G, U>(b.ip, b.ix, b.P, Y + 2 * b.off, b.n, inv_dof, out + 2 * b.off, blockIdx.x - m.start[k]);
 }
 
 // any dim in [1, 64] (src/fftrep.cpp:160-176): one thread per row
@@ -145,10 +159,11 @@ template <class T> __device__ inline T dbd(T g_, T& u, T& gain, T lr, T mom) {
 // un-centred layout; the last block writes mb = {mean_x, mean_y, xmin, ymin, xmax, ymax} of the
 // centred layout (rounding is monotone, so min(y_i - m) = min(y_i) - m exactly).
 template <class T, int MODE>
-__global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
-                                                 const T* __restrict__ A, const T* __restrict__ F,
-                                                 const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
-                                                 double* __restrict__ part, unsigned* counter, T* __restrict__ mb) {
+__device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                              const T* __restrict__ A, const T* __restrict__ F,
+                                              const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
+                                              double* __restrict__ part, unsigned* counter, T* __restrict__ mb,
+                                              int bid, int nblk) {
     __shared__ double shd[TPB];
     __shared__ T sht[TPB];
     typedef typename vec2<T>::type V;
@@ -157,7 +172,7 @@ __global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restric
     T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
     V* Y2 = reinterpret_cast<V*>(Y); V* U2 = reinterpret_cast<V*>(U); V* G2 = reinterpret_cast<V*>(Gn);
     const V* A2 = reinterpret_cast<const V*>(A); const V* F2 = reinterpret_cast<const V*>(F);
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+    for (i64 i = (i64)bid * TPB + threadIdx.x; i < n; i += (i64)nblk * TPB) {
         const V a = A2[i];
         V gx;
         if (MODE == 0) { const V f = F2[i]; gx.x = rho * a.x - f.x * inv; gx.y = rho * a.y - f.y * inv; }
@@ -174,12 +189,12 @@ __global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restric
     x0 = block_reduce(x0, sht, MinOp<T>()); y0 = block_reduce(y0, sht, MinOp<T>());
     x1 = block_reduce(x1, sht, MaxOp<T>()); y1 = block_reduce(y1, sht, MaxOp<T>());
     if (threadIdx.x == 0) {
-        double* p = part + 6 * blockIdx.x;
+        double* p = part + 6 * bid;
         p[0] = sx; p[1] = sy; p[2] = (double)x0; p[3] = (double)y0; p[4] = (double)x1; p[5] = (double)y1;
     }
-    if (!last_block(counter)) return;
+    if (!last_block(counter, nblk)) return;
     sx = 0; sy = 0; x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) {
+    for (int b = threadIdx.x; b < nblk; b += TPB) {
         const double* p = part + 6 * b;
         sx += p[0]; sy += p[1];
         x0 = (T)p[2] < x0 ? (T)p[2] : x0; y0 = (T)p[3] < y0 ? (T)p[3] : y0;
@@ -195,15 +210,44 @@ __global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restric
         *counter = 0u;
     }
 }
+template <class T, int MODE>
+__global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                                 const T* __restrict__ A, const T* __restrict__ F,
+                                                 const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
+                                                 double* __restrict__ part, unsigned* counter, T* __restrict__ mb) {
+    update2_block<T, MODE>(Y, U, Gn, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mb, blockIdx.x, gridDim.x);
+}
 
 template <class T>
-__global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const T* __restrict__ mb) {
+__device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const T* __restrict__ mb, int bid, int nblk) {
     typedef typename vec2<T>::type V;
     const T mx = mb[0], my = mb[1];
     V* Y2 = reinterpret_cast<V*>(Y);
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+    for (i64 i = (i64)bid * TPB + threadIdx.x; i < n; i += (i64)nblk * TPB) {
         V y = Y2[i]; y.x -= mx; y.y -= my; Y2[i] = y;
     }
+}
+template <class T>
+__global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const T* __restrict__ mb) {
+    center2_block<T>(Y, n, mb, blockIdx.x, gridDim.x);
+}
+
+// batched fused step of the t-SNE loop (gradient rho * attr - F / sumQ of each instance) and its recentring
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_update2(const BInst<T>* __restrict__ I, BMap m, T* __restrict__ Y, T* __restrict__ U,
+                                                  T* __restrict__ Gn, const T* __restrict__ A, const T* __restrict__ F,
+                                                  T rho, T mom) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    const i64 o = 2 * b.off;
+    update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, b.inv_out, T(0), rho, b.lr, mom, b.n, b.upart, b.ucounter, b.mb,
+                        blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+}
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_center2(const BInst<T>* __restrict__ I, BMap m, T* __restrict__ Y) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    center2_block<T>(Y + 2 * b.off, b.n, b.mb, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 
 // ---- any dim: elementwise update, then one block per dimension for the mean, then centring
@@ -274,16 +318,21 @@ template <class T> void launch_bbox(const T* Y, i64 n, T* part, unsigned* counte
     MG_HIP(hipGetLastError());
 }
 
+// lanes per row: up to a whole wave, about four edges per lane; the long rows of perplexity affinities get a
+// wave each (random neighbour reads from L2 bound the sweep: more rows per wave measured slower)
+int attraction_lanes(i64 n, i64 nnz) {
+    const double avg = (double)nnz / (double)(n > 0 ? n : 1);
+    int G = 1;
+    while (G < 64 && G * 4 < avg) G <<= 1;
+    return G;
+}
+
 template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, const T* P, const T* Y, i64 n, int dim,
                                           T inv_dof, T* out, i64 nnz, hipStream_t st) {
     if (n == 0) return;
     if (dim == 2) {
-        // lanes per row: up to a whole wave, about four edges per lane; the long rows of perplexity affinities
-        // get a wave each (random neighbour reads from L2 bound the sweep: more rows per wave measured slower)
-        const double avg = (double)nnz / (double)n;
-        int G = 1;
-        while (G < 64 && G * 4 < avg) G <<= 1;
-        const unsigned blocks = (unsigned)((n + (TPB / G) - 1) / (TPB / G));
+        const int G = attraction_lanes(n, nnz);
+        const unsigned blocks = (unsigned)attraction_blocks(n, G);
         switch (G) {
 #define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g, 8>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
             MG_ATTR_CASE(1) MG_ATTR_CASE(2) MG_ATTR_CASE(4) MG_ATTR_CASE(8) MG_ATTR_CASE(16) MG_ATTR_CASE(32) MG_ATTR_CASE(64)
@@ -292,6 +341,29 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
     } else {
         hipLaunchKernelGGL(k_attr_any<T>, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, ip, ix, P, Y, n, dim, inv_dof, out);
     }
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_attraction_batch(const BInst<T>* I, const BMap& m, int G, const T* Y, T inv_dof, T* out,
+                                                hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const dim3 grid((unsigned)m.start[m.count]);
+    switch (G) {
+#define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((kb_attr2<T, g, 8>), grid, dim3(TPB), 0, st, I, m, Y, inv_dof, out); break;
+        MG_ATTR_CASE(1) MG_ATTR_CASE(2) MG_ATTR_CASE(4) MG_ATTR_CASE(8) MG_ATTR_CASE(16) MG_ATTR_CASE(32) MG_ATTR_CASE(64)
+#undef MG_ATTR_CASE
+        default: throw Error(1, "lanes per row must be a power of two <= 64");
+    }
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T* Y, T* U, T* G, const T* A, const T* F,
+                                            T rho, T mom, hipStream_t st) {
+    if (m.count == 0 || m.start[m.count] == 0) return;
+    const dim3 grid((unsigned)m.start[m.count]);
+    hipLaunchKernelGGL(kb_update2<T>, grid, dim3(TPB), 0, st, I, m, Y, U, G, A, F, rho, mom);
+    MG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(kb_center2<T>, grid, dim3(TPB), 0, st, I, m, Y);
     MG_HIP(hipGetLastError());
 }
 
@@ -343,7 +415,9 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
     template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, unsigned*, T*, hipStream_t); \
     template void launch_grad<T>(const T*, const T*, const T*, T, T, const T*, const int32_t*, int, T, i64, int, int, T*, hipStream_t); \
     template void launch_mid_spring<T>(const T*, const int32_t*, T*, i64, int, int, T, hipStream_t); \
-    template void launch_fill<T>(T*, T, i64, hipStream_t);
+    template void launch_fill<T>(T*, T, i64, hipStream_t); \
+    template void launch_attraction_batch<T>(const BInst<T>*, const BMap&, int, const T*, T, T*, hipStream_t); \
+    template void launch_update_batch<T>(const BInst<T>*, const BMap&, T*, T*, T*, const T*, const T*, T, T, hipStream_t);
 MG_INST(float)
 MG_INST(double)
 template void launch_fill<int>(int*, int, i64, hipStream_t);

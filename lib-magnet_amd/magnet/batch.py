@@ -2,9 +2,10 @@
 
 optimize_tsne_batch runs every instance's full-batch t-SNE loop (the loop of optimize_tsne, magnet/tsne.py:132-214)
 through one C-ABI call, mg_tsne_optimize_batch. Instances are spread over a few HIP streams and advance together,
-so their kernels overlap on the device; each instance's result is the one optimize_tsne gives it alone (same
-kernels, same grids, same order of operations). The stream count is picked by autotune_streams: short timed runs
-of each candidate on copies of the states, the fastest kept.
+issued by a few host threads, so their kernels overlap on the device and the launches of different instances are
+issued in parallel; each instance's result is the one optimize_tsne gives it alone (same kernels, same grids, same
+order of operations). The stream and thread counts are picked by autotune_streams: short timed runs of each
+candidate on copies of the states, the fastest kept.
 
 The reference fits one embedding per call; this form has no counterpart there.
 """
@@ -16,6 +17,7 @@ import scipy.sparse as sp
 __all__ = ["optimize_tsne_batch", "autotune_streams", "device_instances"]
 
 _TUNED = {}
+LAST_SWEEP = []   # [(n_streams, n_threads, seconds)] of the last autotuner sweep
 
 
 def device_instances(Ps, Ys, precision="fp32"):
@@ -35,32 +37,39 @@ def device_instances(Ps, Ys, precision="fp32"):
     return out
 
 
-def autotune_streams(instances, candidates=(1, 2, 4, 8), iters=12, early_exaggeration=12.0, **kw):
-    """Stream count with the shortest run of `iters` iterations of the schedule's late phase on copies of the
-    instances' states (the results of a run do not depend on it). Cached per (instances, total points)."""
+CANDIDATES = ((1, 1), (4, 1), (8, 1), (4, 4), (8, 4), (8, 8), (16, 8), (16, 16))
+
+
+def autotune_streams(instances, candidates=CANDIDATES, iters=12, early_exaggeration=12.0, **kw):
+    """(n_streams, n_threads) with the shortest run of `iters` iterations of the schedule's late phase on copies
+    of the instances' states (the results of a run do not depend on the choice). Cached per (instances, total
+    points, dtype)."""
     import torch
     from . import capi
     key = (len(instances), sum(t[3].shape[0] for t in instances) // 4096, str(instances[0][3].dtype))
     if key in _TUNED:
         return _TUNED[key]
     best, best_t = candidates[0], float("inf")
-    for k in candidates:
+    LAST_SWEEP.clear()
+    for k, nt in candidates:
         if k > len(instances):
             continue
         trial = [(ip, ix, dv, Y.clone(), U.clone(), G.clone()) for ip, ix, dv, Y, U, G in instances]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        capi.optimize_batch(trial, n_iter=iters, early_iter=0, early_exaggeration=early_exaggeration, n_streams=k, **kw)
+        capi.optimize_batch(trial, n_iter=iters, early_iter=0, early_exaggeration=early_exaggeration, n_streams=k,
+                            n_threads=nt, **kw)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
+        LAST_SWEEP.append((k, nt, round(t, 4)))
         if t < best_t:
-            best, best_t = k, t
+            best, best_t = (k, nt), t
     _TUNED[key] = best
     return best
 
 
 def optimize_tsne_batch(Ps, Ys, n_iter=500, early_iter=166, early_exaggeration=12.0, exaggeration=1.0,
-                        momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, precision="fp32", n_streams="auto"):
+                        momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, precision="fp32", n_streams="auto", n_threads=1):
     """Full-batch t-SNE of every (P_b, Y_b) pair on one schedule; learning rate max(200, n_b / early_exaggeration)
     per instance. Returns the layouts (float32 for fp32, float64 for fp64) in the order given."""
     from . import capi
@@ -72,6 +81,10 @@ def optimize_tsne_batch(Ps, Ys, n_iter=500, early_iter=166, early_exaggeration=1
     capi.use_torch_stream()
     inst = device_instances(Ps, Ys, precision)
     kw = dict(exaggeration=exaggeration, momentum=momentum, n_interp=n_interp, box_width=box_width, dof=dof)
-    k = autotune_streams(inst, early_exaggeration=early_exaggeration, **kw) if n_streams == "auto" else int(n_streams)
-    capi.optimize_batch(inst, n_iter=n_iter, early_iter=early_iter, early_exaggeration=early_exaggeration, n_streams=k, **kw)
+    if n_streams == "auto":
+        k, nt = autotune_streams(inst, early_exaggeration=early_exaggeration, **kw)
+    else:
+        k, nt = int(n_streams), int(n_threads)
+    capi.optimize_batch(inst, n_iter=n_iter, early_iter=early_iter, early_exaggeration=early_exaggeration, n_streams=k,
+                        n_threads=nt, **kw)
     return [t[3].cpu().numpy() for t in inst]

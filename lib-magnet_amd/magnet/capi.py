@@ -71,7 +71,7 @@ def lib():
             "mg_gather_forces_2d": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
                                   C.POINTER(C.c_double), CALLBACK, vp], i32),
-            "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32], i32),
+            "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -193,7 +193,7 @@ def optimize(indptr, indices, data, Y, U, G, it0=0, it1=None, n_iter=750, early_
 
 def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.0, exaggeration=1.0, learning_rate=0.0,
                    momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, min_boxes=20, max_boxes=1000, n_streams=4,
-                   sumq=False):
+                   n_threads=1, sumq=False):
     """Runs mg_tsne_optimize_batch on a list of (indptr, indices, data, Y, U, G) device-tensor tuples, all of one
     dtype; the states are updated in place. Returns the per-instance sumQ arrays when sumq is set."""
     import numpy as np
@@ -213,7 +213,7 @@ def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.
             raise ValueError("CSR arrays do not match the layout")
         arr[b] = Instance(ip.data_ptr(), ix.data_ptr(), dv.data_ptr(), ix.numel(), Y.shape[0], Y.data_ptr(),
                           U.data_ptr(), G.data_ptr(), sq[b].ctypes.data_as(C.POINTER(C.c_double)) if sumq else None)
-    check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams)))
+    check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams), int(n_threads)))
     return [s[:n_iter] for s in sq] if sumq else None
 
 

@@ -85,8 +85,10 @@ def test_batch_entry_rejects_empty_and_bad_stream_counts():
     L = capi.lib()
     p = capi.Params(10, 3, 12.0, 1.0, 0.0, 0.5, 0.8, 3, 20, 1000, 0.7, 1.0, 0.0, 0.5)
     arr = (capi.Instance * 1)()
-    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 0, arr, 1) != 0
+    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 0, arr, 1, 1) != 0
     assert b"no instances" in L.mg_last_error()
-    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 0) != 0
+    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 0, 1) != 0
     assert b"n_streams" in L.mg_last_error()
-    assert L.mg_tsne_optimize_batch(7, ctypes.byref(p), 1, arr, 1) != 0
+    assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 2, 3) != 0
+    assert b"n_threads" in L.mg_last_error()
+    assert L.mg_tsne_optimize_batch(7, ctypes.byref(p), 1, arr, 1, 1) != 0

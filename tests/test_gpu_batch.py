@@ -1,5 +1,6 @@
-"""The batched form (mg_tsne_optimize_batch): instances of uneven sizes on several HIP streams give, each, exactly
-the result of the single-instance loop (same kernels, grids and order of operations), for any stream count."""
+"""The batched form (mg_tsne_optimize_batch): every stage launched once for many instances of uneven sizes (in
+groups on several HIP streams and host threads) gives each instance exactly the result of its single-instance
+loop (same block bodies, grids, block counts and order of operations), for any grouping."""
 import numpy as np
 import pytest
 
@@ -50,13 +51,13 @@ def host(t):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-@pytest.mark.parametrize("n_streams", [1, 3])
-def test_batch_equals_single_runs(precision, n_streams):
+@pytest.mark.parametrize("n_streams,n_threads", [(1, 1), (3, 1), (4, 2), (5, 5)])
+def test_batch_equals_single_runs(precision, n_streams, n_threads):
     from magnet.batch import device_instances
     probs = cached()
     Ps, Ys = [p for p, _ in probs], [y for _, y in probs]
     batch = device_instances(Ps, Ys, precision)
-    sq = capi.optimize_batch(batch, n_streams=n_streams, sumq=True, **SCHED)
+    sq = capi.optimize_batch(batch, n_streams=n_streams, n_threads=n_threads, sumq=True, **SCHED)
     for b, single in enumerate(device_instances(Ps, Ys, precision)):
         ip, ix, dv, Y, U, G = single
         s1 = capi.optimize(ip, ix, dv, Y, U, G, 0, SCHED["n_iter"], **SCHED)
@@ -84,3 +85,23 @@ def test_batch_rejects_bad_instances():
         capi.optimize_batch([(ip, ix, dv, Yd, U[:-1], G)], **SCHED)
     with pytest.raises(capi.MgError):
         capi.optimize_batch(inst, n_streams=0, **SCHED)
+
+
+def test_more_than_64_instances_and_mixed_fft_sizes():
+    """70 small instances (block ranges split over several launches per stage), spans far apart so that several
+    FFT sizes are in use at once; every instance equals its single run."""
+    from magnet.affinity import perplexity_affinities
+    from magnet.batch import device_instances
+    Ps, Ys = [], []
+    for i in range(70):
+        rng = np.random.default_rng(50 + i)
+        n = int(rng.integers(150, 420))
+        X = (rng.normal(0, 6, (4, 8))[rng.integers(0, 4, n)] + rng.normal(0, 1, (n, 8))).astype(np.float32)
+        Ps.append(perplexity_affinities(X, 10.0))
+        Ys.append(rng.normal(0, 1e-4, (n, 2)) * (1 + 2e5 * (i % 3)))   # layouts of three spans
+    kw = dict(n_iter=30, early_iter=10)
+    batch = device_instances(Ps, Ys, "fp32")
+    capi.optimize_batch(batch, n_streams=2, n_threads=1, **kw)
+    for b, (ip, ix, dv, Y, U, G) in enumerate(device_instances(Ps, Ys, "fp32")):
+        capi.optimize(ip, ix, dv, Y, U, G, 0, kw["n_iter"], **kw)
+        assert np.array_equal(host(batch[b][3]), host(Y)), b
