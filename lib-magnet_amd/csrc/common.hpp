@@ -58,10 +58,13 @@ template <class T, int NI> __device__ inline void lagrange(T u, T* w) {
 }
 
 // Grid of one repulsion call, in the arithmetic type T (float for the FP32 path, as the reference
-// passes float xmin / ymin / width to FFTConv2D::compute).
+// passes float xmin / ymin / width to FFTConv2D::compute), and in double. The box of a point and its
+// coordinates within the box use the double values, which are not rounded to T: the FP32 path then
+// places every point exactly as the FP64 path does (with a grid origin rounded to float, a point within
+// a rounding of a box edge would change box, and carry the other box's interpolation error).
 template <class T> struct GridT {
     T xmin, ymin, width, inv_w;
-    double xmin_d, ymin_d, width_d, inv_w_d;   // the same grid in double (T's values, exactly)
+    double xmin_d, ymin_d, width_d, inv_w_d;
     int nb, N;
 };
 

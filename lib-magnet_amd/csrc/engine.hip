@@ -117,7 +117,7 @@ template <class T> GridT<T> make_grid(double xmin, double ymin, double width, in
     GridT<T> g;
     g.xmin = (T)xmin; g.ymin = (T)ymin; g.width = (T)width;
     g.inv_w = T(1) / g.width;
-    g.xmin_d = (double)g.xmin; g.ymin_d = (double)g.ymin; g.width_d = (double)g.width;
+    g.xmin_d = xmin; g.ymin_d = ymin; g.width_d = width;   // unrounded: boxes and local coordinates of both paths agree
     g.inv_w_d = 1.0 / g.width_d;
     g.nb = nb; g.N = nb * ni;
     return g;

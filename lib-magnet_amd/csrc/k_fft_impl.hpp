@@ -76,8 +76,10 @@ template <class T, int L>
 __global__ __launch_bounds__(fft_block<L>()) void kb_rows_fwd(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    const int nbx = (b.g.N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
-    rows_fwd_block<T, L>(b.W, b.g.N, b.Tr, tws, lb % nbx, lb / nbx);
+    const int N = b.g.N, nbx = (N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
+    const T* W = b.W;
+    cplx<T>* Tr = b.Tr;
+    rows_fwd_block<T, L>(W, N, Tr, tws, lb % nbx, lb / nbx);
 }
 
 // ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
@@ -207,7 +209,10 @@ template <class T, int L, bool CEN>
 __global__ __launch_bounds__(fft_block<L>()) void kb_columns(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    columns_block<T, L, CEN>(b.Wh, b.g.N, b.K[0], b.K[1], b.K[2], b.K[3], b.Ph, tws, blockIdx.x - m.start[k]);
+    const cplx<T>* Wh = b.Wh;
+    cplx<T>* Ph = b.Ph;
+    const T *K0 = b.K[0], *K1 = b.K[1], *K2 = b.K[2], *K3 = b.K[3];
+    columns_block<T, L, CEN>(Wh, b.g.N, K0, K1, K2, K3, Ph, tws, blockIdx.x - m.start[k]);
 }
 
 // ---- rows inverse: blockIdx.y = channel, fft_batch rows per workgroup; planar output (4, N, N)
@@ -249,8 +254,10 @@ template <class T, int L>
 __global__ __launch_bounds__(fft_block<L>()) void kb_rows_inv(const BInst<T>* __restrict__ I, BMap m, T scale, TwSet<T> tws) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    const int nbx = (b.g.N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
-    rows_inv_block<T, L>(b.Tr, b.g.N, b.phi, scale, tws, lb % nbx, lb / nbx);
+    const int N = b.g.N, nbx = (N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
+    const cplx<T>* Tr = b.Tr;
+    T* phi = b.phi;
+    rows_inv_block<T, L>(Tr, N, phi, scale, tws, lb % nbx, lb / nbx);
 }
 
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even

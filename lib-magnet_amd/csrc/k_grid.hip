@@ -377,15 +377,25 @@ __global__ __launch_bounds__(TPB) void kb_spread(const BInst<T>* __restrict__ I,
                                                  const int* __restrict__ vals, const T* __restrict__ Y) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    spread_block<T, NI, CEN>(keys + b.off, vals + b.off, Y, b.n, b.g, b.W, b.item_key, b.item_valid, b.item_val,
-                             blockIdx.x - m.start[k]);
+    const GridT<T> g = b.g;
+    const i64 off = b.off, n = b.n;
+    T* W = b.W;
+    int* ik = b.item_key;
+    int* iv = b.item_valid;
+    T* ival = b.item_val;
+    spread_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, W, ik, iv, ival, blockIdx.x - m.start[k]);
 }
 template <class T, int NI>
 __global__ __launch_bounds__(TPB) void kb_spread_span(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    span_block<T, NI>(keys + b.off, b.n, m.start[k + 1] - m.start[k], b.g, b.W, b.item_key, b.item_valid, b.item_val,
-                      blockIdx.x - m.start[k]);
+    const GridT<T> g = b.g;
+    const i64 off = b.off, n = b.n;
+    T* W = b.W;
+    const int* ik = b.item_key;
+    const int* iv = b.item_valid;
+    const T* ival = b.item_val;
+    span_block<T, NI>(keys + off, n, m.start[k + 1] - m.start[k], g, W, ik, iv, ival, blockIdx.x - m.start[k]);
 }
 
 // batched sort keys: (instance << box_bits) | box key, and the global point index
@@ -395,12 +405,13 @@ __global__ __launch_bounds__(TPB) void kb_keys(const BInst<T>* __restrict__ I, B
     const int k = bmap_find(m, blockIdx.x);
     const int inst = m.inst[k];
     const BInst<T>& b = I[inst];
+    const GridT<T> g = b.g;
     const i64 j = (i64)(blockIdx.x - m.start[k]) * TPB + threadIdx.x;
     if (j >= b.n) return;
     const i64 p = b.off + j;
     typedef typename vec2<T>::type V;
     const V y = reinterpret_cast<const V*>(Y)[p];
-    keys[p] = (inst << box_bits) | box_key<T>(y.x, y.y, b.g);
+    keys[p] = (inst << box_bits) | box_key<T>(y.x, y.y, g);
     vals[p] = (int)p;
 }
 
@@ -419,16 +430,17 @@ __device__ inline bool gather_last_block(unsigned* counter, unsigned nblk) {
     return am_last;
 }
 
+// sum over a block: a butterfly within each wave (fixed order), then thread 0 adds the waves in index order;
+// the result is valid in thread 0
 __device__ inline double block_sum(double v, double* sh) {
-    sh[threadIdx.x] = v;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();   // the previous call's readers are done with sh
+    if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = v;
     __syncthreads();
-    for (int s = TPB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
+    if (threadIdx.x == 0)
+        for (int k = 1; k < TPB / WAVE; ++k) v += sh[k];
+    return v;
 }
 
 template <class T, int NI, bool CEN>
@@ -520,8 +532,16 @@ __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I,
                                                  const int* __restrict__ vals, const T* __restrict__ Y, T* __restrict__ F) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    gather_block<T, NI, CEN>(keys + b.off, vals + b.off, Y, b.n, b.g, b.phi, F, b.zpart, b.counter, b.zout, b.inv_out,
-                             b.sumq, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+    const GridT<T> g = b.g;
+    const i64 off = b.off, n = b.n;
+    const T* phi = b.phi;
+    double* zpart = b.zpart;
+    unsigned* cnt = b.counter;
+    double* zout = b.zout;
+    T* inv = b.inv_out;
+    double* sumq = b.sumq;
+    gather_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, phi, F, zpart, cnt, zout, inv, sumq,
+                             blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 
 }  // namespace

@@ -43,6 +43,35 @@ template <class T, class Op> __device__ T block_reduce(T v, T* sh, Op op) {
 
 template <class T> __device__ inline T pinf() { return std::numeric_limits<T>::infinity(); }
 
+// mean sums and bbox of a block: a butterfly within each wave (fixed order), then thread 0 adds the waves in
+// index order; one barrier. The result is valid in thread 0.
+template <class T> struct MeanBox { double sx, sy; T x0, y0, x1, y1; };
+template <class T> __device__ inline MeanBox<T> block_meanbox(MeanBox<T> m) {
+    __shared__ double sh[6][TPB / WAVE];
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        m.sx += __shfl_xor(m.sx, o); m.sy += __shfl_xor(m.sy, o);
+        const T a = __shfl_xor(m.x0, o), b = __shfl_xor(m.y0, o), c = __shfl_xor(m.x1, o), d = __shfl_xor(m.y1, o);
+        m.x0 = a < m.x0 ? a : m.x0; m.y0 = b < m.y0 ? b : m.y0;
+        m.x1 = c > m.x1 ? c : m.x1; m.y1 = d > m.y1 ? d : m.y1;
+    }
+    const int w = threadIdx.x / WAVE;
+    __syncthreads();   // the previous call's readers are done with sh
+    if (threadIdx.x % WAVE == 0) {
+        sh[0][w] = m.sx; sh[1][w] = m.sy; sh[2][w] = (double)m.x0; sh[3][w] = (double)m.y0; sh[4][w] = (double)m.x1; sh[5][w] = (double)m.y1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < TPB / WAVE; ++k) {
+            m.sx += sh[0][k]; m.sy += sh[1][k];
+            const T a = (T)sh[2][k], b = (T)sh[3][k], c = (T)sh[4][k], d = (T)sh[5][k];
+            m.x0 = a < m.x0 ? a : m.x0; m.y0 = b < m.y0 ? b : m.y0;
+            m.x1 = c > m.x1 ? c : m.x1; m.y1 = d > m.y1 ? d : m.y1;
+        }
+    }
+    return m;
+}
+
 // ---- bbox_2d (src/fftrep.cpp:415-434)
 template <class T>
 __global__ __launch_bounds__(TPB) void k_bbox(const T* __restrict__ Y, i64 n, T* __restrict__ part,
@@ -164,8 +193,6 @@ __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__
                                               const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
                                               double* __restrict__ part, unsigned* counter, T* __restrict__ mb,
                                               int bid, int nblk) {
-    __shared__ double shd[TPB];
-    __shared__ T sht[TPB];
     typedef typename vec2<T>::type V;
     const T inv = d_inv ? *d_inv : inv_h;
     double sx = 0, sy = 0;
@@ -185,12 +212,10 @@ __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__
         x0 = y.x < x0 ? y.x : x0; x1 = y.x > x1 ? y.x : x1;
         y0 = y.y < y0 ? y.y : y0; y1 = y.y > y1 ? y.y : y1;
     }
-    sx = block_reduce(sx, shd, AddOp<double>()); sy = block_reduce(sy, shd, AddOp<double>());
-    x0 = block_reduce(x0, sht, MinOp<T>()); y0 = block_reduce(y0, sht, MinOp<T>());
-    x1 = block_reduce(x1, sht, MaxOp<T>()); y1 = block_reduce(y1, sht, MaxOp<T>());
+    MeanBox<T> m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
     if (threadIdx.x == 0) {
         double* p = part + 6 * bid;
-        p[0] = sx; p[1] = sy; p[2] = (double)x0; p[3] = (double)y0; p[4] = (double)x1; p[5] = (double)y1;
+        p[0] = m.sx; p[1] = m.sy; p[2] = (double)m.x0; p[3] = (double)m.y0; p[4] = (double)m.x1; p[5] = (double)m.y1;
     }
     if (!last_block(counter, nblk)) return;
     sx = 0; sy = 0; x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
@@ -200,13 +225,11 @@ __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__
         x0 = (T)p[2] < x0 ? (T)p[2] : x0; y0 = (T)p[3] < y0 ? (T)p[3] : y0;
         x1 = (T)p[4] > x1 ? (T)p[4] : x1; y1 = (T)p[5] > y1 ? (T)p[5] : y1;
     }
-    sx = block_reduce(sx, shd, AddOp<double>()); sy = block_reduce(sy, shd, AddOp<double>());
-    x0 = block_reduce(x0, sht, MinOp<T>()); y0 = block_reduce(y0, sht, MinOp<T>());
-    x1 = block_reduce(x1, sht, MaxOp<T>()); y1 = block_reduce(y1, sht, MaxOp<T>());
+    m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
     if (threadIdx.x == 0) {
-        const T mx = (T)(sx / (double)n), my = (T)(sy / (double)n);
+        const T mx = (T)(m.sx / (double)n), my = (T)(m.sy / (double)n);
         mb[0] = mx; mb[1] = my;
-        mb[2] = x0 - mx; mb[3] = y0 - my; mb[4] = x1 - mx; mb[5] = y1 - my;
+        mb[2] = m.x0 - mx; mb[3] = m.y0 - my; mb[4] = m.x1 - mx; mb[5] = m.y1 - my;
         *counter = 0u;
     }
 }
@@ -239,8 +262,13 @@ __global__ __launch_bounds__(TPB) void kb_update2(const BInst<T>* __restrict__ I
                                                   T rho, T mom) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    const i64 o = 2 * b.off;
-    update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, b.inv_out, T(0), rho, b.lr, mom, b.n, b.upart, b.ucounter, b.mb,
+    const i64 o = 2 * b.off, n = b.n;
+    const T lr = b.lr;
+    T* inv = b.inv_out;
+    double* part = b.upart;
+    unsigned* cnt = b.ucounter;
+    T* mb = b.mb;
+    update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, inv, T(0), rho, lr, mom, n, part, cnt, mb,
                         blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 template <class T>
