@@ -46,13 +46,11 @@ template <class T> struct BInst {
     cplx<T>* Tr;                // (4, N, Mh)
     T* phi;                     // (4, N, N)
     const T* K[4];              // kernel spectra of (M, delta)
-    double* zpart;              // gather partials, last-block counter, sumQ and 1/sumQ
-    unsigned* counter;
+    double* zpart;              // gather partials, sumQ and 1/sumQ
     double* zout;
     T* inv_out;
     double* sumq;               // this iteration's sumQ slot
-    double* upart;              // update partials, counter, {mean, bbox}
-    unsigned* ucounter;
+    double* upart;              // update partials, {mean, bbox}
     T* mb;
     double delta;               // spectra to (re)compute: cell width delta into Kw, through spec_tmp
     cplx<T>* spec_tmp;

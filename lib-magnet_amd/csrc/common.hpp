@@ -93,7 +93,6 @@ template <class T> struct RepBuffers {
     cplx<T>* Tr;                          // (4, N, Mh) row-major staging of the transposes
     T* phi;                               // (4, N, N) potentials, planar
     double* zpart;                        // per-block Z partials
-    unsigned* counter;                    // last-block counters (zeroed at creation, restored by kernels)
     double* zout;                         // [0] sumQ, [1] 1/max(sumQ, 1e-12)
     T* inv_out;                           // 1/max(sumQ, 1e-12) rounded to T
 };
@@ -106,7 +105,7 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
 // pairs != null adds the mid-range spring w * (Y_i - mean partners) to the gradient.
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h,
                                       T rho, T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim,
-                                      double* part, unsigned* counter, T* mean_bbox, hipStream_t st);
+                                      double* part, T* mean_bbox, hipStream_t st);
 template <class T> void launch_mid_spring(const T* Y, const int32_t* pairs, T* grad, i64 n, int dim, int m, T w,
                                           hipStream_t st);
 // gradient of the non-fused path: combine ? rho * A - F * inv : grad, plus the spring when pairs != null

@@ -184,7 +184,7 @@ template <class T> struct Repulsion {
     std::list<Spec> cache;
     unsigned long tick = 0;
     DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, Tr, phi;
-    DevBuf zpart, counter, zout, inv_out, spec_tmp, table;
+    DevBuf zpart, zout, inv_out, spec_tmp, table;
     int last_N = 0, last_M = 0, sorted_buf = 0;
     GridT<T> last_g{};
     i64 last_n = 0;
@@ -192,8 +192,6 @@ template <class T> struct Repulsion {
     Repulsion(int M, int ni_) : ni(ni_), M_user(M) {
         require(ni >= 1 && ni <= 8, "n_interp must be in [1, 8]");
         fft_init_twiddles();
-        unsigned* c = reinterpret_cast<unsigned*>(counter.ensure(64 * sizeof(unsigned)));
-        MG_HIP(hipMemset(c, 0, 64 * sizeof(unsigned)));
         zout.ensure(4 * sizeof(double));
         inv_out.ensure(4 * sizeof(T));
     }
@@ -268,7 +266,7 @@ template <class T> struct Repulsion {
         b.hist = dptr<int>(hist); b.item_key = dptr<int>(item_key); b.item_valid = dptr<int>(item_valid);
         b.item_val = dptr<T>(item_val); b.W = dptr<T>(W); b.Wh = dptr<cplx<T>>(Wh); b.Ph = dptr<cplx<T>>(Ph);
         b.Tr = dptr<cplx<T>>(Tr); b.phi = dptr<T>(phi); b.zpart = dptr<double>(zpart);
-        b.counter = dptr<unsigned>(counter); b.zout = dptr<double>(zout); b.inv_out = dptr<T>(inv_out);
+        b.zout = dptr<double>(zout); b.inv_out = dptr<T>(inv_out);
         return b;
     }
 
@@ -325,7 +323,7 @@ template <class T> struct Repulsion {
 // ------------------------------------------------------------------------------ whole loop
 template <class T> struct Optimizer {
     Repulsion<T> rep;
-    DevBuf attr, F, grad, part, upd_counter, mb, bb_part, bb_counter, bb_out, sumq;
+    DevBuf attr, F, grad, part, mb, bb_part, bb_counter, bb_out, sumq;
     PinnedBuf host_mb;
     Optimizer(int ni) : rep(0, ni) {}
 
@@ -345,7 +343,6 @@ template <class T> struct Optimizer {
     T inv_dof = T(1);
     double* dsum = nullptr;
     T* hmb = nullptr;
-    unsigned* uc = nullptr;
 
     void attraction() {
         g_prof.begin(ST_ATTR, st);
@@ -373,11 +370,8 @@ template <class T> struct Optimizer {
         F.ensure((size_t)2 * n * sizeof(T));
         if (!fused) grad.ensure((size_t)2 * n * sizeof(T));
         part.ensure((size_t)6 * 1024 * sizeof(double));
-        uc = reinterpret_cast<unsigned*>(upd_counter.ensure(64));
         unsigned* bc = reinterpret_cast<unsigned*>(bb_counter.ensure(64));
-        if (!counters_zeroed) {
-            MG_HIP(hipMemsetAsync(uc, 0, 64, st)); MG_HIP(hipMemsetAsync(bc, 0, 64, st)); counters_zeroed = true;
-        }
+        if (!counters_zeroed) { MG_HIP(hipMemsetAsync(bc, 0, 64, st)); counters_zeroed = true; }
         mb.ensure(8 * sizeof(T));
         bb_part.ensure((size_t)4 * 1024 * sizeof(T));
         bb_out.ensure(4 * sizeof(T));
@@ -409,13 +403,13 @@ template <class T> struct Optimizer {
         g_prof.begin(ST_UPDATE, st);
         if (fused) {
             launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), uc, dptr<T>(mb), st);
+                             dptr<double>(part), dptr<T>(mb), st);
         } else {
             const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
             launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
                            dptr<T>(grad), st);
             launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), uc, dptr<T>(mb), st);
+                             dptr<double>(part), dptr<T>(mb), st);
         }
         g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
         MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
@@ -547,8 +541,8 @@ template <class T> struct BatchEngine {
         const size_t nk = (size_t)ntot * sizeof(int);
         keysA.ensure(nk); valsA.ensure(nk); keysB.ensure(nk); valsB.ensure(nk);
         hist.ensure(((size_t)1024 * std::max(1, sort_hist_blocks(ntot)) + 1024) * sizeof(int));
-        unsigned* cnt = reinterpret_cast<unsigned*>(counters.ensure((size_t)(2 * count + 1) * sizeof(unsigned)));
-        MG_HIP(hipMemsetAsync(cnt, 0, (size_t)(2 * count + 1) * sizeof(unsigned), st));
+        unsigned* cnt = reinterpret_cast<unsigned*>(counters.ensure(sizeof(unsigned)));   // the bbox's last-block counter
+        MG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), st));
         zout.ensure((size_t)2 * count * sizeof(double));
         inv.ensure((size_t)4 * count * sizeof(T));
         upart.ensure((size_t)count * 6 * 1024 * sizeof(double));
@@ -561,7 +555,7 @@ template <class T> struct BatchEngine {
         for (int i = 0; i < count; ++i) K.emplace_back(new DevBuf[4]);
         // starting layouts' bboxes (the update of each iteration writes the next ones)
         for (int i = 0; i < count; ++i)
-            launch_bbox<T>(Y + 2 * off[i], n[i], reinterpret_cast<T*>(upart.p), cnt + 2 * count, mb + 8 * i + 2, st);
+            launch_bbox<T>(Y + 2 * off[i], n[i], reinterpret_cast<T*>(upart.p), cnt, mb + 8 * i + 2, st);
         MG_HIP(hipMemcpyAsync(hm, mb, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
         inv_dof = T(1) / (T)p->dof;
         // the fixed part of the table, and the attraction launches (rows grouped by lanes per row)
@@ -636,7 +630,6 @@ template <class T> struct BatchEngine {
         const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
         const T mom = (T)(early ? p->momentum0 : p->momentum1);
         BInst<T>* h = host_table();
-        unsigned* cnt = dptr<unsigned>(counters);
         for (int i = 0; i < count; ++i) {
             BInst<T>& b = h[i];
             b.g = g[i]; b.M = M[i];
@@ -644,9 +637,9 @@ template <class T> struct BatchEngine {
             b.W = dptr<T>(aW) + pW[i]; b.Wh = dptr<cplx<T>>(aWh) + pWh[i]; b.Ph = dptr<cplx<T>>(aPh) + pPh[i];
             b.Tr = dptr<cplx<T>>(aTr) + pTr[i]; b.phi = dptr<T>(aphi) + pphi[i];
             for (int t = 0; t < 4; ++t) { b.K[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; b.Kw[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; }
-            b.zpart = dptr<double>(azp) + pzp[i]; b.counter = cnt + 2 * i; b.zout = dptr<double>(zout) + 2 * i;
+            b.zpart = dptr<double>(azp) + pzp[i]; b.zout = dptr<double>(zout) + 2 * i;
             b.inv_out = dptr<T>(inv) + 4 * i; b.sumq = dptr<double>(sumq) + (size_t)i * std::max(1, p->n_iter) + it;
-            b.upart = dptr<double>(upart) + (size_t)i * 6 * 1024; b.ucounter = cnt + 2 * i + 1; b.mb = dptr<T>(mbd) + 8 * i;
+            b.upart = dptr<double>(upart) + (size_t)i * 6 * 1024; b.mb = dptr<T>(mbd) + 8 * i;
             b.delta = sdelta[i]; b.spec_tmp = fresh[i] ? dptr<cplx<T>>(aspec) + psp[i] : nullptr;
         }
         MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
@@ -801,11 +794,9 @@ template <class T> static void step_impl(int mode, void* Y, void* U, void* G, co
                                          double rho, double inv, double lr, double mom) {
     require(n >= 1 && dim >= 1 && dim <= 64, "arrays must match Y");
     require(Y && U && G && A && (mode == 1 || F), "null array");
-    DevBuf part, cnt, mb;
-    unsigned* c = reinterpret_cast<unsigned*>(cnt.ensure(64));
-    MG_HIP(hipMemsetAsync(c, 0, 64, g_stream));
+    DevBuf part, mb;
     launch_update<T>(mode, (T*)Y, (T*)U, (T*)G, (const T*)A, (const T*)F, nullptr, (T)inv, (T)rho, (T)lr, (T)mom, nullptr, 0,
-                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), c, (T*)mb.ensure(std::max(8, dim) * sizeof(T)), g_stream);
+                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), g_stream);
     MG_HIP(hipStreamSynchronize(g_stream));
 }
 

@@ -420,15 +420,6 @@ template <class T> struct vec4;
 template <> struct vec4<float> { typedef float4 type; };
 template <> struct vec4<double> { typedef double4 type; };
 
-__device__ inline bool gather_last_block(unsigned* counter, unsigned nblk) {
-    __shared__ bool am_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == nblk - 1;
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last;
-}
 
 // sum over a block: a butterfly within each wave (fixed order), then thread 0 adds the waves in index order;
 // the result is valid in thread 0
@@ -446,9 +437,7 @@ __device__ inline double block_sum(double v, double* sh) {
 template <class T, int NI, bool CEN>
 __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const int* __restrict__ vals,
                                              const T* __restrict__ Y, i64 n, const GridT<T>& g, const T* __restrict__ phi,
-                                             T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
-                                             double* __restrict__ zout, T* __restrict__ inv_out,
-                                             double* __restrict__ sumq_hist, int bid, int nblk) {
+                                             T* __restrict__ F, double* __restrict__ zpart, int bid, int nblk) {
     __shared__ double shd[TPB];
     typedef typename vec2<T>::type V2;
     struct V4 { T x, y, z, w; };
@@ -506,7 +495,12 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
     }
     z = block_sum(z, shd);
     if (threadIdx.x == 0) zpart[bid] = z;
-    if (!gather_last_block(counter, nblk)) return;
+}
+
+// Z of the gather: the nblk block partials in a fixed order; sumQ = Z - n, 1 / max(sumQ, 1e-12) (one block)
+__device__ inline void zfinal_block(const double* __restrict__ zpart, int nblk, i64 n, double* __restrict__ zout,
+                                    void* inv_out, bool fp32, double* __restrict__ sumq_hist) {
+    __shared__ double shd[TPB];
     double s = 0;
     for (int b = threadIdx.x; b < nblk; b += TPB) s += zpart[b];
     s = block_sum(s, shd);
@@ -514,18 +508,20 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
         const double sumq = s - (double)n;
         const double inv = 1.0 / (sumq > 1e-12 ? sumq : 1e-12);
         zout[0] = sumq; zout[1] = inv;
-        *inv_out = (T)inv;
+        if (fp32) *reinterpret_cast<float*>(inv_out) = (float)inv; else *reinterpret_cast<double*>(inv_out) = inv;
         if (sumq_hist) *sumq_hist = sumq;
-        *counter = 0u;
     }
 }
+__global__ __launch_bounds__(TPB) void k_zfinal(const double* __restrict__ zpart, int nblk, i64 n, double* __restrict__ zout,
+                                                void* inv_out, int fp32, double* __restrict__ sumq_hist) {
+    zfinal_block(zpart, nblk, n, zout, inv_out, fp32 != 0, sumq_hist);
+}
+
 template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
                                                 const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
-                                                T* __restrict__ F, double* __restrict__ zpart, unsigned* counter,
-                                                double* __restrict__ zout, T* __restrict__ inv_out,
-                                                double* __restrict__ sumq_hist) {
-    gather_block<T, NI, CEN>(keys, vals, Y, n, g, phi, F, zpart, counter, zout, inv_out, sumq_hist, blockIdx.x, gridDim.x);
+                                                T* __restrict__ F, double* __restrict__ zpart) {
+    gather_block<T, NI, CEN>(keys, vals, Y, n, g, phi, F, zpart, blockIdx.x, gridDim.x);
 }
 template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys,
@@ -536,12 +532,15 @@ __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I,
     const i64 off = b.off, n = b.n;
     const T* phi = b.phi;
     double* zpart = b.zpart;
-    unsigned* cnt = b.counter;
-    double* zout = b.zout;
-    T* inv = b.inv_out;
-    double* sumq = b.sumq;
-    gather_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, phi, F, zpart, cnt, zout, inv, sumq,
-                             blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+    gather_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, phi, F, zpart, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+}
+// one block per instance
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_zfinal(const BInst<T>* __restrict__ I, BMap m) {
+    const BInst<T>& b = I[m.inst[bmap_find(m, blockIdx.x)]];
+    const i64 n = b.n, nb = (n + TPB - 1) / TPB;   // the gather's blocks (gather_blocks)
+    const int nblk = (int)(nb < 1024 ? (nb < 1 ? 1 : nb) : 1024);
+    zfinal_block(b.zpart, nblk, n, b.zout, b.inv_out, sizeof(T) == 4, b.sumq);
 }
 
 }  // namespace
@@ -649,6 +648,10 @@ template <class T> void launch_gather_batch(const BInst<T>* I, const BMap& m, co
     }
 #undef MG_G
     MG_HIP(hipGetLastError());
+    BMap one = m;   // one block per instance
+    for (int k = 0; k <= m.count; ++k) one.start[k] = k;
+    hipLaunchKernelGGL(kb_zfinal<T>, dim3((unsigned)m.count), dim3(TPB), 0, st, I, one);
+    MG_HIP(hipGetLastError());
 }
 
 template <class T, int NI, bool CEN> static void spread_ni(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g,
@@ -679,8 +682,8 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
                                       bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st) {
     const unsigned blocks = (unsigned)gather_blocks(n);
 #define MG_G(NI) do { \
-        if (centred) hipLaunchKernelGGL((k_gather<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist); \
-        else hipLaunchKernelGGL((k_gather<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, b.counter, b.zout, b.inv_out, sumq_hist); \
+        if (centred) hipLaunchKernelGGL((k_gather<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart); \
+        else hipLaunchKernelGGL((k_gather<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart); \
     } while (0)
     switch (ni) {
         case 1: MG_G(1); break; case 2: MG_G(2); break; case 3: MG_G(3); break; case 4: MG_G(4); break;
@@ -688,6 +691,9 @@ template <class T> void launch_gather(const int* keys, const int* vals, const T*
         default: require(false, "n_interp must be in [1, 8]");
     }
 #undef MG_G
+    MG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_zfinal, dim3(1), dim3(TPB), 0, st, (const double*)b.zpart, (int)blocks, n, b.zout, (void*)b.inv_out,
+                       (int)(sizeof(T) == 4), sumq_hist);
     MG_HIP(hipGetLastError());
 }
 

@@ -191,8 +191,7 @@ template <class T, int MODE>
 __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
                                               const T* __restrict__ A, const T* __restrict__ F,
                                               const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
-                                              double* __restrict__ part, unsigned* counter, T* __restrict__ mb,
-                                              int bid, int nblk) {
+                                              double* __restrict__ part, int bid, int nblk) {
     typedef typename vec2<T>::type V;
     const T inv = d_inv ? *d_inv : inv_h;
     double sx = 0, sy = 0;
@@ -217,42 +216,48 @@ __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__
         double* p = part + 6 * bid;
         p[0] = m.sx; p[1] = m.sy; p[2] = (double)m.x0; p[3] = (double)m.y0; p[4] = (double)m.x1; p[5] = (double)m.y1;
     }
-    if (!last_block(counter, nblk)) return;
-    sx = 0; sy = 0; x0 = pinf<T>(); y0 = pinf<T>(); x1 = -pinf<T>(); y1 = -pinf<T>();
+}
+template <class T, int MODE>
+__global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                                 const T* __restrict__ A, const T* __restrict__ F,
+                                                 const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
+                                                 double* __restrict__ part) {
+    update2_block<T, MODE>(Y, U, Gn, A, F, d_inv, inv_h, rho, lr, mom, n, part, blockIdx.x, gridDim.x);
+}
+
+// Recentring after the update: every block reduces the update's nblk block partials itself, in the same fixed
+// order (so all blocks get the same mean), and block 0 writes mb = {mean_x, mean_y, xmin, ymin, xmax, ymax} of
+// the centred layout (rounding is monotone, so min(y_i - m) = min(y_i) - m exactly). No block waits for another.
+template <class T>
+__device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const double* __restrict__ part, T* __restrict__ mb,
+                                              int bid, int nblk) {
+    __shared__ T s_m[2];
+    typedef typename vec2<T>::type V;
+    double sx = 0, sy = 0;
+    T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
     for (int b = threadIdx.x; b < nblk; b += TPB) {
         const double* p = part + 6 * b;
         sx += p[0]; sy += p[1];
         x0 = (T)p[2] < x0 ? (T)p[2] : x0; y0 = (T)p[3] < y0 ? (T)p[3] : y0;
         x1 = (T)p[4] > x1 ? (T)p[4] : x1; y1 = (T)p[5] > y1 ? (T)p[5] : y1;
     }
-    m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
+    const MeanBox<T> m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
     if (threadIdx.x == 0) {
         const T mx = (T)(m.sx / (double)n), my = (T)(m.sy / (double)n);
-        mb[0] = mx; mb[1] = my;
-        mb[2] = m.x0 - mx; mb[3] = m.y0 - my; mb[4] = m.x1 - mx; mb[5] = m.y1 - my;
-        *counter = 0u;
+        s_m[0] = mx; s_m[1] = my;
+        if (bid == 0) { mb[0] = mx; mb[1] = my; mb[2] = m.x0 - mx; mb[3] = m.y0 - my; mb[4] = m.x1 - mx; mb[5] = m.y1 - my; }
     }
-}
-template <class T, int MODE>
-__global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
-                                                 const T* __restrict__ A, const T* __restrict__ F,
-                                                 const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
-                                                 double* __restrict__ part, unsigned* counter, T* __restrict__ mb) {
-    update2_block<T, MODE>(Y, U, Gn, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mb, blockIdx.x, gridDim.x);
-}
-
-template <class T>
-__device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const T* __restrict__ mb, int bid, int nblk) {
-    typedef typename vec2<T>::type V;
-    const T mx = mb[0], my = mb[1];
+    __syncthreads();
+    const T mx = s_m[0], my = s_m[1];
     V* Y2 = reinterpret_cast<V*>(Y);
     for (i64 i = (i64)bid * TPB + threadIdx.x; i < n; i += (i64)nblk * TPB) {
         V y = Y2[i]; y.x -= mx; y.y -= my; Y2[i] = y;
     }
 }
 template <class T>
-__global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const T* __restrict__ mb) {
-    center2_block<T>(Y, n, mb, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const double* __restrict__ part,
+                                                 T* __restrict__ mb) {
+    center2_block<T>(Y, n, part, mb, blockIdx.x, gridDim.x);
 }
 
 // batched fused step of the t-SNE loop (gradient rho * attr - F / sumQ of each instance) and its recentring
@@ -266,16 +271,17 @@ __global__ __launch_bounds__(TPB) void kb_update2(const BInst<T>* __restrict__ I
     const T lr = b.lr;
     T* inv = b.inv_out;
     double* part = b.upart;
-    unsigned* cnt = b.ucounter;
-    T* mb = b.mb;
-    update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, inv, T(0), rho, lr, mom, n, part, cnt, mb,
+    update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, inv, T(0), rho, lr, mom, n, part,
                         blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 template <class T>
 __global__ __launch_bounds__(TPB) void kb_center2(const BInst<T>* __restrict__ I, BMap m, T* __restrict__ Y) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    center2_block<T>(Y + 2 * b.off, b.n, b.mb, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+    const i64 off = b.off, n = b.n;
+    const double* part = b.upart;
+    T* mb = b.mb;
+    center2_block<T>(Y + 2 * off, n, part, mb, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
 }
 
 // ---- any dim: elementwise update, then one block per dimension for the mean, then centring
@@ -397,15 +403,15 @@ template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T*
 
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h, T rho,
                                       T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim, double* part,
-                                      unsigned* counter, T* mean_bbox, hipStream_t st) {
+                                      T* mean_bbox, hipStream_t st) {
     (void)pairs; (void)m; (void)w_mid;
     if (n == 0) return;
     if (dim == 2) {
         const unsigned blocks = grid_for(n, 1024);
-        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mean_bbox);
-        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part, counter, mean_bbox);
+        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part);
+        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part);
         MG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, mean_bbox);
+        hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, (const double*)part, mean_bbox);
     } else {
         require(d_inv == nullptr, "device-side inv_sumQ needs a 2-D layout");
         const i64 total = n * dim;
@@ -440,7 +446,7 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
 #define MG_INST(T) \
     template void launch_bbox<T>(const T*, i64, T*, unsigned*, T*, hipStream_t); \
     template void launch_attraction<T>(const int32_t*, const int32_t*, const T*, const T*, i64, int, T, T*, i64, hipStream_t); \
-    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, unsigned*, T*, hipStream_t); \
+    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, T*, hipStream_t); \
     template void launch_grad<T>(const T*, const T*, const T*, T, T, const T*, const int32_t*, int, T, i64, int, int, T*, hipStream_t); \
     template void launch_mid_spring<T>(const T*, const int32_t*, T*, i64, int, int, T, hipStream_t); \
     template void launch_fill<T>(T*, T, i64, hipStream_t); \
