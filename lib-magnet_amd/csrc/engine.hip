@@ -203,13 +203,15 @@ template <class T> struct Repulsion {
         const double delta = tsne_kernel ? kwidth / ni : -1.0;
         for (auto& s : cache)
             if (s.M == M && s.delta == delta && s.dof == kdof) { s.use = ++tick; return s; }
-        if (cache.size() >= 9) {
-            auto lru = cache.begin();
-            for (auto it = cache.begin(); it != cache.end(); ++it) if (it->use < lru->use) lru = it;
-            cache.erase(lru);
+        Spec* slot = nullptr;
+        if (cache.size() >= 9) {   // the least recently used entry takes the new tables, in its own buffers
+            slot = &cache.front();
+            for (auto& c : cache) if (c.use < slot->use) slot = &c;
+        } else {
+            cache.emplace_back();
+            slot = &cache.back();
         }
-        cache.emplace_back();
-        Spec& s = cache.back();
+        Spec& s = *slot;
         s.M = M; s.delta = delta; s.dof = kdof; s.use = ++tick;
         if (fresh) *fresh = true;
         const int Mh = M / 2 + 1;
