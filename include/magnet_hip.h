@@ -159,6 +159,8 @@ typedef struct {
     double box_width;              /* 0.7 */
     double dof;                    /* Student-t degrees of freedom (1) */
     double mid_weight, mid_phase;  /* mid-range spring (tsne.py:194-198); pairs may be NULL */
+    int32_t relabel_every;         /* locality relabelling of the points every this many iterations (0: 25; < 0: off).
+                                      It never changes a result (no counterpart in the reference) */
 } mg_tsne_params;
 
 /* Called after iteration `it` when it % 50 == 0 or it == n_iter - 1 (tsne.py:180-181) with the device

@@ -308,7 +308,7 @@ py::array_t<double> tsne_optimize(py::array indptr, py::array indices, py::array
     p.n_iter = n_iter; p.early_iter = early_iter; p.early_exaggeration = early_exaggeration; p.exaggeration = exaggeration;
     p.learning_rate = learning_rate; p.momentum0 = momentum0; p.momentum1 = momentum1; p.n_interp = n_interp;
     p.min_boxes = min_boxes; p.max_boxes = max_boxes; p.box_width = box_width; p.dof = dof;
-    p.mid_weight = mid_weight; p.mid_phase = mid_phase;
+    p.mid_weight = mid_weight; p.mid_phase = mid_phase; p.relabel_every = 0;
     if (it1 < 0) it1 = n_iter;
     const bool f64 = py::dtype::of<double>().is(Y.dtype());
     // in-place state must already have the working dtype and layout (forcecast would copy it)

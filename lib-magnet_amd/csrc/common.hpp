@@ -105,7 +105,9 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
 // pairs != null adds the mid-range spring w * (Y_i - mean partners) to the gradient.
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h,
                                       T rho, T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim,
-                                      double* part, T* mean_bbox, hipStream_t st);
+                                      double* part, T* mean_bbox, const int* inv, hipStream_t st);
+// inv != null: the layout is relabelled (position inv[i] holds original point i); the mean and bbox are then
+// summed in original point order
 template <class T> void launch_mid_spring(const T* Y, const int32_t* pairs, T* grad, i64 n, int dim, int m, T w,
                                           hipStream_t st);
 // gradient of the non-fused path: combine ? rho * A - F * inv : grad, plus the spring when pairs != null
@@ -128,6 +130,18 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
                                          const GridT<T>& g, int ni, bool centred, const T* phi, double inv_Z, int n_iter,
                                          double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
                                          hipStream_t st);
+
+// locality relabelling (k_order.hip)
+void launch_identity_labels(int* perm, int* inv, i64 n, hipStream_t st);
+template <class T> void launch_relabel(const int* vals, const int* perm, i64 n, const int32_t* ip, const int32_t* ix, const T* P,
+                                       int* order, int* inv_new, int* len, int* tiles, int32_t* ipn, int32_t* ixn, T* Pn,
+                                       hipStream_t st);
+template <class T> void launch_perm_state(const T* Y, const T* U, const T* G, const int* inv_old, const int* order, i64 n,
+                                          T* Yn, T* Un, T* Gn, hipStream_t st);
+template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
+                                         hipStream_t st);
+template <class T> void launch_unperm(const T* src, const int* inv, i64 n, T* dst, hipStream_t st);
+template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* valsB, int* hist, i64 n, int bits, hipStream_t st);
 
 // FFT stages. M = 2^s or 3 * 2^s, 128 <= M <= MAX_M, N <= M / 2.
 void fft_init_twiddles();   // uploads the twiddle tables once per device

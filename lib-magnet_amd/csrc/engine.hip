@@ -275,7 +275,9 @@ template <class T> struct Repulsion {
     int device_M(int N) const { return fft_size(N); }
 
     // enqueue the whole repulsion; sumQ and 1/sumQ stay on the device (zout, inv_out)
-    void compute(const T* Y, i64 n, const GridT<T>& g, T* F, double* sumq_hist, hipStream_t st) {
+    // inv != null: Y is relabelled (position inv[i] holds original point i); the box sort then takes the points in
+    // original order, so its ties, and every sum after it, are those of the unrelabelled layout
+    void compute(const T* Y, i64 n, const GridT<T>& g, T* F, double* sumq_hist, hipStream_t st, const int* inv = nullptr) {
         require(n >= 1, "Y must be (n, 2) with n >= 1");
         require(g.N >= 1 && g.width > T(0), "grid larger than the buffer");
         require(n < (i64)INT32_MAX, "too many points");
@@ -291,7 +293,14 @@ template <class T> struct Repulsion {
         P.end(ST_SPECTRA, st, fresh ? ntab * Mh * Mh * ts : 0.0);   // tables written when the grid changed
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
-        sorted_buf = launch_box_sort<T>(Y, n, g, b, st);
+        if (inv) {
+            int bits = 0;
+            while ((1ll << bits) < (long long)g.nb * g.nb) ++bits;
+            launch_perm_keys<T>(Y, inv, n, g, b.keysA, b.valsA, st);
+            sorted_buf = launch_key_sort<T>(b.keysA, b.valsA, b.keysB, b.valsB, b.hist, n, std::max(bits, 1), st);
+        } else {
+            sorted_buf = launch_box_sort<T>(Y, n, g, b, st);
+        }
         P.end(ST_SORT, st, 2 * dn * ts + 8 * dn);
         const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
         const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
@@ -345,11 +354,47 @@ template <class T> struct Optimizer {
     T inv_dof = T(1);
     double* dsum = nullptr;
     T* hmb = nullptr;
+    // locality relabelling: the state lives in Yw / Uw / Gw in the order of a recent box sort (position inv[i]
+    // holds original point i, perm is its inverse); the CSR is renumbered to match (cip, cix, cP)
+    bool relabel = false;
+    int relabel_every = 25, since = 0;
+    T *uY = nullptr, *uU = nullptr, *uG = nullptr;
+    const int32_t *cip = nullptr, *cix = nullptr;
+    const T* cP = nullptr;
+    DevBuf Yw, Uw, Gw, Yt, Ut, Gt, perm, inv, order, inv_new, lenb, tiles, ipn, ixn, Pn, Ycb;
 
     void attraction() {
         g_prof.begin(ST_ATTR, st);
-        launch_attraction<T>(ip, ix, data, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
+        launch_attraction<T>(cip, cix, cP, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
         g_prof.end(ST_ATTR, st, attr_bytes);
+    }
+
+    void start_relabelling() {
+        const size_t b = (size_t)2 * n * sizeof(T);
+        T* y = reinterpret_cast<T*>(Yw.ensure(b)); T* u = reinterpret_cast<T*>(Uw.ensure(b)); T* g = reinterpret_cast<T*>(Gw.ensure(b));
+        Yt.ensure(b); Ut.ensure(b); Gt.ensure(b);
+        MG_HIP(hipMemcpyAsync(y, uY, b, hipMemcpyDeviceToDevice, st));
+        MG_HIP(hipMemcpyAsync(u, uU, b, hipMemcpyDeviceToDevice, st));
+        MG_HIP(hipMemcpyAsync(g, uG, b, hipMemcpyDeviceToDevice, st));
+        Y = y; U = u; G = g;
+        const size_t ni = (size_t)n * sizeof(int);
+        perm.ensure(ni); inv.ensure(ni); order.ensure(ni); inv_new.ensure(ni); lenb.ensure(ni);
+        tiles.ensure((size_t)(n / 1024 + 2) * sizeof(int));
+        ipn.ensure((size_t)(n + 1) * sizeof(int32_t)); ixn.ensure((size_t)std::max<i64>(nnz, 1) * sizeof(int32_t));
+        Pn.ensure((size_t)std::max<i64>(nnz, 1) * sizeof(T));
+        launch_identity_labels(dptr<int>(perm), dptr<int>(inv), n, st);
+    }
+
+    // renumber the points in the order of the last box sort, and the CSR (from the caller's) to match
+    void relabel_now() {
+        const int* vals = rep.sorted_buf == 0 ? dptr<int>(rep.valsA) : dptr<int>(rep.valsB);
+        launch_relabel<T>(vals, dptr<int>(perm), n, ip, ix, data, dptr<int>(order), dptr<int>(inv_new), dptr<int>(lenb),
+                          dptr<int>(tiles), dptr<int32_t>(ipn), dptr<int32_t>(ixn), dptr<T>(Pn), st);
+        launch_perm_state<T>(Y, U, G, dptr<int>(inv), dptr<int>(order), n, dptr<T>(Yt), dptr<T>(Ut), dptr<T>(Gt), st);
+        std::swap(Yw.p, Yt.p); std::swap(Uw.p, Ut.p); std::swap(Gw.p, Gt.p);
+        std::swap(perm.p, order.p); std::swap(inv.p, inv_new.p);
+        Y = dptr<T>(Yw); U = dptr<T>(Uw); G = dptr<T>(Gw);
+        cip = dptr<int32_t>(ipn); cix = dptr<int32_t>(ixn); cP = dptr<T>(Pn);
     }
 
     // checks, buffers, the starting layout's bbox (read back), and the first iteration's attraction
@@ -357,6 +402,7 @@ template <class T> struct Optimizer {
                T* G_, const int32_t* pairs_, int m_, int it0_, int it1_, double* h_sumQ_, mg_tsne_callback cb_, void* user_,
                hipStream_t st_) {
         p = p_; ip = ip_; ix = ix_; data = data_; nnz = nnz_; n = n_; Y = Y_; U = U_; G = G_; pairs = pairs_; m = m_;
+        uY = Y_; uU = U_; uG = G_; cip = ip_; cix = ix_; cP = data_; since = 0;
         it0 = it0_; it1 = it1_; h_sumQ = h_sumQ_; cb = cb_; user = user_; st = st_;
         require(p != nullptr, "params must not be null");
         require(n >= 1 && Y && U && G, "Y, U, G must be (n, 2) device arrays");
@@ -385,6 +431,9 @@ template <class T> struct Optimizer {
         inv_dof = T(1) / (T)p->dof;
         attr_bytes = 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T);
         if (!mb_ready) MG_HIP(hipEventCreateWithFlags(&mb_ready, hipEventDisableTiming));
+        relabel = fused && p->relabel_every >= 0 && n >= 2 && it1 - it0 > 1;
+        relabel_every = p->relabel_every > 0 ? p->relabel_every : 25;
+        if (relabel) start_relabelling();
         if (it0 < it1) attraction();
     }
 
@@ -400,22 +449,24 @@ template <class T> struct Optimizer {
         const bool early = it < p->early_iter;
         const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
         const T mom = (T)(early ? p->momentum0 : p->momentum1);
-        rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st);
+        const int* labels = relabel ? dptr<int>(inv) : nullptr;
+        rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st, labels);
         const T* dinv = dptr<T>(rep.inv_out);
         g_prof.begin(ST_UPDATE, st);
         if (fused) {
             launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), dptr<T>(mb), st);
+                             dptr<double>(part), dptr<T>(mb), labels, st);
         } else {
             const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
             launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
                            dptr<T>(grad), st);
             launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), dptr<T>(mb), st);
+                             dptr<double>(part), dptr<T>(mb), nullptr, st);
         }
         g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
         MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
         MG_HIP(hipEventRecord(mb_ready, st));
+        if (relabel && it + 1 < it1 && ++since % relabel_every == 0) relabel_now();
         if (it + 1 < it1) attraction();
     }
 
@@ -423,13 +474,27 @@ template <class T> struct Optimizer {
     bool complete(int it) {
         MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
         if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); g_prof.collect(it); }
-        if (cb && (it % 50 == 0 || it == p->n_iter - 1)) return cb(it, Y, n, user) != 0;
+        if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
+            if (!relabel) return cb(it, Y, n, user) != 0;
+            // the layout in original point order; Y itself is final for iteration it (the event), and the
+            // relabelling that may follow it in the stream is not waited for
+            T* y = reinterpret_cast<T*>(Ycb.ensure((size_t)2 * n * sizeof(T)));
+            MG_HIP(hipStreamSynchronize(st));
+            launch_unperm<T>(Y, dptr<int>(inv), n, y, st);
+            MG_HIP(hipStreamSynchronize(st));
+            return cb(it, y, n, user) != 0;
+        }
         return false;
     }
 
     void finish(int it_end) {
         if (h_sumQ && it_end > it0)
             MG_HIP(hipMemcpyAsync(h_sumQ, dsum, (size_t)(it_end - it0) * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (relabel) {   // the state back to the caller's buffers, in original point order
+            launch_unperm<T>(Y, dptr<int>(inv), n, uY, st);
+            launch_unperm<T>(U, dptr<int>(inv), n, uU, st);
+            launch_unperm<T>(G, dptr<int>(inv), n, uG, st);
+        }
         MG_HIP(hipStreamSynchronize(st));
     }
 
@@ -798,7 +863,7 @@ template <class T> static void step_impl(int mode, void* Y, void* U, void* G, co
     require(Y && U && G && A && (mode == 1 || F), "null array");
     DevBuf part, mb;
     launch_update<T>(mode, (T*)Y, (T*)U, (T*)G, (const T*)A, (const T*)F, nullptr, (T)inv, (T)rho, (T)lr, (T)mom, nullptr, 0,
-                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), g_stream);
+                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), nullptr, g_stream);
     MG_HIP(hipStreamSynchronize(g_stream));
 }
 

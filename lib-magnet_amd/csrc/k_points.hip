@@ -211,6 +211,7 @@ __device__ __forceinline__ void update2_block(T* __restrict__ Y, T* __restrict__
         x0 = y.x < x0 ? y.x : x0; x1 = y.x > x1 ? y.x : x1;
         y0 = y.y < y0 ? y.y : y0; y1 = y.y > y1 ? y.y : y1;
     }
+    if (part == nullptr) return;   // partials taken in original point order by k_meanbox_perm
     MeanBox<T> m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
     if (threadIdx.x == 0) {
         double* p = part + 6 * bid;
@@ -223,6 +224,28 @@ __global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restric
                                                  const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
                                                  double* __restrict__ part) {
     update2_block<T, MODE>(Y, U, Gn, A, F, d_inv, inv_h, rho, lr, mom, n, part, blockIdx.x, gridDim.x);
+}
+
+// The update's block partials of a relabelled layout, in original point order: block b sums the points
+// b * TPB + t (+ k * nblk * TPB) of the original numbering, as the update of the unrelabelled layout does.
+template <class T>
+__global__ __launch_bounds__(TPB) void k_meanbox_perm(const T* __restrict__ Y, const int* __restrict__ inv, i64 n,
+                                                      double* __restrict__ part) {
+    typedef typename vec2<T>::type V;
+    const int bid = blockIdx.x, nblk = gridDim.x;
+    double sx = 0, sy = 0;
+    T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
+    for (i64 i = (i64)bid * TPB + threadIdx.x; i < n; i += (i64)nblk * TPB) {
+        const V y = reinterpret_cast<const V*>(Y)[inv[i]];
+        sx += (double)y.x; sy += (double)y.y;
+        x0 = y.x < x0 ? y.x : x0; x1 = y.x > x1 ? y.x : x1;
+        y0 = y.y < y0 ? y.y : y0; y1 = y.y > y1 ? y.y : y1;
+    }
+    const MeanBox<T> m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
+    if (threadIdx.x == 0) {
+        double* p = part + 6 * bid;
+        p[0] = m.sx; p[1] = m.sy; p[2] = (double)m.x0; p[3] = (double)m.y0; p[4] = (double)m.x1; p[5] = (double)m.y1;
+    }
 }
 
 // Recentring after the update: every block reduces the update's nblk block partials itself, in the same fixed
@@ -403,17 +426,22 @@ template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T*
 
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h, T rho,
                                       T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim, double* part,
-                                      T* mean_bbox, hipStream_t st) {
+                                      T* mean_bbox, const int* inv, hipStream_t st) {
     (void)pairs; (void)m; (void)w_mid;
     if (n == 0) return;
     if (dim == 2) {
         const unsigned blocks = grid_for(n, 1024);
-        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part);
-        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, part);
+        double* upart = inv ? nullptr : part;
+        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
+        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
         MG_HIP(hipGetLastError());
+        if (inv) {
+            hipLaunchKernelGGL(k_meanbox_perm<T>, dim3(blocks), dim3(TPB), 0, st, (const T*)Y, inv, n, part);
+            MG_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, (const double*)part, mean_bbox);
     } else {
-        require(d_inv == nullptr, "device-side inv_sumQ needs a 2-D layout");
+        require(d_inv == nullptr && inv == nullptr, "device-side inv_sumQ and relabelling need a 2-D layout");
         const i64 total = n * dim;
         const unsigned blocks = grid_for(total, 4096);
         if (mode == 0) hipLaunchKernelGGL((k_update_any<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, inv_h, rho, lr, mom, total);
@@ -446,7 +474,7 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
 #define MG_INST(T) \
     template void launch_bbox<T>(const T*, i64, T*, unsigned*, T*, hipStream_t); \
     template void launch_attraction<T>(const int32_t*, const int32_t*, const T*, const T*, i64, int, T, T*, i64, hipStream_t); \
-    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, T*, hipStream_t); \
+    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, T*, const int*, hipStream_t); \
     template void launch_grad<T>(const T*, const T*, const T*, T, T, const T*, const int32_t*, int, T, i64, int, int, T*, hipStream_t); \
     template void launch_mid_spring<T>(const T*, const int32_t*, T*, i64, int, int, T, hipStream_t); \
     template void launch_fill<T>(T*, T, i64, hipStream_t); \

@@ -21,7 +21,8 @@ class Params(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("early_iter", C.c_int32), ("early_exaggeration", C.c_double),
                 ("exaggeration", C.c_double), ("learning_rate", C.c_double), ("momentum0", C.c_double),
                 ("momentum1", C.c_double), ("n_interp", C.c_int32), ("min_boxes", C.c_int32), ("max_boxes", C.c_int32),
-                ("box_width", C.c_double), ("dof", C.c_double), ("mid_weight", C.c_double), ("mid_phase", C.c_double)]
+                ("box_width", C.c_double), ("dof", C.c_double), ("mid_weight", C.c_double), ("mid_phase", C.c_double),
+                ("relabel_every", C.c_int32)]
 
 
 class Instance(C.Structure):
@@ -177,12 +178,12 @@ class FFTConv:
 
 def optimize(indptr, indices, data, Y, U, G, it0=0, it1=None, n_iter=750, early_iter=250, early_exaggeration=12.0,
              exaggeration=1.0, learning_rate=0.0, momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, pairs=None,
-             mid_weight=0.0, mid_phase=0.5, min_boxes=20, max_boxes=1000, callback=None):
+             mid_weight=0.0, mid_phase=0.5, min_boxes=20, max_boxes=1000, callback=None, relabel_every=0):
     """Runs mg_tsne_optimize on device tensors; returns per-iteration sumQ (numpy)."""
     import numpy as np
     it1 = n_iter if it1 is None else it1
     p = Params(n_iter, early_iter, early_exaggeration, exaggeration, learning_rate, momentum[0], momentum[1], n_interp,
-               min_boxes, max_boxes, box_width, dof, mid_weight, mid_phase)
+               min_boxes, max_boxes, box_width, dof, mid_weight, mid_phase, relabel_every)
     sq = np.zeros(max(1, it1 - it0))
     cb = CALLBACK(callback) if callback else CALLBACK(0)
     check(lib().mg_tsne_optimize(prec_of(Y), C.byref(p), ptr(indptr), ptr(indices), ptr(data), indices.numel(), Y.shape[0],

@@ -3,6 +3,8 @@
 FP64: per iteration, from the same state, the GPU step equals the oracle step to relative L2 <= 1e-10
 (the stated bar of the plain FP64 path). FP32: the gradient of the FP32 path against the FP64 one on
 the same layouts (relative L2 <= 1e-4), determinism, resume, callbacks and the reference's quality test."""
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -190,3 +192,29 @@ def test_separates_blobs_like_the_reference():
     from sklearn.neighbors import NearestNeighbors
     nn = NearestNeighbors(n_neighbors=11).fit(Y).kneighbors(Y, return_distance=False)[:, 1:]
     assert (lab[nn] == lab[:, None]).mean() > 0.95
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_relabelling_changes_no_result(dtype):
+    """The locality relabelling (every iteration, every 7, the default, or off) leaves the state, sumQ and the
+    layouts handed to the callback bit-identical."""
+    P, Y0, _ = problem(6000, 20, 6, 11)
+    out = {}
+    for every in (-1, 1, 7, 0):
+        st = State(P, Y0, dtype)
+        seen = {}
+        def cb(it, y, n, user):
+            import torch
+            seen[it] = torch.empty((n, 2), dtype=torch.float32 if dtype == np.float32 else torch.float64, device="cuda")
+            capi.lib().mg_memcpy_dtod(C.c_void_p(seen[it].data_ptr()), C.c_void_p(y),
+                                      C.c_size_t(seen[it].numel() * seen[it].element_size()))
+            return 0
+        sq = capi.optimize(st.ip, st.ix, st.data, st.Y, st.U, st.G, 0, 120, n_iter=120, early_iter=40, callback=cb,
+                           relabel_every=every)
+        out[every] = (st.snapshot(), sq, {k: host(v) for k, v in seen.items()})
+    ref = out[-1]
+    for every in (1, 7, 0):
+        (Y, U, G), sq, seen = out[every]
+        assert np.array_equal(Y, ref[0][0]) and np.array_equal(U, ref[0][1]) and np.array_equal(G, ref[0][2]), every
+        assert np.array_equal(sq, ref[1]), every
+        assert sorted(seen) == sorted(ref[2]) and all(np.array_equal(seen[k], ref[2][k]) for k in seen), every
