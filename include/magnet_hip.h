@@ -126,6 +126,18 @@ int mg_tsne_transform(mg_fftconv* conv, const void* d_Ytrain, int64_t n_train, c
                       int k, void* d_Ynew, int64_t m, int n_iter, double learning_rate, double momentum0, double momentum1,
                       int momentum_switch, double exaggeration, double dof);
 
+/* Selection step of the serving k-NN search (the reference's transform() queries its k-NN index,
+ * magnet/__init__.py:876-900): for each row of the (rows, cols) float32 device tile D, the k (1..32) smallest
+ * entries sorted by (value, column): d_idx / d_val (rows, k). */
+int mg_topk_rows(const float* d_D, int64_t rows, int64_t cols, int k, int32_t* d_idx, float* d_val);
+/* The whole serving k-NN search for rows of d <= 64 float32 features (the reference's k-NN query, magnet/knn.py):
+ * for each of the m query rows d_Q, the k (1..32) nearest of the nt training rows d_T, exact squared euclidean
+ * distances summed over the features in order, sorted by (distance, index): d_idx / d_dist2 (m, k).
+ * self_offset >= 0 leaves out training row q + self_offset for query q (the graph of the training rows). A query's
+ * result does not depend on the other queries of the call. Calls are serialised (one scratch per process). */
+int mg_knn(const float* d_T, int64_t nt, const float* d_Q, int64_t m, int d, int k, int64_t self_offset, int32_t* d_idx,
+           float* d_dist2);
+
 /* ---- fallback kernels (src/fftrep.cpp:51-125, 294-411; magnet/tsne.py:92-129 calls them when FFTConv2D is not
  * used). Box of a point: u = (x - xmin) / width clamped to [0, n_boxes - 1], product-form Lagrange weights;
  * N = n_boxes * n_interp. Sums run in the reference's single-thread order without fused multiply-adds, so the

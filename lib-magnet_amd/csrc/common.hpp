@@ -131,6 +131,14 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
                                          double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
                                          hipStream_t st);
 
+// the k (<= 32) smallest entries of each row of a (rows, cols) float tile, sorted by (value, column) (k_knn.hip)
+void launch_topk_rows(const float* D, i64 rows, i64 cols, int k, int32_t* oi, float* ov, hipStream_t st);
+
+// fused exact k-NN search (features <= 64, k <= 32): squared distances and indices sorted by (distance, index)
+int knn_chunks(i64 m, i64 nt);
+void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
+                int32_t* oi, float* ov, hipStream_t st);
+
 // locality relabelling (k_order.hip)
 void launch_identity_labels(int* perm, int* inv, i64 n, hipStream_t st);
 template <class T> void launch_relabel(const int* vals, const int* perm, i64 n, const int32_t* ip, const int32_t* ix, const T* P,

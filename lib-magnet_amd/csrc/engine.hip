@@ -13,6 +13,7 @@
 #include <atomic>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1057,6 +1058,28 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
         MG_HIP(hipStreamSynchronize(g_stream));
     };
     if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
+    MG_API_END
+}
+
+int mg_knn(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, int64_t self_offset, int32_t* idx, float* dist2) {
+    MG_API_BEGIN
+    static std::mutex mu;   // one scratch for the process: concurrent calls are serialised
+    static DevBuf pd, pi;
+    std::lock_guard<std::mutex> lk(mu);
+    require(m >= 0 && nt >= 1 && (m == 0 || (T && Q && idx && dist2)), "bad k-NN arguments");
+    if (m > 0) {
+        const size_t sz = (size_t)m * knn_chunks(m, nt) * 32;
+        launch_knn(T, nt, Q, m, d, k, self_offset, reinterpret_cast<float*>(pd.ensure(sz * sizeof(float))),
+                   reinterpret_cast<int*>(pi.ensure(sz * sizeof(int))), idx, dist2, g_stream);
+        MG_HIP(hipStreamSynchronize(g_stream));
+    }
+    MG_API_END
+}
+
+int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
+    MG_API_BEGIN
+    require(rows == 0 || (D && idx && val), "null array");
+    launch_topk_rows(D, rows, cols, k, idx, val, g_stream);
     MG_API_END
 }
 

@@ -1,0 +1,224 @@
+// k smallest entries of each row of a distance tile (the selection step of the serving k-NN search): one workgroup
+// per row; each thread keeps the KM smallest (distance, column) pairs of its strided share of the row, sorted, by a
+// compare-exchange pass per candidate; then the TPB lists are merged pairwise through LDS. The order is total
+// (distance, then column), so the result does not depend on the launch shape.
+#include "common.hpp"
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+
+namespace mg {
+
+namespace {
+
+__device__ __forceinline__ bool before(float a, int ia, float b, int ib) { return a < b || (a == b && ia < ib); }
+
+template <int KM, int TPB>
+__global__ __launch_bounds__(TPB) void k_topk_rows(const float* __restrict__ D, i64 cols, int k, int32_t* __restrict__ oi,
+                                                   float* __restrict__ ov) {
+    __shared__ float sd[TPB * KM];
+    __shared__ int si[TPB * KM];
+    const float* row = D + (i64)blockIdx.x * cols;
+    float bd[KM];
+    int bi[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { bd[j] = FLT_MAX; bi[j] = INT_MAX; }
+    auto offer = [&](float x, int ix) {
+        if (!before(x, ix, bd[KM - 1], bi[KM - 1])) return;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) {   // the candidate sinks to its place; the largest falls off the end
+            const bool sw = before(x, ix, bd[j], bi[j]);
+            const float td = bd[j]; const int ti = bi[j];
+            bd[j] = sw ? x : td; bi[j] = sw ? ix : ti;
+            x = sw ? td : x; ix = sw ? ti : ix;
+        }
+    };
+    // the row in rounds of R float4 per thread (loads first, then the candidates in column order per thread)
+    constexpr int R = 4;
+    const bool vec = (cols % 4) == 0 && (reinterpret_cast<uintptr_t>(row) % 16) == 0;
+    i64 c0 = 0;
+    if (vec) {
+        const float4* r4 = reinterpret_cast<const float4*>(row);
+        const i64 n4 = cols / 4;
+        for (i64 b = threadIdx.x; b < n4; b += (i64)TPB * R) {
+            float4 v[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) { const i64 q = b + (i64)u * TPB; v[u] = q < n4 ? r4[q] : make_float4(FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX); }
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const i64 q = b + (i64)u * TPB;
+                if (q >= n4) break;
+                const int c = (int)(4 * q);
+                offer(v[u].x, c); offer(v[u].y, c + 1); offer(v[u].z, c + 2); offer(v[u].w, c + 3);
+            }
+        }
+        c0 = cols;
+    }
+    for (i64 b = c0 + threadIdx.x; b < cols; b += (i64)TPB * 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { const i64 c = b + (i64)u * TPB; v[u] = c < cols ? row[c] : FLT_MAX; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { const i64 c = b + (i64)u * TPB; if (c >= cols) break; offer(v[u], (int)c); }
+    }
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { sd[t * KM + j] = bd[j]; si[t * KM + j] = bi[j]; }
+    __syncthreads();
+    for (int s = TPB / 2; s >= 1; s >>= 1) {   // list t := first KM of merge(list t, list t + s)
+        float md[KM];
+        int mi[KM];
+        if (t < s) {
+            const float* ad = sd + t * KM; const int* ai = si + t * KM;
+            const float* cd = sd + (t + s) * KM; const int* ci = si + (t + s) * KM;
+            int p = 0, q = 0;
+#pragma unroll
+            for (int j = 0; j < KM; ++j) {
+                const bool fa = before(ad[p], ai[p], cd[q], ci[q]);
+                md[j] = fa ? ad[p] : cd[q]; mi[j] = fa ? ai[p] : ci[q];
+                p += fa ? 1 : 0; q += fa ? 0 : 1;
+            }
+        }
+        __syncthreads();
+        if (t < s) {
+#pragma unroll
+            for (int j = 0; j < KM; ++j) { sd[t * KM + j] = md[j]; si[t * KM + j] = mi[j]; }
+        }
+        __syncthreads();
+    }
+    if (t < k) { oi[(i64)blockIdx.x * k + t] = si[t]; ov[(i64)blockIdx.x * k + t] = sd[t]; }
+}
+
+
+// ---- fused k-NN search for rows of up to 64 features: exact squared euclidean distances, summed over the
+// features in order ((q - t)^2, zero padding adds nothing), so a pair's distance does not depend on the launch.
+// Pass 1: block (query tile of 256, training chunk s): one query per thread, the chunk's training rows streamed
+// through LDS in tiles of 64, each thread keeping the KM nearest of the chunk. Pass 2: per query, the S chunk
+// lists merged through LDS.
+constexpr int KNN_Q = 256, KNN_TR = 64;
+
+template <int DP, int KM>
+__global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ T, i64 nt, const float* __restrict__ Q, i64 m,
+                                                      int d, i64 chunk, i64 self_offset, float* __restrict__ pd,
+                                                      int* __restrict__ pi, int S) {
+    __shared__ float sT[KNN_TR][DP + 1];
+    const i64 qi = (i64)blockIdx.x * KNN_Q + threadIdx.x;
+    const int s = blockIdx.y;
+    float q[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) q[j] = (qi < m && j < d) ? Q[qi * d + j] : 0.f;
+    float bd[KM];
+    int bi[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { bd[j] = FLT_MAX; bi[j] = INT_MAX; }
+    const i64 c0 = (i64)s * chunk, c1 = c0 + chunk < nt ? c0 + chunk : nt;
+    const i64 self = (self_offset >= 0 && qi < m) ? qi + self_offset : -1;
+    for (i64 t0 = c0; t0 < c1; t0 += KNN_TR) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KNN_TR * DP; e += KNN_Q) {   // tile rows, zero-padded
+            const int r = e / DP, j = e % DP;
+            sT[r][j] = (t0 + r < c1 && j < d) ? T[(t0 + r) * d + j] : 0.f;
+        }
+        __syncthreads();
+        const int rows = (int)(c1 - t0 < KNN_TR ? c1 - t0 : KNN_TR);
+        for (int r = 0; r < rows; ++r) {
+            float acc = 0.f;
+#pragma unroll
+            for (int j = 0; j < DP; ++j) { const float df = q[j] - sT[r][j]; acc += df * df; }
+            int ix = (int)(t0 + r);
+            if (ix == self) continue;
+            float x = acc;
+            if (!before(x, ix, bd[KM - 1], bi[KM - 1])) continue;
+#pragma unroll
+            for (int j = 0; j < KM; ++j) {
+                const bool sw = before(x, ix, bd[j], bi[j]);
+                const float td = bd[j]; const int ti = bi[j];
+                bd[j] = sw ? x : td; bi[j] = sw ? ix : ti;
+                x = sw ? td : x; ix = sw ? ti : ix;
+            }
+        }
+    }
+    if (qi >= m) return;
+    float* od = pd + (qi * S + s) * KM;
+    int* oi = pi + (qi * S + s) * KM;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { od[j] = bd[j]; oi[j] = bi[j]; }
+}
+
+// one block of S (a power of two) threads per query: list t is chunk t's
+template <int KM>
+__global__ void k_knn_merge(const float* __restrict__ pd, const int* __restrict__ pi, int S, int k, int32_t* __restrict__ oi,
+                            float* __restrict__ ov) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sd = reinterpret_cast<float*>(smem);
+    int* si = reinterpret_cast<int*>(sd + S * KM);
+    const i64 qi = blockIdx.x;
+    const int t = threadIdx.x;
+    for (int j = 0; j < KM; ++j) { sd[t * KM + j] = pd[(qi * S + t) * KM + j]; si[t * KM + j] = pi[(qi * S + t) * KM + j]; }
+    __syncthreads();
+    for (int s = S / 2; s >= 1; s >>= 1) {
+        float md[KM];
+        int mi[KM];
+        if (t < s) {
+            const float* ad = sd + t * KM; const int* ai = si + t * KM;
+            const float* cd = sd + (t + s) * KM; const int* ci = si + (t + s) * KM;
+            int p = 0, r = 0;
+#pragma unroll
+            for (int j = 0; j < KM; ++j) {
+                const bool fa = before(ad[p], ai[p], cd[r], ci[r]);
+                md[j] = fa ? ad[p] : cd[r]; mi[j] = fa ? ai[p] : ci[r];
+                p += fa ? 1 : 0; r += fa ? 0 : 1;
+            }
+        }
+        __syncthreads();
+        if (t < s) {
+#pragma unroll
+            for (int j = 0; j < KM; ++j) { sd[t * KM + j] = md[j]; si[t * KM + j] = mi[j]; }
+        }
+        __syncthreads();
+    }
+    for (int j = t; j < k; j += S) { oi[qi * k + j] = si[j]; ov[qi * k + j] = sd[j]; }   // S may be below k
+}
+}  // namespace
+
+void launch_topk_rows(const float* D, i64 rows, i64 cols, int k, int32_t* oi, float* ov, hipStream_t st) {
+    require(rows >= 0 && cols >= 1 && cols < (i64)INT32_MAX, "bad tile shape");
+    require(k >= 1 && k <= 32 && k <= cols, "k must be in [1, min(32, cols)]");
+    if (rows == 0) return;
+    if (k <= 16) hipLaunchKernelGGL((k_topk_rows<16, 256>), dim3((unsigned)rows), dim3(256), 0, st, D, cols, k, oi, ov);
+    else hipLaunchKernelGGL((k_topk_rows<32, 128>), dim3((unsigned)rows), dim3(128), 0, st, D, cols, k, oi, ov);
+    MG_HIP(hipGetLastError());
+}
+
+}  // namespace mg
+
+namespace mg {
+// k nearest training rows of each query row (features d <= 64): indices and squared distances (m, k), sorted by
+// (distance, index); self_offset >= 0 leaves out training row q + self_offset for query q. Scratch: part_d / part_i,
+// m * S * 32 each, S = knn_chunks(m, nt).
+int knn_chunks(i64 m, i64 nt) {
+    const i64 tiles = (m + KNN_Q - 1) / KNN_Q;
+    int S = 1;
+    while (S < 256 && tiles * S < 1024 && (i64)S * 2 * KNN_TR <= nt) S *= 2;   // about 1024 blocks, >= 64 rows per chunk
+    return S;
+}
+void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
+                int32_t* oi, float* ov, hipStream_t st) {
+    require(d >= 1 && d <= 64, "the fused k-NN search takes up to 64 features");
+    require(k >= 1 && k <= 32 && k <= nt - (self_offset >= 0 ? 1 : 0), "k must be in [1, min(32, rows)]");
+    if (m == 0) return;
+    const int S = knn_chunks(m, nt);
+    const i64 chunk = (nt + S - 1) / S;
+    const dim3 g1((unsigned)((m + KNN_Q - 1) / KNN_Q), (unsigned)S);
+#define MG_K(DP, KM) hipLaunchKernelGGL((k_knn_chunks<DP, KM>), g1, dim3(KNN_Q), 0, st, T, nt, Q, m, d, chunk, self_offset, part_d, part_i, S)
+    if (k <= 16) { if (d <= 16) MG_K(16, 16); else if (d <= 32) MG_K(32, 16); else MG_K(64, 16); }
+    else { if (d <= 16) MG_K(16, 32); else if (d <= 32) MG_K(32, 32); else MG_K(64, 32); }
+#undef MG_K
+    MG_HIP(hipGetLastError());
+    const int KM = k <= 16 ? 16 : 32;
+    const size_t lds = (size_t)S * KM * (sizeof(float) + sizeof(int));
+    if (KM == 16) hipLaunchKernelGGL(k_knn_merge<16>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
+    else hipLaunchKernelGGL(k_knn_merge<32>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
+    MG_HIP(hipGetLastError());
+}
+}  // namespace mg

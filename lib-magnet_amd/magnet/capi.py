@@ -73,6 +73,8 @@ def lib():
             "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
                                   C.POINTER(C.c_double), CALLBACK, vp], i32),
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
+            "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
+            "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -216,6 +218,33 @@ def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.
                           U.data_ptr(), G.data_ptr(), sq[b].ctypes.data_as(C.POINTER(C.c_double)) if sumq else None)
     check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams), int(n_threads)))
     return [s[:n_iter] for s in sq] if sumq else None
+
+
+def topk_rows(D, k):
+    """(indices (rows, k) int32, values (rows, k) float32): the k smallest entries of each row of the float32
+    device tile D, sorted by (value, column)."""
+    import torch
+    if D.dtype != torch.float32 or D.dim() != 2 or not D.is_contiguous():
+        raise ValueError("D must be a contiguous (rows, cols) float32 tensor")
+    idx = torch.empty((D.shape[0], k), dtype=torch.int32, device=D.device)
+    val = torch.empty((D.shape[0], k), dtype=torch.float32, device=D.device)
+    check(lib().mg_topk_rows(ptr(D), D.shape[0], D.shape[1], int(k), ptr(idx), ptr(val)))
+    return idx, val
+
+
+def knn(T, Q, k, exclude_self=False):
+    """(indices (m, k) int32, squared distances (m, k) float32) of the k nearest rows of T (nt, d <= 64) to each row of
+    Q (m, d), float32 device tensors; exclude_self (Q is T) leaves each row out of its own list."""
+    import torch
+    for a in (T, Q):
+        if a.dtype != torch.float32 or a.dim() != 2 or not a.is_contiguous():
+            raise ValueError("T and Q must be contiguous 2-d float32 tensors")
+    if T.shape[1] != Q.shape[1]:
+        raise ValueError("T and Q must have the same number of features")
+    idx = torch.empty((Q.shape[0], k), dtype=torch.int32, device=Q.device)
+    d2 = torch.empty((Q.shape[0], k), dtype=torch.float32, device=Q.device)
+    check(lib().mg_knn(ptr(T), T.shape[0], ptr(Q), Q.shape[0], T.shape[1], int(k), 0 if exclude_self else -1, ptr(idx), ptr(d2)))
+    return idx, d2
 
 
 def profile(on=True):

@@ -109,7 +109,9 @@ class TransformServer:
             t0 = time.perf_counter()
             try:
                 Q = np.concatenate([m._prepare_query(r.X) for r in batch]) if points else np.zeros((0, m._prepared_.shape[1]), np.float32)
-                Y = m._place(torch.as_tensor(Q, device="cuda"), self.n_iter) if points else None
+                bounds = np.cumsum([0] + [len(r.X) for r in batch])
+                segs = list(zip(bounds[:-1].tolist(), bounds[1:].tolist()))
+                Y = m._place(torch.as_tensor(Q, device="cuda"), self.n_iter, segs) if points else None
                 out = Y.cpu().numpy().astype(np.float32) if points else np.zeros((0, 2), np.float32)
                 o = 0
                 t_done = time.perf_counter()

@@ -96,3 +96,51 @@ def test_serving_same_alone_and_in_the_mix(fitted):
         assert o.shape == a.shape and np.allclose(a, o, rtol=0, atol=1e-5 * scale)
     assert all(r.latency is not None and r.latency > 0 for r in reqs)
     assert sum(b[1] for b in srv.batches) == 300 and max(b[1] for b in srv.batches) <= max(120, max(map(len, parts)))
+
+
+@pytest.mark.parametrize("k,cols", [(14, 200_003), (5, 37), (29, 10_000), (1, 7), (16, 16)])
+def test_topk_rows_matches_a_full_sort(k, cols):
+    """mg_topk_rows: the k smallest of each row, ordered by (value, column), as a stable full sort gives them;
+    rows with ties included."""
+    import torch
+    from magnet import capi
+    rng = np.random.default_rng(k + cols)
+    D = rng.random((9, cols)).astype(np.float32)
+    D[3] = np.round(D[3] * 4) / 4          # many ties
+    D[5] = 0.5                             # all equal
+    idx, val = capi.topk_rows(torch.as_tensor(D, device="cuda"), k)
+    idx, val = idx.cpu().numpy(), val.cpu().numpy()
+    order = np.argsort(D, axis=1, kind="stable")[:, :k]
+    assert np.array_equal(idx, order.astype(np.int32))
+    assert np.array_equal(val, np.take_along_axis(D, order, 1))
+
+
+@pytest.mark.parametrize("d,k,exclude,m,nt", [(50, 14, False, 700, 3000), (7, 5, True, 700, 3000), (64, 29, False, 700, 3000),
+                                               (1, 3, True, 700, 3000), (12, 14, True, 500, 500)])
+def test_fused_knn_is_exact(d, k, exclude, m, nt):
+    """mg_knn against a float64 brute force: distances to 1e-5 relative, and the same neighbour set wherever the
+    k-th and (k+1)-th distances are apart; exclude_self leaves each row out of its own list; a query's result does
+    not depend on the other queries of the call. 500 training rows split into fewer chunks than k (the merge
+    then writes k results with fewer threads)."""
+    import torch
+    from magnet import capi
+    rng = np.random.default_rng(d * 100 + k)
+    T = rng.normal(0, 1, (nt, d)).astype(np.float32)
+    Q = T[:m].copy() if exclude else rng.normal(0, 1, (m, d)).astype(np.float32)
+    Tt, Qt = torch.as_tensor(T, device="cuda"), torch.as_tensor(Q, device="cuda")
+    idx, d2 = capi.knn(Tt, Qt, k, exclude)
+    idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
+    D = ((Q.astype(np.float64)[:, None, :] - T.astype(np.float64)[None, :, :]) ** 2).sum(-1)
+    if exclude:
+        D[np.arange(m), np.arange(m)] = np.inf
+    order = np.argsort(D, axis=1, kind="stable")
+    ref = np.take_along_axis(D, order, 1)
+    assert np.allclose(d2, ref[:, :k], rtol=1e-5, atol=1e-6)
+    clear = (ref[:, k] - ref[:, k - 1]) > 1e-4 * np.maximum(ref[:, k], 1e-12)
+    for q in np.nonzero(clear)[0]:
+        assert set(idx[q]) == set(order[q, :k]), q
+    if exclude:
+        assert not np.any(idx == np.arange(m)[:, None])
+    one_i, one_d = capi.knn(Tt, Qt[123:124].contiguous(), k, False)
+    if not exclude:
+        assert np.array_equal(one_i.cpu().numpy()[0], idx[123]) and np.array_equal(one_d.cpu().numpy()[0], d2[123])
