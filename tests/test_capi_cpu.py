@@ -92,3 +92,13 @@ def test_batch_entry_rejects_empty_and_bad_stream_counts():
     assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 2, 3) != 0
     assert b"n_threads" in L.mg_last_error()
     assert L.mg_tsne_optimize_batch(7, ctypes.byref(p), 1, arr, 1, 1) != 0
+
+
+def test_knn_entries_reject_bad_arguments():
+    L = capi.lib()
+    i64 = ctypes.c_int64
+    assert L.mg_knn(None, i64(10), None, i64(3), 4, 2, i64(-1), None, None) != 0     # null arrays
+    assert b"bad k-NN arguments" in L.mg_last_error()
+    assert L.mg_knn(None, i64(10), None, i64(0), 4, 2, i64(-1), None, None) == 0     # no queries: nothing to do
+    assert L.mg_topk_rows(None, i64(0), i64(5), 9, None, None) != 0                  # k above the row length
+    assert b"k must be" in L.mg_last_error()
