@@ -335,6 +335,29 @@ def copy_probe(mib):
     print(json.dumps({"copy_bytes": n * 4, "seconds": dt, "GBs": round(2 * n * 4 / dt / 1e9, 1)}), flush=True)
 
 
+def attraction_probe(workload, reps=5):
+    """The CSR attraction alone (the PMC probe of mg_tsne_attraction): the workload's P, a random layout
+    (N(0, 30^2) per coordinate, seed 1), `reps` launches; prints the mean launch time."""
+    import torch
+    from magnet import capi
+    from magnet.affinity import perplexity_affinities
+    X, _ = make_data(workload)
+    P = perplexity_affinities(X, 30.0)
+    del X
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    ip, ix = dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32))
+    dv = dev((P.data / P.data.sum()).astype(np.float32))
+    Y = dev(np.random.default_rng(1).normal(0, 30, (P.shape[0], 2)).astype(np.float32))
+    capi.attraction(ip, ix, dv, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        capi.attraction(ip, ix, dv, Y)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "attraction", "workload": workload, "n": int(P.shape[0]), "nnz": int(P.nnz),
+                      "mean_us": round(1e6 * (time.perf_counter() - t0) / reps, 1)}), flush=True)
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-worker":
         cpu_baseline_worker(sys.argv[2])
@@ -347,6 +370,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB", help="only run a plain copy of MIB MiB")
+    ap.add_argument("--attraction-probe", action="store_true", help="only time the CSR attraction of the workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the transform() serving leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched leg")
     args = ap.parse_args()
@@ -361,6 +385,11 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.attraction_probe:
+        capi.lib()
+        capi.use_torch_stream()
+        attraction_probe(args.workload)
         return
     capi.lib()
     capi.use_torch_stream()
