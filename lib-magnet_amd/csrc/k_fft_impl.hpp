@@ -299,20 +299,21 @@ __device__ __forceinline__ void spec1_block(double delta, double dof, const doub
     const int ac = live ? a : 0;
     cplx<T> za[QL][RL], zb[QL][RL];
     auto put = [](cplx<T> (&d)[QL][RL]) { return [&d](int, cplx<T> x, int q, int r) { d[q][r] = x; }; };
+    // the first pass reads each input slot (q, r) once, at the same element in both transforms: the y-odd
+    // kernel values of the second transform are kept from the first one's evaluation
+    constexpr int R0 = F::R[0], Q0 = (L / R0 + TH - 1) / TH;
+    T ky_slot[Q0][R0];
     {
-        auto ld = [&](int e, int, int) -> cplx<T> {
+        auto ld = [&](int e, int q, int r) -> cplx<T> {
             double k1, k2, ky;
             kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
+            if constexpr (NT == 4) ky_slot[q][r] = (T)ky;
             return {(T)k1, (T)k2};
         };
         fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, put(za));
     }
     if constexpr (NT == 4) {
-        auto ld = [&](int e, int, int) -> cplx<T> {
-            double k1, k2, ky;
-            kernel_values(ac, e, M, delta, dof, table, round_kernel, k1, k2, ky);
-            return {(T)ky, T(0)};
-        };
+        auto ld = [&](int, int q, int r) -> cplx<T> { return {ky_slot[q][r], T(0)}; };
         fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, put(zb));
     }
     if (!live) return;
