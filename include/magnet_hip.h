@@ -187,11 +187,12 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_in
                      const void* d_data, int64_t nnz, int64_t n, void* d_Y, void* d_U, void* d_G, const int32_t* d_pairs,
                      int m_pairs, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user);
 
-/* Batched form: `count` independent embeddings (CSR P, state Y / U / G of their own sizes, device arrays) on one
- * schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs). Instance b is
- * issued on HIP stream b mod n_streams, stream s by host thread s mod n_threads (1 <= n_threads <= n_streams), and
- * all advance together, so their kernels overlap on the device; each instance's result is the one
- * mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null. The call returns when every
+/* Batched form: `count` (<= 1024) independent embeddings (CSR P, state Y / U / G of their own sizes, device
+ * arrays) on one schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs;
+ * max_boxes <= 1024). Every stage of an iteration is one launch for all instances of a group (one per FFT size for
+ * the FFT stages). The instances are split into n_streams groups of about equal point counts, each on its own HIP
+ * stream, driven by n_threads host threads (1 <= n_threads <= n_streams). Each instance's result is bit for bit
+ * the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null. The call returns when every
  * instance is done. No counterpart in the reference (it fits one embedding per call). */
 typedef struct {
     const int32_t* indptr;
