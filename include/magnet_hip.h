@@ -137,6 +137,10 @@ int mg_topk_rows(const float* d_D, int64_t rows, int64_t cols, int k, int32_t* d
  * result does not depend on the other queries of the call. Calls are serialised (one scratch per process). */
 int mg_knn(const float* d_T, int64_t nt, const float* d_Q, int64_t m, int d, int k, int64_t self_offset, int32_t* d_idx,
            float* d_dist2);
+/* smooth_knn (magnet/knn.py:165-198) on the device, float32: memberships d_vals (m, k) of the sorted neighbour
+ * distances d_dist (m, k), one row per thread; d_mean_all (m) is each row's fallback mean distance (the mean over
+ * its call, used when the row has no non-zero distance). */
+int mg_smooth_knn(const float* d_dist, int64_t m, int k, int n_neighbors, const float* d_mean_all, float* d_vals);
 
 /* ---- fallback kernels (src/fftrep.cpp:51-125, 294-411; magnet/tsne.py:92-129 calls them when FFTConv2D is not
  * used). Box of a point: u = (x - xmin) / width clamped to [0, n_boxes - 1], product-form Lagrange weights;

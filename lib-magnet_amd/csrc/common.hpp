@@ -139,6 +139,9 @@ int knn_chunks(i64 m, i64 nt);
 void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
                 int32_t* oi, float* ov, hipStream_t st);
 
+// smooth k-NN memberships of sorted neighbour distances (m, k), per row; mean_all (m): the fallback mean distance
+void launch_smooth_knn(const float* dist, i64 m, int k, int n_neighbors, const float* mean_all, float* vals, hipStream_t st);
+
 // locality relabelling (k_order.hip)
 void launch_identity_labels(int* perm, int* inv, i64 n, hipStream_t st);
 template <class T> void launch_relabel(const int* vals, const int* perm, i64 n, const int32_t* ip, const int32_t* ix, const T* P,

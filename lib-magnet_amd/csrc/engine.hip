@@ -1076,6 +1076,14 @@ int mg_knn(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, 
     MG_API_END
 }
 
+int mg_smooth_knn(const float* dist, int64_t m, int k, int n_neighbors, const float* mean_all, float* vals) {
+    MG_API_BEGIN
+    require(m == 0 || (dist && mean_all && vals), "null array");
+    require(n_neighbors >= 2, "n_neighbors must be >= 2");
+    launch_smooth_knn(dist, m, k, n_neighbors, mean_all, vals, g_stream);
+    MG_API_END
+}
+
 int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
     MG_API_BEGIN
     require(rows == 0 || (D && idx && val), "null array");

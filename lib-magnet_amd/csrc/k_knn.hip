@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdint>
+#include <cmath>
 
 namespace mg {
 
@@ -219,6 +220,57 @@ void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64
     const size_t lds = (size_t)S * KM * (sizeof(float) + sizeof(int));
     if (KM == 16) hipLaunchKernelGGL(k_knn_merge<16>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
     else hipLaunchKernelGGL(k_knn_merge<32>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
+    MG_HIP(hipGetLastError());
+}
+}  // namespace mg
+
+namespace mg {
+namespace {
+// UMAP's smooth k-NN memberships of one row of sorted neighbour distances (the numpy restatement of the
+// reference, magnet/knn.py:165-198, in float32): rho = the nearest non-zero distance, sigma by bisection so that
+// sum exp(-(d - rho) / sigma) = log2(n_neighbors) * bandwidth, floored at 1e-3 of the row mean (of `mean_all`
+// when the row has no non-zero distance).
+template <int KM>
+__global__ __launch_bounds__(256) void k_smooth_knn(const float* __restrict__ dist, i64 m, int k, float target, int n_iter,
+                                                    const float* __restrict__ mean_all, float* __restrict__ vals) {
+    const i64 row = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (row >= m) return;
+    float d[KM];
+    float rho = 0.f, sum = 0.f;
+    bool has = false;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+        d[j] = j < k ? dist[row * k + j] : 0.f;
+        if (j < k) { sum += d[j]; if (!has && d[j] > 0.f) { rho = d[j]; has = true; } }
+    }
+#pragma unroll
+    for (int j = 0; j < KM; ++j) d[j] = j < k ? fmaxf(d[j] - rho, 0.f) : 0.f;
+    float lo = 0.f, hi = INFINITY, mid = 1.f, sigma = 1.f;
+    bool done = false;
+    for (int it = 0; it < n_iter && !done; ++it) {
+        float psum = 0.f;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) if (j < k) psum += expf(-d[j] / mid);
+        if (fabsf(psum - target) < 1e-5f) { sigma = mid; done = true; break; }
+        const bool gt = psum > target;
+        hi = gt ? mid : hi;
+        lo = gt ? lo : mid;
+        mid = gt ? (lo + hi) / 2.f : (isinf(hi) ? mid * 2.f : (lo + hi) / 2.f);
+    }
+    if (!done) sigma = mid;
+    const float mean_row = sum / (float)k;
+    sigma = has ? fmaxf(sigma, 1e-3f * mean_row) : fmaxf(sigma, 1e-3f * mean_all[row]);
+    for (int j = 0; j < k; ++j) vals[row * k + j] = d[j] <= 0.f ? 1.f : expf(-d[j] / sigma);
+}
+}  // namespace
+
+void launch_smooth_knn(const float* dist, i64 m, int k, int n_neighbors, const float* mean_all, float* vals, hipStream_t st) {
+    require(k >= 1 && k <= 32, "k must be in [1, 32]");
+    if (m == 0) return;
+    const float target = (float)std::log2((double)n_neighbors);
+    const dim3 grid((unsigned)((m + 255) / 256));
+    if (k <= 16) hipLaunchKernelGGL(k_smooth_knn<16>, grid, dim3(256), 0, st, dist, m, k, target, 64, mean_all, vals);
+    else hipLaunchKernelGGL(k_smooth_knn<32>, grid, dim3(256), 0, st, dist, m, k, target, 64, mean_all, vals);
     MG_HIP(hipGetLastError());
 }
 }  // namespace mg

@@ -75,6 +75,7 @@ def lib():
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
+            "mg_smooth_knn": ([vp, i64, i32, i32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -245,6 +246,18 @@ def knn(T, Q, k, exclude_self=False):
     d2 = torch.empty((Q.shape[0], k), dtype=torch.float32, device=Q.device)
     check(lib().mg_knn(ptr(T), T.shape[0], ptr(Q), Q.shape[0], T.shape[1], int(k), 0 if exclude_self else -1, ptr(idx), ptr(d2)))
     return idx, d2
+
+
+def smooth_knn(dist, n_neighbors, mean_all):
+    """Smooth k-NN memberships (m, k) of the sorted neighbour distances dist (m, k) on the device (float32);
+    mean_all (m,): each row's fallback mean distance."""
+    import torch
+    if dist.dtype != torch.float32 or dist.dim() != 2 or not dist.is_contiguous():
+        raise ValueError("dist must be a contiguous (m, k) float32 tensor")
+    mean_all = mean_all.to(torch.float32).contiguous()
+    vals = torch.empty_like(dist)
+    check(lib().mg_smooth_knn(ptr(dist), dist.shape[0], dist.shape[1], int(n_neighbors), ptr(mean_all), ptr(vals)))
+    return vals
 
 
 def profile(on=True):

@@ -144,3 +144,20 @@ def test_fused_knn_is_exact(d, k, exclude, m, nt):
     one_i, one_d = capi.knn(Tt, Qt[123:124].contiguous(), k, False)
     if not exclude:
         assert np.array_equal(one_i.cpu().numpy()[0], idx[123]) and np.array_equal(one_d.cpu().numpy()[0], d2[123])
+
+
+def test_device_smooth_knn_matches_the_numpy_restatement():
+    """mg_smooth_knn against magnet.graph.smooth_knn (the reference's numpy algorithm) on sorted distances,
+    with leading zeros and an all-zero row (the fallback to the call's mean distance)."""
+    import torch
+    from magnet import capi
+    from magnet.graph import smooth_knn
+    rng = np.random.default_rng(3)
+    D = np.sort(rng.gamma(2.0, 1.0, (500, 14)).astype(np.float32), axis=1)
+    D[7, :3] = 0.0
+    D[11] = 0.0
+    ref = smooth_knn(D, 15)
+    mean_all = torch.full((500,), float(D.mean()), device="cuda")
+    got = capi.smooth_knn(torch.as_tensor(D, device="cuda"), 15, mean_all).cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-6)
+    assert np.all(got[11] == 1.0) and np.all(got[7, :3] == 1.0)
