@@ -140,11 +140,15 @@ __device__ __forceinline__ void attr2_block(const int32_t* __restrict__ ip, cons
     for (int o = G / 2; o > 0; o >>= 1) { ax += __shfl_xor(ax, o, G); ay += __shfl_xor(ay, o, G); }
     if (row < n && sub == 0) { V r; r.x = ax; r.y = ay; reinterpret_cast<V*>(out)[row] = r; }
 }
+// Workgroups are dispatched to the 8 XCDs in turn; with a grid of a multiple of 8 blocks, XCD x takes the x-th
+// eighth of the rows, so neighbouring rows (neighbouring points once the layout is relabelled in box order) share
+// one XCD's L2.
 template <class T, int G, int U>
 __global__ __launch_bounds__(TPB) void k_attr2(const int32_t* __restrict__ ip, const int32_t* __restrict__ ix,
                                                const T* __restrict__ P, const T* __restrict__ Y, i64 n, T inv_dof,
                                                T* __restrict__ out) {
-    attr2_block<T, G, U>(ip, ix, P, Y, n, inv_dof, out, blockIdx.x);
+    const int b = (gridDim.x % 8) == 0 ? (int)((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8) : (int)blockIdx.x;
+    attr2_block<T, G, U>(ip, ix, P, Y, n, inv_dof, out, b);
 }
 template <class T, int G, int U>
 __global__ __launch_bounds__(TPB) void kb_attr2(const BInst<T>* __restrict__ I, BMap m, const T* __restrict__ Y, T inv_dof,
@@ -389,7 +393,7 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
     if (n == 0) return;
     if (dim == 2) {
         const int G = attraction_lanes(n, nnz);
-        const unsigned blocks = (unsigned)attraction_blocks(n, G);
+        const unsigned blocks = (unsigned)((attraction_blocks(n, G) + 7) / 8 * 8);   // rows past n: no work
         switch (G) {
 #define MG_ATTR_CASE(g) case g: hipLaunchKernelGGL((k_attr2<T, g, 8>), dim3(blocks), dim3(TPB), 0, st, ip, ix, P, Y, n, inv_dof, out); break;
             MG_ATTR_CASE(1) MG_ATTR_CASE(2) MG_ATTR_CASE(4) MG_ATTR_CASE(8) MG_ATTR_CASE(16) MG_ATTR_CASE(32) MG_ATTR_CASE(64)
