@@ -13,6 +13,15 @@ The per-stage roofline comes from a second, profiled pass of the same iterations
 library's stream, algorithmic bytes from the engine). cpu_baseline times the reference's own compiled
 module (oracle/_ref) on this host's cores on a bounded sample of iterations taken along the GPU
 trajectory.
+
+Two more legs run with the default workload and are reported beside the headline:
+- `serving`: a 200,000 x 50 MAGNET(loss="tsne") model kept resident, transform() requests of 1-4096 rows
+  arriving by a Poisson process at 200 QPS through the continuous-batching server; p50 / p99 latency
+  (`transform_p99_ms`) and the saturated rate;
+- `batched`: 64 independent embeddings (2,000-50,000 points each, 500 iterations) through the batched engine,
+  against the same instances one after another, with a bit-for-bit comparison of the results.
+--workload gmm1m / gmm10k run the 1M and 10k configurations; --attraction-probe times the CSR attraction of a
+workload alone (for PMC counters); --copy-probe times a plain copy (the PMC scale reference).
 """
 import argparse
 import json
