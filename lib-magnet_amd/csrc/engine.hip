@@ -253,13 +253,16 @@ template <class T> struct Repulsion {
         item_key.ensure((size_t)2 * nch * sizeof(int));
         item_valid.ensure((size_t)2 * nch * sizeof(int));
         item_val.ensure((size_t)2 * nch * ni * 3 * ni * sizeof(T));
+        // the grid buffers hold the largest grid of this FFT size (N <= M / 2): a layout that grows by a few
+        // boxes then reuses them, instead of a hipFree (which waits for the device) and a hipMalloc per growth
         const int Mh = M / 2 + 1;
-        W.ensure((size_t)3 * N * N * sizeof(T));
-        Wh.ensure((size_t)3 * Mh * N * sizeof(cplx<T>));
-        Ph.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
-        Tr.ensure((size_t)4 * Mh * N * sizeof(cplx<T>));
+        const size_t Nc = (size_t)std::max(N, M / 2);
+        W.ensure(3 * Nc * Nc * sizeof(T));
+        Wh.ensure(3 * Mh * Nc * sizeof(cplx<T>));
+        Ph.ensure(4 * Mh * Nc * sizeof(cplx<T>));
+        Tr.ensure(4 * Mh * Nc * sizeof(cplx<T>));
 
-        phi.ensure((size_t)4 * N * N * sizeof(T));
+        phi.ensure(4 * Nc * Nc * sizeof(T));
         zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
     }
 
