@@ -138,7 +138,7 @@ def matched_precision(P, snaps, n_iter, early, k):
     return rows
 
 
-def serving_bench(qps=None, duration=None, seed=0):
+def serving_bench(qps=None, duration=None, seed=0, n_workers=3):
     """The serving configuration: fit, then Poisson transform() requests against the resident model through the
     continuous-batching server; also the saturated rate (all requests queued at once)."""
     import torch
@@ -156,7 +156,7 @@ def serving_bench(qps=None, duration=None, seed=0):
     t0 = time.perf_counter()
     m = MAGNET(loss="tsne", random_state=0, transform_epochs=S["n_iter"]).fit(Xtr)
     t_fit = time.perf_counter() - t0
-    srv = TransformServer(m).start()
+    srv = TransformServer(m, n_workers=n_workers).start()
 
     def sampler(r):
         size = int(round(np.exp(r.uniform(0, np.log(4096)))))
@@ -177,7 +177,7 @@ def serving_bench(qps=None, duration=None, seed=0):
     finally:
         srv.stop()
     rep.update({"qps": qps, "duration_s": duration, "fit_s": round(t_fit, 2), "saturated_points_per_s": round(sat, 1),
-                "batches": len(srv.batches), "workload": S["desc"]})
+                "batches": len(srv.batches), "n_workers": n_workers, "workload": S["desc"]})
     return rep
 
 

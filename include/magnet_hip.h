@@ -8,7 +8,8 @@
  *   - `prec` selects the arithmetic: MG_F64 (double arrays) or MG_F32 (float arrays). Index arrays are
  *     int32 in both. Point arrays are (n, dim) row-major, 2-D paths use dim == 2.
  *   - Pointers named d_* are device (HBM) pointers; h_* are host pointers. Work is issued on the
- *     library stream (mg_set_stream, default the null stream) and is ordered on it.
+ *     library stream and is ordered on it: the calling thread's own stream if it set one
+ *     (mg_set_thread_stream), else the process-wide one (mg_set_stream, default the null stream).
  *   - Entry points that return a host scalar (sumQ, bbox) synchronise the stream before returning.
  */
 #ifndef MAGNET_HIP_H
@@ -33,6 +34,13 @@ extern "C" {
 const char* mg_last_error(void);
 int mg_version(void);                                   /* 100 * major + minor */
 int mg_set_stream(void* hip_stream);                    /* hipStream_t; NULL = null stream */
+/* The calling thread's stream, in place of the process-wide one, for the rest of the thread's life. No
+ * counterpart in the reference (its calls run on host threads). For callers that drive the device from
+ * several host threads at once, e.g. a server with more than one batch in flight: entry points called from
+ * different threads on different streams overlap on the device. mg_knn keeps its scratch per thread; the
+ * other entry points share no scratch between threads, except those on one mg_fftconv object, which is
+ * not re-entrant (mg_tsne_transform only reads the object's field and may run from several threads). */
+int mg_set_thread_stream(void* hip_stream);
 int mg_synchronize(void);
 int mg_malloc(void** d_ptr, size_t bytes);
 int mg_free(void* d_ptr);

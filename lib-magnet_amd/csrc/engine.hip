@@ -13,7 +13,6 @@
 #include <atomic>
 #include <list>
 #include <memory>
-#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -29,7 +28,12 @@ template <class T> int launch_gather_forces(const T*, i64, const T*, int, double
 template <class T> void launch_spectral_products(const T*, const T*, const T*, i64, i64, T*, hipStream_t);
 
 static thread_local std::string g_err;
-static hipStream_t g_stream = nullptr;
+// The stream the entry points issue on: the calling thread's own (mg_set_thread_stream) if it set one, else
+// the process-wide one (mg_set_stream, default the null stream).
+static hipStream_t g_stream_all = nullptr;
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local bool t_stream_set = false;
+static inline hipStream_t lib_stream() { return t_stream_set ? t_stream : g_stream_all; }
 
 void hip_check(hipError_t e, const char* what) {
     if (e == hipSuccess) return;
@@ -867,8 +871,8 @@ template <class T> static void step_impl(int mode, void* Y, void* U, void* G, co
     require(Y && U && G && A && (mode == 1 || F), "null array");
     DevBuf part, mb;
     launch_update<T>(mode, (T*)Y, (T*)U, (T*)G, (const T*)A, (const T*)F, nullptr, (T)inv, (T)rho, (T)lr, (T)mom, nullptr, 0,
-                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), nullptr, g_stream);
-    MG_HIP(hipStreamSynchronize(g_stream));
+                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), nullptr, lib_stream());
+    MG_HIP(hipStreamSynchronize(lib_stream()));
 }
 
 // The device spectra use the symmetry of the kernel in each offset separately (K(a, b) = K(-a, b) = K(a, -b)),
@@ -897,8 +901,11 @@ extern "C" {
 const char* mg_last_error(void) { return g_err.c_str(); }
 int mg_version(void) { return 100; }
 
-int mg_set_stream(void* s) { MG_API_BEGIN g_stream = reinterpret_cast<hipStream_t>(s); MG_API_END }
-int mg_synchronize(void) { MG_API_BEGIN MG_HIP(hipStreamSynchronize(g_stream)); MG_API_END }
+int mg_set_stream(void* s) { MG_API_BEGIN g_stream_all = reinterpret_cast<hipStream_t>(s); MG_API_END }
+int mg_set_thread_stream(void* s) {
+    MG_API_BEGIN t_stream = reinterpret_cast<hipStream_t>(s); t_stream_set = true; MG_API_END
+}
+int mg_synchronize(void) { MG_API_BEGIN MG_HIP(hipStreamSynchronize(lib_stream())); MG_API_END }
 int mg_malloc(void** p, size_t bytes) {
     MG_API_BEGIN
     require(p != nullptr, "null output pointer");
@@ -907,9 +914,9 @@ int mg_malloc(void** p, size_t bytes) {
     MG_API_END
 }
 int mg_free(void* p) { MG_API_BEGIN if (p) MG_HIP(hipFree(p)); MG_API_END }
-int mg_memcpy_htod(void* d, const void* h, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, g_stream)); MG_HIP(hipStreamSynchronize(g_stream)); } MG_API_END }
-int mg_memcpy_dtoh(void* h, const void* d, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, g_stream)); MG_HIP(hipStreamSynchronize(g_stream)); } MG_API_END }
-int mg_memcpy_dtod(void* d, const void* s, size_t b) { MG_API_BEGIN if (b) MG_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, g_stream)); MG_API_END }
+int mg_memcpy_htod(void* d, const void* h, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, lib_stream())); MG_HIP(hipStreamSynchronize(lib_stream())); } MG_API_END }
+int mg_memcpy_dtoh(void* h, const void* d, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, lib_stream())); MG_HIP(hipStreamSynchronize(lib_stream())); } MG_API_END }
+int mg_memcpy_dtod(void* d, const void* s, size_t b) { MG_API_BEGIN if (b) MG_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, lib_stream())); MG_API_END }
 int mg_mem_info(size_t* f, size_t* t) { MG_API_BEGIN require(f && t, "null output pointer"); MG_HIP(hipMemGetInfo(f, t)); MG_API_END }
 
 int mg_profile_enable(int on) {
@@ -943,16 +950,16 @@ int mg_bbox_2d(int prec, const void* Y, int64_t n, double* out) {
     require(Y && out && n >= 1, "Y must be (n, 2) with n >= 1");
     DevBuf part, cnt, res;
     unsigned* c = reinterpret_cast<unsigned*>(cnt.ensure(64));
-    MG_HIP(hipMemsetAsync(c, 0, 64, g_stream));
+    MG_HIP(hipMemsetAsync(c, 0, 64, lib_stream()));
     if (prec == MG_F64) {
-        launch_bbox<double>((const double*)Y, n, (double*)part.ensure(4 * 1024 * 8), c, (double*)res.ensure(32), g_stream);
-        MG_HIP(hipMemcpyAsync(out, res.p, 32, hipMemcpyDeviceToHost, g_stream));
-        MG_HIP(hipStreamSynchronize(g_stream));
+        launch_bbox<double>((const double*)Y, n, (double*)part.ensure(4 * 1024 * 8), c, (double*)res.ensure(32), lib_stream());
+        MG_HIP(hipMemcpyAsync(out, res.p, 32, hipMemcpyDeviceToHost, lib_stream()));
+        MG_HIP(hipStreamSynchronize(lib_stream()));
     } else {
         float h[4];
-        launch_bbox<float>((const float*)Y, n, (float*)part.ensure(4 * 1024 * 4), c, (float*)res.ensure(16), g_stream);
-        MG_HIP(hipMemcpyAsync(h, res.p, 16, hipMemcpyDeviceToHost, g_stream));
-        MG_HIP(hipStreamSynchronize(g_stream));
+        launch_bbox<float>((const float*)Y, n, (float*)part.ensure(4 * 1024 * 4), c, (float*)res.ensure(16), lib_stream());
+        MG_HIP(hipMemcpyAsync(h, res.p, 16, hipMemcpyDeviceToHost, lib_stream()));
+        MG_HIP(hipStreamSynchronize(lib_stream()));
         for (int k = 0; k < 4; ++k) out[k] = h[k];
     }
     MG_API_END
@@ -965,9 +972,9 @@ int mg_tsne_attraction(int prec, const int32_t* ip, const int32_t* ix, const voi
     require(dof > 0, "dof must be positive");
     require(n >= 0 && dim >= 1 && dim <= 64, "dim out of range");
     require(n == 0 || (Y && out && (nnz == 0 || data)), "null array");
-    validate_csr(ip, ix, n, nnz, g_stream);
-    if (prec == MG_F64) launch_attraction<double>(ip, ix, (const double*)data, (const double*)Y, n, dim, 1.0 / dof, (double*)out, nnz, g_stream);
-    else launch_attraction<float>(ip, ix, (const float*)data, (const float*)Y, n, dim, 1.f / (float)dof, (float*)out, nnz, g_stream);
+    validate_csr(ip, ix, n, nnz, lib_stream());
+    if (prec == MG_F64) launch_attraction<double>(ip, ix, (const double*)data, (const double*)Y, n, dim, 1.0 / dof, (double*)out, nnz, lib_stream());
+    else launch_attraction<float>(ip, ix, (const float*)data, (const float*)Y, n, dim, 1.f / (float)dof, (float*)out, nnz, lib_stream());
     MG_API_END
 }
 
@@ -993,8 +1000,8 @@ int mg_mid_spring(int prec, const void* Y, const int32_t* pairs, void* grad, int
     check_prec(prec);
     require(n >= 0 && dim >= 1 && dim <= 64 && m >= 1, "bad shapes");
     require(n == 0 || (Y && pairs && grad), "null array");
-    if (prec == MG_F64) launch_mid_spring<double>((const double*)Y, pairs, (double*)grad, n, dim, m, w, g_stream);
-    else launch_mid_spring<float>((const float*)Y, pairs, (float*)grad, n, dim, m, (float)w, g_stream);
+    if (prec == MG_F64) launch_mid_spring<double>((const double*)Y, pairs, (double*)grad, n, dim, m, w, lib_stream());
+    else launch_mid_spring<float>((const float*)Y, pairs, (float*)grad, n, dim, m, (float)w, lib_stream());
     MG_API_END
 }
 
@@ -1039,10 +1046,10 @@ int mg_fftconv_compute(mg_fftconv* c, const void* Y, int64_t n, double xmin, dou
         typedef decltype(tag) T;
         require(n_boxes * r.ni <= r.M_user, "grid larger than the buffer");
         const GridT<T> g = make_grid<T>(xmin, ymin, width, n_boxes, r.ni);
-        r.compute((const T*)Y, n, g, (T*)F, nullptr, g_stream);
+        r.compute((const T*)Y, n, g, (T*)F, nullptr, lib_stream());
         double z[2];
-        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, g_stream));
-        MG_HIP(hipStreamSynchronize(g_stream));
+        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, lib_stream()));
+        MG_HIP(hipStreamSynchronize(lib_stream()));
         if (h_sumQ) *h_sumQ = z[0];
     };
     if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));
@@ -1056,9 +1063,9 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
         typedef decltype(zero) T;
         require(r.last_N > 0, "no compute yet");
         const size_t NN = (size_t)r.last_N * r.last_N;
-        if (d_W) MG_HIP(hipMemcpyAsync(d_W, r.W.p, 3 * NN * sizeof(T), hipMemcpyDeviceToDevice, g_stream));
-        if (d_phi) launch_phi_interleave<T>(dptr<T>(r.phi), reinterpret_cast<T*>(d_phi), (i64)NN, g_stream);
-        MG_HIP(hipStreamSynchronize(g_stream));
+        if (d_W) MG_HIP(hipMemcpyAsync(d_W, r.W.p, 3 * NN * sizeof(T), hipMemcpyDeviceToDevice, lib_stream()));
+        if (d_phi) launch_phi_interleave<T>(dptr<T>(r.phi), reinterpret_cast<T*>(d_phi), (i64)NN, lib_stream());
+        MG_HIP(hipStreamSynchronize(lib_stream()));
     };
     if (c->prec == MG_F64) copy(*c->f64, double(0)); else copy(*c->f32, float(0));
     MG_API_END
@@ -1066,15 +1073,13 @@ int mg_fftconv_grids(mg_fftconv* c, void* d_W, void* d_phi) {
 
 int mg_knn(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, int64_t self_offset, int32_t* idx, float* dist2) {
     MG_API_BEGIN
-    static std::mutex mu;   // one scratch for the process: concurrent calls are serialised
-    static DevBuf pd, pi;
-    std::lock_guard<std::mutex> lk(mu);
+    static thread_local DevBuf pd, pi;   // scratch per host thread: threads on their own streams run concurrently
     require(m >= 0 && nt >= 1 && (m == 0 || (T && Q && idx && dist2)), "bad k-NN arguments");
     if (m > 0) {
         const size_t sz = (size_t)m * knn_chunks(m, nt) * 32;
         launch_knn(T, nt, Q, m, d, k, self_offset, reinterpret_cast<float*>(pd.ensure(sz * sizeof(float))),
-                   reinterpret_cast<int*>(pi.ensure(sz * sizeof(int))), idx, dist2, g_stream);
-        MG_HIP(hipStreamSynchronize(g_stream));
+                   reinterpret_cast<int*>(pi.ensure(sz * sizeof(int))), idx, dist2, lib_stream());
+        MG_HIP(hipStreamSynchronize(lib_stream()));
     }
     MG_API_END
 }
@@ -1083,14 +1088,14 @@ int mg_smooth_knn(const float* dist, int64_t m, int k, int n_neighbors, const fl
     MG_API_BEGIN
     require(m == 0 || (dist && mean_all && vals), "null array");
     require(n_neighbors >= 2, "n_neighbors must be >= 2");
-    launch_smooth_knn(dist, m, k, n_neighbors, mean_all, vals, g_stream);
+    launch_smooth_knn(dist, m, k, n_neighbors, mean_all, vals, lib_stream());
     MG_API_END
 }
 
 int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
     MG_API_BEGIN
     require(rows == 0 || (D && idx && val), "null array");
-    launch_topk_rows(D, rows, cols, k, idx, val, g_stream);
+    launch_topk_rows(D, rows, cols, k, idx, val, lib_stream());
     MG_API_END
 }
 
@@ -1115,11 +1120,11 @@ int mg_tsne_transform(mg_fftconv* c, const void* Ytr, int64_t n_train, const int
         typedef decltype(zero) T;
         require(r.last_N > 0, "the field has not been computed: call mg_fftconv_compute on the training layout first");
         double z[2];
-        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, g_stream));
-        MG_HIP(hipStreamSynchronize(g_stream));
+        MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, lib_stream()));
+        MG_HIP(hipStreamSynchronize(lib_stream()));
         launch_transform<T>((const T*)Ytr, n_train, nbr, (const T*)w, k, (T*)Yn, m, r.last_g, r.ni, r.centred, dptr<T>(r.phi), z[1],
-                            n_iter, lr, mom0, mom1, mom_switch, exaggeration, dof, g_stream);
-        MG_HIP(hipStreamSynchronize(g_stream));
+                            n_iter, lr, mom0, mom1, mom_switch, exaggeration, dof, lib_stream());
+        MG_HIP(hipStreamSynchronize(lib_stream()));
     };
     if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));
     MG_API_END
@@ -1134,9 +1139,9 @@ int mg_interp_scatter(int prec, const void* Y, int64_t n, const void* q, int C, 
     require(nb >= 1 && ni >= 1 && ni <= 8, "bad grid parameters");
     DevBuf ws;
     int* w = reinterpret_cast<int*>(ws.ensure(shim_scratch_ints(n, nb) * sizeof(int)));
-    if (prec == MG_F64) launch_interp_scatter<double>((const double*)Y, n, (const double*)q, C, xmin, ymin, width, nb, ni, (double*)out, w, g_stream);
-    else launch_interp_scatter<float>((const float*)Y, n, (const float*)q, C, xmin, ymin, width, nb, ni, (float*)out, w, g_stream);
-    MG_HIP(hipStreamSynchronize(g_stream));
+    if (prec == MG_F64) launch_interp_scatter<double>((const double*)Y, n, (const double*)q, C, xmin, ymin, width, nb, ni, (double*)out, w, lib_stream());
+    else launch_interp_scatter<float>((const float*)Y, n, (const float*)q, C, xmin, ymin, width, nb, ni, (float*)out, w, lib_stream());
+    MG_HIP(hipStreamSynchronize(lib_stream()));
     MG_API_END
 }
 
@@ -1145,8 +1150,8 @@ int mg_interp_gather(int prec, const void* Y, int64_t n, const void* G, int C, d
     MG_API_BEGIN
     check_prec(prec);
     require(n >= 0 && (n == 0 || (Y && G && out)), "null array");
-    if (prec == MG_F64) launch_interp_gather<double>((const double*)Y, n, (const double*)G, C, xmin, ymin, width, nb, ni, (double*)out, g_stream);
-    else launch_interp_gather<float>((const float*)Y, n, (const float*)G, C, xmin, ymin, width, nb, ni, (float*)out, g_stream);
+    if (prec == MG_F64) launch_interp_gather<double>((const double*)Y, n, (const double*)G, C, xmin, ymin, width, nb, ni, (double*)out, lib_stream());
+    else launch_interp_gather<float>((const float*)Y, n, (const float*)G, C, xmin, ymin, width, nb, ni, (float*)out, lib_stream());
     MG_API_END
 }
 
@@ -1158,9 +1163,9 @@ int mg_scatter_padded_2d(int prec, const void* Y, int64_t n, double xmin, double
     require(nb >= 1 && ni >= 1 && ni <= 8, "bad grid parameters");
     DevBuf ws;
     int* w = reinterpret_cast<int*>(ws.ensure(shim_scratch_ints(n, nb) * sizeof(int)));
-    if (prec == MG_F64) launch_scatter_padded<double>((const double*)Y, n, xmin, ymin, width, nb, ni, M, (double*)out, w, g_stream);
-    else launch_scatter_padded<float>((const float*)Y, n, xmin, ymin, width, nb, ni, M, (float*)out, w, g_stream);
-    MG_HIP(hipStreamSynchronize(g_stream));
+    if (prec == MG_F64) launch_scatter_padded<double>((const double*)Y, n, xmin, ymin, width, nb, ni, M, (double*)out, w, lib_stream());
+    else launch_scatter_padded<float>((const float*)Y, n, xmin, ymin, width, nb, ni, M, (float*)out, w, lib_stream());
+    MG_HIP(hipStreamSynchronize(lib_stream()));
     MG_API_END
 }
 
@@ -1169,8 +1174,8 @@ int mg_spectral_products(int prec, const void* K1, const void* K2, const void* W
     check_prec(prec);
     require(M >= 0 && Mh >= 0, "bad shapes");
     require(M * Mh == 0 || (K1 && K2 && W && out), "null array");
-    if (prec == MG_F64) launch_spectral_products<double>((const double*)K1, (const double*)K2, (const double*)W, M, Mh, (double*)out, g_stream);
-    else launch_spectral_products<float>((const float*)K1, (const float*)K2, (const float*)W, M, Mh, (float*)out, g_stream);
+    if (prec == MG_F64) launch_spectral_products<double>((const double*)K1, (const double*)K2, (const double*)W, M, Mh, (double*)out, lib_stream());
+    else launch_spectral_products<float>((const float*)K1, (const float*)K2, (const float*)W, M, Mh, (float*)out, lib_stream());
     MG_API_END
 }
 
@@ -1182,11 +1187,11 @@ int mg_gather_forces_2d(int prec, const void* Y, int64_t n, const void* phi, int
     DevBuf part;
     double* p = reinterpret_cast<double*>(part.ensure((size_t)std::max<i64>(1, (n + 255) / 256) * sizeof(double)));
     int nparts;
-    if (prec == MG_F64) nparts = launch_gather_forces<double>((const double*)Y, n, (const double*)phi, M, xmin, ymin, width, nb, ni, (double*)F, p, g_stream);
-    else nparts = launch_gather_forces<float>((const float*)Y, n, (const float*)phi, M, xmin, ymin, width, nb, ni, (float*)F, p, g_stream);
+    if (prec == MG_F64) nparts = launch_gather_forces<double>((const double*)Y, n, (const double*)phi, M, xmin, ymin, width, nb, ni, (double*)F, p, lib_stream());
+    else nparts = launch_gather_forces<float>((const float*)Y, n, (const float*)phi, M, xmin, ymin, width, nb, ni, (float*)F, p, lib_stream());
     std::vector<double> h((size_t)std::max(1, nparts), 0.0);
-    if (nparts > 0) MG_HIP(hipMemcpyAsync(h.data(), p, (size_t)nparts * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    MG_HIP(hipStreamSynchronize(g_stream));
+    if (nparts > 0) MG_HIP(hipMemcpyAsync(h.data(), p, (size_t)nparts * sizeof(double), hipMemcpyDeviceToHost, lib_stream()));
+    MG_HIP(hipStreamSynchronize(lib_stream()));
     double s = 0.0;
     for (int k = 0; k < nparts; ++k) s += h[k];
     if (h_sumQ) *h_sumQ = s - (double)n;
@@ -1202,10 +1207,10 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* p, const int32_t* ip, const
     require(nnz == 0 || data != nullptr, "data must be a device array");
     if (prec == MG_F64) {
         Optimizer<double> o(p->n_interp);
-        o.run(p, ip, ix, (const double*)data, nnz, n, (double*)Y, (double*)U, (double*)G, pairs, m, it0, it1, h_sumQ, cb, user, g_stream);
+        o.run(p, ip, ix, (const double*)data, nnz, n, (double*)Y, (double*)U, (double*)G, pairs, m, it0, it1, h_sumQ, cb, user, lib_stream());
     } else {
         Optimizer<float> o(p->n_interp);
-        o.run(p, ip, ix, (const float*)data, nnz, n, (float*)Y, (float*)U, (float*)G, pairs, m, it0, it1, h_sumQ, cb, user, g_stream);
+        o.run(p, ip, ix, (const float*)data, nnz, n, (float*)Y, (float*)U, (float*)G, pairs, m, it0, it1, h_sumQ, cb, user, lib_stream());
     }
     MG_API_END
 }

@@ -48,7 +48,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int, C.c_double
         sig = {
-            "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32),
+            "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32), "mg_set_thread_stream": ([vp], i32),
             "mg_synchronize": ([], i32), "mg_mem_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
             "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
             "mg_profile_enable": ([i32], i32), "mg_fft_size": ([i32], i32),
@@ -99,6 +99,11 @@ def check(rc):
 def use_torch_stream():
     import torch
     check(lib().mg_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def use_thread_stream(stream):
+    """The calling thread's library calls go to this torch stream (the other threads keep theirs)."""
+    check(lib().mg_set_thread_stream(C.c_void_p(stream.cuda_stream)))
 
 
 def ptr(t):

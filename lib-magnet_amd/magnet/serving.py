@@ -3,17 +3,18 @@
 The fitted model's training rows, layout and potential field stay resident in HBM. Requests, each an array of
 new rows, enter a queue. A worker thread forms device batches from them as they arrive:
 
-- A request is admitted into the next batch while the batch's point budget allows.
-- The budget is the smaller of max_batch_points and what a fraction of the free HBM holds for the k-NN
-  distance tiles.
-- Requests that arrive while a batch is on the device wait for the next one.
+- Several batches are in flight at once: n_workers worker threads, each with its own HIP stream, take the
+  requests that have arrived whenever they are free, so requests that arrive while a batch is on the device
+  start on another stream at once and their work overlaps with it on the device.
+- A request is admitted into a batch while the batch's point budget allows. The budget is the smaller of
+  max_batch_points and what a fraction of the free HBM, shared by the workers, holds for the k-NN scratch.
 - A batch runs as one k-NN search and one placement launch; the requests in it share those launches.
 
-New points do not interact, and the k-NN search goes through in fixed-size query tiles, so a request's result
-does not depend on which other requests share its batch. Latency is measured from submission to completion,
-per request.
+New points do not interact, every per-request quantity is computed over the request's own rows, and the k-NN
+search goes through in fixed-size query tiles, so a request's result does not depend on which other requests
+share its batch, nor on what runs beside it. Latency is measured from submission to completion, per request.
 """
-import queue
+import collections
 import threading
 import time
 
@@ -46,89 +47,110 @@ class Request:
 
 
 class TransformServer:
-    def __init__(self, model, max_batch_points=16384, hbm_fraction=0.5, n_iter=None):
+    def __init__(self, model, max_batch_points=16384, hbm_fraction=0.5, n_iter=None, n_workers=3):
         import torch
         if not hasattr(model, "embedding_"):
             raise ValueError("the model must be fitted")
+        if n_workers < 1:
+            raise ValueError("n_workers must be >= 1")
         self.model = model
         self.n_iter = n_iter
+        self.n_workers = int(n_workers)
         model._serving_state()   # training rows, layout and field to the device once
         n_train = len(model.embedding_)
         free, _ = torch.cuda.mem_get_info()
         tile_bytes = 256 * n_train * 4 * 3       # one k-NN distance tile and its top-k scratch
         per_point = 64 * 4 + 2 * 8               # neighbours, weights, position
-        self.max_batch_points = int(max(1, min(max_batch_points, (hbm_fraction * free - tile_bytes) // per_point)))
-        self._q = queue.Queue()
+        share = hbm_fraction * free / self.n_workers
+        self.max_batch_points = int(max(1, min(max_batch_points, (share - tile_bytes) // per_point)))
+        self._pending = collections.deque()
+        self._cv = threading.Condition()
         self._ids = 0
-        self._lock = threading.Lock()
-        self._stop = threading.Event()
-        self._worker = threading.Thread(target=self._run, daemon=True)
-        self.batches = []   # (n_requests, n_points, seconds) of every batch served
+        self._stop = False
+        self._workers = [threading.Thread(target=self._run, daemon=True) for _ in range(self.n_workers)]
+        self.batches = []   # (n_requests, n_points, seconds) of every batch served, in completion order
 
     def start(self):
-        self._worker.start()
+        for w in self._workers:
+            w.start()
         return self
 
     def stop(self):
-        self._stop.set()
-        self._q.put(None)
-        self._worker.join()
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        for w in self._workers:
+            w.join()
+        with self._cv:
+            left = list(self._pending)
+            self._pending.clear()
+        for r in left:
+            r.error = RuntimeError("server stopped")
+            r._done.set()
 
     def submit(self, X):
-        with self._lock:
-            rid = self._ids
+        with self._cv:
+            r = Request(np.asarray(X), self._ids)
             self._ids += 1
-        r = Request(np.asarray(X), rid)
-        self._q.put(r)
+            self._pending.append(r)
+            self._cv.notify()
         return r
+
+    def _next_batch(self):
+        """Everything that has arrived, up to the point budget (at least one request); None when stopping."""
+        with self._cv:
+            while not self._pending and not self._stop:
+                self._cv.wait()
+            if self._stop:
+                return None
+            batch, points = [], 0
+            while self._pending and (not batch or points + len(self._pending[0].X) <= self.max_batch_points):
+                r = self._pending.popleft()
+                batch.append(r)
+                points += len(r.X)
+            if self._pending:
+                self._cv.notify()   # what did not fit goes to the next free worker
+            return batch
 
     def _run(self):
         import torch
+        from . import capi
         m = self.model
-        pending = []
-        while not self._stop.is_set():
-            if not pending:
-                item = self._q.get()
-                if item is None:
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            capi.use_thread_stream(stream)
+            while True:
+                batch = self._next_batch()
+                if batch is None:
                     break
-                pending.append(item)
-            while True:   # everything that has arrived so far
-                try:
-                    item = self._q.get_nowait()
-                except queue.Empty:
-                    break
-                if item is None:
-                    self._stop.set()
-                    break
-                pending.append(item)
-            batch, points = [], 0
-            while pending and (not batch or points + len(pending[0].X) <= self.max_batch_points):
-                r = pending.pop(0)
-                batch.append(r)
-                points += len(r.X)
-            t0 = time.perf_counter()
-            try:
-                Q = np.concatenate([m._prepare_query(r.X) for r in batch]) if points else np.zeros((0, m._prepared_.shape[1]), np.float32)
-                bounds = np.cumsum([0] + [len(r.X) for r in batch])
-                segs = list(zip(bounds[:-1].tolist(), bounds[1:].tolist()))
-                Y = m._place(torch.as_tensor(Q, device="cuda"), self.n_iter, segs) if points else None
-                out = Y.cpu().numpy().astype(np.float32) if points else np.zeros((0, 2), np.float32)
-                o = 0
-                t_done = time.perf_counter()
-                for r in batch:
-                    r.result = out[o:o + len(r.X)]
-                    o += len(r.X)
-                    r.t_done = t_done
-                    r.batch = len(self.batches)
-                    r._done.set()
-            except Exception as e:   # noqa: BLE001 -- reported to every request of the batch
-                for r in batch:
-                    r.error = e
-                    r.t_done = time.perf_counter()
-                    r._done.set()
-            self.batches.append((len(batch), points, time.perf_counter() - t0))
-        for r in pending:
-            r.error = RuntimeError("server stopped")
+                self._serve(m, batch)
+
+    def _serve(self, m, batch):
+        import torch
+        t0 = time.perf_counter()
+        points = sum(len(r.X) for r in batch)
+        try:
+            Q = np.concatenate([m._prepare_query(r.X) for r in batch]) if points else np.zeros((0, m._prepared_.shape[1]), np.float32)
+            bounds = np.cumsum([0] + [len(r.X) for r in batch])
+            segs = list(zip(bounds[:-1].tolist(), bounds[1:].tolist()))
+            Y = m._place(torch.as_tensor(Q, device="cuda"), self.n_iter, segs) if points else None
+            out = Y.cpu().numpy().astype(np.float32) if points else np.zeros((0, 2), np.float32)
+            err = None
+        except Exception as e:   # noqa: BLE001 -- reported to every request of the batch
+            out, err = None, e
+        t_done = time.perf_counter()
+        with self._cv:
+            bid = len(self.batches)
+            self.batches.append((len(batch), points, t_done - t0))
+        o = 0
+        for r in batch:
+            r.batch = bid
+            if err is None:
+                r.result = out[o:o + len(r.X)]
+                o += len(r.X)
+            else:
+                r.error = err
+            r.t_done = t_done
             r._done.set()
 
 

@@ -68,7 +68,10 @@ def test_transform_point_does_not_depend_on_its_batch(fitted):
     assert np.allclose(full[5:6], one, rtol=0, atol=1e-5 * np.abs(full).max())
 
 
-def test_serving_same_alone_and_in_the_mix(fitted):
+@pytest.mark.parametrize("n_workers", [1, 3])
+def test_serving_same_alone_and_in_the_mix(fitted, n_workers):
+    """Each request's result is its result alone, whichever requests share its batch and whichever batches run
+    beside it on the other workers' streams."""
     import threading
     from magnet import TransformServer
     m, X, y = fitted
@@ -77,7 +80,7 @@ def test_serving_same_alone_and_in_the_mix(fitted):
     cuts = np.sort(rng.choice(np.arange(1, 300), 11, replace=False))
     parts = np.split(np.arange(300), cuts)
     alone = [m.transform(Q[p]) for p in parts]
-    srv = TransformServer(m, max_batch_points=120).start()
+    srv = TransformServer(m, max_batch_points=120, n_workers=n_workers).start()
     try:
         reqs = [None] * len(parts)
 
@@ -94,7 +97,7 @@ def test_serving_same_alone_and_in_the_mix(fitted):
     scale = np.abs(np.concatenate(alone)).max()
     for a, o in zip(alone, outs):
         assert o.shape == a.shape and np.allclose(a, o, rtol=0, atol=1e-5 * scale)
-    assert all(r.latency is not None and r.latency > 0 for r in reqs)
+    assert all(r.latency is not None and r.latency > 0 and r.batch is not None for r in reqs)
     assert sum(b[1] for b in srv.batches) == 300 and max(b[1] for b in srv.batches) <= max(120, max(map(len, parts)))
 
 
