@@ -58,6 +58,13 @@ struct DevBuf {
         bytes = want;
         return p;
     }
+    // as ensure, but the old allocation goes to `dead` instead of hipFree, which would wait for the device;
+    // the owner frees `dead` once the stream is known to be idle
+    void* ensure_keep(size_t b, std::vector<void*>& dead) {
+        if (b <= bytes && p) return p;
+        if (p) { dead.push_back(p); p = nullptr; bytes = 0; }
+        return ensure(b);
+    }
 };
 template <class U> inline U* dptr(const DevBuf& b) { return reinterpret_cast<U*>(b.p); }
 
@@ -190,6 +197,9 @@ template <class T> struct Repulsion {
     unsigned long tick = 0;
     DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, Tr, phi;
     DevBuf zpart, zout, inv_out, spec_tmp, table;
+    // allocations replaced while the loop runs (grid buffers, spectra, as the layout grows): freed by
+    // release_dead() after a synchronisation, so that no growth waits for the device
+    std::vector<void*> dead;
     int last_N = 0, last_M = 0, sorted_buf = 0;
     GridT<T> last_g{};
     i64 last_n = 0;
@@ -200,6 +210,14 @@ template <class T> struct Repulsion {
         zout.ensure(4 * sizeof(double));
         inv_out.ensure(4 * sizeof(T));
     }
+    Repulsion(const Repulsion&) = delete;
+    Repulsion& operator=(const Repulsion&) = delete;
+    ~Repulsion() { release_dead(); }
+    // only once the work that used them is complete (the caller has synchronised the stream)
+    void release_dead() {
+        for (void* q : dead) (void)hipFree(q);
+        dead.clear();
+    }
 
     // spectra for the device size M (cached; the reference keeps up to 9, magnet/tsne.py:52-54)
     Spec& spectra(int M, hipStream_t st, bool* fresh = nullptr) {
@@ -208,10 +226,19 @@ template <class T> struct Repulsion {
         const double delta = tsne_kernel ? kwidth / ni : -1.0;
         for (auto& s : cache)
             if (s.M == M && s.delta == delta && s.dof == kdof) { s.use = ++tick; return s; }
+        const int Mh = M / 2 + 1;
         Spec* slot = nullptr;
-        if (cache.size() >= 9) {   // the least recently used entry takes the new tables, in its own buffers
-            slot = &cache.front();
-            for (auto& c : cache) if (c.use < slot->use) slot = &c;
+        if (cache.size() >= 9) {
+            // the least recently used entry whose buffers hold the new tables takes them (while the grid clips,
+            // every iteration brings a new width at one M: one entry is then reused without reallocation);
+            // if none is large enough, the least recently used entry, its buffers grown
+            auto fits = [&](const Spec& c) { return c.K[0].bytes >= (size_t)Mh * Mh * sizeof(T); };
+            for (auto& c : cache)
+                if (fits(c) && (!slot || c.use < slot->use)) slot = &c;
+            if (!slot) {
+                slot = &cache.front();
+                for (auto& c : cache) if (c.use < slot->use) slot = &c;
+            }
         } else {
             cache.emplace_back();
             slot = &cache.back();
@@ -219,10 +246,9 @@ template <class T> struct Repulsion {
         Spec& s = *slot;
         s.M = M; s.delta = delta; s.dof = kdof; s.use = ++tick;
         if (fresh) *fresh = true;
-        const int Mh = M / 2 + 1;
         T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
-        for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure((size_t)Mh * Mh * sizeof(T)));
-        cplx<T>* tmp = reinterpret_cast<cplx<T>*>(spec_tmp.ensure((size_t)4 * Mh * Mh * sizeof(cplx<T>)));
+        for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure_keep((size_t)Mh * Mh * sizeof(T), dead));
+        cplx<T>* tmp = reinterpret_cast<cplx<T>*>(spec_tmp.ensure_keep((size_t)4 * Mh * Mh * sizeof(cplx<T>), dead));
         const double* tab = nullptr;
         if (!tsne_kernel) {
             // resample the caller's table (offsets up to M_in / 2) onto the M x M torus
@@ -261,12 +287,12 @@ template <class T> struct Repulsion {
         // boxes then reuses them, instead of a hipFree (which waits for the device) and a hipMalloc per growth
         const int Mh = M / 2 + 1;
         const size_t Nc = (size_t)std::max(N, M / 2);
-        W.ensure(3 * Nc * Nc * sizeof(T));
-        Wh.ensure(3 * Mh * Nc * sizeof(cplx<T>));
-        Ph.ensure(4 * Mh * Nc * sizeof(cplx<T>));
-        Tr.ensure(4 * Mh * Nc * sizeof(cplx<T>));
+        W.ensure_keep(3 * Nc * Nc * sizeof(T), dead);
+        Wh.ensure_keep(3 * Mh * Nc * sizeof(cplx<T>), dead);
+        Ph.ensure_keep(4 * Mh * Nc * sizeof(cplx<T>), dead);
+        Tr.ensure_keep(4 * Mh * Nc * sizeof(cplx<T>), dead);
 
-        phi.ensure(4 * Nc * Nc * sizeof(T));
+        phi.ensure_keep(4 * Nc * Nc * sizeof(T), dead);
         zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
     }
 
@@ -504,6 +530,7 @@ template <class T> struct Optimizer {
             launch_unperm<T>(G, dptr<int>(inv), n, uG, st);
         }
         MG_HIP(hipStreamSynchronize(st));
+        rep.release_dead();
     }
 
     int run(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
@@ -1050,6 +1077,7 @@ int mg_fftconv_compute(mg_fftconv* c, const void* Y, int64_t n, double xmin, dou
         double z[2];
         MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, lib_stream()));
         MG_HIP(hipStreamSynchronize(lib_stream()));
+        r.release_dead();
         if (h_sumQ) *h_sumQ = z[0];
     };
     if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));
