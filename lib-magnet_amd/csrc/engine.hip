@@ -560,7 +560,8 @@ template <class T> struct Optimizer {
 // run one launch per FFT size in use.
 static int bits_for(long long v) { int b = 0; while ((1ll << b) < v) ++b; return b; }
 
-static void grow(DevBuf& b, size_t need) { if (need > b.bytes) b.ensure(need + need / 4); }
+// arena growth with headroom; the old allocation goes to `dead` (freed after the run's last synchronisation)
+static void grow(DevBuf& b, size_t need, std::vector<void*>& dead) { if (need > b.bytes) b.ensure_keep(need + need / 4, dead); }
 
 template <class F> std::vector<BMap> make_maps(const std::vector<int>& which, F blocks) {
     std::vector<BMap> out;
@@ -595,12 +596,16 @@ template <class T> struct BatchEngine {
     int sorted = 0;
     DevBuf Yc, Uc, Gc, Ac, Fc, keysA, valsA, keysB, valsB, hist, table, mbd, sumq, counters, zout, inv, upart;
     DevBuf aW, aWh, aPh, aTr, aphi, aik, aiv, aival, azp, aspec;
+    std::vector<void*> dead;   // replaced allocations, freed by finish() once the stream is idle
     std::vector<std::unique_ptr<DevBuf[]>> K;   // per instance: ntab kernel spectra
     PinnedBuf htab, hmb;
     std::vector<BMap> attr_maps;
     std::vector<int> attr_G;
     hipEvent_t ready = nullptr;
-    ~BatchEngine() { if (ready) (void)hipEventDestroy(ready); }
+    ~BatchEngine() {
+        release_dead();
+        if (ready) (void)hipEventDestroy(ready);
+    }
 
     BInst<T>* host_table() { return reinterpret_cast<BInst<T>*>(htab.p); }
     const BInst<T>* dev_table() const { return reinterpret_cast<const BInst<T>*>(table.p); }
@@ -712,7 +717,7 @@ template <class T> struct BatchEngine {
             const size_t Mh = M[i] / 2 + 1, N = g[i].N, nch = spread_chunks(n[i]);
             if (fresh[i]) {
                 sM[i] = M[i]; sdelta[i] = delta;
-                for (int t = 0; t < ntab; ++t) K[i][t].ensure(Mh * Mh * sizeof(T));
+                for (int t = 0; t < ntab; ++t) K[i][t].ensure_keep(Mh * Mh * sizeof(T), dead);
                 psp[i] = osp; osp += al(4 * Mh * Mh);
             }
             pW[i] = oW; oW += al(3 * N * N);
@@ -724,10 +729,10 @@ template <class T> struct BatchEngine {
             piv[i] = oiv; oiv += al(2 * nch * ni * 3 * ni);
             pzp[i] = ozp; ozp += al(gather_blocks(n[i]));
         }
-        grow(aW, oW * sizeof(T)); grow(aWh, oWh * sizeof(cplx<T>)); grow(aPh, oPh * sizeof(cplx<T>));
-        grow(aTr, oTr * sizeof(cplx<T>)); grow(aphi, ophi * sizeof(T)); grow(aik, oit * sizeof(int));
-        grow(aiv, oit * sizeof(int)); grow(aival, oiv * sizeof(T)); grow(azp, ozp * sizeof(double));
-        if (osp) grow(aspec, osp * sizeof(cplx<T>));
+        grow(aW, oW * sizeof(T), dead); grow(aWh, oWh * sizeof(cplx<T>), dead); grow(aPh, oPh * sizeof(cplx<T>), dead);
+        grow(aTr, oTr * sizeof(cplx<T>), dead); grow(aphi, ophi * sizeof(T), dead); grow(aik, oit * sizeof(int), dead);
+        grow(aiv, oit * sizeof(int), dead); grow(aival, oiv * sizeof(T), dead); grow(azp, ozp * sizeof(double), dead);
+        if (osp) grow(aspec, osp * sizeof(cplx<T>), dead);
         const bool early = it < p->early_iter;
         const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
         const T mom = (T)(early ? p->momentum0 : p->momentum1);
@@ -812,6 +817,11 @@ template <class T> struct BatchEngine {
                                       (size_t)p->n_iter * sizeof(double), hipMemcpyDeviceToHost, st));
         }
         MG_HIP(hipStreamSynchronize(st));
+        release_dead();
+    }
+    void release_dead() {
+        for (void* q : dead) (void)hipFree(q);
+        dead.clear();
     }
 };
 
