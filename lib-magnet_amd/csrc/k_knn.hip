@@ -102,7 +102,10 @@ template <int DP, int KM>
 __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ T, i64 nt, const float* __restrict__ Q, i64 m,
                                                       int d, i64 chunk, i64 self_offset, float* __restrict__ pd,
                                                       int* __restrict__ pi, int S) {
-    __shared__ float sT[KNN_TR][DP + 1];
+    // rows of DP floats (a multiple of 8), 16-byte aligned: every lane of a wave reads the same training
+    // element (a broadcast), four features per LDS read
+    static_assert(DP % 8 == 0, "DP must be a multiple of 8");
+    __shared__ __align__(16) float sT[KNN_TR][DP];
     const i64 qi = (i64)blockIdx.x * KNN_Q + threadIdx.x;
     const int s = blockIdx.y;
     float q[DP];
@@ -124,8 +127,15 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
         const int rows = (int)(c1 - t0 < KNN_TR ? c1 - t0 : KNN_TR);
         for (int r = 0; r < rows; ++r) {
             float acc = 0.f;
+            const float4* tr = reinterpret_cast<const float4*>(&sT[r][0]);
 #pragma unroll
-            for (int j = 0; j < DP; ++j) { const float df = q[j] - sT[r][j]; acc += df * df; }
+            for (int j = 0; j < DP / 4; ++j) {   // features in order; the zero padding adds exact zeros
+                const float4 t4 = tr[j];
+                float df = q[4 * j] - t4.x; acc += df * df;
+                df = q[4 * j + 1] - t4.y; acc += df * df;
+                df = q[4 * j + 2] - t4.z; acc += df * df;
+                df = q[4 * j + 3] - t4.w; acc += df * df;
+            }
             int ix = (int)(t0 + r);
             if (ix == self) continue;
             float x = acc;
@@ -212,8 +222,13 @@ void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64
     const i64 chunk = (nt + S - 1) / S;
     const dim3 g1((unsigned)((m + KNN_Q - 1) / KNN_Q), (unsigned)S);
 #define MG_K(DP, KM) hipLaunchKernelGGL((k_knn_chunks<DP, KM>), g1, dim3(KNN_Q), 0, st, T, nt, Q, m, d, chunk, self_offset, part_d, part_i, S)
-    if (k <= 16) { if (d <= 16) MG_K(16, 16); else if (d <= 32) MG_K(32, 16); else MG_K(64, 16); }
-    else { if (d <= 16) MG_K(16, 32); else if (d <= 32) MG_K(32, 32); else MG_K(64, 32); }
+    // the query row in registers, padded to the next multiple of 8 features
+#define MG_KD(KM) switch ((d + 7) / 8) { \
+        case 1: MG_K(8, KM); break; case 2: MG_K(16, KM); break; case 3: MG_K(24, KM); break; \
+        case 4: MG_K(32, KM); break; case 5: MG_K(40, KM); break; case 6: MG_K(48, KM); break; \
+        case 7: MG_K(56, KM); break; default: MG_K(64, KM); break; }
+    if (k <= 16) { MG_KD(16) } else { MG_KD(32) }
+#undef MG_KD
 #undef MG_K
     MG_HIP(hipGetLastError());
     const int KM = k <= 16 ? 16 : 32;
