@@ -114,7 +114,7 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
     float bd[KM];
     int bi[KM];
 #pragma unroll
-    for (int j = 0; j < KM; ++j) { bd[j] = FLT_MAX; bi[j] = INT_MAX; }
+    for (int j = 0; j < KM; ++j) { bd[j] = INFINITY; bi[j] = INT_MAX; }   // every finite distance goes before
     const i64 c0 = (i64)s * chunk, c1 = c0 + chunk < nt ? c0 + chunk : nt;
     const i64 self = (self_offset >= 0 && qi < m) ? qi + self_offset : -1;
     for (i64 t0 = c0; t0 < c1; t0 += KNN_TR) {
@@ -139,10 +139,12 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
             int ix = (int)(t0 + r);
             if (ix == self) continue;
             float x = acc;
-            if (!before(x, ix, bd[KM - 1], bi[KM - 1])) continue;
+            // a thread visits its rows in increasing index order, so an entry of equal distance already in the
+            // list has the smaller index: the (distance, index) order reduces to distance < entry
+            if (!(x < bd[KM - 1])) continue;
 #pragma unroll
             for (int j = 0; j < KM; ++j) {
-                const bool sw = before(x, ix, bd[j], bi[j]);
+                const bool sw = x < bd[j];
                 const float td = bd[j]; const int ti = bi[j];
                 bd[j] = sw ? x : td; bi[j] = sw ? ix : ti;
                 x = sw ? td : x; ix = sw ? ti : ix;
