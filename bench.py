@@ -45,7 +45,7 @@ WORKLOADS = {
 BATCHED = dict(count=64, n_min=2000, n_max=50000, d=32, n_iter=500, early=166,
                desc="64 independent embeddings, n_i log-uniform in [2000, 50000] (seed 1000 + i), 32-d Gaussian mixtures "
                     "of 5-20 clusters, perplexity 30, 500 iterations (166 early), FP32")
-SERVING = dict(n_train=200_000, d=50, clusters=50, pool=60_000, qps=200.0, duration=8.0, n_iter=100,
+SERVING = dict(n_train=200_000, d=50, clusters=50, pool=60_000, qps=200.0, duration=16.0, n_iter=100,
                desc="MAGNET(loss='tsne') on 200,000 x 50 (Gaussian mixture, 50 clusters), resident; transform() requests "
                     "of 1-4096 new rows (log-uniform sizes, same mixture) arriving by a Poisson process; 100 iterations each")
 
@@ -162,7 +162,9 @@ def serving_bench(qps=None, duration=None, seed=0, n_workers=3):
         size = int(round(np.exp(r.uniform(0, np.log(4096)))))
         return pool[r.integers(0, len(pool), size)]
     try:
-        srv.submit(pool[:64]).wait(120)   # warm-up: kernels, field, k-NN tiles
+        # warm-up of every worker (its stream's memory pool, its k-NN scratch), with small and large requests
+        for r in [srv.submit(pool[:sz]) for sz in (4096, 64, 1) * n_workers]:
+            r.wait(120)
         t1 = time.perf_counter()
         reqs = poisson_load(srv, sampler, qps, duration, seed + 1)
         wall = time.perf_counter() - t1

@@ -1114,7 +1114,10 @@ int mg_knn(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, 
     static thread_local DevBuf pd, pi;   // scratch per host thread: threads on their own streams run concurrently
     require(m >= 0 && nt >= 1 && (m == 0 || (T && Q && idx && dist2)), "bad k-NN arguments");
     if (m > 0) {
-        const size_t sz = (size_t)m * knn_chunks(m, nt) * 32;
+        // m * chunks stays below 2^19 lists of 32 while there is more than one chunk (about 1024 blocks of 256
+        // queries): the first call takes room for all of them, so that no later call grows the scratch (a
+        // hipFree waits for the whole device, and so for what other threads run beside this one)
+        const size_t sz = std::max((size_t)m * knn_chunks(m, nt) * 32, (size_t)1 << 24);
         launch_knn(T, nt, Q, m, d, k, self_offset, reinterpret_cast<float*>(pd.ensure(sz * sizeof(float))),
                    reinterpret_cast<int*>(pi.ensure(sz * sizeof(int))), idx, dist2, lib_stream());
         MG_HIP(hipStreamSynchronize(lib_stream()));
