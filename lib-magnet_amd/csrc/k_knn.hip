@@ -96,7 +96,7 @@ __global__ __launch_bounds__(TPB) void k_topk_rows(const float* __restrict__ D, 
 // Pass 1: block (query tile of 256, training chunk s): one query per thread, the chunk's training rows streamed
 // through LDS in tiles of 64, each thread keeping the KM nearest of the chunk. Pass 2: per query, the S chunk
 // lists merged through LDS.
-constexpr int KNN_Q = 256, KNN_TR = 64;
+constexpr int KNN_Q = 256, KNN_TR = 64, KNN_SMAX = 1024, KNN_MERGE = 256;
 
 template <int DP, int KM>
 __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ T, i64 nt, const float* __restrict__ Q, i64 m,
@@ -158,31 +158,41 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
     for (int j = 0; j < KM; ++j) { od[j] = bd[j]; oi[j] = bi[j]; }
 }
 
-// one block of S (a power of two) threads per query: list t is chunk t's
+// one block of P = min(S, 256) threads per query (S and P powers of two): thread t first merges the chunk lists
+// t, t + P, t + 2P, ... in turn, then the P lists are merged as a tree through LDS
+template <int KM>
+__device__ __forceinline__ void knn_merge2(const float* ad, const int* ai, const float* cd, const int* ci, float (&md)[KM],
+                                           int (&mi)[KM]) {
+    int p = 0, r = 0;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+        const bool fa = before(ad[p], ai[p], cd[r], ci[r]);
+        md[j] = fa ? ad[p] : cd[r]; mi[j] = fa ? ai[p] : ci[r];
+        p += fa ? 1 : 0; r += fa ? 0 : 1;
+    }
+}
 template <int KM>
 __global__ void k_knn_merge(const float* __restrict__ pd, const int* __restrict__ pi, int S, int k, int32_t* __restrict__ oi,
                             float* __restrict__ ov) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const int P = (int)blockDim.x;
     float* sd = reinterpret_cast<float*>(smem);
-    int* si = reinterpret_cast<int*>(sd + S * KM);
+    int* si = reinterpret_cast<int*>(sd + P * KM);
     const i64 qi = blockIdx.x;
     const int t = threadIdx.x;
     for (int j = 0; j < KM; ++j) { sd[t * KM + j] = pd[(qi * S + t) * KM + j]; si[t * KM + j] = pi[(qi * S + t) * KM + j]; }
-    __syncthreads();
-    for (int s = S / 2; s >= 1; s >>= 1) {
+    for (int l = t + P; l < S; l += P) {   // only this thread's LDS slot: no barrier needed
         float md[KM];
         int mi[KM];
-        if (t < s) {
-            const float* ad = sd + t * KM; const int* ai = si + t * KM;
-            const float* cd = sd + (t + s) * KM; const int* ci = si + (t + s) * KM;
-            int p = 0, r = 0;
+        knn_merge2<KM>(sd + t * KM, si + t * KM, pd + (qi * S + l) * KM, pi + (qi * S + l) * KM, md, mi);
 #pragma unroll
-            for (int j = 0; j < KM; ++j) {
-                const bool fa = before(ad[p], ai[p], cd[r], ci[r]);
-                md[j] = fa ? ad[p] : cd[r]; mi[j] = fa ? ai[p] : ci[r];
-                p += fa ? 1 : 0; r += fa ? 0 : 1;
-            }
-        }
+        for (int j = 0; j < KM; ++j) { sd[t * KM + j] = md[j]; si[t * KM + j] = mi[j]; }
+    }
+    __syncthreads();
+    for (int s = P / 2; s >= 1; s >>= 1) {
+        float md[KM];
+        int mi[KM];
+        if (t < s) knn_merge2<KM>(sd + t * KM, si + t * KM, sd + (t + s) * KM, si + (t + s) * KM, md, mi);
         __syncthreads();
         if (t < s) {
 #pragma unroll
@@ -190,7 +200,7 @@ __global__ void k_knn_merge(const float* __restrict__ pd, const int* __restrict_
         }
         __syncthreads();
     }
-    for (int j = t; j < k; j += S) { oi[qi * k + j] = si[j]; ov[qi * k + j] = sd[j]; }   // S may be below k
+    for (int j = t; j < k; j += P) { oi[qi * k + j] = si[j]; ov[qi * k + j] = sd[j]; }   // P may be below k
 }
 }  // namespace
 
@@ -212,7 +222,7 @@ namespace mg {
 int knn_chunks(i64 m, i64 nt) {
     const i64 tiles = (m + KNN_Q - 1) / KNN_Q;
     int S = 1;
-    while (S < 256 && tiles * S < 1024 && (i64)S * 2 * KNN_TR <= nt) S *= 2;   // about 1024 blocks, >= 64 rows per chunk
+    while (S < KNN_SMAX && tiles * S < 1024 && (i64)S * 2 * KNN_TR <= nt) S *= 2;   // about 1024 blocks, >= 64 rows per chunk
     return S;
 }
 void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
@@ -234,9 +244,10 @@ void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64
 #undef MG_K
     MG_HIP(hipGetLastError());
     const int KM = k <= 16 ? 16 : 32;
-    const size_t lds = (size_t)S * KM * (sizeof(float) + sizeof(int));
-    if (KM == 16) hipLaunchKernelGGL(k_knn_merge<16>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
-    else hipLaunchKernelGGL(k_knn_merge<32>, dim3((unsigned)m), dim3(S), lds, st, part_d, part_i, S, k, oi, ov);
+    const int P = S < KNN_MERGE ? S : KNN_MERGE;
+    const size_t lds = (size_t)P * KM * (sizeof(float) + sizeof(int));   // at most 64 KiB
+    if (KM == 16) hipLaunchKernelGGL(k_knn_merge<16>, dim3((unsigned)m), dim3(P), lds, st, part_d, part_i, S, k, oi, ov);
+    else hipLaunchKernelGGL(k_knn_merge<32>, dim3((unsigned)m), dim3(P), lds, st, part_d, part_i, S, k, oi, ov);
     MG_HIP(hipGetLastError());
 }
 }  // namespace mg

@@ -119,12 +119,14 @@ def test_topk_rows_matches_a_full_sort(k, cols):
 
 
 @pytest.mark.parametrize("d,k,exclude,m,nt", [(50, 14, False, 700, 3000), (7, 5, True, 700, 3000), (64, 29, False, 700, 3000),
-                                               (1, 3, True, 700, 3000), (12, 14, True, 500, 500)])
+                                               (1, 3, True, 700, 3000), (12, 14, True, 500, 500),
+                                               (50, 14, False, 5, 200_000), (7, 29, True, 3, 150_000)])
 def test_fused_knn_is_exact(d, k, exclude, m, nt):
     """mg_knn against a float64 brute force: distances to 1e-5 relative, and the same neighbour set wherever the
     k-th and (k+1)-th distances are apart; exclude_self leaves each row out of its own list; a query's result does
     not depend on the other queries of the call. 500 training rows split into fewer chunks than k (the merge
-    then writes k results with fewer threads)."""
+    then writes k results with fewer threads); a few queries against 150k-200k rows split into 1024 chunks (more
+    lists than merge threads: each thread merges four in turn before the tree)."""
     import torch
     from magnet import capi
     rng = np.random.default_rng(d * 100 + k)
@@ -144,9 +146,10 @@ def test_fused_knn_is_exact(d, k, exclude, m, nt):
         assert set(idx[q]) == set(order[q, :k]), q
     if exclude:
         assert not np.any(idx == np.arange(m)[:, None])
-    one_i, one_d = capi.knn(Tt, Qt[123:124].contiguous(), k, False)
+    q1 = min(123, m - 1)
+    one_i, one_d = capi.knn(Tt, Qt[q1:q1 + 1].contiguous(), k, False)
     if not exclude:
-        assert np.array_equal(one_i.cpu().numpy()[0], idx[123]) and np.array_equal(one_d.cpu().numpy()[0], d2[123])
+        assert np.array_equal(one_i.cpu().numpy()[0], idx[q1]) and np.array_equal(one_d.cpu().numpy()[0], d2[q1])
 
 
 def test_device_smooth_knn_matches_the_numpy_restatement():
