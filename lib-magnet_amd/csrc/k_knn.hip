@@ -136,19 +136,24 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
                 df = q[4 * j + 2] - t4.z; acc += df * df;
                 df = q[4 * j + 3] - t4.w; acc += df * df;
             }
-            int ix = (int)(t0 + r);
+            const int ix = (int)(t0 + r);
             if (ix == self) continue;
-            float x = acc;
-            // a thread visits its rows in increasing index order, so an entry of equal distance already in the
-            // list has the smaller index: the (distance, index) order reduces to distance < entry
+            const float x = acc;
+            // A thread visits its rows in increasing index order, so an entry of equal distance already in the
+            // list has the smaller index and stays before the new one: the new row goes before entry j iff
+            // x < bd[j]. Over the sorted list that test is false up to the insertion point and true after it,
+            // so each slot takes the new row (first true), its left neighbour (later trues) or keeps its entry.
+            // The entries that move keep their order, ties included.
             if (!(x < bd[KM - 1])) continue;
 #pragma unroll
-            for (int j = 0; j < KM; ++j) {
-                const bool sw = x < bd[j];
-                const float td = bd[j]; const int ti = bi[j];
-                bd[j] = sw ? x : td; bi[j] = sw ? ix : ti;
-                x = sw ? td : x; ix = sw ? ti : ix;
+            for (int j = KM - 1; j >= 1; --j) {
+                const bool cj = x < bd[j], cp = x < bd[j - 1];
+                const float nd = cp ? bd[j - 1] : x;
+                const int ni = cp ? bi[j - 1] : ix;
+                bd[j] = cj ? nd : bd[j];
+                bi[j] = cj ? ni : bi[j];
             }
+            if (x < bd[0]) { bd[0] = x; bi[0] = ix; }
         }
     }
     if (qi >= m) return;

@@ -152,6 +152,26 @@ def test_fused_knn_is_exact(d, k, exclude, m, nt):
         assert np.array_equal(one_i.cpu().numpy()[0], idx[q1]) and np.array_equal(one_d.cpu().numpy()[0], d2[q1])
 
 
+@pytest.mark.parametrize("d,k,exclude,m,nt", [(12, 14, False, 300, 3000), (5, 29, True, 40, 2000), (12, 29, False, 3, 150_000),
+                                               (3, 7, True, 4, 100_000)])
+def test_knn_ties_break_by_index(d, k, exclude, m, nt):
+    """Small integer features make every squared distance an exact integer, with many exact ties: mg_knn must
+    give the stable sort's (distance, index) order exactly, across chunk boundaries and through the merge
+    (3000 rows: 16 chunks; 100k-150k rows: 1024 chunks, four lists per merge thread)."""
+    import torch
+    from magnet import capi
+    rng = np.random.default_rng(nt + d)
+    T = rng.integers(0, 4, (nt, d)).astype(np.float32)
+    Q = T[:m].copy() if exclude else rng.integers(0, 4, (m, d)).astype(np.float32)
+    idx, d2 = capi.knn(torch.as_tensor(T, device="cuda"), torch.as_tensor(Q, device="cuda"), k, exclude)
+    D = ((Q.astype(np.int64)[:, None, :] - T.astype(np.int64)[None, :, :]) ** 2).sum(-1).astype(np.float64)
+    if exclude:
+        D[np.arange(m), np.arange(m)] = np.inf
+    order = np.argsort(D, axis=1, kind="stable")[:, :k]
+    assert np.array_equal(idx.cpu().numpy(), order.astype(np.int32))
+    assert np.array_equal(d2.cpu().numpy().astype(np.float64), np.take_along_axis(D, order, 1))
+
+
 def test_device_smooth_knn_matches_the_numpy_restatement():
     """mg_smooth_knn against magnet.graph.smooth_knn (the reference's numpy algorithm) on sorted distances,
     with leading zeros and an all-zero row (the fallback to the call's mean distance)."""
