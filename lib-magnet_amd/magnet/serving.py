@@ -90,6 +90,8 @@ class TransformServer:
 
     def submit(self, X):
         with self._cv:
+            if self._stop:
+                raise RuntimeError("server stopped")
             r = Request(np.asarray(X), self._ids)
             self._ids += 1
             self._pending.append(r)
