@@ -59,10 +59,12 @@ class TransformServer:
         model._serving_state()   # training rows, layout and field to the device once
         n_train = len(model.embedding_)
         free, _ = torch.cuda.mem_get_info()
-        tile_bytes = 256 * n_train * 4 * 3       # one k-NN distance tile and its top-k scratch
+        # per worker: the library's k-NN lists (2 x 2^24 entries of 4 bytes, taken on the thread's first call), or
+        # for rows of more than 64 features one GEMM distance tile of 256 rows and its top-k scratch
+        knn_bytes = max(2 * (1 << 24) * 4, 256 * n_train * 4 * 3)
         per_point = 64 * 4 + 2 * 8               # neighbours, weights, position
         share = hbm_fraction * free / self.n_workers
-        self.max_batch_points = int(max(1, min(max_batch_points, (share - tile_bytes) // per_point)))
+        self.max_batch_points = int(max(1, min(max_batch_points, (share - knn_bytes) // per_point)))
         self._pending = collections.deque()
         self._cv = threading.Condition()
         self._ids = 0
