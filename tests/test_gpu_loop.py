@@ -218,3 +218,71 @@ def test_relabelling_changes_no_result(dtype):
         assert np.array_equal(Y, ref[0][0]) and np.array_equal(U, ref[0][1]) and np.array_equal(G, ref[0][2]), every
         assert np.array_equal(sq, ref[1]), every
         assert sorted(seen) == sorted(ref[2]) and all(np.array_equal(seen[k], ref[2][k]) for k in seen), every
+
+
+SCHED_OPTS = [dict(dof=0.6), dict(dof=2.0), dict(n_interp=2), dict(n_interp=5),
+              dict(exaggeration=4.0, learning_rate=150.0, momentum=(0.3, 0.9)),
+              dict(box_width=1.1, min_boxes=8, max_boxes=12)]
+
+
+@pytest.mark.parametrize("opts", SCHED_OPTS, ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
+def test_fp64_loop_options_match_oracle(opts):
+    """The whole FP64 loop with a non-default kernel exponent, interpolation order, schedule or grid limits
+    (the last clips n_boxes at 12 with a 1.1 box) against the oracle over 20 iterations from a spread layout:
+    every iteration's sumQ to 1e-10 and the final layout to 1e-9."""
+    P, _, _ = problem(2000, 10, 5, seed=3)
+    Y0 = np.random.default_rng(4).normal(0, 3.0, (2000, 2))
+    data = P.data.astype(np.float64)
+    st = State(P, Y0, np.float64)
+    kw = dict(n_iter=40, early_iter=10, **opts)
+    sq = st.run(0, 20, **kw)
+    Yo, Uo, Go = Y0.copy(), np.zeros_like(Y0), np.ones_like(Y0)
+    sqo = O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, 0, 20, **kw)
+    assert np.abs(sq / sqo - 1).max() < 1e-10
+    assert rel(host(st.Y), Yo) < 1e-9
+
+
+@pytest.mark.parametrize("n", [2, 3, 7])
+def test_fp64_loop_tiny_layouts_match_oracle(n):
+    """A handful of points (a complete graph P, default learning rate 200): the first iterations fling them from a
+    span of 0.01 to about 100, with sumQ = sum pot0 - n of order 0.1 for n of order 1, so whole trajectories part
+    by rounding alone; each iteration from the same state matches the oracle to 1e-10."""
+    rng = np.random.default_rng(n)
+    A = sp.csr_matrix(np.ones((n, n)) - np.eye(n))
+    A = (A / A.sum()).tocsr()
+    A.sort_indices()
+    data = A.data.astype(np.float64)
+    st = State(A, rng.normal(0, 1e-2, (n, 2)), np.float64)
+    kw = dict(n_iter=30, early_iter=10)
+    it = 0
+    for t in (0, 1, 4):
+        if t > it:
+            st.run(it, t, **kw)
+        Y, U, G = st.snapshot()
+        sq = st.run(t, t + 1, **kw)
+        it = t + 1
+        Yo, Uo, Go = Y.copy(), U.copy(), G.copy()
+        sqo = O.optimize(A.indptr, A.indices, data, Yo, Uo, Go, t, t + 1, **kw)
+        Yg, Ug, _ = st.snapshot()
+        assert abs(sq[0] / sqo[0] - 1) < 1e-10, (t, sq[0], sqo[0])
+        assert max(rel(Yg - Y, Yo - Y), rel(Ug, Uo)) < 1e-10, t
+
+
+def test_fp64_loop_with_isolated_points_matches_oracle():
+    """CSR rows without entries (points with no affinity): they feel only the repulsion."""
+    P, Y0, _ = problem(2000, 10, 5, seed=3)
+    P = P.tolil()
+    for i in range(0, 2000, 97):
+        P.rows[i] = []
+        P.data[i] = []
+    P = P.tocsr()
+    P = P.multiply(1.0 / P.sum()).tocsr()
+    P.sort_indices()
+    assert np.any(np.diff(P.indptr) == 0)
+    data = P.data.astype(np.float64)
+    st = State(P, Y0, np.float64)
+    sq = st.run(0, 25, **SCHED)
+    Yo, Uo, Go = Y0.copy(), np.zeros_like(Y0), np.ones_like(Y0)
+    sqo = O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, 0, 25, **SCHED)
+    assert np.abs(sq / sqo - 1).max() < 1e-10
+    assert rel(host(st.Y), Yo) < 1e-9
