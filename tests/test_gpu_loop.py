@@ -128,16 +128,16 @@ def test_fp64_nonfused_path_with_mid_pairs():
     assert rel(host(st.Y) - Y, Yo - Y) < 1e-10
 
 
-def fp32_vs_fp64_gradient(st, P, Y, rho):
+def fp32_vs_fp64_gradient(st, P, Y, rho, dof=1.0, ni=3):
     """relative L2 of the FP32 path's gradient against the FP64 path's on the same (float32) layout"""
     Y = Y.astype(np.float32)
     grads = {}
     for dt in (np.float64, np.float32):
         Yd = dev(Y.astype(dt))
-        A = host(capi.attraction(st.ip, st.ix, dev(P.data.astype(dt)), Yd, 1.0))
+        A = host(capi.attraction(st.ip, st.ix, dev(P.data.astype(dt)), Yd, dof))
         xmin, ymin, width, nb = O.grid(Y)
-        conv = capi.FFTConv(capi.MG_F64 if dt == np.float64 else capi.MG_F32, nb * 6, 3)
-        conv.set_tsne(width)
+        conv = capi.FFTConv(capi.MG_F64 if dt == np.float64 else capi.MG_F32, nb * max(6, ni), ni)
+        conv.set_tsne(width, dof)
         s, F = conv.compute(Yd, xmin, ymin, width, nb)
         grads[dt] = rho * A.astype(np.float64) - host(F).astype(np.float64) / s
     return rel(grads[np.float32], grads[np.float64])
@@ -154,6 +154,19 @@ def test_fp32_gradient_against_fp64():
         st.run(it, t, **SCHED); it = t
         e = fp32_vs_fp64_gradient(st, P, host(st.Y), 12.0 if t < 166 else 1.0)
         print(f"iteration {t}: FP32 gradient relative L2 error {e:.2e}")
+        assert e < 1e-4, (t, e)
+
+
+@pytest.mark.parametrize("dof,ni", [(0.6, 3), (2.0, 3), (1.0, 5)])
+def test_fp32_gradient_against_fp64_options(dof, ni):
+    """The same bound with a heavier and a lighter tailed kernel and with five interpolation nodes per box."""
+    P, Y0, _ = problem(2000, 10, 5, seed=3)
+    st = State(P, Y0, np.float64)
+    kw = dict(n_iter=300, early_iter=100, dof=dof, n_interp=ni)
+    it = 0
+    for t in (60, 150, 299):
+        st.run(it, t, **kw); it = t
+        e = fp32_vs_fp64_gradient(st, P, host(st.Y), 12.0 if t < 100 else 1.0, dof, ni)
         assert e < 1e-4, (t, e)
 
 
