@@ -66,6 +66,24 @@ def test_batch_equals_single_runs(precision, n_streams, n_threads):
         assert np.array_equal(sq[b], s1), b
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_equals_single_runs_with_options(precision):
+    """The same equality with dof 0.6, five interpolation nodes, a 1.1 box and n_boxes clipped at 16 from spread
+    starting layouts (every instance's spectra recomputed each iteration), in two groups."""
+    from magnet.batch import device_instances
+    probs = cached()
+    Ps, Ys = [p for p, _ in probs], [y * 5e4 for _, y in probs]
+    opts = dict(n_iter=40, early_iter=15, dof=0.6, n_interp=5, box_width=1.1, min_boxes=8, max_boxes=16)
+    batch = device_instances(Ps, Ys, precision)
+    sq = capi.optimize_batch(batch, n_streams=2, n_threads=1, sumq=True, **opts)
+    for b, single in enumerate(device_instances(Ps, Ys, precision)):
+        ip, ix, dv, Y, U, G = single
+        s1 = capi.optimize(ip, ix, dv, Y, U, G, 0, opts["n_iter"], **opts)
+        for x, y in zip(batch[b][3:], (Y, U, G)):
+            assert np.array_equal(host(x), host(y)), (b, SIZES[b])
+        assert np.array_equal(sq[b], s1), b
+
+
 def test_optimize_tsne_batch_matches_optimize_tsne():
     from magnet.batch import optimize_tsne_batch
     from magnet.tsne import optimize_tsne
