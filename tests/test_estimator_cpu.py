@@ -55,3 +55,43 @@ def test_prepare_data_cosine_rows():
     X = np.array([[3.0, 4.0], [0.0, 0.0]], np.float32)
     Q = prepare_data(X, "cosine")
     assert np.allclose(Q, [[0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _bare_server(budget):
+    """A TransformServer without its device state or worker threads: the admission logic alone."""
+    import collections
+    import threading
+    from magnet.serving import TransformServer
+    srv = TransformServer.__new__(TransformServer)
+    srv._pending = collections.deque()
+    srv._cv = threading.Condition()
+    srv._ids = 0
+    srv._stop = False
+    srv._workers = []
+    srv.max_batch_points = budget
+    srv.batches = []
+    return srv
+
+
+def test_server_batches_in_arrival_order_within_the_budget():
+    """A free worker takes the requests that have arrived, in order, while their rows fit the point budget; a
+    request larger than the budget still forms a batch of its own; nothing is reordered or dropped."""
+    srv = _bare_server(100)
+    sizes = [30, 50, 40, 250, 10, 90, 5]
+    reqs = [srv.submit(np.zeros((s, 3), np.float32)) for s in sizes]
+    batches = []
+    while srv._pending:
+        batches.append([r.id for r in srv._next_batch()])
+    assert batches == [[0, 1], [2], [3], [4, 5], [6]]
+    assert [r.id for r in reqs] == list(range(len(sizes)))
+
+
+def test_server_stop_fails_pending_requests_and_refuses_new_ones():
+    srv = _bare_server(100)
+    r = srv.submit(np.zeros((4, 3), np.float32))
+    srv.stop()
+    with pytest.raises(RuntimeError):
+        r.wait(1)
+    with pytest.raises(RuntimeError):
+        srv.submit(np.zeros((4, 3), np.float32))
+    assert srv._next_batch() is None
