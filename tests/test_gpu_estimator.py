@@ -58,6 +58,18 @@ def test_transform_after_fit(fitted):
     assert (KNeighborsClassifier(10).fit(m.embedding_, y[:1700]).predict(Yq) == y[1700:]).mean() > 0.95
 
 
+def test_transform_of_wide_rows():
+    """Rows of 100 features (past the fused k-NN's 64): the search goes through GEMM distance tiles and
+    mg_topk_rows; the reference's transform test still holds."""
+    from magnet import MAGNET
+    from sklearn.neighbors import KNeighborsClassifier
+    X, y = blobs(2000, d=100, centers=5, seed=6)
+    m = MAGNET(loss="tsne", random_state=0).fit(X[:1700])
+    Yq = m.transform(X[1700:])
+    assert Yq.shape == (300, 2) and np.isfinite(Yq).all()
+    assert (KNeighborsClassifier(10).fit(m.embedding_, y[:1700]).predict(Yq) == y[1700:]).mean() > 0.95
+
+
 def test_transform_point_does_not_depend_on_its_batch(fitted):
     m, X, y = fitted
     Q = X[1700:]
