@@ -200,6 +200,13 @@ template <class T> struct Repulsion {
     // allocations replaced while the loop runs (grid buffers, spectra, as the layout grows): freed by
     // release_dead() after a synchronisation, so that no growth waits for the device
     std::vector<void*> dead;
+    // overlap (set by the loop): spectra that must be recomputed run on a second stream (default priority), beside
+    // the box sort, the spread and the row pass, which leave most of the device idle at the loop's sizes. The side
+    // stream starts after everything issued before the call (ev_go: no table in use is overwritten) and the column
+    // pass waits for it (ev_spec).
+    bool overlap = false;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_go = nullptr, ev_spec = nullptr;
     int last_N = 0, last_M = 0, sorted_buf = 0;
     GridT<T> last_g{};
     i64 last_n = 0;
@@ -212,11 +219,24 @@ template <class T> struct Repulsion {
     }
     Repulsion(const Repulsion&) = delete;
     Repulsion& operator=(const Repulsion&) = delete;
-    ~Repulsion() { release_dead(); }
+    ~Repulsion() {
+        release_dead();
+        if (side) (void)hipStreamSynchronize(side);
+        if (ev_go) (void)hipEventDestroy(ev_go);
+        if (ev_spec) (void)hipEventDestroy(ev_spec);
+        if (side) (void)hipStreamDestroy(side);
+    }
     // only once the work that used them is complete (the caller has synchronised the stream)
     void release_dead() {
         for (void* q : dead) (void)hipFree(q);
         dead.clear();
+    }
+
+    bool have_spectra(int M) const {
+        const double delta = tsne_kernel ? kwidth / ni : -1.0;
+        for (const auto& s : cache)
+            if (s.M == M && s.delta == delta && s.dof == kdof) return true;
+        return false;
     }
 
     // spectra for the device size M (cached; the reference keeps up to 9, magnet/tsne.py:52-54)
@@ -321,10 +341,23 @@ template <class T> struct Repulsion {
         ensure(n, g.N, M);
         Prof& P = g_prof;
         const double ts = sizeof(T), NN = (double)g.N * g.N, cs = 2 * ts, Mh = M / 2 + 1, dn = (double)n;
-        P.begin(ST_SPECTRA, st);
+        const bool side_run = overlap && tsne_kernel && !have_spectra(M);
+        hipStream_t sst = st;
+        if (side_run) {
+            if (!side) {
+                MG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+                MG_HIP(hipEventCreateWithFlags(&ev_go, hipEventDisableTiming));
+                MG_HIP(hipEventCreateWithFlags(&ev_spec, hipEventDisableTiming));
+            }
+            MG_HIP(hipEventRecord(ev_go, st));
+            MG_HIP(hipStreamWaitEvent(side, ev_go, 0));
+            sst = side;
+        }
+        P.begin(ST_SPECTRA, sst);
         bool fresh = false;
-        Spec& s = spectra(M, st, &fresh);
-        P.end(ST_SPECTRA, st, fresh ? ntab * Mh * Mh * ts : 0.0);   // tables written when the grid changed
+        Spec& s = spectra(M, sst, &fresh);
+        P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0);   // tables written when the grid changed
+        if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
         if (inv) {
@@ -347,6 +380,7 @@ template <class T> struct Repulsion {
         P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
         launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
         P.end(ST_TR_FWD, st, 0);
+        if (side_run) MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, st);
@@ -446,6 +480,7 @@ template <class T> struct Optimizer {
         require(!pairs || m >= 1, "pairs must be (n, m) with m >= 1");
         validate_csr(ip, ix, n, nnz, st);
         rep.have_kernel = true; rep.tsne_kernel = true; rep.kdof = p->dof; rep.kwidth = p->box_width;
+        rep.overlap = true;
         lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
         fused = !(pairs && m > 0 && p->mid_weight > 0);
         attr.ensure((size_t)2 * n * sizeof(T));
