@@ -431,10 +431,17 @@ template <class T> struct Optimizer {
     const T* cP = nullptr;
     DevBuf Yw, Uw, Gw, Yt, Ut, Gt, perm, inv, order, inv_new, lenb, tiles, ipn, ixn, Pn, Ycb;
 
+    // the next iteration's attraction, on its own stream (default priority): it needs the updated layout but no
+    // grid, so it runs beside that iteration's repulsion; ev_a0 orders it after everything issued so far
+    // (the update, a relabelling), and the update that reads its output waits for ev_a1
     void attraction() {
-        g_prof.begin(ST_ATTR, st);
-        launch_attraction<T>(cip, cix, cP, Y, n, 2, inv_dof, dptr<T>(attr), nnz, st);
-        g_prof.end(ST_ATTR, st, attr_bytes);
+        MG_HIP(hipEventRecord(ev_a0, st));
+        MG_HIP(hipStreamWaitEvent(aside, ev_a0, 0));
+        g_prof.begin(ST_ATTR, aside);
+        launch_attraction<T>(cip, cix, cP, Y, n, 2, inv_dof, dptr<T>(attr), nnz, aside);
+        g_prof.end(ST_ATTR, aside, attr_bytes);
+        MG_HIP(hipEventRecord(ev_a1, aside));
+        a_pending = true;
     }
 
     void start_relabelling() {
@@ -500,6 +507,12 @@ template <class T> struct Optimizer {
         inv_dof = T(1) / (T)p->dof;
         attr_bytes = 4.0 * (n + 1) + nnz * (4.0 + sizeof(T)) + 4.0 * n * sizeof(T);
         if (!mb_ready) MG_HIP(hipEventCreateWithFlags(&mb_ready, hipEventDisableTiming));
+        if (!aside) {
+            MG_HIP(hipStreamCreateWithFlags(&aside, hipStreamNonBlocking));
+            MG_HIP(hipEventCreateWithFlags(&ev_a0, hipEventDisableTiming));
+            MG_HIP(hipEventCreateWithFlags(&ev_a1, hipEventDisableTiming));
+        }
+        a_pending = false;
         relabel = fused && p->relabel_every >= 0 && n >= 2 && it1 - it0 > 1;
         relabel_every = p->relabel_every > 0 ? p->relabel_every : 25;
         if (relabel) start_relabelling();
@@ -520,6 +533,7 @@ template <class T> struct Optimizer {
         const T mom = (T)(early ? p->momentum0 : p->momentum1);
         const int* labels = relabel ? dptr<int>(inv) : nullptr;
         rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st, labels);
+        if (a_pending) { MG_HIP(hipStreamWaitEvent(st, ev_a1, 0)); a_pending = false; }   // attr of this iteration
         const T* dinv = dptr<T>(rep.inv_out);
         g_prof.begin(ST_UPDATE, st);
         if (fused) {
@@ -542,7 +556,7 @@ template <class T> struct Optimizer {
     // waits for iteration it; returns true when the callback asks to stop
     bool complete(int it) {
         MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
-        if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); g_prof.collect(it); }
+        if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); MG_HIP(hipStreamSynchronize(aside)); g_prof.collect(it); }
         if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
             if (!relabel) return cb(it, Y, n, user) != 0;
             // the layout in original point order; Y itself is final for iteration it (the event), and the
@@ -565,6 +579,8 @@ template <class T> struct Optimizer {
             launch_unperm<T>(G, dptr<int>(inv), n, uG, st);
         }
         MG_HIP(hipStreamSynchronize(st));
+        MG_HIP(hipStreamSynchronize(aside));   // an attraction queued before a callback stopped the loop
+        a_pending = false;
         rep.release_dead();
     }
 
@@ -583,7 +599,16 @@ template <class T> struct Optimizer {
     }
     bool counters_zeroed = false;
     hipEvent_t mb_ready = nullptr;
-    ~Optimizer() { if (mb_ready) (void)hipEventDestroy(mb_ready); }
+    hipStream_t aside = nullptr;   // the attraction's stream
+    hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr;
+    bool a_pending = false;
+    ~Optimizer() {
+        if (aside) (void)hipStreamSynchronize(aside);
+        if (mb_ready) (void)hipEventDestroy(mb_ready);
+        if (ev_a0) (void)hipEventDestroy(ev_a0);
+        if (ev_a1) (void)hipEventDestroy(ev_a1);
+        if (aside) (void)hipStreamDestroy(aside);
+    }
 };
 
 // ------------------------------------------------------------------------------ batched engine
