@@ -662,9 +662,20 @@ template <class T> struct BatchEngine {
     std::vector<BMap> attr_maps;
     std::vector<int> attr_G;
     hipEvent_t ready = nullptr;
+    // the next iteration's attraction runs on its own stream beside the repulsion (ev_a0: after the update; the
+    // update waits for ev_a1), reading a copy of the table's fixed part (CSR, sizes, offsets) made once, since
+    // the table itself is rewritten every iteration
+    DevBuf atable;
+    hipStream_t aside = nullptr;
+    hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr;
+    bool a_pending = false;
     ~BatchEngine() {
+        if (aside) (void)hipStreamSynchronize(aside);
         release_dead();
         if (ready) (void)hipEventDestroy(ready);
+        if (ev_a0) (void)hipEventDestroy(ev_a0);
+        if (ev_a1) (void)hipEventDestroy(ev_a1);
+        if (aside) (void)hipStreamDestroy(aside);
     }
 
     BInst<T>* host_table() { return reinterpret_cast<BInst<T>*>(htab.p); }
@@ -745,14 +756,27 @@ template <class T> struct BatchEngine {
             }
         }
         MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
+        MG_HIP(hipMemcpyAsync(atable.ensure((size_t)count * sizeof(BInst<T>)), h, (size_t)count * sizeof(BInst<T>),
+                              hipMemcpyHostToDevice, st));
         MG_HIP(hipStreamSynchronize(st));
         if (!ready) MG_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        if (!aside) {
+            MG_HIP(hipStreamCreateWithFlags(&aside, hipStreamNonBlocking));
+            MG_HIP(hipEventCreateWithFlags(&ev_a0, hipEventDisableTiming));
+            MG_HIP(hipEventCreateWithFlags(&ev_a1, hipEventDisableTiming));
+        }
+        a_pending = false;
         if (p->n_iter > 0) attraction();
     }
 
     void attraction() {
+        MG_HIP(hipEventRecord(ev_a0, st));
+        MG_HIP(hipStreamWaitEvent(aside, ev_a0, 0));
+        const BInst<T>* I = reinterpret_cast<const BInst<T>*>(atable.p);
         for (size_t k = 0; k < attr_maps.size(); ++k)
-            launch_attraction_batch<T>(dev_table(), attr_maps[k], attr_G[k], dptr<T>(Yc), inv_dof, dptr<T>(Ac), st);
+            launch_attraction_batch<T>(I, attr_maps[k], attr_G[k], dptr<T>(Yc), inv_dof, dptr<T>(Ac), aside);
+        MG_HIP(hipEventRecord(ev_a1, aside));
+        a_pending = true;
     }
 
     // iteration it for every instance: grids from the last bboxes, spectra where the grid changed, the
@@ -856,6 +880,7 @@ template <class T> struct BatchEngine {
         // gather, Z, and the update
         for (const BMap& m : make_maps(all, [&](int i) { return gather_blocks(n[i]); }))
             launch_gather_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, dptr<T>(Fc), st);
+        if (a_pending) { MG_HIP(hipStreamWaitEvent(st, ev_a1, 0)); a_pending = false; }   // Ac of this iteration
         for (const BMap& m : make_maps(all, [&](int i) { return update_blocks(n[i]); }))
             launch_update_batch<T>(I, m, dptr<T>(Yc), dptr<T>(Uc), dptr<T>(Gc), dptr<T>(Ac), dptr<T>(Fc), rho, mom, st);
         MG_HIP(hipMemcpyAsync(hmb.p, mbd.p, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
@@ -877,6 +902,8 @@ template <class T> struct BatchEngine {
                                       (size_t)p->n_iter * sizeof(double), hipMemcpyDeviceToHost, st));
         }
         MG_HIP(hipStreamSynchronize(st));
+        MG_HIP(hipStreamSynchronize(aside));
+        a_pending = false;
         release_dead();
     }
     void release_dead() {
