@@ -194,7 +194,10 @@ typedef int (*mg_tsne_callback)(int it, const void* d_Y, int64_t n, void* user);
 /* Runs iterations [it0, it1) of the n_iter schedule on the device state (Y, U, G), all (n, 2) in `prec`
  * (U = 0 and G = 1 at it0 = 0 for a fresh run; keeping them makes a resumed run identical to an
  * uninterrupted one). The CSR data must be normalised to sum 1 (tsne.py:149-151). d_pairs (n, m_pairs)
- * may be NULL. h_sumQ, when non-NULL, receives sumQ of each iteration (it1 - it0 doubles). */
+ * may be NULL. h_sumQ, when non-NULL, receives sumQ of each iteration (it1 - it0 doubles).
+ * Besides the library stream, the loop runs the next iteration's attraction and any spectra recomputation on
+ * two streams of its own (default priority), ordered against the library stream by events; it returns once
+ * all three have finished. */
 int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_indptr, const int32_t* d_indices,
                      const void* d_data, int64_t nnz, int64_t n, void* d_Y, void* d_U, void* d_G, const int32_t* d_pairs,
                      int m_pairs, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user);
@@ -203,7 +206,8 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_in
  * arrays) on one schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs;
  * max_boxes <= 1024). Every stage of an iteration is one launch for all instances of a group (one per FFT size for
  * the FFT stages). The instances are split into n_streams groups of about equal point counts, each on its own HIP
- * stream, driven by n_threads host threads (1 <= n_threads <= n_streams). Each instance's result is bit for bit
+ * stream (and a second one for its attraction), driven by n_threads host threads (1 <= n_threads <= n_streams).
+ * Each instance's result is bit for bit
  * the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null. The call returns when every
  * instance is done. No counterpart in the reference (it fits one embedding per call). */
 typedef struct {
