@@ -207,9 +207,9 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_in
  * max_boxes <= 1024). Every stage of an iteration is one launch for all instances of a group (one per FFT size for
  * the FFT stages). The instances are split into n_streams groups of about equal point counts, each on its own HIP
  * stream (and a second one for its attraction), driven by n_threads host threads (1 <= n_threads <= n_streams).
- * Each instance's result is bit for bit
- * the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles) may be null. The call returns when every
- * instance is done. No counterpart in the reference (it fits one embedding per call). */
+ * Each instance's result is bit for bit the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles)
+ * may be null. The call returns when every instance is done. No counterpart in the reference (it fits one
+ * embedding per call). */
 typedef struct {
     const int32_t* indptr;
     const int32_t* indices;
