@@ -400,6 +400,20 @@ template <class T> struct Repulsion {
 };
 
 // ------------------------------------------------------------------------------ whole loop
+// The schedule's grid limits, checked before any work: the largest grid the loop may reach (n_interp * max_boxes
+// nodes per side) must fit the largest device FFT, so that a run cannot fail hundreds of iterations in, when the
+// layout first spans that many boxes.
+void check_schedule(const mg_tsne_params* p) {
+    require(p != nullptr, "params must not be null");
+    require(p->n_interp >= 1 && p->n_interp <= 8, "n_interp must be in [1, 8]");
+    require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
+    require(fft_size(p->n_interp * p->max_boxes) > 0,
+            "n_interp * max_boxes = " + std::to_string(p->n_interp * p->max_boxes) +
+                " exceeds 4096, the largest grid side of the device FFT (8192 points): lower max_boxes to " +
+                std::to_string(4096 / p->n_interp) + " or below",
+            MG_ERR_UNSUP);
+}
+
 template <class T> struct Optimizer {
     Repulsion<T> rep;
     DevBuf attr, F, grad, part, mb, bb_part, bb_counter, bb_out, sumq;
@@ -917,7 +931,7 @@ template <class T> struct BatchEngine {
 // decides the grids of one group, the device runs the others.
 template <class T> void optimize_batch(const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_groups,
                                       int n_threads) {
-    require(p != nullptr, "params must not be null");
+    check_schedule(p);
     require(count >= 1 && inst != nullptr, "no instances");
     require(n_groups >= 1 && n_groups <= 64, "n_streams must be in [1, 64]");
     require(n_threads >= 1 && n_threads <= n_groups, "n_threads must be in [1, n_streams]");
@@ -1333,6 +1347,7 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* p, const int32_t* ip, const
     check_prec(prec);
     require(p != nullptr, "params must not be null");
     require(nnz == 0 || data != nullptr, "data must be a device array");
+    check_schedule(p);
     if (prec == MG_F64) {
         Optimizer<double> o(p->n_interp);
         o.run(p, ip, ix, (const double*)data, nnz, n, (double*)Y, (double*)U, (double*)G, pairs, m, it0, it1, h_sumQ, cb, user, lib_stream());

@@ -6,6 +6,8 @@ optimize_tsne runs the whole loop on the device (one call; the state stays in HB
 (default, float32 state and arithmetic with FP64 Z and reductions, like the reference's float32 path)
 or "fp64" (the plain FP64 path). FFTRepulsion2D keeps the reference's per-call interface.
 """
+import warnings
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -136,8 +138,15 @@ def optimize_tsne(P, Y, n_iter=750, early_iter=250, early_exaggeration=12.0, exa
             if callback is not None:
                 return callback(it, Yh)
             return None
+    max_boxes = 1000
+    if n_interp * max_boxes > 4096:
+        # the largest device FFT holds grids of 4096 nodes per side; the reference's limit of 1000 boxes would pass
+        # that with more than 4 nodes per box, so the box count is capped up front instead of failing mid-run
+        max_boxes = 4096 // n_interp
+        warnings.warn(f"n_interp={n_interp}: the grid is capped at {max_boxes} boxes per side (the reference allows 1000); "
+                      "results differ from the reference only once the layout spans more boxes", RuntimeWarning)
     sumq = _native().tsne_optimize(indptr, indices, data, Y, update, gains, int(n_iter), int(early_iter),
                                    float(early_exaggeration), float(exaggeration), float(learning_rate),
                                    float(momentum[0]), float(momentum[1]), int(n_interp), float(box_width), float(dof),
-                                   mid_pairs, float(mid_weight), float(mid_phase), int(it0), int(it1), 20, 1000, cb)
+                                   mid_pairs, float(mid_weight), float(mid_phase), int(it0), int(it1), 20, max_boxes, cb)
     return (Y, np.asarray(sumq)) if return_sumq else Y

@@ -102,3 +102,18 @@ def test_knn_entries_reject_bad_arguments():
     assert L.mg_knn(None, i64(10), None, i64(0), 4, 2, i64(-1), None, None) == 0     # no queries: nothing to do
     assert L.mg_topk_rows(None, i64(0), i64(5), 9, None, None) != 0                  # k above the row length
     assert b"k must be" in L.mg_last_error()
+
+
+def test_optimize_rejects_a_grid_beyond_the_device_fft_up_front():
+    """n_interp * max_boxes above 4096 nodes per side is refused before any device work (the reference's default of
+    1000 boxes with 5 nodes would otherwise fail only when the layout first spans 820 boxes)."""
+    L = capi.lib()
+    i64 = ctypes.c_int64
+    fake = ctypes.c_void_p(16)
+    for ni, mb in ((5, 1000), (3, 1366), (8, 513)):
+        p = capi.Params(10, 3, 12.0, 1.0, 0.0, 0.5, 0.8, ni, 20, mb, 0.7, 1.0, 0.0, 0.5)
+        rc = L.mg_tsne_optimize(capi.MG_F32, ctypes.byref(p), fake, fake, fake, i64(1), i64(1), fake, fake, fake, None, 0, 0, 10,
+                                None, capi.CALLBACK(), None)
+        assert rc == 4 and b"exceeds 4096" in L.mg_last_error(), (ni, mb)
+        arr = (capi.Instance * 1)()
+        assert L.mg_tsne_optimize_batch(capi.MG_F32, ctypes.byref(p), 1, arr, 1, 1) == 4

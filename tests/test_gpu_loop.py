@@ -162,7 +162,7 @@ def test_fp32_gradient_against_fp64_options(dof, ni):
     """The same bound with a heavier and a lighter tailed kernel and with five interpolation nodes per box."""
     P, Y0, _ = problem(2000, 10, 5, seed=3)
     st = State(P, Y0, np.float64)
-    kw = dict(n_iter=300, early_iter=100, dof=dof, n_interp=ni)
+    kw = dict(n_iter=300, early_iter=100, dof=dof, n_interp=ni, max_boxes=min(1000, 4096 // ni // 4 * 4))
     it = 0
     for t in (60, 150, 299):
         st.run(it, t, **kw); it = t
@@ -233,7 +233,7 @@ def test_relabelling_changes_no_result(dtype):
         assert sorted(seen) == sorted(ref[2]) and all(np.array_equal(seen[k], ref[2][k]) for k in seen), every
 
 
-SCHED_OPTS = [dict(dof=0.6), dict(dof=2.0), dict(n_interp=2), dict(n_interp=5),
+SCHED_OPTS = [dict(dof=0.6), dict(dof=2.0), dict(n_interp=2), dict(n_interp=5, max_boxes=816),
               dict(exaggeration=4.0, learning_rate=150.0, momentum=(0.3, 0.9)),
               dict(box_width=1.1, min_boxes=8, max_boxes=12)]
 
