@@ -145,6 +145,13 @@ int mg_topk_rows(const float* d_D, int64_t rows, int64_t cols, int k, int32_t* d
  * result does not depend on the other queries of the call. Calls are serialised (one scratch per process). */
 int mg_knn(const float* d_T, int64_t nt, const float* d_Q, int64_t m, int d, int k, int64_t self_offset, int32_t* d_idx,
            float* d_dist2);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops): the search of mg_knn, with the same results bit for bit, that allocates nothing and does
+ * not synchronise (it returns once its launches are queued on the library stream). Its scratch is a fixed 8 MiB
+ * device array that each device holds until the process ends, so m is limited to 65536 query rows for k <= 16 and
+ * 32768 for k <= 32 (MG_ERR_ARG beyond). Calls that overlap on one device share that array and race. */
+int mg_knn_noalloc(const float* d_T, int64_t nt, const float* d_Q, int64_t m, int d, int k, int64_t self_offset,
+                   int32_t* d_idx, float* d_dist2);
 /* smooth_knn (magnet/knn.py:165-198) on the device, float32: memberships d_vals (m, k) of the sorted neighbour
  * distances d_dist (m, k), one row per thread; d_mean_all (m) is each row's fallback mean distance (the mean over
  * its call, used when the row has no non-zero distance). */

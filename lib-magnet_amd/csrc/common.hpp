@@ -138,6 +138,10 @@ void launch_topk_rows(const float* D, i64 rows, i64 cols, int k, int32_t* oi, fl
 int knn_chunks(i64 m, i64 nt);
 void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
                 int32_t* oi, float* ov, hipStream_t st);
+// the same search with its scratch in a fixed device array (no allocation); at most knn_noalloc_max_queries(k) queries
+i64 knn_noalloc_max_queries(int k);
+void launch_knn_noalloc(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, int32_t* oi, float* ov,
+                        hipStream_t st);
 
 // smooth k-NN memberships of sorted neighbour distances (m, k), per row; mean_all (m): the fallback mean distance
 void launch_smooth_knn(const float* dist, i64 m, int k, int n_neighbors, const float* mean_all, float* vals, hipStream_t st);

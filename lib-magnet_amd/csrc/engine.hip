@@ -1226,6 +1226,14 @@ int mg_knn(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, 
     MG_API_END
 }
 
+int mg_knn_noalloc(const float* T, int64_t nt, const float* Q, int64_t m, int d, int k, int64_t self_offset, int32_t* idx,
+                   float* dist2) {
+    MG_API_BEGIN
+    require(m >= 0 && nt >= 1 && (m == 0 || (T && Q && idx && dist2)), "bad k-NN arguments");
+    launch_knn_noalloc(T, nt, Q, m, d, k, self_offset, idx, dist2, lib_stream());
+    MG_API_END
+}
+
 int mg_smooth_knn(const float* dist, int64_t m, int k, int n_neighbors, const float* mean_all, float* vals) {
     MG_API_BEGIN
     require(m == 0 || (dist && mean_all && vals), "null array");

@@ -7,6 +7,8 @@
 #include <climits>
 #include <cstdint>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace mg {
 
@@ -230,12 +232,10 @@ int knn_chunks(i64 m, i64 nt) {
     while (S < KNN_SMAX && tiles * S < 1024 && (i64)S * 2 * KNN_TR <= nt) S *= 2;   // about 1024 blocks, >= 64 rows per chunk
     return S;
 }
-void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
-                int32_t* oi, float* ov, hipStream_t st) {
-    require(d >= 1 && d <= 64, "the fused k-NN search takes up to 64 features");
-    require(k >= 1 && k <= 32 && k <= nt - (self_offset >= 0 ? 1 : 0), "k must be in [1, min(32, rows)]");
-    if (m == 0) return;
-    const int S = knn_chunks(m, nt);
+// the search with S training chunks per query (a power of two); the result does not depend on S: each chunk list is
+// its chunk's exact nearest KM by (distance, index), and the merge keeps that order
+static void knn_with_chunks(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, int S,
+                            float* part_d, int* part_i, int32_t* oi, float* ov, hipStream_t st) {
     const i64 chunk = (nt + S - 1) / S;
     const dim3 g1((unsigned)((m + KNN_Q - 1) / KNN_Q), (unsigned)S);
 #define MG_K(DP, KM) hipLaunchKernelGGL((k_knn_chunks<DP, KM>), g1, dim3(KNN_Q), 0, st, T, nt, Q, m, d, chunk, self_offset, part_d, part_i, S)
@@ -254,6 +254,63 @@ void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64
     if (KM == 16) hipLaunchKernelGGL(k_knn_merge<16>, dim3((unsigned)m), dim3(P), lds, st, part_d, part_i, S, k, oi, ov);
     else hipLaunchKernelGGL(k_knn_merge<32>, dim3((unsigned)m), dim3(P), lds, st, part_d, part_i, S, k, oi, ov);
     MG_HIP(hipGetLastError());
+}
+
+static void knn_check(i64 nt, int d, int k, i64 self_offset) {
+    require(d >= 1 && d <= 64, "the fused k-NN search takes up to 64 features");
+    require(k >= 1 && k <= 32 && k <= nt - (self_offset >= 0 ? 1 : 0), "k must be in [1, min(32, rows)]");
+}
+
+void launch_knn(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, float* part_d, int* part_i,
+                int32_t* oi, float* ov, hipStream_t st) {
+    knn_check(nt, d, k, self_offset);
+    if (m == 0) return;
+    knn_with_chunks(T, nt, Q, m, d, k, self_offset, knn_chunks(m, nt), part_d, part_i, oi, ov, st);
+}
+
+// ---- the allocation-free search (mg_knn_noalloc): its chunk lists live in `scratch`, a fixed array of this file,
+// KNN_SCRATCH_ENTRIES (distance, index) pairs (8 MiB: the distances, then the indices). Each device holds its own
+// copy until the process ends. Every list slot is written by the chunk pass before the merge reads it, so nothing
+// needs clearing between calls; calls that overlap on one device share the array and race.
+constexpr i64 KNN_SCRATCH_ENTRIES = (i64)1 << 20;
+static __device__ __align__(256) unsigned char scratch[KNN_SCRATCH_ENTRIES * (sizeof(float) + sizeof(int))];
+
+constexpr int KNN_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[KNN_MAX_DEVICES] = {};
+
+// The device address of this device's copy of `scratch`, looked up once per device with that device current. The
+// host-side name `scratch` is only a stand-in: kernels get this address.
+static unsigned char* knn_scratch_on_current_device() {
+    int dev = 0;
+    MG_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < KNN_MAX_DEVICES, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "no device address for the k-NN scratch");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+i64 knn_noalloc_max_queries(int k) { return KNN_SCRATCH_ENTRIES / (k <= 16 ? 16 : 32); }
+
+void launch_knn_noalloc(const float* T, i64 nt, const float* Q, i64 m, int d, int k, i64 self_offset, int32_t* oi, float* ov,
+                        hipStream_t st) {
+    knn_check(nt, d, k, self_offset);
+    const i64 KM = k <= 16 ? 16 : 32, cap = knn_noalloc_max_queries(k);
+    require(m <= cap, "mg_knn_noalloc takes at most " + std::to_string(cap) + " query rows for k = " + std::to_string(k) +
+                          " (its fixed scratch holds " + std::to_string(KNN_SCRATCH_ENTRIES) + " list entries); got " +
+                          std::to_string(m));
+    if (m == 0) return;
+    unsigned char* base = knn_scratch_on_current_device();
+    int S = knn_chunks(m, nt);
+    while (S > 1 && m * S * KM > KNN_SCRATCH_ENTRIES) S /= 2;   // fewer, longer chunks: the same result
+    float* part_d = reinterpret_cast<float*>(base);
+    int* part_i = reinterpret_cast<int*>(base + KNN_SCRATCH_ENTRIES * sizeof(float));
+    knn_with_chunks(T, nt, Q, m, d, k, self_offset, S, part_d, part_i, oi, ov, st);
 }
 }  // namespace mg
 

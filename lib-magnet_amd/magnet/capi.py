@@ -75,6 +75,7 @@ def lib():
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
+            "mg_knn_noalloc": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
             "mg_smooth_knn": ([vp, i64, i32, i32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
@@ -251,6 +252,21 @@ def knn(T, Q, k, exclude_self=False):
     d2 = torch.empty((Q.shape[0], k), dtype=torch.float32, device=Q.device)
     check(lib().mg_knn(ptr(T), T.shape[0], ptr(Q), Q.shape[0], T.shape[1], int(k), 0 if exclude_self else -1, ptr(idx), ptr(d2)))
     return idx, d2
+
+
+def knn_noalloc(T, Q, idx, d2, k, exclude_self=False):
+    """mg_knn_noalloc: the search of knn() into the caller's idx (m, k) int32 / d2 (m, k) float32 device tensors,
+    with no device allocation and no synchronisation (the results are ready once the library stream reaches them)."""
+    import torch
+    for a in (T, Q, d2):
+        if a.dtype != torch.float32 or a.dim() != 2 or not a.is_contiguous():
+            raise ValueError("T, Q and d2 must be contiguous 2-d float32 tensors")
+    if idx.dtype != torch.int32 or tuple(idx.shape) != (Q.shape[0], k) or tuple(d2.shape) != (Q.shape[0], k):
+        raise ValueError("idx / d2 must be (m, k) int32 / float32 tensors")
+    if T.shape[1] != Q.shape[1]:
+        raise ValueError("T and Q must have the same number of features")
+    check(lib().mg_knn_noalloc(ptr(T), T.shape[0], ptr(Q), Q.shape[0], T.shape[1], int(k), 0 if exclude_self else -1,
+                               ptr(idx), ptr(d2)))
 
 
 def smooth_knn(dist, n_neighbors, mean_all):
