@@ -5,14 +5,18 @@ A step is one iteration of the t-SNE schedule (attraction over the CSR P, FFT-in
 gains / momentum update) over the whole synthetic data set. Default workload: the MNIST-shaped
 precision run -- 70,000 x 784 float32 (10 prototypes with ~19% nonzero pixels, + N(0, 0.1^2) noise,
 clipped, quantised to /255), perplexity-30 affinities (90-NN), 2-D, 750 iterations (250 early
-exaggeration) -- with --steps 750 timing the whole schedule from iteration 0. Inputs (P, Y0) are resident
-in HBM before the timed region. value = iterations/s of the FP32 path (FP32 bulk, FP64 Z and
-reductions); the plain FP64 path is timed on the same workload and reported beside it.
+exaggeration). Inputs (P, Y0) are resident in HBM before the timed region. value = iterations/s of the FP32
+path (FP32 bulk, FP64 Z and reductions). With --steps K below the schedule length the timed iterations are K
+iterations in up to five runs of consecutive iterations, one centred in each fifth of the schedule, each
+started from the schedule's own state there (the schedule is run once, untimed, to make those states); with
+K at least the schedule length the whole schedule is timed from iteration 0. The whole FP32 and FP64
+schedules, FP32 against FP64 from the same states, and the FP32 gradient error against FP64 are reported
+beside it, and the 1M configuration (gmm1m) the same way.
 
-The per-stage roofline comes from a second, profiled pass of the same iterations (HIP events on the
-library's stream, algorithmic bytes from the engine). cpu_baseline times the reference's own compiled
-module (oracle/_ref) on this host's cores on a bounded sample of iterations taken along the GPU
-trajectory.
+The per-stage roofline comes from a second, profiled pass of the same timed runs (HIP events on the
+library's stream, algorithmic bytes from the engine), per padded FFT size for the FFT kernels. cpu_baseline
+times the reference's own compiled module (oracle/_ref) on this host's cores on a bounded sample of
+iterations at the same states.
 
 Two more legs run with the default workload and are reported beside the headline:
 - `serving`: a 200,000 x 50 MAGNET(loss="tsne") model kept resident, transform() requests of 1-4096 rows
@@ -75,67 +79,147 @@ def make_data(name, seed=0):
     return X.astype(np.float32), lab
 
 
-def run_gpu(P, Y0, prec, steps, warmup, n_iter, early, profile=False, checkpoints=()):
-    """Whole-loop run of `steps` iterations from Y0 on resident inputs. With profile=True the per-stage HIP
-    event timers run; checkpoints: iterations at which (Y, U, G) device copies are kept (the run is then split
-    at those iterations, which the resumable loop reproduces exactly)."""
-    import torch
-    from magnet import capi
-    dt = np.float32 if prec == "fp32" else np.float64
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
-    ip, ix, data = dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32)), dev(P.data.astype(dt))
-    kw = dict(n_iter=n_iter, early_iter=early)
-    if warmup > 0:
-        Yw, Uw, Gw = dev(Y0.astype(dt)), dev(np.zeros_like(Y0, dt)), dev(np.ones_like(Y0, dt))
-        capi.optimize(ip, ix, data, Yw, Uw, Gw, 0, min(warmup, n_iter), **kw)
-    Y, U, G = dev(Y0.astype(dt)), dev(np.zeros_like(Y0, dt)), dev(np.ones_like(Y0, dt))
-    capi.profile(profile)
-    snaps = {}
-    cuts = sorted(t for t in set(checkpoints) if 0 <= t <= steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    it = 0
-    for c in cuts + [steps]:
+def schedule_segments(steps, n_iter, max_runs=5):
+    """The timed iterations: the whole schedule when steps >= n_iter; else `steps` iterations in up to five runs of
+    consecutive iterations (at least four each), one centred in each of equal strata of the schedule, so that the
+    early phase, the switch, the clipped-grid stretch and the late phase are all timed."""
+    if steps >= n_iter:
+        return [(0, n_iter)]
+    runs = max(1, min(max_runs, steps // 4))
+    lens = [steps // runs + (1 if r < steps % runs else 0) for r in range(runs)]
+    out = []
+    for r, L in enumerate(lens):
+        a = int(round((r + 0.5) * n_iter / runs - L / 2))
+        a = max(0, min(n_iter - L, a))
+        out.append((a, a + L))
+    return out
+
+
+class Loop:
+    """One precision of the whole loop on resident inputs: the CSR on the device and a reusable mg_tsne_session."""
+
+    def __init__(self, P, prec, n_iter, early):
+        import torch
+        from magnet import capi
+        self.dt = np.float32 if prec == "fp32" else np.float64
+        self.tdt = torch.float32 if prec == "fp32" else torch.float64
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        self.csr = (dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32)), dev(P.data.astype(self.dt)))
+        self.sess = capi.Session(*self.csr, n_iter=n_iter, early_iter=early)
+        self.n_iter = n_iter
+
+    def state(self, Y0=None, like=None):
+        import torch
+        if like is not None:
+            return tuple(t.to(self.tdt).clone() for t in like)
+        Y = torch.from_numpy(np.ascontiguousarray(Y0, dtype=self.dt)).cuda()
+        return Y, torch.zeros_like(Y), torch.ones_like(Y)
+
+    def run(self, st, a, b):
+        return self.sess.run(*st, a, b)
+
+    def timed(self, st, a, b):
+        """seconds for iterations [a, b) on state st, bracketed by device synchronisations"""
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.sess.run(*st, a, b)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+
+def checkpoint_run(loop, Y0, starts):
+    """The whole schedule from Y0 in runs split at `starts`, keeping a copy of the state at each; returns (snapshots,
+    seconds of the iterations alone, per-iteration sumQ)."""
+    st = loop.state(Y0)
+    snaps, sec, sq, it = {}, 0.0, [], 0
+    for c in sorted(set(starts)) + [loop.n_iter]:
         if c > it:
-            capi.optimize(ip, ix, data, Y, U, G, it, c, **kw)
+            sec += loop.timed(st, it, c)
             it = c
-        if c in cuts:
-            snaps[c] = (Y.clone(), U.clone(), G.clone())
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
+        if c < loop.n_iter:
+            snaps[c] = tuple(t.clone() for t in st)
+    return snaps, sec
+
+
+def timed_segments(loop, snaps, segs, profile=False):
+    """The timed region: each run of consecutive iterations from the schedule's own state at its start; the
+    per-run times (device-synchronised on both sides) are summed."""
+    from magnet import capi
+    total = 0.0
+    capi.profile(profile)
+    for a, b in segs:
+        st = tuple(t.clone() for t in snaps[a])
+        total += loop.timed(st, a, b)
     stages = capi.profile_read() if profile else None
     if profile:
         stages["_grid"] = capi.profile_grid()
+        stages["_sized"] = capi.profile_sized()
     capi.profile(False)
-    return sec, stages, (Y, U, G, ip, ix, data), snaps
+    return total, stages
 
 
-def matched_precision(P, snaps, n_iter, early, k):
-    """FP32 and FP64 from the same states: k iterations of each from every checkpoint state (FP32 states,
-    widened to float64 for the FP64 run); returns per-checkpoint (t, ms32, ms64)."""
-    import torch
-    from magnet import capi
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
-    csr = {np.float32: None, np.float64: None}
-    for dt in csr:
-        csr[dt] = (dev(P.indptr.astype(np.int32)), dev(P.indices.astype(np.int32)), dev(P.data.astype(dt)))
+def matched_precision(l32, l64, snaps, k):
+    """FP32 and FP64 from the same states: k iterations of each from every snapshot (FP32 states, widened to
+    float64 for the FP64 run); returns per-state (t, ms32, ms64)."""
     rows = []
-    for t, (Y, U, G) in sorted(snaps.items()):
-        t1 = min(t + k, n_iter)
+    for t, st in sorted(snaps.items()):
+        t1 = min(t + k, l32.n_iter)
         if t1 <= t:
             continue
-        ms = {}
-        for dt, tdt in ((np.float32, torch.float32), (np.float64, torch.float64)):
-            Yc, Uc, Gc = Y.to(tdt).clone(), U.to(tdt).clone(), G.to(tdt).clone()
-            ip, ix, data = csr[dt]
-            capi.optimize(ip, ix, data, Yc.clone(), Uc.clone(), Gc.clone(), t, min(t + 2, t1), n_iter=n_iter, early_iter=early)
-            torch.cuda.synchronize()
-            a = time.perf_counter()
-            capi.optimize(ip, ix, data, Yc, Uc, Gc, t, t1, n_iter=n_iter, early_iter=early)
-            torch.cuda.synchronize()
-            ms[dt] = 1e3 * (time.perf_counter() - a) / (t1 - t)
-        rows.append((t, ms[np.float32], ms[np.float64]))
+        ms = []
+        for loop in (l32, l64):
+            s = loop.state(like=st)
+            ms.append(1e3 * loop.timed(s, t, t1) / (t1 - t))
+        rows.append((t, ms[0], ms[1]))
     return rows
+
+
+def gradient_error(P, l32, l64, st, it, early):
+    """Relative L2 of the FP32 path's gradient rho A - F / sumQ against the FP64 path's on the same layout (the FP32
+    state widened), each through the per-call attraction and FFTConv2D objects of its precision."""
+    import torch
+    from magnet import capi
+    from magnet.tsne import grid_params
+    Y32 = st[0].to(torch.float32).contiguous()
+    rho = 12.0 if it < early else 1.0
+    grads = []
+    for loop in (l32, l64):
+        Y = Y32.to(loop.tdt).contiguous()
+        A = capi.attraction(*loop.csr, Y)
+        xmin, ymin, width, nb = grid_params(capi.bbox(Y32))
+        conv = capi.FFTConv(capi.MG_F32 if loop.dt == np.float32 else capi.MG_F64, nb * 6, 3)
+        conv.set_tsne(width)
+        s, F = conv.compute(Y, xmin, ymin, width, nb)
+        grads.append(rho * A.double() - F.double() / s)
+    return float(torch.linalg.norm(grads[0] - grads[1]) / torch.linalg.norm(grads[1]))
+
+
+def precision_run(P, n_iter, early, steps, warmup, profile=True, cpu=False, threads=1):
+    """The headline measurement of one workload: FP32 loop timed over the segments of the schedule, its profiled
+    pass, the whole FP32 and FP64 schedules, FP32 against FP64 from the same states and the FP32 gradient error."""
+    n = P.shape[0]
+    Y0 = np.random.default_rng(1).normal(0, 1e-4, (n, 2))
+    segs = schedule_segments(steps, n_iter)
+    starts = [a for a, _ in segs]
+    match_starts = sorted(set(starts + [int(round(f * (n_iter - 10))) for f in (0.1, 0.3, 0.5, 0.7, 0.9)]))
+    l32 = Loop(P, "fp32", n_iter, early)
+    if warmup > 0:
+        l32.run(l32.state(Y0), 0, min(warmup, n_iter))
+    snaps, full32 = checkpoint_run(l32, Y0, starts + match_starts)
+    sec32, _ = timed_segments(l32, snaps, segs)
+    out = {"segments": segs, "seconds": sec32, "full32_s": full32}
+    if profile:
+        _, out["stages"] = timed_segments(l32, snaps, segs, profile=True)
+    l64 = Loop(P, "fp64", n_iter, early)
+    l64.run(l64.state(Y0), 0, min(10, n_iter))
+    _, full64 = checkpoint_run(l64, Y0, [])
+    out["full64_s"] = full64
+    out["matched"] = matched_precision(l32, l64, {t: snaps[t] for t in match_starts}, 10)
+    out["grad_err"] = [(t, gradient_error(P, l32, l64, snaps[t], t, early)) for t in match_starts]
+    out["snaps"] = snaps
+    out["match_starts"] = match_starts
+    return out
 
 
 def serving_bench(qps=None, duration=None, seed=0, n_workers=3):
@@ -237,8 +321,8 @@ def batched_bench():
 
 def cpu_baseline(P, n_iter, early, states, threads, points):
     """The reference's compiled module on this host, in a process of its own (it registers the module name
-    `_magnet`, like the product): per-iteration time at six points of the schedule (GPU FP32 states), 2 warm-up
-    + 5 timed reference iterations at each; it/s from the mean."""
+    `_magnet`, like the product): per-iteration time at the given points of the schedule (GPU FP32 states),
+    2 warm-up + 5 timed reference iterations at each; it/s from the mean."""
     import subprocess
     import tempfile
     with tempfile.TemporaryDirectory() as tmp:
@@ -311,7 +395,7 @@ def cpu_baseline_run(f, n, n_iter, early, threads, points):
     return {"value": round(1.0 / mean, 3), "unit": "iters/s", "cores": threads, "kind": "reference",
             "sample": f"reference _magnet (oracle/_ref, n_threads={threads}) on the same P: median of 5 iterations "
                       f"(after 2 warm-up) at iterations {points} of the schedule, started from the GPU FP32 states; "
-                      f"it/s = 1 / mean of the six medians ({', '.join(f'{1e3 * v:.1f}' for v in per_it)} ms)"}
+                      f"it/s = 1 / mean of the {len(per_it)} medians ({', '.join(f'{1e3 * v:.1f}' for v in per_it)} ms)"}
 
 
 # single-kernel stages: the roofline's dominant kernel is the one of these with the largest total time
@@ -369,6 +453,63 @@ def attraction_probe(workload, reps=5):
                       "mean_us": round(1e6 * (time.perf_counter() - t0) / reps, 1)}), flush=True)
 
 
+def roofline_and_stages(stages, workload, steps):
+    """Per-stage event timings of the profiled pass and the roofline of its dominant kernel: the (stage, FFT size)
+    of a single-kernel stage with the largest total time (FFT kernels are compiled per size)."""
+    gridstat = stages.pop("_grid")
+    sized = stages.pop("_sized")
+    rows = {k: dict(ms_total=round(v[0], 3), launches=v[1], avg_us=round(1e3 * v[0] / max(v[1], 1), 2),
+                    GBs=round(v[2] / max(v[0], 1e-9) / 1e6, 1)) for k, v in stages.items() if v[1] > 0}
+    (dname, dM), (dms, dcnt, dbytes) = max(((k, v) for k, v in sized.items() if k[0] in SINGLE_KERNEL and v[1] > 0),
+                                           key=lambda kv: kv[1][0])
+    kernel = SINGLE_KERNEL[dname] + (f"<float, {dM}>" if dM else "<float>")
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": kernel, "stage": dname, "fft_size": dM or None, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_of(workload, kernel), "avg_launch_us": round(1e3 * dms / dcnt, 2),
+                "launches": dcnt, "algo_bytes_per_launch": round(dbytes / dcnt)}
+    tot_ms = sum(v[0] for v in stages.values())
+    tot_bytes = sum(v[2] for v in stages.values())
+    extra = {"stages": rows, "stage_sizes": {f"{k}@{M}": dict(ms_total=round(v[0], 3), launches=v[1],
+                                                               avg_us=round(1e3 * v[0] / max(v[1], 1), 2))
+                                             for (k, M), v in sorted(sized.items()) if M and v[1] > 0},
+             "pass_GBs": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+             "grid": {"mean_N": round(gridstat[0], 1), "max_N": gridstat[1], "max_M": gridstat[2]},
+             "algo_bytes_per_iter": round(tot_bytes / steps)}
+    return roofline, extra
+
+
+def schedule_extra(R, n_iter):
+    m32, m64 = sum(r[1] for r in R["matched"]), sum(r[2] for r in R["matched"])
+    errs = [e for _, e in R["grad_err"]]
+    return {
+        "timed_runs": [list(x) for x in R["segments"]],
+        "full_schedule": {"fp32_it_s": round(n_iter / R["full32_s"], 2), "fp64_it_s": round(n_iter / R["full64_s"], 2),
+                          "note": "each precision's whole schedule from Y0 (its own, chaotic trajectory)"},
+        "fp32_over_fp64_same_states": round(m64 / m32, 3),
+        "matched_ms_per_iter": [[t, round(a, 4), round(b, 4)] for t, a, b in R["matched"]],
+        "fp32_gradient_rel_l2": {"max": float(f"{max(errs):.3g}"), "at": [[t, float(f"{e:.3g}")] for t, e in R["grad_err"]]},
+    }
+
+
+def scale_leg(warmup):
+    """The 1M configuration beside the headline: FP32 and FP64 whole schedules, FP32 against FP64 from the same
+    states (10 iterations from each of five) and the FP32 gradient error there."""
+    from magnet.affinity import perplexity_affinities
+    W = WORKLOADS["gmm1m"]
+    t0 = time.time()
+    X, _ = make_data("gmm1m")
+    P = perplexity_affinities(X, 30.0)
+    del X
+    t_p = time.time() - t0
+    R = precision_run(P, W["n_iter"], W["early"], W["n_iter"], min(warmup, 10), profile=False)
+    ex = schedule_extra(R, W["n_iter"])
+    return {"workload": W["desc"], "n": int(P.shape[0]), "nnz": int(P.nnz), "P_build_s": round(t_p, 1),
+            "fp32_it_s": ex["full_schedule"]["fp32_it_s"], "fp64_it_s": ex["full_schedule"]["fp64_it_s"],
+            "fp32_over_fp64_same_states": ex["fp32_over_fp64_same_states"], "matched_ms_per_iter": ex["matched_ms_per_iter"],
+            "fp32_gradient_rel_l2": ex["fp32_gradient_rel_l2"]}
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-worker":
         cpu_baseline_worker(sys.argv[2])
@@ -378,12 +519,12 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="iterations timed (default: the whole schedule)")
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="mnist70k", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB", help="only run a plain copy of MIB MiB")
     ap.add_argument("--attraction-probe", action="store_true", help="only time the CSR attraction of the workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the transform() serving leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched leg")
+    ap.add_argument("--no-scale", action="store_true", help="skip the 1M leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the t-SNE pass of one embedding runs on a single MI355X "
@@ -411,52 +552,31 @@ def main():
     X, lab = make_data(args.workload)
     P = perplexity_affinities(X, 30.0)
     del X
-    Y0 = np.random.default_rng(1).normal(0, 1e-4, (P.shape[0], 2))
     log(f"bench: {W['desc']}: n={P.shape[0]} nnz={P.nnz} (P built in {time.time() - t0:.1f}s)")
-
-    sec32, _, _, _ = run_gpu(P, Y0, "fp32", steps, args.warmup, W["n_iter"], W["early"])
-    log(f"bench: fp32 {steps} iterations in {sec32:.3f}s -> {steps / sec32:.1f} it/s")
-    points = [int(round(f * (steps - 1))) for f in (0.0, 0.2, 0.4, 0.6, 0.8, 1.0)]
-    K_MATCH = 25
-    mpoints = [t for t in points if t + K_MATCH <= steps] + [max(0, steps - K_MATCH)]
-    _, stages, state, snaps = run_gpu(P, Y0, "fp32", steps, 0, W["n_iter"], W["early"], profile=True,
-                                      checkpoints=points + mpoints)
-    gridstat = stages.pop("_grid")
-    stage_rows = {k: dict(ms_total=round(v[0], 3), launches=v[1], avg_us=round(1e3 * v[0] / max(v[1], 1), 2),
-                          GBs=round(v[2] / max(v[0], 1e-9) / 1e6, 1)) for k, v in stages.items() if v[1] > 0}
-    dname, (dms, dcnt, dbytes) = max(((k, v) for k, v in stages.items() if k in SINGLE_KERNEL and v[1] > 0),
-                                     key=lambda kv: kv[1][0])
-    achieved = dbytes / (dms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": SINGLE_KERNEL[dname], "stage": dname, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_of(args.workload, SINGLE_KERNEL[dname]),
-                "avg_launch_us": round(1e3 * dms / dcnt, 2), "algo_bytes_per_launch": round(dbytes / dcnt)}
-    tot_ms = sum(v[0] for v in stages.values())
-    tot_bytes = sum(v[2] for v in stages.values())
-    log(f"bench: stages {json.dumps(stage_rows)}")
-    extra = {"stages": stage_rows, "pass_GBs": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
-             "grid": {"mean_N": round(gridstat[0], 1), "max_N": gridstat[1], "max_M": gridstat[2]},
-             "algo_bytes_per_iter": round(tot_bytes / steps)}
-    if not args.no_fp64:
-        sec64, _, _, _ = run_gpu(P, Y0, "fp64", steps, min(args.warmup, 10), W["n_iter"], W["early"])
-        log(f"bench: fp64 {steps} iterations in {sec64:.3f}s -> {steps / sec64:.1f} it/s (its own trajectory)")
-        extra["fp64"] = {"value": round(steps / sec64, 2), "unit": "iters/s", "ms_per_step": round(1e3 * sec64 / steps, 4),
-                         "note": "the FP64 schedule from the same Y0; the trajectory is chaotic, so its grids differ"}
-        rows = matched_precision(P, {t: snaps[t] for t in mpoints}, W["n_iter"], W["early"], K_MATCH)
-        m32, m64 = sum(r[1] for r in rows), sum(r[2] for r in rows)
-        extra["fp32_over_fp64_same_states"] = round(m64 / m32, 3)
-        extra["matched_ms_per_iter"] = [[t, round(a, 4), round(b, 4)] for t, a, b in rows]
-        extra["fp32_over_fp64_own_trajectories"] = round(sec64 / sec32, 3)
-        log(f"bench: same states FP32 vs FP64 (t, ms32, ms64): {extra['matched_ms_per_iter']} -> {m64 / m32:.2f}x")
+    threads = min(16, os.cpu_count() or 1)
+    R = precision_run(P, W["n_iter"], W["early"], steps, args.warmup)
+    sec32, segs = R["seconds"], R["segments"]
+    log(f"bench: fp32 {steps} iterations in runs {segs}: {sec32:.4f}s -> {steps / sec32:.1f} it/s; "
+        f"whole schedule {W['n_iter'] / R['full32_s']:.1f} it/s")
+    roofline, extra = roofline_and_stages(R["stages"], args.workload, steps)
+    extra.update(schedule_extra(R, W["n_iter"]))
+    log(f"bench: stages {json.dumps(extra['stages'])}")
+    log(f"bench: same states FP32 vs FP64 (t, ms32, ms64): {extra['matched_ms_per_iter']} -> "
+        f"{extra['fp32_over_fp64_same_states']}x; FP32 gradient error {extra['fp32_gradient_rel_l2']}")
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-
+        import torch
         torch.cuda.synchronize()
-        states = {t: tuple(np.ascontiguousarray(a.cpu().numpy()) for a in snaps[t]) for t in points}
+        pts = [a for a, _ in segs] if len(segs) > 1 else R["match_starts"]
+        states = {t: tuple(np.ascontiguousarray(a.cpu().numpy()) for a in R["snaps"][t]) for t in pts}
         tc = time.time()
-        cpu = cpu_baseline(P, W["n_iter"], W["early"], states, threads, points)
+        cpu = cpu_baseline(P, W["n_iter"], W["early"], states, threads, pts)
         log(f"bench: cpu baseline {cpu and cpu['value']} it/s in {time.time() - tc:.1f}s")
+    del R
+    if args.workload == "mnist70k" and not args.no_scale:
+        ts = time.time()
+        extra["gmm1m"] = scale_leg(args.warmup)
+        log(f"bench: gmm1m {json.dumps(extra['gmm1m'])} ({time.time() - ts:.1f}s)")
     serving = None
     if not args.no_serving and args.workload == "mnist70k":
         ts = time.time()

@@ -59,6 +59,9 @@ int mg_profile_enable(int on);
 int mg_profile_reset(void);
 int mg_profile_num_stages(void);
 int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* count, double* algo_bytes);
+/* The same per padded FFT size M (the FFT stages' kernels are compiled per size; other stages report M = 0): the
+ * number of sizes seen in *n_sizes, and up to max_sizes of them in M / total_ms / count / algo_bytes. */
+int mg_profile_sized(int stage, int max_sizes, int* n_sizes, int* M, double* total_ms, int64_t* count, double* algo_bytes);
 /* Grid sizes of the profiled iterations: mean and largest live side N = n_interp * n_boxes, largest M. */
 int mg_profile_grid(double* mean_N, int* max_N, int* max_M);
 
@@ -208,6 +211,20 @@ typedef int (*mg_tsne_callback)(int it, const void* d_Y, int64_t n, void* user);
 int mg_tsne_optimize(int prec, const mg_tsne_params* params, const int32_t* d_indptr, const int32_t* d_indices,
                      const void* d_data, int64_t nnz, int64_t n, void* d_Y, void* d_U, void* d_G, const int32_t* d_pairs,
                      int m_pairs, int it0, int it1, double* h_sumQ, mg_tsne_callback cb, void* user);
+
+/* A reusable optimize_tsne loop over one CSR P (device arrays that must outlive the session): the CSR is validated
+ * once at creation, and the loop's device buffers, streams and kernel-spectra cache persist between runs, so a run
+ * costs only its iterations. mg_tsne_session_run(s, Y, U, G, it0, it1, ...) is mg_tsne_optimize with the session's
+ * P, schedule and pairs; consecutive runs of [a, b) and [b, c) give the state of one run of [a, c). No counterpart in
+ * the reference (its loop is a Python function over per-call kernels); for callers that run a schedule in pieces
+ * (checkpointing, timing, interleaving several layouts). A session is not re-entrant. */
+typedef struct mg_tsne_session mg_tsne_session;
+int mg_tsne_session_create(mg_tsne_session** out, int prec, const mg_tsne_params* params, const int32_t* d_indptr,
+                           const int32_t* d_indices, const void* d_data, int64_t nnz, int64_t n, const int32_t* d_pairs,
+                           int m_pairs);
+int mg_tsne_session_run(mg_tsne_session* s, void* d_Y, void* d_U, void* d_G, int it0, int it1, double* h_sumQ,
+                        mg_tsne_callback cb, void* user);
+int mg_tsne_session_destroy(mg_tsne_session* s);
 
 /* Batched form: `count` (<= 1024) independent embeddings (CSR P, state Y / U / G of their own sizes, device
  * arrays) on one schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs;

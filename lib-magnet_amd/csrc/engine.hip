@@ -150,17 +150,27 @@ struct Prof {
     double pend_bytes[NSTAGE] = {};
     long long count[NSTAGE] = {};
     double n_sum = 0; int n_max = 0, m_max = 0; long long iters = 0;   // live grid side N, padded M
+    // the same per stage and padded FFT size (size 0 for stages whose kernels do not depend on it)
+    int pend_M[NSTAGE] = {};
+    struct Sized { int stage, M; double total, bytes; long long count; };
+    std::vector<Sized> sized;
+    void add_sized(int s, int M, double ms, double by) {
+        for (auto& z : sized)
+            if (z.stage == s && z.M == M) { z.total += ms; z.bytes += by; ++z.count; return; }
+        sized.push_back({s, M, ms, by, 1});
+    }
     void grid(int N, int M) { if (!on) return; n_sum += N; n_max = N > n_max ? N : n_max; m_max = M > m_max ? M : m_max; ++iters; }
     void begin(int s, hipStream_t st) {
         if (!on) return;
         if (!ev[2 * s]) { MG_HIP(hipEventCreate(&ev[2 * s])); MG_HIP(hipEventCreate(&ev[2 * s + 1])); }
         MG_HIP(hipEventRecord(ev[2 * s], st));
     }
-    void end(int s, hipStream_t st, double algo_bytes = 0) {
+    void end(int s, hipStream_t st, double algo_bytes = 0, int M = 0) {
         if (!on) return;
         MG_HIP(hipEventRecord(ev[2 * s + 1], st));
         pending[s] = true;
         pend_bytes[s] = algo_bytes;
+        pend_M[s] = M;
     }
     // after the stream has passed every recorded event
     void collect(int it) {
@@ -170,12 +180,14 @@ struct Prof {
             float ms = 0.f;
             MG_HIP(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
             total[s] += ms; ++count[s]; pending[s] = false; bytes[s] += pend_bytes[s];
+            add_sized(s, pend_M[s], ms, pend_bytes[s]);
             if (debug) fprintf(stderr, "  [it %d] %s %.3f ms\n", it, kStageNames[s], ms);
         }
     }
     void reset() {
         for (int s = 0; s < NSTAGE; ++s) { total[s] = 0; bytes[s] = 0; count[s] = 0; pending[s] = false; }
         n_sum = 0; n_max = 0; m_max = 0; iters = 0;
+        sized.clear();
     }
 };
 static Prof g_prof;
@@ -189,6 +201,9 @@ template <class T> struct Repulsion {
     double kwidth = 0.7, kdof = 1.0;
     int M_in = 0;
     std::vector<double> kvals;   // (2, M_in, M_in)
+    // node spacing of a value table's grid: the centred product set's odd tables x K2, y K2 need the offsets in
+    // layout units, which a value table does not carry (set from the grid of each compute; -1 when unused)
+    double vdelta = -1.0;
     struct Spec { int M; double delta, dof; DevBuf K[4]; unsigned long use; };
     // FP32: the centred product set (node-relative charges); FP64: the reference's
     static constexpr bool centred = sizeof(T) == 4;
@@ -233,7 +248,7 @@ template <class T> struct Repulsion {
     }
 
     bool have_spectra(int M) const {
-        const double delta = tsne_kernel ? kwidth / ni : -1.0;
+        const double delta = tsne_kernel ? kwidth / ni : vdelta;
         for (const auto& s : cache)
             if (s.M == M && s.delta == delta && s.dof == kdof) return true;
         return false;
@@ -243,7 +258,7 @@ template <class T> struct Repulsion {
     Spec& spectra(int M, hipStream_t st, bool* fresh = nullptr) {
         require(have_kernel, "kernels not set");
         if (fresh) *fresh = false;
-        const double delta = tsne_kernel ? kwidth / ni : -1.0;
+        const double delta = tsne_kernel ? kwidth / ni : vdelta;
         for (auto& s : cache)
             if (s.M == M && s.delta == delta && s.dof == kdof) { s.use = ++tick; return s; }
         const int Mh = M / 2 + 1;
@@ -338,6 +353,7 @@ template <class T> struct Repulsion {
         if (!tsne_kernel) require(M_in >= 2 * g.N - 1, "padded size below 2N - 1 (aliased convolution) is not supported", MG_ERR_UNSUP);
         const int M = device_M(g.N);
         require(M > 0, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+        if (!tsne_kernel) vdelta = centred ? g.width_d / ni : -1.0;
         ensure(n, g.N, M);
         Prof& P = g_prof;
         const double ts = sizeof(T), NN = (double)g.N * g.N, cs = 2 * ts, Mh = M / 2 + 1, dn = (double)n;
@@ -356,7 +372,7 @@ template <class T> struct Repulsion {
         P.begin(ST_SPECTRA, sst);
         bool fresh = false;
         Spec& s = spectra(M, sst, &fresh);
-        P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0);   // tables written when the grid changed
+        P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0, M);   // tables written when the grid changed
         if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
@@ -376,21 +392,21 @@ template <class T> struct Repulsion {
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
         P.begin(ST_ROWS_FWD, st);
         launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
-        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs);
+        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs, M);
         P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
         launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
-        P.end(ST_TR_FWD, st, 0);
+        P.end(ST_TR_FWD, st, 0, M);
         if (side_run) MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, st);
-        P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs);
+        P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs, M);
         P.begin(ST_TR_INV, st);
         launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
-        P.end(ST_TR_INV, st, 0);
+        P.end(ST_TR_INV, st, 0, M);
         P.begin(ST_ROWS_INV, st);
         launch_rows_inverse<T>(b.Tr, g.N, M, b.phi, st);
-        P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts);
+        P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts, M);
         P.begin(ST_GATHER, st);
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
         P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
@@ -489,7 +505,7 @@ template <class T> struct Optimizer {
     // checks, buffers, the starting layout's bbox (read back), and the first iteration's attraction
     void begin(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
                T* G_, const int32_t* pairs_, int m_, int it0_, int it1_, double* h_sumQ_, mg_tsne_callback cb_, void* user_,
-               hipStream_t st_) {
+               hipStream_t st_, bool check_csr = true) {
         p = p_; ip = ip_; ix = ix_; data = data_; nnz = nnz_; n = n_; Y = Y_; U = U_; G = G_; pairs = pairs_; m = m_;
         uY = Y_; uU = U_; uG = G_; cip = ip_; cix = ix_; cP = data_; since = 0;
         it0 = it0_; it1 = it1_; h_sumQ = h_sumQ_; cb = cb_; user = user_; st = st_;
@@ -499,7 +515,7 @@ template <class T> struct Optimizer {
         require(p->dof > 0, "dof must be positive");
         require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
         require(!pairs || m >= 1, "pairs must be (n, m) with m >= 1");
-        validate_csr(ip, ix, n, nnz, st);
+        if (check_csr) validate_csr(ip, ix, n, nnz, st);
         rep.have_kernel = true; rep.tsne_kernel = true; rep.kdof = p->dof; rep.kwidth = p->box_width;
         rep.overlap = true;
         lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
@@ -600,9 +616,9 @@ template <class T> struct Optimizer {
 
     int run(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
             T* G_, const int32_t* pairs_, int m_, int it0_, int it1_, double* h_sumQ_, mg_tsne_callback cb_, void* user_,
-            hipStream_t st_) {
+            hipStream_t st_, bool check_csr = true) {
         if (getenv("MAGNET_DEBUG")) { g_prof.on = true; g_prof.debug = true; }
-        begin(p_, ip_, ix_, data_, nnz_, n_, Y_, U_, G_, pairs_, m_, it0_, it1_, h_sumQ_, cb_, user_, st_);
+        begin(p_, ip_, ix_, data_, nnz_, n_, Y_, U_, G_, pairs_, m_, it0_, it1_, h_sumQ_, cb_, user_, st_, check_csr);
         int stopped = 0, it_end = it1;
         for (int it = it0; it < it1; ++it) {
             issue(it);
@@ -987,6 +1003,19 @@ template <class T> void optimize_batch(const mg_tsne_params* p, int count, const
 // ================================================================================== C-ABI
 using namespace mg;
 
+// A reusable whole loop: the CSR is validated once, and the optimiser's buffers, streams and spectra cache persist
+// from one run to the next.
+struct mg_tsne_session {
+    int prec = MG_F32;
+    mg_tsne_params p{};
+    const int32_t *ip = nullptr, *ix = nullptr, *pairs = nullptr;
+    const void* data = nullptr;
+    int64_t nnz = 0, n = 0;
+    int m = 0;
+    std::unique_ptr<Optimizer<float>> f32;
+    std::unique_ptr<Optimizer<double>> f64;
+};
+
 struct mg_fftconv {
     int prec;
     std::unique_ptr<Repulsion<float>> f32;
@@ -1079,6 +1108,20 @@ int mg_profile_get(int stage, const char** name, double* total_ms, int64_t* coun
     if (total_ms) *total_ms = g_prof.total[stage];
     if (algo_bytes) *algo_bytes = g_prof.bytes[stage];
     if (count) *count = g_prof.count[stage];
+    MG_API_END
+}
+
+int mg_profile_sized(int stage, int max_sizes, int* n_sizes, int* M, double* total_ms, int64_t* count, double* algo_bytes) {
+    MG_API_BEGIN
+    require(stage >= 0 && stage < NSTAGE, "stage out of range");
+    require(n_sizes != nullptr && (max_sizes <= 0 || (M && total_ms && count && algo_bytes)), "null output pointer");
+    int k = 0;
+    for (const auto& z : g_prof.sized) {
+        if (z.stage != stage) continue;
+        if (k < max_sizes) { M[k] = z.M; total_ms[k] = z.total; count[k] = z.count; algo_bytes[k] = z.bytes; }
+        ++k;
+    }
+    *n_sizes = k;
     MG_API_END
 }
 
@@ -1365,5 +1408,39 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* p, const int32_t* ip, const
     }
     MG_API_END
 }
+
+int mg_tsne_session_create(mg_tsne_session** out, int prec, const mg_tsne_params* p, const int32_t* ip, const int32_t* ix,
+                           const void* data, int64_t nnz, int64_t n, const int32_t* pairs, int m) {
+    MG_API_BEGIN
+    check_prec(prec);
+    require(out != nullptr, "null output pointer");
+    *out = nullptr;
+    check_schedule(p);
+    require(n >= 1, "n must be >= 1");
+    require(nnz == 0 || data != nullptr, "data must be a device array");
+    require(!pairs || m >= 1, "pairs must be (n, m) with m >= 1");
+    validate_csr(ip, ix, n, nnz, lib_stream());
+    std::unique_ptr<mg_tsne_session> s(new mg_tsne_session());
+    s->prec = prec; s->p = *p; s->ip = ip; s->ix = ix; s->data = data; s->nnz = nnz; s->n = n; s->pairs = pairs; s->m = m;
+    if (prec == MG_F64) s->f64.reset(new Optimizer<double>(p->n_interp));
+    else s->f32.reset(new Optimizer<float>(p->n_interp));
+    *out = s.release();
+    MG_API_END
+}
+
+int mg_tsne_session_run(mg_tsne_session* s, void* Y, void* U, void* G, int it0, int it1, double* h_sumQ, mg_tsne_callback cb,
+                        void* user) {
+    MG_API_BEGIN
+    require(s != nullptr, "null session");
+    if (s->prec == MG_F64)
+        s->f64->run(&s->p, s->ip, s->ix, (const double*)s->data, s->nnz, s->n, (double*)Y, (double*)U, (double*)G, s->pairs, s->m,
+                    it0, it1, h_sumQ, cb, user, lib_stream(), false);
+    else
+        s->f32->run(&s->p, s->ip, s->ix, (const float*)s->data, s->nnz, s->n, (float*)Y, (float*)U, (float*)G, s->pairs, s->m,
+                    it0, it1, h_sumQ, cb, user, lib_stream(), false);
+    MG_API_END
+}
+
+int mg_tsne_session_destroy(mg_tsne_session* s) { MG_API_BEGIN delete s; MG_API_END }
 
 }  // extern "C"

@@ -53,6 +53,8 @@ def lib():
             "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
             "mg_profile_enable": ([i32], i32), "mg_fft_size": ([i32], i32),
             "mg_profile_grid": ([C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
+            "mg_profile_sized": ([i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double)], i32),
             "mg_profile_get": ([i32, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_double)], i32),
             "mg_tsne_attraction": ([i32, vp, vp, vp, i64, vp, i64, i32, d, vp], i32),
@@ -72,6 +74,9 @@ def lib():
             "mg_gather_forces_2d": ([i32, vp, i64, vp, i32, d, d, d, i32, i32, vp, C.POINTER(C.c_double)], i32),
             "mg_tsne_optimize": ([i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, vp, vp, vp, i32, i32, i32,
                                   C.POINTER(C.c_double), CALLBACK, vp], i32),
+            "mg_tsne_session_create": ([C.POINTER(vp), i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, i32], i32),
+            "mg_tsne_session_run": ([vp, vp, vp, vp, i32, i32, C.POINTER(C.c_double), CALLBACK, vp], i32),
+            "mg_tsne_session_destroy": ([vp], i32),
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
@@ -201,6 +206,44 @@ def optimize(indptr, indices, data, Y, U, G, it0=0, it1=None, n_iter=750, early_
     return sq[: max(0, it1 - it0)]
 
 
+class Session:
+    """mg_tsne_session: one CSR P (device tensors kept alive by the object) and one schedule; run(Y, U, G, it0, it1)
+    advances a state in place and returns the per-iteration sumQ; buffers, streams and spectra persist between runs."""
+
+    def __init__(self, indptr, indices, data, prec=None, n_iter=750, early_iter=250, early_exaggeration=12.0,
+                 exaggeration=1.0, learning_rate=0.0, momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, pairs=None,
+                 mid_weight=0.0, mid_phase=0.5, min_boxes=20, max_boxes=1000, relabel_every=0):
+        self.h = C.c_void_p()
+        self._keep = (indptr, indices, data, pairs)
+        self.n_iter = n_iter
+        self.p = Params(n_iter, early_iter, early_exaggeration, exaggeration, learning_rate, momentum[0], momentum[1],
+                        n_interp, min_boxes, max_boxes, box_width, dof, mid_weight, mid_phase, relabel_every)
+        prec = prec_of(data) if prec is None else prec
+        check(lib().mg_tsne_session_create(C.byref(self.h), prec, C.byref(self.p), ptr(indptr), ptr(indices), ptr(data),
+                                           indices.numel(), indptr.numel() - 1, ptr(pairs),
+                                           pairs.shape[1] if pairs is not None else 0))
+
+    def run(self, Y, U, G, it0=0, it1=None, callback=None):
+        import numpy as np
+        it1 = self.n_iter if it1 is None else it1
+        sq = np.zeros(max(1, it1 - it0))
+        cb = CALLBACK(callback) if callback else CALLBACK(0)
+        check(lib().mg_tsne_session_run(self.h, ptr(Y), ptr(U), ptr(G), int(it0), int(it1),
+                                        sq.ctypes.data_as(C.POINTER(C.c_double)), cb, None))
+        return sq[: max(0, it1 - it0)]
+
+    def close(self):
+        if self.h:
+            lib().mg_tsne_session_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.0, exaggeration=1.0, learning_rate=0.0,
                    momentum=(0.5, 0.8), n_interp=3, box_width=0.7, dof=1.0, min_boxes=20, max_boxes=1000, n_streams=4,
                    n_threads=1, sumq=False):
@@ -293,6 +336,23 @@ def profile_read():
         name, tot, cnt, by = C.c_char_p(), C.c_double(), C.c_int64(), C.c_double()
         check(lib().mg_profile_get(s, C.byref(name), C.byref(tot), C.byref(cnt), C.byref(by)))
         out[name.value.decode()] = (tot.value, cnt.value, by.value)
+    return out
+
+
+def profile_sized():
+    """{(stage, M): (total_ms, launches, algorithmic_bytes)} per padded FFT size M (0 for size-independent stages)."""
+    out = {}
+    for s in range(lib().mg_profile_num_stages()):
+        name, tot, cnt, by = C.c_char_p(), C.c_double(), C.c_int64(), C.c_double()
+        check(lib().mg_profile_get(s, C.byref(name), C.byref(tot), C.byref(cnt), C.byref(by)))
+        k = C.c_int()
+        check(lib().mg_profile_sized(s, 0, C.byref(k), None, None, None, None))
+        if k.value == 0:
+            continue
+        Ms, T, N, B = (C.c_int * k.value)(), (C.c_double * k.value)(), (C.c_int64 * k.value)(), (C.c_double * k.value)()
+        check(lib().mg_profile_sized(s, k.value, C.byref(k), Ms, T, N, B))
+        for j in range(k.value):
+            out[(name.value.decode(), Ms[j])] = (T[j], N[j], B[j])
     return out
 
 

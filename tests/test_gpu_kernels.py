@@ -337,18 +337,25 @@ def test_repulsion_is_deterministic():
     assert s1 == s2 and np.array_equal(F1, F2)
 
 
-def test_custom_kernel_values_equal_tsne_kernels():
-    Y = layout(5000, 9, 3.0)
-    xmin, ymin, width, nb = O.grid(Y)
-    N = nb * 3
-    M = 2 * N + 6   # a 5-smooth-like size, resampled onto the device torus
-    off = np.arange(M); off = np.where(off <= M // 2, off, off - M) * (width / 3)
-    r2 = off[:, None] ** 2 + off[None, :] ** 2
-    conv = capi.FFTConv(capi.MG_F64, M, 3)
-    conv.set_values(1 / (1 + r2), 1 / (1 + r2) ** 2)
-    s, F = conv.compute(dev(Y), xmin, ymin, width, nb)
-    so, Fo = O.repulsion(Y, xmin, ymin, width, nb, 3)
-    assert abs(s - so) / so < 1e-12 and rel(host(F), Fo) < 1e-11
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_custom_kernel_values_equal_tsne_kernels(prec):
+    """A kernel given by its values (the form FFTConv2D.set_kernels hands over) against the oracle's t-SNE kernels:
+    FP64 to rounding; FP32 to 1e-5, which needs the node spacing of each compute's grid for the centred product
+    set's odd tables (a value table does not carry it); two layouts of different grid spacing."""
+    for scale in (3.0, 9.0):
+        Y = layout(5000, 9, scale)
+        xmin, ymin, width, nb = O.grid(Y)
+        N = nb * 3
+        M = 2 * N + 6   # a 5-smooth-like size, resampled onto the device torus
+        off = np.arange(M); off = np.where(off <= M // 2, off, off - M) * (width / 3)
+        r2 = off[:, None] ** 2 + off[None, :] ** 2
+        conv = capi.FFTConv(capi.MG_F64 if prec == "fp64" else capi.MG_F32, M, 3)
+        conv.set_values(1 / (1 + r2), 1 / (1 + r2) ** 2)
+        Yd = dev(Y if prec == "fp64" else Y.astype(np.float32))
+        s, F = conv.compute(Yd, xmin, ymin, width, nb)
+        so, Fo = O.repulsion(Y if prec == "fp64" else Y.astype(np.float32).astype(np.float64), xmin, ymin, width, nb, 3)
+        tol_s, tol_F = (1e-12, 1e-11) if prec == "fp64" else (1e-5, 1e-5)
+        assert abs(s - so) / so < tol_s and rel(host(F), Fo) < tol_F, (scale, s, so, rel(host(F), Fo))
 
 
 def test_errors_are_reported():

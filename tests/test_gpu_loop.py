@@ -299,3 +299,24 @@ def test_fp64_loop_with_isolated_points_matches_oracle():
     sqo = O.optimize(P.indptr, P.indices, data, Yo, Uo, Go, 0, 25, **SCHED)
     assert np.abs(sq / sqo - 1).max() < 1e-10
     assert rel(host(st.Y), Yo) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_session_runs_in_pieces_equal_one_call(dtype):
+    """mg_tsne_session: a schedule run in pieces ([0, 17), [17, 60), [60, 90)) on one session gives the state and
+    sumQ of one mg_tsne_optimize call over [0, 90), bit for bit; a second state from the same start on the same
+    session (its buffers and spectra cache warm) gives them again."""
+    import torch
+    P, Y0, _ = problem(4000, 20, 5, 3)
+    kw = dict(n_iter=90, early_iter=30)
+    one = State(P, Y0, dtype)
+    sq_one = one.run(0, 90, **kw)
+    st = State(P, Y0, dtype)
+    sess = capi.Session(st.ip, st.ix, st.data, **kw)
+    for rep in range(2):
+        Y, U, G = dev(Y0.astype(dtype)), dev(np.zeros_like(Y0, dtype)), dev(np.ones_like(Y0, dtype))
+        sq = np.concatenate([sess.run(Y, U, G, a, b) for a, b in ((0, 17), (17, 60), (60, 90))])
+        torch.cuda.synchronize()
+        assert torch.equal(Y, one.Y) and torch.equal(U, one.U) and torch.equal(G, one.G), rep
+        assert np.array_equal(sq, sq_one), rep
+    sess.close()
