@@ -1,5 +1,6 @@
 // CPU oracle for the full-batch t-SNE gradient pass -- TEST INFRASTRUCTURE ONLY (see tsne_oracle.h).
-// Single-threaded and written for clarity: every loop follows the reference file:line cited above it,
+// Written for clarity (the FFT transforms alone split their independent rows and columns over host threads,
+// which changes no result): every loop follows the reference file:line cited above it,
 // in the same arithmetic order, so that the float form can be compared bit for bit with the compiled
 // reference where the reference is itself deterministic (attraction, step, update, spring).
 // Build: g++ -O3 -ffp-contract=off (no fused multiply-add, like the reference's x86-64 build).
@@ -9,6 +10,7 @@
 #include <vector>
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace {
 
@@ -33,9 +35,21 @@ template <class T> inline void lagrange(T u, int ni, T* w) {
     }
 }
 
-// ---- 1-D complex DFT, sign -1 forward, +1 inverse, unnormalised. Radix-2 when L is a power of two,
-// the O(L^2) definition otherwise (used only for small non-power-of-two M in padding tests).
-void dft(cd* a, int L, int sign) {
+// ---- 1-D complex DFT, sign -1 forward, +1 inverse, unnormalised. Radix-2 when L is a power of two, with the
+// twiddles exp(-2 pi i k / L) computed once per transform set (each from its own cos / sin; the inverse uses their
+// conjugates); the O(L^2) definition otherwise (used only for small non-power-of-two M in padding tests).
+struct Twiddles {
+    int L = 0;
+    std::vector<cd> w;   // exp(-2 pi i k / L), k < L / 2
+    explicit Twiddles(int L_) : L(L_) {
+        if ((L & (L - 1)) != 0) return;
+        w.resize(L / 2);
+        for (int k = 0; k < L / 2; ++k) { const double ang = -2.0 * M_PI * k / L; w[k] = cd(std::cos(ang), std::sin(ang)); }
+    }
+};
+
+void dft(cd* a, const Twiddles& T, int sign) {
+    const int L = T.L;
     if (L <= 1) return;
     if ((L & (L - 1)) == 0) {
         for (int i = 1, j = 0; i < L; ++i) {      // bit-reversal permutation
@@ -44,13 +58,16 @@ void dft(cd* a, int L, int sign) {
             j ^= bit;
             if (i < j) std::swap(a[i], a[j]);
         }
-        std::vector<cd> tw(L / 2);   // exp(sign 2 pi i k / L), each from its own cos / sin
-        for (int k = 0; k < L / 2; ++k) { const double ang = sign * 2.0 * M_PI * k / L; tw[k] = cd(std::cos(ang), std::sin(ang)); }
         for (int len = 2; len <= L; len <<= 1) {
             const int step = L / len;
             for (int i = 0; i < L; i += len) {
                 for (int k = 0; k < len / 2; ++k) {
-                    const cd u = a[i + k], v = a[i + k + len / 2] * tw[(size_t)k * step];
+                    const cd t = T.w[(size_t)k * step];
+                    const double tr = t.real(), ti = sign < 0 ? t.imag() : -t.imag();
+                    const cd x = a[i + k + len / 2];
+                    // the complex product written out: the finite-value result of operator*, without its
+                    // library call for infinities
+                    const cd u = a[i + k], v(x.real() * tr - x.imag() * ti, x.real() * ti + x.imag() * tr);
                     a[i + k] = u + v; a[i + k + len / 2] = u - v;
                 }
             }
@@ -69,15 +86,45 @@ void dft(cd* a, int L, int sign) {
     std::copy(out.begin(), out.end(), a);
 }
 
-// 2-D transform of an M x M row-major grid: rows (last axis) then columns.
-void dft2(cd* g, int M, int sign) {
-    std::vector<cd> col(M);
-    for (int r = 0; r < M; ++r) dft(g + (size_t)r * M, M, sign);
-    for (int c = 0; c < M; ++c) {
-        for (int r = 0; r < M; ++r) col[r] = g[(size_t)r * M + c];
-        dft(col.data(), M, sign);
-        for (int r = 0; r < M; ++r) g[(size_t)r * M + c] = col[r];
-    }
+// f(i) for i in [0, count), split over host threads by index (each i is independent, so the result does not
+// depend on the thread count)
+template <class F> void parallel_for(int count, F f) {
+    const int nt = std::max(1, std::min({count, 16, (int)std::thread::hardware_concurrency()}));
+    if (nt <= 1) { for (int i = 0; i < count; ++i) f(i); return; }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] { for (int i = t; i < count; i += nt) f(i); });
+    for (auto& th : pool) th.join();
+}
+
+// The column transforms of an M x M row-major grid, eight adjacent columns copied out together (whole cache
+// lines) and transformed one by one.
+void dft_columns(cd* g, int M, int sign, const Twiddles& T) {
+    constexpr int CB = 8;
+    parallel_for((M + CB - 1) / CB, [&](int blk) {
+        const int c0 = blk * CB, nc = std::min(CB, M - c0);
+        std::vector<cd> cols((size_t)CB * M);
+        for (int r = 0; r < M; ++r)
+            for (int j = 0; j < nc; ++j) cols[(size_t)j * M + r] = g[(size_t)r * M + c0 + j];
+        for (int j = 0; j < nc; ++j) dft(cols.data() + (size_t)j * M, T, sign);
+        for (int r = 0; r < M; ++r)
+            for (int j = 0; j < nc; ++j) g[(size_t)r * M + c0 + j] = cols[(size_t)j * M + r];
+    });
+}
+
+// 2-D transform of an M x M row-major grid: rows (last axis) then columns; only rows [0, live_rows) of the input
+// may be non-zero (the zero rows transform to zero and are skipped).
+void dft2(cd* g, int M, int sign, int live_rows) {
+    const Twiddles T(M);
+    parallel_for(live_rows, [&](int r) { dft(g + (size_t)r * M, T, sign); });
+    dft_columns(g, M, sign, T);
+}
+
+// Inverse-direction 2-D transform of which only rows [0, out_rows) are needed: columns first, then those rows.
+void dft2_rows_out(cd* g, int M, int sign, int out_rows) {
+    const Twiddles T(M);
+    dft_columns(g, M, sign, T);
+    parallel_for(out_rows, [&](int r) { dft(g + (size_t)r * M, T, sign); });
 }
 
 inline double kern1(double r2, double dof) { return std::pow(1.0 + r2 / dof, -dof); }
@@ -93,7 +140,7 @@ void convolve_fft(const std::vector<double>& W, int N, int M, const double* K1h,
         spec[c].assign((size_t)M * M, cd(0, 0));
         for (int gx = 0; gx < N; ++gx)
             for (int gy = 0; gy < N; ++gy) spec[c][(size_t)gx * M + gy] = W[((size_t)c * N + gx) * N + gy];
-        dft2(spec[c].data(), M, -1);
+        dft2(spec[c].data(), M, -1, N);
     }
     // real spectra given on (M, Mh); the kernels are even in both axes, so K(kx, ky) = K(kx, M - ky)
     auto kval = [&](const double* Kh, int kx, int ky) { return Kh[(size_t)kx * Mh + (ky < Mh ? ky : M - ky)]; };
@@ -103,9 +150,10 @@ void convolve_fft(const std::vector<double>& W, int N, int M, const double* K1h,
     for (int o = 0; o < 4; ++o) {
         const int src = o == 0 ? 0 : o - 1;
         const double* Kh = o == 0 ? K1h : K2h;
-        for (int kx = 0; kx < M; ++kx)
+        parallel_for(M, [&](int kx) {
             for (int ky = 0; ky < M; ++ky) prod[(size_t)kx * M + ky] = spec[src][(size_t)kx * M + ky] * kval(Kh, kx, ky);
-        dft2(prod.data(), M, +1);
+        });
+        dft2_rows_out(prod.data(), M, +1, N);
         for (int gx = 0; gx < N; ++gx)
             for (int gy = 0; gy < N; ++gy) phi[((size_t)o * N + gx) * N + gy] = prod[(size_t)gx * M + gy].real() * scale;
     }
@@ -342,7 +390,7 @@ void or_lagrange_f64(double u, int ni, double* w) { lagrange<double>(u, ni, w); 
 void or_kernel_spectra(int M, double delta, double dof, int round_f32, double* K1h, double* K2h) {
     const int Mh = M / 2 + 1;
     std::vector<cd> g1((size_t)M * M), g2((size_t)M * M);
-    for (int a = 0; a < M; ++a) {
+    parallel_for(M, [&](int a) {
         const double ox = (a <= M / 2 ? a : a - M) * delta;
         for (int b = 0; b < M; ++b) {
             const double oy = (b <= M / 2 ? b : b - M) * delta;
@@ -351,8 +399,8 @@ void or_kernel_spectra(int M, double delta, double dof, int round_f32, double* K
             if (round_f32) { k1 = (float)k1; k2 = (float)k2; }
             g1[(size_t)a * M + b] = k1; g2[(size_t)a * M + b] = k2;
         }
-    }
-    dft2(g1.data(), M, -1); dft2(g2.data(), M, -1);
+    });
+    dft2(g1.data(), M, -1, M); dft2(g2.data(), M, -1, M);
     for (int a = 0; a < M; ++a)
         for (int b = 0; b < Mh; ++b) { K1h[(size_t)a * Mh + b] = g1[(size_t)a * M + b].real(); K2h[(size_t)a * Mh + b] = g2[(size_t)a * M + b].real(); }
 }
