@@ -50,6 +50,15 @@ int mg_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 /* HBM bytes free / total on the current device (serving admission). */
 int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
 
+/* Launch-shape choices that change no result, process-wide (no counterpart in the reference; for tuning sweeps):
+ *   "attr_window"  the 2-D attraction's LDS-window sweep: 1 on, 0 off, -1 (default) from 262,144 points on;
+ *   "attr_rows"    rows per block of that sweep (default 256);
+ *   "attr_win_kb"  its window of positions in LDS: 64 (default) or 128 KiB, or 0 for none (rows only);
+ *   "attr_tpb"     its threads per block: 1024 (default), 512 or 256.
+ * MG_ERR_ARG for an unknown name or a value out of range. */
+int mg_tune_set(const char* name, int value);
+int mg_tune_get(const char* name, int* value);
+
 /* Per-stage timing of mg_tsne_optimize with HIP events on the working stream (the reference prints
  * per-stage laps of FFTConv2D::compute under MAGNET_DEBUG, src/fftconv.cpp:69-70; setting MAGNET_DEBUG
  * here enables this and prints every stage of every iteration to stderr). Stages: attraction,
@@ -225,6 +234,13 @@ int mg_tsne_session_create(mg_tsne_session** out, int prec, const mg_tsne_params
 int mg_tsne_session_run(mg_tsne_session* s, void* d_Y, void* d_U, void* d_G, int it0, int it1, double* h_sumQ,
                         mg_tsne_callback cb, void* user);
 int mg_tsne_session_destroy(mg_tsne_session* s);
+/* Per-iteration diagnostics of the session's next runs (no counterpart in the reference): for every iteration of
+ * [it0, it1), h_kl receives KL(P || Q) = sum p log p + dof sum p log(1 + d^2 / dof) + (sum p) log Z on the layout
+ * the iteration starts from (Z its sumQ), and h_grad_norm the L2 norm of its gradient (rho A - F / Z, plus the
+ * mid-range spring when pairs are set). Both in FP64 with compensated sums over the points and edges in original
+ * order, for either precision; an extra CSR sweep per iteration. Host arrays of it1 - it0 doubles; either may be
+ * NULL, both NULL turns the diagnostics off. */
+int mg_tsne_session_trace(mg_tsne_session* s, double* h_kl, double* h_grad_norm);
 
 /* Batched form: `count` (<= 1024) independent embeddings (CSR P, state Y / U / G of their own sizes, device
  * arrays) on one schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs;

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <string>
 #include <stdexcept>
+#include <cmath>
 
 namespace mg {
 
@@ -79,6 +80,29 @@ template <class T> __device__ inline int box_key(T x, T y, const GridT<T>& g) {
     const int by = min(g.nb - 1, max(0, (int)uy));
     return bx * g.nb + by;
 }
+
+// ---- compensated FP64 sums: a running sum s and the rounding error c it has dropped (Neumaier's variant of Kahan
+// summation); two such pairs combine through an error-free TwoSum of their sums. The value is s + c.
+struct CSum { double s, c; };
+__device__ __host__ inline void csum_add(CSum& a, double x) {
+    const double t = a.s + x;
+    a.c += (fabs(a.s) >= fabs(x)) ? (a.s - t) + x : (x - t) + a.s;
+    a.s = t;
+}
+__device__ __host__ inline CSum csum_merge(CSum a, CSum b) {
+    const double t = a.s + b.s, bp = t - a.s;
+    const double e = (a.s - (t - bp)) + (b.s - bp);
+    return {t, a.c + b.c + e};
+}
+
+// ---- launch-shape choices that change no result (set through mg_tune_set; the defaults are measured choices)
+struct Tunables {
+    int attr_window = -1;   // LDS window sweep of the 2-D attraction: 1 on, 0 off, -1 from 262,144 points on
+    int attr_rows = 256;    // rows per block of the window sweep
+    int attr_win_kb = 64;   // LDS window of the sweep: 128 or 64 KiB of positions (0: no window, rows only)
+    int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
+};
+Tunables& tunables();
 
 // ---- scratch of one repulsion engine (device pointers), sized by the host
 template <class T> struct RepBuffers {
@@ -155,6 +179,16 @@ template <class T> void launch_perm_state(const T* Y, const T* U, const T* G, co
                                           T* Yn, T* Un, T* Gn, hipStream_t st);
 template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
                                          hipStream_t st);
+// per-iteration diagnostics (KL divergence, gradient norm) in FP64 with compensated sums; scratch of
+// trace_scratch_doubles() doubles, whose last part launch_trace_plogp fills once per run
+int trace_scratch_doubles();
+template <class T> void launch_trace_plogp(const T* P, i64 nnz, double* scratch, hipStream_t st);
+template <class T> void launch_trace(const int32_t* ip, const int32_t* ix, const T* P, const T* Y, const int* inv, i64 n,
+                                     double dof, const T* A, const T* F, const T* d_inv, T rho, const double* zout,
+                                     double* scratch, double* out, hipStream_t st);
+// relabelling sort input: the Morton key of each point's box (original order), value its current position
+template <class T> void launch_morton_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
+                                           hipStream_t st);
 template <class T> void launch_unperm(const T* src, const int* inv, i64 n, T* dst, hipStream_t st);
 template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* valsB, int* hist, i64 n, int bits, hipStream_t st);
 

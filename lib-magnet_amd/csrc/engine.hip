@@ -28,6 +28,8 @@ template <class T> int launch_gather_forces(const T*, i64, const T*, int, double
 template <class T> void launch_spectral_products(const T*, const T*, const T*, i64, i64, T*, hipStream_t);
 
 static thread_local std::string g_err;
+static Tunables g_tunables;
+Tunables& tunables() { return g_tunables; }
 // The stream the entry points issue on: the calling thread's own (mg_set_thread_stream) if it set one, else
 // the process-wide one (mg_set_stream, default the null stream).
 static hipStream_t g_stream_all = nullptr;
@@ -328,7 +330,7 @@ template <class T> struct Repulsion {
         Tr.ensure_keep(4 * Mh * Nc * sizeof(cplx<T>), dead);
 
         phi.ensure_keep(4 * Nc * Nc * sizeof(T), dead);
-        zpart.ensure((size_t)gather_blocks(n) * sizeof(double));
+        zpart.ensure((size_t)2 * gather_blocks(n) * sizeof(double));   // compensated pairs
     }
 
     RepBuffers<T> bufs() {
@@ -434,6 +436,10 @@ template <class T> struct Optimizer {
     Repulsion<T> rep;
     DevBuf attr, F, grad, part, mb, bb_part, bb_counter, bb_out, sumq;
     PinnedBuf host_mb;
+    // per-iteration diagnostics (KL divergence, gradient norm), when the caller asks for them
+    double *h_kl = nullptr, *h_gnorm = nullptr;
+    DevBuf trace_scratch, trace_out;
+    bool tracing() const { return h_kl || h_gnorm; }
     Optimizer(int ni) : rep(0, ni) {}
 
     // one run: [it0, it1) of the schedule p on the state (Y, U, G), issued on stream st
@@ -490,9 +496,15 @@ template <class T> struct Optimizer {
         launch_identity_labels(dptr<int>(perm), dptr<int>(inv), n, st);
     }
 
-    // renumber the points in the order of the last box sort, and the CSR (from the caller's) to match
+    // renumber the points in the Morton order of their boxes on the last grid (ties in original point order), and the
+    // CSR (from the caller's) to match; the sort uses the repulsion's key buffers, free until its next box sort
     void relabel_now() {
-        const int* vals = rep.sorted_buf == 0 ? dptr<int>(rep.valsA) : dptr<int>(rep.valsB);
+        RepBuffers<T> b = rep.bufs();
+        int bits = 1;
+        while ((1 << bits) < rep.last_g.nb) ++bits;
+        launch_morton_keys<T>(Y, dptr<int>(inv), n, rep.last_g, b.keysA, b.valsA, st);
+        const int buf = launch_key_sort<T>(b.keysA, b.valsA, b.keysB, b.valsB, b.hist, n, 2 * bits, st);
+        const int* vals = buf == 0 ? b.valsA : b.valsB;
         launch_relabel<T>(vals, dptr<int>(perm), n, ip, ix, data, dptr<int>(order), dptr<int>(inv_new), dptr<int>(lenb),
                           dptr<int>(tiles), dptr<int32_t>(ipn), dptr<int32_t>(ixn), dptr<T>(Pn), st);
         launch_perm_state<T>(Y, U, G, dptr<int>(inv), dptr<int>(order), n, dptr<T>(Yt), dptr<T>(Ut), dptr<T>(Gt), st);
@@ -523,7 +535,7 @@ template <class T> struct Optimizer {
         attr.ensure((size_t)2 * n * sizeof(T));
         F.ensure((size_t)2 * n * sizeof(T));
         if (!fused) grad.ensure((size_t)2 * n * sizeof(T));
-        part.ensure((size_t)6 * 1024 * sizeof(double));
+        part.ensure((size_t)8 * 1024 * sizeof(double));   // compensated block partials
         unsigned* bc = reinterpret_cast<unsigned*>(bb_counter.ensure(64));
         if (!counters_zeroed) { MG_HIP(hipMemsetAsync(bc, 0, 64, st)); counters_zeroed = true; }
         mb.ensure(8 * sizeof(T));
@@ -543,6 +555,11 @@ template <class T> struct Optimizer {
             MG_HIP(hipEventCreateWithFlags(&ev_a1, hipEventDisableTiming));
         }
         a_pending = false;
+        if (tracing() && it1 > it0) {
+            trace_scratch.ensure((size_t)trace_scratch_doubles() * sizeof(double));
+            trace_out.ensure((size_t)2 * (it1 - it0) * sizeof(double));
+            launch_trace_plogp<T>(data, nnz, dptr<double>(trace_scratch), st);
+        }
         relabel = fused && p->relabel_every >= 0 && n >= 2 && it1 - it0 > 1;
         relabel_every = p->relabel_every > 0 ? p->relabel_every : 25;
         if (relabel) start_relabelling();
@@ -565,6 +582,10 @@ template <class T> struct Optimizer {
         rep.compute(Y, n, g, dptr<T>(F), dsum + (it - it0), st, labels);
         if (a_pending) { MG_HIP(hipStreamWaitEvent(st, ev_a1, 0)); a_pending = false; }   // attr of this iteration
         const T* dinv = dptr<T>(rep.inv_out);
+        double* dtrace = tracing() ? dptr<double>(trace_out) + 2 * (it - it0) : nullptr;
+        if (dtrace && fused)
+            launch_trace<T>(ip, ix, data, Y, labels, n, p->dof, dptr<T>(attr), dptr<T>(F), dinv, rho, dptr<double>(rep.zout),
+                            dptr<double>(trace_scratch), dtrace, st);
         g_prof.begin(ST_UPDATE, st);
         if (fused) {
             launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
@@ -573,6 +594,9 @@ template <class T> struct Optimizer {
             const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
             launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
                            dptr<T>(grad), st);
+            if (dtrace)
+                launch_trace<T>(ip, ix, data, Y, nullptr, n, p->dof, dptr<T>(grad), nullptr, nullptr, T(0),
+                                dptr<double>(rep.zout), dptr<double>(trace_scratch), dtrace, st);
             launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
                              dptr<double>(part), dptr<T>(mb), nullptr, st);
         }
@@ -603,6 +627,11 @@ template <class T> struct Optimizer {
     void finish(int it_end) {
         if (h_sumQ && it_end > it0)
             MG_HIP(hipMemcpyAsync(h_sumQ, dsum, (size_t)(it_end - it0) * sizeof(double), hipMemcpyDeviceToHost, st));
+        std::vector<double> tr;
+        if (tracing() && it_end > it0) {
+            tr.resize((size_t)2 * (it_end - it0));
+            MG_HIP(hipMemcpyAsync(tr.data(), trace_out.p, tr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
         if (relabel) {   // the state back to the caller's buffers, in original point order
             launch_unperm<T>(Y, dptr<int>(inv), n, uY, st);
             launch_unperm<T>(U, dptr<int>(inv), n, uU, st);
@@ -612,6 +641,10 @@ template <class T> struct Optimizer {
         MG_HIP(hipStreamSynchronize(aside));   // an attraction queued before a callback stopped the loop
         a_pending = false;
         rep.release_dead();
+        for (size_t k = 0; 2 * k < tr.size(); ++k) {
+            if (h_kl) h_kl[k] = tr[2 * k];
+            if (h_gnorm) h_gnorm[k] = tr[2 * k + 1];
+        }
     }
 
     int run(const mg_tsne_params* p_, const int32_t* ip_, const int32_t* ix_, const T* data_, i64 nnz_, i64 n_, T* Y_, T* U_,
@@ -753,7 +786,7 @@ template <class T> struct BatchEngine {
         MG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), st));
         zout.ensure((size_t)2 * count * sizeof(double));
         inv.ensure((size_t)4 * count * sizeof(T));
-        upart.ensure((size_t)count * 6 * 1024 * sizeof(double));
+        upart.ensure((size_t)count * 8 * 1024 * sizeof(double));
         T* mb = reinterpret_cast<T*>(mbd.ensure((size_t)8 * count * sizeof(T)));
         sumq.ensure((size_t)count * std::max(1, p->n_iter) * sizeof(double));
         table.ensure((size_t)count * sizeof(BInst<T>));
@@ -841,7 +874,7 @@ template <class T> struct BatchEngine {
             pphi[i] = ophi; ophi += al(4 * N * N);
             pit[i] = oit; oit += al(2 * nch);
             piv[i] = oiv; oiv += al(2 * nch * ni * 3 * ni);
-            pzp[i] = ozp; ozp += al(gather_blocks(n[i]));
+            pzp[i] = ozp; ozp += al(2 * gather_blocks(n[i]));   // compensated pairs
         }
         grow(aW, oW * sizeof(T), dead); grow(aWh, oWh * sizeof(cplx<T>), dead); grow(aPh, oPh * sizeof(cplx<T>), dead);
         grow(aTr, oTr * sizeof(cplx<T>), dead); grow(aphi, ophi * sizeof(T), dead); grow(aik, oit * sizeof(int), dead);
@@ -860,7 +893,7 @@ template <class T> struct BatchEngine {
             for (int t = 0; t < 4; ++t) { b.K[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; b.Kw[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; }
             b.zpart = dptr<double>(azp) + pzp[i]; b.zout = dptr<double>(zout) + 2 * i;
             b.inv_out = dptr<T>(inv) + 4 * i; b.sumq = dptr<double>(sumq) + (size_t)i * std::max(1, p->n_iter) + it;
-            b.upart = dptr<double>(upart) + (size_t)i * 6 * 1024; b.mb = dptr<T>(mbd) + 8 * i;
+            b.upart = dptr<double>(upart) + (size_t)i * 8 * 1024; b.mb = dptr<T>(mbd) + 8 * i;
             b.delta = sdelta[i]; b.spec_tmp = fresh[i] ? dptr<cplx<T>>(aspec) + psp[i] : nullptr;
         }
         MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
@@ -1038,7 +1071,7 @@ template <class T> static void step_impl(int mode, void* Y, void* U, void* G, co
     require(Y && U && G && A && (mode == 1 || F), "null array");
     DevBuf part, mb;
     launch_update<T>(mode, (T*)Y, (T*)U, (T*)G, (const T*)A, (const T*)F, nullptr, (T)inv, (T)rho, (T)lr, (T)mom, nullptr, 0,
-                     T(0), n, dim, (double*)part.ensure(6 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), nullptr, lib_stream());
+                     T(0), n, dim, (double*)part.ensure(8 * 1024 * 8), (T*)mb.ensure(std::max(8, dim) * sizeof(T)), nullptr, lib_stream());
     MG_HIP(hipStreamSynchronize(lib_stream()));
 }
 
@@ -1085,6 +1118,30 @@ int mg_memcpy_htod(void* d, const void* h, size_t b) { MG_API_BEGIN if (b) { MG_
 int mg_memcpy_dtoh(void* h, const void* d, size_t b) { MG_API_BEGIN if (b) { MG_HIP(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, lib_stream())); MG_HIP(hipStreamSynchronize(lib_stream())); } MG_API_END }
 int mg_memcpy_dtod(void* d, const void* s, size_t b) { MG_API_BEGIN if (b) MG_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, lib_stream())); MG_API_END }
 int mg_mem_info(size_t* f, size_t* t) { MG_API_BEGIN require(f && t, "null output pointer"); MG_HIP(hipMemGetInfo(f, t)); MG_API_END }
+
+int mg_tune_set(const char* name, int value) {
+    MG_API_BEGIN
+    require(name != nullptr, "null name");
+    const std::string k(name);
+    if (k == "attr_window") { require(value >= -1 && value <= 1, "attr_window must be -1, 0 or 1"); g_tunables.attr_window = value; }
+    else if (k == "attr_rows") { require(value >= 64 && value <= 65536, "attr_rows must be in [64, 65536]"); g_tunables.attr_rows = value; }
+    else if (k == "attr_win_kb") { require(value == 0 || value == 64 || value == 128, "attr_win_kb must be 0, 64 or 128"); g_tunables.attr_win_kb = value; }
+    else if (k == "attr_tpb") { require(value == 256 || value == 512 || value == 1024, "attr_tpb must be 256, 512 or 1024"); g_tunables.attr_tpb = value; }
+    else require(false, "unknown tunable: " + k);
+    MG_API_END
+}
+
+int mg_tune_get(const char* name, int* value) {
+    MG_API_BEGIN
+    require(name != nullptr && value != nullptr, "null argument");
+    const std::string k(name);
+    if (k == "attr_window") *value = g_tunables.attr_window;
+    else if (k == "attr_rows") *value = g_tunables.attr_rows;
+    else if (k == "attr_win_kb") *value = g_tunables.attr_win_kb;
+    else if (k == "attr_tpb") *value = g_tunables.attr_tpb;
+    else require(false, "unknown tunable: " + k);
+    MG_API_END
+}
 
 int mg_profile_enable(int on) {
     MG_API_BEGIN
@@ -1438,6 +1495,14 @@ int mg_tsne_session_run(mg_tsne_session* s, void* Y, void* U, void* G, int it0, 
     else
         s->f32->run(&s->p, s->ip, s->ix, (const float*)s->data, s->nnz, s->n, (float*)Y, (float*)U, (float*)G, s->pairs, s->m,
                     it0, it1, h_sumQ, cb, user, lib_stream(), false);
+    MG_API_END
+}
+
+int mg_tsne_session_trace(mg_tsne_session* s, double* h_kl, double* h_grad_norm) {
+    MG_API_BEGIN
+    require(s != nullptr, "null session");
+    if (s->prec == MG_F64) { s->f64->h_kl = h_kl; s->f64->h_gnorm = h_grad_norm; }
+    else { s->f32->h_kl = h_kl; s->f32->h_gnorm = h_grad_norm; }
     MG_API_END
 }
 

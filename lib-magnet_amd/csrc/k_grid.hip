@@ -421,18 +421,19 @@ template <> struct vec4<float> { typedef float4 type; };
 template <> struct vec4<double> { typedef double4 type; };
 
 
-// sum over a block: a butterfly within each wave (fixed order), then thread 0 adds the waves in index order;
-// the result is valid in thread 0
-__device__ inline double block_sum(double v, double* sh) {
+// compensated sum over a block: a butterfly of pairs within each wave (fixed order), then thread 0 merges the waves in
+// index order; the result is valid in thread 0
+__device__ inline CSum block_csum(CSum v, double* sh) {
 #pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = csum_merge(v, {__shfl_xor(v.s, o), __shfl_xor(v.c, o)});
     __syncthreads();   // the previous call's readers are done with sh
-    if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = v;
+    if (threadIdx.x % WAVE == 0) { sh[2 * (threadIdx.x / WAVE)] = v.s; sh[2 * (threadIdx.x / WAVE) + 1] = v.c; }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 1; k < TPB / WAVE; ++k) v += sh[k];
+        for (int k = 1; k < TPB / WAVE; ++k) v = csum_merge(v, {sh[2 * k], sh[2 * k + 1]});
     return v;
 }
+
 
 template <class T, int NI, bool CEN>
 __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const int* __restrict__ vals,
@@ -446,7 +447,7 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
     const T* __restrict__ ph1 = phi + NN;
     const T* __restrict__ ph2 = phi + 2 * NN;
     const T* __restrict__ ph3 = phi + 3 * NN;
-    double z = 0;
+    CSum z = {0.0, 0.0};
     for (i64 p = (i64)bid * TPB + threadIdx.x; p < n; p += (i64)nblk * TPB) {
         const int key = keys[p], idx = vals[p];
         const V2 pt = reinterpret_cast<const V2*>(Y)[idx];
@@ -473,7 +474,7 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
                     fy += w * ((double)c.y * ey + (double)c.w);
                 }
             }
-            z += p0;
+            csum_add(z, p0);
             f.x = (T)fx; f.y = (T)fy;
         } else {
             T p0 = 0, p1 = 0, p2 = 0, p3 = 0;
@@ -488,24 +489,26 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
                     p0 += w * c.x; p1 += w * c.y; p2 += w * c.z; p3 += w * c.w;
                 }
             }
-            z += (double)p0;
+            csum_add(z, (double)p0);
             f.x = p1 * pt.x - p2; f.y = p1 * pt.y - p3;
         }
         reinterpret_cast<V2*>(F)[idx] = f;
     }
-    z = block_sum(z, shd);
-    if (threadIdx.x == 0) zpart[bid] = z;
+    z = block_csum(z, shd);
+    if (threadIdx.x == 0) { zpart[2 * bid] = z.s; zpart[2 * bid + 1] = z.c; }
 }
 
-// Z of the gather: the nblk block partials in a fixed order; sumQ = Z - n, 1 / max(sumQ, 1e-12) (one block)
+// Z of the gather: the nblk compensated block partials merged in a fixed order; sumQ = Z - n (the subtraction
+// taken into the compensated sum before it is rounded), 1 / max(sumQ, 1e-12) (one block)
 __device__ inline void zfinal_block(const double* __restrict__ zpart, int nblk, i64 n, double* __restrict__ zout,
                                     void* inv_out, bool fp32, double* __restrict__ sumq_hist) {
-    __shared__ double shd[TPB];
-    double s = 0;
-    for (int b = threadIdx.x; b < nblk; b += TPB) s += zpart[b];
-    s = block_sum(s, shd);
+    __shared__ double shd[2 * TPB];
+    CSum s = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += TPB) s = csum_merge(s, {zpart[2 * b], zpart[2 * b + 1]});
+    s = block_csum(s, shd);
     if (threadIdx.x == 0) {
-        const double sumq = s - (double)n;
+        s = csum_merge(s, {-(double)n, 0.0});
+        const double sumq = s.s + s.c;
         const double inv = 1.0 / (sumq > 1e-12 ? sumq : 1e-12);
         zout[0] = sumq; zout[1] = inv;
         if (fp32) *reinterpret_cast<float*>(inv_out) = (float)inv; else *reinterpret_cast<double*>(inv_out) = inv;

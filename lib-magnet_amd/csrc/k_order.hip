@@ -1,5 +1,5 @@
-// Locality relabelling of the single-embedding loop: the points are renumbered from time to time in the order of
-// the layout's box sort, so that the attraction's neighbour reads (random rows of Y) fall on nearby memory.
+// Locality relabelling of the single-embedding loop: the points are renumbered from time to time in the Morton
+// order of their grid boxes, so that the attraction's neighbour reads (random rows of Y) fall on nearby memory.
 // The labelling never changes a result: the CSR of a relabelled point keeps its entries in their original order
 // (so its attraction sums in the same order), the box sort breaks ties by original index, and the mean and bbox
 // are summed over the points in original order with the original block decomposition.
@@ -122,6 +122,30 @@ __global__ __launch_bounds__(TPB) void k_perm_keys(const T* __restrict__ Y, cons
     }
 }
 
+// Morton (Z-order) index of the box (bx, by) of a point on grid g, clamped to the grid: the bits of bx and by
+// interleaved, so that boxes close in the plane are mostly close in the order
+template <class T> __device__ inline int morton_key(T x, T y, const GridT<T>& g) {
+    const unsigned bx = (unsigned)min(g.nb - 1, max(0, (int)(((double)x - g.xmin_d) * g.inv_w_d)));
+    const unsigned by = (unsigned)min(g.nb - 1, max(0, (int)(((double)y - g.ymin_d) / g.width_d)));
+    unsigned key = 0;
+#pragma unroll
+    for (int b = 0; b < 15; ++b) key |= (((bx >> b) & 1u) << (2 * b)) | (((by >> b) & 1u) << (2 * b + 1));
+    return (int)key;
+}
+
+// relabelling sort input in original order: the Morton key of original point i, value its current position
+template <class T>
+__global__ __launch_bounds__(TPB) void k_morton_keys(const T* __restrict__ Y, const int* __restrict__ inv, i64 n, GridT<T> g,
+                                                     int* __restrict__ keys, int* __restrict__ vals) {
+    typedef typename vec2<T>::type V;
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        const int p = inv[i];
+        const V y = reinterpret_cast<const V*>(Y)[p];
+        keys[i] = morton_key<T>(y.x, y.y, g);
+        vals[i] = p;
+    }
+}
+
 // original-order copy of one (n, 2) array: dst[i] = src[inv[i]]
 template <class T>
 __global__ __launch_bounds__(TPB) void k_unperm(const T* __restrict__ src, const int* __restrict__ inv, i64 n, T* __restrict__ dst) {
@@ -174,6 +198,13 @@ template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, cons
     MG_HIP(hipGetLastError());
 }
 
+template <class T> void launch_morton_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
+                                           hipStream_t st) {
+    require(g.nb >= 1 && g.nb <= (1 << 15), "grid too large for a Morton key");
+    hipLaunchKernelGGL(k_morton_keys<T>, dim3(blocks_for(n)), dim3(TPB), 0, st, Y, inv, n, g, keys, vals);
+    MG_HIP(hipGetLastError());
+}
+
 template <class T> void launch_unperm(const T* src, const int* inv, i64 n, T* dst, hipStream_t st) {
     hipLaunchKernelGGL(k_unperm<T>, dim3(blocks_for(n)), dim3(TPB), 0, st, src, inv, n, dst);
     MG_HIP(hipGetLastError());
@@ -183,6 +214,7 @@ template <class T> void launch_unperm(const T* src, const int* inv, i64 n, T* ds
     template void launch_relabel<T>(const int*, const int*, i64, const int32_t*, const int32_t*, const T*, int*, int*, int*, int*, int32_t*, int32_t*, T*, hipStream_t); \
     template void launch_perm_state<T>(const T*, const T*, const T*, const int*, const int*, i64, T*, T*, T*, hipStream_t); \
     template void launch_perm_keys<T>(const T*, const int*, i64, const GridT<T>&, int*, int*, hipStream_t); \
+    template void launch_morton_keys<T>(const T*, const int*, i64, const GridT<T>&, int*, int*, hipStream_t); \
     template void launch_unperm<T>(const T*, const int*, i64, T*, hipStream_t);
 MG_INST(float)
 MG_INST(double)

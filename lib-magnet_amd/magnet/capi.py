@@ -51,7 +51,8 @@ def lib():
             "mg_last_error": ([], C.c_char_p), "mg_version": ([], i32), "mg_set_stream": ([vp], i32), "mg_set_thread_stream": ([vp], i32),
             "mg_synchronize": ([], i32), "mg_mem_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
             "mg_bbox_2d": ([i32, vp, i64, C.POINTER(C.c_double)], i32),
-            "mg_profile_enable": ([i32], i32), "mg_fft_size": ([i32], i32),
+            "mg_profile_enable": ([i32], i32),
+            "mg_tune_set": ([C.c_char_p, i32], i32), "mg_tune_get": ([C.c_char_p, C.POINTER(C.c_int)], i32), "mg_fft_size": ([i32], i32),
             "mg_profile_grid": ([C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32), "mg_profile_reset": ([], i32), "mg_profile_num_stages": ([], i32),
             "mg_profile_sized": ([i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_double)], i32),
@@ -77,6 +78,7 @@ def lib():
             "mg_tsne_session_create": ([C.POINTER(vp), i32, C.POINTER(Params), vp, vp, vp, i64, i64, vp, i32], i32),
             "mg_tsne_session_run": ([vp, vp, vp, vp, i32, i32, C.POINTER(C.c_double), CALLBACK, vp], i32),
             "mg_tsne_session_destroy": ([vp], i32),
+            "mg_tsne_session_trace": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
@@ -223,14 +225,24 @@ class Session:
                                            indices.numel(), indptr.numel() - 1, ptr(pairs),
                                            pairs.shape[1] if pairs is not None else 0))
 
-    def run(self, Y, U, G, it0=0, it1=None, callback=None):
+    def run(self, Y, U, G, it0=0, it1=None, callback=None, trace=False):
+        """Iterations [it0, it1) on the state; returns sumQ per iteration, or (sumQ, KL, gradient norm) with trace."""
         import numpy as np
         it1 = self.n_iter if it1 is None else it1
-        sq = np.zeros(max(1, it1 - it0))
+        m = max(1, it1 - it0)
+        sq = np.zeros(m)
         cb = CALLBACK(callback) if callback else CALLBACK(0)
-        check(lib().mg_tsne_session_run(self.h, ptr(Y), ptr(U), ptr(G), int(it0), int(it1),
-                                        sq.ctypes.data_as(C.POINTER(C.c_double)), cb, None))
-        return sq[: max(0, it1 - it0)]
+        dp = C.POINTER(C.c_double)
+        kl, gn = (np.zeros(m), np.zeros(m)) if trace else (None, None)
+        check(lib().mg_tsne_session_trace(self.h, kl.ctypes.data_as(dp) if trace else None,
+                                          gn.ctypes.data_as(dp) if trace else None))
+        try:
+            check(lib().mg_tsne_session_run(self.h, ptr(Y), ptr(U), ptr(G), int(it0), int(it1), sq.ctypes.data_as(dp), cb, None))
+        finally:
+            if trace:
+                check(lib().mg_tsne_session_trace(self.h, None, None))
+        k = max(0, it1 - it0)
+        return (sq[:k], kl[:k], gn[:k]) if trace else sq[:k]
 
     def close(self):
         if self.h:
@@ -322,6 +334,15 @@ def smooth_knn(dist, n_neighbors, mean_all):
     vals = torch.empty_like(dist)
     check(lib().mg_smooth_knn(ptr(dist), dist.shape[0], dist.shape[1], int(n_neighbors), ptr(mean_all), ptr(vals)))
     return vals
+
+
+def tune(name, value=None):
+    """Sets (value given) or reads a launch-shape tunable (mg_tune_set / mg_tune_get); returns the value."""
+    if value is not None:
+        check(lib().mg_tune_set(name.encode(), int(value)))
+    v = C.c_int()
+    check(lib().mg_tune_get(name.encode(), C.byref(v)))
+    return v.value
 
 
 def profile(on=True):

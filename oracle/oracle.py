@@ -182,6 +182,30 @@ def optimize(indptr, indices, data, Y, U=None, G=None, it0=0, it1=None, n_iter=7
     return sq
 
 
+# ---- per-iteration diagnostics (no counterpart in the reference; the KL divergence t-SNE minimises, and the norm of
+# the gradient the loop applies), in float64 numpy
+def kl_divergence(indptr, indices, data, Y, Z, dof=1.0):
+    """KL(P || Q) with q_ij = (1 + |y_i - y_j|^2 / dof)^-dof / Z over the CSR entries of P (P normalised, zeros
+    skipped): sum p log p - sum p log q = sum p log p + dof sum p log1p(d^2 / dof) + (sum p) log Z."""
+    ip = np.asarray(indptr, np.int64)
+    rows = np.repeat(np.arange(len(ip) - 1), np.diff(ip))
+    p = np.asarray(data, np.float64)
+    Yd = np.asarray(Y, np.float64)
+    d2 = ((Yd[rows] - Yd[np.asarray(indices, np.int64)]) ** 2).sum(1)
+    nz = p > 0
+    return float((p[nz] * np.log(p[nz])).sum() + dof * (p * np.log1p(d2 / dof)).sum() + p.sum() * np.log(Z))
+
+
+def gradient(indptr, indices, data, Y, it, n_iter=750, early_iter=250, early_exaggeration=12.0, exaggeration=1.0,
+             n_interp=3, box_width=0.7, dof=1.0, min_boxes=20, max_boxes=1000):
+    """(gradient rho A - F / sumQ of the fused loop at iteration it (magnet/tsne.py:167-184), sumQ) for layout Y."""
+    Yd = np.ascontiguousarray(Y, np.float64)
+    A = attraction(indptr, indices, np.asarray(data, np.float64), Yd, dof)
+    sq, F = repulsion(Yd, *grid(Yd, box_width, min_boxes, max_boxes), n_interp=n_interp, dof=dof)
+    rho = early_exaggeration if it < early_iter else exaggeration
+    return rho * A - F / max(sq, 1e-12), sq
+
+
 # ---- fallback kernels (src/fftrep.cpp:24-125, 294-411); arrays in Y's dtype
 def interp_scatter(Y, charges, xmin, ymin, width, n_boxes, n_interp):
     """(C, N, N) charge grids, points summed in index order."""

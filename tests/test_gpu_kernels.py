@@ -373,3 +373,35 @@ def test_errors_are_reported():
     with pytest.raises(capi.MgError):
         capi.FFTConv(capi.MG_F64, 60, 9)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_windowed_attraction_equals_the_plain_sweep(dtype):
+    """The LDS-window sweep (every window size, block shape and rows per block, and rows only) gives the plain
+    sweep's result bit for bit: rows of 0-300 edges, neighbours near the row (inside the window) and anywhere."""
+    import torch
+    rng = np.random.default_rng(11)
+    n = 300_000
+    lens = rng.integers(0, 300, n)
+    lens[rng.integers(0, n, 2000)] = 0
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    rows = np.repeat(np.arange(n), lens)
+    near = rows + rng.integers(-3000, 3000, rows.size)
+    far = rng.integers(0, n, rows.size)
+    ix = np.where(rng.random(rows.size) < 0.7, near, far).clip(0, n - 1).astype(np.int32)
+    data = rng.random(rows.size).astype(dtype)
+    Y = dev(rng.normal(0, 20, (n, 2)).astype(dtype))
+    dip, dix, dd = dev(ip.astype(np.int32)), dev(ix), dev(data)
+    saved = {k: capi.tune(k) for k in ("attr_window", "attr_rows", "attr_win_kb", "attr_tpb")}
+    try:
+        capi.tune("attr_window", 0)
+        ref = capi.attraction(dip, dix, dd, Y, 0.8)
+        for kb, tpb, rows_pb in ((64, 1024, 256), (128, 1024, 1024), (64, 512, 64), (128, 512, 300), (0, 256, 64)):
+            capi.tune("attr_window", 1); capi.tune("attr_win_kb", kb); capi.tune("attr_tpb", tpb); capi.tune("attr_rows", rows_pb)
+            out = capi.attraction(dip, dix, dd, Y, 0.8)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (kb, tpb, rows_pb)
+    finally:
+        for k, v in saved.items():
+            capi.tune(k, v)

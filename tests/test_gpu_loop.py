@@ -320,3 +320,42 @@ def test_session_runs_in_pieces_equal_one_call(dtype):
         assert torch.equal(Y, one.Y) and torch.equal(U, one.U) and torch.equal(G, one.G), rep
         assert np.array_equal(sq, sq_one), rep
     sess.close()
+
+
+def test_trace_kl_and_gradient_norm_match_oracle():
+    """Per-iteration diagnostics of a session (KL divergence and gradient norm, FP64 with compensated sums) on the
+    10k configuration: FP64 against the oracle's on the same layouts (1e-10 relative) across both phases; FP32 with
+    and without relabelling equal bit for bit, and within 1e-5 (KL, absolute) / 1e-4 (norm, relative) of FP64 over
+    the first iterations (later the two precisions' trajectories part)."""
+    import torch
+    P, Y0, _ = problem()
+    n = P.shape[0]
+    kw = dict(n_iter=500, early_iter=166)
+    out = {}
+    for dt, relabel in ((np.float64, 0), (np.float32, 0), (np.float32, -1)):
+        st = State(P, Y0, dt)
+        sess = capi.Session(st.ip, st.ix, st.data, relabel_every=relabel, **kw)
+        states, sqs, kls, gns = [], [], [], []
+        out_it = 0
+        for a, b in ((0, 3), (160, 170), (300, 303)):
+            if a > 0:
+                sess.run(st.Y, st.U, st.G, out_it, a)
+            states.append(host(st.Y).copy())
+            sq, kl, gn = sess.run(st.Y, st.U, st.G, a, b, trace=True)
+            sqs.append(sq); kls.append(kl); gns.append(gn)
+            out_it = b
+            if dt == np.float64:
+                # the oracle on the layout each iteration starts from (the first of each run)
+                Yt = states[-1]
+                g, sqo = O.gradient(P.indptr, P.indices, P.data.astype(np.float64), Yt, a, **kw)
+                klo = O.kl_divergence(P.indptr, P.indices, P.data.astype(np.float64), Yt, sqo)
+                print(f"t={a}: KL {kl[0]:.12g} oracle {klo:.12g}; |g| {gn[0]:.12g} oracle {np.linalg.norm(g):.12g}")
+                assert abs(kl[0] - klo) <= 1e-10 * max(1.0, abs(klo)) and abs(gn[0] / np.linalg.norm(g) - 1) < 1e-10
+        sess.close()
+        out[(dt, relabel)] = (np.concatenate(kls), np.concatenate(gns))
+    a, b = out[(np.float32, 0)], out[(np.float32, -1)]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    k64, g64 = out[(np.float64, 0)]
+    # the first run's iterations, before the two precisions' trajectories part
+    print("FP32 - FP64 KL", np.abs(a[0][:3] - k64[:3]).max(), "norm rel", np.abs(a[1][:3] / g64[:3] - 1).max())
+    assert np.abs(a[0][:3] - k64[:3]).max() < 1e-5 and np.abs(a[1][:3] / g64[:3] - 1).max() < 1e-4

@@ -284,3 +284,33 @@ def test_one_iteration_follows_the_exact_gradient():
     Yn = Y - lr * Gn * grad
     Yn -= Yn.mean(0)
     assert rel(Y1 - Y, Yn - Y) < 1e-3
+
+
+def test_kl_divergence_closed_form_and_gradient_of_the_oracle():
+    """kl_divergence against the definition sum p log(p / q) on a tiny complete graph with the exact Z (dof 1 and
+    0.7); gradient against the exact O(n^2) t-SNE gradient on a small layout at the interpolation error level."""
+    rng = np.random.default_rng(5)
+    n = 6
+    Y = rng.normal(0, 2, (n, 2))
+    W = rng.random((n, n)); W = W + W.T; np.fill_diagonal(W, 0); W /= W.sum()
+    import scipy.sparse as sp
+    P = sp.csr_matrix(W)
+    for dof in (1.0, 0.7):
+        D2 = ((Y[:, None] - Y[None]) ** 2).sum(-1)
+        K = (1 + D2 / dof) ** (-dof); np.fill_diagonal(K, 0)
+        Z = K.sum()
+        m = W > 0
+        exact = float((W[m] * np.log(W[m] / (K[m] / Z))).sum())
+        assert abs(O.kl_divergence(P.indptr, P.indices, P.data, Y, Z, dof) - exact) < 1e-13
+    # the gradient of the fused loop against exact sums (rho A - F / Z), away from the grid's interpolation error
+    n = 400
+    Y = rng.normal(0, 5, (n, 2))
+    W = rng.random((n, n)) * (rng.random((n, n)) < 0.05); W = W + W.T; np.fill_diagonal(W, 0); W /= W.sum()
+    P = sp.csr_matrix(W)
+    g, sq = O.gradient(P.indptr, P.indices, P.data, Y, it=3, n_iter=10, early_iter=5)
+    D2 = ((Y[:, None] - Y[None]) ** 2).sum(-1)
+    Q = 1 / (1 + D2); np.fill_diagonal(Q, 0)
+    A = ((W * Q)[:, :, None] * (Y[:, None] - Y[None])).sum(1)
+    F = ((Q ** 2)[:, :, None] * (Y[:, None] - Y[None])).sum(1)
+    exact = 12.0 * A - F / Q.sum()
+    assert np.linalg.norm(g - exact) / np.linalg.norm(exact) < 1e-3 and abs(sq / Q.sum() - 1) < 1e-3
