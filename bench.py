@@ -175,6 +175,22 @@ def matched_precision(l32, l64, snaps, k):
     return rows
 
 
+def trace_same_states(l32, l64, snaps):
+    """KL divergence and gradient norm of FP32 and FP64 (the loop's FP64 compensated diagnostics) at each snapshot,
+    one traced iteration each from the same state: [(t, KL32, KL64, |g|32, |g|64)]."""
+    rows = []
+    for t, st in sorted(snaps.items()):
+        if t >= l32.n_iter:
+            continue
+        vals = []
+        for loop in (l32, l64):
+            s = loop.state(like=st)
+            _, kl, gn = loop.sess.run(*s, t, t + 1, trace=True)
+            vals.append((float(kl[0]), float(gn[0])))
+        rows.append((t, vals[0][0], vals[1][0], vals[0][1], vals[1][1]))
+    return rows
+
+
 def gradient_error(P, l32, l64, st, it, early):
     """Relative L2 of the FP32 path's gradient rho A - F / sumQ against the FP64 path's on the same layout (the FP32
     state widened), each through the per-call attraction and FFTConv2D objects of its precision."""
@@ -217,6 +233,7 @@ def precision_run(P, n_iter, early, steps, warmup, profile=True, cpu=False, thre
     out["full64_s"] = full64
     out["matched"] = matched_precision(l32, l64, {t: snaps[t] for t in match_starts}, 10)
     out["grad_err"] = [(t, gradient_error(P, l32, l64, snaps[t], t, early)) for t in match_starts]
+    out["trace"] = trace_same_states(l32, l64, {t: snaps[t] for t in match_starts})
     out["snaps"] = snaps
     out["match_starts"] = match_starts
     return out
@@ -489,6 +506,10 @@ def schedule_extra(R, n_iter):
         "fp32_over_fp64_same_states": round(m64 / m32, 3),
         "matched_ms_per_iter": [[t, round(a, 4), round(b, 4)] for t, a, b in R["matched"]],
         "fp32_gradient_rel_l2": {"max": float(f"{max(errs):.3g}"), "at": [[t, float(f"{e:.3g}")] for t, e in R["grad_err"]]},
+        "kl_and_grad_norm_same_states": {
+            "columns": ["t", "KL fp32", "KL fp64", "|grad| fp32", "|grad| fp64"],
+            "rows": [[t, round(a, 7), round(b, 7), float(f"{c:.7g}"), float(f"{d:.7g}")] for t, a, b, c, d in R["trace"]],
+            "note": "FP64 compensated reductions in both precisions; one traced iteration from each same state"},
     }
 
 
@@ -507,7 +528,7 @@ def scale_leg(warmup):
     return {"workload": W["desc"], "n": int(P.shape[0]), "nnz": int(P.nnz), "P_build_s": round(t_p, 1),
             "fp32_it_s": ex["full_schedule"]["fp32_it_s"], "fp64_it_s": ex["full_schedule"]["fp64_it_s"],
             "fp32_over_fp64_same_states": ex["fp32_over_fp64_same_states"], "matched_ms_per_iter": ex["matched_ms_per_iter"],
-            "fp32_gradient_rel_l2": ex["fp32_gradient_rel_l2"]}
+            "fp32_gradient_rel_l2": ex["fp32_gradient_rel_l2"], "kl_and_grad_norm_same_states": ex["kl_and_grad_norm_same_states"]}
 
 
 def main():

@@ -336,6 +336,13 @@ def smooth_knn(dist, n_neighbors, mean_all):
     return vals
 
 
+def mem_info():
+    """(free, total) HBM bytes of the current device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().mg_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
 def tune(name, value=None):
     """Sets (value given) or reads a launch-shape tunable (mg_tune_set / mg_tune_get); returns the value."""
     if value is not None:

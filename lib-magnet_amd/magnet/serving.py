@@ -6,8 +6,10 @@ new rows, enter a queue. A worker thread forms device batches from them as they 
 - Several batches are in flight at once: n_workers worker threads, each with its own HIP stream, take the
   requests that have arrived whenever they are free, so requests that arrive while a batch is on the device
   start on another stream at once and their work overlaps with it on the device.
-- A request is admitted into a batch while the batch's point budget allows. The budget is the smaller of
-  max_batch_points and what a fraction of the free HBM, shared by the workers, holds for the k-NN scratch.
+- A request is admitted into a batch while the batch's point budget allows. The budget is taken again for every
+  batch, from the HBM free at that moment: the smaller of max_batch_points and the points that this worker's share
+  of hbm_fraction of the free memory holds, after the k-NN scratch a worker takes on its first batch. When memory
+  is short a batch still takes one request, so every request is served.
 - A batch runs as one k-NN search and one placement launch; the requests in it share those launches.
 
 New points do not interact, every per-request quantity is computed over the request's own rows, and the k-NN
@@ -47,8 +49,7 @@ class Request:
 
 
 class TransformServer:
-    def __init__(self, model, max_batch_points=16384, hbm_fraction=0.5, n_iter=None, n_workers=3):
-        import torch
+    def __init__(self, model, max_batch_points=16384, hbm_fraction=0.5, n_iter=None, n_workers=3, mem_info=None):
         if not hasattr(model, "embedding_"):
             raise ValueError("the model must be fitted")
         if n_workers < 1:
@@ -58,19 +59,23 @@ class TransformServer:
         self.n_workers = int(n_workers)
         model._serving_state()   # training rows, layout and field to the device once
         n_train = len(model.embedding_)
-        free, _ = torch.cuda.mem_get_info()
-        # per worker: the library's k-NN lists (2 x 2^24 entries of 4 bytes, taken on the thread's first call), or
-        # for rows of more than 64 features one GEMM distance tile of 256 rows and its top-k scratch
-        knn_bytes = max(2 * (1 << 24) * 4, 256 * n_train * 4 * 3)
-        per_point = 64 * 4 + 2 * 8               # neighbours, weights, position
-        share = hbm_fraction * free / self.n_workers
-        self.max_batch_points = int(max(1, min(max_batch_points, (share - knn_bytes) // per_point)))
+        d = model._prepared_.shape[1]
+        # a worker's first batch takes the library's k-NN lists (2 x 2^24 entries of 4 bytes) or, for rows of more
+        # than 64 features, one GEMM distance tile of 256 rows and its top-k scratch
+        self.knn_bytes = max(2 * (1 << 24) * 4, 256 * n_train * 4 * 3)
+        # per new row: its features, k-NN indices / distances / weights (k <= 32), position, and torch temporaries
+        self.per_point = (d + 32 * 3 + 8) * 4 * 2
+        self.hbm_fraction = float(hbm_fraction)
+        self.max_points = int(max_batch_points)
+        self._mem_info = mem_info or _device_mem_info
+        self._warm = set()   # workers whose k-NN scratch exists
         self._pending = collections.deque()
         self._cv = threading.Condition()
         self._ids = 0
         self._stop = False
         self._workers = [threading.Thread(target=self._run, daemon=True) for _ in range(self.n_workers)]
         self.batches = []   # (n_requests, n_points, seconds) of every batch served, in completion order
+        self.budgets = []   # the point budget each batch was admitted under, in admission order
 
     def start(self):
         for w in self._workers:
@@ -100,18 +105,28 @@ class TransformServer:
             self._cv.notify()
         return r
 
-    def _next_batch(self):
-        """Everything that has arrived, up to the point budget (at least one request); None when stopping."""
+    def budget(self, worker=None):
+        """Points a batch may take now: this worker's share of hbm_fraction of the HBM free at this moment, less the
+        k-NN scratch its first batch allocates, capped by max_batch_points (at least 1)."""
+        free = self._mem_info()
+        share = self.hbm_fraction * free / self.n_workers - (0 if worker in self._warm else self.knn_bytes)
+        return int(max(1, min(self.max_points, share // self.per_point)))
+
+    def _next_batch(self, worker=None):
+        """Everything that has arrived, up to the point budget taken now (at least one request); None when
+        stopping."""
         with self._cv:
             while not self._pending and not self._stop:
                 self._cv.wait()
             if self._stop:
                 return None
+            cap = self.budget(worker)
             batch, points = [], 0
-            while self._pending and (not batch or points + len(self._pending[0].X) <= self.max_batch_points):
+            while self._pending and (not batch or points + len(self._pending[0].X) <= cap):
                 r = self._pending.popleft()
                 batch.append(r)
                 points += len(r.X)
+            self.budgets.append(cap)
             if self._pending:
                 self._cv.notify()   # what did not fit goes to the next free worker
             return batch
@@ -121,13 +136,15 @@ class TransformServer:
         from . import capi
         m = self.model
         stream = torch.cuda.Stream()
+        me = threading.get_ident()
         with torch.cuda.stream(stream):
             capi.use_thread_stream(stream)
             while True:
-                batch = self._next_batch()
+                batch = self._next_batch(me)
                 if batch is None:
                     break
                 self._serve(m, batch)
+                self._warm.add(me)
 
     def _serve(self, m, batch):
         import torch
@@ -156,6 +173,12 @@ class TransformServer:
                 r.error = err
             r.t_done = t_done
             r._done.set()
+
+
+def _device_mem_info():
+    """Free HBM bytes on the current device (mg_mem_info)."""
+    from . import capi
+    return capi.mem_info()[0]
 
 
 def poisson_load(server, sampler, qps, duration, seed=0):

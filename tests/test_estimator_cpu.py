@@ -69,8 +69,12 @@ def _bare_server(budget):
     srv._ids = 0
     srv._stop = False
     srv._workers = []
-    srv.max_batch_points = budget
+    srv.max_points = budget
+    srv.n_workers, srv.hbm_fraction, srv.knn_bytes, srv.per_point = 1, 1.0, 0, 1
+    srv._mem_info = lambda: 1 << 40
+    srv._warm = set()
     srv.batches = []
+    srv.budgets = []
     return srv
 
 
@@ -96,3 +100,21 @@ def test_server_stop_fails_pending_requests_and_refuses_new_ones():
     with pytest.raises(RuntimeError):
         srv.submit(np.zeros((4, 3), np.float32))
     assert srv._next_batch() is None
+
+
+def test_server_admits_each_batch_by_the_hbm_free_at_that_moment():
+    """The point budget is taken from the free HBM for every batch: as free memory shrinks, batches shrink; a
+    worker's first batch also sets aside its k-NN scratch; with no room at all a batch still takes one request."""
+    srv = _bare_server(10_000)
+    srv.n_workers, srv.hbm_fraction, srv.knn_bytes, srv.per_point = 2, 0.5, 4000, 10
+    free = iter([1_000_000, 1_000_000, 200_000, 40_000, 1000])
+    srv._mem_info = lambda: next(free)
+    for s in [100] * 300:
+        srv.submit(np.zeros((s, 3), np.float32))
+    sizes = []
+    for worker in ("a", "a", "a", "a", "b"):
+        sizes.append(sum(len(r.X) for r in srv._next_batch(worker)))
+        srv._warm.add(worker)
+    # budgets: (0.5 * free / 2 - scratch on a cold worker) / 10 bytes per point, capped at 10,000 points
+    assert srv.budgets == [10_000, 10_000, 5_000, 1_000, 1]
+    assert sizes == [10_000, 10_000, 5_000, 1_000, 100]
