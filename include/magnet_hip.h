@@ -169,6 +169,34 @@ int mg_knn_noalloc(const float* d_T, int64_t nt, const float* d_Q, int64_t m, in
  * its call, used when the row has no non-zero distance). */
 int mg_smooth_knn(const float* d_dist, int64_t m, int k, int n_neighbors, const float* d_mean_all, float* d_vals);
 
+/* ---- sampled layout: optimize_layout (src/layout.cpp:343-472, src/bindings.cpp) ----------------------------------
+ * Epochs [epoch_start, epoch_end) of MAGNET's sampled pass on the device, for loss "umap" with the power kernel
+ * q = 1 / (1 + a d^2b), plain SGD (optimizer "sgd"), in the reference's owner parallel mode: every edge (head, tail)
+ * is sampled every epochs_per_sample[e] epochs (state in d_eons / d_eonns, as the reference keeps it), moves its
+ * head by the attraction (times 1.5), negative_sample_rate random nodes per sample repel it, and with mid-range
+ * pairs (n, n_mid) a partner pulls it at rate mid_rate; the partners' share comes through the reverse pairs
+ * (CSR rev_indptr / rev_indices: who chose each node; may be NULL). alpha and the mid-range weight move linearly
+ * from (alpha0, mid_w0) to (alpha1, mid_w1) over the epochs. Nodes below n_fixed do not move. Edges must be sorted
+ * by head. Y (n, 2) float32 in place. Every node is moved by one thread reading the others' positions at the start
+ * of the epoch, so the result is deterministic; its random streams differ from the reference's (per node and epoch
+ * instead of per thread chunk), so it agrees with the reference in law, not in value. The reference's other losses,
+ * kernels, Adam, node exponents and random-walk partners are not on this path (MG_ERR_UNSUP: dim != 2). */
+typedef struct {
+    int32_t epoch_start, epoch_end;
+    float alpha0, alpha1, mid_w0, mid_w1;
+    float a, b, rep_a, rep_b;          /* rep_b <= 0: the repulsion uses (a, b) */
+    float negative_sample_rate, exaggeration, repulsion_strength, mid_rate;
+    int32_t clip_mode;                 /* 0: per coordinate ("coord"), 1: in norm ("norm") */
+    int64_t n_fixed;
+    uint64_t seed;
+    float attraction_scale;            /* factor on the attraction of a head: 1.5 (the owner mode's, for graphs whose
+                                          edges come in both directions; <= 0 also means 1.5) or 1 (edges without
+                                          their reverse, as in out-of-sample placement) */
+} mg_layout_params;
+int mg_optimize_layout(const mg_layout_params* p, float* d_Y, int64_t n, int dim, const int32_t* d_head, const int32_t* d_tail,
+                       const float* d_eps, float* d_eons, float* d_eonns, int64_t n_edges, const int32_t* d_pairs, int n_mid,
+                       const int32_t* d_rev_indptr, const int32_t* d_rev_indices);
+
 /* ---- fallback kernels (src/fftrep.cpp:51-125, 294-411; magnet/tsne.py:92-129 calls them when FFTConv2D is not
  * used). Box of a point: u = (x - xmin) / width clamped to [0, n_boxes - 1], product-form Lagrange weights;
  * N = n_boxes * n_interp. Sums run in the reference's single-thread order without fused multiply-adds, so the

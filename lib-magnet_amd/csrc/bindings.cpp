@@ -324,6 +324,70 @@ py::array_t<double> tsne_optimize(py::array indptr, py::array indices, py::array
                                 arr<float>(U), arr<float>(G), p, pairs, it0, it1, callback);
 }
 
+// ------------------------------------------------------------------------------------ sampled layout
+// optimize_layout with the reference's arguments (src/bindings.cpp:11-23, src/layout.cpp:343-472) over
+// mg_optimize_layout: loss 0 (umap), kernel 0 (power), optimizer 0 (sgd), 2-D, mid-range pairs or none; the options
+// this path does not implement (InfoNCE, other kernels, Adam, per-node exponents, random-walk or two-layer partners)
+// raise. Every parallel mode runs as the reference's owner mode (deterministic); the thread count has no effect.
+// For parallel 0 (hogwild) with fixed nodes, the attraction factor is 1 (the new points' edges have no mirror).
+// Y, epoch_of_next_sample and epoch_of_next_negative_sample are updated in place.
+void optimize_layout(py::array Y_, arr<int32_t> head, arr<int32_t> tail, arr<float> eps, py::array eons_, py::array eonns_,
+                     arr<int32_t> indptr, arr<int32_t> indices, int epoch_start, int epoch_end, float alpha0, float alpha1,
+                     float mid_w0, float mid_w1, float a, float b, float rep_a, float rep_b, float negative_sample_rate, int loss,
+                     float exaggeration, float repulsion_strength, int walk_len, float mid_rate, int optimizer, int clip_mode,
+                     int64_t n_fixed, py::array adam_m, py::array adam_v, py::array adam_t, float beta1, float beta2,
+                     arr<int32_t> walk_lens, arr<int32_t> mid_pairs, arr<int32_t> jump_indptr, arr<int32_t> jump_indices,
+                     float mid_jump, int kernel, arr<float> node_b, uint64_t seed, int n_threads, int parallel,
+                     arr<int32_t> rev_indptr, arr<int32_t> rev_indices) {
+    (void)indptr; (void)indices; (void)adam_m; (void)adam_v; (void)adam_t; (void)beta1; (void)beta2; (void)n_threads;
+    auto f32_inplace = [](const py::array& x) { return py::dtype::of<float>().is(x.dtype()) && (x.flags() & py::array::c_style); };
+    require(f32_inplace(Y_) && f32_inplace(eons_) && f32_inplace(eonns_),
+            "Y and the epoch arrays must be C-contiguous float32 (they are updated in place)");
+    require(Y_.ndim() == 2, "Y must be 2-dimensional");
+    const int64_t n = Y_.shape(0);
+    const int dim = (int)Y_.shape(1);
+    require(head.ndim() == 1, "head must be 1-d");
+    const int64_t m = head.shape(0);
+    require(tail.ndim() == 1 && tail.shape(0) == m && eps.shape(0) == m && eons_.shape(0) == m && eonns_.shape(0) == m,
+            "head, tail and epoch arrays must have the same length");
+    require(parallel == 0 || parallel == 1, "parallel must be 0 (hogwild) or 1 (owner)");
+    require(loss == 0, "this path implements loss 0 (umap) only");
+    require(kernel == 0, "this path implements kernel 0 (power) only");
+    require(optimizer == 0, "this path implements optimizer 0 (sgd) only");
+    require(node_b.size() == 0, "per-node kernel exponents are not implemented on this path");
+    require(jump_indptr.size() == 0 || mid_jump <= 0.f, "two-layer random-walk partners are not implemented on this path");
+    const bool use_mid = (mid_w0 > 0.f || mid_w1 > 0.f) && mid_rate > 0.f;
+    require(!use_mid || mid_pairs.size() > 0 || walk_len <= 0, "random-walk partners are not implemented on this path");
+    (void)walk_lens;
+    const int n_mid = (use_mid && mid_pairs.size() > 0) ? (int)mid_pairs.shape(1) : 0;
+    require(n_mid == 0 || (mid_pairs.ndim() == 2 && mid_pairs.shape(0) == n), "mid_pairs must be (n, m)");
+    mg_layout_params p;
+    p.epoch_start = epoch_start; p.epoch_end = epoch_end; p.alpha0 = alpha0; p.alpha1 = alpha1; p.mid_w0 = mid_w0;
+    p.mid_w1 = mid_w1; p.a = a; p.b = b; p.rep_a = rep_a; p.rep_b = rep_b; p.negative_sample_rate = negative_sample_rate;
+    p.exaggeration = exaggeration; p.repulsion_strength = repulsion_strength; p.mid_rate = mid_rate; p.clip_mode = clip_mode;
+    p.n_fixed = n_fixed; p.seed = seed;
+    // the owner mode's factor 1.5 stands for the pull of the mirrored edge; the hogwild placement of new points
+    // (n_fixed > 0, edges from the new points only) has no mirrored edges and pulls with factor 1
+    p.attraction_scale = (parallel == 0 && n_fixed > 0) ? 1.f : 1.5f;
+    float* Y = static_cast<float*>(Y_.mutable_data());
+    float* eons = static_cast<float*>(eons_.mutable_data());
+    float* eonns = static_cast<float*>(eonns_.mutable_data());
+    Dev dY(Y, (size_t)n * dim * sizeof(float)), dh = up(head), dt = up(tail), de = up(eps);
+    Dev don(eons, (size_t)m * sizeof(float)), donn(eonns, (size_t)m * sizeof(float));
+    Dev dp = n_mid ? up(mid_pairs) : Dev();
+    const bool rev = parallel == 1 && n_mid > 0 && rev_indptr.size() > 0;
+    Dev drp = rev ? up(rev_indptr) : Dev(), dri = rev ? up(rev_indices) : Dev();
+    {
+        py::gil_scoped_release rel;
+        check(mg_optimize_layout(&p, (float*)dY.p, n, dim, (const int32_t*)dh.p, (const int32_t*)dt.p, (const float*)de.p,
+                                 (float*)don.p, (float*)donn.p, m, n_mid ? (const int32_t*)dp.p : nullptr, n_mid,
+                                 rev ? (const int32_t*)drp.p : nullptr, rev ? (const int32_t*)dri.p : nullptr));
+        dY.to_host(Y);
+        don.to_host(eons);
+        donn.to_host(eonns);
+    }
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_magnet, m) {
@@ -364,5 +428,18 @@ PYBIND11_MODULE(_magnet, m) {
           py::arg("dof"), py::arg("pairs") = py::none(), py::arg("mid_weight") = 0.0, py::arg("mid_phase") = 0.5,
           py::arg("it0") = 0, py::arg("it1") = -1, py::arg("min_boxes") = 20, py::arg("max_boxes") = 1000,
           py::arg("callback") = py::none());
+    m.def("optimize_layout", &optimize_layout, "Sampled SGD layout optimisation on the device (in place)",
+          py::arg("Y"), py::arg("head"), py::arg("tail"), py::arg("epochs_per_sample"),
+          py::arg("epoch_of_next_sample"), py::arg("epoch_of_next_negative_sample"),
+          py::arg("indptr"), py::arg("indices"),
+          py::arg("epoch_start"), py::arg("epoch_end"), py::arg("alpha0"), py::arg("alpha1"),
+          py::arg("mid_w0"), py::arg("mid_w1"),
+          py::arg("a"), py::arg("b"), py::arg("rep_a"), py::arg("rep_b"), py::arg("negative_sample_rate"), py::arg("loss"),
+          py::arg("exaggeration"), py::arg("repulsion_strength"), py::arg("walk_len"), py::arg("mid_rate"),
+          py::arg("optimizer"), py::arg("clip_mode"), py::arg("n_fixed"),
+          py::arg("adam_m"), py::arg("adam_v"), py::arg("adam_t"), py::arg("beta1"), py::arg("beta2"),
+          py::arg("walk_lens"), py::arg("mid_pairs"), py::arg("jump_indptr"), py::arg("jump_indices"),
+          py::arg("mid_jump"), py::arg("kernel"), py::arg("node_b"), py::arg("seed"), py::arg("n_threads"), py::arg("parallel") = 0,
+          py::arg("rev_indptr") = arr<int32_t>(0), py::arg("rev_indices") = arr<int32_t>(0));
     m.attr("__version__") = "3.0.2+mi355x";
 }

@@ -179,6 +179,11 @@ template <class T> void launch_perm_state(const T* Y, const T* U, const T* G, co
                                           T* Yn, T* Un, T* Gn, hipStream_t st);
 template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
                                          hipStream_t st);
+// sampled layout optimisation (k_layout.hip): epochs [epoch_start, epoch_end) of the owner-mode SGD on Y (n, 2) in
+// place; scratch of layout_scratch_bytes(n)
+size_t layout_scratch_bytes(i64 n);
+void validate_csr(const int32_t* ip, const int32_t* ix, i64 n, i64 nnz, hipStream_t st);
+
 // per-iteration diagnostics (KL divergence, gradient norm) in FP64 with compensated sums; scratch of
 // trace_scratch_doubles() doubles, whose last part launch_trace_plogp fills once per run
 int trace_scratch_doubles();

@@ -20,6 +20,9 @@
 namespace mg {
 
 int update_blocks(i64 n);
+void launch_layout(const mg_layout_params& p, float* Y, i64 n, const int32_t* head, const int32_t* tail, const float* eps,
+                   float* eons, float* eonns, i64 m, const int32_t* pairs, int n_mid, const int32_t* rev_ptr,
+                   const int32_t* rev_idx, void* scratch, hipStream_t st);
 size_t shim_scratch_ints(i64 n, int nb);
 template <class T> void launch_interp_scatter(const T*, i64, const T*, int, double, double, double, int, int, T*, int*, hipStream_t);
 template <class T> void launch_scatter_padded(const T*, i64, double, double, double, int, int, int, T*, int*, hipStream_t);
@@ -1463,6 +1466,28 @@ int mg_tsne_optimize(int prec, const mg_tsne_params* p, const int32_t* ip, const
         Optimizer<float> o(p->n_interp);
         o.run(p, ip, ix, (const float*)data, nnz, n, (float*)Y, (float*)U, (float*)G, pairs, m, it0, it1, h_sumQ, cb, user, lib_stream());
     }
+    MG_API_END
+}
+
+int mg_optimize_layout(const mg_layout_params* p, float* Y, int64_t n, int dim, const int32_t* head, const int32_t* tail,
+                       const float* eps, float* eons, float* eonns, int64_t m, const int32_t* pairs, int n_mid,
+                       const int32_t* rev_ptr, const int32_t* rev_idx) {
+    MG_API_BEGIN
+    require(p != nullptr, "params must not be null");
+    require(dim == 2, "the device sampled layout is 2-D", MG_ERR_UNSUP);
+    require(n >= 1 && n < (int64_t)INT32_MAX && Y, "Y must be (n, 2) with n >= 1");
+    require(m >= 0 && m < (int64_t)INT32_MAX && (m == 0 || (head && tail && eps && eons && eonns)), "bad edge arrays");
+    require(p->epoch_start >= 0 && p->epoch_end >= p->epoch_start, "invalid epoch range");
+    require(p->negative_sample_rate > 0.f, "negative_sample_rate must be positive");
+    require(p->a > 0.f && p->b > 0.f, "kernel a and b must be positive");
+    require(p->n_fixed >= 0 && p->n_fixed <= n, "n_fixed out of range");
+    require(n_mid == 0 || (pairs && n_mid >= 1), "mid_pairs must be (n, m)");
+    require((rev_ptr == nullptr) == (rev_idx == nullptr), "reverse pairs need indptr and indices");
+    require(p->clip_mode == 0 || p->clip_mode == 1, "clip_mode must be 0 (coord) or 1 (norm)");
+    DevBuf scratch;
+    launch_layout(*p, Y, n, head, tail, eps, eons, eonns, m, pairs, n_mid, rev_ptr, rev_idx,
+                  scratch.ensure(layout_scratch_bytes(n)), lib_stream());
+    MG_HIP(hipStreamSynchronize(lib_stream()));
     MG_API_END
 }
 

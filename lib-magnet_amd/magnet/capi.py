@@ -25,6 +25,14 @@ class Params(C.Structure):
                 ("relabel_every", C.c_int32)]
 
 
+class LayoutParams(C.Structure):
+    _fields_ = [("epoch_start", C.c_int32), ("epoch_end", C.c_int32), ("alpha0", C.c_float), ("alpha1", C.c_float),
+                ("mid_w0", C.c_float), ("mid_w1", C.c_float), ("a", C.c_float), ("b", C.c_float), ("rep_a", C.c_float),
+                ("rep_b", C.c_float), ("negative_sample_rate", C.c_float), ("exaggeration", C.c_float),
+                ("repulsion_strength", C.c_float), ("mid_rate", C.c_float), ("clip_mode", C.c_int32),
+                ("n_fixed", C.c_int64), ("seed", C.c_uint64), ("attraction_scale", C.c_float)]
+
+
 class Instance(C.Structure):
     _fields_ = [("indptr", C.c_void_p), ("indices", C.c_void_p), ("data", C.c_void_p), ("nnz", C.c_int64),
                 ("n", C.c_int64), ("Y", C.c_void_p), ("U", C.c_void_p), ("G", C.c_void_p),
@@ -79,6 +87,7 @@ def lib():
             "mg_tsne_session_run": ([vp, vp, vp, vp, i32, i32, C.POINTER(C.c_double), CALLBACK, vp], i32),
             "mg_tsne_session_destroy": ([vp], i32),
             "mg_tsne_session_trace": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
+            "mg_optimize_layout": ([C.POINTER(LayoutParams), vp, i64, i32, vp, vp, vp, vp, vp, i64, vp, i32, vp, vp], i32),
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
@@ -280,6 +289,20 @@ def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.
                           U.data_ptr(), G.data_ptr(), sq[b].ctypes.data_as(C.POINTER(C.c_double)) if sumq else None)
     check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams), int(n_threads)))
     return [s[:n_iter] for s in sq] if sumq else None
+
+
+def optimize_layout(Y, head, tail, eps, eons, eonns, epoch_start, epoch_end, alpha0, alpha1, mid_w0=0.0, mid_w1=0.0,
+                    a=1.0, b=1.0, rep_a=0.0, rep_b=0.0, negative_sample_rate=5.0, exaggeration=1.0, repulsion_strength=1.0,
+                    mid_rate=0.0, clip_mode=0, n_fixed=0, seed=0, pairs=None, rev_indptr=None, rev_indices=None,
+                    attraction_scale=1.5):
+    """mg_optimize_layout on device tensors: Y (n, 2) float32, edges sorted by head (int32), eps / eons / eonns (m)
+    float32 (the last two updated in place), pairs (n, m) int32 or None, reverse pairs CSR or None."""
+    p = LayoutParams(int(epoch_start), int(epoch_end), alpha0, alpha1, mid_w0, mid_w1, a, b, rep_a, rep_b,
+                     negative_sample_rate, exaggeration, repulsion_strength, mid_rate, int(clip_mode), int(n_fixed),
+                     int(seed) & ((1 << 64) - 1), attraction_scale)
+    check(lib().mg_optimize_layout(C.byref(p), ptr(Y), Y.shape[0], Y.shape[1], ptr(head), ptr(tail), ptr(eps), ptr(eons),
+                                   ptr(eonns), head.numel(), ptr(pairs), pairs.shape[1] if pairs is not None else 0,
+                                   ptr(rev_indptr), ptr(rev_indices)))
 
 
 def topk_rows(D, k):

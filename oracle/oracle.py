@@ -55,6 +55,34 @@ def ref_module():
     return mod
 
 
+class LayoutParams(C.Structure):
+    _fields_ = [("epoch_start", C.c_int32), ("epoch_end", C.c_int32), ("alpha0", C.c_float), ("alpha1", C.c_float),
+                ("mid_w0", C.c_float), ("mid_w1", C.c_float), ("a", C.c_float), ("b", C.c_float), ("rep_a", C.c_float),
+                ("rep_b", C.c_float), ("negative_sample_rate", C.c_float), ("exaggeration", C.c_float),
+                ("repulsion_strength", C.c_float), ("mid_rate", C.c_float), ("clip_mode", C.c_int32),
+                ("n_fixed", C.c_int64), ("seed", C.c_uint64), ("attraction_scale", C.c_float)]
+
+
+def layout_epochs(Y, head, tail, eps, eons, eonns, epoch_start, epoch_end, alpha0, alpha1, mid_w0=0.0, mid_w1=0.0, a=1.0,
+                  b=1.0, rep_a=0.0, rep_b=0.0, negative_sample_rate=5.0, exaggeration=1.0, repulsion_strength=1.0,
+                  mid_rate=0.0, clip_mode=0, n_fixed=0, seed=0, pairs=None, rev_indptr=None, rev_indices=None,
+                    attraction_scale=1.5):
+    """Owner-mode sampled-layout epochs (src/layout.cpp:129-260) on float32 Y (n, 2), eons / eonns in place."""
+    p = LayoutParams(int(epoch_start), int(epoch_end), alpha0, alpha1, mid_w0, mid_w1, a, b, rep_a, rep_b,
+                     negative_sample_rate, exaggeration, repulsion_strength, mid_rate, int(clip_mode), int(n_fixed),
+                     int(seed) & ((1 << 64) - 1), attraction_scale)
+    keep = []
+    def cp(x):
+        if x is None:
+            return None
+        x = np.ascontiguousarray(x, np.int32)
+        keep.append(x)
+        return x.ctypes.data_as(C.c_void_p)
+    lib().or_layout_epochs(C.byref(p), Y, len(Y), np.ascontiguousarray(head, np.int32), np.ascontiguousarray(tail, np.int32),
+                           np.ascontiguousarray(eps, np.float32), eons, eonns, len(head), cp(pairs),
+                           pairs.shape[1] if pairs is not None else 0, cp(rev_indptr), cp(rev_indices))
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -83,6 +111,8 @@ def lib():
             getattr(L, "or_spectral_products_" + suf).argtypes = [fp, fp, fp, C.c_int64, C.c_int64, fp]
             getattr(L, "or_gather_forces_" + suf).argtypes = [fp, C.c_int64, fp, C.c_int, T, T, T, C.c_int, C.c_int, fp]
             getattr(L, "or_gather_forces_" + suf).restype = C.c_double
+        L.or_layout_epochs.argtypes = [C.POINTER(LayoutParams), _f32p, C.c_int64, _i32p, _i32p, _f32p, _f32p, _f32p,
+                                       C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.or_exact_repulsion.argtypes = [_f64p, C.c_int64, C.c_double, _f64p]
         L.or_exact_repulsion.restype = C.c_double
         L.or_lagrange_f64.argtypes = [C.c_double, C.c_int, _f64p]

@@ -602,3 +602,108 @@ OR_SHIMS(double, f64)
 OR_SHIMS(float, f32)
 #undef OR_SHIMS
 }  // extern "C"
+
+// ============================================================== sampled layout (src/layout.cpp:118-276, owner mode)
+// The owner-mode epoch of the reference's optimize_layout (src/layout.cpp:129-260 with OWNER = true, loss 0, power
+// kernel, SGD): node by node in index order, each node's due edges in order, every other node read from the layout
+// at the start of the epoch; the random numbers from the generator of src/rng.hpp, one stream per node and epoch
+// (the reference keeps one per chunk of nodes). Float arithmetic without fused multiply-adds, as the reference.
+namespace {
+struct LRng {
+    uint64_t s;
+    explicit LRng(uint64_t seed) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        s = (z ^ (z >> 31)) | 1ULL;
+    }
+    uint64_t next() { s ^= s >> 12; s ^= s << 25; s ^= s >> 27; return s * 0x2545F4914F6CDD1DULL; }
+    uint32_t below(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
+    float uniform() { return (float)(next() >> 40) * (1.0f / 16777216.0f); }
+};
+void lstep(float& x, float& y, float gx, float gy, float alpha, int clip_mode) {   // src/layout.cpp:80-116
+    float sx = alpha * gx, sy = alpha * gy;
+    const float cap = 4.0f * alpha;
+    if (clip_mode == 1) {
+        const float n2 = sx * sx + sy * sy;
+        if (n2 > cap * cap) { const float f = cap / std::sqrt(n2); sx *= f; sy *= f; }
+    } else {
+        sx = sx > cap ? cap : (sx < -cap ? -cap : sx);
+        sy = sy > cap ? cap : (sy < -cap ? -cap : sy);
+    }
+    x += sx; y += sy;
+}
+}  // namespace
+
+extern "C" void or_layout_epochs(const or_layout_params* p, float* Y, int64_t n, const int32_t* head, const int32_t* tail,
+                                 const float* eps, float* eons, float* eonns, int64_t m, const int32_t* pairs, int n_mid,
+                                 const int32_t* rev_ptr, const int32_t* rev_idx) {
+    const float OWNER_ATT = p->attraction_scale > 0.f ? p->attraction_scale : 1.5f;   // src/layout.cpp:31-38
+    const float rep_a = p->rep_b > 0 ? p->rep_a : p->a, rep_b = p->rep_b > 0 ? p->rep_b : p->b;
+    const float two_ab = 2.f * p->a * p->b, two_b = 2.f * rep_b * p->repulsion_strength;
+    const bool b_one = std::fabs(p->b - 1.f) < 1e-6f, rb_one = std::fabs(rep_b - 1.f) < 1e-6f;
+    std::vector<int64_t> eptr(n + 1, 0);
+    for (int64_t e = 0; e < m; ++e) ++eptr[head[e] + 1];
+    for (int64_t u = 0; u < n; ++u) eptr[u + 1] += eptr[u];
+    std::vector<float> snap(Y, Y + 2 * n);
+    const int span = std::max(1, p->epoch_end - p->epoch_start);
+    for (int epoch = p->epoch_start; epoch < p->epoch_end; ++epoch) {
+        const float frac = (float)(epoch - p->epoch_start) / (float)span;
+        const float alpha = p->alpha0 + (p->alpha1 - p->alpha0) * frac;
+        const float mid_w = p->mid_w0 + (p->mid_w1 - p->mid_w0) * frac;
+        const float fepoch = (float)epoch;
+        for (int64_t u = p->n_fixed; u < n; ++u) {
+            float x = snap[2 * u], y = snap[2 * u + 1];
+            LRng rng(p->seed * 0x9E3779B1ULL + (uint64_t)epoch * 0x85EBCA6BULL + (uint64_t)u * 0xC2B2AE35ULL);
+            for (int64_t e = eptr[u]; e < eptr[u + 1]; ++e) {
+                if (eons[e] > fepoch) continue;
+                const int64_t v = tail[e];
+                const float dx = x - snap[2 * v], dy = y - snap[2 * v + 1];
+                const float d2 = dx * dx + dy * dy;
+                float gx = 0.f, gy = 0.f;
+                if (d2 > 0.f) {
+                    const float pw = b_one ? d2 : std::pow(d2, p->b);
+                    const float c = -OWNER_ATT * p->exaggeration * two_ab * pw / (d2 * (p->a * pw + 1.f));
+                    gx += c * dx; gy += c * dy;
+                }
+                eons[e] += eps[e];
+                const float epn = eps[e] / p->negative_sample_rate;
+                const int n_neg = (int)((fepoch - eonns[e]) / epn);
+                for (int q = 0; q < n_neg; ++q) {
+                    const int64_t k = rng.below((uint32_t)n);
+                    if (k == u) continue;
+                    const float kx = x - snap[2 * k], ky = y - snap[2 * k + 1];
+                    const float dd = kx * kx + ky * ky;
+                    if (dd > 0.f) {
+                        const float pw = rb_one ? dd : std::pow(dd, rep_b);
+                        const float c = two_b / ((0.001f + dd) * (rep_a * pw + 1.f));
+                        gx += c * kx; gy += c * ky;
+                    } else {
+                        const float sc = 1e-3f / std::max(alpha, 1e-6f);
+                        gx += (rng.uniform() - 0.5f) * sc;
+                        gy += (rng.uniform() - 0.5f) * sc;
+                    }
+                }
+                eonns[e] += n_neg * epn;
+                if (mid_w > 0.f && pairs && n_mid > 0 && rng.uniform() < p->mid_rate) {
+                    const int64_t w = pairs[u * n_mid + rng.below((uint32_t)n_mid)];
+                    if (w != u) { gx += -mid_w * (x - snap[2 * w]); gy += -mid_w * (y - snap[2 * w + 1]); }
+                }
+                lstep(x, y, gx, gy, alpha, p->clip_mode);
+            }
+            if (mid_w > 0.f && rev_ptr && n_mid > 0) {
+                const int rs = rev_ptr[u], re = rev_ptr[u + 1];
+                float pr = p->mid_rate * (float)(re - rs) / (float)n_mid;
+                while (pr > 0.f && re > rs) {
+                    if (rng.uniform() < pr) {
+                        const int64_t c = rev_idx[rs + (int)rng.below((uint32_t)(re - rs))];
+                        if (c != u) lstep(x, y, -mid_w * (x - snap[2 * c]), -mid_w * (y - snap[2 * c + 1]), alpha, p->clip_mode);
+                    }
+                    pr -= 1.f;
+                }
+            }
+            Y[2 * u] = x; Y[2 * u + 1] = y;
+        }
+        std::copy(Y, Y + 2 * n, snap.begin());
+    }
+}

@@ -8,7 +8,8 @@
 //   update / step        src/fftrep.cpp:181-266
 //   mid-range spring     src/fftrep.cpp:270-287
 //   optimiser schedule   magnet/tsne.py:132-214
-//   fallback kernels     src/fftrep.cpp:24-125, 294-411 (interp_scatter / interp_gather, scatter_padded_2d,
+//   fallback kernels     src/fftrep.cpp:24-125, 294-411
+//   sampled layout       src/layout.cpp:118-276, 343-472 (owner mode, loss 0, power kernel, SGD) (interp_scatter / interp_gather, scatter_padded_2d,
 //                        spectral_products, gather_forces_2d)
 // Every function exists in a double (_f64) and a float (_f32) form. The f64 form is the reference the
 // FP64 GPU path is checked against; the f32 form follows the reference's float32 arithmetic order and
@@ -110,6 +111,22 @@ double or_gather_forces_f64(const double* Y, int64_t n, const double* phi, int M
                             int n_boxes, int n_interp, double* F);
 double or_gather_forces_f32(const float* Y, int64_t n, const float* phi, int M, float xmin, float ymin, float width,
                             int n_boxes, int n_interp, float* F);
+
+// Sampled layout (src/layout.cpp:118-276, 343-472), owner mode, loss 0, power kernel, SGD: epochs
+// [epoch_start, epoch_end) on Y (n, 2) in place, edges sorted by head; the fields as mg_layout_params.
+typedef struct {
+    int32_t epoch_start, epoch_end;
+    float alpha0, alpha1, mid_w0, mid_w1;
+    float a, b, rep_a, rep_b;
+    float negative_sample_rate, exaggeration, repulsion_strength, mid_rate;
+    int32_t clip_mode;
+    int64_t n_fixed;
+    uint64_t seed;
+    float attraction_scale;
+} or_layout_params;
+void or_layout_epochs(const or_layout_params* p, float* Y, int64_t n, const int32_t* head, const int32_t* tail, const float* eps,
+                      float* eons, float* eonns, int64_t m, const int32_t* pairs, int n_mid, const int32_t* rev_ptr,
+                      const int32_t* rev_idx);
 
 #ifdef __cplusplus
 }

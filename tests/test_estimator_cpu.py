@@ -13,8 +13,14 @@ def test_parameter_errors():
                 dict(loss="tsne", n_components=0), dict(loss="tsne", precision="fp16")):
         with pytest.raises(ValueError):
             MAGNET(**bad).fit(X)
-    with pytest.raises(NotImplementedError):
-        MAGNET(loss="umap").fit(X)
+    for other in (dict(loss="infonce"), dict(loss="umap_exact"), dict(loss="umap", kernel="gaussian"),
+                  dict(loss="hybrid", kernel_b="consistent")):
+        with pytest.raises(NotImplementedError):
+            m = MAGNET(**other)
+            m._validate()
+            m._kernel_params(50)
+    with pytest.raises(ValueError):
+        MAGNET(loss="umap", transform_mode="beam")._validate()
 
 
 def test_smooth_knn_hits_the_target_sum():
