@@ -168,6 +168,25 @@ int mg_knn_noalloc(const float* d_T, int64_t nt, const float* d_Q, int64_t m, in
  * distances d_dist (m, k), one row per thread; d_mean_all (m) is each row's fallback mean distance (the mean over
  * its call, used when the row has no non-zero distance). */
 int mg_smooth_knn(const float* d_dist, int64_t m, int k, int n_neighbors, const float* d_mean_all, float* d_vals);
+/* Perplexity calibration on the device: for each row of k squared neighbour distances d_dist2 (n, k) float32, the
+ * Gaussian precision beta whose conditional distribution p_j|i = exp(-beta (d_ij - min_j d_ij)) / sum has entropy
+ * log(perplexity) (natural log) within tol, by bisection from beta = 1 (doubling / halving while a bound is open),
+ * at most n_iter steps, each row on its own; d_P (n, k) float64 receives p_j|i, d_beta (n, may be NULL) beta. No
+ * counterpart in the reference library, which takes these affinities from openTSNE in its benchmark scripts
+ * (benchmarks/opentsne_audit.py:18-23); the configurations it reports on are built from them. */
+int mg_perplexity_rows(const float* d_dist2, int64_t n, int k, double perplexity, double tol, int n_iter, double* d_P,
+                       double* d_beta);
+/* Symmetric CSR of a directed k-NN graph d_idx (n, k) int32 (indices in [0, n)) with values d_val (n, k) float64:
+ * each unordered pair {i, j} gets a = its i -> j value(s) and b = its j -> i value(s), and the entries (i, j) and
+ * (j, i) the value
+ *   mode 0: (a + b) * param                                   ((P + P^T) / 2n for param = 1 / 2n),
+ *   mode 1: param (a + b - a b) + (1 - param) a b             (the fuzzy union mixed with the intersection by
+ *                                                              set_op_mix_ratio = param: magnet/knn.py:215-230),
+ * in float64, stored as float32. Diagonal entries are dropped, and zeros (mode 0: in float64, mode 1: in float32).
+ * Rows sorted by column. d_indptr (n + 1), d_indices / d_data of capacity cap (2 n k always suffices); *h_nnz gets the
+ * entry count (MG_ERR_ARG when it exceeds cap). Deterministic; allocates its scratch (about 20 n k bytes). */
+int mg_symmetrize_knn(int mode, const int32_t* d_idx, const double* d_val, int64_t n, int k, double param,
+                      int32_t* d_indptr, int32_t* d_indices, float* d_data, int64_t cap, int64_t* h_nnz);
 
 /* ---- sampled layout: optimize_layout (src/layout.cpp:343-472, src/bindings.cpp) ----------------------------------
  * Epochs [epoch_start, epoch_end) of MAGNET's sampled pass on the device, for loss "umap" with the power kernel

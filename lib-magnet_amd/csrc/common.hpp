@@ -169,6 +169,14 @@ void launch_knn_noalloc(const float* T, i64 nt, const float* Q, i64 m, int d, in
 
 // smooth k-NN memberships of sorted neighbour distances (m, k), per row; mean_all (m): the fallback mean distance
 void launch_smooth_knn(const float* dist, i64 m, int k, int n_neighbors, const float* mean_all, float* vals, hipStream_t st);
+// perplexity calibration of k squared neighbour distances per row (P (n, k) double, beta (n) double or null)
+void launch_perplexity(const float* d2, i64 n, int k, double perplexity, double tol, int n_iter, double* P, double* beta,
+                       hipStream_t st);
+// symmetrisation of a directed k-NN graph into CSR (mode 0: (a + b) * param; 1: fuzzy union mixed by param); scratch of
+// symmetrize_scratch_ints(n, k) ints; returns nnz, or -nnz when nnz exceeds cap (nothing written but the row pointers)
+i64 symmetrize_scratch_ints(i64 n, int k);
+i64 symmetrize_knn(int mode, const int32_t* idx, const double* val, i64 n, int k, double param, int32_t* indptr,
+                   int32_t* out_idx, float* out_val, i64 cap, int* scratch, hipStream_t st);
 
 // locality relabelling (k_order.hip)
 void launch_identity_labels(int* perm, int* inv, i64 n, hipStream_t st);

@@ -1345,6 +1345,28 @@ int mg_smooth_knn(const float* dist, int64_t m, int k, int n_neighbors, const fl
     MG_API_END
 }
 
+int mg_perplexity_rows(const float* d2, int64_t n, int k, double perplexity, double tol, int n_iter, double* P, double* beta) {
+    MG_API_BEGIN
+    require(n >= 0, "n must be >= 0");
+    require(n == 0 || (d2 && P), "null array");
+    launch_perplexity(d2, n, k, perplexity, tol, n_iter, P, beta, lib_stream());
+    MG_API_END
+}
+
+int mg_symmetrize_knn(int mode, const int32_t* idx, const double* val, int64_t n, int k, double param, int32_t* indptr,
+                      int32_t* out_idx, float* out_val, int64_t cap, int64_t* h_nnz) {
+    MG_API_BEGIN
+    require(idx && val && indptr && out_idx && out_val && h_nnz, "null array");
+    require(n >= 1 && k >= 1, "n and k must be positive");
+    DevBuf scratch;
+    scratch.ensure((size_t)symmetrize_scratch_ints(n, k) * sizeof(int));
+    const i64 nnz = symmetrize_knn(mode, idx, val, n, k, param, indptr, out_idx, out_val, cap, dptr<int>(scratch), lib_stream());
+    *h_nnz = nnz < 0 ? -nnz : nnz;
+    require(nnz >= 0, "output capacity " + std::to_string(cap) + " < " + std::to_string(-nnz) + " entries");
+    MG_HIP(hipStreamSynchronize(lib_stream()));   // the scratch is freed on return
+    MG_API_END
+}
+
 int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
     MG_API_BEGIN
     require(rows == 0 || (D && idx && val), "null array");

@@ -5,10 +5,10 @@ package keeps its t-SNE interface -- optimize_tsne, FFTRepulsion2D, MAGNET(loss=
 transform -- over a HIP implementation, plus a continuous-batching transform service (TransformServer).
 """
 from .tsne import FFTRepulsion2D, optimize_tsne, grid_params
-from .affinity import perplexity_affinities, exact_knn, calibrate_perplexity
+from .affinity import perplexity_affinities
 from .estimator import MAGNET
 from .serving import TransformServer
 
 __version__ = "3.0.2+mi355x"
-__all__ = ["FFTRepulsion2D", "optimize_tsne", "grid_params", "perplexity_affinities", "exact_knn",
-           "calibrate_perplexity", "MAGNET", "TransformServer", "__version__"]
+__all__ = ["FFTRepulsion2D", "optimize_tsne", "grid_params", "perplexity_affinities", "MAGNET", "TransformServer",
+           "__version__"]

@@ -93,6 +93,8 @@ def lib():
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
             "mg_knn_noalloc": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
             "mg_smooth_knn": ([vp, i64, i32, i32, vp, vp], i32),
+            "mg_perplexity_rows": ([vp, i64, i32, C.c_double, C.c_double, i32, vp, vp], i32),
+            "mg_symmetrize_knn": ([i32, vp, vp, i64, i32, C.c_double, vp, vp, vp, i64, C.POINTER(C.c_int64)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -357,6 +359,44 @@ def smooth_knn(dist, n_neighbors, mean_all):
     vals = torch.empty_like(dist)
     check(lib().mg_smooth_knn(ptr(dist), dist.shape[0], dist.shape[1], int(n_neighbors), ptr(mean_all), ptr(vals)))
     return vals
+
+
+def perplexity_rows(d2, perplexity=30.0, tol=1e-5, n_iter=100, with_beta=False):
+    """Conditional affinities p_j|i (n, k) float64 of the squared neighbour distances d2 (n, k) float32 (device), the
+    entropy of each row log(perplexity) within tol (bisection on the Gaussian precision, at most n_iter steps);
+    with_beta: also the precisions (n,)."""
+    import torch
+    if d2.dtype != torch.float32 or d2.dim() != 2 or not d2.is_contiguous():
+        raise ValueError("d2 must be a contiguous (n, k) float32 tensor")
+    P = torch.empty(d2.shape, dtype=torch.float64, device=d2.device)
+    beta = torch.empty(d2.shape[0], dtype=torch.float64, device=d2.device) if with_beta else None
+    check(lib().mg_perplexity_rows(ptr(d2), d2.shape[0], d2.shape[1], float(perplexity), float(tol), int(n_iter), ptr(P),
+                                   ptr(beta) if beta is not None else None))
+    return (P, beta) if with_beta else P
+
+
+def symmetrize_knn(idx, val, mode=0, param=None):
+    """Symmetric CSR (indptr (n+1) int32, indices int32, data float32; device tensors) of the directed graph
+    idx (n, k) int32 with values val (n, k) float64: mode 0 gives each pair (a + b) * param (param defaults to 1 / 2n:
+    the perplexity affinities (P + P^T) / 2n), mode 1 param (a + b - a b) + (1 - param) a b (the fuzzy union,
+    param = set_op_mix_ratio, default 1)."""
+    import torch
+    if idx.dtype != torch.int32 or idx.dim() != 2 or not idx.is_contiguous():
+        raise ValueError("idx must be a contiguous (n, k) int32 tensor")
+    val = val.to(torch.float64).contiguous()
+    if val.shape != idx.shape:
+        raise ValueError("val must have the shape of idx")
+    n, k = idx.shape
+    if param is None:
+        param = 1.0 / (2.0 * n) if mode == 0 else 1.0
+    cap = 2 * n * k
+    indptr = torch.empty(n + 1, dtype=torch.int32, device=idx.device)
+    indices = torch.empty(max(cap, 1), dtype=torch.int32, device=idx.device)
+    data = torch.empty(max(cap, 1), dtype=torch.float32, device=idx.device)
+    nnz = C.c_int64(0)
+    check(lib().mg_symmetrize_knn(int(mode), ptr(idx), ptr(val), n, k, float(param), ptr(indptr), ptr(indices), ptr(data),
+                                  cap, C.byref(nnz)))
+    return indptr, indices[: nnz.value].clone(), data[: nnz.value].clone()
 
 
 def mem_info():

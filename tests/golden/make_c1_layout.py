@@ -30,13 +30,14 @@ N_ITER, EARLY = 500, 166
 
 
 def c1_data():
-    """The small configuration's data and affinities (as bench.py builds them for the 10k workload), on the host."""
-    from magnet.affinity import perplexity_affinities
+    """The small configuration's data and affinities (as bench.py builds them for the 10k workload), on the host
+    (the oracle's exact k-NN, perplexity calibration and symmetrisation)."""
+    from affinity_oracle import perplexity_affinities
     rng = np.random.default_rng(0)
     n, d, k = 10_000, 50, 10
     lab = np.repeat(np.arange(k), n // k)
     X = (rng.normal(0, 5, (k, d))[lab] + rng.normal(0, 1, (n, d))).astype(np.float32)
-    P = perplexity_affinities(X, 30.0, device="cpu")
+    P = perplexity_affinities(X, 30.0)
     Y0 = np.random.default_rng(1).normal(0, 1e-4, (n, 2))
     return X, lab, P, Y0
 

@@ -59,18 +59,6 @@ def test_grid_decision_matches_oracle():
         assert grid_params(bb) == O.grid(Y)
 
 
-def test_perplexity_calibration_hits_the_entropy():
-    from magnet.affinity import calibrate_perplexity, perplexity_affinities
-    rng = np.random.default_rng(1)
-    d2 = np.sort(rng.gamma(2.0, 3.0, (200, 90)), axis=1)
-    Pc = calibrate_perplexity(d2, 30.0)
-    H = -(Pc * np.log(np.maximum(Pc, 1e-300))).sum(1)
-    assert np.allclose(H, np.log(30.0), atol=1e-4) and np.allclose(Pc.sum(1), 1)
-    X = rng.normal(0, 1, (400, 5)).astype(np.float32)
-    P = perplexity_affinities(X, 10.0, device="cpu")
-    assert abs(P.sum() - 1) < 1e-5 and abs(P - P.T).max() < 1e-8 and P.indices.dtype == np.int32
-
-
 def test_fft_size_choice():
     """smallest 2^s or 3 * 2^s >= 2N, from 128 up to 8192 (no device needed)"""
     from magnet import capi

@@ -185,11 +185,11 @@ def test_knn_ties_break_by_index(d, k, exclude, m, nt):
 
 
 def test_device_smooth_knn_matches_the_numpy_restatement():
-    """mg_smooth_knn against magnet.graph.smooth_knn (the reference's numpy algorithm) on sorted distances,
+    """mg_smooth_knn against the oracle's smooth_knn (the reference's numpy algorithm) on sorted distances,
     with leading zeros and an all-zero row (the fallback to the call's mean distance)."""
     import torch
     from magnet import capi
-    from magnet.graph import smooth_knn
+    from affinity_oracle import smooth_knn
     rng = np.random.default_rng(3)
     D = np.sort(rng.gamma(2.0, 1.0, (500, 14)).astype(np.float32), axis=1)
     D[7, :3] = 0.0

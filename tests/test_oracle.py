@@ -342,3 +342,77 @@ def test_layout_oracle_matches_the_reference_in_law():
     layouts after 200 epochs separate the blobs (10-NN accuracy > 0.95, the reference's test_owner_parallel_mode) and
     have the same spread within 15%."""
     _run_reference_check("law")
+
+
+# ---------------------------------------------------------------------------------------------- upstream rows
+def test_perplexity_rows_hit_the_entropy_and_a_root_finders_beta():
+    """perplexity_rows: every row's entropy is log(perplexity) within the tolerance, its p sums to 1, and its beta is
+    the root of H(beta) = log(perplexity) that scipy's brentq finds (an independent method) to the tolerance's width."""
+    from scipy.optimize import brentq
+    from affinity_oracle import perplexity_rows
+    rng = np.random.default_rng(1)
+    d2 = np.sort(rng.gamma(2.0, 3.0, (200, 90)), axis=1).astype(np.float32)
+    d2[5] = 0.0            # all-zero row: uniform p (any beta)
+    d2[6, 1:] = 1e3        # a row whose mass can only sit on one neighbour
+    for perp in (5.0, 30.0):
+        Pc, beta = perplexity_rows(d2, perp, tol=1e-10, n_iter=200)
+        assert np.allclose(Pc.sum(1), 1.0, rtol=0, atol=1e-12)
+        H = -(Pc * np.log(np.maximum(Pc, 1e-300))).sum(1)
+        ok = np.ones(200, bool); ok[[5, 6]] = False
+        assert np.abs(H[ok] - np.log(perp)).max() < 1e-8
+        assert np.allclose(Pc[5], 1 / 90)
+        for i in (0, 17, 100, 199):
+            d = d2[i].astype(np.float64) - d2[i].min()
+            def f(b):
+                w = np.exp(-d * b)
+                return np.log(w.sum()) + b * (d * w).sum() / w.sum() - np.log(perp)
+            root = brentq(f, 1e-8, 1e3, xtol=1e-14)
+            assert abs(beta[i] / root - 1) < 1e-6, (i, beta[i], root)
+
+
+def test_symmetrize_sum_is_p_plus_p_transpose_over_2n():
+    """symmetrize_knn mode 0 against the dense (P + P^T) / 2n on the same values (ties, a value of zero, a repeated
+    neighbour and a self entry included): the same entries and values (float32 of the float64 sum)."""
+    from affinity_oracle import symmetrize_knn
+    rng = np.random.default_rng(4)
+    n, k = 120, 7
+    idx = np.array([rng.choice(n, k, replace=False) for _ in range(n)])
+    val = rng.random((n, k))
+    val[3, 2] = 0.0
+    idx[9, 1] = idx[9, 0]          # a neighbour listed twice: its values add up
+    idx[10, 4] = 10                 # a self entry: dropped
+    S = symmetrize_knn(idx, val, 0)
+    D = np.zeros((n, n))
+    np.add.at(D, (np.repeat(np.arange(n), k), idx.ravel()), val.ravel())
+    D = (D + D.T) * (1.0 / (2.0 * n))
+    np.fill_diagonal(D, 0.0)
+    assert np.array_equal(S.toarray(), D.astype(np.float32))
+    assert S.nnz == int((D != 0).sum()) and S.has_sorted_indices
+
+
+def test_fuzzy_union_matches_the_dense_formula():
+    """symmetrize_knn mode 1 against A + A^T - A * A^T (set_op_mix_ratio 1) and the mix with the intersection
+    (0.3): symmetric, no diagonal, zeros dropped (magnet/knn.py:215-230)."""
+    from affinity_oracle import symmetrize_knn
+    rng = np.random.default_rng(2)
+    n, k = 300, 10
+    ind = np.array([rng.choice(np.delete(np.arange(n), i), k, replace=False) for i in range(n)])
+    v = rng.random((n, k)).astype(np.float32)
+    v[0, :3] = 0.0
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), k), ind.ravel()] = v.ravel().astype(np.float64)
+    for r in (1.0, 0.3):
+        P = symmetrize_knn(ind, v.astype(np.float64), 1, r).toarray()
+        want = (r * (A + A.T - A * A.T) + (1 - r) * A * A.T).astype(np.float32)
+        assert np.array_equal(P, want) and np.array_equal(P, P.T) and not np.diag(P).any()
+
+
+def test_smooth_knn_hits_the_target_sum():
+    """smooth_knn: the nearest non-zero neighbour has membership 1 and each row sums to log2(k) (magnet/knn.py:165-198)."""
+    from affinity_oracle import smooth_knn
+    rng = np.random.default_rng(1)
+    d = np.sort(rng.uniform(0.1, 3.0, (500, 14)).astype(np.float32), axis=1)
+    v = smooth_knn(d, 15)
+    assert v.shape == d.shape and (v > 0).all() and (v <= 1).all()
+    assert np.allclose(v[:, 0], 1.0)
+    assert np.abs(v.sum(1) - np.log2(15)).max() < 1e-3
