@@ -187,6 +187,20 @@ int mg_perplexity_rows(const float* d_dist2, int64_t n, int k, double perplexity
  * entry count (MG_ERR_ARG when it exceeds cap). Deterministic; allocates its scratch (about 20 n k bytes). */
 int mg_symmetrize_knn(int mode, const int32_t* d_idx, const double* d_val, int64_t n, int k, double param,
                       int32_t* d_indptr, int32_t* d_indices, float* d_data, int64_t cap, int64_t* h_nnz);
+/* nn_descent (src/nndescent.cpp:390-583, the binding of :636-645) on the device, dense rows: the k (<= 128) nearest
+ * other rows of each row of d_X (n, d) float32, as d_idx (n, k) int32 and squared euclidean distances d_dist2 (n, k),
+ * each row sorted by (distance, index). Random-projection trees (n_trees, leaf_size) seed the lists, random rows fill
+ * what they leave open, then up to n_iters rounds of candidate sampling (at most max_candidates new and old per node,
+ * capped at 64) and local joins, stopping when a round changes at most delta * n * k entries. Deterministic for a
+ * seed; agrees with the reference's graph in law (recall against exact search), not in value. Allocates its working
+ * memory (about 30 n k bytes plus the join's proposals); synchronises the library stream. verbose prints a line per
+ * tree and round to stderr. */
+int mg_nn_descent(const float* d_X, int64_t n, int d, int k, int max_candidates, int n_iters, float delta, int n_trees,
+                  int leaf_size, uint64_t seed, int verbose, int32_t* d_idx, float* d_dist2);
+/* mid_near_pairs_dense (src/nndescent.cpp:730-767) on the device: d_out (n, n_pairs) int32, for pair p of row i the
+ * second nearest (squared euclidean, ties by index) of n_cand rows drawn by a hash of (seed, i, p, c) (a draw equal to
+ * i moves to i + 1). The reference draws from per-thread generators: the partners agree in law, not in value. */
+int mg_mid_near_pairs(const float* d_X, int64_t n, int d, int n_pairs, int n_cand, uint64_t seed, int32_t* d_out);
 
 /* ---- sampled layout: optimize_layout (src/layout.cpp:343-472, src/bindings.cpp) ----------------------------------
  * Epochs [epoch_start, epoch_end) of MAGNET's sampled pass on the device, for loss "umap" with the power kernel

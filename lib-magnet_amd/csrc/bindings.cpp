@@ -56,6 +56,40 @@ void validate_csr_host(const arr<int32_t>& indptr, const arr<int32_t>& indices, 
 }
 
 // ---------------------------------------------------------------------------------- per-call kernels
+// nn_descent (src/nndescent.cpp:636-645): (indices (n, k) int32, squared distances (n, k) float32), rows sorted
+py::tuple nn_descent(arr<float> X, int k, int max_candidates, int n_iters, float delta, int n_trees, int leaf_size,
+                     uint64_t seed, int, bool verbose) {
+    require(X.ndim() == 2, "X must be 2-d");
+    require(X.shape(1) >= 1, "X must have at least one column");
+    const int64_t n = X.shape(0);
+    require(k >= 1 && k < n, "k must be between 1 and n - 1");
+    arr<int32_t> idx({n, (int64_t)k});
+    arr<float> dist({n, (int64_t)k});
+    Dev dx = up(X), di((size_t)n * k * sizeof(int32_t)), dd((size_t)n * k * sizeof(float));
+    {
+        py::gil_scoped_release rel;
+        check(mg_nn_descent((const float*)dx.p, n, (int)X.shape(1), k, max_candidates, n_iters, delta, n_trees, leaf_size, seed,
+                            verbose ? 1 : 0, (int32_t*)di.p, (float*)dd.p));
+        di.to_host(idx.mutable_data());
+        dd.to_host(dist.mutable_data());
+    }
+    return py::make_tuple(idx, dist);
+}
+
+// mid_near_pairs_dense (src/nndescent.cpp:761-767): (n, n_pairs) int32
+arr<int32_t> mid_near_pairs_dense(arr<float> X, int n_pairs, int n_cand, uint64_t seed, int) {
+    require(X.ndim() == 2 && X.shape(1) >= 1, "X must be 2-d");
+    const int64_t n = X.shape(0);
+    arr<int32_t> out({n, (int64_t)n_pairs});
+    Dev dx = up(X), dout((size_t)n * (n_pairs > 0 ? n_pairs : 1) * sizeof(int32_t));
+    {
+        py::gil_scoped_release rel;
+        check(mg_mid_near_pairs((const float*)dx.p, n, (int)X.shape(1), n_pairs, n_cand, seed, (int32_t*)dout.p));
+        dout.to_host(out.mutable_data());
+    }
+    return out;
+}
+
 arr<float> tsne_attraction(arr<int32_t> indptr, arr<int32_t> indices, arr<float> data, arr<float> Y, float dof, int) {
     require(dof > 0.f, "dof must be positive");
     require(Y.ndim() == 2, "Y must be 2-d");
@@ -428,6 +462,12 @@ PYBIND11_MODULE(_magnet, m) {
           py::arg("dof"), py::arg("pairs") = py::none(), py::arg("mid_weight") = 0.0, py::arg("mid_phase") = 0.5,
           py::arg("it0") = 0, py::arg("it1") = -1, py::arg("min_boxes") = 20, py::arg("max_boxes") = 1000,
           py::arg("callback") = py::none());
+    m.def("mid_near_pairs_dense", &mid_near_pairs_dense, "PaCMAP-style mid-near partners on the device", py::arg("X"),
+          py::arg("n_pairs"), py::arg("n_cand"), py::arg("seed"), py::arg("n_threads") = -1);
+    m.def("nn_descent", &nn_descent, "NN-descent k-NN graph on the device: (indices, squared distances), rows sorted",
+          py::arg("X"), py::arg("k"), py::arg("max_candidates") = 50, py::arg("n_iters") = 10, py::arg("delta") = 0.001f,
+          py::arg("n_trees") = 4, py::arg("leaf_size") = 30, py::arg("seed") = 0, py::arg("n_threads") = -1,
+          py::arg("verbose") = false);
     m.def("optimize_layout", &optimize_layout, "Sampled SGD layout optimisation on the device (in place)",
           py::arg("Y"), py::arg("head"), py::arg("tail"), py::arg("epochs_per_sample"),
           py::arg("epoch_of_next_sample"), py::arg("epoch_of_next_negative_sample"),

@@ -175,6 +175,13 @@ void launch_perplexity(const float* d2, i64 n, int k, double perplexity, double 
 // symmetrisation of a directed k-NN graph into CSR (mode 0: (a + b) * param; 1: fuzzy union mixed by param); scratch of
 // symmetrize_scratch_ints(n, k) ints; returns nnz, or -nnz when nnz exceeds cap (nothing written but the row pointers)
 i64 symmetrize_scratch_ints(i64 n, int k);
+// exclusive scan of n ints into out[0 .. n] (out[n] the total), with scan_tmp_ints(n) ints of scratch
+i64 scan_tmp_ints(i64 n);
+void launch_exscan(const int* a, i64 n, int* out, int* tmp, hipStream_t st);
+// NN-descent k-NN graph of the rows of X (n, d): out_idx / out_dist (n, K), squared distances, sorted; synchronises st
+void nn_descent(const float* X, i64 n, int d, int K, int max_candidates, int n_iters, float delta, int n_trees, int leaf_size,
+                uint64_t seed, int32_t* out_idx, float* out_dist, int verbose, hipStream_t st);
+void mid_near_pairs(const float* X, i64 n, int d, int n_pairs, int n_cand, uint64_t seed, int32_t* out, hipStream_t st);
 i64 symmetrize_knn(int mode, const int32_t* idx, const double* val, i64 n, int k, double param, int32_t* indptr,
                    int32_t* out_idx, float* out_val, i64 cap, int* scratch, hipStream_t st);
 

@@ -1367,6 +1367,21 @@ int mg_symmetrize_knn(int mode, const int32_t* idx, const double* val, int64_t n
     MG_API_END
 }
 
+int mg_nn_descent(const float* X, int64_t n, int d, int k, int max_candidates, int n_iters, float delta, int n_trees,
+                  int leaf_size, uint64_t seed, int verbose, int32_t* idx, float* dist2) {
+    MG_API_BEGIN
+    require(X && idx && dist2, "null array");
+    nn_descent(X, n, d, k, max_candidates, n_iters, delta, n_trees, leaf_size, seed, idx, dist2, verbose, lib_stream());
+    MG_API_END
+}
+
+int mg_mid_near_pairs(const float* X, int64_t n, int d, int n_pairs, int n_cand, uint64_t seed, int32_t* out) {
+    MG_API_BEGIN
+    require(X && out, "null array");
+    mid_near_pairs(X, n, d, n_pairs, n_cand, seed, out, lib_stream());
+    MG_API_END
+}
+
 int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
     MG_API_BEGIN
     require(rows == 0 || (D && idx && val), "null array");

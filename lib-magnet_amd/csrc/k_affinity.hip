@@ -314,6 +314,8 @@ void exscan(const int* a, i64 n, int* out, int* tmp, hipStream_t st) {
 
 i64 scan_tmp_ints(i64 n) { return std::max<i64>(1, (n + TILE - 1) / TILE) + 1; }
 
+void launch_exscan(const int* a, i64 n, int* out, int* tmp, hipStream_t st) { exscan(a, n, out, tmp, st); }
+
 void launch_perplexity(const float* d2, i64 n, int k, double perplexity, double tol, int n_iter, double* P, double* beta,
                        hipStream_t st) {
     require(k >= 1 && k <= 512, "the perplexity calibration takes 1 to 512 neighbours per row");

@@ -94,6 +94,8 @@ def lib():
             "mg_knn_noalloc": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
             "mg_smooth_knn": ([vp, i64, i32, i32, vp, vp], i32),
             "mg_perplexity_rows": ([vp, i64, i32, C.c_double, C.c_double, i32, vp, vp], i32),
+            "mg_nn_descent": ([vp, i64, i32, i32, i32, i32, C.c_float, i32, i32, C.c_uint64, i32, vp, vp], i32),
+            "mg_mid_near_pairs": ([vp, i64, i32, i32, i32, C.c_uint64, vp], i32),
             "mg_symmetrize_knn": ([i32, vp, vp, i64, i32, C.c_double, vp, vp, vp, i64, C.POINTER(C.c_int64)], i32),
         }
         for name, (args, res) in sig.items():
@@ -397,6 +399,31 @@ def symmetrize_knn(idx, val, mode=0, param=None):
     check(lib().mg_symmetrize_knn(int(mode), ptr(idx), ptr(val), n, k, float(param), ptr(indptr), ptr(indices), ptr(data),
                                   cap, C.byref(nnz)))
     return indptr, indices[: nnz.value].clone(), data[: nnz.value].clone()
+
+
+def nn_descent(X, k, max_candidates=50, n_iters=10, delta=0.001, n_trees=4, leaf_size=30, seed=0, verbose=False):
+    """(indices (n, k) int32, squared distances (n, k) float32) of the NN-descent graph of the rows of X (contiguous
+    (n, d) float32 device tensor), rows sorted by (distance, index)."""
+    import torch
+    if X.dtype != torch.float32 or X.dim() != 2 or not X.is_contiguous():
+        raise ValueError("X must be a contiguous (n, d) float32 tensor")
+    n = X.shape[0]
+    idx = torch.empty((n, k), dtype=torch.int32, device=X.device)
+    d2 = torch.empty((n, k), dtype=torch.float32, device=X.device)
+    check(lib().mg_nn_descent(ptr(X), n, X.shape[1], int(k), int(max_candidates), int(n_iters), float(delta), int(n_trees),
+                              int(leaf_size), int(seed) & 0xFFFFFFFFFFFFFFFF, 1 if verbose else 0, ptr(idx), ptr(d2)))
+    return idx, d2
+
+
+def mid_near_pairs(X, n_pairs=5, n_cand=6, seed=0):
+    """(n, n_pairs) int32 device tensor: the second nearest of n_cand hashed draws per pair (mg_mid_near_pairs)."""
+    import torch
+    if X.dtype != torch.float32 or X.dim() != 2 or not X.is_contiguous():
+        raise ValueError("X must be a contiguous (n, d) float32 tensor")
+    out = torch.empty((X.shape[0], n_pairs), dtype=torch.int32, device=X.device)
+    check(lib().mg_mid_near_pairs(ptr(X), X.shape[0], X.shape[1], int(n_pairs), int(n_cand), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                  ptr(out)))
+    return out
 
 
 def mem_info():

@@ -416,3 +416,23 @@ def test_smooth_knn_hits_the_target_sum():
     assert v.shape == d.shape and (v > 0).all() and (v <= 1).all()
     assert np.allclose(v[:, 0], 1.0)
     assert np.abs(v.sum(1) - np.log2(15)).max() < 1e-3
+
+
+def test_mid_near_rule_and_its_hash():
+    """mid_near_pairs: the draws come from splitmix64's finaliser (its published first output from state 0 is
+    0xE220A8397B1DCDAF); every partner is a drawn row other than the row itself, and at most one drawn row is
+    strictly nearer (the second nearest, src/nndescent.cpp:730-759)."""
+    from affinity_oracle import _mix64, hash4, mid_near_pairs
+    assert int(_mix64(np.uint64(0))) == 0xE220A8397B1DCDAF
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(300, 4)).astype(np.float32)
+    out = mid_near_pairs(X, 3, 6, seed=11)
+    for i in (0, 7, 299):
+        for p in range(3):
+            j = (hash4(11, i, p, np.arange(6)) % np.uint64(300)).astype(np.int64)
+            j = np.where(j == i, (j + 1) % 300, j)
+            d = ((X[j].astype(np.float64) - X[i]) ** 2).sum(1)
+            partner = out[i, p]
+            assert partner != i and partner in j
+            dp = ((X[partner].astype(np.float64) - X[i]) ** 2).sum()
+            assert (d < dp).sum() <= 1
