@@ -1,11 +1,11 @@
 """Host side of MAGNET(loss="tsne"): parameter checks as the reference's (tests/test_magnet.py:425-428), and the
-mid-near rule, the data preparation and the server's admission logic (the memberships and the fuzzy union run on the
-device and are tested there)."""
+data preparation and the server's admission logic (the memberships, the fuzzy union, the mid-near pairs and the k-NN
+graph run on the device and are tested there)."""
 import numpy as np
 import pytest
 
 from magnet import MAGNET
-from magnet.graph import mid_near_pairs, prepare_data
+from magnet.graph import prepare_data
 
 
 def test_parameter_errors():
@@ -22,19 +22,6 @@ def test_parameter_errors():
             m._kernel_params(50)
     with pytest.raises(ValueError):
         MAGNET(loss="umap", transform_mode="beam")._validate()
-
-
-def test_mid_near_pairs_follow_the_rule():
-    rng = np.random.default_rng(3)
-    X = rng.normal(size=(400, 6)).astype(np.float32)
-    pairs = mid_near_pairs(X, 3, 6, random_state=7)
-    assert pairs.shape == (400, 3) and (pairs != np.arange(400)[:, None]).all()
-    # replay the candidates of the same generator: each pair is the second closest of its six candidates
-    cand = np.random.default_rng(7).integers(0, 400, (400, 3, 6))
-    cand = np.where(cand == np.arange(400)[:, None, None], (cand + 1) % 400, cand)
-    d = ((X[cand] - X[:, None, None, :]) ** 2).sum(-1)
-    second = np.take_along_axis(cand, np.argsort(d, axis=-1, kind="stable")[..., 1:2], -1)[..., 0]
-    assert np.array_equal(pairs, second)
 
 
 def test_prepare_data_cosine_rows():
