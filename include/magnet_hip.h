@@ -140,8 +140,10 @@ int mg_fftconv_grids(mg_fftconv* conv, void* d_W, void* d_phi);
  * like the rows of the training P) and the repulsion of the training layout, read from the potential field `conv`
  * holds from its last mg_fftconv_compute (call it on Y_train first); gradient rho A - F/Z with Z the training
  * layout's, gains and momentum as in the training update (momentum0 before iteration momentum_switch, then
- * momentum1). The training layout is not moved, and every new point is independent of the others: a point's
- * result does not depend on which points share the call. */
+ * momentum1). A point outside the field's grid takes the far-field repulsion of the training layout as n_train
+ * points at its centroid (the centroid of the layout of the last mg_fftconv_compute). The training layout is not
+ * moved, and every new point is independent of the others: a point's result does not depend on which points share
+ * the call. */
 int mg_tsne_transform(mg_fftconv* conv, const void* d_Ytrain, int64_t n_train, const int32_t* d_nbr, const void* d_w,
                       int k, void* d_Ynew, int64_t m, int n_iter, double learning_rate, double momentum0, double momentum1,
                       int momentum_switch, double exaggeration, double dof);

@@ -153,7 +153,9 @@ int gather_blocks(i64 n);
 template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_t* nbr, const T* w, int k, T* Yn, i64 m,
                                          const GridT<T>& g, int ni, bool centred, const T* phi, double inv_Z, int n_iter,
                                          double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
-                                         hipStream_t st);
+                                         const double* centroid, hipStream_t st);
+// the mean of a layout (n, 2) in double into c[0], c[1]
+template <class T> void launch_centroid(const T* Y, i64 n, double* c, hipStream_t st);
 
 // the k (<= 32) smallest entries of each row of a (rows, cols) float tile, sorted by (value, column) (k_knn.hip)
 void launch_topk_rows(const float* D, i64 rows, i64 cols, int k, int32_t* oi, float* ov, hipStream_t st);

@@ -216,7 +216,7 @@ template <class T> struct Repulsion {
     std::list<Spec> cache;
     unsigned long tick = 0;
     DevBuf keysA, valsA, keysB, valsB, hist, item_key, item_valid, item_val, W, Wh, Ph, Tr, phi;
-    DevBuf zpart, zout, inv_out, spec_tmp, table;
+    DevBuf zpart, zout, inv_out, spec_tmp, table, centroid;
     // allocations replaced while the loop runs (grid buffers, spectra, as the layout grows): freed by
     // release_dead() after a synchronisation, so that no growth waits for the device
     std::vector<void*> dead;
@@ -1288,6 +1288,7 @@ int mg_fftconv_compute(mg_fftconv* c, const void* Y, int64_t n, double xmin, dou
         require(n_boxes * r.ni <= r.M_user, "grid larger than the buffer");
         const GridT<T> g = make_grid<T>(xmin, ymin, width, n_boxes, r.ni);
         r.compute((const T*)Y, n, g, (T*)F, nullptr, lib_stream());
+        launch_centroid<T>((const T*)Y, n, reinterpret_cast<double*>(r.centroid.ensure(2 * sizeof(double))), lib_stream());
         double z[2];
         MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, lib_stream()));
         MG_HIP(hipStreamSynchronize(lib_stream()));
@@ -1413,7 +1414,7 @@ int mg_tsne_transform(mg_fftconv* c, const void* Ytr, int64_t n_train, const int
         MG_HIP(hipMemcpyAsync(z, r.zout.p, sizeof(z), hipMemcpyDeviceToHost, lib_stream()));
         MG_HIP(hipStreamSynchronize(lib_stream()));
         launch_transform<T>((const T*)Ytr, n_train, nbr, (const T*)w, k, (T*)Yn, m, r.last_g, r.ni, r.centred, dptr<T>(r.phi), z[1],
-                            n_iter, lr, mom0, mom1, mom_switch, exaggeration, dof, lib_stream());
+                            n_iter, lr, mom0, mom1, mom_switch, exaggeration, dof, dptr<double>(r.centroid), lib_stream());
         MG_HIP(hipStreamSynchronize(lib_stream()));
     };
     if (c->prec == MG_F64) run(*c->f64, double(0)); else run(*c->f32, float(0));

@@ -711,7 +711,8 @@ template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void k_transform(const T* __restrict__ Ytr, T inv_ntr, const int32_t* __restrict__ nbr,
                                                    const T* __restrict__ w, int k, T* __restrict__ Yn, i64 m, GridT<T> g,
                                                    const T* __restrict__ phi, T inv_Z, int n_iter, T lr, T mom0, T mom1,
-                                                   int mom_switch, T rho, T inv_dof) {
+                                                   int mom_switch, T rho, T inv_dof, const double* __restrict__ centroid,
+                                                   double n_train, double dof) {
     const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
     if (i >= m) return;
     typedef typename vec2<T>::type V2;
@@ -731,8 +732,17 @@ __global__ __launch_bounds__(TPB) void k_transform(const T* __restrict__ Ytr, T 
         int bx, by;
         T wx[NI], wy[NI];
         double fxu, fyu;
-        weights<T, NI, CEN>(y.x, y.y, box_key<T>(y.x, y.y, g), g, bx, by, wx, wy, fxu, fyu);
         T fx, fy;
+        const double gxu = ((double)y.x - g.xmin_d) * g.inv_w_d, gyu = ((double)y.y - g.ymin_d) / g.width_d;
+        if (!(gxu >= 0.0 && gxu < (double)g.nb && gyu >= 0.0 && gyu < (double)g.nb)) {
+            // outside the training grid the field is not interpolated: the training layout acts as n_train points at
+            // its centroid, sum_j K2(d_ij) (y - y_j) ~ n_train K2(|y - c|) (y - c), which decays like the exact sum
+            const double dx = (double)y.x - centroid[0], dy = (double)y.y - centroid[1];
+            const double k2 = pow(1.0 + (dx * dx + dy * dy) / dof, -dof - 1.0);
+            fx = (T)(n_train * k2 * dx);
+            fy = (T)(n_train * k2 * dy);
+        } else {
+        weights<T, NI, CEN>(y.x, y.y, box_key<T>(y.x, y.y, g), g, bx, by, wx, wy, fxu, fyu);
         if (CEN) {
             double sx = 0, sy = 0;
 #pragma unroll
@@ -763,6 +773,7 @@ __global__ __launch_bounds__(TPB) void k_transform(const T* __restrict__ Ytr, T 
             }
             fx = p1 * y.x - p2; fy = p1 * y.y - p3;
         }
+        }
         const T mom = it < mom_switch ? mom0 : mom1;
         const T g0 = rho * ax - fx * inv_Z, g1 = rho * ay - fy * inv_Z;
         // delta-bar-delta gains and momentum (src/fftrep.cpp:245-255)
@@ -777,18 +788,42 @@ __global__ __launch_bounds__(TPB) void k_transform(const T* __restrict__ Ytr, T 
     reinterpret_cast<V2*>(Yn)[i] = y;
 }
 
+// the mean of a layout (n, 2) in double, one block, fixed order: c[0], c[1]
+template <class T>
+__global__ __launch_bounds__(TPB) void k_centroid(const T* __restrict__ Y, i64 n, double* __restrict__ c) {
+    __shared__ double sh[2][TPB];
+    double sx = 0, sy = 0;
+    for (i64 i = threadIdx.x; i < n; i += TPB) { sx += (double)Y[2 * i]; sy += (double)Y[2 * i + 1]; }
+    sh[0][threadIdx.x] = sx;
+    sh[1][threadIdx.x] = sy;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { sh[0][threadIdx.x] += sh[0][threadIdx.x + o]; sh[1][threadIdx.x] += sh[1][threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { c[0] = sh[0][0] / (double)n; c[1] = sh[1][0] / (double)n; }
+}
+
+template <class T> void launch_centroid(const T* Y, i64 n, double* c, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_centroid<T>, dim3(1), dim3(TPB), 0, st, Y, n, c);
+    MG_HIP(hipGetLastError());
+}
+
 template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_t* nbr, const T* w, int k, T* Yn, i64 m,
                                          const GridT<T>& g, int ni, bool centred, const T* phi, double inv_Z, int n_iter,
                                          double lr, double mom0, double mom1, int mom_switch, double rho, double dof,
-                                         hipStream_t st) {
+                                         const double* centroid, hipStream_t st) {
     if (m == 0) return;
     const unsigned blocks = (unsigned)((m + TPB - 1) / TPB);
     const T inv_ntr = (T)(1.0 / (double)n_train);
 #define MG_T(NI) do { \
         if (centred) hipLaunchKernelGGL((k_transform<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, Ytr, inv_ntr, nbr, w, k, Yn, m, g, phi, \
-                                        (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof)); \
+                                        (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof), centroid, \
+                                        (double)n_train, dof); \
         else hipLaunchKernelGGL((k_transform<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, Ytr, inv_ntr, nbr, w, k, Yn, m, g, phi, \
-                                (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof)); \
+                                (T)inv_Z, n_iter, (T)lr, (T)mom0, (T)mom1, mom_switch, (T)rho, (T)(1.0 / dof), centroid, \
+                                (double)n_train, dof); \
     } while (0)
     switch (ni) {
         case 1: MG_T(1); break; case 2: MG_T(2); break; case 3: MG_T(3); break; case 4: MG_T(4); break;
@@ -804,7 +839,8 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
     template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
     template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
     template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t); \
-    template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, hipStream_t); \
+    template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, const double*, hipStream_t); \
+    template void launch_centroid<T>(const T*, i64, double*, hipStream_t); \
     template void launch_keys_batch<T>(const BInst<T>*, const BMap&, const T*, int, int*, int*, hipStream_t); \
     template void launch_spread_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, hipStream_t); \
     template void launch_gather_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, T*, hipStream_t);

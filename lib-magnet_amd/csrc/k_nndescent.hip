@@ -24,6 +24,7 @@
 // against exact search), not in value.
 #include "common.hpp"
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 namespace mg {
@@ -613,6 +614,9 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
     require(K <= KMAX, "k is at most 128 on this path");
     require(max_candidates >= 1 && n_iters >= 0 && leaf_size >= 2 && n_trees >= 0, "invalid NN-descent parameters");
     const int n = (int)n_;
+    auto clock = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    double t_mark = clock();
+    auto lap = [&]() { MG_HIP(hipStreamSynchronize(st)); const double t = clock(), dt = t - t_mark; t_mark = t; return dt; };
     NND s;
     s.X = X; s.n = n; s.d = d; s.K = K; s.m = std::min(std::min(max_candidates, K), SMAX / 2); s.st = st;
     const i64 nK = (i64)n * K;
@@ -664,8 +668,10 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
             int* gsize = gbeg + nseg;
             int* gnew = gsize + nseg;
             hipLaunchKernelGGL(k_leaf_groups, dim3(grid_for(nseg)), dim3(TPB), 0, st, seg_start, nseg, gbeg, gsize, gnew);
+            const double t_split = verbose ? lap() : 0.0;
             const i64 u = s.join(nseg, gbeg, 0, gsize, gnew, perm);
-            if (verbose) fprintf(stderr, "nn-descent tree %d: %d leaves, %lld insertions\n", tree, nseg, (long long)u);
+            if (verbose) fprintf(stderr, "nn-descent tree %d: %d leaves (%.3f s), %lld insertions (%.3f s)\n", tree, nseg, t_split,
+                                 (long long)u, lap());
         }
     }
     // ---- random neighbours for open slots, then every node for rows still open (small n)
@@ -699,6 +705,7 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
             s.apply(P, Pt, Ps, Pd, tcount);
         }
     }
+    if (verbose) fprintf(stderr, "nn-descent random fill: %.3f s\n", lap());
     // ---- rounds (src/nndescent.cpp:485-560)
     {
         const int m = s.m;
@@ -724,8 +731,10 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
                                gnew, gsize);
             hipLaunchKernelGGL(k_unflag, dim3((unsigned)n), dim3(WAVE), 0, st, s.L, s.F, K, members, gnew, n, m);
             MG_HIP(hipGetLastError());
+            const double t_cand = verbose ? lap() : 0.0;
             const i64 u = s.join(n, nullptr, 2 * m, gsize, gnew, members);
-            if (verbose) fprintf(stderr, "nn-descent iteration %d: %lld updates\n", it + 1, (long long)u);
+            if (verbose) fprintf(stderr, "nn-descent iteration %d: candidates %.3f s, join %.3f s, %lld updates\n", it + 1, t_cand, lap(),
+                                 (long long)u);
             if (u <= (i64)(delta * (double)n * K)) break;
         }
     }

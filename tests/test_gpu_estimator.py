@@ -80,6 +80,51 @@ def test_transform_point_does_not_depend_on_its_batch(fitted):
     assert np.allclose(full[5:6], one, rtol=0, atol=1e-5 * np.abs(full).max())
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_transform_point_outside_the_grid_feels_the_far_field(precision):
+    """A new point outside the training layout's grid feels the layout's far-field repulsion, the training points
+    taken at their centroid, which decays like r^-3 (not an extrapolation of the field, which grows with distance).
+    One iteration from rest moves a point by -lr * gain * g, gain 1.2 where g > 0 and 0.8 elsewhere (a learning rate
+    of 1e6 makes the step large against the coordinates' rounding): the gradient recovered from that step matches the
+    exact rho A - F / Z on the host (exact pairwise sums) to 1e-3 at 10 to 50 layout spans; a point inside the grid
+    matches to the field's interpolation error."""
+    import torch
+    from magnet import MAGNET
+    X, y = blobs(2000, d=10, centers=5, seed=5)
+    m = MAGNET(loss="tsne", random_state=0, precision=precision).fit(X[:1700])
+    conv, (Xtr, Ytr, dt, lr) = m._serving_state()
+    Yt = Ytr.double().cpu().numpy()
+    n = len(Yt)
+    span = float(np.abs(Yt).max())
+    k = 14
+    nbr = np.tile(np.arange(k, dtype=np.int32), (5, 1))
+    w = np.full((5, k), 1.0 / k)
+    start = np.array([[10.0, 0.0], [0.0, -50.0], [30.0, 30.0], [-10.0, 7.0], [0.0, 0.0]]) * span
+    start[4] = Yt[:k].mean(0) + 0.1
+    start = start.astype(np.float32).astype(np.float64)   # representable in either precision
+    Yn = torch.as_tensor(start, dtype=dt, device="cuda")
+    lr_big = 1e6
+    conv.transform(Ytr, torch.as_tensor(nbr, device="cuda"), torch.as_tensor(w, dtype=dt, device="cuda"), Yn, n_iter=1,
+                   learning_rate=lr_big, momentum=(0.5, 0.8), momentum_switch=1, exaggeration=1.0, dof=1.0)
+    step = Yn.double().cpu().numpy() - start
+    g_dev = -step / (lr_big * np.where(step < 0, 1.2, 0.8))
+    D2 = ((Yt[:, None] - Yt[None]) ** 2).sum(-1)
+    Q = 1.0 / (1.0 + D2)
+    np.fill_diagonal(Q, 0.0)
+    Z = Q.sum()
+    g_ref = []
+    for i in range(5):
+        d = start[i] - Yt
+        q = 1.0 / (1.0 + (d ** 2).sum(1))
+        A = ((w[i] / n) * q[:k])[:, None] * d[:k]
+        F = ((q ** 2)[:, None] * d).sum(0)
+        g_ref.append(A.sum(0) - F / Z)
+    g_ref = np.array(g_ref)
+    rel = np.linalg.norm(g_dev - g_ref, axis=1) / np.linalg.norm(g_ref, axis=1)
+    print(f"{precision}: layout span {span:.1f}; gradient relative error far {rel[:4]}, inside {rel[4]:.2e}")
+    assert (rel[:4] < 1e-3).all() and rel[4] < 2e-2
+
+
 @pytest.mark.parametrize("n_workers", [1, 3])
 def test_serving_same_alone_and_in_the_mix(fitted, n_workers):
     """Each request's result is its result alone, whichever requests share its batch and whichever batches run
