@@ -34,7 +34,7 @@ namespace {
 constexpr int TPB = 256;
 constexpr int SMAX = 128;                                        // members of one join pass
 constexpr int DC = 32;                                           // features per LDS stage
-constexpr int PPT = (SMAX * (SMAX - 1) / 2 + TPB - 1) / TPB;     // pairs per thread at most
+constexpr int PPT_MAX = (SMAX * (SMAX - 1) / 2 + TPB - 1) / TPB;  // pairs per thread at most
 constexpr int KMAX = 128;                                        // list length at most
 constexpr uint64_t LIST_EMPTY = 0x7F800000FFFFFFFFULL;           // (inf, -1)
 constexpr uint64_t CAND_EMPTY = ~0ULL;
@@ -77,15 +77,27 @@ __global__ __launch_bounds__(TPB) void k_iota(int* __restrict__ a, i64 n) {
 }
 
 // ------------------------------------------------------------------------------------------------ trees
-// key of position p = 2 * segment + side; segments of at most `leaf` points keep side 0
+// tb trees of n points: position q holds point q % n, in segment q / n (tree q / n's root)
+__global__ __launch_bounds__(TPB) void k_rp_init(int* __restrict__ perm, int* __restrict__ seg_of, int* __restrict__ seg_start,
+                                                 int n, int tb) {
+    const int N = n * tb;
+    for (int q = blockIdx.x * TPB + threadIdx.x; q < N; q += gridDim.x * TPB) {
+        perm[q] = q % n;
+        seg_of[q] = q / n;
+        if (q % n == 0) seg_start[q / n] = q;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) seg_start[tb] = N;
+}
+// key of position p (N positions, trees of n) = 2 * segment + side; segments of at most `leaf` points keep side 0
 __global__ __launch_bounds__(TPB) void k_rp_keys(const float* __restrict__ X, int d, const int* __restrict__ perm,
-                                                 const int* __restrict__ seg_of, const int* __restrict__ seg_start, int n,
-                                                 int leaf, uint64_t seed, int tree, int depth, int* __restrict__ keys) {
-    for (int p = blockIdx.x * TPB + threadIdx.x; p < n; p += gridDim.x * TPB) {
+                                                 const int* __restrict__ seg_of, const int* __restrict__ seg_start, int N, int n,
+                                                 int leaf, uint64_t seed, int t0, int depth, int* __restrict__ keys) {
+    for (int p = blockIdx.x * TPB + threadIdx.x; p < N; p += gridDim.x * TPB) {
         const int s = seg_of[p], b0 = seg_start[s], sz = seg_start[s + 1] - b0;
+        const int tree = t0 + p / n;
         int side = 0;
         if (sz > leaf) {
-            const uint64_t h = hash4(seed, 0x7265u + tree, depth, s);
+            const uint64_t h = hash4(seed, 0x7265u + t0, depth, s);
             const int ia = b0 + (int)((h & 0xffffffffu) % (uint32_t)sz);
             int ib = b0 + (int)((h >> 32) % (uint32_t)sz);
             if (ib == ia) ib = b0 + (ia - b0 + 1) % sz;
@@ -96,7 +108,6 @@ __global__ __launch_bounds__(TPB) void k_rp_keys(const float* __restrict__ X, in
         keys[p] = 2 * s + side;
     }
 }
-__global__ void k_set2(int* __restrict__ a, int v0, int v1) { a[0] = v0; a[1] = v1; }
 // first position of every run of equal keys
 __global__ __launch_bounds__(TPB) void k_rp_marks(const int* __restrict__ keys, int n, int* __restrict__ flag) {
     for (int p = blockIdx.x * TPB + threadIdx.x; p < n; p += gridDim.x * TPB) flag[p] = p == 0 || keys[p] != keys[p - 1];
@@ -164,7 +175,7 @@ __device__ inline int block_exscan_i(int v, int* sh, int* total) {
 // One group per block: members[gbeg[g] .. + gsize[g]), the first gnew[g] of them new. Pairs new-new and new-old; a
 // group larger than SMAX is joined as cliques of SMAX consecutive members. WRITE = false: count[g] = proposals that
 // beat an end's worst entry; WRITE = true: those proposals at offs[g] on, in a fixed order, and tcount[target] += 1.
-template <bool WRITE>
+template <bool WRITE, int PPT>
 __global__ __launch_bounds__(TPB) void k_join(const float* __restrict__ X, int d, const int* __restrict__ gbeg, int gstride,
                                               const int* __restrict__ gsize, const int* __restrict__ gnew,
                                               const int* __restrict__ members, int g0, int ng, const uint64_t* __restrict__ L,
@@ -179,6 +190,10 @@ __global__ __launch_bounds__(TPB) void k_join(const float* __restrict__ X, int d
     if (blockIdx.x >= ng) return;
     const i64 beg = gbeg ? (i64)gbeg[g] : (i64)g * gstride;
     const int size = gsize[g], nnew = gnew[g];
+    if (nnew == 0) {   // no new member: no pair to join
+        if (!WRITE && threadIdx.x == 0) count[blockIdx.x] = 0;
+        return;
+    }
     int base = WRITE ? offs[blockIdx.x] : 0;
     int total_count = 0;
     for (int c0 = 0; c0 < size; c0 += SMAX) {
@@ -556,12 +571,25 @@ struct NND {
     }
 
     // joins groups [0, ng): counts, then ranges of groups whose proposals fit `cap`, each written, sorted and merged
-    i64 join(int ng, const int* gbeg, int gstride, const int* gsize, const int* gnew, const int* members) {
+    // the join kernel for groups of at most smax members (its pairs per thread)
+    template <bool WRITE>
+    void launch_join(int smax, int g0, int ng, const int* gbeg, int gstride, const int* gsize, const int* gnew,
+                     const int* members, int* count, const int* offs, int* Pt, int* Ps, float* Pd, int* tcount) {
+        const int s = std::min(smax, SMAX), pmax = s * (s - 1) / 2;
+#define MG_JOIN(P_) hipLaunchKernelGGL((k_join<WRITE, P_>), dim3((unsigned)ng), dim3(TPB), 0, st, X, d, gbeg, gstride, gsize, gnew, \
+                                       members, g0, ng, L, K, count, offs, Pt, Ps, Pd, tcount)
+        if (pmax <= 4 * TPB) MG_JOIN(4);
+        else if (pmax <= 8 * TPB) MG_JOIN(8);
+        else if (pmax <= 20 * TPB) MG_JOIN(20);
+        else MG_JOIN(PPT_MAX);
+#undef MG_JOIN
+        MG_HIP(hipGetLastError());
+    }
+
+    i64 join(int ng, int smax, const int* gbeg, int gstride, const int* gsize, const int* gnew, const int* members) {
         if (ng == 0) return 0;
         int* cnt = bCnt.get<int>((size_t)ng + 1);
-        hipLaunchKernelGGL((k_join<false>), dim3((unsigned)ng), dim3(TPB), 0, st, X, d, gbeg, gstride, gsize, gnew, members, 0, ng,
-                           L, K, cnt, nullptr, nullptr, nullptr, nullptr, nullptr);
-        MG_HIP(hipGetLastError());
+        launch_join<false>(smax, 0, ng, gbeg, gstride, gsize, gnew, members, cnt, nullptr, nullptr, nullptr, nullptr, nullptr);
         std::vector<int> hc(ng);
         MG_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
@@ -572,10 +600,10 @@ struct NND {
             i64 tot = 0;
             int g1 = g0;
             while (g1 < ng && (g1 == g0 || tot + hc[g1] <= cap)) tot += hc[g1++];
-            // counts of this range against the lists as they are now (earlier ranges have merged): no more than before
+            // counts of this range against the lists as they are now: those of the first pass for the first range,
+            // taken again after earlier ranges have merged (no more than before)
             int* c2 = bCnt.get<int>((size_t)ng + 1);
-            hipLaunchKernelGGL((k_join<false>), dim3((unsigned)(g1 - g0)), dim3(TPB), 0, st, X, d, gbeg, gstride, gsize, gnew, members,
-                               g0, g1 - g0, L, K, c2, nullptr, nullptr, nullptr, nullptr, nullptr);
+            if (g0 > 0) launch_join<false>(smax, g0, g1 - g0, gbeg, gstride, gsize, gnew, members, c2, nullptr, nullptr, nullptr, nullptr, nullptr);
             int* offs = bGb.get<int>((size_t)(g1 - g0) + 1);
             launch_exscan(c2, g1 - g0, offs, bTmp.get<int>(scan_tmp_ints(std::max(g1 - g0, n))), st);
             const int P = host_int(offs + (g1 - g0));
@@ -584,9 +612,7 @@ struct NND {
             float* Pd = bPd.get<float>(P);
             int* tcount = bMisc.get<int>((size_t)n + 8);
             hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(TPB), 0, st, tcount, (i64)n, 0);
-            hipLaunchKernelGGL((k_join<true>), dim3((unsigned)(g1 - g0)), dim3(TPB), 0, st, X, d, gbeg, gstride, gsize, gnew, members,
-                               g0, g1 - g0, L, K, nullptr, offs, Pt, Ps, Pd, tcount);
-            MG_HIP(hipGetLastError());
+            launch_join<true>(smax, g0, g1 - g0, gbeg, gstride, gsize, gnew, members, nullptr, offs, Pt, Ps, Pd, tcount);
             updates += apply(P, Pt, Ps, Pd, tcount);
             // the range's counts were re-taken: hc of later groups stays an upper bound
             for (int g = g0; g < g1; ++g) hc[g] = 0;
@@ -627,51 +653,53 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nK)), dim3(TPB), 0, st, s.L, nK, LIST_EMPTY);
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(nK)), dim3(TPB), 0, st, s.F, nK, (uint8_t)0);
 
-    // ---- trees (src/nndescent.cpp:238-330, 414-461)
+    // ---- trees (src/nndescent.cpp:238-330, 414-461), several at once: tree t of a batch owns positions [t n, (t + 1) n)
+    // of one order, its segments never cross into another's, so one sort per level splits every tree's segments
     if (n_trees > 0) {
         const int max_depth = (int)std::ceil(std::log2(std::max(2.0, (double)n / leaf_size))) + 8;
+        const int TB = (int)std::max<i64>(1, std::min<i64>(n_trees, 0x3fffffffLL / n));   // trees per batch (keys < 2^31)
+        const i64 NT = (i64)TB * n;
         Buf bperm, bsegof, bsegst, bflag, bex, bk2, bv2, bhist, btmp;
-        int* perm = bperm.get<int>(n);
-        int* seg_of = bsegof.get<int>(n);
-        int* seg_start = bsegst.get<int>((size_t)n + 1);
-        int* flag = bflag.get<int>(n);
-        int* ex = bex.get<int>((size_t)n + 1);
-        int* keys = bk2.get<int>(2 * (size_t)n);
-        int* vals2 = bv2.get<int>(n);
-        int* hist = bhist.get<int>((size_t)1024 * std::max(1, sort_hist_blocks(n)) + 1024);
-        int* tmp = btmp.get<int>(scan_tmp_ints(n));
+        int* perm = bperm.get<int>(NT);
+        int* seg_of = bsegof.get<int>(NT);
+        int* seg_start = bsegst.get<int>((size_t)NT + 1);
+        int* flag = bflag.get<int>(NT);
+        int* ex = bex.get<int>((size_t)NT + 1);
+        int* keys = bk2.get<int>(2 * (size_t)NT);
+        int* vals2 = bv2.get<int>(NT);
+        int* hist = bhist.get<int>((size_t)1024 * std::max(1, sort_hist_blocks(NT)) + 1024);
+        int* tmp = btmp.get<int>(scan_tmp_ints(NT));
         int* mx = s.misc + 2;
-        for (int tree = 0; tree < n_trees; ++tree) {
-            hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(TPB), 0, st, perm, (i64)n);
-            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(TPB), 0, st, seg_of, (i64)n, 0);
-            int nseg = 1;
-            hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, st, seg_start, 0, n);
-            int maxseg = n;
+        for (int t0 = 0; t0 < n_trees; t0 += TB) {
+            const int tb = std::min(TB, n_trees - t0);
+            const int N = tb * n;
+            hipLaunchKernelGGL(k_rp_init, dim3(grid_for(N)), dim3(TPB), 0, st, perm, seg_of, seg_start, n, tb);
+            int nseg = tb, maxseg = n;
             for (int depth = 0; depth < max_depth && maxseg > leaf_size; ++depth) {
-                hipLaunchKernelGGL(k_rp_keys, dim3(grid_for(n)), dim3(TPB), 0, st, X, d, perm, seg_of, seg_start, n, leaf_size,
-                                   seed, tree, depth, keys);
-                int* kb = keys + n;
-                const int cur = launch_key_sort<float>(keys, perm, kb, vals2, hist, n, bits_for(2 * (i64)nseg), st);
+                hipLaunchKernelGGL(k_rp_keys, dim3(grid_for(N)), dim3(TPB), 0, st, X, d, perm, seg_of, seg_start, N, n, leaf_size,
+                                   seed, t0, depth, keys);
+                int* kb = keys + N;
+                const int cur = launch_key_sort<float>(keys, perm, kb, vals2, hist, N, bits_for(2 * (i64)nseg), st);
                 int* sk = cur == 0 ? keys : kb;
-                if (cur != 0) MG_HIP(hipMemcpyAsync(perm, vals2, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
-                hipLaunchKernelGGL(k_rp_marks, dim3(grid_for(n)), dim3(TPB), 0, st, sk, n, flag);
-                launch_exscan(flag, n, ex, tmp, st);
-                hipLaunchKernelGGL(k_rp_segments, dim3(grid_for(n)), dim3(TPB), 0, st, flag, ex, n, seg_of, seg_start);
+                if (cur != 0) MG_HIP(hipMemcpyAsync(perm, vals2, (size_t)N * sizeof(int), hipMemcpyDeviceToDevice, st));
+                hipLaunchKernelGGL(k_rp_marks, dim3(grid_for(N)), dim3(TPB), 0, st, sk, N, flag);
+                launch_exscan(flag, N, ex, tmp, st);
+                hipLaunchKernelGGL(k_rp_segments, dim3(grid_for(N)), dim3(TPB), 0, st, flag, ex, N, seg_of, seg_start);
                 hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(TPB), 0, st, mx, (i64)1, 0);
-                hipLaunchKernelGGL(k_rp_maxseg, dim3(grid_for(n)), dim3(TPB), 0, st, seg_start, ex + n, mx);
+                hipLaunchKernelGGL(k_rp_maxseg, dim3(grid_for(N)), dim3(TPB), 0, st, seg_start, ex + N, mx);
                 MG_HIP(hipGetLastError());
-                nseg = s.host_int(ex + n);
+                nseg = s.host_int(ex + N);
                 maxseg = s.host_int(mx);
             }
-            // leaf cliques (oversized leaves: cliques of SMAX consecutive members)
+            // leaf cliques, tree after tree (oversized leaves: cliques of SMAX consecutive members)
             int* gbeg = s.bGs.get<int>((size_t)3 * nseg);
             int* gsize = gbeg + nseg;
             int* gnew = gsize + nseg;
             hipLaunchKernelGGL(k_leaf_groups, dim3(grid_for(nseg)), dim3(TPB), 0, st, seg_start, nseg, gbeg, gsize, gnew);
             const double t_split = verbose ? lap() : 0.0;
-            const i64 u = s.join(nseg, gbeg, 0, gsize, gnew, perm);
-            if (verbose) fprintf(stderr, "nn-descent tree %d: %d leaves (%.3f s), %lld insertions (%.3f s)\n", tree, nseg, t_split,
-                                 (long long)u, lap());
+            const i64 u = s.join(nseg, maxseg, gbeg, 0, gsize, gnew, perm);
+            if (verbose) fprintf(stderr, "nn-descent trees %d-%d: %d leaves (%.3f s), %lld insertions (%.3f s)\n", t0, t0 + tb - 1, nseg,
+                                 t_split, (long long)u, lap());
         }
     }
     // ---- random neighbours for open slots, then every node for rows still open (small n)
@@ -732,7 +760,7 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
             hipLaunchKernelGGL(k_unflag, dim3((unsigned)n), dim3(WAVE), 0, st, s.L, s.F, K, members, gnew, n, m);
             MG_HIP(hipGetLastError());
             const double t_cand = verbose ? lap() : 0.0;
-            const i64 u = s.join(n, nullptr, 2 * m, gsize, gnew, members);
+            const i64 u = s.join(n, 2 * m, nullptr, 2 * m, gsize, gnew, members);
             if (verbose) fprintf(stderr, "nn-descent iteration %d: candidates %.3f s, join %.3f s, %lld updates\n", it + 1, t_cand, lap(),
                                  (long long)u);
             if (u <= (i64)(delta * (double)n * K)) break;
