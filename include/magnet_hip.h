@@ -152,7 +152,8 @@ int mg_tsne_transform(mg_fftconv* conv, const void* d_Ytrain, int64_t n_train, c
  * magnet/__init__.py:876-900): for each row of the (rows, cols) float32 device tile D, the k (1..32) smallest
  * entries sorted by (value, column): d_idx / d_val (rows, k). */
 int mg_topk_rows(const float* d_D, int64_t rows, int64_t cols, int k, int32_t* d_idx, float* d_val);
-/* The whole serving k-NN search for rows of d <= 64 float32 features (the reference's k-NN query, magnet/knn.py):
+/* The whole serving k-NN search for rows of d float32 features (the reference's k-NN query, magnet/knn.py; rows of
+ * more than 64 features go through the features in chunks of 64, with the same sums):
  * for each of the m query rows d_Q, the k (1..32) nearest of the nt training rows d_T, exact squared euclidean
  * distances summed over the features in order, sorted by (distance, index): d_idx / d_dist2 (m, k).
  * self_offset >= 0 leaves out training row q + self_offset for query q (the graph of the training rows). A query's

@@ -165,6 +165,78 @@ __global__ __launch_bounds__(KNN_Q) void k_knn_chunks(const float* __restrict__ 
     for (int j = 0; j < KM; ++j) { od[j] = bd[j]; oi[j] = bi[j]; }
 }
 
+// The same search for rows of more than 64 features: the features go through in chunks of 32 (the query's chunk
+// in registers, the tile's in LDS), each thread keeping the running distances of the tile's 32 rows, so every
+// distance is still the sum over the features in order; the tile's rows then enter the list in index order.
+template <int KM>
+__global__ __launch_bounds__(KNN_Q) void k_knn_chunks_wide(const float* __restrict__ T, i64 nt, const float* __restrict__ Q,
+                                                           i64 m, int d, i64 chunk, i64 self_offset, float* __restrict__ pd,
+                                                           int* __restrict__ pi, int S) {
+    constexpr int FC = 32, TRW = 32;   // features per stage, training rows per tile
+    __shared__ __align__(16) float sT[TRW][FC];
+    const i64 qi = (i64)blockIdx.x * KNN_Q + threadIdx.x;
+    const int s = blockIdx.y;
+    const float* qrow = Q + (qi < m ? qi : 0) * d;
+    float bd[KM];
+    int bi[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { bd[j] = INFINITY; bi[j] = INT_MAX; }
+    const i64 c0 = (i64)s * chunk, c1 = c0 + chunk < nt ? c0 + chunk : nt;
+    const i64 self = (self_offset >= 0 && qi < m) ? qi + self_offset : -1;
+    for (i64 t0 = c0; t0 < c1; t0 += TRW) {
+        float acc[TRW];
+#pragma unroll
+        for (int r = 0; r < TRW; ++r) acc[r] = 0.f;
+        for (int f0 = 0; f0 < d; f0 += FC) {
+            float q[FC];
+#pragma unroll
+            for (int j = 0; j < FC; ++j) q[j] = (qi < m && f0 + j < d) ? qrow[f0 + j] : 0.f;
+            __syncthreads();
+            for (int e = threadIdx.x; e < TRW * FC; e += KNN_Q) {
+                const int r = e / FC, j = e % FC;
+                sT[r][j] = (t0 + r < c1 && f0 + j < d) ? T[(t0 + r) * d + f0 + j] : 0.f;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < TRW; ++r) {
+                const float4* tr = reinterpret_cast<const float4*>(&sT[r][0]);
+                float a = acc[r];
+#pragma unroll
+                for (int j = 0; j < FC / 4; ++j) {   // features in order; the zero padding adds exact zeros
+                    const float4 t4 = tr[j];
+                    float df = q[4 * j] - t4.x; a += df * df;
+                    df = q[4 * j + 1] - t4.y; a += df * df;
+                    df = q[4 * j + 2] - t4.z; a += df * df;
+                    df = q[4 * j + 3] - t4.w; a += df * df;
+                }
+                acc[r] = a;
+            }
+        }
+        const int rows = (int)(c1 - t0 < TRW ? c1 - t0 : TRW);
+#pragma unroll
+        for (int r = 0; r < TRW; ++r) {
+            const int ix = (int)(t0 + r);
+            if (r >= rows || ix == self) continue;
+            const float x = acc[r];
+            if (!(x < bd[KM - 1])) continue;   // insertion as in k_knn_chunks: ties keep index order
+#pragma unroll
+            for (int j = KM - 1; j >= 1; --j) {
+                const bool cj = x < bd[j], cp = x < bd[j - 1];
+                const float nd = cp ? bd[j - 1] : x;
+                const int ni = cp ? bi[j - 1] : ix;
+                bd[j] = cj ? nd : bd[j];
+                bi[j] = cj ? ni : bi[j];
+            }
+            if (x < bd[0]) { bd[0] = x; bi[0] = ix; }
+        }
+    }
+    if (qi >= m) return;
+    float* od = pd + (qi * S + s) * KM;
+    int* oi = pi + (qi * S + s) * KM;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) { od[j] = bd[j]; oi[j] = bi[j]; }
+}
+
 // one block of P = min(S, 256) threads per query (S and P powers of two): thread t first merges the chunk lists
 // t, t + P, t + 2P, ... in turn, then the P lists are merged as a tree through LDS
 template <int KM>
@@ -223,7 +295,7 @@ void launch_topk_rows(const float* D, i64 rows, i64 cols, int k, int32_t* oi, fl
 }  // namespace mg
 
 namespace mg {
-// k nearest training rows of each query row (features d <= 64): indices and squared distances (m, k), sorted by
+// k nearest training rows of each query row (any number of features): indices and squared distances (m, k), sorted by
 // (distance, index); self_offset >= 0 leaves out training row q + self_offset for query q. Scratch: part_d / part_i,
 // m * S * 32 each, S = knn_chunks(m, nt).
 int knn_chunks(i64 m, i64 nt) {
@@ -244,7 +316,10 @@ static void knn_with_chunks(const float* T, i64 nt, const float* Q, i64 m, int d
         case 1: MG_K(8, KM); break; case 2: MG_K(16, KM); break; case 3: MG_K(24, KM); break; \
         case 4: MG_K(32, KM); break; case 5: MG_K(40, KM); break; case 6: MG_K(48, KM); break; \
         case 7: MG_K(56, KM); break; default: MG_K(64, KM); break; }
-    if (k <= 16) { MG_KD(16) } else { MG_KD(32) }
+    if (d > 64) {
+        if (k <= 16) hipLaunchKernelGGL((k_knn_chunks_wide<16>), g1, dim3(KNN_Q), 0, st, T, nt, Q, m, d, chunk, self_offset, part_d, part_i, S);
+        else hipLaunchKernelGGL((k_knn_chunks_wide<32>), g1, dim3(KNN_Q), 0, st, T, nt, Q, m, d, chunk, self_offset, part_d, part_i, S);
+    } else if (k <= 16) { MG_KD(16) } else { MG_KD(32) }
 #undef MG_KD
 #undef MG_K
     MG_HIP(hipGetLastError());
@@ -257,7 +332,7 @@ static void knn_with_chunks(const float* T, i64 nt, const float* Q, i64 m, int d
 }
 
 static void knn_check(i64 nt, int d, int k, i64 self_offset) {
-    require(d >= 1 && d <= 64, "the fused k-NN search takes up to 64 features");
+    require(d >= 1, "rows must have at least one feature");
     require(k >= 1 && k <= 32 && k <= nt - (self_offset >= 0 ? 1 : 0), "k must be in [1, min(32, rows)]");
 }
 

@@ -322,7 +322,7 @@ def topk_rows(D, k):
 
 
 def knn(T, Q, k, exclude_self=False):
-    """(indices (m, k) int32, squared distances (m, k) float32) of the k nearest rows of T (nt, d <= 64) to each row of
+    """(indices (m, k) int32, squared distances (m, k) float32) of the k nearest rows of T (nt, d) to each row of
     Q (m, d), float32 device tensors; exclude_self (Q is T) leaves each row out of its own list."""
     import torch
     for a in (T, Q):

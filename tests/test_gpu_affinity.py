@@ -80,23 +80,30 @@ def test_symmetrize_equals_the_oracle_bit_for_bit():
 
 
 def test_device_p_builder_equals_the_oracle_chain():
-    """perplexity_affinities on the device against the oracle's calibration and symmetrisation of the same
-    neighbour lists (the device search's): the same sparsity; values within one float32 rounding (the calibrations'
-    sums run in different orders) but for at most 0.1% of them (rows whose bisection took another branch within
-    rounding), and all within 1e-5 relative; P sums to 1 and equals its transpose exactly."""
+    """The device P builder against the oracle's calibration and symmetrisation of the same neighbour lists (the
+    device's exact search and its NN-descent): the same sparsity; values within one float32 rounding (the
+    calibrations' sums run in different orders) but for at most 0.1% of them (rows whose bisection took another
+    branch within rounding), and all within 1e-5 relative; P sums to 1 and equals its transpose exactly;
+    perplexity_affinities is the composition of the search and affinities_from_knn."""
     import torch
-    from magnet.affinity import perplexity_affinities
+    from magnet.affinity import perplexity_affinities, affinities_from_knn
     from magnet.graph import knn_query
     rng = np.random.default_rng(3)
-    for n, d in ((5000, 20), (3000, 100)):   # the fused search (d <= 64) and the tiled one
+    for n, d, how in ((5000, 20, "exact"), (3000, 100, "exact"), (6000, 40, "nndescent")):
         X = (rng.normal(0, 4, (10, d))[rng.integers(0, 10, n)] + rng.normal(0, 1, (n, d))).astype(np.float32)
-        P = perplexity_affinities(X, 30.0)
-        idx, d2 = knn_query(X, X, 90, exclude_self=True, squared=True, tile=4096)
+        P = perplexity_affinities(X, 30.0, knn=how)
+        Xt = torch.as_tensor(X, device="cuda")
+        if how == "exact":
+            idx, d2 = knn_query(X, X, 90, exclude_self=True, squared=True, tile=4096)
+        else:
+            idx, d2 = capi.nn_descent(Xt, 90, 50, 10, 0.001, int(max(2, min(8, round(np.sqrt(n) / 40)))), 30, 0)
+        P2 = affinities_from_knn(idx, d2, 30.0)
+        assert np.array_equal(P.indptr, P2.indptr) and np.array_equal(P.indices, P2.indices) and np.array_equal(P.data, P2.data)
         Pc, _ = AO.perplexity_rows(host(d2), 30.0)
         S = AO.symmetrize_knn(host(idx), Pc, 0)
         assert np.array_equal(P.indptr, S.indptr) and np.array_equal(P.indices, S.indices)
         ulp = np.spacing(np.abs(S.data))
-        print(f"n={n} d={d}: nnz {P.nnz}, values equal {np.mean(P.data == S.data):.5f}, max |diff| / ulp "
+        print(f"n={n} d={d} {how}: nnz {P.nnz}, values equal {np.mean(P.data == S.data):.5f}, max |diff| / ulp "
               f"{(np.abs(P.data - S.data) / ulp).max():.1f}")
         assert np.mean(np.abs(P.data - S.data) > ulp) < 1e-3
         assert np.allclose(P.data, S.data, rtol=1e-5, atol=0)

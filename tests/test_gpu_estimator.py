@@ -177,7 +177,9 @@ def test_topk_rows_matches_a_full_sort(k, cols):
 
 @pytest.mark.parametrize("d,k,exclude,m,nt", [(50, 14, False, 700, 3000), (7, 5, True, 700, 3000), (64, 29, False, 700, 3000),
                                                (1, 3, True, 700, 3000), (12, 14, True, 500, 500),
-                                               (50, 14, False, 5, 200_000), (7, 29, True, 3, 150_000)])
+                                               (50, 14, False, 5, 200_000), (7, 29, True, 3, 150_000),
+                                               (65, 14, False, 300, 3000), (100, 5, True, 300, 3000),
+                                               (784, 29, False, 200, 5000), (300, 14, False, 4, 70_000)])
 def test_fused_knn_is_exact(d, k, exclude, m, nt):
     """mg_knn against a float64 brute force: distances to 1e-5 relative, and the same neighbour set wherever the
     k-th and (k+1)-th distances are apart; exclude_self leaves each row out of its own list; a query's result does
@@ -192,7 +194,11 @@ def test_fused_knn_is_exact(d, k, exclude, m, nt):
     Tt, Qt = torch.as_tensor(T, device="cuda"), torch.as_tensor(Q, device="cuda")
     idx, d2 = capi.knn(Tt, Qt, k, exclude)
     idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
-    D = ((Q.astype(np.float64)[:, None, :] - T.astype(np.float64)[None, :, :]) ** 2).sum(-1)
+    Q64, T64 = Q.astype(np.float64), T.astype(np.float64)
+    if m * nt * d <= 50_000_000:
+        D = ((Q64[:, None, :] - T64[None, :, :]) ** 2).sum(-1)
+    else:
+        D = (Q64 ** 2).sum(1)[:, None] + (T64 ** 2).sum(1)[None, :] - 2.0 * Q64 @ T64.T
     if exclude:
         D[np.arange(m), np.arange(m)] = np.inf
     order = np.argsort(D, axis=1, kind="stable")
@@ -210,7 +216,7 @@ def test_fused_knn_is_exact(d, k, exclude, m, nt):
 
 
 @pytest.mark.parametrize("d,k,exclude,m,nt", [(12, 14, False, 300, 3000), (5, 29, True, 40, 2000), (12, 29, False, 3, 150_000),
-                                               (3, 7, True, 4, 100_000)])
+                                               (3, 7, True, 4, 100_000), (100, 14, False, 300, 3000), (200, 29, True, 40, 2000)])
 def test_knn_ties_break_by_index(d, k, exclude, m, nt):
     """Small integer features make every squared distance an exact integer, with many exact ties: mg_knn must
     give the stable sort's (distance, index) order exactly, across chunk boundaries and through the merge
@@ -221,7 +227,8 @@ def test_knn_ties_break_by_index(d, k, exclude, m, nt):
     T = rng.integers(0, 4, (nt, d)).astype(np.float32)
     Q = T[:m].copy() if exclude else rng.integers(0, 4, (m, d)).astype(np.float32)
     idx, d2 = capi.knn(torch.as_tensor(T, device="cuda"), torch.as_tensor(Q, device="cuda"), k, exclude)
-    D = ((Q.astype(np.int64)[:, None, :] - T.astype(np.int64)[None, :, :]) ** 2).sum(-1).astype(np.float64)
+    Qi, Ti = Q.astype(np.int64), T.astype(np.int64)
+    D = ((Qi ** 2).sum(1)[:, None] + (Ti ** 2).sum(1)[None, :] - 2 * Qi @ Ti.T).astype(np.float64)
     if exclude:
         D[np.arange(m), np.arange(m)] = np.inf
     order = np.argsort(D, axis=1, kind="stable")[:, :k]
