@@ -204,6 +204,13 @@ int mg_nn_descent(const float* d_X, int64_t n, int d, int k, int max_candidates,
  * second nearest (squared euclidean, ties by index) of n_cand rows drawn by a hash of (seed, i, p, c) (a draw equal to
  * i moves to i + 1). The reference draws from per-thread generators: the partners agree in law, not in value. */
 int mg_mid_near_pairs(const float* d_X, int64_t n, int d, int n_pairs, int n_cand, uint64_t seed, int32_t* d_out);
+/* The exact k-NN graph of the rows of d_X (n, d) float32, k <= 128: for each row its k nearest other rows by squared
+ * euclidean distance (one fused multiply-add per feature, in order), sorted by (distance, index), as d_idx (n, k)
+ * and d_dist2 (n, k). NN-descent's lists bound the search (every true neighbour lies within the k-th distance of the
+ * approximate list), and a scan of all pairs keeps the rows within that bound. No counterpart in the reference,
+ * whose exact search is brute force in Python (benchmarks); the graph of the perplexity affinities the
+ * configurations use. Allocates its scratch (about 16 k n bytes and NN-descent's); synchronises the library stream. */
+int mg_knn_exact_graph(const float* d_X, int64_t n, int d, int k, int32_t* d_idx, float* d_dist2);
 
 /* ---- sampled layout: optimize_layout (src/layout.cpp:343-472, src/bindings.cpp) ----------------------------------
  * Epochs [epoch_start, epoch_end) of MAGNET's sampled pass on the device, for loss "umap" with the power kernel

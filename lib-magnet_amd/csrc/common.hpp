@@ -184,6 +184,8 @@ void launch_exscan(const int* a, i64 n, int* out, int* tmp, hipStream_t st);
 void nn_descent(const float* X, i64 n, int d, int K, int max_candidates, int n_iters, float delta, int n_trees, int leaf_size,
                 uint64_t seed, int32_t* out_idx, float* out_dist, int verbose, hipStream_t st);
 void mid_near_pairs(const float* X, i64 n, int d, int n_pairs, int n_cand, uint64_t seed, int32_t* out, hipStream_t st);
+// exact k nearest other rows of every row of X (k <= 128), NN-descent's lists as the bound; synchronises st
+void knn_self_exact(const float* X, i64 n, int d, int K, int32_t* out_idx, float* out_dist, hipStream_t st);
 i64 symmetrize_knn(int mode, const int32_t* idx, const double* val, i64 n, int k, double param, int32_t* indptr,
                    int32_t* out_idx, float* out_val, i64 cap, int* scratch, hipStream_t st);
 

@@ -1383,6 +1383,13 @@ int mg_mid_near_pairs(const float* X, int64_t n, int d, int n_pairs, int n_cand,
     MG_API_END
 }
 
+int mg_knn_exact_graph(const float* X, int64_t n, int d, int k, int32_t* idx, float* dist2) {
+    MG_API_BEGIN
+    require(X && idx && dist2, "null array");
+    knn_self_exact(X, n, d, k, idx, dist2, lib_stream());
+    MG_API_END
+}
+
 int mg_topk_rows(const float* D, int64_t rows, int64_t cols, int k, int32_t* idx, float* val) {
     MG_API_BEGIN
     require(rows == 0 || (D && idx && val), "null array");

@@ -506,6 +506,100 @@ __global__ __launch_bounds__(TPB) void k_mid_near(const float* __restrict__ X, i
     }
 }
 
+// ------------------------------------------------------------------------------------------------ exact graph
+// Exact k nearest other rows of every row, bounded by an approximate graph: tau[q] = the largest distance of q's
+// approximate list, so q's true k nearest all lie within tau[q]. A scan over all (query, row) pairs, 64 x 64 per
+// tile (4 x 4 per thread, features through LDS 32 at a time, each distance one fused multiply-add per feature in
+// order, as NN-descent forms it), appends every row within tau[q] to q's buffer; the buffer is then sorted by
+// (distance, index) and its first k kept.
+constexpr int XQ = 64, XT = 64, XF = 32, XSPAN = 4096;
+__global__ __launch_bounds__(TPB) void k_knn_threshold(const float* __restrict__ X, int n, int d, const float* __restrict__ tau,
+                                                       int cap, int* __restrict__ cnt, uint64_t* __restrict__ buf) {
+    __shared__ float sQ[XQ][XF + 1], sT[XT][XF + 1];
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    const int qb = blockIdx.x * XQ, tb = blockIdx.y * XSPAN, te = min(n, tb + XSPAN);
+    int qi[4];
+    float tq[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        qi[i] = qb + ty * 4 + i;
+        tq[i] = qi[i] < n ? tau[qi[i]] : -1.0f;
+    }
+    for (int t0 = tb; t0 < te; t0 += XT) {
+        float acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+        for (int f0 = 0; f0 < d; f0 += XF) {
+            const int fc = min(XF, d - f0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < XQ * XF; e += TPB) {
+                const int r = e / XF, f = e % XF;
+                sQ[r][f] = (qb + r < n && f < fc) ? X[(i64)(qb + r) * d + f0 + f] : 0.f;
+                sT[r][f] = (t0 + r < te && f < fc) ? X[(i64)(t0 + r) * d + f0 + f] : 0.f;
+            }
+            __syncthreads();
+            for (int f = 0; f < fc; ++f) {
+                float a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { a[i] = sQ[ty * 4 + i][f]; b[i] = sT[tx * 4 + i][f]; }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float t = a[i] - b[j];
+                        acc[i][j] = fmaf(t, t, acc[i][j]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int t = t0 + tx * 4 + j;
+                if (qi[i] < n && t < te && t != qi[i] && acc[i][j] <= tq[i]) {
+                    const int pos = atomicAdd(&cnt[qi[i]], 1);
+                    if (pos < cap) buf[(i64)qi[i] * cap + pos] = pack(acc[i][j], t);
+                }
+            }
+    }
+}
+__global__ __launch_bounds__(TPB) void k_tau(const float* __restrict__ dist, int n, int K, float* __restrict__ tau) {
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) tau[i] = dist[(i64)i * K + K - 1];
+}
+__global__ __launch_bounds__(TPB) void k_max_i32(const int* __restrict__ a, int n, int* __restrict__ mx) {
+    int m = 0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) m = max(m, a[i]);
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if (lane_id() == 0) atomicMax(mx, m);
+}
+// one row per block: its min(cnt, cap) candidates sorted by (distance, index) in LDS (bitonic, P2 slots), first K out
+template <int P2>
+__global__ __launch_bounds__(TPB) void k_select_sorted(const uint64_t* __restrict__ buf, const int* __restrict__ cnt, int cap,
+                                                       int K, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    __shared__ uint64_t sk[P2];
+    const int q = blockIdx.x;
+    const int c = min(cnt[q], cap);
+    for (int e = threadIdx.x; e < P2; e += TPB) sk[e] = e < c ? buf[(i64)q * cap + e] : ~0ULL;
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1)
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+            for (int e = threadIdx.x; e < P2 / 2; e += TPB) {
+                const int lo = 2 * e - (e & (stride - 1)), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = sk[lo], b = sk[hi];
+                if ((a > b) == up) { sk[lo] = b; sk[hi] = a; }
+            }
+            __syncthreads();
+        }
+    for (int j = threadIdx.x; j < K; j += TPB) {
+        const uint64_t k = sk[j];
+        out_idx[(i64)q * K + j] = k == ~0ULL ? -1 : key_node(k);
+        out_dist[(i64)q * K + j] = __uint_as_float((uint32_t)(k >> 32));
+    }
+}
+
 unsigned grid_for(i64 n) {
     const i64 b = (n + TPB - 1) / TPB;
     return (unsigned)(b < 1 ? 1 : b > 65536 ? 65536 : b);
@@ -630,6 +724,47 @@ void mid_near_pairs(const float* X, i64 n, int d, int n_pairs, int n_cand, uint6
     require(n < (1ll << 31) && d >= 1, "at most 2^31 - 1 rows of at least one column");
     hipLaunchKernelGGL(k_mid_near, dim3(grid_for(n * n_pairs)), dim3(TPB), 0, st, X, d, (int)n, n_pairs, n_cand, seed, out);
     MG_HIP(hipGetLastError());
+}
+
+void knn_self_exact(const float* X, i64 n_, int d, int K, int32_t* out_idx, float* out_dist, hipStream_t st) {
+    require(n_ >= 2 && n_ < (1ll << 30) && d >= 1, "the exact graph takes 2 to 2^30 - 1 rows of at least one column");
+    require(K >= 1 && K < n_ && K <= KMAX, "k must be between 1 and min(n - 1, 128)");
+    require(n_ <= (i64)65535 * XSPAN, "the exact graph takes at most 65535 * 4096 rows");
+    const int n = (int)n_;
+    const int n_trees = (int)std::max(2.0, std::min(8.0, std::round(std::sqrt((double)n) / 40.0)));
+    nn_descent(X, n, d, K, 50, 10, 0.001f, n_trees, 30, 0, out_idx, out_dist, 0, st);   // the bound
+    Buf btau, bcnt, bbuf, bmx;
+    float* tau = btau.get<float>(n);
+    int* cnt = bcnt.get<int>(n);
+    int* mx = bmx.get<int>(1);
+    hipLaunchKernelGGL(k_tau, dim3(grid_for(n)), dim3(TPB), 0, st, out_dist, n, K, tau);
+    int cap = std::max(2 * K, 64);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        uint64_t* buf = bbuf.get<uint64_t>((size_t)n * cap);
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(TPB), 0, st, cnt, (i64)n, 0);
+        hipLaunchKernelGGL(k_knn_threshold, dim3((unsigned)((n + XQ - 1) / XQ), (unsigned)((n + XSPAN - 1) / XSPAN)), dim3(TPB), 0, st,
+                           X, n, d, tau, cap, cnt, buf);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(TPB), 0, st, mx, (i64)1, 0);
+        hipLaunchKernelGGL(k_max_i32, dim3(grid_for(n)), dim3(TPB), 0, st, cnt, n, mx);
+        MG_HIP(hipGetLastError());
+        int h_mx = 0;
+        MG_HIP(hipMemcpyAsync(&h_mx, mx, sizeof(int), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipStreamSynchronize(st));
+        require(h_mx >= K, "the exact search found fewer than k rows within the bound");
+        if (h_mx <= cap) {
+#define MG_SEL(P) hipLaunchKernelGGL((k_select_sorted<P>), dim3((unsigned)n), dim3(TPB), 0, st, buf, cnt, cap, K, out_idx, out_dist)
+            if (cap <= 256) MG_SEL(256);
+            else if (cap <= 1024) MG_SEL(1024);
+            else MG_SEL(4096);
+#undef MG_SEL
+            MG_HIP(hipGetLastError());
+            MG_HIP(hipStreamSynchronize(st));
+            return;
+        }
+        require(h_mx <= 4096, "more than 4096 rows tie within a row's k-th distance");
+        cap = h_mx;   // a second scan with room for every row's candidates (ties of duplicate rows)
+    }
+    require(false, "the exact search did not settle");
 }
 
 void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_iters, float delta, int n_trees,

@@ -96,6 +96,7 @@ def lib():
             "mg_perplexity_rows": ([vp, i64, i32, C.c_double, C.c_double, i32, vp, vp], i32),
             "mg_nn_descent": ([vp, i64, i32, i32, i32, i32, C.c_float, i32, i32, C.c_uint64, i32, vp, vp], i32),
             "mg_mid_near_pairs": ([vp, i64, i32, i32, i32, C.c_uint64, vp], i32),
+            "mg_knn_exact_graph": ([vp, i64, i32, i32, vp, vp], i32),
             "mg_symmetrize_knn": ([i32, vp, vp, i64, i32, C.c_double, vp, vp, vp, i64, C.POINTER(C.c_int64)], i32),
         }
         for name, (args, res) in sig.items():
@@ -412,6 +413,19 @@ def nn_descent(X, k, max_candidates=50, n_iters=10, delta=0.001, n_trees=4, leaf
     d2 = torch.empty((n, k), dtype=torch.float32, device=X.device)
     check(lib().mg_nn_descent(ptr(X), n, X.shape[1], int(k), int(max_candidates), int(n_iters), float(delta), int(n_trees),
                               int(leaf_size), int(seed) & 0xFFFFFFFFFFFFFFFF, 1 if verbose else 0, ptr(idx), ptr(d2)))
+    return idx, d2
+
+
+def knn_exact_graph(X, k):
+    """(indices (n, k) int32, squared distances (n, k) float32): the exact k nearest other rows of every row of X
+    (contiguous (n, d) float32 device tensor, k <= 128), sorted by (distance, index)."""
+    import torch
+    if X.dtype != torch.float32 or X.dim() != 2 or not X.is_contiguous():
+        raise ValueError("X must be a contiguous (n, d) float32 tensor")
+    n = X.shape[0]
+    idx = torch.empty((n, k), dtype=torch.int32, device=X.device)
+    d2 = torch.empty((n, k), dtype=torch.float32, device=X.device)
+    check(lib().mg_knn_exact_graph(ptr(X), n, X.shape[1], int(k), ptr(idx), ptr(d2)))
     return idx, d2
 
 

@@ -87,14 +87,13 @@ def test_device_p_builder_equals_the_oracle_chain():
     perplexity_affinities is the composition of the search and affinities_from_knn."""
     import torch
     from magnet.affinity import perplexity_affinities, affinities_from_knn
-    from magnet.graph import knn_query
     rng = np.random.default_rng(3)
     for n, d, how in ((5000, 20, "exact"), (3000, 100, "exact"), (6000, 40, "nndescent")):
         X = (rng.normal(0, 4, (10, d))[rng.integers(0, 10, n)] + rng.normal(0, 1, (n, d))).astype(np.float32)
         P = perplexity_affinities(X, 30.0, knn=how)
         Xt = torch.as_tensor(X, device="cuda")
         if how == "exact":
-            idx, d2 = knn_query(X, X, 90, exclude_self=True, squared=True, tile=4096)
+            idx, d2 = capi.knn_exact_graph(Xt, 90)
         else:
             idx, d2 = capi.nn_descent(Xt, 90, 50, 10, 0.001, int(max(2, min(8, round(np.sqrt(n) / 40)))), 30, 0)
         P2 = affinities_from_knn(idx, d2, 30.0)

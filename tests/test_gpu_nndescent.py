@@ -191,3 +191,42 @@ def test_mid_near_pairs_equal_the_oracle():
     print(f"mid-near partners equal on {np.mean(got == want):.5f}")
     assert np.mean(got == want) > 0.999 and not (got == np.arange(5000)[:, None]).any()
     assert np.array_equal(_magnet.mid_near_pairs_dense(X, 5, 6, 9, -1), got)
+
+
+@pytest.mark.parametrize("n,d,k,kind", [(3000, 20, 90, "normal"), (2000, 5, 30, "integer"), (4000, 784, 90, "normal"),
+                                        (500, 5, 100, "duplicates"), (1000, 3, 5, "two_values")])
+def test_exact_graph_equals_brute_force(n, d, k, kind):
+    """mg_knn_exact_graph against a float64 brute force: distances to 1e-5 and the same neighbour set wherever the
+    k-th and (k+1)-th distances are apart; with integer features (exact distances, many ties) the stable
+    (distance, index) order exactly; duplicate rows (ties at distance 0 beyond the bound's buffer: the second scan)."""
+    import torch
+    rng = np.random.default_rng(n + d)
+    if kind == "normal":
+        X = rng.normal(0, 1, (n, d)).astype(np.float32)
+    elif kind == "integer":
+        X = rng.integers(0, 4, (n, d)).astype(np.float32)
+    elif kind == "duplicates":
+        X = np.repeat(rng.normal(size=(10, d)).astype(np.float32), n // 10, axis=0)
+    else:
+        X = np.repeat(np.array([[0, 0, 0], [1, 2, 3]], np.float32), n // 2, axis=0)
+    idx, d2 = capi.knn_exact_graph(torch.as_tensor(X, device="cuda"), k)
+    idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
+    X64 = X.astype(np.float64)
+    D = (X64 ** 2).sum(1)[:, None] + (X64 ** 2).sum(1)[None, :] - 2.0 * X64 @ X64.T
+    if kind != "normal":
+        D = ((X64[:, None, :] - X64[None, :, :]) ** 2).sum(-1) if n * n * d <= 30_000_000 else D
+    np.fill_diagonal(D, np.inf)
+    order = np.argsort(D, axis=1, kind="stable")
+    ref = np.take_along_axis(D, order, 1)
+    assert not (idx == np.arange(n)[:, None]).any() and (idx >= 0).all()
+    if kind == "normal":
+        assert np.allclose(d2, ref[:, :k], rtol=1e-5, atol=1e-5)
+        clear = (ref[:, k] - ref[:, k - 1]) > 1e-4 * np.maximum(ref[:, k], 1e-12)
+        bad = [q for q in np.nonzero(clear)[0] if set(idx[q]) != set(order[q, :k])]
+        assert not bad, bad[:5]
+    else:   # exact ties (integer features or repeated rows): the stable order itself
+        assert np.array_equal(idx, order[:, :k].astype(np.int32))
+        if kind == "duplicates":
+            assert np.allclose(d2, ref[:, :k], rtol=1e-5, atol=1e-6)
+        else:
+            assert np.array_equal(d2.astype(np.float64), ref[:, :k])
