@@ -13,8 +13,8 @@ namespace mg {
 namespace {
 
 constexpr int TPB = 256;
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = TPB * RS_ITEMS;
+// items per thread of a sort pass: 8, or 2 below 262,144 keys, where the blocks of a pass, not its bytes, bound it
+inline int rs_items(i64 n) { return n <= 262144 ? 2 : 8; }
 constexpr int SP_PT = 4;                 // points per thread in the spread
 constexpr int SP_CHUNK = TPB * SP_PT;    // points per spread chunk (one workgroup)
 
@@ -54,14 +54,14 @@ template <int NI> __device__ inline double node_offset(double f, double width, i
 // bases themselves.
 template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
-                                                 i64 n, int shift, int* __restrict__ hist, int nblocks) {
+                                                 i64 n, int shift, int* __restrict__ hist, int nblocks, int items) {
     constexpr int NBK = 1 << DB;
     __shared__ int h[NBK];
     for (int d = threadIdx.x; d < NBK; d += TPB) h[d] = 0;
     __syncthreads();
     typedef typename vec2<T>::type V;
-    const i64 base = (i64)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
+    const i64 base = (i64)blockIdx.x * TPB * items;
+    for (int r = 0; r < items; ++r) {
         const i64 i = base + r * TPB + threadIdx.x;
         if (i < n) {
             int key;
@@ -113,7 +113,7 @@ template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
                                                     const int* __restrict__ vals_in, i64 n, int shift,
                                                     const int* __restrict__ hist, const int* __restrict__ tot, int nblocks,
-                                                    int* __restrict__ keys_out, int* __restrict__ vals_out, int kmask) {
+                                                    int* __restrict__ keys_out, int* __restrict__ vals_out, int kmask, int items) {
     constexpr int NBK = 1 << DB, PER = NBK / TPB > 0 ? NBK / TPB : 1;
     __shared__ int off[NBK];
     __shared__ int wcnt[TPB / WAVE][NBK];
@@ -130,8 +130,8 @@ __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, Gri
         for (int u = 0; u < PER; ++u) { const int d = t * PER + u; if (d < NBK) off[d] = ex + hist[(size_t)d * nblocks + blockIdx.x]; ex += loc[u]; }
     }
     typedef typename vec2<T>::type V;
-    const i64 base = (i64)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
+    const i64 base = (i64)blockIdx.x * TPB * items;
+    for (int r = 0; r < items; ++r) {
         const i64 i = base + r * TPB + t;
         const bool valid = i < n;
         int key = 0, val = 0;
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(TPB) void kb_zfinal(const BInst<T>* __restrict__ I,
 
 }  // namespace
 
-int sort_hist_blocks(i64 n) { return (int)((n + RS_TILE - 1) / RS_TILE); }
+int sort_hist_blocks(i64 n) { const i64 tile = (i64)TPB * rs_items(n); return (int)((n + tile - 1) / tile); }
 int spread_chunks(i64 n) { return (int)((n + SP_CHUNK - 1) / SP_CHUNK); }
 int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024); }
 
@@ -558,7 +558,7 @@ int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? 
 template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int* keysA,
                                               int* valsA, int* keysB, int* valsB, int* hist, int passes, hipStream_t st,
                                               int kmask = -1) {
-    const int nblocks = sort_hist_blocks(n);
+    const int nblocks = sort_hist_blocks(n), items = rs_items(n);
     int* tot = hist + (size_t)(1 << DB) * nblocks;
     const int first = keys0 ? 1 : 0;
     const int* kin = keys0; const int* vin = vals0;
@@ -570,14 +570,14 @@ template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, con
         int* kout = cur == 0 ? keysA : keysB;
         int* vout = cur == 0 ? valsA : valsB;
         const bool fromY = p == 0 && !keys0;
-        if (fromY) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks);
-        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks);
+        if (fromY) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items);
+        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items);
         MG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, hist, nblocks, tot);
         MG_HIP(hipGetLastError());
         const int mk = p == passes - 1 ? kmask : -1;
-        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk);
-        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk);
+        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items);
         MG_HIP(hipGetLastError());
         kin = kout; vin = vout;
     }
