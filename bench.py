@@ -211,6 +211,9 @@ def gradient_error(P, l32, l64, st, it, early):
     return float(torch.linalg.norm(grads[0] - grads[1]) / torch.linalg.norm(grads[1]))
 
 
+from magnet.tune import SPACE as TUNE_SPACE  # noqa: E402
+
+
 def precision_run(P, n_iter, early, steps, warmup, profile=True, cpu=False, threads=1):
     """The headline measurement of one workload: FP32 loop timed over the segments of the schedule, its profiled
     pass, the whole FP32 and FP64 schedules, FP32 against FP64 from the same states and the FP32 gradient error."""
@@ -223,8 +226,13 @@ def precision_run(P, n_iter, early, steps, warmup, profile=True, cpu=False, thre
     if warmup > 0:
         l32.run(l32.state(Y0), 0, min(warmup, n_iter))
     snaps, full32 = checkpoint_run(l32, Y0, starts + match_starts)
+    # launch shapes that change no result, picked by timing 10 iterations of the late phase (magnet.tune)
+    from magnet.tune import autotune_loop
+    t_tune = match_starts[-1]
+    chosen, tune_log = autotune_loop(l32.sess, snaps[t_tune], t_tune, iters=10)
     sec32, _ = timed_segments(l32, snaps, segs)
-    out = {"segments": segs, "seconds": sec32, "full32_s": full32}
+    out = {"segments": segs, "seconds": sec32, "full32_s": full32,
+           "autotune": {"at_iteration": t_tune, "chosen": chosen, "sweep_s_per_10_iters": tune_log}}
     if profile:
         _, out["stages"] = timed_segments(l32, snaps, segs, profile=True)
     l64 = Loop(P, "fp64", n_iter, early)
@@ -333,7 +341,8 @@ def batched_bench():
     return {"workload": B["desc"], "instances": B["count"], "points": int(npts), "n_streams": k, "host_threads": nt,
             "instance_iters_per_s": round(inst_it / t_bat, 1), "sequential_instance_iters_per_s": round(inst_it / t_seq, 1),
             "speedup_vs_sequential": round(t_seq / t_bat, 3), "batch_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
-            "autotune_s": round(t_tune, 2), "autotune_sweep_12_iters": [list(x) for x in LAST_SWEEP], "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
+            "autotune_s": round(t_tune, 2), "autotune_sweep_12_iters": [list(x) for x in LAST_SWEEP],
+            "sort_items": capi.tune("sort_items"), "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
 
 
 def cpu_baseline(P, n_iter, early, states, threads, points):
@@ -581,6 +590,9 @@ def main():
         f"whole schedule {W['n_iter'] / R['full32_s']:.1f} it/s")
     roofline, extra = roofline_and_stages(R["stages"], args.workload, steps)
     extra.update(schedule_extra(R, W["n_iter"]))
+    extra["autotune"] = R["autotune"]
+    for name, _ in TUNE_SPACE:   # later legs start from the defaults (the batched leg tunes its own)
+        capi.tune(name, dict(TUNE_SPACE)[name][0])
     log(f"bench: stages {json.dumps(extra['stages'])}")
     log(f"bench: same states FP32 vs FP64 (t, ms32, ms64): {extra['matched_ms_per_iter']} -> "
         f"{extra['fp32_over_fp64_same_states']}x; FP32 gradient error {extra['fp32_gradient_rel_l2']}")

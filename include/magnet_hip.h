@@ -54,8 +54,10 @@ int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
  *   "attr_window"  the 2-D attraction's LDS-window sweep: 1 on, 0 off, -1 (default) from 262,144 points on;
  *   "attr_rows"    rows per block of that sweep (default 256);
  *   "attr_win_kb"  its window of positions in LDS: 64 (default) or 128 KiB, or 0 for none (rows only);
- *   "attr_tpb"     its threads per block: 1024 (default), 512 or 256.
- * MG_ERR_ARG for an unknown name or a value out of range. */
+ *   "attr_tpb"     its threads per block: 1024 (default), 512 or 256;
+ *   "sort_items"   keys per thread of the radix sorts of up to 2^20 keys: 1, 2, 4 or 8, or 0 (default: 2 up to
+ *                  262,144 keys, 8 above).
+ * MG_ERR_ARG for an unknown name or a value out of range. magnet.tune.autotune picks them by timing. */
 int mg_tune_set(const char* name, int value);
 int mg_tune_get(const char* name, int* value);
 

@@ -101,6 +101,7 @@ struct Tunables {
     int attr_rows = 256;    // rows per block of the window sweep
     int attr_win_kb = 64;   // LDS window of the sweep: 128 or 64 KiB of positions (0: no window, rows only)
     int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
+    int sort_items = 0;     // keys per thread of a radix-sort pass up to 2^20 keys: 1, 2, 4 or 8 (0: 2 up to 262,144 keys, else 8)
 };
 Tunables& tunables();
 
@@ -142,6 +143,8 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st);
 // box sort: returns which buffer holds the result (0 = A, 1 = B)
 template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st);
 int sort_hist_blocks(i64 n);
+// ints of a radix sort's digit-row scratch for n keys, enough for any sort_items setting
+i64 sort_hist_ints(i64 n);
 int spread_chunks(i64 n);
 // centred: charges (1, x - x_node, y - y_node) and forces from the centred potentials (FP32 path)
 template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,

@@ -318,7 +318,7 @@ template <class T> struct Repulsion {
         const size_t nk = (size_t)std::max<i64>(n, 1);
         keysA.ensure(nk * sizeof(int)); valsA.ensure(nk * sizeof(int));
         keysB.ensure(nk * sizeof(int)); valsB.ensure(nk * sizeof(int));
-        hist.ensure(((size_t)1024 * std::max(1, sort_hist_blocks(n)) + 1024) * sizeof(int));   // digit rows + totals
+        hist.ensure(((size_t)sort_hist_ints(n)) * sizeof(int));   // digit rows + totals
         const int nch = std::max(1, spread_chunks(n));
         item_key.ensure((size_t)2 * nch * sizeof(int));
         item_valid.ensure((size_t)2 * nch * sizeof(int));
@@ -784,7 +784,7 @@ template <class T> struct BatchEngine {
         }
         const size_t nk = (size_t)ntot * sizeof(int);
         keysA.ensure(nk); valsA.ensure(nk); keysB.ensure(nk); valsB.ensure(nk);
-        hist.ensure(((size_t)1024 * std::max(1, sort_hist_blocks(ntot)) + 1024) * sizeof(int));
+        hist.ensure(((size_t)sort_hist_ints(ntot)) * sizeof(int));
         unsigned* cnt = reinterpret_cast<unsigned*>(counters.ensure(sizeof(unsigned)));   // the bbox's last-block counter
         MG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), st));
         zout.ensure((size_t)2 * count * sizeof(double));
@@ -1130,6 +1130,7 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "attr_rows") { require(value >= 64 && value <= 65536, "attr_rows must be in [64, 65536]"); g_tunables.attr_rows = value; }
     else if (k == "attr_win_kb") { require(value == 0 || value == 64 || value == 128, "attr_win_kb must be 0, 64 or 128"); g_tunables.attr_win_kb = value; }
     else if (k == "attr_tpb") { require(value == 256 || value == 512 || value == 1024, "attr_tpb must be 256, 512 or 1024"); g_tunables.attr_tpb = value; }
+    else if (k == "sort_items") { require(value == 0 || value == 1 || value == 2 || value == 4 || value == 8, "sort_items must be 0, 1, 2, 4 or 8"); g_tunables.sort_items = value; }
     else require(false, "unknown tunable: " + k);
     MG_API_END
 }
@@ -1142,6 +1143,7 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "attr_rows") *value = g_tunables.attr_rows;
     else if (k == "attr_win_kb") *value = g_tunables.attr_win_kb;
     else if (k == "attr_tpb") *value = g_tunables.attr_tpb;
+    else if (k == "sort_items") *value = g_tunables.sort_items;
     else require(false, "unknown tunable: " + k);
     MG_API_END
 }

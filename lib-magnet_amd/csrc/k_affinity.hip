@@ -336,7 +336,7 @@ void launch_perplexity(const float* d2, i64 n, int k, double perplexity, double 
 
 i64 symmetrize_scratch_ints(i64 n, int k) {
     const i64 E2 = 2 * n * (i64)k;
-    return 5 * E2 + ((i64)1024 * std::max(1, sort_hist_blocks(E2)) + 1024) + scan_tmp_ints(E2) + scan_tmp_ints(n) + n + 8;
+    return 5 * E2 + (sort_hist_ints(E2)) + scan_tmp_ints(E2) + scan_tmp_ints(n) + n + 8;
 }
 
 i64 symmetrize_knn(int mode, const int32_t* idx, const double* val, i64 n, int k, double param, int32_t* indptr,
@@ -353,7 +353,7 @@ i64 symmetrize_knn(int mode, const int32_t* idx, const double* val, i64 n, int k
     int* valsB = keysB + E2;
     int* keep = valsB + E2;
     int* hist = keep + E2;
-    int* offs = hist + ((i64)1024 * std::max(1, sort_hist_blocks(E2)) + 1024);
+    int* offs = hist + (sort_hist_ints(E2));
     int* rtmp = offs + scan_tmp_ints(E2);
     int* rcount = rtmp + scan_tmp_ints(n);
     int* bad = rcount + n;

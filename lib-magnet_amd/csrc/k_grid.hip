@@ -13,8 +13,13 @@ namespace mg {
 namespace {
 
 constexpr int TPB = 256;
-// items per thread of a sort pass: 8, or 2 below 262,144 keys, where the blocks of a pass, not its bytes, bound it
-inline int rs_items(i64 n) { return n <= 262144 ? 2 : 8; }
+// items per thread of a sort pass: 8, or 2 up to 262,144 keys, where the blocks of a pass, not its bytes, bound it; up to
+// 2^20 keys the sort_items tunable overrides (the digit rows are sized for any setting: sort_hist_ints)
+inline int rs_items(i64 n) {
+    if (n > (1 << 20)) return 8;
+    const int t = tunables().sort_items;
+    return t > 0 ? t : (n <= 262144 ? 2 : 8);
+}
 constexpr int SP_PT = 4;                 // points per thread in the spread
 constexpr int SP_CHUNK = TPB * SP_PT;    // points per spread chunk (one workgroup)
 
@@ -549,6 +554,10 @@ __global__ __launch_bounds__(TPB) void kb_zfinal(const BInst<T>* __restrict__ I,
 }  // namespace
 
 int sort_hist_blocks(i64 n) { const i64 tile = (i64)TPB * rs_items(n); return (int)((n + tile - 1) / tile); }
+i64 sort_hist_ints(i64 n) {
+    const i64 tile = (i64)TPB * (n > (1 << 20) ? 8 : 1);
+    return (i64)1024 * std::max<i64>(1, (n + tile - 1) / tile) + 1024;
+}
 int spread_chunks(i64 n) { return (int)((n + SP_CHUNK - 1) / SP_CHUNK); }
 int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024); }
 

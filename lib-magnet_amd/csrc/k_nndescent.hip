@@ -651,7 +651,7 @@ struct NND {
         int* va = bVa.get<int>(P);
         int* kb = bKb.get<int>(P);
         int* vb = bVb.get<int>(P);
-        int* hist = bHist.get<int>((size_t)1024 * std::max(1, sort_hist_blocks(P)) + 1024);
+        int* hist = bHist.get<int>((size_t)sort_hist_ints(P));
         MG_HIP(hipMemcpyAsync(ka, Ptgt, (size_t)P * sizeof(int), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_iota, dim3(grid_for(P)), dim3(TPB), 0, st, va, P);
         const int cur = launch_key_sort<float>(ka, va, kb, vb, hist, P, bits_for(n), st);
@@ -802,7 +802,7 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
         int* ex = bex.get<int>((size_t)NT + 1);
         int* keys = bk2.get<int>(2 * (size_t)NT);
         int* vals2 = bv2.get<int>(NT);
-        int* hist = bhist.get<int>((size_t)1024 * std::max(1, sort_hist_blocks(NT)) + 1024);
+        int* hist = bhist.get<int>((size_t)sort_hist_ints(NT));
         int* tmp = btmp.get<int>(scan_tmp_ints(NT));
         int* mx = s.misc + 2;
         for (int t0 = 0; t0 < n_trees; t0 += TB) {
@@ -877,7 +877,7 @@ void nn_descent(const float* X, i64 n_, int d, int K, int max_candidates, int n_
         int* rv = bv.get<int>(nK);
         int* rk2 = bk2.get<int>(nK);
         int* rv2 = bv2.get<int>(nK);
-        int* hist = bhist.get<int>((size_t)1024 * std::max(1, sort_hist_blocks(nK)) + 1024);
+        int* hist = bhist.get<int>((size_t)sort_hist_ints(nK));
         int* rcount = brc.get<int>((size_t)n + 1);
         int* rptr = brp.get<int>((size_t)n + 1);
         int* tmp = btmp.get<int>(scan_tmp_ints(n));

@@ -202,7 +202,7 @@ template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* 
 int sort_hist_blocks(i64 n);
 
 size_t shim_scratch_ints(i64 n, int nb) {
-    return (size_t)4 * std::max<i64>(n, 1) + (size_t)1024 * std::max(1, sort_hist_blocks(n)) + 1024 + (size_t)2 * nb * nb;
+    return (size_t)4 * std::max<i64>(n, 1) + (size_t)sort_hist_ints(n) + (size_t)2 * nb * nb;
 }
 
 // the sorted point order and per-box ranges of the scatter, in `ws` (shim_scratch_ints ints)
@@ -210,7 +210,7 @@ template <class T> static const int* shim_sort(const T* Y, i64 n, const ShimBox<
     const size_t nn = (size_t)std::max<i64>(n, 1);
     int* keysA = ws; int* valsA = ws + nn; int* keysB = ws + 2 * nn; int* valsB = ws + 3 * nn;
     int* hist = ws + 4 * nn;
-    start = hist + (size_t)1024 * std::max(1, sort_hist_blocks(n)) + 1024;
+    start = hist + (size_t)sort_hist_ints(n);
     end = start + (size_t)B.nb * B.nb;
     hipLaunchKernelGGL(k_shim_zero_int, dim3(grid1(2 * (i64)B.nb * B.nb)), dim3(SH_TPB), 0, st, start, 2 * (i64)B.nb * B.nb);
     if (n == 0) return valsA;

@@ -5,7 +5,8 @@ through one C-ABI call, mg_tsne_optimize_batch. Instances are spread over a few 
 issued by a few host threads, so their kernels overlap on the device and the launches of different instances are
 issued in parallel; each instance's result is the one optimize_tsne gives it alone (same kernels, same grids, same
 order of operations). The stream and thread counts are picked by autotune_streams: short timed runs of each
-candidate on copies of the states, the fastest kept.
+candidate on copies of the states, the fastest kept; then the launch shapes that change no result (magnet.tune: the
+radix sort's keys per thread) are swept the same way.
 
 The reference fits one embedding per call; this form has no counterpart there.
 """
@@ -64,6 +65,15 @@ def autotune_streams(instances, candidates=CANDIDATES, iters=12, early_exaggerat
         LAST_SWEEP.append((k, nt, round(t, 4)))
         if t < best_t:
             best, best_t = (k, nt), t
+    # then the sort's keys per thread at that choice (the batch sorts all its points at once)
+    from .tune import autotune
+
+    def run():
+        trial = [(ip, ix, dv, Y.clone(), U.clone(), G.clone()) for ip, ix, dv, Y, U, G in instances]
+        capi.optimize_batch(trial, n_iter=iters, early_iter=0, early_exaggeration=early_exaggeration, n_streams=best[0],
+                            n_threads=best[1], **kw)
+    chosen, log = autotune(run, (("sort_items", (0, 2, 4, 8)),), repeats=2)
+    LAST_SWEEP.extend(log)
     _TUNED[key] = best
     return best
 

@@ -359,3 +359,36 @@ def test_trace_kl_and_gradient_norm_match_oracle():
     # the first run's iterations, before the two precisions' trajectories part
     print("FP32 - FP64 KL", np.abs(a[0][:3] - k64[:3]).max(), "norm rel", np.abs(a[1][:3] / g64[:3] - 1).max())
     assert np.abs(a[0][:3] - k64[:3]).max() < 1e-5 and np.abs(a[1][:3] / g64[:3] - 1).max() < 1e-4
+
+
+def test_tunables_change_no_result():
+    """Every launch shape the autotuner may pick (the sorts' keys per thread, the attraction's LDS window on or off,
+    its rows, window and threads per block) gives the same state bit for bit, FP32 and FP64; the autotuner leaves
+    one of them set."""
+    import torch
+    from magnet.tune import autotune_loop, SPACE
+    P, Y0, _ = problem()
+    kw = dict(n_iter=500, early_iter=166)
+    settings = [dict(sort_items=0), dict(sort_items=1), dict(sort_items=8), dict(attr_window=1, attr_rows=64, attr_win_kb=64,
+                attr_tpb=512), dict(attr_window=1, attr_rows=512, attr_win_kb=128, attr_tpb=1024), dict(attr_window=0)]
+    try:
+        for dt in (np.float32, np.float64):
+            outs = []
+            for setting in settings:
+                for name, values in SPACE:
+                    capi.tune(name, values[0])
+                for name, v in setting.items():
+                    capi.tune(name, v)
+                st = State(P, Y0, dt)
+                capi.optimize(st.ip, st.ix, st.data, st.Y, st.U, st.G, 0, 40, **kw)
+                outs.append(host(st.Y).copy())
+            assert all(np.array_equal(outs[0], o) for o in outs[1:]), dt
+        st = State(P, Y0, np.float32)
+        sess = capi.Session(st.ip, st.ix, st.data, **kw)
+        chosen, log = autotune_loop(sess, (st.Y, st.U, st.G), 0, iters=5)
+        sess.close()
+        print("autotuner chose", chosen)
+        assert set(chosen) == {name for name, _ in SPACE} and all(capi.tune(k) == v for k, v in chosen.items())
+    finally:
+        for name, values in SPACE:
+            capi.tune(name, values[0])
