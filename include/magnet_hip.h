@@ -56,7 +56,10 @@ int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
  *   "attr_win_kb"  its window of positions in LDS: 64 (default) or 128 KiB, or 0 for none (rows only);
  *   "attr_tpb"     its threads per block: 1024 (default), 512 or 256;
  *   "sort_items"   keys per thread of the radix sorts of up to 2^20 keys: 1, 2, 4 or 8, or 0 (default: 2 up to
- *                  262,144 keys, 8 above).
+ *                  262,144 keys, 8 above);
+ *   "batch_graph"  1 (default): mg_tsne_optimize_batch replays its captured iteration (a hipGraph); 0: it issues the
+ *                  same launches one by one;
+ *   "batch_chunk"  iterations mg_tsne_optimize_batch queues between two reads of the instances' states (default 8).
  * MG_ERR_ARG for an unknown name or a value out of range. magnet.tune.autotune picks them by timing. */
 int mg_tune_set(const char* name, int value);
 int mg_tune_get(const char* name, int* value);
@@ -110,8 +113,8 @@ int mg_mid_spring(int prec, const void* d_Y, const int32_t* d_pairs, void* d_gra
 typedef struct mg_fftconv mg_fftconv;
 
 /* FFTConv2D(M, n_interp) (src/fftconv.cpp:45-47). M is the padded size the caller's spectra use; the
- * device transforms run on the smallest power of two >= max(M, 2N) (any M >= 2N - 1 gives the same
- * convolution). n_interp in [1, 8]. */
+ * device transforms run on mg_fft_size(N), the smallest of 2^s and 3 * 2^s that is >= 2N, whatever M is
+ * (any padded size >= 2N - 1 gives the same convolution). n_interp in [1, 8]. */
 int mg_fftconv_create(mg_fftconv** out, int prec, int M, int n_interp);
 int mg_fftconv_destroy(mg_fftconv* conv);
 
@@ -318,8 +321,13 @@ int mg_tsne_session_trace(mg_tsne_session* s, double* h_kl, double* h_grad_norm)
 /* Batched form: `count` (<= 1024) independent embeddings (CSR P, state Y / U / G of their own sizes, device
  * arrays) on one schedule p (learning_rate 0: per instance max(200, n / early_exaggeration); no mid-range pairs;
  * max_boxes <= 1024). Every stage of an iteration is one launch for all instances of a group (one per FFT size for
- * the FFT stages). The instances are split into n_streams groups of about equal point counts, each on its own HIP
- * stream (and a second one for its attraction), driven by n_threads host threads (1 <= n_threads <= n_streams).
+ * the FFT stages). The iteration is device-driven: a kernel decides every instance's grid from the bbox of its last
+ * update, and the launches depend only on a plan of each instance's window of FFT sizes (the one its grid needs and
+ * the sizes either side), so the iteration is captured once in a hipGraph and replayed; the host reads the
+ * instances' states every few iterations and plans again when one nears the edge of its window (an instance whose
+ * grid leaves its window waits, its state unchanged, for the next plan). The instances are split into n_streams
+ * groups of about equal point counts, each on its own HIP stream (and a second one for its attraction), driven by
+ * n_threads host threads (1 <= n_threads <= n_streams).
  * Each instance's result is bit for bit the one mg_tsne_optimize gives it alone. h_sumQ (host, n_iter doubles)
  * may be null. The call returns when every instance is done. No counterpart in the reference (it fits one
  * embedding per call). */
@@ -333,6 +341,13 @@ typedef struct {
 } mg_tsne_instance;
 int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_streams,
                            int n_threads);
+/* Counters of the last mg_tsne_optimize_batch call, summed over its groups (no counterpart in the reference): plans
+ * made (a plan fixes each instance's window of FFT sizes, its buffers, the launches of an iteration and their
+ * hipGraph), iterations issued (the schedule's length, plus those some instance spent waiting for a plan), graph
+ * replays among them, the kernel launches they hold, reads of the instances' states, instance stalls seen at those
+ * reads, and the host seconds spent planning. Any pointer may be NULL. */
+int mg_tsne_batch_stats(int64_t* plans, int64_t* iterations, int64_t* replays, int64_t* launches, int64_t* checks,
+                        int64_t* stalls, double* plan_seconds);
 
 #ifdef __cplusplus
 }

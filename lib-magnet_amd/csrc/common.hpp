@@ -69,6 +69,54 @@ template <class T> struct GridT {
     int nb, N;
 };
 
+// Grid decision of FFTRepulsion2D._grid (magnet/tsne.py:30-46) on a bounding box, in double. The same function
+// runs on the host (the single loop) and on the device (the device-driven batch), so both decide the same grid.
+struct GridDecision { double xmin, ymin, width; int nb; };
+__host__ __device__ inline GridDecision decide_grid(const double* bb, double bw, int minb, int maxb) {
+    const double lox = bb[0] - 1e-3, loy = bb[1] - 1e-3, hix = bb[2] + 1e-3, hiy = bb[3] + 1e-3;
+    const double sx = hix - lox, sy = hiy - loy;
+    const double span = sx > sy ? sx : sy;
+    double nbf = ceil(span / bw);
+    nbf = nbf < (double)minb ? (double)minb : nbf;
+    nbf = nbf > (double)maxb ? (double)maxb : nbf;
+    int nb = (int)nbf;
+    nb = ((nb + 3) / 4) * 4;
+    nb = nb < maxb ? nb : maxb;
+    const double q = span / nb;
+    const double width = (nb * bw >= span) ? bw : (q > 1e-12 ? q : 1e-12);
+    return {lox, loy, width, nb};
+}
+
+template <class T> __host__ __device__ inline GridT<T> make_grid(double xmin, double ymin, double width, int nb, int ni) {
+    GridT<T> g;
+    g.xmin = (T)xmin; g.ymin = (T)ymin; g.width = (T)width;
+    g.inv_w = T(1) / g.width;
+    g.xmin_d = xmin; g.ymin_d = ymin; g.width_d = width;   // unrounded: boxes and local coordinates of both paths agree
+    g.inv_w_d = 1.0 / g.width_d;
+    g.nb = nb; g.N = nb * ni;
+    return g;
+}
+
+// The padded FFT sizes with compiled plans: 2^s and 3 * 2^s from 128 to MAX_M; the smallest M >= 2N (0 when none).
+__host__ __device__ inline int fft_size(int N) {
+    for (int M = 128; M <= MAX_M; M *= 2) {
+        if (3 * (M / 4) >= 2 * N && 3 * (M / 4) >= 128) return 3 * (M / 4);
+        if (M >= 2 * N) return M;
+    }
+    return 0;
+}
+// the supported size after / before M (0 when none)
+__host__ __device__ inline int fft_size_next(int M) { return fft_size(M / 2 + 1); }
+__host__ __device__ inline int fft_size_prev(int M) {
+    int best = 0;
+    for (int P = 128; P <= MAX_M; P *= 2) {
+        const int a = 3 * (P / 4);
+        if (a >= 128 && a < M && a > best) best = a;
+        if (P < M && P > best) best = P;
+    }
+    return best;
+}
+
 // Box of a point: x-box by multiplication with 1/width (the counting-sort key of src/fftconv.cpp:88),
 // y-box by division (:106-110). Key = bx * nb + by. Evaluated in double for both paths: the FP32 path
 // then puts every point in the box the FP64 path would on the same grid (float evaluation moves
@@ -102,8 +150,17 @@ struct Tunables {
     int attr_win_kb = 64;   // LDS window of the sweep: 128 or 64 KiB of positions (0: no window, rows only)
     int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
     int sort_items = 0;     // keys per thread of a radix-sort pass up to 2^20 keys: 1, 2, 4 or 8 (0: 2 up to 262,144 keys, else 8)
+    int batch_graph = 1;    // the batched loop replays its iteration as a captured hipGraph (0: issues the launches)
+    int batch_chunk = 16;   // iterations the batched loop queues between two looks at the instances' states
+    int batch_window_down = 1;   // the batched plan's window of FFT sizes holds the size below the current one
+    int batch_wl_grid = 2048;    // most blocks of a batched grid-stage launch over its work list
 };
 Tunables& tunables();
+
+// Raises a kernel's dynamic LDS limit to `bytes` (when above the 64 KiB default), once per kernel and size for the
+// process: the HIP runtime's function attributes are shared by every host thread, so the calls are serialised
+// under one lock and none is made again on later launches (nor while a stream is being captured).
+void kernel_lds_limit(const void* f, int bytes);
 
 // ---- scratch of one repulsion engine (device pointers), sized by the host
 template <class T> struct RepBuffers {
@@ -221,7 +278,6 @@ template <class T> int launch_key_sort(int* keysA, int* valsA, int* keysB, int* 
 
 // FFT stages. M = 2^s or 3 * 2^s, 128 <= M <= MAX_M, N <= M / 2.
 void fft_init_twiddles();   // uploads the twiddle tables once per device
-int fft_size(int N);        // smallest supported M >= 2N (0 when none)
 template <class T> void launch_rows_forward(const T* W, int N, int M, cplx<T>* out, hipStream_t st);   // out (3, N, Mh)
 template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int batch, int R, int C, hipStream_t st);
 // K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]. Wh (3, Mh, N),

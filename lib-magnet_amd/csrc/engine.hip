@@ -11,8 +11,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <chrono>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -31,6 +33,7 @@ template <class T> int launch_gather_forces(const T*, i64, const T*, int, double
 template <class T> void launch_spectral_products(const T*, const T*, const T*, i64, i64, T*, hipStream_t);
 
 static thread_local std::string g_err;
+static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 static Tunables g_tunables;
 Tunables& tunables() { return g_tunables; }
 // The stream the entry points issue on: the calling thread's own (mg_set_thread_stream) if it set one, else
@@ -115,29 +118,6 @@ void validate_csr(const int32_t* ip, const int32_t* ix, i64 n, i64 nnz, hipStrea
     require(!(h & 2), "indptr must be non-decreasing");
     require(!(h & 4), "indices must have indptr[n] entries");
     require(!(h & 8), "column index out of range");
-}
-
-// ---- grid decision of FFTRepulsion2D._grid (magnet/tsne.py:30-46) on the bbox (in double)
-struct GridDecision { double xmin, ymin, width; int nb; };
-GridDecision decide_grid(const double* bb, double bw, int minb, int maxb) {
-    const double lox = bb[0] - 1e-3, loy = bb[1] - 1e-3, hix = bb[2] + 1e-3, hiy = bb[3] + 1e-3;
-    const double span = std::max(hix - lox, hiy - loy);
-    double nbf = std::ceil(span / bw);
-    nbf = std::min((double)maxb, std::max((double)minb, nbf));
-    int nb = (int)nbf;
-    nb = std::min(maxb, ((nb + 3) / 4) * 4);
-    const double width = (nb * bw >= span) ? bw : std::max(span / nb, 1e-12);
-    return {lox, loy, width, nb};
-}
-
-template <class T> GridT<T> make_grid(double xmin, double ymin, double width, int nb, int ni) {
-    GridT<T> g;
-    g.xmin = (T)xmin; g.ymin = (T)ymin; g.width = (T)width;
-    g.inv_w = T(1) / g.width;
-    g.xmin_d = xmin; g.ymin_d = ymin; g.width_d = width;   // unrounded: boxes and local coordinates of both paths agree
-    g.inv_w_d = 1.0 / g.width_d;
-    g.nb = nb; g.N = nb * ni;
-    return g;
 }
 
 // ---- per-stage timing with HIP events on the working stream (the reference's MAGNET_DEBUG laps,
@@ -710,44 +690,92 @@ template <class T> struct BatchEngine {
     static constexpr int ntab = centred ? 4 : 2;
     const mg_tsne_params* p = nullptr;
     std::vector<const mg_tsne_instance*> user;
-    int count = 0, ni = 3;
+    int count = 0, ni = 3, Mmin = 128, Mmax = 0;
     hipStream_t st = nullptr;
     std::vector<i64> n, off;
-    std::vector<int> M, sM;
-    std::vector<double> sdelta;
-    std::vector<GridT<T>> g;
-    std::vector<bool> fresh;
     i64 ntot = 0;
     T inv_dof = T(1);
-    int sorted = 0;
-    DevBuf Yc, Uc, Gc, Ac, Fc, keysA, valsA, keysB, valsB, hist, table, mbd, sumq, counters, zout, inv, upart;
-    DevBuf aW, aWh, aPh, aTr, aphi, aik, aiv, aival, azp, aspec;
-    std::vector<void*> dead;   // replaced allocations, freed by finish() once the stream is idle
-    std::vector<std::unique_ptr<DevBuf[]>> K;   // per instance: ntab kernel spectra
-    PinnedBuf htab, hmb;
+    BSched sched{};
+    DevBuf Yc, Uc, Gc, Ac, Fc, keysA, valsA, keysB, valsB, hist, atable, mbd, sumq, counters, zout, inv, upart, state;
+    DevBuf aW, aWh, aPh, aTr, aphi, aik, aiv, aival, azp, spec_stage;
+    DevBuf table[2];          // the plans' tables, in turn (a new plan's table is written while the last one's
+    PinnedBuf htab[2], hrep[2];   // iterations still run); the states read back after each stretch, in turn
+    std::vector<void*> dead;  // replaced allocations, freed once the stream is idle (at the end of the run)
+    // kernel spectra of unclipped grids (cell width box_width / ni), per FFT size, shared by all instances
+    struct Spec { int M; DevBuf K[4]; };
+    std::list<Spec> shared;
+    // per instance whose window holds the largest size: its own spectra of a clipped grid, and their staging
+    std::vector<std::unique_ptr<DevBuf[]>> Kc;
     std::vector<BMap> attr_maps;
     std::vector<int> attr_G;
-    hipEvent_t ready = nullptr;
-    // the next iteration's attraction runs on its own stream beside the repulsion (ev_a0: after the update; the
-    // update waits for ev_a1), reading a copy of the table's fixed part (CSR, sizes, offsets) made once, since
-    // the table itself is rewritten every iteration
-    DevBuf atable;
-    hipStream_t aside = nullptr;
-    hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr;
-    bool a_pending = false;
-    ~BatchEngine() {
-        if (aside) (void)hipStreamSynchronize(aside);
-        release_dead();
-        if (ready) (void)hipEventDestroy(ready);
-        if (ev_a0) (void)hipEventDestroy(ev_a0);
-        if (ev_a1) (void)hipEventDestroy(ev_a1);
-        if (aside) (void)hipStreamDestroy(aside);
-    }
+    hipStream_t aside = nullptr, cst = nullptr;   // the attraction's stream, the capture stream
+    hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr, ev_chunk[2] = {nullptr, nullptr};
+    // ---- the plan: each instance's window of FFT sizes, the block ranges of every launch, the captured iteration
+    std::vector<int> Mcur, Mlo, Mhi;
+    std::vector<int> sizes;   // the FFT sizes of the windows, ascending
+    int box_bits = 1, sorted = 0, tcur = 0;
+    struct Launches {
+        std::vector<BMap> spec1, spect;                 // clipped grids at the largest size
+        std::vector<BMap> keys, spread, gather, update;
+        // work lists: offsets of rows / columns / inverse rows per size, and of the transposes and the fill, in the
+        // list buffer, with the fixed grid of each launch
+        std::vector<int> rows_f, cols, rows_i, g_rows_f, g_cols, g_rows_i;
+        int tr0 = 0, tr1 = 0, fill = 0, g_tr0 = 0, g_tr1 = 0, g_fill = 0;
+        int ndesc = 0;
+    } L;
+    DevBuf wdesc[2], wcand[2], wlist[2];   // per plan, in turn with the tables
+    PinnedBuf hdesc[2], hcand[2];          // their host staging (the copies run after the iterations queued before)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<hipGraph_t, hipGraphExec_t>> old_graphs;   // destroyed at the end of the run
+    bool use_graph = true;
+    int chunk = 8;
+    bool done = false;
+    // stretches queued and not yet read back (oldest first): their report slot and iteration count
+    std::vector<std::pair<int, int>> pending;
+    int slot = 0;
+    std::vector<int> last_next;   // next_it of every instance at the last read-back
+    // counters of the run (mg_tsne_batch_stats)
+    long long n_plans = 0, n_replays = 0, n_launches = 0, n_checks = 0, n_launches_per_iter = 0, n_stalls = 0;
+    long long n_iters_issued = 0;
+    double plan_s = 0;
 
-    BInst<T>* host_table() { return reinterpret_cast<BInst<T>*>(htab.p); }
-    const BInst<T>* dev_table() const { return reinterpret_cast<const BInst<T>*>(table.p); }
+    ~BatchEngine() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (aside) (void)hipStreamSynchronize(aside);
+        drop_graphs();
+        release_dead();
+        for (hipEvent_t e : {ev_a0, ev_a1, ev_chunk[0], ev_chunk[1]}) if (e) (void)hipEventDestroy(e);
+        if (aside) (void)hipStreamDestroy(aside);
+        if (cst) (void)hipStreamDestroy(cst);
+    }
+    void drop_graphs() {
+        if (exec || graph) old_graphs.push_back({graph, exec});
+        for (auto& g : old_graphs) {
+            if (g.second) (void)hipGraphExecDestroy(g.second);
+            if (g.first) (void)hipGraphDestroy(g.first);
+        }
+        old_graphs.clear();
+        graph = nullptr; exec = nullptr;
+    }
+    BInst<T>* host_table() { return reinterpret_cast<BInst<T>*>(htab[tcur].p); }
+    BInst<T>* dev_table() const { return reinterpret_cast<BInst<T>*>(table[tcur].p); }
     const int* sorted_keys() const { return dptr<int>(sorted == 0 ? keysA : keysB); }
     const int* sorted_vals() const { return dptr<int>(sorted == 0 ? valsA : valsB); }
+
+    // the window of sizes around M that the schedule can reach: the size before it and the two after it (layouts
+    // mostly grow; a grid that leaves the window stalls its instance until the host's next plan, which comes a
+    // stretch or two of iterations later)
+    int window_lo(int M) const {
+        if (!tunables().batch_window_down) return M;
+        const int lo = fft_size_prev(M);
+        return lo >= Mmin ? lo : M;
+    }
+    int window_hi(int M) const {
+        int hi = M;
+        for (int k = 0; k < 2; ++k) { const int nx = fft_size_next(hi); if (nx > 0 && nx <= Mmax) hi = nx; }
+        return hi;
+    }
 
     void begin(const mg_tsne_params* p_, const std::vector<const mg_tsne_instance*>& inst, hipStream_t st_) {
         p = p_; user = inst; st = st_;
@@ -760,8 +788,12 @@ template <class T> struct BatchEngine {
         require(p->box_width > 0 && p->min_boxes >= 1 && p->max_boxes >= p->min_boxes, "bad grid parameters");
         require(p->max_boxes <= 1024, "max_boxes above 1024 is not supported in a batch", MG_ERR_UNSUP);
         fft_init_twiddles();
-        n.resize(count); off.resize(count); M.assign(count, 0); sM.assign(count, 0); sdelta.assign(count, -1.0);
-        g.resize(count); fresh.assign(count, false);
+        const Tunables& tu = tunables();
+        use_graph = tu.batch_graph != 0;
+        chunk = tu.batch_chunk;
+        Mmin = fft_size(ni * std::min(p->min_boxes, p->max_boxes));
+        Mmax = fft_size(ni * p->max_boxes);
+        n.resize(count); off.resize(count);
         ntot = 0;
         for (int i = 0; i < count; ++i) {
             const mg_tsne_instance& I = *inst[i];
@@ -792,18 +824,30 @@ template <class T> struct BatchEngine {
         upart.ensure((size_t)count * 8 * 1024 * sizeof(double));
         T* mb = reinterpret_cast<T*>(mbd.ensure((size_t)8 * count * sizeof(T)));
         sumq.ensure((size_t)count * std::max(1, p->n_iter) * sizeof(double));
-        table.ensure((size_t)count * sizeof(BInst<T>));
-        htab.ensure((size_t)count * sizeof(BInst<T>));
+        for (int k = 0; k < 2; ++k) {
+            table[k].ensure((size_t)count * sizeof(BInst<T>));
+            htab[k].ensure((size_t)count * sizeof(BInst<T>));
+            hrep[k].ensure((size_t)count * sizeof(BState));
+            // work-list staging for any plan: 3 lists per FFT size (13 sizes) and 3 more, each instance a candidate
+            // of at most 4 sizes' lists and of the 3 others
+            hdesc[k].ensure((size_t)(3 * 13 + 3) * sizeof(WorkDesc));
+            hcand[k].ensure((size_t)count * (3 * 4 + 3) * sizeof(int));
+        }
+        BState* S = reinterpret_cast<BState*>(state.ensure((size_t)count * sizeof(BState)));
+        BState* hs = reinterpret_cast<BState*>(hrep[0].p);
+        PinnedBuf hmb;
         T* hm = reinterpret_cast<T*>(hmb.ensure((size_t)8 * count * sizeof(T)));
-        K.clear();
-        for (int i = 0; i < count; ++i) K.emplace_back(new DevBuf[4]);
+        Kc.clear();
+        for (int i = 0; i < count; ++i) Kc.emplace_back(new DevBuf[5]);
         // starting layouts' bboxes (the update of each iteration writes the next ones)
         for (int i = 0; i < count; ++i)
             launch_bbox<T>(Y + 2 * off[i], n[i], reinterpret_cast<T*>(upart.p), cnt, mb + 8 * i + 2, st);
         MG_HIP(hipMemcpyAsync(hm, mb, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
         inv_dof = T(1) / (T)p->dof;
+        sched = {p->n_iter, p->early_iter, ni, p->min_boxes, p->max_boxes, count, p->box_width,
+                 p->early_exaggeration, p->exaggeration, p->momentum0, p->momentum1, p->box_width / ni};
         // the fixed part of the table, and the attraction launches (rows grouped by lanes per row)
-        BInst<T>* h = host_table();
+        BInst<T>* h = reinterpret_cast<BInst<T>*>(htab[0].p);
         std::memset((void*)h, 0, (size_t)count * sizeof(BInst<T>));
         std::vector<std::vector<int>> byG(7);
         for (int i = 0; i < count; ++i) {
@@ -811,9 +855,15 @@ template <class T> struct BatchEngine {
             h[i].ip = I.indptr; h[i].ix = I.indices; h[i].P = reinterpret_cast<const T*>(I.data);
             h[i].n = n[i]; h[i].off = off[i];
             h[i].lr = (T)(p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n[i] / p->early_exaggeration));
+            h[i].zout = dptr<double>(zout) + 2 * i;
+            h[i].inv_out = dptr<T>(inv) + 4 * i;
+            h[i].upart = dptr<double>(upart) + (size_t)i * 8 * 1024;
+            h[i].mb = dptr<T>(mbd) + 8 * i;
+            h[i].sumq_base = dptr<double>(sumq) + (size_t)i * std::max(1, p->n_iter);
             const int Gl = attraction_lanes(n[i], I.nnz);
             byG[bits_for(Gl)].push_back(i);
         }
+        std::memcpy(htab[1].p, htab[0].p, (size_t)count * sizeof(BInst<T>));
         attr_maps.clear(); attr_G.clear();
         for (int b = 0; b < 7; ++b) {
             const int Gl = 1 << b;
@@ -821,140 +871,272 @@ template <class T> struct BatchEngine {
                 attr_maps.push_back(m); attr_G.push_back(Gl);
             }
         }
-        MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
         MG_HIP(hipMemcpyAsync(atable.ensure((size_t)count * sizeof(BInst<T>)), h, (size_t)count * sizeof(BInst<T>),
                               hipMemcpyHostToDevice, st));
+        for (int i = 0; i < count; ++i) hs[i] = {0, 0, 0, 0, -1.0};
+        MG_HIP(hipMemcpyAsync(S, hs, (size_t)count * sizeof(BState), hipMemcpyHostToDevice, st));
         MG_HIP(hipStreamSynchronize(st));
-        if (!ready) MG_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-        if (!aside) {
-            MG_HIP(hipStreamCreateWithFlags(&aside, hipStreamNonBlocking));
-            MG_HIP(hipEventCreateWithFlags(&ev_a0, hipEventDisableTiming));
-            MG_HIP(hipEventCreateWithFlags(&ev_a1, hipEventDisableTiming));
-        }
-        a_pending = false;
-        if (p->n_iter > 0) attraction();
-    }
-
-    void attraction() {
-        MG_HIP(hipEventRecord(ev_a0, st));
-        MG_HIP(hipStreamWaitEvent(aside, ev_a0, 0));
-        const BInst<T>* I = reinterpret_cast<const BInst<T>*>(atable.p);
-        for (size_t k = 0; k < attr_maps.size(); ++k)
-            launch_attraction_batch<T>(I, attr_maps[k], attr_G[k], dptr<T>(Yc), inv_dof, dptr<T>(Ac), aside);
-        MG_HIP(hipEventRecord(ev_a1, aside));
-        a_pending = true;
-    }
-
-    // iteration it for every instance: grids from the last bboxes, spectra where the grid changed, the
-    // repulsion stages, the update, the bbox read-back; then the next iteration's attraction
-    void issue(int it) {
-        const T* hm = reinterpret_cast<const T*>(hmb.p);
-        int box_bits = 1;
-        size_t oW = 0, oWh = 0, oPh = 0, oTr = 0, ophi = 0, oit = 0, oiv = 0, ozp = 0, osp = 0;
-        auto al = [](size_t x) { return (x + 63) / 64 * 64; };
-        std::vector<size_t> pW(count), pWh(count), pPh(count), pTr(count), pphi(count), pit(count), piv(count), pzp(count), psp(count);
+        last_next.assign(count, 0);
+        // the first plan from the starting grids
+        Mcur.assign(count, 0);
         for (int i = 0; i < count; ++i) {
             const double bb[4] = {(double)hm[8 * i + 2], (double)hm[8 * i + 3], (double)hm[8 * i + 4], (double)hm[8 * i + 5]};
             require(std::isfinite(bb[0]) && std::isfinite(bb[1]) && std::isfinite(bb[2]) && std::isfinite(bb[3]),
                     "layout is not finite");
             const GridDecision gd = decide_grid(bb, p->box_width, p->min_boxes, p->max_boxes);
-            g[i] = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, ni);
-            M[i] = fft_size(g[i].N);
-            require(M[i] > 0, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
-            box_bits = std::max(box_bits, bits_for((long long)gd.nb * gd.nb));
-            const double delta = gd.width / ni;
-            fresh[i] = !(sM[i] == M[i] && sdelta[i] == delta);
-            const size_t Mh = M[i] / 2 + 1, N = g[i].N, nch = spread_chunks(n[i]);
-            if (fresh[i]) {
-                sM[i] = M[i]; sdelta[i] = delta;
-                for (int t = 0; t < ntab; ++t) K[i][t].ensure_keep(Mh * Mh * sizeof(T), dead);
-                psp[i] = osp; osp += al(4 * Mh * Mh);
-            }
-            pW[i] = oW; oW += al(3 * N * N);
-            pWh[i] = oWh; oWh += al(3 * Mh * N);
-            pPh[i] = oPh; oPh += al(4 * Mh * N);
-            pTr[i] = oTr; oTr += al(4 * Mh * N);
-            pphi[i] = ophi; ophi += al(4 * N * N);
+            Mcur[i] = fft_size(gd.nb * ni);
+            require(Mcur[i] > 0, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+        }
+        if (!aside) {
+            MG_HIP(hipStreamCreateWithFlags(&aside, hipStreamNonBlocking));
+            MG_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
+            MG_HIP(hipEventCreateWithFlags(&ev_a0, hipEventDisableTiming));
+            MG_HIP(hipEventCreateWithFlags(&ev_a1, hipEventDisableTiming));
+            MG_HIP(hipEventCreateWithFlags(&ev_chunk[0], hipEventDisableTiming));
+            MG_HIP(hipEventCreateWithFlags(&ev_chunk[1], hipEventDisableTiming));
+        }
+        done = p->n_iter == 0;
+        tcur = 1;   // plan() takes the other table
+        if (!done) plan();
+    }
+
+    // the shared spectra of size M (computed on the stream the first time a window holds M)
+    const Spec& shared_spectra(int M) {
+        for (const Spec& s : shared) if (s.M == M) return s;
+        shared.emplace_back();
+        Spec& s = shared.back();
+        s.M = M;
+        const size_t Mh = M / 2 + 1;
+        T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure(Mh * Mh * sizeof(T)));
+        cplx<T>* tmp = reinterpret_cast<cplx<T>*>(spec_stage.ensure_keep(4 * Mh * Mh * sizeof(cplx<T>), dead));
+        launch_spectra<T>(M, sched.delta0, p->dof, nullptr, centred, ntab, tmp, Kp, st);
+        return s;
+    }
+
+    // windows around Mcur, buffers for their largest grids, spectra of new sizes, block ranges and the table (the one
+    // the last plan did not use, so the iterations still queued keep theirs); then the captured iteration. Nothing
+    // here waits for the device: replaced buffers stay allocated until the run ends.
+    void plan() {
+        const double t0 = now_s();
+        tcur ^= 1;
+        Mlo.assign(count, 0); Mhi.assign(count, 0);
+        sizes.clear();
+        int nbmax = 1;
+        size_t oW = 0, oWh = 0, oPh = 0, oTr = 0, ophi = 0, oit = 0, oiv = 0, ozp = 0;
+        auto al = [](size_t x) { return (x + 63) / 64 * 64; };
+        std::vector<size_t> pW(count), pWh(count), pPh(count), pTr(count), pphi(count), pit(count), piv(count), pzp(count);
+        for (int i = 0; i < count; ++i) {
+            Mlo[i] = window_lo(Mcur[i]); Mhi[i] = window_hi(Mcur[i]);
+            for (int S = Mlo[i]; S > 0 && S <= Mhi[i]; S = fft_size_next(S))
+                if (std::find(sizes.begin(), sizes.end(), S) == sizes.end()) sizes.push_back(S);
+            const size_t Nc = (size_t)Mhi[i] / 2, Mh = Nc + 1, nch = spread_chunks(n[i]);
+            nbmax = std::max(nbmax, std::min(p->max_boxes, (int)(Nc / ni)));
+            pW[i] = oW; oW += al(3 * Nc * Nc);
+            pWh[i] = oWh; oWh += al(3 * Mh * Nc);
+            pPh[i] = oPh; oPh += al(4 * Mh * Nc);
+            pTr[i] = oTr; oTr += al(4 * Mh * Nc);
+            pphi[i] = ophi; ophi += al(4 * Nc * Nc);
             pit[i] = oit; oit += al(2 * nch);
             piv[i] = oiv; oiv += al(2 * nch * ni * 3 * ni);
             pzp[i] = ozp; ozp += al(2 * gather_blocks(n[i]));   // compensated pairs
+            // a window that holds the largest size may clip: the instance's own tables, allocated once at that size
+            if (Mhi[i] == Mmax && !Kc[i][0].p) {
+                const size_t Mhx = Mmax / 2 + 1;
+                for (int t = 0; t < ntab; ++t) Kc[i][t].ensure(Mhx * Mhx * sizeof(T));
+                Kc[i][4].ensure(4 * Mhx * Mhx * sizeof(cplx<T>));
+            }
         }
+        std::sort(sizes.begin(), sizes.end());
+        box_bits = bits_for((long long)nbmax * nbmax);
         grow(aW, oW * sizeof(T), dead); grow(aWh, oWh * sizeof(cplx<T>), dead); grow(aPh, oPh * sizeof(cplx<T>), dead);
         grow(aTr, oTr * sizeof(cplx<T>), dead); grow(aphi, ophi * sizeof(T), dead); grow(aik, oit * sizeof(int), dead);
         grow(aiv, oit * sizeof(int), dead); grow(aival, oiv * sizeof(T), dead); grow(azp, ozp * sizeof(double), dead);
-        if (osp) grow(aspec, osp * sizeof(cplx<T>), dead);
-        const bool early = it < p->early_iter;
-        const T rho = (T)(early ? p->early_exaggeration : p->exaggeration);
-        const T mom = (T)(early ? p->momentum0 : p->momentum1);
         BInst<T>* h = host_table();
         for (int i = 0; i < count; ++i) {
             BInst<T>& b = h[i];
-            b.g = g[i]; b.M = M[i];
+            b.Mlo = Mlo[i]; b.Mhi = Mhi[i];
             b.item_key = dptr<int>(aik) + pit[i]; b.item_valid = dptr<int>(aiv) + pit[i]; b.item_val = dptr<T>(aival) + piv[i];
             b.W = dptr<T>(aW) + pW[i]; b.Wh = dptr<cplx<T>>(aWh) + pWh[i]; b.Ph = dptr<cplx<T>>(aPh) + pPh[i];
             b.Tr = dptr<cplx<T>>(aTr) + pTr[i]; b.phi = dptr<T>(aphi) + pphi[i];
-            for (int t = 0; t < 4; ++t) { b.K[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; b.Kw[t] = t < ntab ? dptr<T>(K[i][t]) : nullptr; }
-            b.zpart = dptr<double>(azp) + pzp[i]; b.zout = dptr<double>(zout) + 2 * i;
-            b.inv_out = dptr<T>(inv) + 4 * i; b.sumq = dptr<double>(sumq) + (size_t)i * std::max(1, p->n_iter) + it;
-            b.upart = dptr<double>(upart) + (size_t)i * 8 * 1024; b.mb = dptr<T>(mbd) + 8 * i;
-            b.delta = sdelta[i]; b.spec_tmp = fresh[i] ? dptr<cplx<T>>(aspec) + psp[i] : nullptr;
+            int w = 0;
+            for (int S = Mlo[i]; S > 0 && S <= Mhi[i] && w < 4; S = fft_size_next(S), ++w) {
+                const Spec& sp = shared_spectra(S);
+                b.KM[w] = S;
+                for (int t = 0; t < 4; ++t) b.KS[w][t] = t < ntab ? dptr<T>(sp.K[t]) : nullptr;
+            }
+            for (; w < 4; ++w) { b.KM[w] = b.KM[w - 1]; for (int t = 0; t < 4; ++t) b.KS[w][t] = b.KS[w - 1][t]; }
+            for (int t = 0; t < 4; ++t) b.Kw[t] = (t < ntab && Kc[i][t].p) ? dptr<T>(Kc[i][t]) : nullptr;
+            b.spec_tmp = Kc[i][4].p ? dptr<cplx<T>>(Kc[i][4]) : nullptr;
+            b.zpart = dptr<double>(azp) + pzp[i];
         }
-        MG_HIP(hipMemcpyAsync(table.p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
-        const BInst<T>* I = dev_table();
-        std::vector<int> all(count);
-        for (int i = 0; i < count; ++i) all[i] = i;
-        std::vector<int> sizes;
-        for (int i = 0; i < count; ++i) if (std::find(sizes.begin(), sizes.end(), M[i]) == sizes.end()) sizes.push_back(M[i]);
-        std::sort(sizes.begin(), sizes.end());
-        auto of_size = [&](int Ms, bool only_fresh) {
-            std::vector<int> w;
-            for (int i = 0; i < count; ++i) if (M[i] == Ms && (!only_fresh || fresh[i])) w.push_back(i);
-            return w;
+        MG_HIP(hipMemcpyAsync(table[tcur].p, h, (size_t)count * sizeof(BInst<T>), hipMemcpyHostToDevice, st));
+        // block ranges
+        std::vector<int> all(count), clip;
+        for (int i = 0; i < count; ++i) { all[i] = i; if (Mhi[i] == Mmax && Kc[i][0].p) clip.push_back(i); }
+        L = Launches();
+        const int Mhx = Mmax / 2 + 1;
+        L.spec1 = make_maps(clip, [&](int) { return fft_col_blocks(Mmax); });
+        L.spect = make_maps(clip, [&](int) { return transpose_blocks(Mhx, Mhx, ntab / 2); });
+        // work lists: candidates and descriptors; a launch's grid is its most blocks (the largest grids of the
+        // windows), at most wl_grid
+        std::vector<WorkDesc> desc;
+        std::vector<int> cand;
+        int woff = 0;
+        const int gmax = tunables().batch_wl_grid;
+        auto add = [&](int kind, int M, const std::vector<int>& c, long long cap_blocks) {
+            desc.push_back({kind, M, (int)c.size(), woff, (int)cand.size()});
+            cand.insert(cand.end(), c.begin(), c.end());
+            woff += wl_ints((int)c.size());
+            return std::make_pair(desc.back().off, (int)std::min<long long>(cap_blocks, gmax));
         };
-        // kernel spectra of the changed grids
-        for (int Ms : sizes) {
-            const std::vector<int> w = of_size(Ms, true);
-            if (w.empty()) continue;
-            const int Mh = Ms / 2 + 1;
-            const auto m1 = make_maps(w, [&](int) { return fft_col_blocks(Ms); });
-            const auto mt = make_maps(w, [&](int) { return transpose_blocks(Mh, Mh, ntab == 4 ? 2 : 1); });
-            for (size_t k = 0; k < m1.size(); ++k) launch_spectra_batch<T>(I, m1[k], mt[k], Ms, p->dof, centred, ntab, st);
+        for (int S : sizes) {
+            std::vector<int> w;
+            long long br = 0, bc = 0, bi = 0;
+            for (int i = 0; i < count; ++i)
+                if (Mlo[i] <= S && S <= Mhi[i]) {
+                    w.push_back(i);
+                    br += fft_row_blocks(S / 2, S, 3); bc += fft_col_blocks(S); bi += fft_row_blocks(S / 2, S, 4);
+                }
+            auto a = add(WK_ROWS3, S, w, br); L.rows_f.push_back(a.first); L.g_rows_f.push_back(a.second);
+            a = add(WK_COLS, S, w, bc); L.cols.push_back(a.first); L.g_cols.push_back(a.second);
+            a = add(WK_ROWS4, S, w, bi); L.rows_i.push_back(a.first); L.g_rows_i.push_back(a.second);
         }
-        // box sort of all points by (instance, box)
-        for (const BMap& m : make_maps(all, [&](int i) { return key_blocks(n[i]); }))
-            launch_keys_batch<T>(I, m, dptr<T>(Yc), box_bits, dptr<int>(keysA), dptr<int>(valsA), st);
-        sorted = launch_key_sort_masked(dptr<int>(keysA), dptr<int>(valsA), dptr<int>(keysB), dptr<int>(valsB), dptr<int>(hist),
-                                        ntot, box_bits + bits_for(count), box_bits, st);
-        // spread
-        launch_fill<T>(dptr<T>(aW), T(0), (i64)oW, st);
-        for (const BMap& m : make_maps(all, [&](int i) { return spread_chunks(n[i]); }))
-            launch_spread_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, st);
-        // FFT convolution
-        for (int Ms : sizes)
-            for (const BMap& m : make_maps(of_size(Ms, false), [&](int i) { return fft_row_blocks(g[i].N, Ms, 3); }))
-                launch_rows_forward_batch<T>(I, m, Ms, st);
-        for (const BMap& m : make_maps(all, [&](int i) { return transpose_blocks(g[i].N, M[i] / 2 + 1, 3); }))
-            launch_transpose_batch<T>(I, m, 0, st);
-        for (int Ms : sizes)
-            for (const BMap& m : make_maps(of_size(Ms, false), [&](int) { return fft_col_blocks(Ms); }))
-                launch_columns_batch<T>(I, m, Ms, st);
-        for (const BMap& m : make_maps(all, [&](int i) { return transpose_blocks(M[i] / 2 + 1, g[i].N, 4); }))
-            launch_transpose_batch<T>(I, m, 1, st);
-        for (int Ms : sizes)
-            for (const BMap& m : make_maps(of_size(Ms, false), [&](int i) { return fft_row_blocks(g[i].N, Ms, 4); }))
-                launch_rows_inverse_batch<T>(I, m, Ms, st);
-        // gather, Z, and the update
-        for (const BMap& m : make_maps(all, [&](int i) { return gather_blocks(n[i]); }))
-            launch_gather_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, dptr<T>(Fc), st);
-        if (a_pending) { MG_HIP(hipStreamWaitEvent(st, ev_a1, 0)); a_pending = false; }   // Ac of this iteration
-        for (const BMap& m : make_maps(all, [&](int i) { return update_blocks(n[i]); }))
-            launch_update_batch<T>(I, m, dptr<T>(Yc), dptr<T>(Uc), dptr<T>(Gc), dptr<T>(Ac), dptr<T>(Fc), rho, mom, st);
-        MG_HIP(hipMemcpyAsync(hmb.p, mbd.p, (size_t)8 * count * sizeof(T), hipMemcpyDeviceToHost, st));
-        MG_HIP(hipEventRecord(ready, st));
-        if (it + 1 < p->n_iter) attraction();
+        long long b0 = 0, b1 = 0, bf = 0;
+        for (int i = 0; i < count; ++i) {
+            b0 += transpose_cap_blocks(Mhi[i], 0); b1 += transpose_cap_blocks(Mhi[i], 1); bf += fill_blocks(Mhi[i]);
+        }
+        auto a = add(WK_TR0, 0, all, b0); L.tr0 = a.first; L.g_tr0 = a.second;
+        a = add(WK_TR1, 0, all, b1); L.tr1 = a.first; L.g_tr1 = a.second;
+        a = add(WK_FILL, 0, all, bf); L.fill = a.first; L.g_fill = a.second;
+        L.ndesc = (int)desc.size();
+        require(desc.size() * sizeof(WorkDesc) <= hdesc[tcur].bytes && cand.size() * sizeof(int) <= hcand[tcur].bytes,
+                "work-list staging too small");
+        void* hd = hdesc[tcur].p;
+        void* hc = hcand[tcur].p;
+        std::memcpy(hd, desc.data(), desc.size() * sizeof(WorkDesc));
+        std::memcpy(hc, cand.data(), cand.size() * sizeof(int));
+        MG_HIP(hipMemcpyAsync(wdesc[tcur].ensure_keep(desc.size() * sizeof(WorkDesc), dead), hd, desc.size() * sizeof(WorkDesc),
+                              hipMemcpyHostToDevice, st));
+        MG_HIP(hipMemcpyAsync(wcand[tcur].ensure_keep(cand.size() * sizeof(int), dead), hc, cand.size() * sizeof(int),
+                              hipMemcpyHostToDevice, st));
+        wlist[tcur].ensure_keep((size_t)woff * sizeof(int), dead);
+        L.keys = make_maps(all, [&](int i) { return key_blocks(n[i]); });
+        L.spread = make_maps(all, [&](int i) { return spread_chunks(n[i]); });
+        L.gather = make_maps(all, [&](int i) { return gather_blocks(n[i]); });
+        L.update = make_maps(all, [&](int i) { return update_blocks(n[i]); });
+        if (exec || graph) old_graphs.push_back({graph, exec});
+        graph = nullptr; exec = nullptr;
+        if (use_graph) {
+            // one capture at a time in the process (groups plan from several host threads)
+            static std::mutex capture_mu;
+            std::lock_guard<std::mutex> lk(capture_mu);
+            MG_HIP(hipStreamBeginCapture(cst, hipStreamCaptureModeRelaxed));
+            try {
+                iteration(cst);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cst, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            MG_HIP(hipStreamEndCapture(cst, &graph));
+            MG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        } else {
+            n_launches_per_iter = count_launches();
+        }
+        ++n_plans;
+        plan_s += now_s() - t0;
+    }
+    long long count_launches() const {
+        return 2 + (long long)attr_maps.size() + 3 * (long long)L.spec1.size() + (long long)L.keys.size()
+               + 3 * ((box_bits + bits_for(count) + 9) / 10) + 3 + 2 * L.spread.size() + 2 * L.gather.size()
+               + 2 * L.update.size() + 3 * (long long)sizes.size();
     }
 
-    void complete() { MG_HIP(hipEventSynchronize(ready)); }
+    // one iteration of every instance on stream s: the decision, the attraction beside the repulsion, the update
+    void iteration(hipStream_t s) {
+        BInst<T>* I = dev_table();
+        launch_decide_batch<T>(I, dptr<BState>(state), sched, s);
+        launch_worklists<T>(I, dptr<WorkDesc>(wdesc[tcur]), L.ndesc, dptr<int>(wcand[tcur]), dptr<int>(wlist[tcur]), s);
+        // the attraction needs the last update only: it runs on its own stream beside the repulsion
+        MG_HIP(hipEventRecord(ev_a0, s));
+        MG_HIP(hipStreamWaitEvent(aside, ev_a0, 0));
+        const BInst<T>* IA = reinterpret_cast<const BInst<T>*>(atable.p);
+        for (size_t a = 0; a < attr_maps.size(); ++a)
+            launch_attraction_batch<T>(IA, attr_maps[a], attr_G[a], dptr<T>(Yc), inv_dof, dptr<T>(Ac), aside);
+        MG_HIP(hipEventRecord(ev_a1, aside));
+        // kernel spectra of the clipped grids whose width changed
+        for (size_t q = 0; q < L.spec1.size(); ++q)
+            launch_spectra_batch<T>(I, L.spec1[q], L.spect[q], Mmax, p->dof, centred, ntab, s);
+        // box sort of all points by (instance, box)
+        for (const BMap& m : L.keys) launch_keys_batch<T>(I, m, dptr<T>(Yc), box_bits, dptr<int>(keysA), dptr<int>(valsA), s);
+        sorted = launch_key_sort_masked(dptr<int>(keysA), dptr<int>(valsA), dptr<int>(keysB), dptr<int>(valsB), dptr<int>(hist),
+                                        ntot, box_bits + bits_for(count), box_bits, s);
+        // spread
+        const int* wl = dptr<int>(wlist[tcur]);
+        launch_fill_wl<T>(I, wl + L.fill, L.g_fill, s);
+        for (const BMap& m : L.spread) launch_spread_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, s);
+        // FFT convolution
+        for (size_t z = 0; z < sizes.size(); ++z) launch_rows_forward_wl<T>(I, wl + L.rows_f[z], sizes[z], L.g_rows_f[z], s);
+        launch_transpose_wl<T>(I, wl + L.tr0, 0, L.g_tr0, s);
+        for (size_t z = 0; z < sizes.size(); ++z) launch_columns_wl<T>(I, wl + L.cols[z], sizes[z], L.g_cols[z], s);
+        launch_transpose_wl<T>(I, wl + L.tr1, 1, L.g_tr1, s);
+        for (size_t z = 0; z < sizes.size(); ++z) launch_rows_inverse_wl<T>(I, wl + L.rows_i[z], sizes[z], L.g_rows_i[z], s);
+        // gather, Z, and the update
+        for (const BMap& m : L.gather) launch_gather_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, dptr<T>(Fc), s);
+        MG_HIP(hipStreamWaitEvent(s, ev_a1, 0));
+        for (const BMap& m : L.update) launch_update_batch<T>(I, m, dptr<T>(Yc), dptr<T>(Uc), dptr<T>(Gc), dptr<T>(Ac), dptr<T>(Fc), s);
+        n_launches_per_iter = count_launches();
+    }
+
+    // queue the next stretch of iterations and the read-back of the instances' states (at most two stretches
+    // outstanding: the device runs one while the host reads the other and plans)
+    void launch() {
+        if (done || pending.size() >= 2) return;
+        int left = 0;
+        for (int i = 0; i < count; ++i) left = std::max(left, p->n_iter - last_next[i]);
+        for (const auto& q : pending) left -= q.second;
+        if (left <= 0 && !pending.empty()) return;
+        const int k = std::max(1, std::min(chunk, left));
+        for (int r = 0; r < k; ++r) {
+            ++n_iters_issued;
+            if (use_graph) { MG_HIP(hipGraphLaunch(exec, st)); ++n_replays; }
+            else iteration(st);
+            n_launches += n_launches_per_iter;
+        }
+        MG_HIP(hipMemcpyAsync(hrep[slot].p, state.p, (size_t)count * sizeof(BState), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipEventRecord(ev_chunk[slot], st));
+        pending.push_back({slot, k});
+        slot ^= 1;
+    }
+
+    // wait for the oldest stretch; plan again when an instance stalled or reached the edge of its window
+    void check() {
+        if (done || pending.empty()) return;
+        const int sl = pending.front().first;
+        pending.erase(pending.begin());
+        MG_HIP(hipEventSynchronize(ev_chunk[sl]));
+        ++n_checks;
+        const BState* hs = reinterpret_cast<const BState*>(hrep[sl].p);
+        bool all_done = true, replan = false;
+        for (int i = 0; i < count; ++i) {
+            require(hs[i].stall != 2, "layout is not finite");
+            last_next[i] = hs[i].next_it;
+            if (hs[i].next_it >= p->n_iter) continue;
+            all_done = false;
+            const int M = hs[i].M;
+            require(M > 0 && M <= Mmax, "grid too large for the device FFT (n_interp * n_boxes > 4096)", MG_ERR_UNSUP);
+            if (hs[i].stall == 1) ++n_stalls;
+            if (hs[i].stall == 1 || M != Mcur[i]) replan = true;
+        }
+        if (all_done) { done = true; return; }
+        if (replan) {
+            for (int i = 0; i < count; ++i)
+                if (hs[i].next_it < p->n_iter) Mcur[i] = hs[i].M;
+            plan();
+        }
+    }
 
     void finish() {
         const T* Y = dptr<T>(Yc); const T* U = dptr<T>(Uc); const T* G = dptr<T>(Gc);
@@ -969,7 +1151,8 @@ template <class T> struct BatchEngine {
         }
         MG_HIP(hipStreamSynchronize(st));
         MG_HIP(hipStreamSynchronize(aside));
-        a_pending = false;
+        pending.clear();
+        drop_graphs();
         release_dead();
     }
     void release_dead() {
@@ -978,9 +1161,14 @@ template <class T> struct BatchEngine {
     }
 };
 
+// counters of the last batched run (summed over its groups)
+static std::atomic<long long> g_bs_plans{0}, g_bs_replays{0}, g_bs_launches{0}, g_bs_checks{0}, g_bs_iters{0}, g_bs_stalls{0};
+static std::atomic<long long> g_bs_plan_us{0};
+
 // The batch in n_groups groups of about equal point counts (largest instances dealt first), each group one
-// batched engine on its own HIP stream; group s is driven by host thread s mod n_threads. While the host
-// decides the grids of one group, the device runs the others.
+// batched engine on its own HIP stream; group s is driven by host thread s mod n_threads. A thread queues a stretch
+// of iterations for each of its groups, then checks each in turn, so the device runs one group while the host
+// plans another.
 template <class T> void optimize_batch(const mg_tsne_params* p, int count, const mg_tsne_instance* inst, int n_groups,
                                       int n_threads) {
     check_schedule(p);
@@ -1016,10 +1204,14 @@ template <class T> void optimize_batch(const mg_tsne_params* p, int count, const
             MG_HIP(hipSetDevice(dev));
             std::vector<BatchEngine<T>*> mine;
             for (int s = t; s < S; s += NT) mine.push_back(eng[s].get());
-            if (p->n_iter > 0) for (auto* e : mine) e->issue(0);
-            for (int it = 1; it < p->n_iter && !failed.load(std::memory_order_relaxed); ++it)
-                for (auto* e : mine) { e->complete(); e->issue(it); }
-            for (auto* e : mine) { e->complete(); e->finish(); }
+            for (auto* e : mine) e->launch();
+            for (;;) {
+                bool busy = false;
+                for (auto* e : mine) if (!e->done) { e->launch(); busy = true; }
+                if (!busy || failed.load(std::memory_order_relaxed)) break;
+                for (auto* e : mine) e->check();
+            }
+            for (auto* e : mine) e->finish();
         } catch (const Error& e) {
             codes[t] = e.code; errs[t] = e.what(); failed = true;
         } catch (const std::exception& e) {
@@ -1031,6 +1223,12 @@ template <class T> void optimize_batch(const mg_tsne_params* p, int count, const
     work(0);
     for (auto& th : pool) th.join();
     for (auto x : streams) (void)hipStreamSynchronize(x);
+    long long pl = 0, rp = 0, la = 0, ch = 0, us = 0, sl = 0, it = 0;
+    for (auto& e : eng) {
+        pl += e->n_plans; rp += e->n_replays; la += e->n_launches; ch += e->n_checks; us += (long long)(e->plan_s * 1e6);
+        sl += e->n_stalls; it += e->n_iters_issued;
+    }
+    g_bs_plans = pl; g_bs_replays = rp; g_bs_launches = la; g_bs_checks = ch; g_bs_plan_us = us; g_bs_iters = it; g_bs_stalls = sl;
     for (int t = 0; t < NT; ++t) if (codes[t]) throw Error(codes[t], errs[t]);
 }
 
@@ -1131,6 +1329,10 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "attr_win_kb") { require(value == 0 || value == 64 || value == 128, "attr_win_kb must be 0, 64 or 128"); g_tunables.attr_win_kb = value; }
     else if (k == "attr_tpb") { require(value == 256 || value == 512 || value == 1024, "attr_tpb must be 256, 512 or 1024"); g_tunables.attr_tpb = value; }
     else if (k == "sort_items") { require(value == 0 || value == 1 || value == 2 || value == 4 || value == 8, "sort_items must be 0, 1, 2, 4 or 8"); g_tunables.sort_items = value; }
+    else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
+    else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
+    else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }
+    else if (k == "batch_wl_grid") { require(value >= 64 && value <= 65536, "batch_wl_grid must be in [64, 65536]"); g_tunables.batch_wl_grid = value; }
     else require(false, "unknown tunable: " + k);
     MG_API_END
 }
@@ -1144,6 +1346,10 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "attr_win_kb") *value = g_tunables.attr_win_kb;
     else if (k == "attr_tpb") *value = g_tunables.attr_tpb;
     else if (k == "sort_items") *value = g_tunables.sort_items;
+    else if (k == "batch_graph") *value = g_tunables.batch_graph;
+    else if (k == "batch_chunk") *value = g_tunables.batch_chunk;
+    else if (k == "batch_window_down") *value = g_tunables.batch_window_down;
+    else if (k == "batch_wl_grid") *value = g_tunables.batch_wl_grid;
     else require(false, "unknown tunable: " + k);
     MG_API_END
 }
@@ -1406,6 +1612,19 @@ int mg_tsne_optimize_batch(int prec, const mg_tsne_params* p, int count, const m
     require(p != nullptr, "params must not be null");
     if (prec == MG_F64) optimize_batch<double>(p, count, inst, n_streams, n_threads);
     else optimize_batch<float>(p, count, inst, n_streams, n_threads);
+    MG_API_END
+}
+
+int mg_tsne_batch_stats(int64_t* plans, int64_t* iterations, int64_t* replays, int64_t* launches, int64_t* checks,
+                        int64_t* stalls, double* plan_seconds) {
+    MG_API_BEGIN
+    if (plans) *plans = g_bs_plans.load();
+    if (iterations) *iterations = g_bs_iters.load();
+    if (stalls) *stalls = g_bs_stalls.load();
+    if (replays) *replays = g_bs_replays.load();
+    if (launches) *launches = g_bs_launches.load();
+    if (checks) *checks = g_bs_checks.load();
+    if (plan_seconds) *plan_seconds = 1e-6 * (double)g_bs_plan_us.load();
     MG_API_END
 }
 

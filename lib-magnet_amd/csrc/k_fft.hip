@@ -83,18 +83,26 @@ template <class T> static TwSet<T> set_of(const Tables& t, const cplx<T>* base, 
 template <> TwSet<float> fft_twiddle_set<float>(int L) { const Tables& t = g_tw[current_device()]; return set_of<float>(t, t.f32, L); }
 template <> TwSet<double> fft_twiddle_set<double>(int L) { const Tables& t = g_tw[current_device()]; return set_of<double>(t, t.f64, L); }
 
+void kernel_lds_limit(const void* f, int bytes) {
+    struct Set { int dev; const void* f; int bytes; };
+    static std::mutex mu;
+    static std::vector<Set> done;
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& d : done)
+        if (d.dev == dev && d.f == f) {
+            if (d.bytes >= bytes) return;
+            MG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            d.bytes = bytes;
+            return;
+        }
+    MG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done.push_back({dev, f, bytes});
+}
+
 // blocks of the batched row and column passes (transforms per workgroup as in fft_batch)
 static int batch_of(int L) { (void)size_index(L); return L / 16 >= 256 ? 1 : 256 / (L / 16); }
 int fft_row_blocks(int N, int M, int channels) { const int B = batch_of(M / 2); return (N + B - 1) / B * channels; }
 int fft_col_blocks(int M) { const int B = batch_of(M); return (M / 2 + 1 + B - 1) / B; }
-
-// the transform sizes with compiled plans: 2^s and 3 * 2^s from 128 to 8192
-int fft_size(int N) {
-    for (int M = 128; M <= MAX_M; M *= 2) {
-        if (3 * (M / 4) >= 2 * N && 3 * (M / 4) >= 128) return 3 * (M / 4);
-        if (M >= 2 * N) return M;
-    }
-    return 0;
-}
 
 }  // namespace mg

@@ -71,15 +71,22 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_fwd(const T* __restrict
                                                              TwSet<T> tws) {
     rows_fwd_block<T, L>(W, N, out, tws, blockIdx.x, blockIdx.y);
 }
-// batched: instance i's blocks cover (ceil(N / batch) row groups) x (3 channels), row groups fastest
+// over a work list: virtual block v of instance k is the single launch's block (row group lb % nbx, channel
+// lb / nbx) of that instance's live grid
 template <class T, int L>
-__global__ __launch_bounds__(fft_block<L>()) void kb_rows_fwd(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
-    const int k = bmap_find(m, blockIdx.x);
-    const BInst<T>& b = I[m.inst[k]];
-    const int N = b.g.N, nbx = (N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
-    const T* W = b.W;
-    cplx<T>* Tr = b.Tr;
-    rows_fwd_block<T, L>(W, N, Tr, tws, lb % nbx, lb / nbx);
+__global__ __launch_bounds__(fft_block<L>()) void kw_rows_fwd(const BInst<T>* __restrict__ I, const int* __restrict__ wl,
+                                                              TwSet<T> tws) {
+    constexpr int B = fft_batch<L>();
+    const int total = wl[0];
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int k = wl_find(wl, v);
+        const BInst<T>& b = I[wl_inst(wl, k)];
+        const int N = b.g.N, nbx = (N + B - 1) / B, lb = v - wl_start(wl, k);
+        const T* W = b.W;
+        cplx<T>* Tr = b.Tr;
+        rows_fwd_block<T, L>(W, N, Tr, tws, lb % nbx, lb / nbx);
+        __syncthreads();   // the LDS is taken by the next virtual block
+    }
 }
 
 // ---- batched transpose of complex matrices, src (B, R, C) -> dst (B, C, R), 32 x 32 tiles through LDS
@@ -104,20 +111,32 @@ template <class T>
 __global__ __launch_bounds__(256) void k_transpose(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int R, int C) {
     transpose_block<T>(src, dst, R, C, blockIdx.x, blockIdx.y, blockIdx.z);
 }
-// batched transposes of each instance's buffers (which: see launch_transpose_batch)
+// the spectra's transpose (stage 1 -> 2, spec_tmp) of the clipped instances at size Ms whose tables are fresh
 template <class T>
-__global__ __launch_bounds__(256) void kb_transpose(const BInst<T>* __restrict__ I, BMap m, int which) {
+__global__ __launch_bounds__(256) void kb_transpose_spec(const BInst<T>* __restrict__ I, BMap m, int planes, int Ms) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
-    const int Mh = b.M / 2 + 1;
-    const cplx<T>* src;
-    cplx<T>* dst;
-    int R, C;
-    if (which == 0) { src = b.Tr; dst = b.Wh; R = b.g.N; C = Mh; }
-    else if (which == 1) { src = b.Ph; dst = b.Tr; R = Mh; C = b.g.N; }
-    else { src = b.spec_tmp; dst = b.spec_tmp + 2 * (size_t)Mh * Mh; R = Mh; C = Mh; }
-    const int nx = (C + 31) / 32, ny = (R + 31) / 32, lb = blockIdx.x - m.start[k];
-    transpose_block<T>(src, dst, R, C, lb % nx, (lb / nx) % ny, lb / (nx * ny));
+    if (!b.active || !b.fresh || b.M != Ms) return;
+    const int Mh = Ms / 2 + 1, nx = (Mh + 31) / 32, lb = blockIdx.x - m.start[k];
+    (void)planes;
+    transpose_block<T>(b.spec_tmp, b.spec_tmp + 2 * (size_t)Mh * Mh, Mh, Mh, lb % nx, (lb / nx) % nx, lb / (nx * nx));
+}
+// the grid passes' transposes over a work list (which 0: Tr -> Wh, 1: Ph -> Tr), tiles numbered over the live
+// matrices; a transpose moves values only
+template <class T>
+__global__ __launch_bounds__(256) void kw_transpose(const BInst<T>* __restrict__ I, const int* __restrict__ wl, int which) {
+    const int total = wl[0];
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int k = wl_find(wl, v);
+        const BInst<T>& b = I[wl_inst(wl, k)];
+        const int Mh = b.M / 2 + 1, N = b.g.N;
+        const cplx<T>* src = which == 0 ? b.Tr : b.Ph;
+        cplx<T>* dst = which == 0 ? b.Wh : b.Tr;
+        const int R = which == 0 ? N : Mh, C = which == 0 ? Mh : N;
+        const int nx = (C + 31) / 32, ny = (R + 31) / 32, lb = v - wl_start(wl, k);
+        transpose_block<T>(src, dst, R, C, lb % nx, (lb / nx) % ny, lb / (nx * ny));
+        __syncthreads();   // the tile is taken by the next virtual block
+    }
 }
 
 // kernel table value at column frequency kx (stored for kx <= M/2; `odd` flips the sign above M/2)
@@ -206,13 +225,18 @@ __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __res
     columns_block<T, L, CEN>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x);
 }
 template <class T, int L, bool CEN>
-__global__ __launch_bounds__(fft_block<L>()) void kb_columns(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
-    const int k = bmap_find(m, blockIdx.x);
-    const BInst<T>& b = I[m.inst[k]];
-    const cplx<T>* Wh = b.Wh;
-    cplx<T>* Ph = b.Ph;
-    const T *K0 = b.K[0], *K1 = b.K[1], *K2 = b.K[2], *K3 = b.K[3];
-    columns_block<T, L, CEN>(Wh, b.g.N, K0, K1, K2, K3, Ph, tws, blockIdx.x - m.start[k]);
+__global__ __launch_bounds__(fft_block<L>()) void kw_columns(const BInst<T>* __restrict__ I, const int* __restrict__ wl,
+                                                             TwSet<T> tws) {
+    const int total = wl[0];
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int k = wl_find(wl, v);
+        const BInst<T>& b = I[wl_inst(wl, k)];
+        const cplx<T>* Wh = b.Wh;
+        cplx<T>* Ph = b.Ph;
+        const T *K0 = b.K[0], *K1 = b.K[1], *K2 = b.K[2], *K3 = b.K[3];
+        columns_block<T, L, CEN>(Wh, b.g.N, K0, K1, K2, K3, Ph, tws, v - wl_start(wl, k));
+        __syncthreads();
+    }
 }
 
 // ---- rows inverse: blockIdx.y = channel, fft_batch rows per workgroup; planar output (4, N, N)
@@ -251,13 +275,19 @@ __global__ __launch_bounds__(fft_block<L>()) void k_rows_inv(const cplx<T>* __re
     rows_inv_block<T, L>(X, N, phi, scale, tws, blockIdx.x, blockIdx.y);
 }
 template <class T, int L>
-__global__ __launch_bounds__(fft_block<L>()) void kb_rows_inv(const BInst<T>* __restrict__ I, BMap m, T scale, TwSet<T> tws) {
-    const int k = bmap_find(m, blockIdx.x);
-    const BInst<T>& b = I[m.inst[k]];
-    const int N = b.g.N, nbx = (N + fft_batch<L>() - 1) / fft_batch<L>(), lb = blockIdx.x - m.start[k];
-    const cplx<T>* Tr = b.Tr;
-    T* phi = b.phi;
-    rows_inv_block<T, L>(Tr, N, phi, scale, tws, lb % nbx, lb / nbx);
+__global__ __launch_bounds__(fft_block<L>()) void kw_rows_inv(const BInst<T>* __restrict__ I, const int* __restrict__ wl, T scale,
+                                                              TwSet<T> tws) {
+    constexpr int B = fft_batch<L>();
+    const int total = wl[0];
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int k = wl_find(wl, v);
+        const BInst<T>& b = I[wl_inst(wl, k)];
+        const int N = b.g.N, nbx = (N + B - 1) / B, lb = v - wl_start(wl, k);
+        const cplx<T>* Tr = b.Tr;
+        T* phi = b.phi;
+        rows_inv_block<T, L>(Tr, N, phi, scale, tws, lb % nbx, lb / nbx);
+        __syncthreads();
+    }
 }
 
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
@@ -339,6 +369,7 @@ __global__ __launch_bounds__(fft_block<L>()) void kb_spec1(const BInst<T>* __res
                                                            const double* __restrict__ table, int round_kernel, TwSet<T> tws) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active || !b.fresh || b.M != L) return;
     const size_t plane = (size_t)(L / 2 + 1) * (L / 2 + 1);
     spec1_block<T, L, NT>(b.delta, dof, table, round_kernel, b.spec_tmp, b.spec_tmp + plane, tws, blockIdx.x - m.start[k]);
 }
@@ -403,6 +434,7 @@ template <class T, int L, int NT>
 __global__ __launch_bounds__(fft_block<L>()) void kb_spec2(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active || !b.fresh || b.M != L) return;
     const size_t plane = (size_t)(L / 2 + 1) * (L / 2 + 1);
     spec2_block<T, L, NT>(b.spec_tmp + 2 * plane, b.spec_tmp + 3 * plane, b.Kw[0], b.Kw[1], b.Kw[2], b.Kw[3], tws,
                           blockIdx.x - m.start[k]);
@@ -416,7 +448,7 @@ __global__ void k_interleave4(const T* __restrict__ src, T* __restrict__ dst, i6
 }
 
 template <class F> void set_lds(F* f, int bytes) {
-    if (bytes > 65536) MG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (bytes > 65536) kernel_lds_limit(reinterpret_cast<const void*>(f), bytes);
 }
 
 template <class T, int L> int lds_bytes() { return fft_batch<L>() * padded(L) * (int)sizeof(cplx<T>); }
@@ -506,44 +538,42 @@ template <class T> void launch_phi_interleave(const T* planar, T* out, i64 NN, h
     MG_HIP(hipGetLastError());
 }
 
-template <class T> void launch_rows_forward_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
-    if (m.count == 0 || m.start[m.count] == 0) return;
-    const dim3 grid((unsigned)m.start[m.count]);
+template <class T> void launch_rows_forward_wl(const BInst<T>* I, const int* wl, int M, int grid, hipStream_t st) {
+    if (grid <= 0) return;
 #define MG_C(L_) case L_: { \
-        const int by = lds_bytes<T, L_>(); set_lds(kb_rows_fwd<T, L_>, by); \
-        hipLaunchKernelGGL((kb_rows_fwd<T, L_>), grid, dim3(fft_block<L_>()), by, st, I, m, fft_twiddle_set<T>(L_)); \
+        const int by = lds_bytes<T, L_>(); set_lds(kw_rows_fwd<T, L_>, by); \
+        hipLaunchKernelGGL((kw_rows_fwd<T, L_>), dim3(grid), dim3(fft_block<L_>()), by, st, I, wl, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
     MG_HIP(hipGetLastError());
 }
 
-template <class T> void launch_transpose_batch(const BInst<T>* I, const BMap& m, int which, hipStream_t st) {
-    if (m.count == 0 || m.start[m.count] == 0) return;
-    hipLaunchKernelGGL(kb_transpose<T>, dim3((unsigned)m.start[m.count]), dim3(256), 0, st, I, m, which);
+template <class T> void launch_transpose_wl(const BInst<T>* I, const int* wl, int which, int grid, hipStream_t st) {
+    require(which == 0 || which == 1, "grid transposes are 0 or 1");
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(kw_transpose<T>, dim3(grid), dim3(256), 0, st, I, wl, which);
     MG_HIP(hipGetLastError());
 }
 
-template <class T> void launch_columns_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
-    if (m.count == 0 || m.start[m.count] == 0) return;
+template <class T> void launch_columns_wl(const BInst<T>* I, const int* wl, int M, int grid, hipStream_t st) {
+    if (grid <= 0) return;
     constexpr bool CEN = sizeof(T) == 4;
-    const dim3 grid((unsigned)m.start[m.count]);
 #define MG_C(L_) case L_: { \
-        const int by = lds_bytes<T, L_>(); set_lds(kb_columns<T, L_, CEN>, by); \
-        hipLaunchKernelGGL((kb_columns<T, L_, CEN>), grid, dim3(fft_block<L_>()), by, st, I, m, fft_twiddle_set<T>(L_)); \
+        const int by = lds_bytes<T, L_>(); set_lds(kw_columns<T, L_, CEN>, by); \
+        hipLaunchKernelGGL((kw_columns<T, L_, CEN>), dim3(grid), dim3(fft_block<L_>()), by, st, I, wl, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
     MG_HIP(hipGetLastError());
 }
 
-template <class T> void launch_rows_inverse_batch(const BInst<T>* I, const BMap& m, int M, hipStream_t st) {
-    if (m.count == 0 || m.start[m.count] == 0) return;
+template <class T> void launch_rows_inverse_wl(const BInst<T>* I, const int* wl, int M, int grid, hipStream_t st) {
+    if (grid <= 0) return;
     const T scale = (T)(1.0 / ((double)M * (double)M));
-    const dim3 grid((unsigned)m.start[m.count]);
 #define MG_C(L_) case L_: { \
-        const int by = lds_bytes<T, L_>(); set_lds(kb_rows_inv<T, L_>, by); \
-        hipLaunchKernelGGL((kb_rows_inv<T, L_>), grid, dim3(fft_block<L_>()), by, st, I, m, scale, fft_twiddle_set<T>(L_)); \
+        const int by = lds_bytes<T, L_>(); set_lds(kw_rows_inv<T, L_>, by); \
+        hipLaunchKernelGGL((kw_rows_inv<T, L_>), dim3(grid), dim3(fft_block<L_>()), by, st, I, wl, scale, fft_twiddle_set<T>(L_)); \
         break; }
     switch (M / 2) { MG_ROW_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
@@ -560,7 +590,9 @@ template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, 
         const int by = lds_bytes<T, L_>(); set_lds(kb_spec1<T, L_, NT>, by); set_lds(kb_spec2<T, L_, NT>, by); \
         hipLaunchKernelGGL((kb_spec1<T, L_, NT>), grid, dim3(fft_block<L_>()), by, st, I, m1, dof, (const double*)nullptr, (int)round_kernel, fft_twiddle_set<T>(L_)); \
         MG_HIP(hipGetLastError()); \
-        launch_transpose_batch<T>(I, mt, NT == 4 ? 3 : 2, st); \
+        if (mt.count > 0 && mt.start[mt.count] > 0) \
+            hipLaunchKernelGGL(kb_transpose_spec<T>, dim3((unsigned)mt.start[mt.count]), dim3(256), 0, st, I, mt, NT / 2, M); \
+        MG_HIP(hipGetLastError()); \
         hipLaunchKernelGGL((kb_spec2<T, L_, NT>), grid, dim3(fft_block<L_>()), by, st, I, m1, fft_twiddle_set<T>(L_)); \
     } while (0)
 #define MG_C(L_) case L_: if (ntab == 4) MG_S(L_, 4); else MG_S(L_, 2); break;
@@ -577,10 +609,10 @@ template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, 
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
     template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t); \
     template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t); \
-    template void launch_rows_forward_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
-    template void launch_transpose_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
-    template void launch_columns_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
-    template void launch_rows_inverse_batch<T>(const BInst<T>*, const BMap&, int, hipStream_t); \
+    template void launch_rows_forward_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
+    template void launch_transpose_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
+    template void launch_columns_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
+    template void launch_rows_inverse_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
     template void launch_spectra_batch<T>(const BInst<T>*, const BMap&, const BMap&, int, double, bool, int, hipStream_t);
 
 }  // namespace mg

@@ -382,6 +382,7 @@ __global__ __launch_bounds__(TPB) void kb_spread(const BInst<T>* __restrict__ I,
                                                  const int* __restrict__ vals, const T* __restrict__ Y) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active) return;
     const GridT<T> g = b.g;
     const i64 off = b.off, n = b.n;
     T* W = b.W;
@@ -394,6 +395,7 @@ template <class T, int NI>
 __global__ __launch_bounds__(TPB) void kb_spread_span(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active) return;
     const GridT<T> g = b.g;
     const i64 off = b.off, n = b.n;
     T* W = b.W;
@@ -403,20 +405,25 @@ __global__ __launch_bounds__(TPB) void kb_spread_span(const BInst<T>* __restrict
     span_block<T, NI>(keys + off, n, m.start[k + 1] - m.start[k], g, W, ik, iv, ival, blockIdx.x - m.start[k]);
 }
 
-// batched sort keys: (instance << box_bits) | box key, and the global point index
+// batched sort keys: (instance << box_bits) | box key, and the global point index; an inactive instance's points
+// all take box 0 (they are sorted with the rest and not used)
 template <class T>
 __global__ __launch_bounds__(TPB) void kb_keys(const BInst<T>* __restrict__ I, BMap m, const T* __restrict__ Y, int box_bits,
                                                int* __restrict__ keys, int* __restrict__ vals) {
     const int k = bmap_find(m, blockIdx.x);
     const int inst = m.inst[k];
     const BInst<T>& b = I[inst];
-    const GridT<T> g = b.g;
     const i64 j = (i64)(blockIdx.x - m.start[k]) * TPB + threadIdx.x;
     if (j >= b.n) return;
     const i64 p = b.off + j;
-    typedef typename vec2<T>::type V;
-    const V y = reinterpret_cast<const V*>(Y)[p];
-    keys[p] = (inst << box_bits) | box_key<T>(y.x, y.y, g);
+    int box = 0;
+    if (b.active) {
+        const GridT<T> g = b.g;
+        typedef typename vec2<T>::type V;
+        const V y = reinterpret_cast<const V*>(Y)[p];
+        box = box_key<T>(y.x, y.y, g);
+    }
+    keys[p] = (inst << box_bits) | box;
     vals[p] = (int)p;
 }
 
@@ -536,6 +543,7 @@ __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I,
                                                  const int* __restrict__ vals, const T* __restrict__ Y, T* __restrict__ F) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active) return;
     const GridT<T> g = b.g;
     const i64 off = b.off, n = b.n;
     const T* phi = b.phi;
@@ -546,9 +554,145 @@ __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I,
 template <class T>
 __global__ __launch_bounds__(TPB) void kb_zfinal(const BInst<T>* __restrict__ I, BMap m) {
     const BInst<T>& b = I[m.inst[bmap_find(m, blockIdx.x)]];
+    if (!b.active) return;
     const i64 n = b.n, nb = (n + TPB - 1) / TPB;   // the gather's blocks (gather_blocks)
     const int nblk = (int)(nb < 1024 ? (nb < 1 ? 1 : nb) : 1024);
     zfinal_block(b.zpart, nblk, n, b.zout, b.inv_out, sizeof(T) == 4, b.sumq);
+}
+
+// batched clear of the live charge grid (3 N^2 values) of every active instance, 4096 values per virtual block
+template <class T>
+__global__ __launch_bounds__(TPB) void kw_fill_w(const BInst<T>* __restrict__ I, const int* __restrict__ wl) {
+    const int total = wl[0];
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int k = wl_find(wl, v);
+        const BInst<T>& b = I[wl_inst(wl, k)];
+        const i64 cnt = 3ll * b.g.N * b.g.N, base = (i64)(v - wl_start(wl, k)) * 4096;
+        T* W = b.W;
+        for (int j = threadIdx.x; j < 4096; j += TPB)
+            if (base + j < cnt) W[base + j] = T(0);
+    }
+}
+
+// transforms per workgroup of the FFT passes of length L (fft_batch of fft_core.hpp)
+__device__ inline int fft_batch_of(int L) { return L / 16 >= 256 ? 1 : 256 / (L / 16); }
+
+// inclusive prefix sum over the TPB threads of a block (wave scans, then the wave totals through LDS)
+__device__ inline int block_inclusive_scan(int v, int* sh) {
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == WAVE - 1) sh[wv] = v;
+    __syncthreads();
+    int add = 0;
+    for (int w = 0; w < wv; ++w) add += sh[w];
+    __syncthreads();
+    return v + add;
+}
+
+// The work lists of one iteration, one block per list: each candidate instance that runs the iteration (and, for the
+// FFT passes, sits at the list's size) takes, in candidate order, the blocks its single launch would: rows
+// ceil(N / batch) per channel, columns ceil(Mh / batch), transposes the 32 x 32 tiles of its live matrices, the fill
+// 4096 values each.
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_worklists(const BInst<T>* __restrict__ I, const WorkDesc* __restrict__ D,
+                                                    const int* __restrict__ cand, int* __restrict__ wl) {
+    __shared__ int sh[TPB / WAVE];
+    const WorkDesc w = D[blockIdx.x];
+    int* L = wl + w.off;
+    int* inst = L + 3;
+    int* start = L + 3 + w.cap;
+    int n0 = 0, tot0 = 0;
+    for (int base = 0; base < w.cap; base += TPB) {
+        const int c = base + threadIdx.x;
+        int i = -1, nb = 0;
+        if (c < w.cap) {
+            i = cand[w.cand + c];
+            const BInst<T>& b = I[i];
+            if (b.active) {
+                const int N = b.g.N, M = b.M, Mh = M / 2 + 1;
+                if (w.kind == WK_ROWS3 || w.kind == WK_ROWS4) {
+                    if (M == w.M) { const int B = fft_batch_of(M / 2); nb = (N + B - 1) / B * (w.kind == WK_ROWS3 ? 3 : 4); }
+                } else if (w.kind == WK_COLS) {
+                    if (M == w.M) { const int B = fft_batch_of(M); nb = (Mh + B - 1) / B; }
+                } else if (w.kind == WK_TR0) {
+                    nb = ((Mh + 31) / 32) * ((N + 31) / 32) * 3;
+                } else if (w.kind == WK_TR1) {
+                    nb = ((N + 31) / 32) * ((Mh + 31) / 32) * 4;
+                } else {
+                    nb = (int)((3ll * N * N + 4095) / 4096);
+                }
+            }
+        }
+        const int f = nb > 0 ? 1 : 0;
+        const int pos = block_inclusive_scan(f, sh) - f;
+        const int off = block_inclusive_scan(nb, sh);
+        if (f) {
+            inst[n0 + pos] = i;
+            start[n0 + pos + 1] = tot0 + off;
+        }
+        // the chunk's totals, from its last thread
+        if (threadIdx.x == TPB - 1) { sh[0] = pos + f; sh[1] = off; }
+        __syncthreads();
+        n0 += sh[0];
+        tot0 += sh[1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        start[0] = 0;
+        L[0] = tot0;
+        L[1] = n0;
+        L[2] = w.cap;
+    }
+}
+
+// The iteration's decision for every instance (one thread each): the grid of FFTRepulsion2D._grid
+// (magnet/tsne.py:30-46) from the bbox of the last update (mb[2..5]), its FFT size against the plan's window, the
+// spectra to recompute (the reference's cache key (M, width)), and the schedule's exaggeration and momentum
+// (magnet/tsne.py:176-181). An instance whose size falls outside the window, or whose layout is not finite, does not
+// run the iteration and keeps its state; the host sees the stall and plans again.
+template <class T>
+__global__ __launch_bounds__(TPB) void kb_decide(BInst<T>* __restrict__ I, BState* __restrict__ S, BSched q) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= q.count) return;
+    BInst<T>& b = I[i];
+    BState& s = S[i];
+    b.active = 0;
+    b.fresh = 0;
+    if (s.next_it >= q.n_iter) return;
+    const double bb[4] = {(double)b.mb[2], (double)b.mb[3], (double)b.mb[4], (double)b.mb[5]};
+    if (!(isfinite(bb[0]) && isfinite(bb[1]) && isfinite(bb[2]) && isfinite(bb[3]))) { s.stall = 2; return; }
+    const GridDecision gd = decide_grid(bb, q.box_width, q.min_boxes, q.max_boxes);
+    const GridT<T> g = make_grid<T>(gd.xmin, gd.ymin, gd.width, gd.nb, q.ni);
+    const int M = fft_size(g.N);
+    s.M = M;
+    if (M == 0 || M < b.Mlo || M > b.Mhi) { s.stall = 1; return; }
+    s.stall = 0;
+    const int it = s.next_it;
+    s.next_it = it + 1;
+    const bool early = it < q.early_iter;
+    b.rho = (T)(early ? q.rho0 : q.rho1);
+    b.mom = (T)(early ? q.mom0 : q.mom1);
+    const double delta = gd.width / q.ni;
+    if (delta == q.delta0 || b.Kw[0] == nullptr) {
+        // an unclipped grid (width = box_width): the batch's shared spectra of size M
+        int w = 0;
+        for (int j = 1; j < 4; ++j) if (b.KM[j] == M) w = j;
+        for (int t = 0; t < 4; ++t) b.K[t] = b.KS[w][t];
+    } else {
+        // a clipped grid at the largest size: the instance's own tables, recomputed when the width changed
+        // (the reference's cache key (M, width))
+        if (!(s.sM == M && s.sdelta == delta)) { b.fresh = 1; s.sM = M; s.sdelta = delta; }
+        for (int t = 0; t < 4; ++t) b.K[t] = b.Kw[t];
+    }
+    b.g = g;
+    b.M = M;
+    b.delta = delta;
+    b.sumq = b.sumq_base + it;
+    b.active = 1;
 }
 
 }  // namespace
@@ -617,6 +761,23 @@ int launch_key_sort_masked(int* keysA, int* valsA, int* keysB, int* valsB, int* 
     require(bits >= 1 && bits <= 31 && keep_bits >= 1 && keep_bits <= bits, "bad key widths");
     const int kmask = keep_bits >= 31 ? -1 : (int)((1u << keep_bits) - 1u);
     return sort_keys_any<float>(nullptr, nullptr, keysA, valsA, n, bits, keysA, valsA, keysB, valsB, hist, st, kmask);
+}
+
+template <class T> void launch_fill_wl(const BInst<T>* I, const int* wl, int grid, hipStream_t st) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(kw_fill_w<T>, dim3(grid), dim3(TPB), 0, st, I, wl);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_worklists(const BInst<T>* I, const WorkDesc* d, int nd, const int* cand, int* wl, hipStream_t st) {
+    if (nd <= 0) return;
+    hipLaunchKernelGGL(kb_worklists<T>, dim3((unsigned)nd), dim3(TPB), 0, st, I, d, cand, wl);
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_decide_batch(BInst<T>* I, BState* S, const BSched& q, hipStream_t st) {
+    hipLaunchKernelGGL(kb_decide<T>, dim3((unsigned)((q.count + TPB - 1) / TPB)), dim3(TPB), 0, st, I, S, q);
+    MG_HIP(hipGetLastError());
 }
 
 template <class T> void launch_keys_batch(const BInst<T>* I, const BMap& m, const T* Y, int box_bits, int* keys, int* vals,
@@ -851,6 +1012,9 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
     template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, const double*, hipStream_t); \
     template void launch_centroid<T>(const T*, i64, double*, hipStream_t); \
     template void launch_keys_batch<T>(const BInst<T>*, const BMap&, const T*, int, int*, int*, hipStream_t); \
+    template void launch_fill_wl<T>(const BInst<T>*, const int*, int, hipStream_t); \
+    template void launch_worklists<T>(const BInst<T>*, const WorkDesc*, int, const int*, int*, hipStream_t); \
+    template void launch_decide_batch<T>(BInst<T>*, BState*, const BSched&, hipStream_t); \
     template void launch_spread_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, hipStream_t); \
     template void launch_gather_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, T*, hipStream_t);
 MG_INST(float)

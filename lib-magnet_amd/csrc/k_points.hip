@@ -346,15 +346,16 @@ __global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const
     center2_block<T>(Y, n, part, mb, blockIdx.x, gridDim.x);
 }
 
-// batched fused step of the t-SNE loop (gradient rho * attr - F / sumQ of each instance) and its recentring
+// batched fused step of the t-SNE loop (gradient rho * attr - F / sumQ of each active instance, its iteration's
+// exaggeration and momentum from the table) and its recentring
 template <class T>
 __global__ __launch_bounds__(TPB) void kb_update2(const BInst<T>* __restrict__ I, BMap m, T* __restrict__ Y, T* __restrict__ U,
-                                                  T* __restrict__ Gn, const T* __restrict__ A, const T* __restrict__ F,
-                                                  T rho, T mom) {
+                                                  T* __restrict__ Gn, const T* __restrict__ A, const T* __restrict__ F) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active) return;
     const i64 o = 2 * b.off, n = b.n;
-    const T lr = b.lr;
+    const T lr = b.lr, rho = b.rho, mom = b.mom;
     T* inv = b.inv_out;
     double* part = b.upart;
     update2_block<T, 0>(Y + o, U + o, Gn + o, A + o, F + o, inv, T(0), rho, lr, mom, n, part,
@@ -364,6 +365,7 @@ template <class T>
 __global__ __launch_bounds__(TPB) void kb_center2(const BInst<T>* __restrict__ I, BMap m, T* __restrict__ Y) {
     const int k = bmap_find(m, blockIdx.x);
     const BInst<T>& b = I[m.inst[k]];
+    if (!b.active) return;
     const i64 off = b.off, n = b.n;
     const double* part = b.upart;
     T* mb = b.mb;
@@ -552,7 +554,7 @@ int attraction_lanes(i64 n, i64 nnz) {
 template <class T> constexpr int attr_window() { return (128 * 1024) / (2 * (int)sizeof(T)); }
 
 template <class F> static void set_dyn_lds(F* f, int bytes) {
-    if (bytes > 65536) MG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (bytes > 65536) kernel_lds_limit(reinterpret_cast<const void*>(f), bytes);
 }
 
 template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, const T* P, const T* Y, i64 n, int dim,
@@ -607,10 +609,10 @@ template <class T> void launch_attraction_batch(const BInst<T>* I, const BMap& m
 }
 
 template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T* Y, T* U, T* G, const T* A, const T* F,
-                                            T rho, T mom, hipStream_t st) {
+                                            hipStream_t st) {
     if (m.count == 0 || m.start[m.count] == 0) return;
     const dim3 grid((unsigned)m.start[m.count]);
-    hipLaunchKernelGGL(kb_update2<T>, grid, dim3(TPB), 0, st, I, m, Y, U, G, A, F, rho, mom);
+    hipLaunchKernelGGL(kb_update2<T>, grid, dim3(TPB), 0, st, I, m, Y, U, G, A, F);
     MG_HIP(hipGetLastError());
     hipLaunchKernelGGL(kb_center2<T>, grid, dim3(TPB), 0, st, I, m, Y);
     MG_HIP(hipGetLastError());
@@ -694,7 +696,7 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
     template void launch_trace_plogp<T>(const T*, i64, double*, hipStream_t); \
     template void launch_trace<T>(const int32_t*, const int32_t*, const T*, const T*, const int*, i64, double, const T*, const T*, const T*, T, const double*, double*, double*, hipStream_t); \
     template void launch_attraction_batch<T>(const BInst<T>*, const BMap&, int, const T*, T, T*, hipStream_t); \
-    template void launch_update_batch<T>(const BInst<T>*, const BMap&, T*, T*, T*, const T*, const T*, T, T, hipStream_t);
+    template void launch_update_batch<T>(const BInst<T>*, const BMap&, T*, T*, T*, const T*, const T*, hipStream_t);
 MG_INST(float)
 MG_INST(double)
 template void launch_fill<int>(int*, int, i64, hipStream_t);

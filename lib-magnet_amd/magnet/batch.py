@@ -1,12 +1,14 @@
 """Batched form of the t-SNE pass: many independent embeddings of uneven sizes on one schedule.
 
 optimize_tsne_batch runs every instance's full-batch t-SNE loop (the loop of optimize_tsne, magnet/tsne.py:132-214)
-through one C-ABI call, mg_tsne_optimize_batch. Instances are spread over a few HIP streams and advance together,
-issued by a few host threads, so their kernels overlap on the device and the launches of different instances are
-issued in parallel; each instance's result is the one optimize_tsne gives it alone (same kernels, same grids, same
-order of operations). The stream and thread counts are picked by autotune_streams: short timed runs of each
-candidate on copies of the states, the fastest kept; then the launch shapes that change no result (magnet.tune: the
-radix sort's keys per thread) are swept the same way.
+through one C-ABI call, mg_tsne_optimize_batch. Every stage of an iteration is one launch for all instances; the
+iteration is device-driven (a kernel decides each instance's grid from its last bounding box), so its launches are
+captured once in a hipGraph and replayed, and the host only reads the instances' states every few iterations to
+plan again when one outgrows its window of FFT sizes. Each instance's result is the one optimize_tsne gives it alone
+(same kernels, same grids, same order of operations). The stream and thread counts are picked by autotune_streams:
+short timed runs of each candidate on copies of the states, the fastest kept; then the launch choices that change
+no result (magnet.tune: the radix sort's keys per thread; graph replay or launches one by one) are swept the same
+way.
 
 The reference fits one embedding per call; this form has no counterpart there.
 """
@@ -72,7 +74,7 @@ def autotune_streams(instances, candidates=CANDIDATES, iters=12, early_exaggerat
         trial = [(ip, ix, dv, Y.clone(), U.clone(), G.clone()) for ip, ix, dv, Y, U, G in instances]
         capi.optimize_batch(trial, n_iter=iters, early_iter=0, early_exaggeration=early_exaggeration, n_streams=best[0],
                             n_threads=best[1], **kw)
-    chosen, log = autotune(run, (("sort_items", (0, 2, 4, 8)),), repeats=2)
+    chosen, log = autotune(run, (("sort_items", (0, 2, 4, 8)), ("batch_graph", (1, 0))), repeats=2)
     LAST_SWEEP.extend(log)
     _TUNED[key] = best
     return best

@@ -89,6 +89,7 @@ def lib():
             "mg_tsne_session_trace": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
             "mg_optimize_layout": ([C.POINTER(LayoutParams), vp, i64, i32, vp, vp, vp, vp, vp, i64, vp, i32, vp, vp], i32),
             "mg_tsne_optimize_batch": ([i32, C.POINTER(Params), i32, C.POINTER(Instance), i32, i32], i32),
+            "mg_tsne_batch_stats": ([C.POINTER(C.c_int64)] * 6 + [C.POINTER(C.c_double)], i32),
             "mg_topk_rows": ([vp, i64, i64, i32, vp, vp], i32),
             "mg_knn": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
             "mg_knn_noalloc": ([vp, i64, vp, i64, i32, i32, i64, vp, vp], i32),
@@ -294,6 +295,16 @@ def optimize_batch(instances, n_iter=750, early_iter=250, early_exaggeration=12.
                           U.data_ptr(), G.data_ptr(), sq[b].ctypes.data_as(C.POINTER(C.c_double)) if sumq else None)
     check(lib().mg_tsne_optimize_batch(prec, C.byref(p), len(instances), arr, int(n_streams), int(n_threads)))
     return [s[:n_iter] for s in sq] if sumq else None
+
+
+def batch_stats():
+    """{plans, iterations, replays, launches, checks, stalls, plan_s} of the last optimize_batch call
+    (mg_tsne_batch_stats)."""
+    v = [C.c_int64() for _ in range(6)]
+    ps = C.c_double()
+    check(lib().mg_tsne_batch_stats(*[C.byref(x) for x in v], C.byref(ps)))
+    names = ("plans", "iterations", "replays", "launches", "checks", "stalls")
+    return dict({k: x.value for k, x in zip(names, v)}, plan_s=ps.value)
 
 
 def optimize_layout(Y, head, tail, eps, eons, eonns, epoch_start, epoch_end, alpha0, alpha1, mid_w0=0.0, mid_w1=0.0,

@@ -123,3 +123,35 @@ def test_more_than_64_instances_and_mixed_fft_sizes():
     for b, (ip, ix, dv, Y, U, G) in enumerate(device_instances(Ps, Ys, "fp32")):
         capi.optimize(ip, ix, dv, Y, U, G, 0, kw["n_iter"], **kw)
         assert np.array_equal(host(batch[b][3]), host(Y)), b
+
+
+@pytest.mark.parametrize("graph,spread", [(1, False), (0, False), (1, True), (0, True)])
+def test_device_driven_batch_with_stalls_and_replans(graph, spread):
+    """The batch decides every grid on the device and replays its captured iteration (batch_graph 1) or issues the
+    same launches one by one (0). Reading the states every 2 iterations, with windows that hold no smaller FFT size
+    (every shrink stalls its instance until the next plan), many plans and stalls happen; each instance still equals
+    its single run bit for bit, and the counters add up: every schedule iteration issued, stalls only as extra ones."""
+    from magnet.batch import device_instances
+    probs = cached()
+    Ps = [p for p, _ in probs]
+    # spread: starting layouts of span about 200 that contract under the exaggerated attraction, so grids shrink
+    Ys = [np.random.default_rng(7 + b).normal(0, 30, y.shape) if spread else y for b, (_, y) in enumerate(probs)]
+    old = {k: capi.tune(k) for k in ("batch_graph", "batch_chunk", "batch_window_down")}
+    try:
+        capi.tune("batch_graph", graph); capi.tune("batch_chunk", 2); capi.tune("batch_window_down", 0)
+        batch = device_instances(Ps, Ys, "fp32")
+        capi.optimize_batch(batch, n_streams=1, n_threads=1, **SCHED)
+        st = capi.batch_stats()
+    finally:
+        for k, v in old.items():
+            capi.tune(k, v)
+    print(st)
+    assert st["plans"] >= 2 and st["checks"] >= SCHED["n_iter"] // 2
+    assert st["iterations"] >= SCHED["n_iter"] and st["replays"] == (st["iterations"] if graph else 0)
+    assert st["launches"] > 0
+    if spread:
+        assert st["stalls"] > 0 and st["iterations"] > SCHED["n_iter"]
+    for b, (ip, ix, dv, Y, U, G) in enumerate(device_instances(Ps, Ys, "fp32")):
+        capi.optimize(ip, ix, dv, Y, U, G, 0, SCHED["n_iter"], **SCHED)
+        for x, y in zip(batch[b][3:], (Y, U, G)):
+            assert np.array_equal(host(x), host(y)), (b, SIZES[b])
