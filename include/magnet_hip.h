@@ -57,9 +57,14 @@ int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
  *   "attr_tpb"     its threads per block: 1024 (default), 512 or 256;
  *   "sort_items"   keys per thread of the radix sorts of up to 2^20 keys: 1, 2, 4 or 8, or 0 (default: 2 up to
  *                  262,144 keys, 8 above);
+ *   "sort_digit"   digit bits of the radix sorts of keys of 9 to 16 bits (two passes either way): 8 (default) or 10;
  *   "batch_graph"  1 (default): mg_tsne_optimize_batch replays its captured iteration (a hipGraph); 0: it issues the
  *                  same launches one by one;
- *   "batch_chunk"  iterations mg_tsne_optimize_batch queues between two reads of the instances' states (default 8).
+ *   "batch_chunk"  iterations mg_tsne_optimize_batch queues between two reads of the instances' states (default 16);
+ *   "batch_window_down", "batch_window_up"  FFT sizes below (0 or 1, default 1) and above (0 to 2, default 2) the
+ *                  current one in each instance's window (a grid that leaves its window stalls its instance until the
+ *                  next plan);
+ *   "batch_wl_grid" most blocks of a batched grid-stage launch over its work list (default 2048).
  * MG_ERR_ARG for an unknown name or a value out of range. magnet.tune.autotune picks them by timing. */
 int mg_tune_set(const char* name, int value);
 int mg_tune_get(const char* name, int* value);

@@ -150,9 +150,11 @@ struct Tunables {
     int attr_win_kb = 64;   // LDS window of the sweep: 128 or 64 KiB of positions (0: no window, rows only)
     int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
     int sort_items = 0;     // keys per thread of a radix-sort pass up to 2^20 keys: 1, 2, 4 or 8 (0: 2 up to 262,144 keys, else 8)
+    int sort_digit = 8;     // digit bits of the radix sorts of keys of 9-16 bits: 8 (two passes either way), or 10
     int batch_graph = 1;    // the batched loop replays its iteration as a captured hipGraph (0: issues the launches)
     int batch_chunk = 16;   // iterations the batched loop queues between two looks at the instances' states
     int batch_window_down = 1;   // the batched plan's window of FFT sizes holds the size below the current one
+    int batch_window_up = 2;     // and this many sizes above it
     int batch_wl_grid = 2048;    // most blocks of a batched grid-stage launch over its work list
 };
 Tunables& tunables();
@@ -197,8 +199,10 @@ template <class T> void launch_grad(const T* A, const T* F, const T* d_inv, T in
                                     int m, T w, i64 n, int dim, int combine, T* grad, hipStream_t st);
 template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st);
 
-// box sort: returns which buffer holds the result (0 = A, 1 = B)
-template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st);
+// box sort: returns which buffer holds the result (0 = A, 1 = B); inv != null: the layout is relabelled (position
+// inv[i] holds original point i) and the points are taken in original order, values their positions
+template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st,
+                                       const int* inv = nullptr);
 int sort_hist_blocks(i64 n);
 // ints of a radix sort's digit-row scratch for n keys, enough for any sort_items setting
 i64 sort_hist_ints(i64 n);
@@ -256,8 +260,6 @@ template <class T> void launch_relabel(const int* vals, const int* perm, i64 n, 
                                        hipStream_t st);
 template <class T> void launch_perm_state(const T* Y, const T* U, const T* G, const int* inv_old, const int* order, i64 n,
                                           T* Yn, T* Un, T* Gn, hipStream_t st);
-template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
-                                         hipStream_t st);
 // sampled layout optimisation (k_layout.hip): epochs [epoch_start, epoch_end) of the owner-mode SGD on Y (n, 2) in
 // place; scratch of layout_scratch_bytes(n)
 size_t layout_scratch_bytes(i64 n);

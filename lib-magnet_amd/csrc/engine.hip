@@ -361,14 +361,7 @@ template <class T> struct Repulsion {
         if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
-        if (inv) {
-            int bits = 0;
-            while ((1ll << bits) < (long long)g.nb * g.nb) ++bits;
-            launch_perm_keys<T>(Y, inv, n, g, b.keysA, b.valsA, st);
-            sorted_buf = launch_key_sort<T>(b.keysA, b.valsA, b.keysB, b.valsB, b.hist, n, std::max(bits, 1), st);
-        } else {
-            sorted_buf = launch_box_sort<T>(Y, n, g, b, st);
-        }
+        sorted_buf = launch_box_sort<T>(Y, n, g, b, st, inv);
         P.end(ST_SORT, st, 2 * dn * ts + 8 * dn);
         const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
         const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
@@ -773,7 +766,7 @@ template <class T> struct BatchEngine {
     }
     int window_hi(int M) const {
         int hi = M;
-        for (int k = 0; k < 2; ++k) { const int nx = fft_size_next(hi); if (nx > 0 && nx <= Mmax) hi = nx; }
+        for (int k = 0; k < tunables().batch_window_up; ++k) { const int nx = fft_size_next(hi); if (nx > 0 && nx <= Mmax) hi = nx; }
         return hi;
     }
 
@@ -1329,9 +1322,11 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "attr_win_kb") { require(value == 0 || value == 64 || value == 128, "attr_win_kb must be 0, 64 or 128"); g_tunables.attr_win_kb = value; }
     else if (k == "attr_tpb") { require(value == 256 || value == 512 || value == 1024, "attr_tpb must be 256, 512 or 1024"); g_tunables.attr_tpb = value; }
     else if (k == "sort_items") { require(value == 0 || value == 1 || value == 2 || value == 4 || value == 8, "sort_items must be 0, 1, 2, 4 or 8"); g_tunables.sort_items = value; }
+    else if (k == "sort_digit") { require(value == 8 || value == 10, "sort_digit must be 8 or 10"); g_tunables.sort_digit = value; }
     else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
     else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
     else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }
+    else if (k == "batch_window_up") { require(value >= 0 && value <= 2, "batch_window_up must be 0, 1 or 2"); g_tunables.batch_window_up = value; }
     else if (k == "batch_wl_grid") { require(value >= 64 && value <= 65536, "batch_wl_grid must be in [64, 65536]"); g_tunables.batch_wl_grid = value; }
     else require(false, "unknown tunable: " + k);
     MG_API_END
@@ -1346,9 +1341,11 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "attr_win_kb") *value = g_tunables.attr_win_kb;
     else if (k == "attr_tpb") *value = g_tunables.attr_tpb;
     else if (k == "sort_items") *value = g_tunables.sort_items;
+    else if (k == "sort_digit") *value = g_tunables.sort_digit;
     else if (k == "batch_graph") *value = g_tunables.batch_graph;
     else if (k == "batch_chunk") *value = g_tunables.batch_chunk;
     else if (k == "batch_window_down") *value = g_tunables.batch_window_down;
+    else if (k == "batch_window_up") *value = g_tunables.batch_window_up;
     else if (k == "batch_wl_grid") *value = g_tunables.batch_wl_grid;
     else require(false, "unknown tunable: " + k);
     MG_API_END

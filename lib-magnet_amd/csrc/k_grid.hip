@@ -57,9 +57,11 @@ template <int NI> __device__ inline double node_offset(double f, double width, i
 // for up to 1024 boxes per side). Per pass: per-block digit counts, hist[d][block]; an exclusive scan of each
 // digit's row (one workgroup per digit); the stable scatter, whose blocks turn the digit totals into digit
 // bases themselves.
+// FIRST: keys from the points (the box of point src = inv ? inv[i] : i, value src), else keys_in
 template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
-                                                 i64 n, int shift, int* __restrict__ hist, int nblocks, int items) {
+                                                 i64 n, int shift, int* __restrict__ hist, int nblocks, int items,
+                                                 const int* __restrict__ inv) {
     constexpr int NBK = 1 << DB;
     __shared__ int h[NBK];
     for (int d = threadIdx.x; d < NBK; d += TPB) h[d] = 0;
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(TPB) void k_rs_hist(const T* __restrict__ Y, GridT<
         const i64 i = base + r * TPB + threadIdx.x;
         if (i < n) {
             int key;
-            if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); }
+            if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[inv ? (i64)inv[i] : i]; key = box_key<T>(p.x, p.y, g); }
             else key = keys_in[i];
             atomicAdd(&h[(key >> shift) & (NBK - 1)], 1);
         }
@@ -118,7 +120,8 @@ template <class T, bool FIRST, int DB>
 __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, GridT<T> g, const int* __restrict__ keys_in,
                                                     const int* __restrict__ vals_in, i64 n, int shift,
                                                     const int* __restrict__ hist, const int* __restrict__ tot, int nblocks,
-                                                    int* __restrict__ keys_out, int* __restrict__ vals_out, int kmask, int items) {
+                                                    int* __restrict__ keys_out, int* __restrict__ vals_out, int kmask, int items,
+                                                    const int* __restrict__ inv) {
     constexpr int NBK = 1 << DB, PER = NBK / TPB > 0 ? NBK / TPB : 1;
     __shared__ int off[NBK];
     __shared__ int wcnt[TPB / WAVE][NBK];
@@ -141,7 +144,11 @@ __global__ __launch_bounds__(TPB) void k_rs_scatter(const T* __restrict__ Y, Gri
         const bool valid = i < n;
         int key = 0, val = 0;
         if (valid) {
-            if (FIRST) { const V p = reinterpret_cast<const V*>(Y)[i]; key = box_key<T>(p.x, p.y, g); val = (int)i; }
+            if (FIRST) {
+                const i64 src = inv ? (i64)inv[i] : i;
+                const V p = reinterpret_cast<const V*>(Y)[src];
+                key = box_key<T>(p.x, p.y, g); val = (int)src;
+            }
             else { key = keys_in[i]; val = vals_in[i]; }
         }
         const int d = (key >> shift) & (NBK - 1);
@@ -710,7 +717,7 @@ int gather_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1024 ? 
 // keys come from Y and 1 when they are precomputed in A; returns the buffer holding the result.
 template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int* keysA,
                                               int* valsA, int* keysB, int* valsB, int* hist, int passes, hipStream_t st,
-                                              int kmask = -1) {
+                                              int kmask = -1, const int* inv = nullptr) {
     const int nblocks = sort_hist_blocks(n), items = rs_items(n);
     int* tot = hist + (size_t)(1 << DB) * nblocks;
     const int first = keys0 ? 1 : 0;
@@ -723,14 +730,14 @@ template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, con
         int* kout = cur == 0 ? keysA : keysB;
         int* vout = cur == 0 ? valsA : valsB;
         const bool fromY = p == 0 && !keys0;
-        if (fromY) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items);
-        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items);
+        if (fromY) hipLaunchKernelGGL((k_rs_hist<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items, inv);
+        else hipLaunchKernelGGL((k_rs_hist<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, n, shift, hist, nblocks, items, (const int*)nullptr);
         MG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_rs_rowscan, dim3(1 << DB), dim3(TPB), 0, st, hist, nblocks, tot);
         MG_HIP(hipGetLastError());
         const int mk = p == passes - 1 ? kmask : -1;
-        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items);
-        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items);
+        if (fromY) hipLaunchKernelGGL((k_rs_scatter<T, true, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items, inv);
+        else hipLaunchKernelGGL((k_rs_scatter<T, false, DB>), dim3(nblocks), dim3(TPB), 0, st, Y, gg, kin, vin, n, shift, hist, tot, nblocks, kout, vout, mk, items, (const int*)nullptr);
         MG_HIP(hipGetLastError());
         kin = kout; vin = vout;
     }
@@ -738,17 +745,20 @@ template <class T, int DB> static int sort_db(const T* Y, const GridT<T>* g, con
 }
 
 template <class T> static int sort_keys_any(const T* Y, const GridT<T>* g, const int* keys0, const int* vals0, i64 n, int bits,
-                                            int* kA, int* vA, int* kB, int* vB, int* hist, hipStream_t st, int kmask = -1) {
-    if (bits <= 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 1, st, kmask);
-    if (bits <= 30) return sort_db<T, 10>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 9) / 10, st, kmask);
-    return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 7) / 8, st, kmask);
+                                            int* kA, int* vA, int* kB, int* vB, int* hist, hipStream_t st, int kmask = -1,
+                                            const int* inv = nullptr) {
+    if (bits <= 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 1, st, kmask, inv);
+    // two passes either way up to 16 bits: 8-bit digits keep the digit table and its scan four times smaller
+    if (bits <= 16 && tunables().sort_digit == 8) return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, 2, st, kmask, inv);
+    if (bits <= 30) return sort_db<T, 10>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 9) / 10, st, kmask, inv);
+    return sort_db<T, 8>(Y, g, keys0, vals0, n, kA, vA, kB, vB, hist, (bits + 7) / 8, st, kmask, inv);
 }
 
-template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st) {
+template <class T> int launch_box_sort(const T* Y, i64 n, const GridT<T>& g, RepBuffers<T>& b, hipStream_t st, const int* inv) {
     const long long nkeys = (long long)g.nb * g.nb;
     int bits = 0;
     while ((1ll << bits) < nkeys) ++bits;
-    return sort_keys_any<T>(Y, &g, nullptr, nullptr, n, bits, b.keysA, b.valsA, b.keysB, b.valsB, b.hist, st);
+    return sort_keys_any<T>(Y, &g, nullptr, nullptr, n, bits, b.keysA, b.valsA, b.keysB, b.valsB, b.hist, st, -1, inv);
 }
 
 // stable sort of precomputed keys (< 2^bits) with their values, both in A; hist as for the box sort
@@ -1005,7 +1015,7 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
 }
 
 #define MG_INST(T) \
-    template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t); \
+    template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t, const int*); \
     template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
     template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
     template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t); \

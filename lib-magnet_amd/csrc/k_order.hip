@@ -110,18 +110,6 @@ __global__ __launch_bounds__(TPB) void k_perm_state(const T* __restrict__ Y, con
 }
 
 // sort input in original order: key of original point i, value its current position
-template <class T>
-__global__ __launch_bounds__(TPB) void k_perm_keys(const T* __restrict__ Y, const int* __restrict__ inv, i64 n, GridT<T> g,
-                                                   int* __restrict__ keys, int* __restrict__ vals) {
-    typedef typename vec2<T>::type V;
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
-        const int p = inv[i];
-        const V y = reinterpret_cast<const V*>(Y)[p];
-        keys[i] = box_key<T>(y.x, y.y, g);
-        vals[i] = p;
-    }
-}
-
 // Morton (Z-order) index of the box (bx, by) of a point on grid g, clamped to the grid: the bits of bx and by
 // interleaved, so that boxes close in the plane are mostly close in the order
 template <class T> __device__ inline int morton_key(T x, T y, const GridT<T>& g) {
@@ -192,12 +180,6 @@ template <class T> void launch_perm_state(const T* Y, const T* U, const T* G, co
     MG_HIP(hipGetLastError());
 }
 
-template <class T> void launch_perm_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
-                                         hipStream_t st) {
-    hipLaunchKernelGGL(k_perm_keys<T>, dim3(blocks_for(n)), dim3(TPB), 0, st, Y, inv, n, g, keys, vals);
-    MG_HIP(hipGetLastError());
-}
-
 template <class T> void launch_morton_keys(const T* Y, const int* inv, i64 n, const GridT<T>& g, int* keys, int* vals,
                                            hipStream_t st) {
     require(g.nb >= 1 && g.nb <= (1 << 15), "grid too large for a Morton key");
@@ -213,7 +195,6 @@ template <class T> void launch_unperm(const T* src, const int* inv, i64 n, T* ds
 #define MG_INST(T) \
     template void launch_relabel<T>(const int*, const int*, i64, const int32_t*, const int32_t*, const T*, int*, int*, int*, int*, int32_t*, int32_t*, T*, hipStream_t); \
     template void launch_perm_state<T>(const T*, const T*, const T*, const int*, const int*, i64, T*, T*, T*, hipStream_t); \
-    template void launch_perm_keys<T>(const T*, const int*, i64, const GridT<T>&, int*, int*, hipStream_t); \
     template void launch_morton_keys<T>(const T*, const int*, i64, const GridT<T>&, int*, int*, hipStream_t); \
     template void launch_unperm<T>(const T*, const int*, i64, T*, hipStream_t);
 MG_INST(float)

@@ -292,6 +292,40 @@ __global__ __launch_bounds__(TPB) void k_update2(T* __restrict__ Y, T* __restric
     update2_block<T, MODE>(Y, U, Gn, A, F, d_inv, inv_h, rho, lr, mom, n, part, blockIdx.x, gridDim.x);
 }
 
+// The fused update of a relabelled layout (position inv[i] holds original point i) whose points all sit in L2: block b
+// updates the points b * TPB + t (+ k * nblk * TPB) of the original numbering, where they now are, and takes the mean
+// and bbox partials of the updated positions in that order -- the partials k_meanbox_perm takes after k_update2, in one
+// launch. Each point's update is its own, so the order changes nothing in it.
+template <class T, int MODE>
+__global__ __launch_bounds__(TPB) void k_update2_perm(T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Gn,
+                                                      const T* __restrict__ A, const T* __restrict__ F,
+                                                      const T* __restrict__ d_inv, T inv_h, T rho, T lr, T mom, i64 n,
+                                                      const int* __restrict__ inv, double* __restrict__ part) {
+    typedef typename vec2<T>::type V;
+    const T invq = d_inv ? *d_inv : inv_h;
+    const int bid = blockIdx.x, nblk = gridDim.x;
+    CSum sx = {0.0, 0.0}, sy = {0.0, 0.0};
+    T x0 = pinf<T>(), y0 = pinf<T>(), x1 = -pinf<T>(), y1 = -pinf<T>();
+    V* Y2 = reinterpret_cast<V*>(Y); V* U2 = reinterpret_cast<V*>(U); V* G2 = reinterpret_cast<V*>(Gn);
+    const V* A2 = reinterpret_cast<const V*>(A); const V* F2 = reinterpret_cast<const V*>(F);
+    for (i64 i = (i64)bid * TPB + threadIdx.x; i < n; i += (i64)nblk * TPB) {
+        const i64 q = inv[i];
+        const V a = A2[q];
+        V gx;
+        if (MODE == 0) { const V f = F2[q]; gx.x = rho * a.x - f.x * invq; gx.y = rho * a.y - f.y * invq; }
+        else gx = a;
+        V u = U2[q], g = G2[q], y = Y2[q];
+        y.x = y.x + dbd(gx.x, u.x, g.x, lr, mom);
+        y.y = y.y + dbd(gx.y, u.y, g.y, lr, mom);
+        U2[q] = u; G2[q] = g; Y2[q] = y;
+        csum_add(sx, (double)y.x); csum_add(sy, (double)y.y);
+        x0 = y.x < x0 ? y.x : x0; x1 = y.x > x1 ? y.x : x1;
+        y0 = y.y < y0 ? y.y : y0; y1 = y.y > y1 ? y.y : y1;
+    }
+    const MeanBox<T> m = block_meanbox<T>({sx, sy, x0, y0, x1, y1});
+    if (threadIdx.x == 0) write_meanbox<T>(part + MB_PART * bid, m);
+}
+
 // The update's block partials of a relabelled layout, in original point order: block b sums the points
 // b * TPB + t (+ k * nblk * TPB) of the original numbering, as the update of the unrelabelled layout does.
 template <class T>
@@ -625,13 +659,19 @@ template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, co
     if (n == 0) return;
     if (dim == 2) {
         const unsigned blocks = grid_for(n, 1024);
-        double* upart = inv ? nullptr : part;
-        if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
-        else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
-        MG_HIP(hipGetLastError());
-        if (inv) {
-            hipLaunchKernelGGL(k_meanbox_perm<T>, dim3(blocks), dim3(TPB), 0, st, (const T*)Y, inv, n, part);
+        if (inv && n < 262144) {   // the layout sits in L2: the update in original order, partials in one launch
+            if (mode == 0) hipLaunchKernelGGL((k_update2_perm<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, inv, part);
+            else hipLaunchKernelGGL((k_update2_perm<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, inv, part);
             MG_HIP(hipGetLastError());
+        } else {
+            double* upart = inv ? nullptr : part;
+            if (mode == 0) hipLaunchKernelGGL((k_update2<T, 0>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
+            else hipLaunchKernelGGL((k_update2<T, 1>), dim3(blocks), dim3(TPB), 0, st, Y, U, G, A, F, d_inv, inv_h, rho, lr, mom, n, upart);
+            MG_HIP(hipGetLastError());
+            if (inv) {
+                hipLaunchKernelGGL(k_meanbox_perm<T>, dim3(blocks), dim3(TPB), 0, st, (const T*)Y, inv, n, part);
+                MG_HIP(hipGetLastError());
+            }
         }
         hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, (const double*)part, mean_bbox);
     } else {

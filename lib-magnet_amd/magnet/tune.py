@@ -1,4 +1,5 @@
-"""Autotuner of the launch shapes that change no result (mg_tune_set): the radix sorts' keys per thread and the
+"""Autotuner of the launch shapes that change no result (mg_tune_set): the radix sorts' keys per thread and digit
+width, and the
 2-D attraction's LDS-window sweep (on or off, rows per block, window size, threads per block).
 
 autotune(run, space) times `run` (a short stretch of real work on copies of the state) for each candidate value of
@@ -12,6 +13,7 @@ __all__ = ["SPACE", "autotune", "autotune_loop"]
 # candidates per tunable, in sweep order (the first value of each is its default)
 SPACE = (
     ("sort_items", (0, 1, 2, 4, 8)),
+    ("sort_digit", (8, 10)),
     ("attr_window", (-1, 0, 1)),
     ("attr_rows", (256, 64, 128, 512)),
     ("attr_win_kb", (64, 128, 0)),
