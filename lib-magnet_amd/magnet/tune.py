@@ -1,6 +1,5 @@
 """Autotuner of the launch shapes that change no result (mg_tune_set): the radix sorts' keys per thread and digit
-width, and the
-2-D attraction's LDS-window sweep (on or off, rows per block, window size, threads per block).
+width, and the 2-D attraction's LDS-window sweep (on or off, rows per block, window size, threads per block).
 
 autotune(run, space) times `run` (a short stretch of real work on copies of the state) for each candidate value of
 each tunable in turn, keeps the fastest, and leaves it set: a coordinate sweep, one tunable at a time, in the order
