@@ -147,10 +147,12 @@ template <class T> void launch_columns_wl(const BInst<T>* I, const int* wl, int 
 template <class T> void launch_rows_inverse_wl(const BInst<T>* I, const int* wl, int M, int grid, hipStream_t st);
 // which: 0 = Tr (3, N, Mh) -> Wh (3, Mh, N) (list WK_TR0); 1 = Ph (4, Mh, N) -> Tr (4, N, Mh) (WK_TR1)
 template <class T> void launch_transpose_wl(const BInst<T>* I, const int* wl, int which, int grid, hipStream_t st);
-// spectra of the instances at size M whose table is fresh: stage 1, transpose (mt: transpose_blocks(Mh, Mh, ntab / 2)
-// per instance), stage 2
-template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, int M, double dof,
-                                             bool round_kernel, int ntab, hipStream_t st);
+// spectra of the instances at size M whose table is fresh: stage 1 (m1: fft_col_blocks(M)), transpose (mt:
+// transpose_blocks(Mh, Mh, 1)), stage 2 (ntab 2: m1; ntab 4, by pairs of columns: m2, spec_pair_blocks(M), then the
+// transposed table, mr: transpose_blocks(Mh, Mh, 1))
+template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, const BMap& m2, const BMap& mr,
+                                             int M, double dof, bool round_kernel, int ntab, hipStream_t st);
+int spec_pair_blocks(int M);
 int fft_row_blocks(int N, int M, int channels);
 int fft_col_blocks(int M);
 inline int transpose_blocks(int R, int C, int batch) { return ((C + 31) / 32) * ((R + 31) / 32) * batch; }

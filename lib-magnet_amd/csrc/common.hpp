@@ -287,11 +287,12 @@ template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int b
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
                                        hipStream_t st);
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st);   // X (4, N, Mh)
-// kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M), even
+// sym (ntab 4): the kernel is symmetric in x <-> y (K(a, b) = K(b, a)); y K2's table is then x K2's transposed
+// (fewer transforms). kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M), even
 // in each offset); ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx]
 // (round_kernel: kernel values rounded to float before the transform). tmp: 4 Mh^2 complex values.
 template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
-                                       cplx<T>* tmp, T* const* K, hipStream_t st);
+                                       cplx<T>* tmp, T* const* K, hipStream_t st, bool sym = false);
 template <class T> void launch_phi_interleave(const T* planar, T* out, i64 NN, hipStream_t st);   // (4,N,N) -> (N,N,4)
 
 }  // namespace mg

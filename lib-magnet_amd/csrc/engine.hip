@@ -186,6 +186,7 @@ template <class T> struct Repulsion {
     double kwidth = 0.7, kdof = 1.0;
     int M_in = 0;
     std::vector<double> kvals;   // (2, M_in, M_in)
+    bool kvals_sym = false;      // the value tables are symmetric in x <-> y
     // node spacing of a value table's grid: the centred product set's odd tables x K2, y K2 need the offsets in
     // layout units, which a value table does not carry (set from the grid of each compute; -1 when unused)
     double vdelta = -1.0;
@@ -290,7 +291,7 @@ template <class T> struct Repulsion {
             MG_HIP(hipStreamSynchronize(st));
             tab = d;
         }
-        launch_spectra<T>(M, delta, kdof, tab, tsne_kernel && sizeof(T) == 4, ntab, tmp, Kp, st);
+        launch_spectra<T>(M, delta, kdof, tab, tsne_kernel && sizeof(T) == 4, ntab, tmp, Kp, st, tsne_kernel || kvals_sym);
         return s;
     }
 
@@ -708,7 +709,7 @@ template <class T> struct BatchEngine {
     std::vector<int> sizes;   // the FFT sizes of the windows, ascending
     int box_bits = 1, sorted = 0, tcur = 0;
     struct Launches {
-        std::vector<BMap> spec1, spect;                 // clipped grids at the largest size
+        std::vector<BMap> spec1, spect, spec2, specr;   // clipped grids at the largest size
         std::vector<BMap> keys, spread, gather, update;
         // work lists: offsets of rows / columns / inverse rows per size, and of the transposes and the fill, in the
         // list buffer, with the fixed grid of each launch
@@ -903,7 +904,7 @@ template <class T> struct BatchEngine {
         T* Kp[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int t = 0; t < ntab; ++t) Kp[t] = reinterpret_cast<T*>(s.K[t].ensure(Mh * Mh * sizeof(T)));
         cplx<T>* tmp = reinterpret_cast<cplx<T>*>(spec_stage.ensure_keep(4 * Mh * Mh * sizeof(cplx<T>), dead));
-        launch_spectra<T>(M, sched.delta0, p->dof, nullptr, centred, ntab, tmp, Kp, st);
+        launch_spectra<T>(M, sched.delta0, p->dof, nullptr, centred, ntab, tmp, Kp, st, true);
         return s;
     }
 
@@ -970,7 +971,9 @@ template <class T> struct BatchEngine {
         L = Launches();
         const int Mhx = Mmax / 2 + 1;
         L.spec1 = make_maps(clip, [&](int) { return fft_col_blocks(Mmax); });
-        L.spect = make_maps(clip, [&](int) { return transpose_blocks(Mhx, Mhx, ntab / 2); });
+        L.spect = make_maps(clip, [&](int) { return transpose_blocks(Mhx, Mhx, 1); });
+        L.spec2 = make_maps(clip, [&](int) { return spec_pair_blocks(Mmax); });
+        L.specr = make_maps(clip, [&](int) { return transpose_blocks(Mhx, Mhx, 1); });
         // work lists: candidates and descriptors; a launch's grid is its most blocks (the largest grids of the
         // windows), at most wl_grid
         std::vector<WorkDesc> desc;
@@ -1042,7 +1045,7 @@ template <class T> struct BatchEngine {
         plan_s += now_s() - t0;
     }
     long long count_launches() const {
-        return 2 + (long long)attr_maps.size() + 3 * (long long)L.spec1.size() + (long long)L.keys.size()
+        return 2 + (long long)attr_maps.size() + (ntab == 4 ? 4 : 3) * (long long)L.spec1.size() + (long long)L.keys.size()
                + 3 * ((box_bits + bits_for(count) + 9) / 10) + 3 + 2 * L.spread.size() + 2 * L.gather.size()
                + 2 * L.update.size() + 3 * (long long)sizes.size();
     }
@@ -1061,7 +1064,7 @@ template <class T> struct BatchEngine {
         MG_HIP(hipEventRecord(ev_a1, aside));
         // kernel spectra of the clipped grids whose width changed
         for (size_t q = 0; q < L.spec1.size(); ++q)
-            launch_spectra_batch<T>(I, L.spec1[q], L.spect[q], Mmax, p->dof, centred, ntab, s);
+            launch_spectra_batch<T>(I, L.spec1[q], L.spect[q], L.spec2[q], L.specr[q], Mmax, p->dof, centred, ntab, s);
         // box sort of all points by (instance, box)
         for (const BMap& m : L.keys) launch_keys_batch<T>(I, m, dptr<T>(Yc), box_bits, dptr<int>(keysA), dptr<int>(valsA), s);
         sorted = launch_key_sort_masked(dptr<int>(keysA), dptr<int>(valsA), dptr<int>(keysB), dptr<int>(valsB), dptr<int>(hist),
@@ -1287,6 +1290,11 @@ template <class T> static void set_values(Repulsion<T>& r, int M_in, const doubl
     r.kvals.assign((size_t)2 * M_in * M_in, 0.0);
     std::memcpy(r.kvals.data(), K1, (size_t)M_in * M_in * sizeof(double));
     std::memcpy(r.kvals.data() + (size_t)M_in * M_in, K2, (size_t)M_in * M_in * sizeof(double));
+    bool sym = true;
+    for (int a = 0; a < M_in && sym; ++a)
+        for (int b = 0; b < a && sym; ++b)
+            sym = K1[(size_t)a * M_in + b] == K1[(size_t)b * M_in + a] && K2[(size_t)a * M_in + b] == K2[(size_t)b * M_in + a];
+    r.kvals_sym = sym;
     r.M_in = M_in; r.tsne_kernel = false; r.have_kernel = true; r.cache.clear();
 }
 

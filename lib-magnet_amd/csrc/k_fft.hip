@@ -104,5 +104,6 @@ void kernel_lds_limit(const void* f, int bytes) {
 static int batch_of(int L) { (void)size_index(L); return L / 16 >= 256 ? 1 : 256 / (L / 16); }
 int fft_row_blocks(int N, int M, int channels) { const int B = batch_of(M / 2); return (N + B - 1) / B * channels; }
 int fft_col_blocks(int M) { const int B = batch_of(M); return (M / 2 + 1 + B - 1) / B; }
+int spec_pair_blocks(int M) { const int B = batch_of(M), np = (M / 2 + 2) / 2; return (np + B - 1) / B; }
 
 }  // namespace mg

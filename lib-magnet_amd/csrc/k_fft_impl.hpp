@@ -440,6 +440,102 @@ __global__ __launch_bounds__(fft_block<L>()) void kb_spec2(const BInst<T>* __res
                           blockIdx.x - m.start[k]);
 }
 
+// Stage 2 of the centred set by pairs of columns, for kernels symmetric in x <-> y (the t-SNE kernel, radial tables):
+// y K2 is then x K2 transposed, so only K1, K2 and Tx are transformed here and Ty is Tx transposed afterwards. Per pair
+// (ky0 = 2p, ky1 = 2p + 1), FFT(T1 + i T2) of each column gives K1 and K2 as real and imaginary parts, as spec2_block
+// does; one FFT(X0 + i X1) of the two columns' X = x_a T2 (odd in a, so FFT(X) = i U) gives Tx of both: U0 = Im Z,
+// U1 = -Re Z. The two odd columns share a transform (like magnitudes); an even and an odd table never do.
+template <class T, int L>
+__device__ __forceinline__ void spec2p_block(const cplx<T>* __restrict__ Q0, double delta, T* __restrict__ K1,
+                                             T* __restrict__ K2, T* __restrict__ TX, const TwSet<T>& tws, int bx) {
+    using F = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), B = fft_batch<L>(), M = L, Mh = L / 2 + 1, RL = F::R[F::P - 1];
+    constexpr int QL = L / RL / TH;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    const int t = threadIdx.x % TH, bb = threadIdx.x / TH;
+    const int pr = bx * B + bb, ky0 = 2 * pr, ky1 = 2 * pr + 1;
+    const bool live0 = ky0 < Mh, live1 = ky1 < Mh;
+    const size_t row0 = (size_t)(live0 ? ky0 : 0) * Mh, row1 = (size_t)(live1 ? ky1 : 0) * Mh;
+    cplx<T> z[QL][RL];
+    auto put = [&](int, cplx<T> x, int q, int r) { z[q][r] = x; };
+    for (int c = 0; c < 2; ++c) {
+        const size_t row = c == 0 ? row0 : row1;
+        const bool live = c == 0 ? live0 : live1;
+        auto ld = [&](int e, int, int) -> cplx<T> { return Q0[row + (e <= M / 2 ? e : M - e)]; };
+        fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ld, put);
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < QL; ++q)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int k = t + q * TH + r * (L / RL);
+                    if (k >= Mh) continue;
+                    K1[row + k] = z[q][r].x;
+                    K2[row + k] = z[q][r].y;
+                }
+        }
+    }
+    auto ldx = [&](int e, int, int) -> cplx<T> {
+        const int a = e <= M / 2 ? e : M - e;
+        const T xa = a == M / 2 ? T(0) : (T)(a * delta);
+        const T s = e <= M / 2 ? xa : -xa;
+        return {s * Q0[row0 + a].y, s * Q0[row1 + a].y};
+    };
+    fft_run<T, F, -1, TH, FN>(t, lds, bb * L, tws.fwd, ldx, put);
+#pragma unroll
+    for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const int k = t + q * TH + r * (L / RL);
+            if (k >= Mh) continue;
+            if (live0) TX[row0 + k] = z[q][r].y;
+            if (live1) TX[row1 + k] = -z[q][r].x;
+        }
+}
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void k_spec2p(const cplx<T>* __restrict__ Q0, double delta, T* __restrict__ K1,
+                                                           T* __restrict__ K2, T* __restrict__ TX, TwSet<T> tws) {
+    spec2p_block<T, L>(Q0, delta, K1, K2, TX, tws, blockIdx.x);
+}
+template <class T, int L>
+__global__ __launch_bounds__(fft_block<L>()) void kb_spec2p(const BInst<T>* __restrict__ I, BMap m, TwSet<T> tws) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    if (!b.active || !b.fresh || b.M != L) return;
+    const size_t plane = (size_t)(L / 2 + 1) * (L / 2 + 1);
+    spec2p_block<T, L>(b.spec_tmp + 2 * plane, b.delta, b.Kw[0], b.Kw[1], b.Kw[2], tws, blockIdx.x - m.start[k]);
+}
+
+// real (R, C) -> (C, R), 32 x 32 tiles through LDS (Ty from Tx)
+template <class T>
+__device__ __forceinline__ void transpose_real_block(const T* __restrict__ src, T* __restrict__ dst, int R, int C, int bx, int by) {
+    __shared__ T tile[32][33];
+    const int c0 = bx * 32, r0 = by * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < R && c < C) tile[i][tx] = src[(size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (r < R && c < C) dst[(size_t)c * R + r] = tile[tx][i];
+    }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose_real(const T* __restrict__ src, T* __restrict__ dst, int R, int C) {
+    transpose_real_block<T>(src, dst, R, C, blockIdx.x, blockIdx.y);
+}
+template <class T>
+__global__ __launch_bounds__(256) void kb_transpose_real_spec(const BInst<T>* __restrict__ I, BMap m, int Ms) {
+    const int k = bmap_find(m, blockIdx.x);
+    const BInst<T>& b = I[m.inst[k]];
+    if (!b.active || !b.fresh || b.M != Ms) return;
+    const int Mh = Ms / 2 + 1, nx = (Mh + 31) / 32, lb = blockIdx.x - m.start[k];
+    transpose_real_block<T>(b.Kw[2], b.Kw[3], Mh, Mh, lb % nx, lb / nx);
+}
+
 // potentials planar (4, N, N) -> interleaved (N, N, 4)
 template <class T>
 __global__ void k_interleave4(const T* __restrict__ src, T* __restrict__ dst, i64 NN) {
@@ -503,7 +599,7 @@ template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* p
 }
 
 template <class T> void launch_spectra(int M, double delta, double dof, const double* table, bool round_kernel, int ntab,
-                                       cplx<T>* tmp, T* const* K, hipStream_t st) {
+                                       cplx<T>* tmp, T* const* K, hipStream_t st, bool sym) {
     require(ntab == 2 || ntab == 4, "ntab must be 2 or 4");
     const int Mh = M / 2 + 1;
     const size_t plane = (size_t)Mh * Mh;
@@ -521,14 +617,24 @@ template <class T> void launch_spectra(int M, double delta, double dof, const do
         hipLaunchKernelGGL((k_spec2<T, L_, NT>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
                            Q0, Q1, K[0], K[1], K[2], K[3], fft_twiddle_set<T>(L_)); \
     } while (0)
+#define MG_S2P(L_) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_spec2p<T, L_>, by); \
+        const int np = (Mh + 1) / 2; \
+        hipLaunchKernelGGL((k_spec2p<T, L_>), dim3((np + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
+                           Q0, delta, K[0], K[1], K[2], fft_twiddle_set<T>(L_)); \
+        MG_HIP(hipGetLastError()); \
+        hipLaunchKernelGGL(k_transpose_real<T>, dim3((Mh + 31) / 32, (Mh + 31) / 32), dim3(256), 0, st, (const T*)K[2], K[3], Mh, Mh); \
+    } while (0)
 #define MG_C(L_) case L_: \
-        if (ntab == 4) { MG_S1(L_, 4); launch_transpose<T>(P0, Q0, 2, Mh, Mh, st); MG_S2(L_, 4); } \
+        if (ntab == 4 && sym) { MG_S1(L_, 2); launch_transpose<T>(P0, Q0, 1, Mh, Mh, st); MG_S2P(L_); } \
+        else if (ntab == 4) { MG_S1(L_, 4); launch_transpose<T>(P0, Q0, 2, Mh, Mh, st); MG_S2(L_, 4); } \
         else { MG_S1(L_, 2); launch_transpose<T>(P0, Q0, 1, Mh, Mh, st); MG_S2(L_, 2); } \
         break;
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
 #undef MG_S1
 #undef MG_S2
+#undef MG_S2P
     MG_HIP(hipGetLastError());
 }
 
@@ -580,9 +686,10 @@ template <class T> void launch_rows_inverse_wl(const BInst<T>* I, const int* wl,
     MG_HIP(hipGetLastError());
 }
 
-// spectra of the instances in m1 (all of FFT size M): stage 1, transpose (blocks mt), stage 2
-template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, int M, double dof,
-                                             bool round_kernel, int ntab, hipStream_t st) {
+// spectra of the instances in m1 (all of FFT size M): stage 1, transpose (blocks mt), stage 2 (the centred set by
+// pairs of columns, blocks m2, then Ty from Tx, blocks mr; the t-SNE kernel is symmetric)
+template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, const BMap& m2, const BMap& mr,
+                                             int M, double dof, bool round_kernel, int ntab, hipStream_t st) {
     if (m1.count == 0 || m1.start[m1.count] == 0) return;
     require(ntab == 2 || ntab == 4, "ntab must be 2 or 4");
     const dim3 grid((unsigned)m1.start[m1.count]);
@@ -595,10 +702,24 @@ template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, 
         MG_HIP(hipGetLastError()); \
         hipLaunchKernelGGL((kb_spec2<T, L_, NT>), grid, dim3(fft_block<L_>()), by, st, I, m1, fft_twiddle_set<T>(L_)); \
     } while (0)
-#define MG_C(L_) case L_: if (ntab == 4) MG_S(L_, 4); else MG_S(L_, 2); break;
+#define MG_SP(L_) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(kb_spec1<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); \
+        hipLaunchKernelGGL((kb_spec1<T, L_, 2>), grid, dim3(fft_block<L_>()), by, st, I, m1, dof, (const double*)nullptr, (int)round_kernel, fft_twiddle_set<T>(L_)); \
+        MG_HIP(hipGetLastError()); \
+        if (mt.count > 0 && mt.start[mt.count] > 0) \
+            hipLaunchKernelGGL(kb_transpose_spec<T>, dim3((unsigned)mt.start[mt.count]), dim3(256), 0, st, I, mt, 1, M); \
+        MG_HIP(hipGetLastError()); \
+        if (m2.count > 0 && m2.start[m2.count] > 0) \
+            hipLaunchKernelGGL((kb_spec2p<T, L_>), dim3((unsigned)m2.start[m2.count]), dim3(fft_block<L_>()), by, st, I, m2, fft_twiddle_set<T>(L_)); \
+        MG_HIP(hipGetLastError()); \
+        if (mr.count > 0 && mr.start[mr.count] > 0) \
+            hipLaunchKernelGGL(kb_transpose_real_spec<T>, dim3((unsigned)mr.start[mr.count]), dim3(256), 0, st, I, mr, M); \
+    } while (0)
+#define MG_C(L_) case L_: if (ntab == 4) MG_SP(L_); else MG_S(L_, 2); break;
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
 #undef MG_S
+#undef MG_SP
     MG_HIP(hipGetLastError());
 }
 
@@ -607,12 +728,12 @@ template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, 
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
     template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
-    template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t); \
+    template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t, bool); \
     template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t); \
     template void launch_rows_forward_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
     template void launch_transpose_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
     template void launch_columns_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
     template void launch_rows_inverse_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \
-    template void launch_spectra_batch<T>(const BInst<T>*, const BMap&, const BMap&, int, double, bool, int, hipStream_t);
+    template void launch_spectra_batch<T>(const BInst<T>*, const BMap&, const BMap&, const BMap&, const BMap&, int, double, bool, int, hipStream_t);
 
 }  // namespace mg
