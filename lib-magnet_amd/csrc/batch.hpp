@@ -153,6 +153,7 @@ template <class T> void launch_transpose_wl(const BInst<T>* I, const int* wl, in
 template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, const BMap& mt, const BMap& m2, const BMap& mr,
                                              int M, double dof, bool round_kernel, int ntab, hipStream_t st);
 int spec_pair_blocks(int M);
+template <class T> void prepare_fft_kernels();   // every FFT kernel's LDS limit, set before threads start
 int fft_row_blocks(int N, int M, int channels);
 int fft_col_blocks(int M);
 inline int transpose_blocks(int R, int C, int batch) { return ((C + 31) / 32) * ((R + 31) / 32) * batch; }

@@ -189,7 +189,10 @@ template <class T> void launch_attraction(const int32_t* ip, const int32_t* ix, 
 // pairs != null adds the mid-range spring w * (Y_i - mean partners) to the gradient.
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h,
                                       T rho, T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim,
-                                      double* part, T* mean_bbox, const int* inv, hipStream_t st);
+                                      double* part, T* mean_bbox, const int* inv, hipStream_t st, T* host_mb = nullptr,
+                                      int* host_tag = nullptr, int tag = 0);
+// host_mb / host_tag (pinned host memory, 2-D only): mean_bbox's six values are also written there, then tag into
+// *host_tag (system scope, after the values), so the host can wait for them without a copy or an event
 // inv != null: the layout is relabelled (position inv[i] holds original point i); the mean and bbox are then
 // summed in original point order
 template <class T> void launch_mid_spring(const T* Y, const int32_t* pairs, T* grad, i64 n, int dim, int m, T w,

@@ -435,6 +435,8 @@ template <class T> struct Optimizer {
     T inv_dof = T(1);
     double* dsum = nullptr;
     T* hmb = nullptr;
+    int* htag = nullptr;
+    PinnedBuf host_tag;
     // locality relabelling: the state lives in Yw / Uw / Gw in the order of a recent box sort (position inv[i]
     // holds original point i, perm is its inverse); the CSR is renumbered to match (cip, cix, cP)
     bool relabel = false;
@@ -520,6 +522,8 @@ template <class T> struct Optimizer {
         bb_out.ensure(4 * sizeof(T));
         dsum = reinterpret_cast<double*>(sumq.ensure((size_t)std::max(1, it1 - it0) * sizeof(double)));
         hmb = reinterpret_cast<T*>(host_mb.ensure(8 * sizeof(T)));
+        htag = reinterpret_cast<int*>(host_tag.ensure(sizeof(int)));
+        __atomic_store_n(htag, -1, __ATOMIC_RELEASE);
         launch_bbox<T>(Y, n, dptr<T>(bb_part), bc, dptr<T>(bb_out), st);
         MG_HIP(hipMemcpyAsync(hmb + 2, dptr<T>(bb_out), 4 * sizeof(T), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
@@ -566,7 +570,7 @@ template <class T> struct Optimizer {
         g_prof.begin(ST_UPDATE, st);
         if (fused) {
             launch_update<T>(0, Y, U, G, dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), dptr<T>(mb), labels, st);
+                             dptr<double>(part), dptr<T>(mb), labels, st, hmb, htag, it);
         } else {
             const double w_mid = p->mid_weight * std::max(0.0, 1.0 - it / std::max(1.0, p->mid_phase * p->n_iter)) / (double)n;
             launch_grad<T>(dptr<T>(attr), dptr<T>(F), dinv, T(0), rho, Y, w_mid > 0 ? pairs : nullptr, m, (T)w_mid, n, 2, 1,
@@ -575,20 +579,37 @@ template <class T> struct Optimizer {
                 launch_trace<T>(ip, ix, data, Y, nullptr, n, p->dof, dptr<T>(grad), nullptr, nullptr, T(0),
                                 dptr<double>(rep.zout), dptr<double>(trace_scratch), dtrace, st);
             launch_update<T>(1, Y, U, G, dptr<T>(grad), nullptr, nullptr, T(0), T(0), (T)lr, mom, nullptr, 0, T(0), n, 2,
-                             dptr<double>(part), dptr<T>(mb), nullptr, st);
+                             dptr<double>(part), dptr<T>(mb), nullptr, st, hmb, htag, it);
         }
         g_prof.end(ST_UPDATE, st, (fused ? 20.0 : 24.0) * n * sizeof(T));
-        MG_HIP(hipMemcpyAsync(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost, st));
-        MG_HIP(hipEventRecord(mb_ready, st));
+        MG_HIP(hipEventRecord(mb_ready, st));   // the recentring kernel wrote the bbox to hmb and `it` to *htag
         if (relabel && it + 1 < it1 && ++since % relabel_every == 0) relabel_now();
         if (it + 1 < it1) attraction();
     }
 
     // waits for iteration it; returns true when the callback asks to stop
+    // the bbox of iteration it is in hmb once *htag reads it (written after it by the recentring kernel); Y, U, G are
+    // final once mb_ready has passed
+    void wait_bbox(int it) {
+        for (unsigned spin = 1;; ++spin) {
+            if (__atomic_load_n(htag, __ATOMIC_ACQUIRE) == it) return;
+            if ((spin & 255) == 0) {
+                const hipError_t q = hipEventQuery(mb_ready);
+                if (q == hipSuccess) {
+                    if (__atomic_load_n(htag, __ATOMIC_ACQUIRE) != it)   // not seen yet: read the device copy
+                        MG_HIP(hipMemcpy(hmb, dptr<T>(mb), 6 * sizeof(T), hipMemcpyDeviceToHost));
+                    return;
+                }
+                if (q != hipErrorNotReady) MG_HIP(q);
+            }
+        }
+    }
     bool complete(int it) {
-        MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
+        const bool cb_due = cb && (it % 50 == 0 || it == p->n_iter - 1);
+        if (cb_due || g_prof.on) MG_HIP(hipEventSynchronize(mb_ready));   // Y, U, G and the bbox of iteration it are final
+        else wait_bbox(it);
         if (g_prof.on) { MG_HIP(hipStreamSynchronize(st)); MG_HIP(hipStreamSynchronize(aside)); g_prof.collect(it); }
-        if (cb && (it % 50 == 0 || it == p->n_iter - 1)) {
+        if (cb_due) {
             if (!relabel) return cb(it, Y, n, user) != 0;
             // the layout in original point order; Y itself is final for iteration it (the event), and the
             // relabelling that may follow it in the stream is not waited for
@@ -1182,6 +1203,8 @@ template <class T> void optimize_batch(const mg_tsne_params* p, int count, const
         groups[s].push_back(&inst[i]);
         load[s] += inst[i].n;
     }
+    fft_init_twiddles();
+    prepare_fft_kernels<T>();   // no kernel attribute is set while the groups' threads launch and capture
     std::vector<hipStream_t> streams(S, nullptr);
     struct Guard { std::vector<hipStream_t>& s; ~Guard() { for (auto x : s) if (x) (void)hipStreamDestroy(x); } } guard{streams};
     for (auto& s : streams) MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

@@ -723,7 +723,25 @@ template <class T> void launch_spectra_batch(const BInst<T>* I, const BMap& m1, 
     MG_HIP(hipGetLastError());
 }
 
+// Raises the LDS limit of every FFT kernel of every size, once, before a batch's host threads start: a kernel's first
+// launch would otherwise set it (hipFuncSetAttribute) in whichever thread launches it first, while other threads
+// launch and capture the same kernels.
+template <class T> void prepare_fft_kernels() {
+    constexpr bool CEN = sizeof(T) == 4;
+#define MG_R(L_) { const int by = lds_bytes<T, L_>(); set_lds(k_rows_fwd<T, L_>, by); set_lds(kw_rows_fwd<T, L_>, by); \
+                   set_lds(k_rows_inv<T, L_>, by); set_lds(kw_rows_inv<T, L_>, by); }
+#define MG_K(L_) { const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN>, by); set_lds(kw_columns<T, L_, CEN>, by); \
+                   set_lds(k_spec1<T, L_, 2>, by); set_lds(k_spec1<T, L_, 4>, by); set_lds(k_spec2<T, L_, 2>, by); \
+                   set_lds(k_spec2<T, L_, 4>, by); set_lds(k_spec2p<T, L_>, by); set_lds(kb_spec1<T, L_, 2>, by); \
+                   set_lds(kb_spec2<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); }
+    MG_ROW_SIZES(MG_R)
+    MG_COL_SIZES(MG_K)
+#undef MG_R
+#undef MG_K
+}
+
 #define MG_FFT_INST(T) \
+    template void prepare_fft_kernels<T>(); \
     template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
     template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t); \

@@ -350,7 +350,7 @@ __global__ __launch_bounds__(TPB) void k_meanbox_perm(const T* __restrict__ Y, c
 // the centred layout (rounding is monotone, so min(y_i - m) = min(y_i) - m exactly). No block waits for another.
 template <class T>
 __device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const double* __restrict__ part, T* __restrict__ mb,
-                                              int bid, int nblk) {
+                                              int bid, int nblk, T* hmb = nullptr, int* htag = nullptr, int tag = 0) {
     __shared__ T s_m[2];
     typedef typename vec2<T>::type V;
     CSum sx = {0.0, 0.0}, sy = {0.0, 0.0};
@@ -365,7 +365,14 @@ __device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const do
     if (threadIdx.x == 0) {
         const T mx = (T)((m.sx.s + m.sx.c) / (double)n), my = (T)((m.sy.s + m.sy.c) / (double)n);
         s_m[0] = mx; s_m[1] = my;
-        if (bid == 0) { mb[0] = mx; mb[1] = my; mb[2] = m.x0 - mx; mb[3] = m.y0 - my; mb[4] = m.x1 - mx; mb[5] = m.y1 - my; }
+        if (bid == 0) {
+            mb[0] = mx; mb[1] = my; mb[2] = m.x0 - mx; mb[3] = m.y0 - my; mb[4] = m.x1 - mx; mb[5] = m.y1 - my;
+            if (hmb) {   // the same into pinned host memory, then the tag the host waits for
+                hmb[0] = mb[0]; hmb[1] = mb[1]; hmb[2] = mb[2]; hmb[3] = mb[3]; hmb[4] = mb[4]; hmb[5] = mb[5];
+                __threadfence_system();
+                __hip_atomic_store(htag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
     __syncthreads();
     const T mx = s_m[0], my = s_m[1];
@@ -376,8 +383,8 @@ __device__ __forceinline__ void center2_block(T* __restrict__ Y, i64 n, const do
 }
 template <class T>
 __global__ __launch_bounds__(TPB) void k_center2(T* __restrict__ Y, i64 n, const double* __restrict__ part,
-                                                 T* __restrict__ mb) {
-    center2_block<T>(Y, n, part, mb, blockIdx.x, gridDim.x);
+                                                 T* __restrict__ mb, T* __restrict__ hmb, int* __restrict__ htag, int tag) {
+    center2_block<T>(Y, n, part, mb, blockIdx.x, gridDim.x, hmb, htag, tag);
 }
 
 // batched fused step of the t-SNE loop (gradient rho * attr - F / sumQ of each active instance, its iteration's
@@ -654,7 +661,7 @@ template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T*
 
 template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, const T* F, const T* d_inv, T inv_h, T rho,
                                       T lr, T mom, const int32_t* pairs, int m, T w_mid, i64 n, int dim, double* part,
-                                      T* mean_bbox, const int* inv, hipStream_t st) {
+                                      T* mean_bbox, const int* inv, hipStream_t st, T* host_mb, int* host_tag, int tag) {
     (void)pairs; (void)m; (void)w_mid;
     if (n == 0) return;
     if (dim == 2) {
@@ -673,7 +680,7 @@ template <class T> void launch_update(int mode, T* Y, T* U, T* G, const T* A, co
                 MG_HIP(hipGetLastError());
             }
         }
-        hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, (const double*)part, mean_bbox);
+        hipLaunchKernelGGL(k_center2<T>, dim3(blocks), dim3(TPB), 0, st, Y, n, (const double*)part, mean_bbox, host_mb, host_tag, tag);
     } else {
         require(d_inv == nullptr && inv == nullptr, "device-side inv_sumQ and relabelling need a 2-D layout");
         const i64 total = n * dim;
@@ -729,7 +736,7 @@ template <class T> void launch_fill(T* p, T v, i64 count, hipStream_t st) {
 #define MG_INST(T) \
     template void launch_bbox<T>(const T*, i64, T*, unsigned*, T*, hipStream_t); \
     template void launch_attraction<T>(const int32_t*, const int32_t*, const T*, const T*, i64, int, T, T*, i64, hipStream_t); \
-    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, T*, const int*, hipStream_t); \
+    template void launch_update<T>(int, T*, T*, T*, const T*, const T*, const T*, T, T, T, T, const int32_t*, int, T, i64, int, double*, T*, const int*, hipStream_t, T*, int*, int); \
     template void launch_grad<T>(const T*, const T*, const T*, T, T, const T*, const int32_t*, int, T, i64, int, int, T*, hipStream_t); \
     template void launch_mid_spring<T>(const T*, const int32_t*, T*, i64, int, int, T, hipStream_t); \
     template void launch_fill<T>(T*, T, i64, hipStream_t); \
