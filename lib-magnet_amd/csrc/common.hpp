@@ -150,6 +150,7 @@ struct Tunables {
     int attr_win_kb = 64;   // LDS window of the sweep: 128 or 64 KiB of positions (0: no window, rows only)
     int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
     int sort_items = 0;     // keys per thread of a radix-sort pass up to 2^20 keys: 1, 2, 4 or 8 (0: 2 up to 262,144 keys, else 8)
+    int fft_strided_max = 1024;   // padded FFT sizes up to which the column pass works in the row layout (no transposes)
     int sort_digit = 8;     // digit bits of the radix sorts of keys of 9-16 bits: 8 (two passes either way), or 10
     int batch_graph = 1;    // the batched loop replays its iteration as a captured hipGraph (0: issues the launches)
     int batch_chunk = 16;   // iterations the batched loop queues between two looks at the instances' states
@@ -288,7 +289,9 @@ template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int b
 // K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]. Wh (3, Mh, N),
 // Ph (4, Mh, N). centred must be true for float and false for double.
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
-                                       hipStream_t st);
+                                       hipStream_t st, bool strided = false);
+// strided: Wh is the row pass's (3, N, Mh) output and Ph receives (4, N, Mh) for the inverse row pass, with no
+// transposes (the same values)
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st);   // X (4, N, Mh)
 // sym (ntab 4): the kernel is symmetric in x <-> y (K(a, b) = K(b, a)); y K2's table is then x K2's transposed
 // (fewer transforms). kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M), even

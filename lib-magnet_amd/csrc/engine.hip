@@ -372,19 +372,25 @@ template <class T> struct Repulsion {
         P.begin(ST_ROWS_FWD, st);
         launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
         P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs, M);
-        P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
-        launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
-        P.end(ST_TR_FWD, st, 0, M);
+        // small grids: the column pass reads and writes the row layout in place of the two transposes (L2-resident)
+        const bool strided = M <= tunables().fft_strided_max;
+        if (!strided) {
+            P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
+            launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
+            P.end(ST_TR_FWD, st, 0, M);
+        }
         if (side_run) MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
-        launch_columns<T>(b.Wh, g.N, M, Kc, centred, b.Ph, st);
+        launch_columns<T>(strided ? b.Tr : b.Wh, g.N, M, Kc, centred, b.Ph, st, strided);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs, M);
-        P.begin(ST_TR_INV, st);
-        launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
-        P.end(ST_TR_INV, st, 0, M);
+        if (!strided) {
+            P.begin(ST_TR_INV, st);
+            launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
+            P.end(ST_TR_INV, st, 0, M);
+        }
         P.begin(ST_ROWS_INV, st);
-        launch_rows_inverse<T>(b.Tr, g.N, M, b.phi, st);
+        launch_rows_inverse<T>(strided ? b.Ph : b.Tr, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts, M);
         P.begin(ST_GATHER, st);
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
@@ -1354,6 +1360,7 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "attr_tpb") { require(value == 256 || value == 512 || value == 1024, "attr_tpb must be 256, 512 or 1024"); g_tunables.attr_tpb = value; }
     else if (k == "sort_items") { require(value == 0 || value == 1 || value == 2 || value == 4 || value == 8, "sort_items must be 0, 1, 2, 4 or 8"); g_tunables.sort_items = value; }
     else if (k == "sort_digit") { require(value == 8 || value == 10, "sort_digit must be 8 or 10"); g_tunables.sort_digit = value; }
+    else if (k == "fft_strided_max") { require(value >= 0 && value <= 8192, "fft_strided_max must be in [0, 8192]"); g_tunables.fft_strided_max = value; }
     else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
     else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
     else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }
@@ -1373,6 +1380,7 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "attr_tpb") *value = g_tunables.attr_tpb;
     else if (k == "sort_items") *value = g_tunables.sort_items;
     else if (k == "sort_digit") *value = g_tunables.sort_digit;
+    else if (k == "fft_strided_max") *value = g_tunables.fft_strided_max;
     else if (k == "batch_graph") *value = g_tunables.batch_graph;
     else if (k == "batch_chunk") *value = g_tunables.batch_chunk;
     else if (k == "batch_window_down") *value = g_tunables.batch_window_down;

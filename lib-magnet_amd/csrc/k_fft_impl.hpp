@@ -149,7 +149,9 @@ template <class T, int M> __device__ __forceinline__ T ktab(const T* __restrict_
 // ---- columns: one column ky per fft_th threads. Channel 0 is transformed once and its spectrum kept in
 // registers (slot (q, r) of thread t holds frequency t + q TH + r M / 16); each output is a product formed
 // in those registers and transformed back by the reversed plan, with no LDS round trip in between.
-template <class T, int L, bool CEN>
+// STR: the input and output stay in the row passes' layout, (3, N, Mh) in and (4, N, Mh) out, read and written with
+// stride Mh (for grids small enough to sit in L2); no transposes then, and the same values
+template <class T, int L, bool CEN, bool STR = false>
 __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, int N,
                                               const T* __restrict__ K1, const T* __restrict__ K2,
                                               const T* __restrict__ TX, const T* __restrict__ TY,
@@ -167,19 +169,19 @@ __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, in
     const int kyc = live ? ky : 0;
     cplx<T> s0[QL][RL], v[QL][RL];
     auto fwd = [&](int ch, cplx<T> (&dst)[QL][RL]) {
-        const cplx<T>* col = Wh + ((size_t)ch * Mh + kyc) * N;
+        const cplx<T>* col = STR ? Wh + (size_t)ch * N * Mh + kyc : Wh + ((size_t)ch * Mh + kyc) * N;
         auto ld = [&](int e, int, int r) -> cplx<T> {
             if (2 * r >= R0 || e >= N) return czero<T>();   // rows >= N (all of r >= R0/2) are padding
-            return col[e];
+            return STR ? col[(size_t)e * Mh] : col[e];
         };
         auto st = [&](int, cplx<T> x, int q, int r) { dst[q][r] = x; };
         fft_run<T, F, -1, TH, FN>(t, lds, b * L, tws.fwd, ld, st);
     };
     auto inv = [&](int o, cplx<T> (&src)[QL][RL]) {
-        cplx<T>* dst = Ph + ((size_t)o * Mh + kyc) * N;
+        cplx<T>* dst = STR ? Ph + (size_t)o * N * Mh + kyc : Ph + ((size_t)o * Mh + kyc) * N;
         auto ld = [&](int, int q, int r) { return src[q][r]; };
         auto st = [&](int e, cplx<T> x, int, int r) {
-            if (2 * r < R0 && e < N && live) dst[e] = x;   // only the N live rows
+            if (2 * r < R0 && e < N && live) { if (STR) dst[(size_t)e * Mh] = x; else dst[e] = x; }   // only the N live rows
         };
         fft_run<T, I, +1, TH, FN>(t, lds, b * L, tws.rev, ld, st);
     };
@@ -217,12 +219,12 @@ __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, in
         inv(ch + 1, v);
     }
 }
-template <class T, int L, bool CEN>
+template <class T, int L, bool CEN, bool STR>
 __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
                                                             const T* __restrict__ K1, const T* __restrict__ K2,
                                                             const T* __restrict__ TX, const T* __restrict__ TY,
                                                             cplx<T>* __restrict__ Ph, TwSet<T> tws) {
-    columns_block<T, L, CEN>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x);
+    columns_block<T, L, CEN, STR>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x);
 }
 template <class T, int L, bool CEN>
 __global__ __launch_bounds__(fft_block<L>()) void kw_columns(const BInst<T>* __restrict__ I, const int* __restrict__ wl,
@@ -571,18 +573,19 @@ template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int b
 }
 
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
-                                       hipStream_t st) {
+                                       hipStream_t st, bool strided) {
     require(N <= M / 2, "FFT size below 2N");
     require(centred == (sizeof(T) == 4), "the FP32 path uses the centred product set, the FP64 path the reference's");
     constexpr bool CEN = sizeof(T) == 4;
     const int Mh = M / 2 + 1;
-#define MG_C(L_) case L_: { \
-        const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN>, by); \
-        hipLaunchKernelGGL((k_columns<T, L_, CEN>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
-                           Wh, N, K[0], K[1], K[2], K[3], Ph, fft_twiddle_set<T>(L_)); \
-        break; }
+#define MG_L(L_, S_) do { \
+        const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN, S_>, by); \
+        hipLaunchKernelGGL((k_columns<T, L_, CEN, S_>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
+                           Wh, N, K[0], K[1], K[2], K[3], Ph, fft_twiddle_set<T>(L_)); } while (0)
+#define MG_C(L_) case L_: if (strided) MG_L(L_, true); else MG_L(L_, false); break;
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
+#undef MG_L
     MG_HIP(hipGetLastError());
 }
 
@@ -730,7 +733,8 @@ template <class T> void prepare_fft_kernels() {
     constexpr bool CEN = sizeof(T) == 4;
 #define MG_R(L_) { const int by = lds_bytes<T, L_>(); set_lds(k_rows_fwd<T, L_>, by); set_lds(kw_rows_fwd<T, L_>, by); \
                    set_lds(k_rows_inv<T, L_>, by); set_lds(kw_rows_inv<T, L_>, by); }
-#define MG_K(L_) { const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN>, by); set_lds(kw_columns<T, L_, CEN>, by); \
+#define MG_K(L_) { const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN, false>, by); \
+                   set_lds(k_columns<T, L_, CEN, true>, by); set_lds(kw_columns<T, L_, CEN>, by); \
                    set_lds(k_spec1<T, L_, 2>, by); set_lds(k_spec1<T, L_, 4>, by); set_lds(k_spec2<T, L_, 2>, by); \
                    set_lds(k_spec2<T, L_, 4>, by); set_lds(k_spec2p<T, L_>, by); set_lds(kb_spec1<T, L_, 2>, by); \
                    set_lds(kb_spec2<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); }
@@ -744,7 +748,7 @@ template <class T> void prepare_fft_kernels() {
     template void prepare_fft_kernels<T>(); \
     template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
-    template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t); \
+    template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t, bool); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
     template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t, bool); \
     template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t); \
