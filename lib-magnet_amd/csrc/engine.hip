@@ -351,15 +351,21 @@ template <class T> struct Repulsion {
                 MG_HIP(hipEventCreateWithFlags(&ev_go, hipEventDisableTiming));
                 MG_HIP(hipEventCreateWithFlags(&ev_spec, hipEventDisableTiming));
             }
-            MG_HIP(hipEventRecord(ev_go, st));
+            MG_HIP(hipEventRecord(ev_go, st));   // before this call's first launch
             MG_HIP(hipStreamWaitEvent(side, ev_go, 0));
             sst = side;
         }
-        P.begin(ST_SPECTRA, sst);
-        bool fresh = false;
-        Spec& s = spectra(M, sst, &fresh);
-        P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0, M);   // tables written when the grid changed
-        if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
+        // the spectra, which have slack until the column pass, are issued after the sort, the spread and the row
+        // pass, whose launches then reach the device first
+        Spec* sp = nullptr;
+        auto issue_spectra = [&]() {
+            P.begin(ST_SPECTRA, sst);
+            bool fresh = false;
+            sp = &spectra(M, sst, &fresh);
+            P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0, M);   // tables written when the grid changed
+            if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
+        };
+        if (!side_run) issue_spectra();
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
         sorted_buf = launch_box_sort<T>(Y, n, g, b, st, inv);
@@ -379,7 +385,11 @@ template <class T> struct Repulsion {
             launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
             P.end(ST_TR_FWD, st, 0, M);
         }
-        if (side_run) MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
+        if (side_run) {
+            issue_spectra();
+            MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
+        }
+        const Spec& s = *sp;
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(strided ? b.Tr : b.Wh, g.N, M, Kc, centred, b.Ph, st, strided);
