@@ -440,6 +440,79 @@ def traffic_of(workload, kernel):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
+FFT_COL_KERNELS = ("k_columns", "kw_columns", "k_spec1", "k_spec2", "k_spec2p", "kb_spec1", "kb_spec2", "kb_spec2p")
+FFT_ROW_KERNELS = ("k_rows_fwd", "k_rows_inv", "kw_rows_fwd", "kw_rows_inv")
+
+
+def kernel_key(name):
+    """A rocprofv3 kernel name as the traffic table keys it: the kernel's own name with its element type, and for the
+    FFT kernels the padded FFT size M (the row kernels are compiled for M / 2)."""
+    s = name.replace("(anonymous namespace)::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    depth, cut = 0, len(s)
+    for i, ch in enumerate(s):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            cut = i
+            break
+    s = s[:cut]
+    base, _, targs = s.partition("<")
+    base = base.split("::")[-1]
+    args, depth, cur = [], 0, ""
+    for ch in targs[:-1] if targs.endswith(">") else targs:
+        if ch == "," and depth == 0:
+            args.append(cur.strip()); cur = ""
+            continue
+        depth += ch == "<"
+        depth -= ch == ">"
+        cur += ch
+    if cur.strip():
+        args.append(cur.strip())
+    if base in FFT_ROW_KERNELS and len(args) > 1:
+        return f"{base}<{args[0]}, {2 * int(args[1])}>"
+    if base in FFT_COL_KERNELS and len(args) > 1:
+        return f"{base}<{args[0]}, {args[1]}>"
+    return f"{base}<{args[0]}>" if args else base
+
+
+def pmc_per_kernel(directory, counter):
+    """{kernel key: (dispatches, mean counter value)} from the rocprofv3 --pmc CSV files under `directory`."""
+    import csv
+    import glob
+    acc = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = kernel_key(row["Kernel_Name"])
+                n, v = acc.get(k, (0, 0.0))
+                acc[k] = (n + 1, v + float(row["Counter_Value"]))
+    return {k: (n, v / n) for k, (n, v) in acc.items()}
+
+
+def traffic_table(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib, out):
+    """The HBM traffic table bench.py reads (`roofline.traffic`): FETCH_SIZE and WRITE_SIZE per dispatch (KiB) of
+    every kernel from two --pmc runs, each scaled so that the same counter reads the known size of a plain device
+    copy (`--copy-probe copy_mib`, profiled the same way)."""
+    copy_bytes = copy_mib * (1 << 20)
+    cf = max(pmc_per_kernel(copy_fetch_dir, "FETCH_SIZE").items(), key=lambda kv: kv[1][1])
+    cw = max(pmc_per_kernel(copy_write_dir, "WRITE_SIZE").items(), key=lambda kv: kv[1][1])
+    ff, wf = copy_bytes / (cf[1][1] * 1024), copy_bytes / (cw[1][1] * 1024)
+    fetch, write = pmc_per_kernel(fetch_dir, "FETCH_SIZE"), pmc_per_kernel(write_dir, "WRITE_SIZE")
+    table = {"_scale": {"copy_bytes": copy_bytes, "copy_kernel": cf[0], "FETCH_SIZE_KiB": cf[1][1],
+                        "WRITE_SIZE_KiB": cw[1][1], "fetch_factor": ff, "write_factor": wf}}
+    for k in sorted(set(fetch) & set(write), key=lambda k: -(fetch[k][0] * fetch[k][1] + write[k][0] * write[k][1])):
+        fb, wb = fetch[k][1] * 1024 * ff, write[k][1] * 1024 * wf
+        table[k] = {"dispatches": fetch[k][0], "fetch_bytes_per_launch": round(fb), "write_bytes_per_launch": round(wb),
+                    "bytes_per_launch": round(fb + wb)}
+    with open(out, "w") as f:
+        json.dump(table, f, indent=1)
+    print(json.dumps({"traffic_table": out, "kernels": len(table) - 1, "scale": table["_scale"]}), flush=True)
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of `mib` MiB (the PMC scale reference): prints its size and time."""
     import torch
@@ -543,6 +616,10 @@ def scale_leg(warmup):
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-worker":
         cpu_baseline_worker(sys.argv[2])
+        return
+    if len(sys.argv) == 8 and sys.argv[1] == "--traffic-from":
+        # --traffic-from FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB OUT_JSON (host only)
+        traffic_table(*sys.argv[2:6], int(sys.argv[6]), sys.argv[7])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -59,7 +59,9 @@ int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
  *                  262,144 keys, 8 above);
  *   "sort_digit"   digit bits of the radix sorts of keys of 9 to 16 bits (two passes either way): 8 (default) or 10;
  *   "fft_strided_max" padded FFT sizes up to which the column pass of the repulsion's 2-D FFT works in the row
- *                  pass's layout, with no transposes (default 1024; 0: always transpose);
+ *                  pass's layout, with no transposes (default 1024; 0: never);
+ *   "fft_fold"     above that size, rows per block of the row passes that write and read the column pass's layout
+ *                  themselves: 4 (default) or 2 (FP64 takes 2), or 0 for row passes and transposes;
  *   "batch_graph"  1 (default): mg_tsne_optimize_batch replays its captured iteration (a hipGraph); 0: it issues the
  *                  same launches one by one;
  *   "batch_chunk"  iterations mg_tsne_optimize_batch queues between two reads of the instances' states (default 16);

@@ -151,6 +151,8 @@ struct Tunables {
     int attr_tpb = 1024;    // threads per block of the sweep: 1024, 512 or 256
     int sort_items = 0;     // keys per thread of a radix-sort pass up to 2^20 keys: 1, 2, 4 or 8 (0: 2 up to 262,144 keys, else 8)
     int fft_strided_max = 1024;   // padded FFT sizes up to which the column pass works in the row layout (no transposes)
+    int fft_fold = 4;       // above that size, rows per block of the row passes that write / read the column layout
+                            // themselves (4 or 2; FP64 takes 2), or 0: row passes and transposes
     int sort_digit = 8;     // digit bits of the radix sorts of keys of 9-16 bits: 8 (two passes either way), or 10
     int batch_graph = 1;    // the batched loop replays its iteration as a captured hipGraph (0: issues the launches)
     int batch_chunk = 16;   // iterations the batched loop queues between two looks at the instances' states
@@ -293,6 +295,11 @@ template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T*
 // strided: Wh is the row pass's (3, N, Mh) output and Ph receives (4, N, Mh) for the inverse row pass, with no
 // transposes (the same values)
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st);   // X (4, N, Mh)
+// the row passes with the transposes folded in: W (3, N, N) -> Wh (3, Mh, N) and Ph (4, Mh, N) -> phi (4, N, N), the
+// same values as a row pass and a transpose; `rows` rows per block (4 or 2; FP64 takes 2); padded sizes 512-8192
+template <class T> bool fold_supported(int M);
+template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<T>* Wh, int rows, hipStream_t st);
+template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M, T* phi, int rows, hipStream_t st);
 // sym (ntab 4): the kernel is symmetric in x <-> y (K(a, b) = K(b, a)); y K2's table is then x K2's transposed
 // (fewer transforms). kernel spectra on the M x M torus: values from (delta, dof) or from a host-built table (double, (2, M, M), even
 // in each offset); ntab = 2 (K1, K2) or 4 (K1, K2, x K2, y K2); outputs (Mh, Mh) real, [ky][kx]

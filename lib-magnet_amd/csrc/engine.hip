@@ -375,12 +375,15 @@ template <class T> struct Repulsion {
         P.begin(ST_SPREAD, st);
         launch_spread<T>(keys, vals, Y, n, g, ni, centred, b, st);
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
-        P.begin(ST_ROWS_FWD, st);
-        launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
-        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs, M);
-        // small grids: the column pass reads and writes the row layout in place of the two transposes (L2-resident)
+        // small grids: the column pass reads and writes the row layout in place of the two transposes (L2-resident);
+        // larger ones: the row passes write and read the column layout themselves (folded), or through transposes
         const bool strided = M <= tunables().fft_strided_max;
-        if (!strided) {
+        const bool folded = !strided && tunables().fft_fold > 0 && fold_supported<T>(M);
+        P.begin(ST_ROWS_FWD, st);
+        if (folded) launch_rows_forward_fold<T>(b.W, g.N, M, b.Wh, tunables().fft_fold, st);
+        else launch_rows_forward<T>(b.W, g.N, M, b.Tr, st);
+        P.end(ST_ROWS_FWD, st, 3 * NN * ts + 3 * Mh * g.N * cs, M);
+        if (!strided && !folded) {
             P.begin(ST_TR_FWD, st);   // layout change only: no algorithmic traffic
             launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
             P.end(ST_TR_FWD, st, 0, M);
@@ -394,13 +397,14 @@ template <class T> struct Repulsion {
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
         launch_columns<T>(strided ? b.Tr : b.Wh, g.N, M, Kc, centred, b.Ph, st, strided);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs, M);
-        if (!strided) {
+        if (!strided && !folded) {
             P.begin(ST_TR_INV, st);
             launch_transpose<T>(b.Ph, b.Tr, 4, M / 2 + 1, g.N, st);
             P.end(ST_TR_INV, st, 0, M);
         }
         P.begin(ST_ROWS_INV, st);
-        launch_rows_inverse<T>(strided ? b.Ph : b.Tr, g.N, M, b.phi, st);
+        if (folded) launch_rows_inverse_fold<T>(b.Ph, g.N, M, b.phi, tunables().fft_fold, st);
+        else launch_rows_inverse<T>(strided ? b.Ph : b.Tr, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts, M);
         P.begin(ST_GATHER, st);
         launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
@@ -1371,6 +1375,7 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "sort_items") { require(value == 0 || value == 1 || value == 2 || value == 4 || value == 8, "sort_items must be 0, 1, 2, 4 or 8"); g_tunables.sort_items = value; }
     else if (k == "sort_digit") { require(value == 8 || value == 10, "sort_digit must be 8 or 10"); g_tunables.sort_digit = value; }
     else if (k == "fft_strided_max") { require(value >= 0 && value <= 8192, "fft_strided_max must be in [0, 8192]"); g_tunables.fft_strided_max = value; }
+    else if (k == "fft_fold") { require(value == 0 || value == 2 || value == 4, "fft_fold must be 0, 2 or 4"); g_tunables.fft_fold = value; }
     else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
     else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
     else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }
@@ -1391,6 +1396,7 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "sort_items") *value = g_tunables.sort_items;
     else if (k == "sort_digit") *value = g_tunables.sort_digit;
     else if (k == "fft_strided_max") *value = g_tunables.fft_strided_max;
+    else if (k == "fft_fold") *value = g_tunables.fft_fold;
     else if (k == "batch_graph") *value = g_tunables.batch_graph;
     else if (k == "batch_chunk") *value = g_tunables.batch_chunk;
     else if (k == "batch_window_down") *value = g_tunables.batch_window_down;

@@ -292,6 +292,101 @@ __global__ __launch_bounds__(fft_block<L>()) void kw_rows_inv(const BInst<T>* __
     }
 }
 
+// ---- the row passes with the transposes folded in (grids above the strided column pass). A block takes RB
+// consecutive rows of one channel, its RB transforms in LDS at element offsets r S (S = L + 16: each buffer starts
+// 17 pad slots further, which turns the banks of the RB buffers against each other), and writes or reads the
+// column layout (channel, ky, row) in segments of RB rows (32 bytes for RB x 8 and RB x 16 bytes). Blocks are
+// numbered so that each of the 8 XCDs (the dispatcher hands consecutive blocks to them in turn) takes a contiguous
+// run of row groups: the blocks writing or reading the other segments of a 128-byte line then run in the same L2
+// at about the same time. The arithmetic is the row kernels'.
+constexpr int kXcds = 8;
+inline int xcd_grid(int total) { return (total + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_block(int total) {   // the block's index in row-group order; -1 past the end
+    const int per = (int)gridDim.x / kXcds;              // gridDim.x = xcd_grid(total)
+    const int v = ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
+    return v < total ? v : -1;
+}
+template <class T, int L, int RB> constexpr int fold_block() { return RB * fft_th<L>(); }
+template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded(RB * (L + 16)) * (int)sizeof(cplx<T>); }
+
+// W (3, N, N) -> Wh (3, Mh, N): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments
+template <class T, int L, int RB>
+__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(const T* __restrict__ W, int N, cplx<T>* __restrict__ Wh,
+                                                                         TwSet<T> tws) {
+    using Pl = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, R0 = Pl::R[0], S = L + 16;
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    const int G = (N + RB - 1) / RB;
+    const int v = xcd_block(3 * G);
+    if (v < 0) return;
+    const int c = v / G, row0 = (v % G) * RB;
+    const int t = threadIdx.x % TH, b = threadIdx.x / TH;
+    const int gx = row0 + b;
+    const bool live = gx < N, even = (N & 1) == 0;
+    const T* src = W + ((size_t)c * N + (live ? gx : 0)) * N;
+    auto ld = [&](int e, int, int r) -> cplx<T> {
+        cplx<T> z = czero<T>();
+        if (2 * r >= R0) return z;   // e >= L/2: samples 2e >= L >= N are padding
+        const int n0 = 2 * e;
+        if (live && n0 < N) {
+            if (even) { const V2 p = reinterpret_cast<const V2*>(src)[e]; z.x = p.x; z.y = p.y; }
+            else { z.x = src[n0]; if (n0 + 1 < N) z.y = src[n0 + 1]; }
+        }
+        return z;
+    };
+    auto none = [](int, cplx<T>, int, int) {};
+    fft_run<T, Pl, -1, TH, TO_LDS>(t, lds, b * S, tws.fwd, ld, none);
+    __syncthreads();
+    cplx<T>* dst = Wh + (size_t)c * Mh * N + row0;
+    for (int i = threadIdx.x; i < Mh * RB; i += fold_block<T, L, RB>()) {
+        const int r = i % RB, k = i / RB;
+        if (row0 + r < N) dst[(size_t)k * N + r] = r2c_post<T, L>(lds, r * S, k, tws.half);
+    }
+}
+
+// Ph (4, Mh, N) -> phi (4, N, N): the Mh inputs of the RB rows by segments into LDS, then rows_inv_block's transform
+// per row with its inputs from there
+template <class T, int L, int RB>
+__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(const cplx<T>* __restrict__ Ph, int N, T* __restrict__ phi,
+                                                                         T scale, TwSet<T> tws) {
+    using Pl = Rev<typename FwdPlan<T, L>::type>;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1], S = L + 16;
+    static_assert(Pl::P > 1, "the first pass must read all its inputs before it writes the buffer");
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    const int G = (N + RB - 1) / RB;
+    const int v = xcd_block(4 * G);
+    if (v < 0) return;
+    const int c = v / G, row0 = (v % G) * RB;
+    const cplx<T>* src = Ph + (size_t)c * Mh * N + row0;
+    for (int i = threadIdx.x; i < Mh * RB; i += fold_block<T, L, RB>()) {
+        const int r = i % RB, k = i / RB;
+        lds[pad(r * S + k)] = row0 + r < N ? src[(size_t)k * N + r] : czero<T>();
+    }
+    __syncthreads();
+    const int t = threadIdx.x % TH, b = threadIdx.x / TH;
+    const int gx = row0 + b, base = b * S;
+    const bool live = gx < N, even = (N & 1) == 0;
+    auto ld = [&](int e, int, int) -> cplx<T> {
+        const cplx<T> xk = lds[pad(base + e)], xc = cconj(lds[pad(base + L - e)]);
+        const cplx<T> ev = cadd(xk, xc);
+        const cplx<T> od = cmul(csub(xk, xc), cconj(tws.half[e]));
+        return {ev.x - od.y, ev.y + od.x};
+    };
+    T* dst = phi + ((size_t)c * N + (live ? gx : 0)) * N;
+    auto st = [&](int e, cplx<T> v, int, int r) {
+        if (2 * r >= RL) return;   // e >= L/2: samples 2e >= L >= N are padding
+        const int n0 = 2 * e;
+        if (!live || n0 >= N) return;
+        if (even) { V2 p; p.x = v.x * scale; p.y = v.y * scale; reinterpret_cast<V2*>(dst)[e] = p; }
+        else { dst[n0] = v.x * scale; if (n0 + 1 < N) dst[n0 + 1] = v.y * scale; }
+    };
+    fft_run<T, Pl, +1, TH, FN>(t, lds, base, tws.rev, ld, st);
+}
+
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
 // in each offset except x K2 (odd in a) and y K2 (odd in b). Values in double, rounded to float first when
 // round_kernel (the reference builds its tables in float32), transformed in T.
@@ -589,6 +684,48 @@ template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T*
     MG_HIP(hipGetLastError());
 }
 
+#define MG_FOLD_SIZES(X) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
+// rows per block of the folded row passes: 4 or 2 (FP32), 2 (FP64; its LDS holds no more)
+template <class T> int fold_rows(int want) { return sizeof(T) == 8 ? 2 : (want >= 4 ? 4 : 2); }
+
+template <class T> bool fold_supported(int M) {
+    switch (M / 2) {
+#define MG_C(L_) case L_: return true;
+        MG_FOLD_SIZES(MG_C)
+#undef MG_C
+        default: return false;
+    }
+}
+
+template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<T>* Wh, int rows, hipStream_t st) {
+    require(N <= M / 2, "FFT size below 2N");
+    const int rb = fold_rows<T>(rows);
+#define MG_R(L_, RB_) do { \
+        const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_fwd_fold<T, L_, RB_>, by); \
+        hipLaunchKernelGGL((k_rows_fwd_fold<T, L_, RB_>), dim3(xcd_grid(3 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
+                           W, N, Wh, fft_twiddle_set<T>(L_)); } while (0)
+#define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_R(L_, 4); else MG_R(L_, 2); } else MG_R(L_, 2); break;
+    switch (M / 2) { MG_FOLD_SIZES(MG_C) default: throw Error(ERR_UNSUP, "no folded row pass at this FFT size"); }
+#undef MG_C
+#undef MG_R
+    MG_HIP(hipGetLastError());
+}
+
+template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M, T* phi, int rows, hipStream_t st) {
+    require(N <= M / 2, "FFT size below 2N");
+    const int rb = fold_rows<T>(rows);
+    const T scale = (T)(1.0 / ((double)M * (double)M));
+#define MG_R(L_, RB_) do { \
+        const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold<T, L_, RB_>, by); \
+        hipLaunchKernelGGL((k_rows_inv_fold<T, L_, RB_>), dim3(xcd_grid(4 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
+                           Ph, N, phi, scale, fft_twiddle_set<T>(L_)); } while (0)
+#define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_R(L_, 4); else MG_R(L_, 2); } else MG_R(L_, 2); break;
+    switch (M / 2) { MG_FOLD_SIZES(MG_C) default: throw Error(ERR_UNSUP, "no folded row pass at this FFT size"); }
+#undef MG_C
+#undef MG_R
+    MG_HIP(hipGetLastError());
+}
+
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st) {
     require(N <= M / 2, "FFT size below 2N");
     const T scale = (T)(1.0 / ((double)M * (double)M));
@@ -738,10 +875,15 @@ template <class T> void prepare_fft_kernels() {
                    set_lds(k_spec1<T, L_, 2>, by); set_lds(k_spec1<T, L_, 4>, by); set_lds(k_spec2<T, L_, 2>, by); \
                    set_lds(k_spec2<T, L_, 4>, by); set_lds(k_spec2p<T, L_>, by); set_lds(kb_spec1<T, L_, 2>, by); \
                    set_lds(kb_spec2<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); }
+#define MG_F(L_) { set_lds(k_rows_fwd_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); set_lds(k_rows_inv_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); \
+                   if constexpr (sizeof(T) == 4) { set_lds(k_rows_fwd_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); \
+                                                   set_lds(k_rows_inv_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); } }
     MG_ROW_SIZES(MG_R)
     MG_COL_SIZES(MG_K)
+    MG_FOLD_SIZES(MG_F)
 #undef MG_R
 #undef MG_K
+#undef MG_F
 }
 
 #define MG_FFT_INST(T) \
@@ -750,6 +892,9 @@ template <class T> void prepare_fft_kernels() {
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
     template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t, bool); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
+    template bool fold_supported<T>(int); \
+    template void launch_rows_forward_fold<T>(const T*, int, int, cplx<T>*, int, hipStream_t); \
+    template void launch_rows_inverse_fold<T>(const cplx<T>*, int, int, T*, int, hipStream_t); \
     template void launch_spectra<T>(int, double, double, const double*, bool, int, cplx<T>*, T* const*, hipStream_t, bool); \
     template void launch_phi_interleave<T>(const T*, T*, i64, hipStream_t); \
     template void launch_rows_forward_wl<T>(const BInst<T>*, const int*, int, int, hipStream_t); \

@@ -267,6 +267,36 @@ def test_repulsion_every_fft_size(M, ni, nb):
     assert abs(s - so32) / so32 < 2e-6 and rel(F, Fo32) < 2e-6, (abs(s - so32) / so32, rel(F, Fo32))
 
 
+@pytest.mark.parametrize("M,ni,nb", [g for g in FFT_SIZE_GRIDS if g[0] in (512, 1536, 3072, 6144)] + [(8192, 4, 1000)])
+def test_repulsion_fft_layouts_identical(M, ni, nb):
+    """The three layouts of the 2-D FFT's middle -- the column pass in the row layout (strided), row passes that write
+    and read the column layout themselves (folded, 4 or 2 rows per block), row passes and transposes -- give the same
+    sumQ and forces bit for bit, FP32 and FP64, at the sizes where each applies."""
+    N = ni * nb
+    assert capi.fft_size(N) == M
+    rng = np.random.default_rng(M + 1)
+    span = nb * 0.7
+    Y = rng.uniform(0.05, span - 0.05, (3000, 2))
+    layouts = [dict(fft_strided_max=8192, fft_fold=4), dict(fft_strided_max=0, fft_fold=0),
+               dict(fft_strided_max=0, fft_fold=4), dict(fft_strided_max=0, fft_fold=2)]
+    saved = {k: capi.tune(k) for k in ("fft_strided_max", "fft_fold")}
+    try:
+        for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
+            outs = []
+            for lay in layouts:
+                for k, v in lay.items():
+                    capi.tune(k, v)
+                conv = capi.FFTConv(prec, 2 * N, ni)
+                conv.set_tsne(0.7)
+                s, F = conv.compute(dev(Y.astype(dt)), 0.0, 0.0, 0.7, nb)
+                outs.append((s, host(F)))
+            for s, F in outs[1:]:
+                assert s == outs[0][0] and np.array_equal(F, outs[0][1]), (M, dt)
+    finally:
+        for k, v in saved.items():
+            capi.tune(k, v)
+
+
 def test_repulsion_largest_fft_size():
     """M = 8192 (n_interp 4, 1000 boxes): FP64 against the exact O(n^2) sum at the interpolation error level
     of tests/test_magnet.py:221-231, and FP32 against FP64 on the same float32 layout."""

@@ -363,14 +363,15 @@ def test_trace_kl_and_gradient_norm_match_oracle():
 
 def test_tunables_change_no_result():
     """Every launch shape the autotuner may pick (the sorts' keys per thread and digit width, the column pass in the
-    row layout or between transposes, the attraction's LDS window on or off, its rows, window and threads per block)
-    gives the same state bit for bit, FP32 and FP64; the autotuner leaves one of them set."""
+    row layout or between folded row passes or transposes, the attraction's LDS window on or off, its rows, window and
+    threads per block) gives the same state bit for bit, FP32 and FP64; the autotuner leaves one of them set."""
     import torch
     from magnet.tune import autotune_loop, SPACE
     P, Y0, _ = problem()
     kw = dict(n_iter=500, early_iter=166)
     settings = [dict(sort_items=0), dict(sort_items=1), dict(sort_items=8), dict(attr_window=1, attr_rows=64, attr_win_kb=64,
-                attr_tpb=512), dict(attr_window=1, attr_rows=512, attr_win_kb=128, attr_tpb=1024), dict(attr_window=0), dict(fft_strided_max=0), dict(sort_digit=10)]
+                attr_tpb=512), dict(attr_window=1, attr_rows=512, attr_win_kb=128, attr_tpb=1024), dict(attr_window=0), dict(fft_strided_max=0), dict(fft_strided_max=0, fft_fold=0),
+                dict(fft_strided_max=0, fft_fold=2), dict(sort_digit=10)]
     try:
         for dt in (np.float32, np.float64):
             outs = []
