@@ -340,9 +340,11 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(cons
     fft_run<T, Pl, -1, TH, TO_LDS>(t, lds, b * S, tws.fwd, ld, none);
     __syncthreads();
     cplx<T>* dst = Wh + (size_t)c * Mh * N + row0;
-    for (int i = threadIdx.x; i < Mh * RB; i += fold_block<T, L, RB>()) {
-        const int r = i % RB, k = i / RB;
-        if (row0 + r < N) dst[(size_t)k * N + r] = r2c_post<T, L>(lds, r * S, k, tws.half);
+    constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB;
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {   // unrolled: the twiddle loads of every step are in flight together
+        const int i = threadIdx.x + u * NB, r = i % RB, k = i / RB;
+        if (i < Mh * RB && row0 + r < N) dst[(size_t)k * N + r] = r2c_post<T, L>(lds, r * S, k, tws.half);
     }
 }
 
@@ -362,9 +364,19 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(cons
     if (v < 0) return;
     const int c = v / G, row0 = (v % G) * RB;
     const cplx<T>* src = Ph + (size_t)c * Mh * N + row0;
-    for (int i = threadIdx.x; i < Mh * RB; i += fold_block<T, L, RB>()) {
-        const int r = i % RB, k = i / RB;
-        lds[pad(r * S + k)] = row0 + r < N ? src[(size_t)k * N + r] : czero<T>();
+    constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB;
+    {
+        cplx<T> x[IT];   // every segment load of the thread issued before the first LDS store
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int i = threadIdx.x + u * NB, r = i % RB, k = i / RB;
+            x[u] = i < Mh * RB && row0 + r < N ? src[(size_t)k * N + r] : czero<T>();
+        }
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int i = threadIdx.x + u * NB, r = i % RB, k = i / RB;
+            if (i < Mh * RB) lds[pad(r * S + k)] = x[u];
+        }
     }
     __syncthreads();
     const int t = threadIdx.x % TH, b = threadIdx.x / TH;
