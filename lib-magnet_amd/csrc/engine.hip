@@ -355,8 +355,9 @@ template <class T> struct Repulsion {
             MG_HIP(hipStreamWaitEvent(side, ev_go, 0));
             sst = side;
         }
-        // the spectra, which have slack until the column pass, are issued after the sort, the spread and the row
-        // pass, whose launches then reach the device first
+        // the spectra have slack until the column pass: they are issued first, after the box sort (default), or after
+        // the row pass (tunable spec_issue 0, 1, 2), so that the launches of the chain before them reach the device first
+        const int spec_at = side_run ? tunables().spec_issue : 0;
         Spec* sp = nullptr;
         auto issue_spectra = [&]() {
             P.begin(ST_SPECTRA, sst);
@@ -365,11 +366,12 @@ template <class T> struct Repulsion {
             P.end(ST_SPECTRA, sst, fresh ? ntab * Mh * Mh * ts : 0.0, M);   // tables written when the grid changed
             if (side_run) MG_HIP(hipEventRecord(ev_spec, side));
         };
-        if (!side_run) issue_spectra();
+        if (spec_at == 0) issue_spectra();
         RepBuffers<T> b = bufs();
         P.begin(ST_SORT, st);
         sorted_buf = launch_box_sort<T>(Y, n, g, b, st, inv);
         P.end(ST_SORT, st, 2 * dn * ts + 8 * dn);
+        if (spec_at == 1) issue_spectra();
         const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
         const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
         P.begin(ST_SPREAD, st);
@@ -388,10 +390,8 @@ template <class T> struct Repulsion {
             launch_transpose<T>(b.Tr, b.Wh, 3, g.N, M / 2 + 1, st);
             P.end(ST_TR_FWD, st, 0, M);
         }
-        if (side_run) {
-            issue_spectra();
-            MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
-        }
+        if (spec_at == 2) issue_spectra();
+        if (side_run) MG_HIP(hipStreamWaitEvent(st, ev_spec, 0));
         const Spec& s = *sp;
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
@@ -1376,6 +1376,7 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "sort_digit") { require(value == 8 || value == 10, "sort_digit must be 8 or 10"); g_tunables.sort_digit = value; }
     else if (k == "fft_strided_max") { require(value >= 0 && value <= 8192, "fft_strided_max must be in [0, 8192]"); g_tunables.fft_strided_max = value; }
     else if (k == "fft_fold") { require(value == 0 || value == 2 || value == 4, "fft_fold must be 0, 2 or 4"); g_tunables.fft_fold = value; }
+    else if (k == "spec_issue") { require(value >= 0 && value <= 2, "spec_issue must be 0, 1 or 2"); g_tunables.spec_issue = value; }
     else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
     else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
     else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }
@@ -1397,6 +1398,7 @@ int mg_tune_get(const char* name, int* value) {
     else if (k == "sort_digit") *value = g_tunables.sort_digit;
     else if (k == "fft_strided_max") *value = g_tunables.fft_strided_max;
     else if (k == "fft_fold") *value = g_tunables.fft_fold;
+    else if (k == "spec_issue") *value = g_tunables.spec_issue;
     else if (k == "batch_graph") *value = g_tunables.batch_graph;
     else if (k == "batch_chunk") *value = g_tunables.batch_chunk;
     else if (k == "batch_window_down") *value = g_tunables.batch_window_down;

@@ -371,7 +371,7 @@ def test_tunables_change_no_result():
     kw = dict(n_iter=500, early_iter=166)
     settings = [dict(sort_items=0), dict(sort_items=1), dict(sort_items=8), dict(attr_window=1, attr_rows=64, attr_win_kb=64,
                 attr_tpb=512), dict(attr_window=1, attr_rows=512, attr_win_kb=128, attr_tpb=1024), dict(attr_window=0), dict(fft_strided_max=0), dict(fft_strided_max=0, fft_fold=0),
-                dict(fft_strided_max=0, fft_fold=2), dict(sort_digit=10)]
+                dict(fft_strided_max=0, fft_fold=2), dict(sort_digit=10), dict(spec_issue=0), dict(spec_issue=2)]
     try:
         for dt in (np.float32, np.float64):
             outs = []
