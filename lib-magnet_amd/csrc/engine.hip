@@ -701,6 +701,11 @@ template <class T> struct Optimizer {
 // run one launch per FFT size in use.
 static int bits_for(long long v) { int b = 0; while ((1ll << b) < v) ++b; return b; }
 
+// Groups plan and finish from several host threads. A stream capture must not overlap a call that may synchronise the
+// device (hipFree, graph destruction) or allocate in another thread, so plan() -- its allocations, staging and
+// capture -- and the releases of finish() hold this lock.
+static std::mutex g_graph_mu;
+
 // arena growth with headroom; the old allocation goes to `dead` (freed after the run's last synchronisation)
 static void grow(DevBuf& b, size_t need, std::vector<void*>& dead) { if (need > b.bytes) b.ensure_keep(need + need / 4, dead); }
 
@@ -953,6 +958,7 @@ template <class T> struct BatchEngine {
     // the last plan did not use, so the iterations still queued keep theirs); then the captured iteration. Nothing
     // here waits for the device: replaced buffers stay allocated until the run ends.
     void plan() {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
         const double t0 = now_s();
         tcur ^= 1;
         Mlo.assign(count, 0); Mhi.assign(count, 0);
@@ -1064,10 +1070,7 @@ template <class T> struct BatchEngine {
         L.update = make_maps(all, [&](int i) { return update_blocks(n[i]); });
         if (exec || graph) old_graphs.push_back({graph, exec});
         graph = nullptr; exec = nullptr;
-        if (use_graph) {
-            // one capture at a time in the process (groups plan from several host threads)
-            static std::mutex capture_mu;
-            std::lock_guard<std::mutex> lk(capture_mu);
+        if (use_graph) {   // one capture at a time in the process (g_graph_mu)
             MG_HIP(hipStreamBeginCapture(cst, hipStreamCaptureModeRelaxed));
             try {
                 iteration(cst);
@@ -1189,6 +1192,7 @@ template <class T> struct BatchEngine {
         MG_HIP(hipStreamSynchronize(st));
         MG_HIP(hipStreamSynchronize(aside));
         pending.clear();
+        std::lock_guard<std::mutex> lk(g_graph_mu);
         drop_graphs();
         release_dead();
     }

@@ -293,9 +293,11 @@ __global__ __launch_bounds__(fft_block<L>()) void kw_rows_inv(const BInst<T>* __
 }
 
 // ---- the row passes with the transposes folded in (grids above the strided column pass). A block takes RB
-// consecutive rows of one channel, its RB transforms in LDS at element offsets r S (S = L + 16: each buffer starts
-// 17 pad slots further, which turns the banks of the RB buffers against each other), and writes or reads the
-// column layout (channel, ky, row) in segments of RB rows (32 bytes for RB x 8 and RB x 16 bytes). Blocks are
+// consecutive rows of one channel, its RB transforms in LDS at element offsets r S, and writes or reads the column
+// layout (channel, ky, row) in segments of RB rows (32 bytes for RB x 8 and RB x 16 bytes). S = L + 16 m makes the
+// padded starts of consecutive buffers differ by an odd multiple of 8 elements modulo 32 (the pad adds one element
+// per 16, so buffer r starts at r (17 L / 16 + 17 m)): the 8 columns x 4 rows of FP32 elements (or 8 x 2 of FP64)
+// that one half-wave stores or loads then fall on distinct banks. Blocks are
 // numbered so that each of the 8 XCDs (the dispatcher hands consecutive blocks to them in turn) takes a contiguous
 // run of row groups: the blocks writing or reading the other segments of a 128-byte line then run in the same L2
 // at about the same time. The arithmetic is the row kernels'.
@@ -307,14 +309,19 @@ __device__ __forceinline__ int xcd_block(int total) {   // the block's index in 
     return v < total ? v : -1;
 }
 template <class T, int L, int RB> constexpr int fold_block() { return RB * fft_th<L>(); }
-template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded(RB * (L + 16)) * (int)sizeof(cplx<T>); }
+// the smaller m with 17 (L / 16 + m) = 8 or 24 (mod 32); 17 is its own inverse modulo 32
+constexpr int fold_m(int L, int target) { return (((target - (17 * L / 16) % 32) % 32 + 32) % 32) * 17 % 32; }
+template <int L> constexpr int fold_stride() { return L + 16 * (fold_m(L, 8) < fold_m(L, 24) ? fold_m(L, 8) : fold_m(L, 24)); }
+static_assert((17 * fold_stride<3072>() / 16) % 16 == 8 && (17 * fold_stride<768>() / 16) % 16 == 8 &&
+              (17 * fold_stride<256>() / 16) % 16 == 8, "row buffers an odd multiple of 8 elements apart modulo 32");
+template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded(RB * fold_stride<L>()) * (int)sizeof(cplx<T>); }
 
 // W (3, N, N) -> Wh (3, Mh, N): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments
 template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(const T* __restrict__ W, int N, cplx<T>* __restrict__ Wh,
                                                                          TwSet<T> tws) {
     using Pl = typename FwdPlan<T, L>::type;
-    constexpr int TH = fft_th<L>(), Mh = L + 1, R0 = Pl::R[0], S = L + 16;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, R0 = Pl::R[0], S = fold_stride<L>();
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
@@ -354,7 +361,7 @@ template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(const cplx<T>* __restrict__ Ph, int N, T* __restrict__ phi,
                                                                          T scale, TwSet<T> tws) {
     using Pl = Rev<typename FwdPlan<T, L>::type>;
-    constexpr int TH = fft_th<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1], S = L + 16;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1], S = fold_stride<L>();
     static_assert(Pl::P > 1, "the first pass must read all its inputs before it writes the buffer");
     typedef typename vec2<T>::type V2;
     extern __shared__ __align__(16) unsigned char smem[];
