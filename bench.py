@@ -226,13 +226,14 @@ def precision_run(P, n_iter, early, steps, warmup, profile=True, cpu=False, thre
     if warmup > 0:
         l32.run(l32.state(Y0), 0, min(warmup, n_iter))
     snaps, full32 = checkpoint_run(l32, Y0, starts + match_starts)
-    # launch shapes that change no result, picked by timing 10 iterations of the late phase (magnet.tune)
+    # launch shapes that change no result, picked by timing 10 iterations from the state at 30% of the schedule
+    # (where the grid is largest while it clips) and 10 from the late phase, together (magnet.tune)
     from magnet.tune import autotune_loop
-    t_tune = match_starts[-1]
-    chosen, tune_log = autotune_loop(l32.sess, snaps[t_tune], t_tune, iters=10)
+    t_tune = [int(round(0.3 * (n_iter - 10))), match_starts[-1]]
+    chosen, tune_log = autotune_loop(l32.sess, [(snaps[t], t) for t in t_tune], None, iters=10, repeats=3)
     sec32, _ = timed_segments(l32, snaps, segs)
     out = {"segments": segs, "seconds": sec32, "full32_s": full32,
-           "autotune": {"at_iteration": t_tune, "chosen": chosen, "sweep_s_per_10_iters": tune_log}}
+           "autotune": {"at_iterations": t_tune, "chosen": chosen, "sweep_s_per_2x10_iters": tune_log}}
     if profile:
         _, out["stages"] = timed_segments(l32, snaps, segs, profile=True)
     l64 = Loop(P, "fp64", n_iter, early)

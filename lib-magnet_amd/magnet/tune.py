@@ -49,11 +49,15 @@ def autotune(run, space=SPACE, repeats=2):
     return chosen, log
 
 
-def autotune_loop(session, state, it0, iters=10, space=SPACE):
+def autotune_loop(session, state, it0, iters=10, space=SPACE, repeats=2):
     """autotune over `iters` iterations [it0, it0 + iters) of a capi.Session from copies of `state` (Y, U, G device
-    tensors); the state itself is not moved."""
+    tensors); the state itself is not moved. `state` may also be a list of (state, it0) pairs: each candidate is
+    then timed over `iters` iterations from every one of them (phases whose grids differ weigh by their cost)."""
+    pairs = state if isinstance(state, list) else [(state, it0)]
+
     def run():
-        s = tuple(t.clone() for t in state)
-        session.run(*s, it0, it0 + iters)
-    run()   # warm the session's buffers and spectra for this stretch
-    return autotune(run, space)
+        for st, t0 in pairs:
+            s = tuple(t.clone() for t in st)
+            session.run(*s, t0, t0 + iters)
+    run()   # warm the session's buffers and spectra for these stretches
+    return autotune(run, space, repeats=repeats)
