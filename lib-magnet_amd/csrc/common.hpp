@@ -292,12 +292,15 @@ template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int b
 // K: spectra tables K1, K2 (and x K2, y K2 when centred), each (M/2+1, M/2+1), [ky][kx <= M/2]. Wh (3, Mh, N),
 // Ph (4, Mh, N). centred must be true for float and false for double.
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
-                                       hipStream_t st, bool strided = false);
+                                       hipStream_t st, bool strided = false, int LD = 0);
 // strided: Wh is the row pass's (3, N, Mh) output and Ph receives (4, N, Mh) for the inverse row pass, with no
-// transposes (the same values)
+// transposes (the same values); otherwise column ky of channel c starts at (c Mh + ky) LD (LD 0: N)
 template <class T> void launch_rows_inverse(const cplx<T>* X, int N, int M, T* phi, hipStream_t st);   // X (4, N, Mh)
-// the row passes with the transposes folded in: W (3, N, N) -> Wh (3, Mh, N) and Ph (4, Mh, N) -> phi (4, N, N), the
-// same values as a row pass and a transpose; `rows` rows per block (4 or 2; FP64 takes 2); padded sizes 512-8192
+// the row passes with the transposes folded in: W (3, N, N) -> Wh (3, Mh, LD) and Ph (4, Mh, LD) -> phi (4, N, N), the
+// same values as a row pass and a transpose; `rows` rows per block (4 or 2; FP64 takes 2); padded sizes 512-8192. The
+// column layout's stride LD = fold_stride_n(N) rounds N up to 4 rows, so every 32-byte segment is aligned (pass the
+// same LD to launch_columns; LD <= M/2 holds, M/2 being a multiple of 4)
+inline int fold_stride_n(int N) { return (N + 3) & ~3; }
 template <class T> bool fold_supported(int M);
 template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<T>* Wh, int rows, hipStream_t st);
 template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M, T* phi, int rows, hipStream_t st);

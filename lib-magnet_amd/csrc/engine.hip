@@ -395,7 +395,7 @@ template <class T> struct Repulsion {
         const Spec& s = *sp;
         P.begin(ST_COLUMNS, st);
         const T* Kc[4] = {dptr<T>(s.K[0]), dptr<T>(s.K[1]), dptr<T>(s.K[2]), dptr<T>(s.K[3])};
-        launch_columns<T>(strided ? b.Tr : b.Wh, g.N, M, Kc, centred, b.Ph, st, strided);
+        launch_columns<T>(strided ? b.Tr : b.Wh, g.N, M, Kc, centred, b.Ph, st, strided, folded ? fold_stride_n(g.N) : 0);
         P.end(ST_COLUMNS, st, 3 * Mh * g.N * cs + ntab * Mh * Mh * ts + 4 * Mh * g.N * cs, M);
         if (!strided && !folded) {
             P.begin(ST_TR_INV, st);

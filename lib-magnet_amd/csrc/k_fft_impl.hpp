@@ -150,12 +150,13 @@ template <class T, int M> __device__ __forceinline__ T ktab(const T* __restrict_
 // registers (slot (q, r) of thread t holds frequency t + q TH + r M / 16); each output is a product formed
 // in those registers and transformed back by the reversed plan, with no LDS round trip in between.
 // STR: the input and output stay in the row passes' layout, (3, N, Mh) in and (4, N, Mh) out, read and written with
-// stride Mh (for grids small enough to sit in L2); no transposes then, and the same values
+// stride Mh (for grids small enough to sit in L2); no transposes then, and the same values. Otherwise column ky of
+// channel c starts at (c Mh + ky) LD (LD >= N: the folded row passes pad it to whole 32-byte segments)
 template <class T, int L, bool CEN, bool STR = false>
 __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, int N,
                                               const T* __restrict__ K1, const T* __restrict__ K2,
                                               const T* __restrict__ TX, const T* __restrict__ TY,
-                                              cplx<T>* __restrict__ Ph, const TwSet<T>& tws, int bx) {
+                                              cplx<T>* __restrict__ Ph, const TwSet<T>& tws, int bx, int LD) {
     using F = typename FwdPlan<T, L>::type;
     using I = Rev<F>;
     constexpr int TH = fft_th<L>(), B = fft_batch<L>(), Mh = L / 2 + 1, R0 = F::R[0], RL = F::R[F::P - 1];
@@ -169,7 +170,7 @@ __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, in
     const int kyc = live ? ky : 0;
     cplx<T> s0[QL][RL], v[QL][RL];
     auto fwd = [&](int ch, cplx<T> (&dst)[QL][RL]) {
-        const cplx<T>* col = STR ? Wh + (size_t)ch * N * Mh + kyc : Wh + ((size_t)ch * Mh + kyc) * N;
+        const cplx<T>* col = STR ? Wh + (size_t)ch * N * Mh + kyc : Wh + ((size_t)ch * Mh + kyc) * LD;
         auto ld = [&](int e, int, int r) -> cplx<T> {
             if (2 * r >= R0 || e >= N) return czero<T>();   // rows >= N (all of r >= R0/2) are padding
             return STR ? col[(size_t)e * Mh] : col[e];
@@ -178,7 +179,7 @@ __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, in
         fft_run<T, F, -1, TH, FN>(t, lds, b * L, tws.fwd, ld, st);
     };
     auto inv = [&](int o, cplx<T> (&src)[QL][RL]) {
-        cplx<T>* dst = STR ? Ph + (size_t)o * N * Mh + kyc : Ph + ((size_t)o * Mh + kyc) * N;
+        cplx<T>* dst = STR ? Ph + (size_t)o * N * Mh + kyc : Ph + ((size_t)o * Mh + kyc) * LD;
         auto ld = [&](int, int q, int r) { return src[q][r]; };
         auto st = [&](int e, cplx<T> x, int, int r) {
             if (2 * r < R0 && e < N && live) { if (STR) dst[(size_t)e * Mh] = x; else dst[e] = x; }   // only the N live rows
@@ -223,8 +224,8 @@ template <class T, int L, bool CEN, bool STR>
 __global__ __launch_bounds__(fft_block<L>()) void k_columns(const cplx<T>* __restrict__ Wh, int N,
                                                             const T* __restrict__ K1, const T* __restrict__ K2,
                                                             const T* __restrict__ TX, const T* __restrict__ TY,
-                                                            cplx<T>* __restrict__ Ph, TwSet<T> tws) {
-    columns_block<T, L, CEN, STR>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x);
+                                                            cplx<T>* __restrict__ Ph, TwSet<T> tws, int LD) {
+    columns_block<T, L, CEN, STR>(Wh, N, K1, K2, TX, TY, Ph, tws, blockIdx.x, LD);
 }
 template <class T, int L, bool CEN>
 __global__ __launch_bounds__(fft_block<L>()) void kw_columns(const BInst<T>* __restrict__ I, const int* __restrict__ wl,
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(fft_block<L>()) void kw_columns(const BInst<T>* __r
         const cplx<T>* Wh = b.Wh;
         cplx<T>* Ph = b.Ph;
         const T *K0 = b.K[0], *K1 = b.K[1], *K2 = b.K[2], *K3 = b.K[3];
-        columns_block<T, L, CEN>(Wh, b.g.N, K0, K1, K2, K3, Ph, tws, v - wl_start(wl, k));
+        columns_block<T, L, CEN>(Wh, b.g.N, K0, K1, K2, K3, Ph, tws, v - wl_start(wl, k), b.g.N);
         __syncthreads();
     }
 }
@@ -319,7 +320,7 @@ template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded
 // W (3, N, N) -> Wh (3, Mh, N): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments
 template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(const T* __restrict__ W, int N, cplx<T>* __restrict__ Wh,
-                                                                         TwSet<T> tws) {
+                                                                         int LD, TwSet<T> tws) {
     using Pl = typename FwdPlan<T, L>::type;
     constexpr int TH = fft_th<L>(), Mh = L + 1, R0 = Pl::R[0], S = fold_stride<L>();
     typedef typename vec2<T>::type V2;
@@ -346,20 +347,20 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(cons
     auto none = [](int, cplx<T>, int, int) {};
     fft_run<T, Pl, -1, TH, TO_LDS>(t, lds, b * S, tws.fwd, ld, none);
     __syncthreads();
-    cplx<T>* dst = Wh + (size_t)c * Mh * N + row0;
+    cplx<T>* dst = Wh + (size_t)c * Mh * LD + row0;
     constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB;
 #pragma unroll
     for (int u = 0; u < IT; ++u) {   // unrolled: the twiddle loads of every step are in flight together
         const int i = threadIdx.x + u * NB, r = i % RB, k = i / RB;
-        if (i < Mh * RB && row0 + r < N) dst[(size_t)k * N + r] = r2c_post<T, L>(lds, r * S, k, tws.half);
+        if (i < Mh * RB && row0 + r < N) dst[(size_t)k * LD + r] = r2c_post<T, L>(lds, r * S, k, tws.half);
     }
 }
 
 // Ph (4, Mh, N) -> phi (4, N, N): the Mh inputs of the RB rows by segments into LDS, then rows_inv_block's transform
 // per row with its inputs from there
 template <class T, int L, int RB>
-__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(const cplx<T>* __restrict__ Ph, int N, T* __restrict__ phi,
-                                                                         T scale, TwSet<T> tws) {
+__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(const cplx<T>* __restrict__ Ph, int N, int LD,
+                                                                         T* __restrict__ phi, T scale, TwSet<T> tws) {
     using Pl = Rev<typename FwdPlan<T, L>::type>;
     constexpr int TH = fft_th<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1], S = fold_stride<L>();
     static_assert(Pl::P > 1, "the first pass must read all its inputs before it writes the buffer");
@@ -370,14 +371,14 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(cons
     const int v = xcd_block(4 * G);
     if (v < 0) return;
     const int c = v / G, row0 = (v % G) * RB;
-    const cplx<T>* src = Ph + (size_t)c * Mh * N + row0;
+    const cplx<T>* src = Ph + (size_t)c * Mh * LD + row0;
     constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB;
     {
         cplx<T> x[IT];   // every segment load of the thread issued before the first LDS store
 #pragma unroll
         for (int u = 0; u < IT; ++u) {
             const int i = threadIdx.x + u * NB, r = i % RB, k = i / RB;
-            x[u] = i < Mh * RB && row0 + r < N ? src[(size_t)k * N + r] : czero<T>();
+            x[u] = i < Mh * RB && row0 + r < N ? src[(size_t)k * LD + r] : czero<T>();
         }
 #pragma unroll
         for (int u = 0; u < IT; ++u) {
@@ -687,15 +688,17 @@ template <class T> void launch_transpose(const cplx<T>* src, cplx<T>* dst, int b
 }
 
 template <class T> void launch_columns(const cplx<T>* Wh, int N, int M, const T* const* K, bool centred, cplx<T>* Ph,
-                                       hipStream_t st, bool strided) {
+                                       hipStream_t st, bool strided, int LD) {
     require(N <= M / 2, "FFT size below 2N");
+    if (LD <= 0) LD = N;
+    require(LD >= N && LD <= M / 2, "column stride outside [N, M/2]");
     require(centred == (sizeof(T) == 4), "the FP32 path uses the centred product set, the FP64 path the reference's");
     constexpr bool CEN = sizeof(T) == 4;
     const int Mh = M / 2 + 1;
 #define MG_L(L_, S_) do { \
         const int by = lds_bytes<T, L_>(); set_lds(k_columns<T, L_, CEN, S_>, by); \
         hipLaunchKernelGGL((k_columns<T, L_, CEN, S_>), dim3((Mh + fft_batch<L_>() - 1) / fft_batch<L_>()), dim3(fft_block<L_>()), by, st, \
-                           Wh, N, K[0], K[1], K[2], K[3], Ph, fft_twiddle_set<T>(L_)); } while (0)
+                           Wh, N, K[0], K[1], K[2], K[3], Ph, fft_twiddle_set<T>(L_), LD); } while (0)
 #define MG_C(L_) case L_: if (strided) MG_L(L_, true); else MG_L(L_, false); break;
     switch (M) { MG_COL_SIZES(MG_C) default: throw Error(ERR_UNSUP, "FFT size not supported"); }
 #undef MG_C
@@ -718,11 +721,12 @@ template <class T> bool fold_supported(int M) {
 
 template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<T>* Wh, int rows, hipStream_t st) {
     require(N <= M / 2, "FFT size below 2N");
+    const int LD = fold_stride_n(N);
     const int rb = fold_rows<T>(rows);
 #define MG_R(L_, RB_) do { \
         const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_fwd_fold<T, L_, RB_>, by); \
         hipLaunchKernelGGL((k_rows_fwd_fold<T, L_, RB_>), dim3(xcd_grid(3 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
-                           W, N, Wh, fft_twiddle_set<T>(L_)); } while (0)
+                           W, N, Wh, LD, fft_twiddle_set<T>(L_)); } while (0)
 #define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_R(L_, 4); else MG_R(L_, 2); } else MG_R(L_, 2); break;
     switch (M / 2) { MG_FOLD_SIZES(MG_C) default: throw Error(ERR_UNSUP, "no folded row pass at this FFT size"); }
 #undef MG_C
@@ -732,12 +736,13 @@ template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<
 
 template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M, T* phi, int rows, hipStream_t st) {
     require(N <= M / 2, "FFT size below 2N");
+    const int LD = fold_stride_n(N);
     const int rb = fold_rows<T>(rows);
     const T scale = (T)(1.0 / ((double)M * (double)M));
 #define MG_R(L_, RB_) do { \
         const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold<T, L_, RB_>, by); \
         hipLaunchKernelGGL((k_rows_inv_fold<T, L_, RB_>), dim3(xcd_grid(4 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
-                           Ph, N, phi, scale, fft_twiddle_set<T>(L_)); } while (0)
+                           Ph, N, LD, phi, scale, fft_twiddle_set<T>(L_)); } while (0)
 #define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_R(L_, 4); else MG_R(L_, 2); } else MG_R(L_, 2); break;
     switch (M / 2) { MG_FOLD_SIZES(MG_C) default: throw Error(ERR_UNSUP, "no folded row pass at this FFT size"); }
 #undef MG_C
@@ -909,7 +914,7 @@ template <class T> void prepare_fft_kernels() {
     template void prepare_fft_kernels<T>(); \
     template void launch_rows_forward<T>(const T*, int, int, cplx<T>*, hipStream_t); \
     template void launch_transpose<T>(const cplx<T>*, cplx<T>*, int, int, int, hipStream_t); \
-    template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t, bool); \
+    template void launch_columns<T>(const cplx<T>*, int, int, const T* const*, bool, cplx<T>*, hipStream_t, bool, int); \
     template void launch_rows_inverse<T>(const cplx<T>*, int, int, T*, hipStream_t); \
     template bool fold_supported<T>(int); \
     template void launch_rows_forward_fold<T>(const T*, int, int, cplx<T>*, int, hipStream_t); \

@@ -170,10 +170,11 @@ class FFTConv:
         self.h = C.c_void_p()
         self.prec = prec
         check(lib().mg_fftconv_create(C.byref(self.h), prec, M, n_interp))
+        self._destroy = lib().mg_fftconv_destroy   # kept: module globals may be gone when the interpreter exits
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mg_fftconv_destroy(self.h)
+        if getattr(self, "h", None) and getattr(self, "_destroy", None):
+            self._destroy(self.h)
             self.h = None
 
     def set_tsne(self, width, dof=1.0):
