@@ -407,6 +407,95 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(cons
     fft_run<T, Pl, +1, TH, FN>(t, lds, base, tws.rev, ld, st);
 }
 
+// The inverse pass, persistent: one block per CU walks its XCD's run of row groups (the same order as above). Its
+// twiddle tables sit in LDS, so the transforms issue no global load, and the next group's segment loads, held in
+// registers, are in flight while the current group's transforms run (loads complete in order, so a global twiddle
+// load inside the transforms would wait for them). For the sizes whose buffers and tables fit in LDS.
+template <class T, int L, int RB> constexpr int fold_p_lds_bytes() {
+    return (padded(RB * fold_stride<L>()) + plan_tw_size<Rev<typename FwdPlan<T, L>::type>>() + L + 1) * (int)sizeof(cplx<T>);
+}
+template <class T, int L, int RB> constexpr bool fold_p_fits() { return fold_p_lds_bytes<T, L, RB>() <= 160 * 1024; }
+
+template <class T, int L, int RB>
+__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold_p(const cplx<T>* __restrict__ Ph, int N, int LD,
+                                                                           T* __restrict__ phi, T scale, TwSet<T> tws) {
+    using Pl = Rev<typename FwdPlan<T, L>::type>;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, RL = Pl::R[Pl::P - 1], S = fold_stride<L>();
+    constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB, NTW = plan_tw_size<Pl>();
+    static_assert(Pl::P > 1, "the first pass must read all its inputs before it writes the buffer");
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* tw_l = lds + padded(RB * S);
+    cplx<T>* half_l = tw_l + NTW;
+    const int G = (N + RB - 1) / RB, total = 4 * G;
+    const int per = (int)gridDim.x / kXcds, x = (int)blockIdx.x % kXcds, GX = (total + kXcds - 1) / kXcds;
+    int jj = (int)blockIdx.x / kXcds;
+    auto valid = [&](int j) { return j < GX && x * GX + j < total; };
+    if (!valid(jj)) return;   // uniform over the block
+    for (int i = threadIdx.x; i < NTW; i += NB) tw_l[i] = tws.rev[i];
+    for (int i = threadIdx.x; i <= L; i += NB) half_l[i] = tws.half[i];
+    cplx<T> xr[IT];
+    auto load = [&](int v) {
+        const int c = v / G, row0 = (v % G) * RB;
+        const cplx<T>* src = Ph + (size_t)c * Mh * LD + row0;
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));   // per call: the slot addresses are not kept live across the loop
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int i = tid + u * NB, r = i % RB, k = i / RB;
+            xr[u] = i < Mh * RB && row0 + r < N ? src[(size_t)k * LD + r] : czero<T>();
+        }
+    };
+    load(x * GX + jj);
+    for (;;) {
+        const int v = x * GX + jj;
+        const int c = v / G, row0 = (v % G) * RB;
+        {
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+#pragma unroll
+            for (int u = 0; u < IT; ++u) {
+                const int i = tid + u * NB, r = i % RB, k = i / RB;
+                if (i < Mh * RB) lds[pad(r * S + k)] = xr[u];
+            }
+        }
+        __syncthreads();   // also orders the twiddle copies before their first use
+        const int jn = jj + per;
+        const bool more = valid(jn);
+        if (more) load(x * GX + jn);
+        {
+            // an offset the compiler cannot see through, renewed per group: the twiddle reads stay inside the loop
+            // instead of being hoisted out of it and held in registers across every group
+            int opaque = 0, tid = threadIdx.x;
+            asm volatile("" : "+v"(opaque), "+v"(tid));
+            const cplx<T>* twg = tw_l + opaque;
+            const cplx<T>* halfg = half_l + opaque;
+            const int t = tid % TH, b = tid / TH;
+            const int gx = row0 + b, base = b * S;
+            const bool live = gx < N, even = (N & 1) == 0;
+            auto ld = [&](int e, int, int) -> cplx<T> {
+                const cplx<T> xk = lds[pad(base + e)], xc = cconj(lds[pad(base + L - e)]);
+                const cplx<T> ev = cadd(xk, xc);
+                const cplx<T> od = cmul(csub(xk, xc), cconj(halfg[e]));
+                return {ev.x - od.y, ev.y + od.x};
+            };
+            T* dst = phi + ((size_t)c * N + (live ? gx : 0)) * N;
+            auto st = [&](int e, cplx<T> w, int, int r) {
+                if (2 * r >= RL) return;   // e >= L/2: samples 2e >= L >= N are padding
+                const int n0 = 2 * e;
+                if (!live || n0 >= N) return;
+                if (even) { V2 p; p.x = w.x * scale; p.y = w.y * scale; reinterpret_cast<V2*>(dst)[e] = p; }
+                else { dst[n0] = w.x * scale; if (n0 + 1 < N) dst[n0 + 1] = w.y * scale; }
+            };
+            fft_run<T, Pl, +1, TH, FN>(t, lds, base, twg, ld, st);
+        }
+        __syncthreads();   // the buffers are staged again
+        if (!more) break;
+        jj = jn;
+    }
+}
+
 // ---- kernel spectra. Offsets (a, b) on the M x M torus, a along x (gx), b along y (gy); every table is even
 // in each offset except x K2 (odd in a) and y K2 (odd in b). Values in double, rounded to float first when
 // round_kernel (the reference builds its tables in float32), transformed in T.
@@ -739,6 +828,25 @@ template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M
     const int LD = fold_stride_n(N);
     const int rb = fold_rows<T>(rows);
     const T scale = (T)(1.0 / ((double)M * (double)M));
+    if (tunables().fft_fold_persist) {
+        // one block per CU (the LDS holds one), at most as many as row groups per XCD, times the 8 XCDs
+        int cus = 256;
+        {
+            int dev = 0, n = 0;
+            if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+                cus = n;
+        }
+#define MG_P(L_, RB_) do { if constexpr (fold_p_fits<T, L_, RB_>()) { \
+            const int G = (N + RB_ - 1) / RB_, GX = (4 * G + kXcds - 1) / kXcds, per = std::min(GX, std::max(1, cus / kXcds)); \
+            const int by = fold_p_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold_p<T, L_, RB_>, by); \
+            hipLaunchKernelGGL((k_rows_inv_fold_p<T, L_, RB_>), dim3(per * kXcds), dim3(fold_block<T, L_, RB_>()), by, st, \
+                               Ph, N, LD, phi, scale, fft_twiddle_set<T>(L_)); \
+            MG_HIP(hipGetLastError()); return; } } while (0)
+#define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_P(L_, 4); else MG_P(L_, 2); } else MG_P(L_, 2); break;
+        switch (M / 2) { MG_FOLD_SIZES(MG_C) default: break; }
+#undef MG_C
+#undef MG_P
+    }
 #define MG_R(L_, RB_) do { \
         const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold<T, L_, RB_>, by); \
         hipLaunchKernelGGL((k_rows_inv_fold<T, L_, RB_>), dim3(xcd_grid(4 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
@@ -900,8 +1008,11 @@ template <class T> void prepare_fft_kernels() {
                    set_lds(k_spec2<T, L_, 4>, by); set_lds(k_spec2p<T, L_>, by); set_lds(kb_spec1<T, L_, 2>, by); \
                    set_lds(kb_spec2<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); }
 #define MG_F(L_) { set_lds(k_rows_fwd_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); set_lds(k_rows_inv_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); \
+                   if constexpr (fold_p_fits<T, L_, 2>()) set_lds(k_rows_inv_fold_p<T, L_, 2>, fold_p_lds_bytes<T, L_, 2>()); \
                    if constexpr (sizeof(T) == 4) { set_lds(k_rows_fwd_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); \
-                                                   set_lds(k_rows_inv_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); } }
+                                                   set_lds(k_rows_inv_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); \
+                                                   if constexpr (fold_p_fits<T, L_, 4>()) \
+                                                       set_lds(k_rows_inv_fold_p<T, L_, 4>, fold_p_lds_bytes<T, L_, 4>()); } }
     MG_ROW_SIZES(MG_R)
     MG_COL_SIZES(MG_K)
     MG_FOLD_SIZES(MG_F)

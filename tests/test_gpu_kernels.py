@@ -271,17 +271,18 @@ def test_repulsion_every_fft_size(M, ni, nb):
                          + [(1536, 3, 255), (3072, 3, 511), (6144, 3, 999)])
 def test_repulsion_fft_layouts_identical(M, ni, nb):
     """The three layouts of the 2-D FFT's middle -- the column pass in the row layout (strided), row passes that write
-    and read the column layout themselves (folded, 4 or 2 rows per block, its column stride padded to 4 rows), row
-    passes and transposes -- give the same sumQ and forces bit for bit, FP32 and FP64, at the sizes where each applies
-    (grid sides N that are and are not multiples of 4)."""
+    and read the column layout themselves (folded, 4 or 2 rows per block, its column stride padded to 4 rows; the
+    inverse pass persistent or one block per row group), row passes and transposes -- give the same sumQ and forces bit
+    for bit, FP32 and FP64, at the sizes where each applies (grid sides N that are and are not multiples of 4)."""
     N = ni * nb
     assert capi.fft_size(N) == M
     rng = np.random.default_rng(M + 1)
     span = nb * 0.7
     Y = rng.uniform(0.05, span - 0.05, (3000, 2))
     layouts = [dict(fft_strided_max=8192, fft_fold=4), dict(fft_strided_max=0, fft_fold=0),
-               dict(fft_strided_max=0, fft_fold=4), dict(fft_strided_max=0, fft_fold=2)]
-    saved = {k: capi.tune(k) for k in ("fft_strided_max", "fft_fold")}
+               dict(fft_strided_max=0, fft_fold=4), dict(fft_strided_max=0, fft_fold=2),
+               dict(fft_strided_max=0, fft_fold=4, fft_fold_persist=0), dict(fft_strided_max=0, fft_fold=2, fft_fold_persist=0)]
+    saved = {k: capi.tune(k) for k in ("fft_strided_max", "fft_fold", "fft_fold_persist")}
     try:
         for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
             outs = []
