@@ -17,7 +17,8 @@
 //
 // Twiddles: per plan and pass, a table of exp(-2 pi i k 2^b / (Ns R)) for k < Ns and 2^b < R in global memory
 // (consecutive k on consecutive lanes: coalesced, L1-resident), correctly rounded from long double on the
-// host. A pass reads w^(2^b) and forms the other powers with products (at most three deep).
+// host. A pass reads w^(2^b) and forms the other powers with products (at most three deep): in FP32 each from an
+// earlier power, in FP64 each from the lookups of its set bits, so that only the lookups stay live.
 #pragma once
 #include "common.hpp"
 #include <utility>
@@ -128,18 +129,36 @@ constexpr int log2c(int r) { int l = 0; while ((1 << l) < r) ++l; return l; }
 template <class T, int R, int Ns, int SIGN>
 __device__ __forceinline__ void twiddle_apply(cplx<T>* a, const cplx<T>* __restrict__ tab, int k) {
     constexpr int NB = log2c(R);   // lookups at 2^b, b < NB (2^b <= R - 1)
-    cplx<T> w[R];
+    cplx<T> p[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         cplx<T> x = tab[b * Ns + k];
         if (SIGN > 0) x.y = -x.y;
-        w[1 << b] = x;
+        p[b] = x;
     }
+    if constexpr (sizeof(T) == 8) {
+        // w^r as the product of the lookups of r's set bits, lowest first: only the NB lookups stay live (FP64: the
+        // registers this frees let the column pass hold more waves, which pays there)
 #pragma unroll
-    for (int r = 3; r < R; ++r)
-        if (r != hibit(r)) w[r] = cmul(w[r - hibit(r)], w[hibit(r)]);
+        for (int r = 1; r < R; ++r) {
+            cplx<T> w = {T(1), T(0)};
+            bool first = true;
 #pragma unroll
-    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+            for (int b = 0; b < NB; ++b)
+                if (r & (1 << b)) { w = first ? p[b] : cmul(w, p[b]); first = false; }
+            a[r] = cmul(a[r], w);
+        }
+    } else {
+        // w^r = w^(r - h) w^h, h the highest power of two in r: one product per power
+        cplx<T> w[R];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) w[1 << b] = p[b];
+#pragma unroll
+        for (int r = 3; r < R; ++r)
+            if (r != hibit(r)) w[r] = cmul(w[r - hibit(r)], w[hibit(r)]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+    }
 }
 
 // ---- plans

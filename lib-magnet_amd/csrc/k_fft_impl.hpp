@@ -169,22 +169,34 @@ __device__ __forceinline__ void columns_block(const cplx<T>* __restrict__ Wh, in
     const bool live = ky < Mh;
     const int kyc = live ? ky : 0;
     cplx<T> s0[QL][RL], v[QL][RL];
+    // FP64: each transform recomputes its thread's LDS addresses and reloads its twiddles (an index and an offset the
+    // compiler cannot see through); kept across the seven transforms, they would hold the kernel above the registers
+    // of three waves per SIMD (M = 6144: 518 against 585 us). FP32 keeps them (its kernel measured slower without)
+    auto opaque = [&](int& tq, int& z) {
+        if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(tq), "+v"(z));
+    };
     auto fwd = [&](int ch, cplx<T> (&dst)[QL][RL]) {
+        int tq = t, z = 0;
+        opaque(tq, z);
+        const cplx<T>* twf = tws.fwd + z;
         const cplx<T>* col = STR ? Wh + (size_t)ch * N * Mh + kyc : Wh + ((size_t)ch * Mh + kyc) * LD;
         auto ld = [&](int e, int, int r) -> cplx<T> {
             if (2 * r >= R0 || e >= N) return czero<T>();   // rows >= N (all of r >= R0/2) are padding
             return STR ? col[(size_t)e * Mh] : col[e];
         };
         auto st = [&](int, cplx<T> x, int q, int r) { dst[q][r] = x; };
-        fft_run<T, F, -1, TH, FN>(t, lds, b * L, tws.fwd, ld, st);
+        fft_run<T, F, -1, TH, FN>(tq, lds, b * L, twf, ld, st);
     };
     auto inv = [&](int o, cplx<T> (&src)[QL][RL]) {
+        int tq = t, z = 0;
+        opaque(tq, z);
+        const cplx<T>* twr = tws.rev + z;
         cplx<T>* dst = STR ? Ph + (size_t)o * N * Mh + kyc : Ph + ((size_t)o * Mh + kyc) * LD;
         auto ld = [&](int, int q, int r) { return src[q][r]; };
         auto st = [&](int e, cplx<T> x, int, int r) {
             if (2 * r < R0 && e < N && live) { if (STR) dst[(size_t)e * Mh] = x; else dst[e] = x; }   // only the N live rows
         };
-        fft_run<T, I, +1, TH, FN>(t, lds, b * L, tws.rev, ld, st);
+        fft_run<T, I, +1, TH, FN>(tq, lds, b * L, twr, ld, st);
     };
     const T* k1 = K1 + (size_t)kyc * Mh;
     const T* k2 = K2 + (size_t)kyc * Mh;
