@@ -322,11 +322,19 @@ __device__ __forceinline__ int xcd_block(int total) {   // the block's index in 
     return v < total ? v : -1;
 }
 template <class T, int L, int RB> constexpr int fold_block() { return RB * fft_th<L>(); }
-// the smaller m with 17 (L / 16 + m) = 8 or 24 (mod 32); 17 is its own inverse modulo 32
-constexpr int fold_m(int L, int target) { return (((target - (17 * L / 16) % 32) % 32 + 32) % 32) * 17 % 32; }
+// the smaller m >= 1 with 17 (L / 16 + m) = 8 or 24 (mod 32); 17 is its own inverse modulo 32. m >= 1 leaves the
+// inverse pass the slot after each row's L elements for X_L (with m = 0 it would be the next row's X_0)
+constexpr int fold_m(int L, int target) {
+    const int m = (((target - (17 * L / 16) % 32) % 32 + 32) % 32) * 17 % 32;
+    return m == 0 ? 32 : m;
+}
 template <int L> constexpr int fold_stride() { return L + 16 * (fold_m(L, 8) < fold_m(L, 24) ? fold_m(L, 8) : fold_m(L, 24)); }
-static_assert((17 * fold_stride<3072>() / 16) % 16 == 8 && (17 * fold_stride<768>() / 16) % 16 == 8 &&
-              (17 * fold_stride<256>() / 16) % 16 == 8, "row buffers an odd multiple of 8 elements apart modulo 32");
+template <int L> constexpr bool fold_stride_ok() {
+    return (17 * fold_stride<L>() / 16) % 16 == 8 && fold_stride<L>() >= L + 16;
+}
+static_assert(fold_stride_ok<256>() && fold_stride_ok<384>() && fold_stride_ok<512>() && fold_stride_ok<768>() &&
+              fold_stride_ok<1024>() && fold_stride_ok<1536>() && fold_stride_ok<2048>() && fold_stride_ok<3072>() &&
+              fold_stride_ok<4096>(), "row buffers an odd multiple of 8 elements apart modulo 32, each with room for X_L");
 template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded(RB * fold_stride<L>()) * (int)sizeof(cplx<T>); }
 
 // W (3, N, N) -> Wh (3, Mh, N): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments

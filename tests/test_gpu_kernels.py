@@ -267,7 +267,7 @@ def test_repulsion_every_fft_size(M, ni, nb):
     assert abs(s - so32) / so32 < 2e-6 and rel(F, Fo32) < 2e-6, (abs(s - so32) / so32, rel(F, Fo32))
 
 
-@pytest.mark.parametrize("M,ni,nb", [g for g in FFT_SIZE_GRIDS if g[0] in (512, 1536, 3072, 6144)] + [(8192, 4, 1000)]
+@pytest.mark.parametrize("M,ni,nb", [g for g in FFT_SIZE_GRIDS if g[0] in (512, 768, 1024, 1536, 3072, 6144)] + [(8192, 4, 1000)]
                          + [(1536, 3, 255), (3072, 3, 511), (6144, 3, 999)])
 def test_repulsion_fft_layouts_identical(M, ni, nb):
     """The three layouts of the 2-D FFT's middle -- the column pass in the row layout (strided), row passes that write
