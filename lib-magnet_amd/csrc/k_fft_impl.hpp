@@ -310,10 +310,10 @@ __global__ __launch_bounds__(fft_block<L>()) void kw_rows_inv(const BInst<T>* __
 // layout (channel, ky, row) in segments of RB rows (32 bytes for RB x 8 and RB x 16 bytes). S = L + 16 m makes the
 // padded starts of consecutive buffers differ by an odd multiple of 8 elements modulo 32 (the pad adds one element
 // per 16, so buffer r starts at r (17 L / 16 + 17 m)): the 8 columns x 4 rows of FP32 elements (or 8 x 2 of FP64)
-// that one half-wave stores or loads then fall on distinct banks. Blocks are
-// numbered so that each of the 8 XCDs (the dispatcher hands consecutive blocks to them in turn) takes a contiguous
-// run of row groups: the blocks writing or reading the other segments of a 128-byte line then run in the same L2
-// at about the same time. The arithmetic is the row kernels'.
+// that one half-wave stores or loads then fall on distinct banks. Blocks are numbered so that each of the 8 XCDs (the
+// dispatcher hands consecutive blocks to them in turn) takes a contiguous run of row groups: the blocks writing or
+// reading the other segments of a 128-byte line then run in the same L2 at about the same time. The column layout's
+// stride LD rounds N up to 4 rows, so every segment is 32-byte aligned. The arithmetic is the row kernels'.
 constexpr int kXcds = 8;
 inline int xcd_grid(int total) { return (total + kXcds - 1) / kXcds * kXcds; }
 __device__ __forceinline__ int xcd_block(int total) {   // the block's index in row-group order; -1 past the end
@@ -337,7 +337,7 @@ static_assert(fold_stride_ok<256>() && fold_stride_ok<384>() && fold_stride_ok<5
               fold_stride_ok<4096>(), "row buffers an odd multiple of 8 elements apart modulo 32, each with room for X_L");
 template <class T, int L, int RB> constexpr int fold_lds_bytes() { return padded(RB * fold_stride<L>()) * (int)sizeof(cplx<T>); }
 
-// W (3, N, N) -> Wh (3, Mh, N): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments
+// W (3, N, N) -> Wh (3, Mh, LD): rows_fwd_block's transform per row, then the Mh outputs of the RB rows by segments
 template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(const T* __restrict__ W, int N, cplx<T>* __restrict__ Wh,
                                                                          int LD, TwSet<T> tws) {
@@ -376,7 +376,7 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold(cons
     }
 }
 
-// Ph (4, Mh, N) -> phi (4, N, N): the Mh inputs of the RB rows by segments into LDS, then rows_inv_block's transform
+// Ph (4, Mh, LD) -> phi (4, N, N): the Mh inputs of the RB rows by segments into LDS, then rows_inv_block's transform
 // per row with its inputs from there
 template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold(const cplx<T>* __restrict__ Ph, int N, int LD,
