@@ -564,6 +564,10 @@ template <class T> struct Optimizer {
         relabel = fused && p->relabel_every >= 0 && n >= 2 && it1 - it0 > 1;
         relabel_every = p->relabel_every > 0 ? p->relabel_every : 25;
         if (relabel) start_relabelling();
+        // a run longer than the interval relabels after its first iteration (the first grid), then at the interval: a
+        // layout in random point order would otherwise run its first iterations without locality (1M points in random
+        // order: 1.63 against 1.07 ms per iteration)
+        since = relabel && it1 - it0 > relabel_every ? relabel_every - 1 : 0;
         if (it0 < it1) attraction();
     }
 
