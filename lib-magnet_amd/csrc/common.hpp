@@ -168,6 +168,7 @@ Tunables& tunables();
 // Raises a kernel's dynamic LDS limit to `bytes` (when above the 64 KiB default), once per kernel and size for the
 // process: the HIP runtime's function attributes are shared by every host thread, so the calls are serialised
 // under one lock and none is made again on later launches (nor while a stream is being captured).
+int device_cus();   // compute units of the current device (read once per device)
 void kernel_lds_limit(const void* f, int bytes);
 
 // ---- scratch of one repulsion engine (device pointers), sized by the host

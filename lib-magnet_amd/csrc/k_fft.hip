@@ -5,6 +5,7 @@
 #include "batch.hpp"
 #include <cmath>
 #include <mutex>
+#include <atomic>
 #include <vector>
 
 namespace mg {
@@ -82,6 +83,20 @@ template <class T> static TwSet<T> set_of(const Tables& t, const cplx<T>* base, 
 }
 template <> TwSet<float> fft_twiddle_set<float>(int L) { const Tables& t = g_tw[current_device()]; return set_of<float>(t, t.f32, L); }
 template <> TwSet<double> fft_twiddle_set<double>(int L) { const Tables& t = g_tw[current_device()]; return set_of<double>(t, t.f64, L); }
+
+int device_cus() {
+    static std::atomic<int> cus[64] = {};   // per device, read once
+    const int dev = current_device();
+    if (dev < 0 || dev >= 64) return 256;
+    int c = cus[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
+        int n = 0;
+        MG_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        c = n > 0 ? n : 256;
+        cus[dev].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
 
 void kernel_lds_limit(const void* f, int bytes) {
     struct Set { int dev; const void* f; int bytes; };

@@ -850,12 +850,7 @@ template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M
     const T scale = (T)(1.0 / ((double)M * (double)M));
     if (tunables().fft_fold_persist) {
         // one block per CU (the LDS holds one), at most as many as row groups per XCD, times the 8 XCDs
-        int cus = 256;
-        {
-            int dev = 0, n = 0;
-            if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-                cus = n;
-        }
+        const int cus = device_cus();
 #define MG_P(L_, RB_) do { if constexpr (fold_p_fits<T, L_, RB_>()) { \
             const int G = (N + RB_ - 1) / RB_, GX = (4 * G + kXcds - 1) / kXcds, per = std::min(GX, std::max(1, cus / kXcds)); \
             const int by = fold_p_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold_p<T, L_, RB_>, by); \
