@@ -218,10 +218,13 @@ int sort_hist_blocks(i64 n);
 i64 sort_hist_ints(i64 n);
 int spread_chunks(i64 n);
 // centred: charges (1, x - x_node, y - y_node) and forces from the centred potentials (FP32 path)
+// fill: clear the grid's 3 N^2 charges first (false: the caller keeps the grid zero between calls)
 template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      bool centred, RepBuffers<T>& b, hipStream_t st);
+                                      bool centred, RepBuffers<T>& b, hipStream_t st, bool fill = true);
+// Wclr != null: the gather also sets the charge nodes of every occupied box of W = Wclr back to zero
 template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st);
+                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st,
+                                      T* Wclr = nullptr);
 int gather_blocks(i64 n);
 // out-of-sample placement of m new points against a fixed training layout and its potential field phi
 template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_t* nbr, const T* w, int k, T* Yn, i64 m,

@@ -206,6 +206,11 @@ template <class T> struct Repulsion {
     // stream starts after everything issued before the call (ev_go: no table in use is overwritten) and the column
     // pass waits for it (ev_spec).
     bool overlap = false;
+    // clear_w (set by the loop): the gather sets the boxes the spread wrote back to zero, so W stays zero between
+    // calls and the spread needs no fill of the grid; w_zero is W.p while that holds (a fill of the whole buffer
+    // establishes it after an allocation, and after a call that issued the spread but not the gather)
+    bool clear_w = false;
+    const void* w_zero = nullptr;
     hipStream_t side = nullptr;
     hipEvent_t ev_go = nullptr, ev_spec = nullptr;
     int last_N = 0, last_M = 0, sorted_buf = 0;
@@ -375,7 +380,9 @@ template <class T> struct Repulsion {
         const int* keys = sorted_buf == 0 ? b.keysA : b.keysB;
         const int* vals = sorted_buf == 0 ? b.valsA : b.valsB;
         P.begin(ST_SPREAD, st);
-        launch_spread<T>(keys, vals, Y, n, g, ni, centred, b, st);
+        if (clear_w && w_zero != W.p) launch_fill<T>(b.W, T(0), (i64)(W.bytes / sizeof(T)), st);
+        w_zero = nullptr;
+        launch_spread<T>(keys, vals, Y, n, g, ni, centred, b, st, !clear_w);
         P.end(ST_SPREAD, st, 8 * dn + 2 * dn * ts + 3 * NN * ts);
         // small grids: the column pass reads and writes the row layout in place of the two transposes (L2-resident);
         // larger ones: the row passes write and read the column layout themselves (folded), or through transposes
@@ -407,7 +414,8 @@ template <class T> struct Repulsion {
         else launch_rows_inverse<T>(strided ? b.Ph : b.Tr, g.N, M, b.phi, st);
         P.end(ST_ROWS_INV, st, 4 * Mh * g.N * cs + 4 * NN * ts, M);
         P.begin(ST_GATHER, st);
-        launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st);
+        launch_gather<T>(keys, vals, Y, n, g, ni, centred, b.phi, F, b, sumq_hist, st, clear_w ? b.W : nullptr);
+        if (clear_w) w_zero = W.p;
         P.end(ST_GATHER, st, 8 * dn + 2 * dn * ts + 4 * NN * ts + 2 * dn * ts);
         last_N = g.N; last_M = M; last_n = n; last_g = g;
         P.grid(g.N, M);
@@ -529,6 +537,7 @@ template <class T> struct Optimizer {
         if (check_csr) validate_csr(ip, ix, n, nnz, st);
         rep.have_kernel = true; rep.tsne_kernel = true; rep.kdof = p->dof; rep.kwidth = p->box_width;
         rep.overlap = true;
+        rep.clear_w = true;
         lr = p->learning_rate > 0 ? p->learning_rate : std::max(200.0, (double)n / p->early_exaggeration);
         fused = !(pairs && m > 0 && p->mid_weight > 0);
         attr.ensure((size_t)2 * n * sizeof(T));

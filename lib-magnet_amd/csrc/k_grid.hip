@@ -457,7 +457,8 @@ __device__ inline CSum block_csum(CSum v, double* sh) {
 template <class T, int NI, bool CEN>
 __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const int* __restrict__ vals,
                                              const T* __restrict__ Y, i64 n, const GridT<T>& g, const T* __restrict__ phi,
-                                             T* __restrict__ F, double* __restrict__ zpart, int bid, int nblk) {
+                                             T* __restrict__ F, double* __restrict__ zpart, int bid, int nblk,
+                                             T* __restrict__ Wclr = nullptr) {
     __shared__ double shd[TPB];
     typedef typename vec2<T>::type V2;
     struct V4 { T x, y, z, w; };
@@ -474,6 +475,16 @@ __device__ __forceinline__ void gather_block(const int* __restrict__ keys, const
         T wx[NI], wy[NI];
         double fxu, fyu;
         weights<T, NI, CEN>(pt.x, pt.y, key, g, bx, by, wx, wy, fxu, fyu);
+        if (Wclr != nullptr && (p == 0 || keys[p - 1] != key)) {
+            // the box's first point: its charge nodes, the only ones the spread wrote, back to zero (the row pass has
+            // read them), so that the next spread needs no fill of the whole grid
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+                for (int a = 0; a < NI; ++a)
+#pragma unroll
+                    for (int b = 0; b < NI; ++b) Wclr[((size_t)ch * g.N + bx * NI + a) * g.N + by * NI + b] = T(0);
+        }
         V2 f;
         if (CEN) {
             // F = sum_g w(g) (phi1(g) (y_i - y_g) + psi(g)), offsets and sums in double
@@ -544,8 +555,8 @@ __global__ __launch_bounds__(TPB) void k_zfinal(const double* __restrict__ zpart
 template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void k_gather(const int* __restrict__ keys, const int* __restrict__ vals,
                                                 const T* __restrict__ Y, i64 n, GridT<T> g, const T* __restrict__ phi,
-                                                T* __restrict__ F, double* __restrict__ zpart) {
-    gather_block<T, NI, CEN>(keys, vals, Y, n, g, phi, F, zpart, blockIdx.x, gridDim.x);
+                                                T* __restrict__ F, double* __restrict__ zpart, T* __restrict__ Wclr) {
+    gather_block<T, NI, CEN>(keys, vals, Y, n, g, phi, F, zpart, blockIdx.x, gridDim.x, Wclr);
 }
 template <class T, int NI, bool CEN>
 __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I, BMap m, const int* __restrict__ keys,
@@ -851,8 +862,8 @@ template <class T, int NI, bool CEN> static void spread_ni(const int* keys, cons
 }
 
 template <class T> void launch_spread(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      bool centred, RepBuffers<T>& b, hipStream_t st) {
-    launch_fill<T>(b.W, T(0), (i64)3 * g.N * g.N, st);
+                                      bool centred, RepBuffers<T>& b, hipStream_t st, bool fill) {
+    if (fill) launch_fill<T>(b.W, T(0), (i64)3 * g.N * g.N, st);
     if (n == 0) return;
 #define MG_S(NI) (centred ? spread_ni<T, NI, true>(keys, vals, Y, n, g, b, st) : spread_ni<T, NI, false>(keys, vals, Y, n, g, b, st))
     switch (ni) {
@@ -864,11 +875,12 @@ template <class T> void launch_spread(const int* keys, const int* vals, const T*
 }
 
 template <class T> void launch_gather(const int* keys, const int* vals, const T* Y, i64 n, const GridT<T>& g, int ni,
-                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st) {
+                                      bool centred, const T* phi, T* F, RepBuffers<T>& b, double* sumq_hist, hipStream_t st,
+                                      T* Wclr) {
     const unsigned blocks = (unsigned)gather_blocks(n);
 #define MG_G(NI) do { \
-        if (centred) hipLaunchKernelGGL((k_gather<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart); \
-        else hipLaunchKernelGGL((k_gather<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart); \
+        if (centred) hipLaunchKernelGGL((k_gather<T, NI, true>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, Wclr); \
+        else hipLaunchKernelGGL((k_gather<T, NI, false>), dim3(blocks), dim3(TPB), 0, st, keys, vals, Y, n, g, phi, F, b.zpart, Wclr); \
     } while (0)
     switch (ni) {
         case 1: MG_G(1); break; case 2: MG_G(2); break; case 3: MG_G(3); break; case 4: MG_G(4); break;
@@ -1019,8 +1031,8 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
 #define MG_INST(T) \
     template int launch_box_sort<T>(const T*, i64, const GridT<T>&, RepBuffers<T>&, hipStream_t, const int*); \
     template int launch_key_sort<T>(int*, int*, int*, int*, int*, i64, int, hipStream_t); \
-    template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t); \
-    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t); \
+    template void launch_spread<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, RepBuffers<T>&, hipStream_t, bool); \
+    template void launch_gather<T>(const int*, const int*, const T*, i64, const GridT<T>&, int, bool, const T*, T*, RepBuffers<T>&, double*, hipStream_t, T*); \
     template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, const double*, hipStream_t); \
     template void launch_centroid<T>(const T*, i64, double*, hipStream_t); \
     template void launch_keys_batch<T>(const BInst<T>*, const BMap&, const T*, int, int*, int*, hipStream_t); \
