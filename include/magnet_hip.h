@@ -62,8 +62,10 @@ int mg_mem_info(size_t* free_bytes, size_t* total_bytes);
  *                  pass's layout, with no transposes (default 1024; 0: never);
  *   "fft_fold"     above that size, rows per block of the row passes that write and read the column pass's layout
  *                  themselves: 4 (default) or 2 (FP64 takes 2), or 0 for row passes and transposes;
- *   "fft_fold_persist" 1 (default): the folded inverse row pass runs one block per CU over its XCD's row groups with
- *                  the next group's loads in flight, where its buffers and twiddles fit in LDS; 0: one block per group;
+ *   "fft_fold_persist" bit 0: the folded inverse row pass runs one block per CU over its XCD's row groups with the
+ *                  next group's loads in flight; bit 1: the folded forward pass does; 3 (default) both, 0 neither
+ *                  (one block per group). Used where the per-group form holds one block per CU and the persistent
+ *                  form's buffers and twiddles fit in LDS;
  *   "spec_issue"   when the kernel spectra are recomputed beside the loop's repulsion: issued first (0), after the
  *                  box sort (1, default) or after the row pass (2);
  *   "batch_graph"  1 (default): mg_tsne_optimize_batch replays its captured iteration (a hipGraph); 0: it issues the

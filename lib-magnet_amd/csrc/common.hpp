@@ -153,8 +153,9 @@ struct Tunables {
     int fft_strided_max = 1024;   // padded FFT sizes up to which the column pass works in the row layout (no transposes)
     int fft_fold = 4;       // above that size, rows per block of the row passes that write / read the column layout
                             // themselves (4 or 2; FP64 takes 2), or 0: row passes and transposes
-    int fft_fold_persist = 1;   // the folded inverse row pass as one block per CU over its XCD's row groups, with the
-                                // next group's loads in flight (1), or one block per row group (0)
+    int fft_fold_persist = 3;   // folded row passes as one block per CU over its XCD's row groups, with the next group's
+                                // loads in flight: bit 0 the inverse pass, bit 1 the forward pass (else one block per
+                                // row group); used where the per-group form holds one block per CU
     int spec_issue = 1;     // recomputed spectra (second stream) issued first (0), after the box sort (1) or the row pass (2)
     int sort_digit = 8;     // digit bits of the radix sorts of keys of 9-16 bits: 8 (two passes either way), or 10
     int batch_graph = 1;    // the batched loop replays its iteration as a captured hipGraph (0: issues the launches)

@@ -1394,7 +1394,7 @@ int mg_tune_set(const char* name, int value) {
     else if (k == "fft_strided_max") { require(value >= 0 && value <= 8192, "fft_strided_max must be in [0, 8192]"); g_tunables.fft_strided_max = value; }
     else if (k == "fft_fold") { require(value == 0 || value == 2 || value == 4, "fft_fold must be 0, 2 or 4"); g_tunables.fft_fold = value; }
     else if (k == "spec_issue") { require(value >= 0 && value <= 2, "spec_issue must be 0, 1 or 2"); g_tunables.spec_issue = value; }
-    else if (k == "fft_fold_persist") { require(value == 0 || value == 1, "fft_fold_persist must be 0 or 1"); g_tunables.fft_fold_persist = value; }
+    else if (k == "fft_fold_persist") { require(value >= 0 && value <= 3, "fft_fold_persist must be in [0, 3]"); g_tunables.fft_fold_persist = value; }
     else if (k == "batch_graph") { require(value == 0 || value == 1, "batch_graph must be 0 or 1"); g_tunables.batch_graph = value; }
     else if (k == "batch_chunk") { require(value >= 1 && value <= 64, "batch_chunk must be in [1, 64]"); g_tunables.batch_chunk = value; }
     else if (k == "batch_window_down") { require(value == 0 || value == 1, "batch_window_down must be 0 or 1"); g_tunables.batch_window_down = value; }

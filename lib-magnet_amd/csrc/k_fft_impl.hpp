@@ -435,6 +435,10 @@ template <class T, int L, int RB> constexpr int fold_p_lds_bytes() {
     return (padded(RB * fold_stride<L>()) + plan_tw_size<Rev<typename FwdPlan<T, L>::type>>() + L + 1) * (int)sizeof(cplx<T>);
 }
 template <class T, int L, int RB> constexpr bool fold_p_fits() { return fold_p_lds_bytes<T, L, RB>() <= 160 * 1024; }
+// The persistent forms pay where the per-group form holds one block per CU (its LDS above half of the CU's): M = 6144,
+// 4 FP32 rows per block, 0.84 against 0.86 ms per convolution; where two blocks fit (M <= 4096) the per-group form
+// hides more latency (M = 2048: 0.227 against 0.235 ms, M = 4096: 0.401 against 0.416 ms).
+template <class T, int L, int RB> constexpr bool fold_persist_pays() { return 2 * fold_lds_bytes<T, L, RB>() > 160 * 1024; }
 
 template <class T, int L, int RB>
 __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold_p(const cplx<T>* __restrict__ Ph, int N, int LD,
@@ -511,6 +515,90 @@ __global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_inv_fold_p(co
             fft_run<T, Pl, +1, TH, FN>(t, lds, base, twg, ld, st);
         }
         __syncthreads();   // the buffers are staged again
+        if (!more) break;
+        jj = jn;
+    }
+}
+
+// The forward pass, persistent in the same way: one block per CU walks its XCD's run of row groups with the twiddle
+// tables in LDS; the next group's row samples (each thread's live inputs of the first radix pass, in registers) are
+// loaded while the current group is transformed and its segments written.
+template <class T, int L, int RB> constexpr int fold_pf_lds_bytes() {
+    return (padded(RB * fold_stride<L>()) + plan_tw_size<typename FwdPlan<T, L>::type>() + L + 1) * (int)sizeof(cplx<T>);
+}
+template <class T, int L, int RB> constexpr bool fold_pf_fits() { return fold_pf_lds_bytes<T, L, RB>() <= 160 * 1024; }
+
+template <class T, int L, int RB>
+__global__ __launch_bounds__((fold_block<T, L, RB>())) void k_rows_fwd_fold_p(const T* __restrict__ W, int N, cplx<T>* __restrict__ Wh,
+                                                                           int LD, TwSet<T> tws) {
+    using Pl = typename FwdPlan<T, L>::type;
+    constexpr int TH = fft_th<L>(), Mh = L + 1, R0 = Pl::R[0], NI0 = L / R0, S = fold_stride<L>();
+    constexpr int NB = fold_block<T, L, RB>(), IT = (Mh * RB + NB - 1) / NB, NTW = plan_tw_size<Pl>();
+    static_assert(Pl::P > 1 && NI0 <= TH, "one first-pass item per thread, and the first pass has no twiddles");
+    typedef typename vec2<T>::type V2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* tw_l = lds + padded(RB * S);
+    cplx<T>* half_l = tw_l + NTW;
+    const int G = (N + RB - 1) / RB, total = 3 * G;
+    const int per = (int)gridDim.x / kXcds, x = (int)blockIdx.x % kXcds, GX = (total + kXcds - 1) / kXcds;
+    int jj = (int)blockIdx.x / kXcds;
+    auto valid = [&](int j) { return j < GX && x * GX + j < total; };
+    if (!valid(jj)) return;   // uniform over the block
+    for (int i = threadIdx.x; i < NTW; i += NB) tw_l[i] = tws.fwd[i];
+    for (int i = threadIdx.x; i <= L; i += NB) half_l[i] = tws.half[i];
+    const bool even = (N & 1) == 0;
+    cplx<T> xr[R0 / 2];   // item t's inputs r < R0 / 2 (element t + r L / R0; the others are padding) of the next group
+    auto load = [&](int v) {
+        const int c = v / G, row0 = (v % G) * RB;
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));   // per call: the addresses are not kept live across the loop
+        const int t = tid % TH, gx = row0 + tid / TH;
+        const bool live = gx < N && t < NI0;
+        const T* src = W + ((size_t)c * N + (gx < N ? gx : 0)) * N;
+#pragma unroll
+        for (int r = 0; r < R0 / 2; ++r) {
+            const int e = t + r * NI0, n0 = 2 * e;
+            cplx<T> z = czero<T>();
+            if (live && n0 < N) {
+                if (even) { const V2 p = reinterpret_cast<const V2*>(src)[e]; z.x = p.x; z.y = p.y; }
+                else { z.x = src[n0]; if (n0 + 1 < N) z.y = src[n0 + 1]; }
+            }
+            xr[r] = z;
+        }
+    };
+    load(x * GX + jj);
+    for (;;) {
+        const int v = x * GX + jj;
+        const int c = v / G, row0 = (v % G) * RB;
+        cplx<T> xc[R0 / 2];
+#pragma unroll
+        for (int r = 0; r < R0 / 2; ++r) xc[r] = xr[r];
+        const int jn = jj + per;
+        const bool more = valid(jn);
+        if (more) load(x * GX + jn);
+        {
+            // an offset the compiler cannot see through, renewed per group: the twiddle reads stay inside the loop
+            int opaque = 0, tid = threadIdx.x;
+            asm volatile("" : "+v"(opaque), "+v"(tid));
+            const int t = tid % TH, b = tid / TH;
+            auto ld = [&](int, int, int r) -> cplx<T> { return 2 * r < R0 ? xc[r] : czero<T>(); };
+            auto none = [](int, cplx<T>, int, int) {};
+            fft_run<T, Pl, -1, TH, TO_LDS>(t, lds, b * S, tw_l + opaque, ld, none);
+        }
+        __syncthreads();   // the RB transforms are in LDS (the first group's barriers also order the table copies)
+        {
+            int opaque = 0, tid = threadIdx.x;
+            asm volatile("" : "+v"(opaque), "+v"(tid));
+            const cplx<T>* halfg = half_l + opaque;
+            cplx<T>* dst = Wh + (size_t)c * Mh * LD + row0;
+#pragma unroll
+            for (int u = 0; u < IT; ++u) {
+                const int i = tid + u * NB, r = i % RB, k = i / RB;
+                if (i < Mh * RB && row0 + r < N) dst[(size_t)k * LD + r] = r2c_post<T, L>(lds, r * S, k, halfg);
+            }
+        }
+        __syncthreads();   // the buffers are written again by the next group's transforms
         if (!more) break;
         jj = jn;
     }
@@ -832,6 +920,20 @@ template <class T> void launch_rows_forward_fold(const T* W, int N, int M, cplx<
     require(N <= M / 2, "FFT size below 2N");
     const int LD = fold_stride_n(N);
     const int rb = fold_rows<T>(rows);
+    if (tunables().fft_fold_persist & 2) {
+        // one block per CU (the LDS holds one), at most as many as row groups per XCD, times the 8 XCDs
+        const int cus = device_cus();
+#define MG_P(L_, RB_) do { if constexpr (fold_pf_fits<T, L_, RB_>() && fold_persist_pays<T, L_, RB_>()) { \
+            const int G = (N + RB_ - 1) / RB_, GX = (3 * G + kXcds - 1) / kXcds, per = std::min(GX, std::max(1, cus / kXcds)); \
+            const int by = fold_pf_lds_bytes<T, L_, RB_>(); set_lds(k_rows_fwd_fold_p<T, L_, RB_>, by); \
+            hipLaunchKernelGGL((k_rows_fwd_fold_p<T, L_, RB_>), dim3(per * kXcds), dim3(fold_block<T, L_, RB_>()), by, st, \
+                               W, N, Wh, LD, fft_twiddle_set<T>(L_)); \
+            MG_HIP(hipGetLastError()); return; } } while (0)
+#define MG_C(L_) case L_: if constexpr (sizeof(T) == 4) { if (rb == 4) MG_P(L_, 4); else MG_P(L_, 2); } else MG_P(L_, 2); break;
+        switch (M / 2) { MG_FOLD_SIZES(MG_C) default: break; }
+#undef MG_C
+#undef MG_P
+    }
 #define MG_R(L_, RB_) do { \
         const int by = fold_lds_bytes<T, L_, RB_>(); set_lds(k_rows_fwd_fold<T, L_, RB_>, by); \
         hipLaunchKernelGGL((k_rows_fwd_fold<T, L_, RB_>), dim3(xcd_grid(3 * ((N + RB_ - 1) / RB_))), dim3(fold_block<T, L_, RB_>()), by, st, \
@@ -848,10 +950,10 @@ template <class T> void launch_rows_inverse_fold(const cplx<T>* Ph, int N, int M
     const int LD = fold_stride_n(N);
     const int rb = fold_rows<T>(rows);
     const T scale = (T)(1.0 / ((double)M * (double)M));
-    if (tunables().fft_fold_persist) {
+    if (tunables().fft_fold_persist & 1) {
         // one block per CU (the LDS holds one), at most as many as row groups per XCD, times the 8 XCDs
         const int cus = device_cus();
-#define MG_P(L_, RB_) do { if constexpr (fold_p_fits<T, L_, RB_>()) { \
+#define MG_P(L_, RB_) do { if constexpr (fold_p_fits<T, L_, RB_>() && fold_persist_pays<T, L_, RB_>()) { \
             const int G = (N + RB_ - 1) / RB_, GX = (4 * G + kXcds - 1) / kXcds, per = std::min(GX, std::max(1, cus / kXcds)); \
             const int by = fold_p_lds_bytes<T, L_, RB_>(); set_lds(k_rows_inv_fold_p<T, L_, RB_>, by); \
             hipLaunchKernelGGL((k_rows_inv_fold_p<T, L_, RB_>), dim3(per * kXcds), dim3(fold_block<T, L_, RB_>()), by, st, \
@@ -1024,10 +1126,13 @@ template <class T> void prepare_fft_kernels() {
                    set_lds(kb_spec2<T, L_, 2>, by); set_lds(kb_spec2p<T, L_>, by); }
 #define MG_F(L_) { set_lds(k_rows_fwd_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); set_lds(k_rows_inv_fold<T, L_, 2>, fold_lds_bytes<T, L_, 2>()); \
                    if constexpr (fold_p_fits<T, L_, 2>()) set_lds(k_rows_inv_fold_p<T, L_, 2>, fold_p_lds_bytes<T, L_, 2>()); \
+                   if constexpr (fold_pf_fits<T, L_, 2>()) set_lds(k_rows_fwd_fold_p<T, L_, 2>, fold_pf_lds_bytes<T, L_, 2>()); \
                    if constexpr (sizeof(T) == 4) { set_lds(k_rows_fwd_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); \
                                                    set_lds(k_rows_inv_fold<T, L_, 4>, fold_lds_bytes<T, L_, 4>()); \
                                                    if constexpr (fold_p_fits<T, L_, 4>()) \
-                                                       set_lds(k_rows_inv_fold_p<T, L_, 4>, fold_p_lds_bytes<T, L_, 4>()); } }
+                                                       set_lds(k_rows_inv_fold_p<T, L_, 4>, fold_p_lds_bytes<T, L_, 4>()); \
+                                                   if constexpr (fold_pf_fits<T, L_, 4>()) \
+                                                       set_lds(k_rows_fwd_fold_p<T, L_, 4>, fold_pf_lds_bytes<T, L_, 4>()); } }
     MG_ROW_SIZES(MG_R)
     MG_COL_SIZES(MG_K)
     MG_FOLD_SIZES(MG_F)

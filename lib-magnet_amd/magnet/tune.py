@@ -15,7 +15,7 @@ SPACE = (
     ("sort_digit", (8, 10)),
     ("fft_strided_max", (1024, 0, 2048)),
     ("fft_fold", (4, 2, 0)),
-    ("fft_fold_persist", (1, 0)),
+    ("fft_fold_persist", (3, 0)),
     ("spec_issue", (1, 0, 2)),
     ("attr_window", (-1, 0, 1)),
     ("attr_rows", (256, 64, 128, 512)),

@@ -281,7 +281,8 @@ def test_repulsion_fft_layouts_identical(M, ni, nb):
     Y = rng.uniform(0.05, span - 0.05, (3000, 2))
     layouts = [dict(fft_strided_max=8192, fft_fold=4), dict(fft_strided_max=0, fft_fold=0),
                dict(fft_strided_max=0, fft_fold=4), dict(fft_strided_max=0, fft_fold=2),
-               dict(fft_strided_max=0, fft_fold=4, fft_fold_persist=0), dict(fft_strided_max=0, fft_fold=2, fft_fold_persist=0)]
+               dict(fft_strided_max=0, fft_fold=4, fft_fold_persist=0), dict(fft_strided_max=0, fft_fold=2, fft_fold_persist=0),
+               dict(fft_strided_max=0, fft_fold=4, fft_fold_persist=1), dict(fft_strided_max=0, fft_fold=4, fft_fold_persist=2)]
     saved = {k: capi.tune(k) for k in ("fft_strided_max", "fft_fold", "fft_fold_persist")}
     try:
         for prec, dt in ((capi.MG_F64, np.float64), (capi.MG_F32, np.float32)):
