@@ -443,7 +443,7 @@ def traffic_of(workload, kernel):
 
 FFT_COL_KERNELS = ("k_columns", "kw_columns", "k_spec1", "k_spec2", "k_spec2p", "kb_spec1", "kb_spec2", "kb_spec2p")
 FFT_ROW_KERNELS = ("k_rows_fwd", "k_rows_inv", "kw_rows_fwd", "kw_rows_inv", "k_rows_fwd_fold", "k_rows_inv_fold",
-                   "k_rows_inv_fold_p")
+                   "k_rows_fwd_fold_p", "k_rows_inv_fold_p")
 
 
 def kernel_key(name):
