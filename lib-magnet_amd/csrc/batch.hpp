@@ -95,7 +95,7 @@ struct BSched {
 // the ints [total, n, cap, inst[cap], start[cap + 1]]: n instances with live work, instance inst[k] owning the
 // virtual blocks [start[k], start[k + 1]), total = start[n]. The stage's launch has a fixed grid whose blocks stride
 // over the virtual blocks, so its shape does not depend on the grids, and no block is dispatched for nothing.
-enum WorkKind { WK_ROWS3 = 0, WK_COLS = 1, WK_ROWS4 = 2, WK_TR0 = 3, WK_TR1 = 4, WK_FILL = 5 };
+enum WorkKind { WK_ROWS3 = 0, WK_COLS = 1, WK_ROWS4 = 2, WK_TR0 = 3, WK_TR1 = 4 };
 struct WorkDesc { int kind, M, cap, off, cand; };   // cand: offset of the list's candidate instances
 inline int wl_ints(int cap) { return 3 + 2 * cap + 1; }
 __device__ __forceinline__ int wl_find(const int* wl, int v) {
@@ -125,8 +125,6 @@ template <class T> void launch_update_batch(const BInst<T>* I, const BMap& m, T*
 // the iteration's schedule values (one thread per instance)
 template <class T> void launch_decide_batch(BInst<T>* I, BState* S, const BSched& q, hipStream_t st);
 // clears each active instance's charge grid W (3 N^2 values, 4096 per virtual block of the list; grid blocks)
-template <class T> void launch_fill_wl(const BInst<T>* I, const int* wl, int grid, hipStream_t st);
-inline int fill_blocks(int Mhi) { const long long c = 3ll * (Mhi / 2) * (Mhi / 2); return (int)((c + 4095) / 4096); }
 // keys (inst << box_bits | box key) and global point indices of every point (m: 256-point blocks); the points of
 // an inactive instance all take box 0
 template <class T> void launch_keys_batch(const BInst<T>* I, const BMap& m, const T* Y, int box_bits, int* keys, int* vals,

@@ -751,6 +751,7 @@ template <class T> struct BatchEngine {
     BSched sched{};
     DevBuf Yc, Uc, Gc, Ac, Fc, keysA, valsA, keysB, valsB, hist, atable, mbd, sumq, counters, zout, inv, upart, state;
     DevBuf aW, aWh, aPh, aTr, aphi, aik, aiv, aival, azp, spec_stage;
+    const void* w_zero = nullptr;   // aW.p once its charge grids are zero (then kept zero by the gathers)
     DevBuf table[2];          // the plans' tables, in turn (a new plan's table is written while the last one's
     PinnedBuf htab[2], hrep[2];   // iterations still run); the states read back after each stretch, in turn
     std::vector<void*> dead;  // replaced allocations, freed once the stream is idle (at the end of the run)
@@ -770,10 +771,10 @@ template <class T> struct BatchEngine {
     struct Launches {
         std::vector<BMap> spec1, spect, spec2, specr;   // clipped grids at the largest size
         std::vector<BMap> keys, spread, gather, update;
-        // work lists: offsets of rows / columns / inverse rows per size, and of the transposes and the fill, in the
-        // list buffer, with the fixed grid of each launch
+        // work lists: offsets of rows / columns / inverse rows per size, and of the transposes, in the list buffer,
+        // with the fixed grid of each launch
         std::vector<int> rows_f, cols, rows_i, g_rows_f, g_cols, g_rows_i;
-        int tr0 = 0, tr1 = 0, fill = 0, g_tr0 = 0, g_tr1 = 0, g_fill = 0;
+        int tr0 = 0, tr1 = 0, g_tr0 = 0, g_tr1 = 0;
         int ndesc = 0;
     } L;
     DevBuf wdesc[2], wcand[2], wlist[2];   // per plan, in turn with the tables
@@ -1006,6 +1007,9 @@ template <class T> struct BatchEngine {
         grow(aW, oW * sizeof(T), dead); grow(aWh, oWh * sizeof(cplx<T>), dead); grow(aPh, oPh * sizeof(cplx<T>), dead);
         grow(aTr, oTr * sizeof(cplx<T>), dead); grow(aphi, ophi * sizeof(T), dead); grow(aik, oit * sizeof(int), dead);
         grow(aiv, oit * sizeof(int), dead); grow(aival, oiv * sizeof(T), dead); grow(azp, ozp * sizeof(double), dead);
+        // the charge grids stay zero between iterations (each gather zeroes the boxes its spread wrote): a new buffer
+        // is cleared once, after the iterations queued before, which use the old one
+        if (w_zero != aW.p) { launch_fill<T>(dptr<T>(aW), T(0), (i64)(aW.bytes / sizeof(T)), st); w_zero = aW.p; }
         BInst<T>* h = host_table();
         for (int i = 0; i < count; ++i) {
             BInst<T>& b = h[i];
@@ -1058,13 +1062,10 @@ template <class T> struct BatchEngine {
             a = add(WK_COLS, S, w, bc); L.cols.push_back(a.first); L.g_cols.push_back(a.second);
             a = add(WK_ROWS4, S, w, bi); L.rows_i.push_back(a.first); L.g_rows_i.push_back(a.second);
         }
-        long long b0 = 0, b1 = 0, bf = 0;
-        for (int i = 0; i < count; ++i) {
-            b0 += transpose_cap_blocks(Mhi[i], 0); b1 += transpose_cap_blocks(Mhi[i], 1); bf += fill_blocks(Mhi[i]);
-        }
+        long long b0 = 0, b1 = 0;
+        for (int i = 0; i < count; ++i) { b0 += transpose_cap_blocks(Mhi[i], 0); b1 += transpose_cap_blocks(Mhi[i], 1); }
         auto a = add(WK_TR0, 0, all, b0); L.tr0 = a.first; L.g_tr0 = a.second;
         a = add(WK_TR1, 0, all, b1); L.tr1 = a.first; L.g_tr1 = a.second;
-        a = add(WK_FILL, 0, all, bf); L.fill = a.first; L.g_fill = a.second;
         L.ndesc = (int)desc.size();
         require(desc.size() * sizeof(WorkDesc) <= hdesc[tcur].bytes && cand.size() * sizeof(int) <= hcand[tcur].bytes,
                 "work-list staging too small");
@@ -1103,7 +1104,7 @@ template <class T> struct BatchEngine {
     }
     long long count_launches() const {
         return 2 + (long long)attr_maps.size() + (ntab == 4 ? 4 : 3) * (long long)L.spec1.size() + (long long)L.keys.size()
-               + 3 * ((box_bits + bits_for(count) + 9) / 10) + 3 + 2 * L.spread.size() + 2 * L.gather.size()
+               + 3 * ((box_bits + bits_for(count) + 9) / 10) + 2 + 2 * L.spread.size() + 2 * L.gather.size()
                + 2 * L.update.size() + 3 * (long long)sizes.size();
     }
 
@@ -1128,7 +1129,6 @@ template <class T> struct BatchEngine {
                                         ntot, box_bits + bits_for(count), box_bits, s);
         // spread
         const int* wl = dptr<int>(wlist[tcur]);
-        launch_fill_wl<T>(I, wl + L.fill, L.g_fill, s);
         for (const BMap& m : L.spread) launch_spread_batch<T>(I, m, sorted_keys(), sorted_vals(), dptr<T>(Yc), ni, centred, s);
         // FFT convolution
         for (size_t z = 0; z < sizes.size(); ++z) launch_rows_forward_wl<T>(I, wl + L.rows_f[z], sizes[z], L.g_rows_f[z], s);

@@ -568,7 +568,8 @@ __global__ __launch_bounds__(TPB) void kb_gather(const BInst<T>* __restrict__ I,
     const i64 off = b.off, n = b.n;
     const T* phi = b.phi;
     double* zpart = b.zpart;
-    gather_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, phi, F, zpart, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k]);
+    gather_block<T, NI, CEN>(keys + off, vals + off, Y, n, g, phi, F, zpart, blockIdx.x - m.start[k], m.start[k + 1] - m.start[k],
+                             b.W);   // the instance's charge grid stays zero between iterations
 }
 // one block per instance
 template <class T>
@@ -578,20 +579,6 @@ __global__ __launch_bounds__(TPB) void kb_zfinal(const BInst<T>* __restrict__ I,
     const i64 n = b.n, nb = (n + TPB - 1) / TPB;   // the gather's blocks (gather_blocks)
     const int nblk = (int)(nb < 1024 ? (nb < 1 ? 1 : nb) : 1024);
     zfinal_block(b.zpart, nblk, n, b.zout, b.inv_out, sizeof(T) == 4, b.sumq);
-}
-
-// batched clear of the live charge grid (3 N^2 values) of every active instance, 4096 values per virtual block
-template <class T>
-__global__ __launch_bounds__(TPB) void kw_fill_w(const BInst<T>* __restrict__ I, const int* __restrict__ wl) {
-    const int total = wl[0];
-    for (int v = blockIdx.x; v < total; v += gridDim.x) {
-        const int k = wl_find(wl, v);
-        const BInst<T>& b = I[wl_inst(wl, k)];
-        const i64 cnt = 3ll * b.g.N * b.g.N, base = (i64)(v - wl_start(wl, k)) * 4096;
-        T* W = b.W;
-        for (int j = threadIdx.x; j < 4096; j += TPB)
-            if (base + j < cnt) W[base + j] = T(0);
-    }
 }
 
 // transforms per workgroup of the FFT passes of length L (fft_batch of fft_core.hpp)
@@ -615,8 +602,7 @@ __device__ inline int block_inclusive_scan(int v, int* sh) {
 
 // The work lists of one iteration, one block per list: each candidate instance that runs the iteration (and, for the
 // FFT passes, sits at the list's size) takes, in candidate order, the blocks its single launch would: rows
-// ceil(N / batch) per channel, columns ceil(Mh / batch), transposes the 32 x 32 tiles of its live matrices, the fill
-// 4096 values each.
+// ceil(N / batch) per channel, columns ceil(Mh / batch), transposes the 32 x 32 tiles of its live matrices.
 template <class T>
 __global__ __launch_bounds__(TPB) void kb_worklists(const BInst<T>* __restrict__ I, const WorkDesc* __restrict__ D,
                                                     const int* __restrict__ cand, int* __restrict__ wl) {
@@ -642,8 +628,6 @@ __global__ __launch_bounds__(TPB) void kb_worklists(const BInst<T>* __restrict__
                     nb = ((Mh + 31) / 32) * ((N + 31) / 32) * 3;
                 } else if (w.kind == WK_TR1) {
                     nb = ((N + 31) / 32) * ((Mh + 31) / 32) * 4;
-                } else {
-                    nb = (int)((3ll * N * N + 4095) / 4096);
                 }
             }
         }
@@ -784,12 +768,6 @@ int launch_key_sort_masked(int* keysA, int* valsA, int* keysB, int* valsB, int* 
     require(bits >= 1 && bits <= 31 && keep_bits >= 1 && keep_bits <= bits, "bad key widths");
     const int kmask = keep_bits >= 31 ? -1 : (int)((1u << keep_bits) - 1u);
     return sort_keys_any<float>(nullptr, nullptr, keysA, valsA, n, bits, keysA, valsA, keysB, valsB, hist, st, kmask);
-}
-
-template <class T> void launch_fill_wl(const BInst<T>* I, const int* wl, int grid, hipStream_t st) {
-    if (grid <= 0) return;
-    hipLaunchKernelGGL(kw_fill_w<T>, dim3(grid), dim3(TPB), 0, st, I, wl);
-    MG_HIP(hipGetLastError());
 }
 
 template <class T> void launch_worklists(const BInst<T>* I, const WorkDesc* d, int nd, const int* cand, int* wl, hipStream_t st) {
@@ -1036,7 +1014,6 @@ template <class T> void launch_transform(const T* Ytr, i64 n_train, const int32_
     template void launch_transform<T>(const T*, i64, const int32_t*, const T*, int, T*, i64, const GridT<T>&, int, bool, const T*, double, int, double, double, double, int, double, double, const double*, hipStream_t); \
     template void launch_centroid<T>(const T*, i64, double*, hipStream_t); \
     template void launch_keys_batch<T>(const BInst<T>*, const BMap&, const T*, int, int*, int*, hipStream_t); \
-    template void launch_fill_wl<T>(const BInst<T>*, const int*, int, hipStream_t); \
     template void launch_worklists<T>(const BInst<T>*, const WorkDesc*, int, const int*, int*, hipStream_t); \
     template void launch_decide_batch<T>(BInst<T>*, BState*, const BSched&, hipStream_t); \
     template void launch_spread_batch<T>(const BInst<T>*, const BMap&, const int*, const int*, const T*, int, bool, hipStream_t); \
