@@ -309,9 +309,10 @@ def batched_instances(seed0=1000):
     return Ps, Ys
 
 
-def batched_bench():
-    """All instances through mg_tsne_optimize_batch (stream count from the autotuner) against the same instances
-    one after another through mg_tsne_optimize; both from resident inputs, results compared bit for bit."""
+def batched_bench(tune=True):
+    """All instances through mg_tsne_optimize_batch (stream count from the autotuner; one stream and one host thread
+    with tune=False) against the same instances one after another through mg_tsne_optimize; both from resident
+    inputs, results compared bit for bit."""
     import torch
     from magnet import capi
     from magnet.batch import device_instances, autotune_streams, LAST_SWEEP
@@ -329,7 +330,7 @@ def batched_bench():
     t_seq = time.perf_counter() - t1
     bat = device_instances(Ps, Ys, "fp32")
     t2 = time.perf_counter()
-    k, nt = autotune_streams(bat)
+    k, nt = autotune_streams(bat) if tune else (1, 1)
     t_tune = time.perf_counter() - t2
     torch.cuda.synchronize()
     t3 = time.perf_counter()
@@ -342,7 +343,7 @@ def batched_bench():
     return {"workload": B["desc"], "instances": B["count"], "points": int(npts), "n_streams": k, "host_threads": nt,
             "instance_iters_per_s": round(inst_it / t_bat, 1), "sequential_instance_iters_per_s": round(inst_it / t_seq, 1),
             "speedup_vs_sequential": round(t_seq / t_bat, 3), "batch_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
-            "autotune_s": round(t_tune, 2), "autotune_sweep_12_iters": [list(x) for x in LAST_SWEEP],
+            "autotune_s": round(t_tune, 2), "autotune_sweep_12_iters": [list(x) for x in LAST_SWEEP] if tune else None,
             "sort_items": capi.tune("sort_items"), "P_build_s": round(t_build, 1), "identical_to_single_runs": bool(same)}
 
 
@@ -633,6 +634,8 @@ def main():
     ap.add_argument("--attraction-probe", action="store_true", help="only time the CSR attraction of the workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the transform() serving leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched leg")
+    ap.add_argument("--batched-probe", action="store_true",
+                    help="only the batched leg, on one stream and one host thread (no autotuning sweep)")
     ap.add_argument("--no-scale", action="store_true", help="skip the 1M leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -646,6 +649,11 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.batched_probe:
+        capi.lib()
+        capi.use_torch_stream()
+        print(json.dumps(batched_bench(tune=False)), flush=True)
         return
     if args.attraction_probe:
         capi.lib()
